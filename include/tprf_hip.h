@@ -1,0 +1,132 @@
+/*
+ * tprf_hip.h -- C ABI of the MI355X-native lattice bubble + RPA library
+ * (libtprf_hip.so).
+ *
+ * One entry point per function of the TRIQS/tprf 3.2 hot path
+ * (c++/triqs_tprf/lattice).  The reference takes TRIQS gf objects; here each
+ * gf is exploded into its mesh parameters and a data pointer laid out exactly
+ * as the reference stores gf data: row-major [mesh0][mesh1][target...].
+ *
+ *   - every tprf_c128 pointer is a DEVICE pointer (hipMalloc'd or a torch CUDA
+ *     tensor's data_ptr) on the context's device; the library never frees
+ *     caller memory;
+ *   - calls are asynchronous and ordered on the context's stream
+ *     (tprf_ctx_set_stream); none synchronises unless stated;
+ *   - a context is not thread-safe; use one per host thread;
+ *   - fermionic Matsubara mesh (n_max = N): 2N points, n = -N..N-1;
+ *     bosonic Matsubara mesh (n_max = N): 2N-1 points, m = -(N-1)..N-1;
+ *     imaginary-time mesh: ntau points tau_j = j beta/(ntau-1);
+ *     lattice dims (d0,d1,d2), linear index (i0*d1 + i1)*d2 + i2, d2 = 1 in 2-D.
+ *
+ * Return value: TPRF_OK or a negative TPRF_ERR_* code; tprf_ctx_last_error()
+ * gives the message.  The reference raises C++ exceptions at the matching
+ * places (cited per code).
+ */
+#ifndef TPRF_HIP_H
+#define TPRF_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { double re, im; } tprf_c128; /* layout of std::complex<double> */
+typedef struct tprf_ctx tprf_ctx;
+
+#define TPRF_FERMION 0
+#define TPRF_BOSON 1
+
+#define TPRF_SIGMA_NONE 0
+#define TPRF_SIGMA_W 1  /* sigma[w][a][b]     (g_w_cvt overload, gf.hpp:123)  */
+#define TPRF_SIGMA_WK 2 /* sigma[w][k][a][b]  (g_wk_cvt overload, gf.hpp:83)  */
+
+#define TPRF_OK 0
+#define TPRF_ERR_ARG (-1)            /* bad shape or argument                                  */
+#define TPRF_ERR_POSITIVE_ONLY (-2)  /* fourier_matsubara.cpp:196,221 (not representable here)  */
+#define TPRF_ERR_MESH_TOO_SHORT (-3) /* L < 2 (last_index+1): fourier_matsubara.cpp:105,128,201,251 */
+#define TPRF_ERR_TAIL_FIT (-4)       /* tail fit error >= 1e-3: fourier_matsubara.cpp:227      */
+#define TPRF_ERR_TAIL_ORDER (-5)     /* fewer than 5 tail coefficients: fourier_matsubara.cpp:235 */
+#define TPRF_ERR_SINGULAR (-6)       /* zero pivot in a Dyson or RPA inversion                  */
+#define TPRF_ERR_HIP (-7)            /* HIP runtime error                                      */
+#define TPRF_ERR_NOMEM (-8)
+#define TPRF_ERR_UNSUPPORTED (-9)    /* size outside the kernels' limits (see each function)   */
+
+/* ---- context ---------------------------------------------------------- */
+/* The reference has no context object: plans (fourier.hpp:177, :111) are made
+ * per call.  Here plans, twiddle and tail tables live in the context.       */
+int tprf_ctx_create(tprf_ctx **ctx, int device);
+int tprf_ctx_destroy(tprf_ctx *ctx);
+int tprf_ctx_set_stream(tprf_ctx *ctx, void *hip_stream); /* hipStream_t; NULL = default stream */
+const char *tprf_ctx_last_error(const tprf_ctx *ctx);
+/* Largest RMS tail-fit residual of the fitted omega->tau calls since the last
+ * reset (blocking: synchronises the stream).  The reference asserts < 1e-3
+ * and warns above 1e-6 (fourier_matsubara.cpp:227-234); the fitted zeroth
+ * moment is reported too (warning above 1e-6, :238-239).                   */
+int tprf_ctx_tail_status(tprf_ctx *ctx, double *max_fit_error, double *max_abs_m0);
+int tprf_ctx_tail_status_reset(tprf_ctx *ctx);
+
+/* ---- a1: Dyson --------------------------------------------------------- */
+/* lattice_dyson_g_wk(mu, e_k, sigma_w | sigma_wk), c++/triqs_tprf/lattice/gf.hpp:83,123,
+ * gf.cpp:91-132; Sigma = 0 (sigma_kind NONE) is lattice_dyson_g0_wk, gf.cpp:41-64.
+ * e_k[nk][n][n], sigma per sigma_kind, out g_wk[2 n_iw][nk][n][n].  n <= 6.  */
+int tprf_lattice_dyson_g_wk(tprf_ctx *ctx, double beta, int n_iw, double mu, long nk, int norb,
+                            const tprf_c128 *e_k, int sigma_kind, const tprf_c128 *sigma, tprf_c128 *g_wk);
+
+/* ---- a2 / a6: lattice FFT ---------------------------------------------- */
+/* fourier_wk_to_wr, gf.hpp:276 / gf.cpp:209 -> fourier.hpp:165-197:
+ *   out[w][r][t] = (1/N_k) sum_k exp(-2 pi i k.r / dims) in[w][k][t].
+ * Also chi_wr_from_chi_wk (chi_imtime.cpp:294).  ntarget = n^2 (G) or n^4 (chi).
+ * Every dimension must factor into primes <= 13 and be <= 4096.            */
+int tprf_fourier_wk_to_wr(tprf_ctx *ctx, long nw, const int dims[3], long ntarget, const tprf_c128 *in,
+                          tprf_c128 *out);
+/* fourier_wr_to_wk, gf.cpp:214 -> fourier.hpp:199-231 (no scaling,
+ * exp(+2 pi i k.r)); also chi_wk_from_chi_wr, chi_imtime.cpp:289-292.      */
+int tprf_fourier_wr_to_wk(tprf_ctx *ctx, long nw, const int dims[3], long ntarget, const tprf_c128 *in,
+                          tprf_c128 *out);
+
+/* ---- a3: Matsubara -> imaginary time ----------------------------------- */
+/* fourier_wr_to_tr(g_wr, nt), gf.hpp:296 / gf.cpp:262 -> fourier.hpp:98-130 ->
+ * fourier_matsubara.cpp:194-308.  Also chi_tr_from_chi_wr (statistic BOSON,
+ * chi_imtime.cpp:284).  in[nmesh][ncol] -> out[ntau][ncol] where ncol is the
+ * product of the remaining dims (r and target).  ntau <= 0 selects the default
+ * 6 n_max + 1.  moments: NULL = fit the tail per column (the reference's
+ * fit_tail call, :225-246), else device array [3][ncol] of m1, m2, m3 (the
+ * reference's explicit-moment overload, fourier.hpp:46).  L = ntau-1 must factor
+ * into primes <= 13 and be <= 6144.                                          */
+int tprf_fourier_wr_to_tr(tprf_ctx *ctx, int statistic, double beta, int n_max, int ntau, long ncol,
+                          const tprf_c128 *in, tprf_c128 *out, const tprf_c128 *moments);
+
+/* ---- a5: imaginary time -> Matsubara ----------------------------------- */
+/* chi_wr_from_chi_tr(chi_tr, nw), chi_imtime.hpp:87 / chi_imtime.cpp:279-282 ->
+ * fourier.hpp:132-163 -> fourier_matsubara.cpp:42-184.  Also fourier_tr_to_wr
+ * (statistic FERMION, gf.cpp:257).  in[ntau][ncol] -> out[nmesh(nw)][ncol].  */
+int tprf_fourier_tr_to_wr(tprf_ctx *ctx, int statistic, double beta, int ntau, int nw, long ncol,
+                          const tprf_c128 *in, tprf_c128 *out);
+
+/* ---- a4: bubble product ------------------------------------------------ */
+/* chi0_tr_from_grt_PH(g_tr), chi_imtime.hpp:38 / chi_imtime.cpp:88-136:
+ * chi0[j][r][a][b][c][d] = G[j][r][d][a] * G[ntau-1-j][-r][b][c].            */
+int tprf_chi0_tr_from_grt_PH(tprf_ctx *ctx, int ntau, const int dims[3], int norb, const tprf_c128 *g_tr,
+                             tprf_c128 *chi0_tr);
+/* chi0_wr_from_grt_PH(g_tr, nw), chi_imtime.hpp:40 / chi_imtime.cpp:139-182:
+ * the product fused with the bosonic tau->nu transform of a5 (chi0_tr is never
+ * stored).  out[2nw-1][nr][n^4].                                            */
+int tprf_chi0_wr_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb, int nw,
+                             const tprf_c128 *g_tr, tprf_c128 *chi0_wr);
+/* chi0_w0r_from_grt_PH(g_tr), chi_imtime.cpp:185-228 (trapezoid, nu = 0). out[1][nr][n^4] */
+int tprf_chi0_w0r_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb,
+                              const tprf_c128 *g_tr, tprf_c128 *chi0_wr);
+/* chi_w0r_from_chi_tr(chi_tr), chi_imtime.cpp:250-277 (trapezoid, nu = 0). */
+int tprf_chi_w0r_from_chi_tr(tprf_ctx *ctx, double beta, int ntau, long nr, int norb, const tprf_c128 *chi_tr,
+                             tprf_c128 *chi_wr);
+
+/* ---- a7: RPA ----------------------------------------------------------- */
+/* solve_rpa_PH(chi0, U), rpa.hpp:43 / rpa.cpp:29-68: for every mesh point,
+ * M_chi = inverse(1 - M_chi0 M_U) M_chi0 with M_X[(a,b),(d,c)] = X_abcd.
+ * chi0[npts][n^4], U[n^4] (device), out chi[npts][n^4].  n <= 4.           */
+int tprf_solve_rpa_PH(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *chi0, const tprf_c128 *U,
+                      tprf_c128 *chi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,229 @@
+"""numpy/ctypes front end of the CPU oracle (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+bench.py use this module, always as the checker.  It loads
+oracle/libtprf_oracle.so (built by ``make -C oracle``) and exposes one
+function per restated reference function, taking and returning complex128
+numpy arrays in the reference's row-major layout.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libtprf_oracle.so")
+_lib = None
+
+FERMION, BOSON = 0, 1
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def build():
+    import subprocess
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        d, i, l, p = ctypes.c_double, ctypes.c_int, ctypes.c_long, ctypes.c_void_p
+        sig = {
+            "or_lattice_dyson_g_wk": [d, i, d, i, i, p, i, p, p],
+            "or_fourier_lattice": [l, p, l, p, p, i],
+            "or_fourier_w_to_t": [i, d, i, i, l, p, p, p, p, p],
+            "or_fourier_t_to_w": [i, d, i, i, l, p, p],
+            "or_chi0_tr_from_grt_PH": [i, p, i, p, p],
+            "or_chi0_wr_from_grt_PH": [d, i, p, i, i, p, p],
+            "or_chi0_w0r_from_grt_PH": [d, i, p, i, p, p],
+            "or_chi_w0r_from_chi_tr": [d, i, l, i, p, p],
+            "or_solve_rpa_PH": [l, i, p, p, p],
+            "or_lindhard_chi00": [d, d, i, p, i, p, p],
+            "or_eigh": [i, p, p, p],
+            "or_fft1d": [i, i, p, p],
+            "or_vandermonde_inverse8": [p],
+            "or_fermi": [d],
+            "or_num_threads": [],
+            "or_set_num_threads": [i],
+        }
+        for name, args in sig.items():
+            f = getattr(_lib, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        _lib.or_fermi.restype = ctypes.c_double
+        _lib.or_vandermonde_inverse8.restype = None
+        _lib.or_set_num_threads.restype = None
+    return _lib
+
+
+def _c(a):
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    return a
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _dims(dims):
+    dims = tuple(int(x) for x in dims) + (1,) * (3 - len(dims))
+    return (ctypes.c_int * 3)(*dims), dims
+
+
+def _check(st, what):
+    if st != 0:
+        raise OracleError(f"{what}: oracle status {st}")
+
+
+def set_num_threads(n):
+    lib().or_set_num_threads(int(n))
+
+
+def num_threads():
+    return lib().or_num_threads()
+
+
+def fermi(e):
+    return lib().or_fermi(float(e))
+
+
+def fft1d(x, sign):
+    x = _c(x)
+    out = np.empty_like(x)
+    _check(lib().or_fft1d(len(x), int(sign), _p(x), _p(out)), "fft1d")
+    return out
+
+
+def vandermonde_inverse8():
+    out = np.empty(64, dtype=np.float64)
+    lib().or_vandermonde_inverse8(_p(out))
+    return out.reshape(8, 8)
+
+
+def eigh(a):
+    a = _c(a)
+    n = a.shape[0]
+    ev = np.empty(n, dtype=np.float64)
+    vec = np.empty((n, n), dtype=np.complex128)
+    _check(lib().or_eigh(n, _p(a), _p(ev), _p(vec)), "eigh")
+    return ev, vec
+
+
+def lattice_dyson_g_wk(beta, n_iw, mu, e_k, sigma=None):
+    """e_k[nk, n, n] -> g_wk[2 n_iw, nk, n, n]; sigma None, [2n_iw,n,n] or [2n_iw,nk,n,n]."""
+    e_k = _c(e_k)
+    nk, n = e_k.shape[0], e_k.shape[1]
+    kind, sp = 0, None
+    if sigma is not None:
+        sigma = _c(sigma)
+        kind = 1 if sigma.ndim == 3 else 2
+        sp = _p(sigma)
+    out = np.empty((2 * n_iw, nk, n, n), dtype=np.complex128)
+    _check(lib().or_lattice_dyson_g_wk(beta, n_iw, mu, nk, n, _p(e_k), kind, sp, _p(out)), "dyson")
+    return out
+
+
+def fourier_lattice(x, dims, direction):
+    """x[nouter, nk, *target]; direction -1 = k->r (scaled 1/N_k), +1 = r->k."""
+    x = _c(x)
+    cd, dims = _dims(dims)
+    nouter, nk = x.shape[0], x.shape[1]
+    assert nk == dims[0] * dims[1] * dims[2]
+    ntarget = int(np.prod(x.shape[2:])) if x.ndim > 2 else 1
+    out = np.empty_like(x)
+    _check(lib().or_fourier_lattice(nouter, cd, ntarget, _p(x), _p(out), int(direction)), "fourier_lattice")
+    return out
+
+
+def fourier_w_to_t(gw, beta, statistic=FERMION, ntau=None, moments=None, return_moments=False):
+    """gw[nmesh, *cols] -> gt[ntau, *cols] (ntau default 6*n_max+1)."""
+    gw = _c(gw)
+    nmesh = gw.shape[0]
+    n_max = nmesh // 2 if statistic == FERMION else (nmesh + 1) // 2
+    if ntau is None:
+        ntau = 6 * n_max + 1
+    ncol = int(np.prod(gw.shape[1:])) if gw.ndim > 1 else 1
+    out = np.empty((ntau,) + gw.shape[1:], dtype=np.complex128)
+    mp = None
+    if moments is not None:
+        moments = _c(moments).reshape(3, ncol)
+        mp = _p(moments)
+    mout = np.empty((3, ncol), dtype=np.complex128)
+    err = ctypes.c_double(0.0)
+    st = lib().or_fourier_w_to_t(statistic, beta, n_max, ntau, ncol, _p(gw), _p(out), mp, _p(mout),
+                                 ctypes.cast(ctypes.pointer(err), ctypes.c_void_p))
+    _check(st, "fourier_w_to_t")
+    if return_moments:
+        return out, mout.reshape((3,) + gw.shape[1:]), err.value
+    return out
+
+
+def fourier_t_to_w(gt, beta, nw, statistic=BOSON):
+    gt = _c(gt)
+    ntau = gt.shape[0]
+    ncol = int(np.prod(gt.shape[1:])) if gt.ndim > 1 else 1
+    nout = 2 * nw if statistic == FERMION else 2 * nw - 1
+    out = np.empty((nout,) + gt.shape[1:], dtype=np.complex128)
+    _check(lib().or_fourier_t_to_w(statistic, beta, ntau, nw, ncol, _p(gt), _p(out)), "fourier_t_to_w")
+    return out
+
+
+def chi0_tr_from_grt_PH(g_tr, dims):
+    g_tr = _c(g_tr)
+    ntau, nr, n = g_tr.shape[0], g_tr.shape[1], g_tr.shape[2]
+    cd, _ = _dims(dims)
+    out = np.empty((ntau, nr, n, n, n, n), dtype=np.complex128)
+    _check(lib().or_chi0_tr_from_grt_PH(ntau, cd, n, _p(g_tr), _p(out)), "chi0_tr")
+    return out
+
+
+def chi0_wr_from_grt_PH(g_tr, dims, beta, nw):
+    g_tr = _c(g_tr)
+    ntau, nr, n = g_tr.shape[0], g_tr.shape[1], g_tr.shape[2]
+    cd, _ = _dims(dims)
+    out = np.empty((2 * nw - 1, nr, n, n, n, n), dtype=np.complex128)
+    _check(lib().or_chi0_wr_from_grt_PH(beta, ntau, cd, n, nw, _p(g_tr), _p(out)), "chi0_wr")
+    return out
+
+
+def chi0_w0r_from_grt_PH(g_tr, dims, beta):
+    g_tr = _c(g_tr)
+    ntau, nr, n = g_tr.shape[0], g_tr.shape[1], g_tr.shape[2]
+    cd, _ = _dims(dims)
+    out = np.empty((1, nr, n, n, n, n), dtype=np.complex128)
+    _check(lib().or_chi0_w0r_from_grt_PH(beta, ntau, cd, n, _p(g_tr), _p(out)), "chi0_w0r")
+    return out
+
+
+def chi_w0r_from_chi_tr(chi_tr, beta):
+    chi_tr = _c(chi_tr)
+    ntau, nr, n = chi_tr.shape[0], chi_tr.shape[1], chi_tr.shape[2]
+    out = np.empty((1, nr, n, n, n, n), dtype=np.complex128)
+    _check(lib().or_chi_w0r_from_chi_tr(beta, ntau, nr, n, _p(chi_tr), _p(out)), "chi_w0r")
+    return out
+
+
+def solve_rpa_PH(chi0, U):
+    chi0 = _c(chi0)
+    U = _c(U)
+    n = U.shape[0]
+    npts = chi0.size // n ** 4
+    out = np.empty_like(chi0)
+    _check(lib().or_solve_rpa_PH(npts, n, _p(chi0), _p(U), _p(out)), "solve_rpa")
+    return out
+
+
+def lindhard_chi00(e_k, dims, beta, mu, nw):
+    e_k = _c(e_k)
+    n = e_k.shape[1]
+    cd, dims = _dims(dims)
+    nk = e_k.shape[0]
+    out = np.empty((2 * nw - 1, nk, n, n, n, n), dtype=np.complex128)
+    _check(lib().or_lindhard_chi00(beta, mu, nw, cd, n, _p(e_k), _p(out)), "lindhard")
+    return out

@@ -1,0 +1,323 @@
+"""Pins of the CPU oracle (oracle/tprf_oracle.c) against closed forms,
+brute-force sums, library routines and the reference's own known-answer
+tests.  CPU only."""
+import fractions
+
+import numpy as np
+import pytest
+
+from conftest import relerr
+
+
+def rnd(rng, *shape):
+    return rng.normal(size=shape) + 1j * rng.normal(size=shape)
+
+
+# ------------------------------------------------------------------ FFT
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 9, 11, 12, 13, 16, 17, 25, 48, 64])
+def test_fft1d_vs_brute_force(oracle, n):
+    rng = np.random.default_rng(n)
+    x = rnd(rng, n)
+    j = np.arange(n)
+    for sign in (-1, 1):
+        W = np.exp(sign * 2j * np.pi * (np.outer(j, j) % n) / n)  # exponent reduced mod n
+        assert relerr(oracle.fft1d(x, sign), W @ x) <= 1e-14
+
+
+@pytest.mark.parametrize("n", [384, 1536, 3000, 3072])
+def test_fft1d_vs_numpy(oracle, n):
+    x = rnd(np.random.default_rng(n), n)
+    assert relerr(oracle.fft1d(x, -1), np.fft.fft(x)) <= 1e-14
+    assert relerr(oracle.fft1d(x, +1), np.fft.ifft(x) * n) <= 1e-14
+
+
+@pytest.mark.parametrize("dims", [(4, 3, 2), (16, 16, 1), (5, 1, 7), (1, 1, 1)])
+def test_lattice_fft_sign_and_scale(oracle, dims):
+    """k->r = FFTW_FORWARD / N_k, r->k = FFTW_BACKWARD (fourier_lattice.cpp:39-58)."""
+    rng = np.random.default_rng(sum(dims))
+    nk = int(np.prod(dims))
+    x = rnd(rng, 3, nk, 2, 2)
+    xr = x.reshape((3,) + dims + (2, 2))
+    ref_kr = np.fft.fftn(xr, axes=(1, 2, 3)).reshape(x.shape) / nk
+    ref_rk = np.fft.ifftn(xr, axes=(1, 2, 3)).reshape(x.shape) * nk
+    assert relerr(oracle.fourier_lattice(x, dims, -1), ref_kr) <= 1e-14
+    assert relerr(oracle.fourier_lattice(x, dims, +1), ref_rk) <= 1e-14
+
+
+def test_bubble_k_sum_identity(oracle):
+    """The structural identity of test/c++/bubble.cpp:43-61 in the conventions of
+    fourier_lattice.cpp:39-58: with G(r) = 1/N sum_k e^{-ikr} G(k) and
+    P(q) = sum_r e^{+iqr} G(r) G(-r), P(q) = 1/N sum_k G(k) G(k - q).  It fixes
+    the r <-> -r and q orientation of the product stage."""
+    rng = np.random.default_rng(2)
+    dims = (5, 3, 1)
+    nk = 15
+    g_k = rnd(rng, nk)
+    g_r = oracle.fourier_lattice(g_k.reshape(1, nk), dims, -1)[0]
+    from tprf_amd.gf import MeshCycLat
+    lat = MeshCycLat(dims)
+    prod_r = np.array([g_r[r] * g_r[lat.minus(r)] for r in range(nk)])
+    lhs = oracle.fourier_lattice(prod_r.reshape(1, nk), dims, +1)[0]
+    k = np.array(np.unravel_index(np.arange(nk), dims)).T
+    rhs = np.empty(nk, dtype=complex)
+    wrong = np.empty(nk, dtype=complex)
+    for q in range(nk):
+        rhs[q] = np.sum(g_k * g_k[np.ravel_multi_index(((k - k[q]) % dims).T, dims)]) / nk
+        wrong[q] = np.sum(g_k * g_k[np.ravel_multi_index(((k[q] - k) % dims).T, dims)]) / nk
+    assert relerr(lhs, rhs) <= 1e-13
+    assert relerr(lhs, wrong) > 1e-3  # G(r) G(+r) instead of G(r) G(-r) differs
+
+
+# ------------------------------------------------------------------ Dyson
+@pytest.mark.parametrize("n", [1, 2, 3, 5])
+def test_dyson_vs_numpy_inv(oracle, n):
+    """gf.py:65-70,112-130 of the reference tests Dyson against np.linalg.inv."""
+    rng = np.random.default_rng(n)
+    nk, n_iw, beta, mu = 7, 9, 3.0, -0.4
+    a = rnd(rng, nk, n, n)
+    e = 0.5 * (a + np.conj(np.swapaxes(a, 1, 2)))
+    s = rnd(rng, 2 * n_iw, nk, n, n) * 0.2
+    g = oracle.lattice_dyson_g_wk(beta, n_iw, mu, e, s)
+    wn = np.pi * (2 * np.arange(-n_iw, n_iw) + 1) / beta
+    M = (1j * wn + mu)[:, None, None, None] * np.eye(n) - e[None] - s
+    assert relerr(g, np.linalg.inv(M)) <= 1e-13
+    s_w = s[:, 0]
+    g2 = oracle.lattice_dyson_g_wk(beta, n_iw, mu, e, s_w)
+    M2 = (1j * wn + mu)[:, None, None, None] * np.eye(n) - e[None] - s_w[:, None]
+    assert relerr(g2, np.linalg.inv(M2)) <= 1e-13
+
+
+# ------------------------------------------------------------------ tail fit and w -> tau
+def test_tail_fit_exact_on_polynomial(oracle):
+    """Data that are exactly a polynomial in 1/(i w) of degree <= K are fitted exactly."""
+    beta, n_iw = 10.0, 200
+    wn = np.pi * (2 * np.arange(-n_iw, n_iw) + 1) / beta
+    iw = 1j * wn
+    m = np.array([0.0, 1.0, 0.3 - 0.2j, 0.5 + 0.1j, -0.7, 0.2j])
+    g = sum(m[p] / iw ** p for p in range(1, 6))
+    _, mom, err = oracle.fourier_w_to_t(g.reshape(-1, 1), beta, return_moments=True)
+    assert np.allclose(mom[:, 0], m[1:4], rtol=0, atol=1e-8)
+    assert err < 1e-13
+
+
+def test_tail_fit_free_moments(oracle):
+    """For G = 1/(i w - e) the moments are 1, e, e^2."""
+    beta, n_iw = 40.0, 256
+    eps = np.array([-1.3, 0.2, 2.1])
+    wn = np.pi * (2 * np.arange(-n_iw, n_iw) + 1) / beta
+    g = 1 / (1j * wn[:, None] - eps[None])
+    _, mom, err = oracle.fourier_w_to_t(g, beta, return_moments=True)
+    assert np.abs(mom[0] - 1).max() < 1e-10
+    assert np.abs(mom[1] - eps).max() < 1e-7
+    assert np.abs(mom[2] - eps ** 2).max() < 1e-5
+
+
+def test_w_to_t_single_pole_in_model_is_exact(oracle):
+    """G = 1/(i w - 1) with its exact moments (1, 1, 1) is one pole of the
+    fermionic model (b = 1): the residual vanishes and G(tau) =
+    -e^{-tau}/(1 + e^{-beta}) exactly, including G(beta) = -(G(0) + m1)."""
+    beta, n_iw = 6.0, 64
+    wn = np.pi * (2 * np.arange(-n_iw, n_iw) + 1) / beta
+    g = (1 / (1j * wn - 1.0)).reshape(-1, 1)
+    mom = np.ones((3, 1), dtype=complex)
+    gt = oracle.fourier_w_to_t(g, beta, moments=mom)[:, 0]
+    tau = np.arange(6 * n_iw + 1) * beta / (6 * n_iw)
+    ref = -np.exp(-tau) / (1 + np.exp(-beta))
+    assert np.abs(gt - ref).max() < 1e-14
+
+
+@pytest.mark.parametrize("eps", [-2.0, 0.3, 1.7])
+def test_w_to_t_free_fermion(oracle, eps):
+    """G0(tau) = -e^{-e tau} / (1 + e^{-beta e}) from the fitted-tail transform."""
+    beta, n_iw = 10.0, 512
+    wn = np.pi * (2 * np.arange(-n_iw, n_iw) + 1) / beta
+    g = (1 / (1j * wn - eps)).reshape(-1, 1)
+    gt = oracle.fourier_w_to_t(g, beta)[:, 0]
+    tau = np.arange(6 * n_iw + 1) * beta / (6 * n_iw)
+    ref = -np.exp(-eps * tau) / (1 + np.exp(-beta * eps))
+    assert np.abs(gt - ref).max() < 1e-7
+
+
+def test_t_to_w_boson_two_poles(oracle):
+    """chi(tau) of 1/(i nu - e) - 1/(i nu + e), transformed back to frequency."""
+    beta, e, nw = 8.0, 0.9, 40
+    L = 6 * 200
+    tau = np.arange(L + 1) * beta / L
+    # int_0^beta e^{i nu tau} f(tau) = 1/(i nu - e) - 1/(i nu + e) for
+    chi_t = np.exp(-e * tau) / (np.exp(-beta * e) - 1) - np.exp(-e * (beta - tau)) / (1 - np.exp(-e * beta))
+    chi_w = oracle.fourier_t_to_w(chi_t.reshape(-1, 1), beta, nw)[:, 0]
+    m = np.arange(-(nw - 1), nw)
+    inu = 2j * np.pi * m / beta
+    exact = 1 / (inu - e) - 1 / (inu + e)
+    assert np.abs(chi_w - exact).max() < 1e-6
+
+
+def test_vandermonde_inverse_exact(oracle):
+    V = np.array([[m ** j for j in range(8)] for m in range(1, 9)], dtype=object)
+    F = [[fractions.Fraction(int(V[i, j])) for j in range(8)] for i in range(8)]
+    # exact inverse by Gauss-Jordan over the rationals
+    A = [row[:] + [fractions.Fraction(int(i == j)) for j in range(8)] for i, row in enumerate(F)]
+    for k in range(8):
+        p = next(i for i in range(k, 8) if A[i][k] != 0)
+        A[k], A[p] = A[p], A[k]
+        piv = A[k][k]
+        A[k] = [x / piv for x in A[k]]
+        for i in range(8):
+            if i != k and A[i][k] != 0:
+                f = A[i][k]
+                A[i] = [a - f * b for a, b in zip(A[i], A[k])]
+    exact = np.array([[float(A[i][8 + j]) for j in range(8)] for i in range(8)])
+    assert np.array_equal(oracle.vandermonde_inverse8(), exact)
+
+
+# ------------------------------------------------------------------ Lindhard and the tau route
+def test_eigh_vs_numpy(oracle):
+    rng = np.random.default_rng(4)
+    for n in (1, 2, 3, 5):
+        a = rnd(rng, n, n)
+        a = 0.5 * (a + a.conj().T)
+        ev, vec = oracle.eigh(a)
+        assert np.allclose(ev, np.linalg.eigvalsh(a), atol=1e-13)
+        assert relerr(vec @ np.diag(ev) @ vec.conj().T, a) <= 1e-13
+
+
+def lindhard_numpy(e_k, dims, beta, mu, nw):
+    """Lindhard bubble written with numpy eigh and broadcasting."""
+    nk, n = e_k.shape[0], e_k.shape[1]
+    ev, U = np.linalg.eigh(e_k - mu * np.eye(n))
+    k = np.array(np.unravel_index(np.arange(nk), dims)).T
+    f = lambda x: 1 / (np.exp(x) + 1)
+    m = np.arange(-(nw - 1), nw)
+    inu = 2j * np.pi * m / beta
+    out = np.zeros((len(m), nk, n, n, n, n), dtype=complex)
+    for q in range(nk):
+        kq = np.ravel_multi_index(((k + k[q]) % dims).T, dims)
+        de = ev[kq][:, None, :] - ev[:, :, None]          # [k, i, j]
+        dn = f(beta * ev)[:, :, None] - f(beta * ev[kq])[:, None, :]
+        for mi, z in enumerate(inu):
+            with np.errstate(divide="ignore", invalid="ignore"):
+                F = dn / (z + de)
+            if abs(z) < 1e-10:
+                deg = np.abs(de) < 1e-10
+                F = np.where(deg, beta / (4 * np.cosh(0.5 * beta * ev)[:, :, None] ** 2), F)
+            out[mi, q] = np.einsum("kai,kdi,kcj,kbj,kij->abcd", U, U.conj(), U[kq], U[kq].conj(), F) / nk
+    return out
+
+
+def test_lindhard_vs_numpy(oracle):
+    rng = np.random.default_rng(8)
+    dims, n = (3, 2, 1), 2
+    a = rnd(rng, 6, n, n)
+    e = 0.5 * (a + np.conj(np.swapaxes(a, 1, 2)))
+    got = oracle.lindhard_chi00(e, dims, 5.0, 0.2, 3)
+    assert relerr(got, lindhard_numpy(e, dims, 5.0, 0.2, 3)) <= 1e-12
+
+
+def chi00_square_lattice_ek():
+    """Two-orbital square lattice of test/python/chi00_square_lattice.py:47-76."""
+    h = np.array([[-0.3, -0.5], [-0.5, 0.4]])
+    T = -np.array([[1.0, 0.23], [0.23, 0.5]])
+    ks = [(2 * np.pi * i / 2, 2 * np.pi * j / 2) for i in range(2) for j in range(2)]
+    return np.array([h + 2 * (np.cos(kx) + np.cos(ky)) * T for kx, ky in ks]).astype(complex)
+
+
+def test_reference_kat_chi00_square_lattice(oracle):
+    """test/python/chi00_square_lattice.py:122-177: beta 20, 500 freqs, 2x2 k, 2
+    orbitals; the tau-route chi0(nu=0) matches Lindhard to decimal 8 (stored
+    product + tau->nu), the trapezoid variants to decimal 4, chi_wk to decimal 7."""
+    beta, mu, n_iw, dims = 20.0, 0.0, 500, (2, 2, 1)
+    e = chi00_square_lattice_ek()
+    g_tr = oracle.fourier_w_to_t(oracle.fourier_lattice(oracle.lattice_dyson_g_wk(beta, n_iw, mu, e), dims, -1), beta)
+    lind = oracle.lindhard_chi00(e, dims, beta, mu, 1)
+    lind_wr = oracle.fourier_lattice(lind, dims, -1)
+    chi_tr = oracle.chi0_tr_from_grt_PH(g_tr, dims)
+    chi_wr = oracle.fourier_t_to_w(chi_tr, beta, 1)
+    np.testing.assert_array_almost_equal(lind_wr, chi_wr, decimal=8)
+    np.testing.assert_array_almost_equal(lind_wr, oracle.chi_w0r_from_chi_tr(chi_tr, beta), decimal=4)
+    np.testing.assert_array_almost_equal(lind_wr, oracle.chi0_w0r_from_grt_PH(g_tr, dims, beta), decimal=4)
+    np.testing.assert_array_almost_equal(lind, oracle.fourier_lattice(chi_wr, dims, +1), decimal=7)
+    # fused product + tau->nu equals the stored-product chain (chi0_save_memory.py:26-30)
+    assert relerr(oracle.chi0_wr_from_grt_PH(g_tr, dims, beta, 1), chi_wr) <= 1e-13
+
+
+def test_tau_route_vs_lindhard_chiral_lattice(oracle):
+    """A 3x4 lattice with complex hoppings (e(k) != e(-k)) pins the orientation of
+    q and of r -> -r.  The reference's tau route pairs G_da(k) with G_bc(k - q)
+    (chi_imtime.cpp:127 with fourier_lattice.cpp:39-58) while lindhard_chi00 pairs
+    U_k(a,i) U_k^*(d,i) with U_{k+q}(c,j) U_{k+q}^*(b,j) (lindhard_chi00.cpp:82-84):
+    on such a lattice the two reference functions agree as
+    chi_tau[nu, q, a, b, c, d] = chi_lindhard[-nu, q, b, a, d, c]; on the
+    inversion-symmetric lattices of the reference's tests this is the identity."""
+    rng = np.random.default_rng(9)
+    dims, n, beta, mu, n_iw, nw = (3, 4, 1), 2, 8.0, 0.1, 400, 4
+    Tx = rnd(rng, n, n) * 0.4
+    Ty = rnd(rng, n, n) * 0.4
+    h = rnd(rng, n, n) * 0.3
+    h = 0.5 * (h + h.conj().T)
+    k = np.array(np.unravel_index(np.arange(12), dims)).T * 2 * np.pi / np.array(dims)
+    e = np.array([h + Tx * np.exp(1j * kk[0]) + Tx.conj().T * np.exp(-1j * kk[0]) + Ty * np.exp(1j * kk[1])
+                  + Ty.conj().T * np.exp(-1j * kk[1]) for kk in k])
+    assert np.abs(e[1] - e[2]).max() > 0.1  # k and -k differ
+    g_tr = oracle.fourier_w_to_t(oracle.fourier_lattice(oracle.lattice_dyson_g_wk(beta, n_iw, mu, e), dims, -1), beta)
+    chi_wk = oracle.fourier_lattice(oracle.chi0_wr_from_grt_PH(g_tr, dims, beta, nw), dims, +1)
+    lind = oracle.lindhard_chi00(e, dims, beta, mu, nw)
+    assert relerr(chi_wk, np.einsum("...abcd->...badc", lind[::-1])) <= 1e-6
+    assert relerr(chi_wk, lind) > 1e-2
+
+
+# ------------------------------------------------------------------ RPA
+def test_rpa_single_band(oracle):
+    rng = np.random.default_rng(1)
+    chi0 = rnd(rng, 50, 1, 1, 1, 1) * 0.3
+    U = np.full((1, 1, 1, 1), 1.7 + 0j)
+    assert relerr(oracle.solve_rpa_PH(chi0, U), chi0 / (1 - 1.7 * chi0)) <= 1e-14
+
+
+def test_rpa_vs_numpy_solve(oracle):
+    """PH grouping (a,b),(d,c) of rpa.cpp:40,54-55 written with numpy reshapes."""
+    rng = np.random.default_rng(2)
+    n = 3
+    chi0 = rnd(rng, 20, n, n, n, n) * 0.1
+    U = rnd(rng, n, n, n, n) * 0.3
+    X = chi0.transpose(0, 1, 2, 4, 3).reshape(-1, n * n, n * n)
+    MU = U.transpose(0, 1, 3, 2).reshape(n * n, n * n)
+    Y = np.linalg.solve(np.eye(n * n)[None] - X @ MU, X)
+    ref = Y.reshape(-1, n, n, n, n).transpose(0, 1, 2, 4, 3)
+    assert relerr(oracle.solve_rpa_PH(chi0, U), ref) <= 1e-13
+
+
+def test_rpa_fazekas_spinful_hubbard(oracle):
+    """test/python/mean_field.py:336-351: with the spinful Hubbard tensor and a
+    spin-diagonal bubble chi0_abcd = chi0 d_ad d_bc, chi_SzSz = 1/2 chi0/(1 - U chi0)
+    (Fazekas eq. 7.43 with the extra 1/2)."""
+    from tprf_amd.rpa_tensor import spinful_hubbard_tensor
+    U = 1.3
+    chi0 = np.array([0.1, 0.35, -0.2 + 0.05j, 0.6])
+    d = np.eye(2)
+    c0 = chi0[:, None, None, None, None] * np.einsum("ad,bc->abcd", d, d)[None]
+    chi = oracle.solve_rpa_PH(c0, spinful_hubbard_tensor(U))
+    Sz = 0.5 * np.diag([1.0, -1.0])
+    szsz = np.einsum("pabcd,ab,cd->p", chi, Sz, Sz)
+    assert relerr(szsz, 0.5 * chi0 / (1 - U * chi0)) <= 1e-14
+
+
+def test_kanamori_tensor_rules():
+    """interaction_tensor_charge_spin_factorization.py:60-63 and rpa_tensor.py:94-118."""
+    from tprf_amd.rpa_tensor import kanamori_charge_and_spin_quartic_interaction_tensors as K
+    U, J = 2.3, 0.4
+    Uc, Us = K(3, U, U - 2 * J, J, J)
+    assert Us[1, 1, 1, 1] == U and Uc[2, 2, 2, 2] == U
+    assert Us[0, 1, 1, 0] == U - 2 * J and Uc[0, 1, 1, 0] == -(U - 2 * J) + 2 * J
+    assert Us[0, 0, 1, 1] == J and Uc[0, 0, 1, 1] == 2 * (U - 2 * J) - J
+    assert Us[0, 1, 0, 1] == J and Uc[0, 1, 0, 1] == J
+    assert np.count_nonzero(Us) == 3 + 3 * 6
+
+
+def test_workload_filling_and_hermiticity():
+    from tprf_amd import workloads
+    wl = workloads.cubic_3orb(seed=0, L=8)
+    assert np.abs(wl.e_k - np.conj(np.swapaxes(wl.e_k, 1, 2))).max() < 1e-15
+    ev = np.linalg.eigvalsh(wl.e_k).ravel()
+    assert abs(workloads.fermi(wl.beta * (ev - wl.mu)).mean() - 1 / 3) < 1e-10
+    sq = workloads.plain_square()
+    assert sq.e_k.shape == (1024, 1, 1) and abs(sq.e_k[0, 0, 0] + 4) < 1e-15

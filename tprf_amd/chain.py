@@ -1,0 +1,65 @@
+"""The forward chain e_k -> G(w,k) -> G(w,r) -> G(tau,r) -> chi0(nu,r) -> chi0(nu,q)
+-> chi_RPA(nu,q) on preallocated device buffers (one "eval").
+
+Stages and the reference functions they replace:
+  lattice_dyson_g_wk (gf.cpp:126), fourier_wk_to_wr (gf.cpp:209),
+  fourier_wr_to_tr (gf.cpp:262), chi0_wr_from_grt_PH (chi_imtime.cpp:139, the
+  fused product + tau->nu of chi_imtime.cpp:279), chi_wk_from_chi_wr
+  (chi_imtime.cpp:289), solve_rpa_PH (rpa.cpp:66).
+Every stage is one asynchronous C-ABI call on torch's current stream; the
+tail-fit status is read once, after the chain.
+"""
+import torch
+
+from . import _lib
+
+
+class BubbleRPA:
+    STAGES = ("dyson", "wk_to_wr", "wr_to_tr", "chi0_wr", "wr_to_wk", "rpa")
+
+    def __init__(self, wl, device="cuda", save_intermediates=True):
+        self.wl = wl
+        dev = torch.device(device)
+        self.dev = dev
+        n, nk = wl.norb, wl.nk
+        c128 = torch.complex128
+        self.e_k = torch.from_numpy(wl.e_k).to(dev).contiguous()
+        self.U = torch.from_numpy(wl.U).to(dev).contiguous()
+        self.g_wk = torch.empty((2 * wl.n_iw, nk, n, n), dtype=c128, device=dev)
+        self.g_wr = torch.empty_like(self.g_wk)
+        self.g_tr = torch.empty((wl.ntau, nk, n, n), dtype=c128, device=dev)
+        self.chi0_wr = torch.empty((wl.n_nu, nk, n, n, n, n), dtype=c128, device=dev)
+        self.chi0_wk = torch.empty_like(self.chi0_wr)
+        self.chi_wk = torch.empty_like(self.chi0_wr)
+        self.dims = _lib.dims3(wl.dims)
+        self.ctx = _lib.ctx(dev.index if dev.index is not None else torch.cuda.current_device())
+
+    def stage(self, name):
+        wl, c, p = self.wl, self.ctx, _lib.ptr
+        n, nk = wl.norb, wl.nk
+        if name == "dyson":
+            _lib.call("tprf_lattice_dyson_g_wk", c, wl.beta, wl.n_iw, wl.mu, nk, n, p(self.e_k), _lib.SIGMA_NONE, None,
+                      p(self.g_wk))
+        elif name == "wk_to_wr":
+            _lib.call("tprf_fourier_wk_to_wr", c, 2 * wl.n_iw, self.dims, n * n, p(self.g_wk), p(self.g_wr))
+        elif name == "wr_to_tr":
+            _lib.call("tprf_fourier_wr_to_tr", c, _lib.FERMION, wl.beta, wl.n_iw, wl.ntau, nk * n * n, p(self.g_wr),
+                      p(self.g_tr), None)
+        elif name == "chi0_wr":
+            _lib.call("tprf_chi0_wr_from_grt_PH", c, wl.beta, wl.ntau, self.dims, n, wl.nb, p(self.g_tr),
+                      p(self.chi0_wr))
+        elif name == "wr_to_wk":
+            _lib.call("tprf_fourier_wr_to_wk", c, wl.n_nu, self.dims, n ** 4, p(self.chi0_wr), p(self.chi0_wk))
+        elif name == "rpa":
+            _lib.call("tprf_solve_rpa_PH", c, wl.n_nu * nk, n, p(self.chi0_wk), p(self.U), p(self.chi_wk))
+        else:
+            raise KeyError(name)
+
+    def run(self):
+        _lib.tail_status_reset(self.ctx)
+        for s in self.STAGES:
+            self.stage(s)
+
+    def check_status(self):
+        """Synchronise and return (max tail-fit residual, max |m0|); raises on singular inversions."""
+        return _lib.tail_status(self.ctx)

@@ -1,0 +1,505 @@
+// capi.cpp -- host side of libtprf_hip: context, plan and table caches, and the
+// extern "C" entry points declared in include/tprf_hip.h.
+//
+// Entry points validate shapes the way the reference's preconditions do,
+// build (once per shape) the FFT plans and Matsubara tables in device memory,
+// and launch the kernels of kernels.hip on the context's stream.
+#include "../../include/tprf_hip.h"
+
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "common.hpp"
+
+using tprf::c2;
+
+namespace tprf {
+hipError_t set_kernel_attributes();
+}
+
+struct FftEntry {
+  tprf::FftDev dev;
+  c2 *tw = nullptr;
+};
+
+struct TailEntry {
+  tprf::TailFitDev dev;
+  int *idx = nullptr;
+  c2 *pinv = nullptr;
+  c2 *vander = nullptr;
+};
+
+struct MatsEntry {
+  tprf::MatsDev dev;
+  double *psi = nullptr;
+  c2 *phase = nullptr;
+};
+
+struct tprf_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  double *d_status = nullptr;
+  std::map<int, FftEntry> fft;
+  std::map<std::tuple<int, double, int>, TailEntry> fits;
+  std::map<std::tuple<int, double, int>, MatsEntry> mats;
+  tprf::Vinv8 vinv;
+};
+
+namespace {
+
+constexpr long double kPiL = 3.141592653589793238462643383279502884L;
+
+int fail(tprf_ctx *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int hip_fail(tprf_ctx *ctx, hipError_t e, const char *where) {
+  return fail(ctx, TPRF_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define TPRF_HIP_TRY(ctx, expr, where)          \
+  do {                                          \
+    hipError_t _e = (expr);                     \
+    if (_e != hipSuccess) return hip_fail(ctx, _e, where); \
+  } while (0)
+
+bool factorize(int n, std::vector<int> &f) {
+  f.clear();
+  while (n % 8 == 0) { f.push_back(8); n /= 8; }
+  while (n % 4 == 0) { f.push_back(4); n /= 4; }
+  while (n % 2 == 0) { f.push_back(2); n /= 2; }
+  for (int p : {3, 5, 7, 11, 13})
+    while (n % p == 0) { f.push_back(p); n /= p; }
+  return n == 1 && (int)f.size() <= tprf::kMaxPass;
+}
+
+template <typename T>
+int upload(tprf_ctx *ctx, T **dst, const std::vector<T> &src) {
+  TPRF_HIP_TRY(ctx, hipMalloc((void **)dst, sizeof(T) * src.size()), "hipMalloc");
+  TPRF_HIP_TRY(ctx, hipMemcpyAsync(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, ctx->stream),
+               "hipMemcpyAsync");
+  TPRF_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  return TPRF_OK;
+}
+
+// FFT plan of length n (twiddles exp(-2 pi i k/n), long double angles)
+int get_fft(tprf_ctx *ctx, int n, const tprf::FftDev **out) {
+  auto it = ctx->fft.find(n);
+  if (it != ctx->fft.end()) { *out = &it->second.dev; return TPRF_OK; }
+  std::vector<int> f;
+  if (n < 1 || !factorize(n, f))
+    return fail(ctx, TPRF_ERR_UNSUPPORTED, "FFT length " + std::to_string(n) + " has a prime factor above 13");
+  std::vector<c2> tw(n);
+  for (int k = 0; k < n; ++k) {
+    long double a = 2.0L * kPiL * (long double)k / (long double)n;
+    tw[k] = c2{(double)cosl(a), -(double)sinl(a)};
+  }
+  FftEntry e;
+  int st = upload(ctx, &e.tw, tw);
+  if (st) return st;
+  e.dev.n = n;
+  e.dev.npass = (int)f.size();
+  for (int i = 0; i < tprf::kMaxPass; ++i) e.dev.radix[i] = i < (int)f.size() ? f[i] : 1;
+  e.dev.tw = e.tw;
+  auto res = ctx->fft.emplace(n, e);
+  *out = &res.first->second.dev;
+  return TPRF_OK;
+}
+
+double matsubara(int statistic, long n, double beta) {
+  return statistic == TPRF_FERMION ? M_PI * (double)(2 * n + 1) / beta : 2.0 * M_PI * (double)n / beta;
+}
+
+// Tail-fit operator of one Matsubara mesh.  Window and order restate the
+// TRIQS 3.2 tail_fitter defaults that fit_tail uses (called at
+// c++/triqs_tprf/fourier/fourier_matsubara.cpp:226): 20% of each half mesh,
+// at most 30 points per side from the outermost frequency inward with an
+// integer step, expansion order min(9, floor(ln 1e16 / ln omega_max)), fit in
+// z = omega_max / (i omega_n).  The least-squares operator pinv = R^-1 Q^H is
+// built once by modified Gram-Schmidt (two passes) in long double.
+int get_tail(tprf_ctx *ctx, int statistic, double beta, int n_max, const tprf::TailFitDev **out) {
+  auto key = std::make_tuple(statistic, beta, n_max);
+  auto it = ctx->fits.find(key);
+  if (it != ctx->fits.end()) { *out = &it->second.dev; return TPRF_OK; }
+  const int N = statistic == TPRF_FERMION ? 2 * n_max : 2 * n_max - 1;
+  const int first = statistic == TPRF_FERMION ? -n_max : -(n_max - 1);
+  const double wmax = statistic == TPRF_FERMION ? M_PI * (2 * n_max - 1) / beta : 2 * M_PI * (n_max - 1) / beta;
+  int W = (int)std::lround(0.2 * N / 2.0);
+  if (W < 1) W = 1;
+  const int npts = W < 30 ? W : 30;
+  int step = W / npts;
+  if (step < 1) step = 1;
+  if ((npts - 1) * step >= (N - 1) / 2)
+    return fail(ctx, TPRF_ERR_TAIL_ORDER, "Matsubara mesh too small for the high-frequency tail fit");
+  const int nfit = 2 * npts;
+  int K = 9;
+  if (wmax > 1.0) {
+    int k2 = (int)std::floor(std::log(1e16) / std::log(wmax));
+    if (k2 < K) K = k2;
+  }
+  if (K > nfit - 1) K = nfit - 1;
+  if (K < 4)
+    return fail(ctx, TPRF_ERR_TAIL_ORDER,
+                "ERROR: Inverse Fourier implementation requires at least a proper 3rd high-frequency moment");
+  const int nc = K + 1;
+  std::vector<int> idx(nfit);
+  for (int i = 0; i < npts; ++i) { idx[i] = i * step; idx[npts + i] = N - 1 - i * step; }
+  typedef std::complex<long double> lc;
+  std::vector<lc> V(nfit * nc), Q(nfit * nc), R(nc * nc, lc(0));
+  for (int r = 0; r < nfit; ++r) {
+    lc z = lc(wmax) / lc(0.0L, (long double)matsubara(statistic, first + idx[r], beta));
+    lc pw = 1;
+    for (int c = 0; c < nc; ++c) { V[r * nc + c] = pw; pw *= z; }
+  }
+  for (int c = 0; c < nc; ++c) {
+    std::vector<lc> v(nfit);
+    for (int r = 0; r < nfit; ++r) v[r] = V[r * nc + c];
+    for (int pass = 0; pass < 2; ++pass)
+      for (int p = 0; p < c; ++p) {
+        lc s = 0;
+        for (int r = 0; r < nfit; ++r) s += std::conj(Q[r * nc + p]) * v[r];
+        for (int r = 0; r < nfit; ++r) v[r] -= s * Q[r * nc + p];
+        R[p * nc + c] += s;
+      }
+    long double nv = 0;
+    for (int r = 0; r < nfit; ++r) nv += std::norm(v[r]);
+    nv = std::sqrt(nv);
+    R[c * nc + c] = nv;
+    for (int r = 0; r < nfit; ++r) Q[r * nc + c] = v[r] / nv;
+  }
+  // pinv = R^-1 Q^H  (nc x nfit)
+  std::vector<c2> pinv(nc * nfit), vd(nfit * nc);
+  for (int f = 0; f < nfit; ++f) {
+    std::vector<lc> x(nc);
+    for (int r = nc - 1; r >= 0; --r) {
+      lc s = std::conj(Q[f * nc + r]);
+      for (int c = r + 1; c < nc; ++c) s -= R[r * nc + c] * x[c];
+      x[r] = s / R[r * nc + r];
+    }
+    for (int r = 0; r < nc; ++r) pinv[r * nfit + f] = c2{(double)x[r].real(), (double)x[r].imag()};
+  }
+  for (int i = 0; i < nfit * nc; ++i) vd[i] = c2{(double)V[i].real(), (double)V[i].imag()};
+  TailEntry e;
+  int st;
+  if ((st = upload(ctx, &e.idx, idx))) return st;
+  if ((st = upload(ctx, &e.pinv, pinv))) return st;
+  if ((st = upload(ctx, &e.vander, vd))) return st;
+  e.dev.n_fit = nfit;
+  e.dev.ncoef = nc;
+  e.dev.idx = e.idx;
+  e.dev.pinv = e.pinv;
+  e.dev.vander = e.vander;
+  long double wp = 1;
+  for (int p = 0; p < 4; ++p) { e.dev.wpow[p] = (double)wp; wp *= (long double)wmax; }
+  auto res = ctx->fits.emplace(key, e);
+  *out = &res.first->second.dev;
+  return TPRF_OK;
+}
+
+// oneFermion / oneBoson with a = 1, c++/triqs_tprf/fourier/fourier_matsubara.cpp:31-37
+double one_fermion(double b, double tau, double beta) {
+  return -(b >= 0 ? std::exp(-b * tau) / (1 + std::exp(-beta * b))
+                  : std::exp(b * (beta - tau)) / (1 + std::exp(beta * b)));
+}
+double one_boson(double b, double tau, double beta) {
+  return (b >= 0 ? std::exp(-b * tau) / (std::exp(-beta * b) - 1) : std::exp(b * (beta - tau)) / (1 - std::exp(b * beta)));
+}
+
+int get_mats(tprf_ctx *ctx, int statistic, double beta, int L, const tprf::MatsDev **out) {
+  auto key = std::make_tuple(statistic, beta, L);
+  auto it = ctx->mats.find(key);
+  if (it != ctx->mats.end()) { *out = &it->second.dev; return TPRF_OK; }
+  const double bf[3] = {0.0, 1.0, -1.0}, bb[3] = {-0.5, -1.0, 1.0};
+  std::vector<double> psi(3 * (size_t)L);
+  std::vector<c2> phase(L);
+  for (int j = 0; j < L; ++j) {
+    const double tau = (double)j * (beta / (double)L);
+    for (int q = 0; q < 3; ++q)
+      psi[q * L + j] = statistic == TPRF_FERMION ? one_fermion(bf[q], tau, beta) : one_boson(bb[q], tau, beta);
+    long double a = kPiL * (long double)j / (long double)L;
+    phase[j] = c2{(double)cosl(a), -(double)sinl(a)};
+  }
+  MatsEntry e;
+  int st;
+  if ((st = upload(ctx, &e.psi, psi))) return st;
+  if ((st = upload(ctx, &e.phase, phase))) return st;
+  e.dev.statistic = statistic;
+  e.dev.L = L;
+  e.dev.beta = beta;
+  e.dev.psi = e.psi;
+  e.dev.phase = e.phase;
+  auto res = ctx->mats.emplace(key, e);
+  *out = &res.first->second.dev;
+  return TPRF_OK;
+}
+
+// exact inverse of V_{m,j} = m^{j-1}, m, j = 1..8: row j-1 holds the t^{j-1}
+// coefficients of the Lagrange polynomials on the nodes 1..8 (integer ratios).
+void vandermonde_inverse8(double *vinv) {
+  for (int m = 1; m <= 8; ++m) {
+    double poly[9] = {1, 0, 0, 0, 0, 0, 0, 0, 0};
+    double den = 1;
+    int deg = 0;
+    for (int k = 1; k <= 8; ++k) {
+      if (k == m) continue;
+      for (int d = deg + 1; d >= 1; --d) poly[d] = poly[d - 1] - k * poly[d];
+      poly[0] = -k * poly[0];
+      ++deg;
+      den *= (double)(m - k);
+    }
+    for (int j = 0; j < 8; ++j) vinv[j * 8 + (m - 1)] = poly[j] / den;
+  }
+}
+
+bool dims_ok(const int dims[3]) {
+  if (!dims) return false;
+  for (int d = 0; d < 3; ++d)
+    if (dims[d] < 1 || dims[d] > 4096) return false;
+  return true;
+}
+
+int check_launch(tprf_ctx *ctx, hipError_t e, const char *what) {
+  if (e != hipSuccess) return hip_fail(ctx, e, what);
+  return TPRF_OK;
+}
+
+int lattice_fft(tprf_ctx *ctx, long nw, const int dims[3], long ntarget, const c2 *in, c2 *out, int sign) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (nw < 1 || ntarget < 1 || !dims_ok(dims) || !in || !out) return fail(ctx, TPRF_ERR_ARG, "lattice fft: bad shape");
+  const long nk = (long)dims[0] * dims[1] * dims[2];
+  const double scale = sign < 0 ? 1.0 / (double)nk : 1.0;
+  int last = -1;
+  for (int d = 0; d < 3; ++d)
+    if (dims[d] > 1) last = d;
+  if (last < 0) {  // a single k-point: copy (scale is 1)
+    TPRF_HIP_TRY(ctx, hipMemcpyAsync(out, in, sizeof(c2) * nw * ntarget, hipMemcpyDeviceToDevice, ctx->stream),
+                 "hipMemcpyAsync");
+    return TPRF_OK;
+  }
+  const c2 *src = in;
+  for (int d = 2; d >= 0; --d) {
+    if (dims[d] == 1) continue;
+    const tprf::FftDev *p;
+    int st = get_fft(ctx, dims[d], &p);
+    if (st) return st;
+    long outer = nw, inner = ntarget;
+    for (int e = 0; e < d; ++e) outer *= dims[e];
+    for (int e = d + 1; e < 3; ++e) inner *= dims[e];
+    // the last axis transformed is axis 0's lowest non-trivial index: apply the scale there
+    int lowest = 0;
+    while (dims[lowest] == 1) ++lowest;
+    st = check_launch(ctx, tprf::launch_lattice_axis(ctx->stream, sign, src, out, outer, dims[d], inner, *p,
+                                                     d == lowest ? scale : 1.0),
+                      "lattice axis kernel");
+    if (st) return st;
+    src = out;
+  }
+  return TPRF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tprf_ctx_create(tprf_ctx **out, int device) {
+  if (!out) return TPRF_ERR_ARG;
+  *out = nullptr;
+  tprf_ctx *ctx = new tprf_ctx();
+  ctx->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipMalloc((void **)&ctx->d_status, 4 * sizeof(double));
+  if (e == hipSuccess) e = hipMemset(ctx->d_status, 0, 4 * sizeof(double));
+  if (e == hipSuccess) e = tprf::set_kernel_attributes();
+  if (e != hipSuccess) {
+    delete ctx;
+    return TPRF_ERR_HIP;
+  }
+  vandermonde_inverse8(ctx->vinv.v);
+  *out = ctx;
+  return TPRF_OK;
+}
+
+int tprf_ctx_destroy(tprf_ctx *ctx) {
+  if (!ctx) return TPRF_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  else (void)hipDeviceSynchronize();
+  for (auto &kv : ctx->fft) (void)hipFree(kv.second.tw);
+  for (auto &kv : ctx->fits) { (void)hipFree(kv.second.idx); (void)hipFree(kv.second.pinv); (void)hipFree(kv.second.vander); }
+  for (auto &kv : ctx->mats) { (void)hipFree(kv.second.psi); (void)hipFree(kv.second.phase); }
+  (void)hipFree(ctx->d_status);
+  delete ctx;
+  return TPRF_OK;
+}
+
+int tprf_ctx_set_stream(tprf_ctx *ctx, void *stream) {
+  if (!ctx) return TPRF_ERR_ARG;
+  ctx->stream = (hipStream_t)stream;
+  return TPRF_OK;
+}
+
+const char *tprf_ctx_last_error(const tprf_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int tprf_ctx_tail_status(tprf_ctx *ctx, double *max_fit_error, double *max_abs_m0) {
+  if (!ctx) return TPRF_ERR_ARG;
+  double h[4];
+  TPRF_HIP_TRY(ctx, hipMemcpyAsync(h, ctx->d_status, sizeof(h), hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");
+  TPRF_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  if (max_fit_error) *max_fit_error = h[0];
+  if (max_abs_m0) *max_abs_m0 = h[1];
+  if (h[2] != 0.0) return fail(ctx, TPRF_ERR_SINGULAR, "singular matrix in a Dyson or RPA inversion");
+  return TPRF_OK;
+}
+
+int tprf_ctx_tail_status_reset(tprf_ctx *ctx) {
+  if (!ctx) return TPRF_ERR_ARG;
+  TPRF_HIP_TRY(ctx, hipMemsetAsync(ctx->d_status, 0, 4 * sizeof(double), ctx->stream), "hipMemsetAsync");
+  return TPRF_OK;
+}
+
+int tprf_lattice_dyson_g_wk(tprf_ctx *ctx, double beta, int n_iw, double mu, long nk, int norb, const tprf_c128 *e_k,
+                            int sigma_kind, const tprf_c128 *sigma, tprf_c128 *g_wk) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (!(beta > 0) || n_iw < 1 || nk < 1 || !e_k || !g_wk) return fail(ctx, TPRF_ERR_ARG, "lattice_dyson_g_wk: bad argument");
+  if (sigma_kind < 0 || sigma_kind > 2 || (sigma_kind && !sigma))
+    return fail(ctx, TPRF_ERR_ARG, "lattice_dyson_g_wk: bad sigma");
+  if (norb < 1 || norb > 6) return fail(ctx, TPRF_ERR_UNSUPPORTED, "lattice_dyson_g_wk: norb must be 1..6");
+  return check_launch(ctx,
+                      tprf::launch_dyson(ctx->stream, beta, n_iw, mu, nk, norb, (const c2 *)e_k, sigma_kind,
+                                         (const c2 *)sigma, (c2 *)g_wk, tprf::Status{ctx->d_status}),
+                      "dyson kernel");
+}
+
+int tprf_fourier_wk_to_wr(tprf_ctx *ctx, long nw, const int dims[3], long ntarget, const tprf_c128 *in,
+                          tprf_c128 *out) {
+  return lattice_fft(ctx, nw, dims, ntarget, (const c2 *)in, (c2 *)out, -1);
+}
+
+int tprf_fourier_wr_to_wk(tprf_ctx *ctx, long nw, const int dims[3], long ntarget, const tprf_c128 *in,
+                          tprf_c128 *out) {
+  return lattice_fft(ctx, nw, dims, ntarget, (const c2 *)in, (c2 *)out, +1);
+}
+
+int tprf_fourier_wr_to_tr(tprf_ctx *ctx, int statistic, double beta, int n_max, int ntau, long ncol,
+                          const tprf_c128 *in, tprf_c128 *out, const tprf_c128 *moments) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if ((statistic != TPRF_FERMION && statistic != TPRF_BOSON) || !(beta > 0) || n_max < 1 || ncol < 1 || !in || !out)
+    return fail(ctx, TPRF_ERR_ARG, "fourier_wr_to_tr: bad argument");
+  if (ntau <= 0) ntau = 6 * n_max + 1;  // TRIQS make_adjoint_mesh default
+  const int L = ntau - 1;
+  if (L < 2 * n_max)
+    return fail(ctx, TPRF_ERR_MESH_TOO_SHORT,
+                "Inverse Fourier: The time mesh mush be at least twice as long as the freq mesh");
+  const int ld = (L + 1) | 1;
+  if (2L * ld * (long)sizeof(c2) > tprf::kMatsLdsBytes)
+    return fail(ctx, TPRF_ERR_UNSUPPORTED, "fourier_wr_to_tr: ntau too large for one workgroup");
+  const tprf::FftDev *p;
+  const tprf::MatsDev *mt;
+  tprf::TailFitDev fit{};
+  int st;
+  if ((st = get_fft(ctx, L, &p))) return st;
+  if ((st = get_mats(ctx, statistic, beta, L, &mt))) return st;
+  if (!moments) {
+    const tprf::TailFitDev *f;
+    if ((st = get_tail(ctx, statistic, beta, n_max, &f))) return st;
+    fit = *f;
+  }
+  return check_launch(ctx,
+                      tprf::launch_w2t(ctx->stream, statistic, beta, n_max, ncol, (const c2 *)in, (c2 *)out,
+                                       (const c2 *)moments, fit, *mt, *p, tprf::Status{ctx->d_status}),
+                      "w2t kernel");
+}
+
+int tprf_fourier_tr_to_wr(tprf_ctx *ctx, int statistic, double beta, int ntau, int nw, long ncol,
+                          const tprf_c128 *in, tprf_c128 *out) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if ((statistic != TPRF_FERMION && statistic != TPRF_BOSON) || !(beta > 0) || ntau < 10 || ncol < 1 || !in || !out)
+    return fail(ctx, TPRF_ERR_ARG, "fourier_tr_to_wr: bad argument");
+  const int L = ntau - 1;
+  if (nw <= 0) nw = L / 6;  // TRIQS make_adjoint_mesh default
+  if (nw < 1 || L < 2 * nw)
+    return fail(ctx, TPRF_ERR_MESH_TOO_SHORT,
+                "Fourier: The time mesh mush be at least twice as long as the number of positive frequencies");
+  const int ld = (L + 1) | 1;
+  if (2L * ld * (long)sizeof(c2) > tprf::kMatsLdsBytes)
+    return fail(ctx, TPRF_ERR_UNSUPPORTED, "fourier_tr_to_wr: ntau too large for one workgroup");
+  const tprf::FftDev *p;
+  const tprf::MatsDev *mt;
+  int st;
+  if ((st = get_fft(ctx, L, &p))) return st;
+  if ((st = get_mats(ctx, statistic, beta, L, &mt))) return st;
+  return check_launch(ctx,
+                      tprf::launch_t2w(ctx->stream, statistic, beta, nw, ncol, (const c2 *)in, (c2 *)out, *mt, *p,
+                                       ctx->vinv),
+                      "t2w kernel");
+}
+
+int tprf_chi0_tr_from_grt_PH(tprf_ctx *ctx, int ntau, const int dims[3], int norb, const tprf_c128 *g_tr,
+                             tprf_c128 *chi0_tr) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (ntau < 2 || !dims_ok(dims) || norb < 1 || norb > 8 || !g_tr || !chi0_tr)
+    return fail(ctx, TPRF_ERR_ARG, "chi0_tr_from_grt_PH: bad argument");
+  return check_launch(ctx, tprf::launch_chi0_tr(ctx->stream, ntau, dims, norb, (const c2 *)g_tr, (c2 *)chi0_tr),
+                      "chi0_tr kernel");
+}
+
+int tprf_chi0_wr_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb, int nw,
+                             const tprf_c128 *g_tr, tprf_c128 *chi0_wr) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (!(beta > 0) || ntau < 10 || !dims_ok(dims) || norb < 1 || norb > 4 || nw < 1 || !g_tr || !chi0_wr)
+    return fail(ctx, TPRF_ERR_ARG, "chi0_wr_from_grt_PH: bad argument");
+  const int L = ntau - 1;
+  if (L < 2 * nw)
+    return fail(ctx, TPRF_ERR_MESH_TOO_SHORT,
+                "Fourier: The time mesh mush be at least twice as long as the number of positive frequencies");
+  const int ld = (L + 1) | 1;
+  if (2L * ld * (long)sizeof(c2) > tprf::kMatsLdsBytes)
+    return fail(ctx, TPRF_ERR_UNSUPPORTED, "chi0_wr_from_grt_PH: ntau too large for one workgroup");
+  const tprf::FftDev *p;
+  const tprf::MatsDev *mt;
+  int st;
+  if ((st = get_fft(ctx, L, &p))) return st;
+  if ((st = get_mats(ctx, TPRF_BOSON, beta, L, &mt))) return st;
+  return check_launch(ctx,
+                      tprf::launch_chi0_wr(ctx->stream, beta, nw, dims, norb, (const c2 *)g_tr, (c2 *)chi0_wr, *mt,
+                                           *p, ctx->vinv),
+                      "chi0_wr kernel");
+}
+
+int tprf_chi0_w0r_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb,
+                              const tprf_c128 *g_tr, tprf_c128 *chi0_wr) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (!(beta > 0) || ntau < 2 || !dims_ok(dims) || norb < 1 || norb > 8 || !g_tr || !chi0_wr)
+    return fail(ctx, TPRF_ERR_ARG, "chi0_w0r_from_grt_PH: bad argument");
+  return check_launch(ctx, tprf::launch_chi0_w0r(ctx->stream, beta, ntau, dims, norb, (const c2 *)g_tr, (c2 *)chi0_wr),
+                      "chi0_w0r kernel");
+}
+
+int tprf_chi_w0r_from_chi_tr(tprf_ctx *ctx, double beta, int ntau, long nr, int norb, const tprf_c128 *chi_tr,
+                             tprf_c128 *chi_wr) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (!(beta > 0) || ntau < 2 || nr < 1 || norb < 1 || norb > 8 || !chi_tr || !chi_wr)
+    return fail(ctx, TPRF_ERR_ARG, "chi_w0r_from_chi_tr: bad argument");
+  return check_launch(ctx, tprf::launch_chi_w0r(ctx->stream, beta, ntau, nr, norb, (const c2 *)chi_tr, (c2 *)chi_wr),
+                      "chi_w0r kernel");
+}
+
+int tprf_solve_rpa_PH(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *chi0, const tprf_c128 *U, tprf_c128 *chi) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (npts < 1 || !chi0 || !U || !chi) return fail(ctx, TPRF_ERR_ARG, "solve_rpa_PH: bad argument");
+  if (norb < 1 || norb > 4) return fail(ctx, TPRF_ERR_UNSUPPORTED, "solve_rpa_PH: norb must be 1..4");
+  return check_launch(ctx,
+                      tprf::launch_rpa(ctx->stream, npts, norb, (const c2 *)chi0, (const c2 *)U, (c2 *)chi,
+                                       tprf::Status{ctx->d_status}),
+                      "rpa kernel");
+}
+
+}  // extern "C"

@@ -1,0 +1,92 @@
+// common.hpp -- shared device helpers and launcher declarations for libtprf_hip.
+//
+// Complex numbers are double2 {x = re, y = im}, the layout of std::complex<double>
+// and of tprf_c128.  All arithmetic is FP64.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace tprf {
+
+using c2 = double2;
+
+__host__ __device__ __forceinline__ c2 mk(double re, double im) { return c2{re, im}; }
+__host__ __device__ __forceinline__ c2 cadd(c2 a, c2 b) { return c2{a.x + b.x, a.y + b.y}; }
+__host__ __device__ __forceinline__ c2 csub(c2 a, c2 b) { return c2{a.x - b.x, a.y - b.y}; }
+__host__ __device__ __forceinline__ c2 cmul(c2 a, c2 b) { return c2{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__host__ __device__ __forceinline__ c2 cscale(c2 a, double s) { return c2{a.x * s, a.y * s}; }
+__host__ __device__ __forceinline__ c2 cconj(c2 a) { return c2{a.x, -a.y}; }
+__host__ __device__ __forceinline__ double cabs2(c2 a) { return a.x * a.x + a.y * a.y; }
+// a / b, computed as a * conj(b) / |b|^2
+__host__ __device__ __forceinline__ c2 cdiv(c2 a, c2 b) {
+  double d = 1.0 / (b.x * b.x + b.y * b.y);
+  return c2{(a.x * b.x + a.y * b.y) * d, (a.y * b.x - a.x * b.y) * d};
+}
+// a * (i s): multiply by a purely imaginary number
+__host__ __device__ __forceinline__ c2 cmul_i(c2 a, double s) { return c2{-a.y * s, a.x * s}; }
+
+constexpr int kMaxPass = 24;
+
+// One FFT length: Stockham passes with the listed radices, twiddle table
+// tw[k] = exp(-2 pi i k / n) in device memory.
+struct FftDev {
+  int n;
+  int npass;
+  int radix[kMaxPass];
+  const c2 *tw;
+};
+
+// Least-squares tail fit of one Matsubara mesh (see tail_fit_make in capi.cpp).
+constexpr int kMaxFit = 64;
+constexpr int kMaxOrder = 10;  // coefficients 0..9
+struct TailFitDev {
+  int n_fit;             // fitted frequencies
+  int ncoef;             // K + 1
+  const int *idx;        // [n_fit] data indices into the Matsubara mesh
+  const c2 *pinv;        // [ncoef][n_fit] least-squares operator
+  const c2 *vander;      // [n_fit][ncoef] normalised Vandermonde
+  double wpow[4];        // omega_max^p, p = 0..3
+};
+
+// Per (statistic, beta, L) tables of the Matsubara transforms.
+struct MatsDev {
+  int statistic;         // 0 fermion, 1 boson
+  int L;                 // ntau - 1
+  double beta;
+  const double *psi;     // [3][L] pole-model functions of tau_j (a = 1)
+  const c2 *phase;       // [L] exp(-i pi j / L) (fermions), unused for bosons
+};
+
+// Inverse 8x8 Vandermonde of the boundary-derivative fit, passed by value.
+struct Vinv8 {
+  double v[64];
+};
+
+// Status words written by kernels (device memory, 4 doubles):
+//   [0] max RMS tail-fit residual, [1] max |m0|, [2] singular-matrix flag.
+struct Status {
+  double *d;
+};
+
+// ---- launchers (kernels.hip) -------------------------------------------
+hipError_t launch_dyson(hipStream_t s, double beta, int n_iw, double mu, long nk, int n, const c2 *ek, int sigma_kind,
+                        const c2 *sigma, c2 *g, Status st);
+hipError_t launch_lattice_axis(hipStream_t s, int sign, const c2 *in, c2 *out, long outer, int len, long inner,
+                               const FftDev &p, double scale);
+hipError_t launch_w2t(hipStream_t s, int statistic, double beta, int n_max, long ncol, const c2 *in, c2 *out,
+                      const c2 *moments, const TailFitDev &fit, const MatsDev &mt, const FftDev &p, Status st);
+hipError_t launch_t2w(hipStream_t s, int statistic, double beta, int nw, long ncol, const c2 *in, c2 *out,
+                      const MatsDev &mt, const FftDev &p, const Vinv8 &vinv);
+hipError_t launch_chi0_wr(hipStream_t s, double beta, int nw, const int dims[3], int n, const c2 *gtr, c2 *out,
+                          const MatsDev &mt, const FftDev &p, const Vinv8 &vinv);
+hipError_t launch_chi0_tr(hipStream_t s, int ntau, const int dims[3], int n, const c2 *gtr, c2 *out);
+hipError_t launch_chi0_w0r(hipStream_t s, double beta, int ntau, const int dims[3], int n, const c2 *gtr, c2 *out);
+hipError_t launch_chi_w0r(hipStream_t s, double beta, int ntau, long nr, int n, const c2 *chi_tr, c2 *out);
+hipError_t launch_rpa(hipStream_t s, long npts, int n, const c2 *chi0, const c2 *U, c2 *chi, Status st);
+
+// largest LDS footprint (bytes) a Matsubara-axis kernel may request
+constexpr int kMatsLdsBytes = 128 * 1024;
+int mats_cols_per_block(int L);
+
+}  // namespace tprf

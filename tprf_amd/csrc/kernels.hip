@@ -1,0 +1,648 @@
+// kernels.hip -- CDNA4 (gfx950) kernels of the lattice bubble + RPA chain.
+//
+// Every kernel computes in complex FP64.  Data are in the reference's
+// row-major gf layout [mesh0][mesh1][target], so each kernel's global
+// accesses are contiguous along the innermost (orbital / r) index.
+#include "common.hpp"
+#include "fft_device.hpp"
+
+namespace tprf {
+
+constexpr double kPi = 3.14159265358979323846;
+
+__device__ __forceinline__ void atomic_max_nonneg(double *addr, double v) {
+  // non-negative doubles order like their bit patterns
+  atomicMax(reinterpret_cast<unsigned long long *>(addr), (unsigned long long)__double_as_longlong(v));
+}
+
+// ------------------------------------------------------------------------
+// a1 Dyson: G[w,k] = inverse((i w_n + mu) 1 - e_k - Sigma), one thread per
+// (w, k); Gauss-Jordan with partial pivoting on [M | 1] in registers.
+template <int N>
+__global__ void __launch_bounds__(256) k_dyson(double beta, int n_iw, double mu, long nk, const c2 *__restrict__ ek,
+                                               int sigma_kind, const c2 *__restrict__ sigma, c2 *__restrict__ g,
+                                               long npts, double *status) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= npts) return;
+  const long iw = idx / nk, k = idx - iw * nk;
+  const double wn = kPi * (double)(2 * (iw - n_iw) + 1) / beta;
+  c2 A[N][N], B[N][N];
+  const c2 *e = ek + k * N * N;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      c2 v = e[i * N + j];
+      v.x = -v.x;
+      v.y = -v.y;
+      if (i == j) { v.x += mu; v.y += wn; }
+      if (sigma_kind == 1) v = csub(v, sigma[iw * N * N + i * N + j]);
+      else if (sigma_kind == 2) v = csub(v, sigma[idx * N * N + i * N + j]);
+      A[i][j] = v;
+      B[i][j] = (i == j) ? c2{1.0, 0.0} : c2{0.0, 0.0};
+    }
+  bool singular = false;
+#pragma unroll
+  for (int kk = 0; kk < N; ++kk) {
+    int p = kk;
+    double best = cabs2(A[kk][kk]);
+#pragma unroll
+    for (int i = kk + 1; i < N; ++i) {
+      double v = cabs2(A[i][kk]);
+      if (v > best) { best = v; p = i; }
+    }
+    if (best == 0.0) singular = true;
+#pragma unroll
+    for (int i = kk + 1; i < N; ++i) {
+      if (p == i) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          c2 t = A[kk][j]; A[kk][j] = A[i][j]; A[i][j] = t;
+          t = B[kk][j]; B[kk][j] = B[i][j]; B[i][j] = t;
+        }
+      }
+    }
+    const c2 inv = cdiv(c2{1.0, 0.0}, A[kk][kk]);
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      A[kk][j] = cmul(A[kk][j], inv);
+      B[kk][j] = cmul(B[kk][j], inv);
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      if (i == kk) continue;
+      const c2 f = A[i][kk];
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        A[i][j] = csub(A[i][j], cmul(f, A[kk][j]));
+        B[i][j] = csub(B[i][j], cmul(f, B[kk][j]));
+      }
+    }
+  }
+  if (singular) status[2] = 1.0;
+  c2 *o = g + idx * N * N;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) o[i * N + j] = B[i][j];
+}
+
+hipError_t launch_dyson(hipStream_t s, double beta, int n_iw, double mu, long nk, int n, const c2 *ek, int sigma_kind,
+                        const c2 *sigma, c2 *g, Status st) {
+  const long npts = 2L * n_iw * nk;
+  const int bs = 256;
+  const long nb = (npts + bs - 1) / bs;
+  switch (n) {
+#define TPRF_DYSON_CASE(N)                                                                               \
+  case N:                                                                                                \
+    hipLaunchKernelGGL(k_dyson<N>, dim3(nb), dim3(bs), 0, s, beta, n_iw, mu, nk, ek, sigma_kind, sigma, g, \
+                       npts, st.d);                                                                      \
+    break;
+    TPRF_DYSON_CASE(1)
+    TPRF_DYSON_CASE(2)
+    TPRF_DYSON_CASE(3)
+    TPRF_DYSON_CASE(4)
+    TPRF_DYSON_CASE(5)
+    TPRF_DYSON_CASE(6)
+#undef TPRF_DYSON_CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// a2 / a6: one axis of the batched lattice FFT.  The tensor is viewed as
+// [outer][len][inner]; a workgroup loads C consecutive inner columns of one
+// outer index (contiguous C*16 bytes per row), transforms them in LDS and
+// writes them back scaled.
+template <int S>
+__global__ void __launch_bounds__(256) k_lattice_axis(const c2 *__restrict__ in, c2 *__restrict__ out, long outer,
+                                                      int len, long inner, int C, FftDev p, double scale) {
+  extern __shared__ c2 sm[];
+  const int ld = len | 1;
+  c2 *a = sm, *b = sm + C * ld;
+  const long ntile_in = (inner + C - 1) / C;
+  const long tile = xcd_remap(blockIdx.x, gridDim.x);
+  const long o = tile / ntile_in;
+  const long c0 = (tile - o * ntile_in) * C;
+  const int cn = (int)((inner - c0) < C ? (inner - c0) : C);
+  const c2 *src = in + o * len * inner + c0;
+  for (int t = threadIdx.x; t < len * cn; t += blockDim.x) {
+    const int j = t / cn, c = t - j * cn;
+    a[c * ld + j] = src[(long)j * inner + c];
+  }
+  __syncthreads();
+  c2 *r = fft_lds<S>(a, b, ld, cn, p, threadIdx.x, blockDim.x);
+  c2 *dst = out + o * len * inner + c0;
+  for (int t = threadIdx.x; t < len * cn; t += blockDim.x) {
+    const int j = t / cn, c = t - j * cn;
+    dst[(long)j * inner + c] = cscale(r[c * ld + j], scale);
+  }
+}
+
+hipError_t launch_lattice_axis(hipStream_t s, int sign, const c2 *in, c2 *out, long outer, int len, long inner,
+                               const FftDev &p, double scale) {
+  const int ld = len | 1;
+  long C = 2048 / ld;
+  if (C < 1) C = 1;
+  if (C > inner) C = inner;
+  const long ntile = outer * ((inner + C - 1) / C);
+  const size_t lds = 2 * C * ld * sizeof(c2);
+  if (sign < 0)
+    hipLaunchKernelGGL(k_lattice_axis<-1>, dim3(ntile), dim3(256), lds, s, in, out, outer, len, inner, (int)C, p,
+                       scale);
+  else
+    hipLaunchKernelGGL(k_lattice_axis<1>, dim3(ntile), dim3(256), lds, s, in, out, outer, len, inner, (int)C, p,
+                       scale);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// Matsubara transforms.  Pole model of fourier_matsubara.cpp:150-171,274-288:
+// fermions b = (0, 1, -1), bosons b = (-1/2, -1, 1).
+template <int STAT>
+struct Poles;
+template <>
+struct Poles<0> {
+  static constexpr double b0 = 0.0, b1 = 1.0, b2 = -1.0;
+  __device__ static void coef(c2 m1, c2 m2, c2 m3, c2 *a) {
+    a[0] = csub(m1, m3);
+    a[1] = cscale(cadd(m2, m3), 0.5);
+    a[2] = cscale(csub(m3, m2), 0.5);
+  }
+};
+template <>
+struct Poles<1> {
+  static constexpr double b0 = -0.5, b1 = -1.0, b2 = 1.0;
+  __device__ static void coef(c2 m1, c2 m2, c2 m3, c2 *a) {
+    a[0] = cscale(csub(m1, m3), 4.0 / 3.0);
+    a[1] = csub(m3, cscale(cadd(m1, m2), 0.5));
+    a[2] = cadd(cadd(cscale(m1, 1.0 / 6.0), cscale(m2, 0.5)), cscale(m3, 1.0 / 3.0));
+  }
+};
+
+// sum_i a_i / (i w - b_i)
+template <int STAT>
+__device__ __forceinline__ c2 pole_sum(const c2 *a, double w) {
+  c2 iw = c2{0.0, w};
+  c2 s = cdiv(a[0], c2{-Poles<STAT>::b0, w});
+  s = cadd(s, cdiv(a[1], c2{-Poles<STAT>::b1, w}));
+  s = cadd(s, cdiv(a[2], c2{-Poles<STAT>::b2, w}));
+  (void)iw;
+  return s;
+}
+
+template <int STAT>
+__device__ __forceinline__ double matsubara_w(long n, double beta) {
+  return STAT == 0 ? kPi * (double)(2 * n + 1) / beta : 2.0 * kPi * (double)n / beta;
+}
+
+int mats_cols_per_block(int L) {
+  // two LDS buffers of C columns of (L+1) rounded to odd
+  const int ld = (L + 1) | 1;
+  int C = (int)(kMatsLdsBytes / (2 * ld * (int)sizeof(c2)));
+  if (C > 16) C = 16;
+  return C < 1 ? 1 : C;
+}
+
+// a3: Matsubara -> imaginary time for C columns (fourier_matsubara.cpp:218-308).
+template <int STAT>
+__global__ void __launch_bounds__(256) k_w2t(double beta, int n_max, long ncol, int C, const c2 *__restrict__ in,
+                                             c2 *__restrict__ out, const c2 *__restrict__ moments, TailFitDev fit,
+                                             MatsDev mt, FftDev p, double *status) {
+  extern __shared__ c2 sm[];
+  const int L = mt.L;
+  const int ld = (L + 1) | 1;
+  c2 *a = sm, *b = sm + C * ld;
+  __shared__ c2 s_coef[16][kMaxOrder];
+  __shared__ c2 s_a[16][3];
+  __shared__ c2 s_m1[16];
+  __shared__ double s_res[16];
+  const long ntile = (ncol + C - 1) / C;
+  const long tile = xcd_remap(blockIdx.x, gridDim.x);
+  const long c0 = tile * C;
+  const int cn = (int)((ncol - c0) < C ? (ncol - c0) : C);
+  (void)ntile;
+  const int nmesh = (STAT == 0) ? 2 * n_max : 2 * n_max - 1;
+  const int first = (STAT == 0) ? -n_max : -(n_max - 1);
+
+  // 1. tail moments
+  if (moments == nullptr) {
+    for (int t = threadIdx.x; t < cn * fit.ncoef; t += blockDim.x) {
+      const int c = t / fit.ncoef, q = t - c * fit.ncoef;
+      c2 s = c2{0.0, 0.0};
+      for (int f = 0; f < fit.n_fit; ++f) s = cadd(s, cmul(fit.pinv[q * fit.n_fit + f], in[(long)fit.idx[f] * ncol + c0 + c]));
+      s_coef[c][q] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < cn) {
+      const int c = threadIdx.x;
+      double r2 = 0.0;
+      for (int f = 0; f < fit.n_fit; ++f) {
+        c2 v = in[(long)fit.idx[f] * ncol + c0 + c];
+        for (int q = 0; q < fit.ncoef; ++q) v = csub(v, cmul(fit.vander[f * fit.ncoef + q], s_coef[c][q]));
+        r2 += cabs2(v);
+      }
+      s_res[c] = sqrt(r2 / fit.n_fit);
+      c2 m1 = cscale(s_coef[c][1], fit.wpow[1]);
+      c2 m2 = cscale(s_coef[c][2], fit.wpow[2]);
+      c2 m3 = cscale(s_coef[c][3], fit.wpow[3]);
+      Poles<STAT>::coef(m1, m2, m3, s_a[c]);
+      s_m1[c] = m1;
+      atomic_max_nonneg(&status[0], s_res[c]);
+      atomic_max_nonneg(&status[1], sqrt(cabs2(s_coef[c][0])));
+    }
+  } else if (threadIdx.x < cn) {
+    const int c = threadIdx.x;
+    c2 m1 = moments[c0 + c], m2 = moments[ncol + c0 + c], m3 = moments[2 * ncol + c0 + c];
+    Poles<STAT>::coef(m1, m2, m3, s_a[c]);
+    s_m1[c] = m1;
+  }
+  // 2. zero the FFT input, then scatter (1/beta)(G - model) to (n + L) % L
+  for (int t = threadIdx.x; t < cn * ld; t += blockDim.x) a[t] = c2{0.0, 0.0};
+  __syncthreads();
+  const double ib = 1.0 / beta;
+  for (int t = threadIdx.x; t < nmesh * cn; t += blockDim.x) {
+    const int i = t / cn, c = t - i * cn;
+    const long n = first + i;
+    const double w = matsubara_w<STAT>(n, beta);
+    c2 v = csub(in[(long)i * ncol + c0 + c], pole_sum<STAT>(s_a[c], w));
+    a[c * ld + (int)((n + L) % L)] = cscale(v, ib);
+  }
+  __syncthreads();
+  // 3. forward FFT of length L
+  c2 *y = fft_lds<-1>(a, b, ld, cn, p, threadIdx.x, blockDim.x);
+  // 4. G(tau_j) = y_j [exp(-i pi j/L)] + sum a_i psi_i(tau_j); G(tau_L) = -/+ (G(tau_0) + m1)
+  const double pm = (STAT == 0) ? -1.0 : 1.0;
+  for (int t = threadIdx.x; t < (L + 1) * cn; t += blockDim.x) {
+    const int j = t / cn, c = t - j * cn;
+    const int jj = (j == L) ? 0 : j;
+    c2 v = y[c * ld + jj];
+    if (STAT == 0) v = cmul(v, mt.phase[jj]);
+    const c2 *ac = s_a[c];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const double ps = mt.psi[q * L + jj];
+      v.x += ac[q].x * ps;
+      v.y += ac[q].y * ps;
+    }
+    if (j == L) v = cscale(cadd(v, s_m1[c]), pm);
+    out[(long)j * ncol + c0 + c] = v;
+  }
+}
+
+hipError_t launch_w2t(hipStream_t s, int statistic, double beta, int n_max, long ncol, const c2 *in, c2 *out,
+                      const c2 *moments, const TailFitDev &fit, const MatsDev &mt, const FftDev &p, Status st) {
+  const int C = mats_cols_per_block(mt.L);
+  const int ld = (mt.L + 1) | 1;
+  const long nb = (ncol + C - 1) / C;
+  const size_t lds = 2 * (size_t)C * ld * sizeof(c2);
+  if (statistic == 0)
+    hipLaunchKernelGGL(k_w2t<0>, dim3(nb), dim3(256), lds, s, beta, n_max, ncol, C, in, out, moments, fit, mt, p,
+                       st.d);
+  else
+    hipLaunchKernelGGL(k_w2t<1>, dim3(nb), dim3(256), lds, s, beta, n_max, ncol, C, in, out, moments, fit, mt, p,
+                       st.d);
+  return hipGetLastError();
+}
+
+// a5 core, shared by k_t2w and the fused product kernel: given the L+1 tau
+// samples of each column in a[c*ld + j], compute the boundary moments
+// (fit_derivatives, fourier_matsubara.cpp:42-97), subtract the pole model,
+// transform and write the 2nw-1 (bosons) or 2nw (fermions) frequencies.
+template <int STAT>
+__device__ void t2w_core(c2 *a, c2 *b, int ld, int cn, double beta, int nw, const MatsDev &mt, const FftDev &p,
+                         const Vinv8 &vinv, c2 (*s_a)[3], c2 *dst, long dst_row, long dst_col_stride) {
+  const int L = mt.L;
+  const double dtau = beta / (double)L;
+  if (threadIdx.x < cn) {
+    const int c = threadIdx.x;
+    const c2 *g = a + c * ld;
+    c2 gl0 = c2{0, 0}, gl1 = c2{0, 0}, gr0 = c2{0, 0}, gr1 = c2{0, 0};
+#pragma unroll
+    for (int m = 1; m <= 8; ++m) {
+      const double tm = (double)m * dtau;
+      const c2 dl = c2{(g[m].x - g[0].x) / tm, (g[m].y - g[0].y) / tm};
+      const c2 dr = c2{(g[L].x - g[L - m].x) / tm, (g[L].y - g[L - m].y) / tm};
+      gl0 = cadd(gl0, cscale(dl, vinv.v[0 * 8 + m - 1]));
+      gl1 = cadd(gl1, cscale(dl, vinv.v[1 * 8 + m - 1]));
+      gr0 = cadd(gr0, cscale(dr, vinv.v[0 * 8 + m - 1]));
+      gr1 = cadd(gr1, cscale(dr, vinv.v[1 * 8 + m - 1]));
+    }
+    const double sign = (STAT == 0) ? -1.0 : 1.0;
+    c2 m2 = csub(gl0, cscale(gr0, sign));
+    c2 m3 = cscale(cadd(gl1, cscale(gr1, sign)), -2.0 / dtau);
+    c2 m1 = (STAT == 0) ? cscale(cadd(g[0], g[L]), -1.0) : cscale(csub(g[0], g[L]), -1.0);
+    Poles<STAT>::coef(m1, m2, m3, s_a[c]);
+  }
+  __syncthreads();
+  const double fact = beta / (double)L;
+  for (int t = threadIdx.x; t < cn * L; t += blockDim.x) {
+    const int c = t / L, j = t - c * L;
+    const c2 *ac = s_a[c];
+    c2 model = c2{0, 0};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const double ps = mt.psi[q * L + j];
+      model.x += ac[q].x * ps;
+      model.y += ac[q].y * ps;
+    }
+    c2 v = csub(a[c * ld + j], model);
+    if (STAT == 0) v = cmul(cconj(mt.phase[j]), v);  // exp(+i pi tau_j / beta)
+    a[c * ld + j] = cscale(v, fact);
+  }
+  __syncthreads();
+  c2 *y = fft_lds<1>(a, b, ld, cn, p, threadIdx.x, blockDim.x);
+  const int nout = (STAT == 0) ? 2 * nw : 2 * nw - 1;
+  const int first = (STAT == 0) ? -nw : -(nw - 1);
+  for (int t = threadIdx.x; t < nout * cn; t += blockDim.x) {
+    const int i = t / cn, c = t - i * cn;
+    const long n = first + i;
+    const double w = matsubara_w<STAT>(n, beta);
+    c2 v = cadd(y[c * ld + (int)((n + L) % L)], pole_sum<STAT>(s_a[c], w));
+    dst[(long)i * dst_row + c * dst_col_stride] = v;
+  }
+}
+
+template <int STAT>
+__global__ void __launch_bounds__(256) k_t2w(double beta, int nw, long ncol, int C, const c2 *__restrict__ in,
+                                             c2 *__restrict__ out, MatsDev mt, FftDev p, Vinv8 vinv) {
+  extern __shared__ c2 sm[];
+  const int L = mt.L;
+  const int ld = (L + 1) | 1;
+  c2 *a = sm, *b = sm + C * ld;
+  __shared__ c2 s_a[16][3];
+  const long tile = xcd_remap(blockIdx.x, gridDim.x);
+  const long c0 = tile * C;
+  const int cn = (int)((ncol - c0) < C ? (ncol - c0) : C);
+  for (int t = threadIdx.x; t < (L + 1) * cn; t += blockDim.x) {
+    const int j = t / cn, c = t - j * cn;
+    a[c * ld + j] = in[(long)j * ncol + c0 + c];
+  }
+  __syncthreads();
+  t2w_core<STAT>(a, b, ld, cn, beta, nw, mt, p, vinv, s_a, out + c0, ncol, 1);
+}
+
+hipError_t launch_t2w(hipStream_t s, int statistic, double beta, int nw, long ncol, const c2 *in, c2 *out,
+                      const MatsDev &mt, const FftDev &p, const Vinv8 &vinv) {
+  const int C = mats_cols_per_block(mt.L);
+  const int ld = (mt.L + 1) | 1;
+  const long nb = (ncol + C - 1) / C;
+  const size_t lds = 2 * (size_t)C * ld * sizeof(c2);
+  if (statistic == 0)
+    hipLaunchKernelGGL(k_t2w<0>, dim3(nb), dim3(256), lds, s, beta, nw, ncol, C, in, out, mt, p, vinv);
+  else
+    hipLaunchKernelGGL(k_t2w<1>, dim3(nb), dim3(256), lds, s, beta, nw, ncol, C, in, out, mt, p, vinv);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// a4 / a4': bubble product chi0_abcd(tau_j, r) = G_da(tau_j, r) G_bc(tau_{L-j}, -r)
+// (chi_imtime.cpp:127).
+__device__ __forceinline__ long minus_r(long r, int d0, int d1, int d2) {
+  const long i2 = r % d2, i1 = (r / d2) % d1, i0 = r / ((long)d1 * d2);
+  const long j0 = (d0 - i0) % d0, j1 = (d1 - i1) % d1, j2 = (d2 - i2) % d2;
+  return (j0 * d1 + j1) * d2 + j2;
+}
+
+// fused product + bosonic tau->nu: one workgroup per (r, C consecutive abcd)
+__global__ void __launch_bounds__(256) k_chi0_wr(double beta, int nw, int d0, int d1, int d2, int n, int C,
+                                                 const c2 *__restrict__ gtr, c2 *__restrict__ out, MatsDev mt,
+                                                 FftDev p, Vinv8 vinv) {
+  extern __shared__ c2 sm[];
+  const int L = mt.L;
+  const int ld = (L + 1) | 1;
+  c2 *a = sm, *b = sm + C * ld;
+  __shared__ c2 s_a[16][3];
+  const long nr = (long)d0 * d1 * d2;
+  const int n2 = n * n, n4 = n2 * n2;
+  const int tiles_per_r = (n4 + C - 1) / C;
+  const long tile = xcd_remap(blockIdx.x, gridDim.x);
+  const long r = tile / tiles_per_r;
+  const int q0 = (int)(tile - r * tiles_per_r) * C;
+  const int cn = (n4 - q0) < C ? (n4 - q0) : C;
+  const long mr = minus_r(r, d0, d1, d2);
+  for (int t = threadIdx.x; t < (L + 1) * cn; t += blockDim.x) {
+    const int j = t / cn, c = t - j * cn;
+    const int q = q0 + c;
+    const int dd = q % n, cc = (q / n) % n, bb = (q / n2) % n, aa = q / (n2 * n);
+    const c2 gp = gtr[((long)j * nr + r) * n2 + dd * n + aa];
+    const c2 gm = gtr[((long)(L - j) * nr + mr) * n2 + bb * n + cc];
+    a[c * ld + j] = cmul(gp, gm);
+  }
+  __syncthreads();
+  t2w_core<1>(a, b, ld, cn, beta, nw, mt, p, vinv, s_a, out + r * n4 + q0, nr * n4, 1);
+}
+
+hipError_t launch_chi0_wr(hipStream_t s, double beta, int nw, const int dims[3], int n, const c2 *gtr, c2 *out,
+                          const MatsDev &mt, const FftDev &p, const Vinv8 &vinv) {
+  const int n4 = n * n * n * n;
+  int C = mats_cols_per_block(mt.L);
+  if (C > n4) C = n4;
+  const int ld = (mt.L + 1) | 1;
+  const long nr = (long)dims[0] * dims[1] * dims[2];
+  const long nb = nr * ((n4 + C - 1) / C);
+  const size_t lds = 2 * (size_t)C * ld * sizeof(c2);
+  hipLaunchKernelGGL(k_chi0_wr, dim3(nb), dim3(256), lds, s, beta, nw, dims[0], dims[1], dims[2], n, C, gtr, out, mt,
+                     p, vinv);
+  return hipGetLastError();
+}
+
+// materialised product, one thread per output element
+__global__ void __launch_bounds__(256) k_chi0_tr(int ntau, int d0, int d1, int d2, int n, const c2 *__restrict__ gtr,
+                                                 c2 *__restrict__ out, long total) {
+  const long nr = (long)d0 * d1 * d2;
+  const int n2 = n * n, n4 = n2 * n2;
+  const int L = ntau - 1;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const int q = (int)(idx % n4);
+    const long jr = idx / n4;
+    const long r = jr % nr, j = jr / nr;
+    const long mr = minus_r(r, d0, d1, d2);
+    const int dd = q % n, cc = (q / n) % n, bb = (q / n2) % n, aa = q / (n2 * n);
+    const c2 gp = gtr[(j * nr + r) * n2 + dd * n + aa];
+    const c2 gm = gtr[((L - j) * nr + mr) * n2 + bb * n + cc];
+    out[idx] = cmul(gp, gm);
+  }
+}
+
+hipError_t launch_chi0_tr(hipStream_t s, int ntau, const int dims[3], int n, const c2 *gtr, c2 *out) {
+  const long nr = (long)dims[0] * dims[1] * dims[2];
+  const long total = (long)ntau * nr * n * n * n * n;
+  long nb = (total + 255) / 256;
+  if (nb > 65536L * 8) nb = 65536L * 8;
+  hipLaunchKernelGGL(k_chi0_tr, dim3(nb), dim3(256), 0, s, ntau, dims[0], dims[1], dims[2], n, gtr, out, total);
+  return hipGetLastError();
+}
+
+// chi_trapz_tau (chi_imtime.cpp:230-247) of the product, one thread per (r, abcd)
+__global__ void __launch_bounds__(256) k_chi0_w0r(double beta, int ntau, int d0, int d1, int d2, int n,
+                                                  const c2 *__restrict__ gtr, c2 *__restrict__ out) {
+  const long nr = (long)d0 * d1 * d2;
+  const int n2 = n * n, n4 = n2 * n2;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nr * n4) return;
+  const int q = (int)(idx % n4);
+  const long r = idx / n4;
+  const long mr = minus_r(r, d0, d1, d2);
+  const int dd = q % n, cc = (q / n) % n, bb = (q / n2) % n, aa = q / (n2 * n);
+  const int L = ntau - 1;
+  c2 s = c2{0, 0}, first = c2{0, 0}, last = c2{0, 0};
+  for (int j = 0; j < ntau; ++j) {
+    const c2 v = cmul(gtr[((long)j * nr + r) * n2 + dd * n + aa], gtr[((long)(L - j) * nr + mr) * n2 + bb * n + cc]);
+    s = cadd(s, v);
+    if (j == 0) first = v;
+    if (j == L) last = v;
+  }
+  s = csub(s, cscale(first, 0.5));
+  s = csub(s, cscale(last, 0.5));
+  out[idx] = cscale(s, beta / (double)L);
+}
+
+hipError_t launch_chi0_w0r(hipStream_t s, double beta, int ntau, const int dims[3], int n, const c2 *gtr, c2 *out) {
+  const long nr = (long)dims[0] * dims[1] * dims[2];
+  const long total = nr * n * n * n * n;
+  hipLaunchKernelGGL(k_chi0_w0r, dim3((total + 255) / 256), dim3(256), 0, s, beta, ntau, dims[0], dims[1], dims[2], n,
+                     gtr, out);
+  return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) k_chi_w0r(double beta, int ntau, long ncol, const c2 *__restrict__ chi_tr,
+                                                 c2 *__restrict__ out) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ncol) return;
+  c2 s = c2{0, 0};
+  for (int j = 0; j < ntau; ++j) s = cadd(s, chi_tr[(long)j * ncol + idx]);
+  s = csub(s, cscale(chi_tr[idx], 0.5));
+  s = csub(s, cscale(chi_tr[(long)(ntau - 1) * ncol + idx], 0.5));
+  out[idx] = cscale(s, beta / (double)(ntau - 1));
+}
+
+hipError_t launch_chi_w0r(hipStream_t s, double beta, int ntau, long nr, int n, const c2 *chi_tr, c2 *out) {
+  const long ncol = nr * n * n * n * n;
+  hipLaunchKernelGGL(k_chi_w0r, dim3((ncol + 255) / 256), dim3(256), 0, s, beta, ntau, ncol, chi_tr, out);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// a7 RPA: for each point, M_chi = inverse(1 - M_chi0 M_U) M_chi0 (rpa.cpp:57),
+// solved as (1 - X U) Y = X by Gauss-Jordan with partial pivoting.  A group
+// of N2 = n^2 lanes owns one point, lane i holding row i of [A | X]; the pivot
+// row is exchanged through LDS.
+template <int N>
+__global__ void __launch_bounds__(256) k_rpa(long npts, const c2 *__restrict__ chi0, const c2 *__restrict__ U,
+                                             c2 *__restrict__ chi, double *status) {
+  constexpr int N2 = N * N;
+  constexpr int P = 64 / N2;  // points per wave
+  __shared__ c2 sU[N2 * N2];
+  __shared__ c2 slot[4 * P][2 * N2];
+  for (int t = threadIdx.x; t < N2 * N2; t += blockDim.x) {
+    // t = ((a*N + b)*N + c)*N + d  ->  M_U[(a,b),(d,c)]
+    const int d = t % N, c = (t / N) % N, ab = t / N2;
+    sU[ab * N2 + d * N + c] = U[t];
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int gidx = lane / N2, i = lane - gidx * N2;
+  const int slot_id = wave * P + gidx;
+  const long pt = ((long)blockIdx.x * 4 + wave) * P + gidx;
+  const bool valid = (gidx < P) && (pt < npts);
+  c2 X[N2], A[N2];
+  if (valid) {
+    const c2 *row = chi0 + pt * N2 * N2 + i * N2;
+#pragma unroll
+    for (int c = 0; c < N; ++c)
+#pragma unroll
+      for (int d = 0; d < N; ++d) X[d * N + c] = row[c * N + d];
+  } else {
+#pragma unroll
+    for (int j = 0; j < N2; ++j) X[j] = c2{0, 0};
+  }
+#pragma unroll
+  for (int j = 0; j < N2; ++j) {
+    c2 s = (i == j) ? c2{1.0, 0.0} : c2{0.0, 0.0};
+#pragma unroll
+    for (int l = 0; l < N2; ++l) s = csub(s, cmul(X[l], sU[l * N2 + j]));
+    A[j] = s;
+  }
+  bool used = false;
+  int myk = 0;
+  c2 pv = c2{1.0, 0.0};
+  bool singular = false;
+#pragma unroll
+  for (int k = 0; k < N2; ++k) {
+    const double mag = (!used && valid) ? cabs2(A[k]) : -1.0;
+    int best_i = 0;
+    double best = -2.0;
+#pragma unroll
+    for (int s = 0; s < N2; ++s) {
+      const double m = __shfl(mag, (lane - i) + s, 64);
+      if (m > best) { best = m; best_i = s; }
+    }
+    const bool is_piv = (i == best_i);
+    if (valid && best <= 0.0) singular = true;
+    if (is_piv && gidx < P) {
+      used = true;
+      myk = k;
+      pv = A[k];
+#pragma unroll
+      for (int j = 0; j < N2; ++j) { slot[slot_id][j] = A[j]; slot[slot_id][N2 + j] = X[j]; }
+    }
+    __syncthreads();
+    if (!is_piv && gidx < P) {
+      const c2 f = cdiv(A[k], slot[slot_id][k]);
+#pragma unroll
+      for (int j = k + 1; j < N2; ++j) A[j] = csub(A[j], cmul(f, slot[slot_id][j]));
+      A[k] = c2{0.0, 0.0};
+#pragma unroll
+      for (int j = 0; j < N2; ++j) X[j] = csub(X[j], cmul(f, slot[slot_id][N2 + j]));
+    }
+    __syncthreads();
+  }
+  if (singular) status[2] = 1.0;
+  if (valid) {
+    const c2 inv = cdiv(c2{1.0, 0.0}, pv);
+    c2 *row = chi + pt * N2 * N2 + myk * N2;
+#pragma unroll
+    for (int c = 0; c < N; ++c)
+#pragma unroll
+      for (int d = 0; d < N; ++d) row[c * N + d] = cmul(X[d * N + c], inv);
+  }
+}
+
+hipError_t launch_rpa(hipStream_t s, long npts, int n, const c2 *chi0, const c2 *U, c2 *chi, Status st) {
+  switch (n) {
+#define TPRF_RPA_CASE(N)                                                                           \
+  case N: {                                                                                        \
+    constexpr int P = 64 / (N * N);                                                                \
+    const long nb = (npts + 4 * P - 1) / (4 * P);                                                  \
+    hipLaunchKernelGGL(k_rpa<N>, dim3(nb), dim3(256), 0, s, npts, chi0, U, chi, st.d);              \
+    break;                                                                                         \
+  }
+    TPRF_RPA_CASE(1)
+    TPRF_RPA_CASE(2)
+    TPRF_RPA_CASE(3)
+    TPRF_RPA_CASE(4)
+#undef TPRF_RPA_CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace tprf
+
+namespace tprf {
+
+// Kernels that request more than 64 KiB of dynamic LDS opt in once per process.
+hipError_t set_kernel_attributes() {
+  const int bytes = kMatsLdsBytes;
+  const void *fns[] = {(const void *)k_w2t<0>, (const void *)k_w2t<1>, (const void *)k_t2w<0>,
+                       (const void *)k_t2w<1>, (const void *)k_chi0_wr};
+  for (const void *f : fns) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+}  // namespace tprf

@@ -1,0 +1,257 @@
+"""Drop-in for the hot-path functions of ``triqs_tprf.lattice``.
+
+Same names, argument meanings and defaults as the reference's cpp2py module
+(python/triqs_tprf/lattice_desc.py); each call runs on the GPU through the C
+ABI of libtprf_hip.so and returns a new ``Gf`` whose data stays on the GPU.
+Inputs may hold numpy or torch data; they are moved to the current device.
+
+Errors the reference raises as C++ exceptions raise ``TprfError`` (a
+RuntimeError) here; its stderr warnings are Python warnings.
+"""
+import warnings
+
+import numpy as np
+import torch
+
+from . import _lib
+from .gf import BOSON, FERMION, Gf, MeshBrZone, MeshCycLat, MeshImFreq, MeshImTime, MeshProduct
+
+__all__ = [
+    "lattice_dyson_g0_wk", "lattice_dyson_g_wk", "fourier_wk_to_wr", "fourier_wr_to_wk", "fourier_wr_to_tr",
+    "fourier_tr_to_wr", "chi0_tr_from_grt_PH", "chi0_wr_from_grt_PH", "chi0_w0r_from_grt_PH",
+    "chi_w0r_from_chi_tr", "chi_wr_from_chi_tr", "chi_tr_from_chi_wr", "chi_wk_from_chi_wr",
+    "chi_wr_from_chi_wk", "solve_rpa_PH",
+]
+
+
+def _dev(t):
+    if isinstance(t, np.ndarray):
+        t = torch.from_numpy(np.ascontiguousarray(t))
+    if not t.is_cuda:
+        t = t.to("cuda")
+    if t.dtype != torch.complex128:
+        t = t.to(torch.complex128)
+    return t.contiguous()
+
+
+def _stat_code(mesh):
+    return _lib.FERMION if mesh.statistic == FERMION else _lib.BOSON
+
+
+# ---------------------------------------------------------------------------
+# a1 Dyson -- lattice_desc.py:27,57,119 -> c++/triqs_tprf/lattice/gf.cpp:41-132
+
+def lattice_dyson_g0_wk(mu, e_k, mesh):
+    """G0[w,k] = [(i w_n + mu) 1 - e_k]^-1 (gf.cpp:41-64)."""
+    return _dyson(mu, e_k, mesh, None)
+
+
+def lattice_dyson_g_wk(mu, e_k, sigma_w=None, sigma_wk=None):
+    """G[w,k] = [(i w_n + mu) 1 - e_k - Sigma]^-1 with Sigma(w) or Sigma(w,k) (gf.cpp:91-132)."""
+    if (sigma_w is None) == (sigma_wk is None):
+        raise TypeError("give exactly one of sigma_w, sigma_wk")
+    sigma = sigma_w if sigma_w is not None else sigma_wk
+    wmesh = sigma.mesh[0] if isinstance(sigma.mesh, MeshProduct) else sigma.mesh
+    return _dyson(mu, e_k, wmesh, sigma)
+
+
+def _dyson(mu, e_k, wmesh, sigma):
+    if not isinstance(wmesh, MeshImFreq) or wmesh.statistic != FERMION:
+        raise ValueError("Dyson equation needs a fermionic Matsubara mesh")
+    kmesh = e_k.mesh
+    n = e_k.target_shape[0]
+    ek = _dev(e_k.data)
+    c = _lib.ctx(ek.device.index)
+    out = torch.empty((wmesh.size, kmesh.size, n, n), dtype=torch.complex128, device=ek.device)
+    kind, sp = _lib.SIGMA_NONE, None
+    if sigma is not None:
+        sd = _dev(sigma.data)
+        kind = _lib.SIGMA_WK if isinstance(sigma.mesh, MeshProduct) else _lib.SIGMA_W
+        if kind == _lib.SIGMA_WK and sigma.mesh[1].size != kmesh.size:
+            raise ValueError("sigma_wk k-mesh does not match e_k")
+        sp = _lib.ptr(sd)
+    _lib.tail_status_reset(c)
+    _lib.call("tprf_lattice_dyson_g_wk", c, wmesh.beta, wmesh.n_max, float(mu), kmesh.size, n, _lib.ptr(ek), kind,
+              sp, _lib.ptr(out))
+    _lib.tail_status(c)  # raises on a singular matrix
+    return Gf(MeshProduct(wmesh, kmesh), data=out)
+
+
+# ---------------------------------------------------------------------------
+# a2 / a6 lattice FFTs -- gf.cpp:209-215, chi_imtime.cpp:289-297 -> fourier.hpp:165-231
+
+def _lattice(g, to_r):
+    wmesh, lmesh = g.mesh[0], g.mesh[1]
+    if to_r and not isinstance(lmesh, MeshBrZone):
+        raise ValueError("expected a (freq, k) Green's function")
+    if not to_r and not isinstance(lmesh, MeshCycLat):
+        raise ValueError("expected a (freq, r) Green's function")
+    x = _dev(g.data)
+    c = _lib.ctx(x.device.index)
+    out = torch.empty_like(x)
+    ntarget = int(np.prod(g.target_shape)) if g.target_shape else 1
+    name = "tprf_fourier_wk_to_wr" if to_r else "tprf_fourier_wr_to_wk"
+    _lib.call(name, c, wmesh.size, _lib.dims3(lmesh.dims), ntarget, _lib.ptr(x), _lib.ptr(out))
+    new = MeshCycLat(lmesh.dims) if to_r else MeshBrZone(lmesh.dims)
+    return Gf(MeshProduct(wmesh, new), data=out)
+
+
+def fourier_wk_to_wr(g_wk):
+    """G(w, r) = (1/N_k) sum_k e^{-i k.r} G(w, k) (FFTW_FORWARD then /N_k)."""
+    return _lattice(g_wk, True)
+
+
+def fourier_wr_to_wk(g_wr):
+    """G(w, k) = sum_r e^{+i k.r} G(w, r) (FFTW_BACKWARD, no scale)."""
+    return _lattice(g_wr, False)
+
+
+def chi_wk_from_chi_wr(chi_wr):
+    return _lattice(chi_wr, False)
+
+
+def chi_wr_from_chi_wk(chi_wk):
+    return _lattice(chi_wk, True)
+
+
+# ---------------------------------------------------------------------------
+# a3 Matsubara -> imaginary time -- gf.cpp:262-265, chi_imtime.cpp:284-287 ->
+# fourier.hpp:98-130 -> fourier_matsubara.cpp:194-308
+
+def _w_to_t(g, nt, moments=None):
+    wmesh, rmesh = g.mesh[0], g.mesh[1]
+    if not isinstance(wmesh, MeshImFreq):
+        raise ValueError("expected a Matsubara mesh first")
+    x = _dev(g.data)
+    c = _lib.ctx(x.device.index)
+    ntau = 6 * (wmesh.last_index + 1) + 1 if nt is None or nt < 0 else int(nt)
+    ncol = x[0].numel()
+    out = torch.empty((ntau,) + tuple(x.shape[1:]), dtype=torch.complex128, device=x.device)
+    mp = None
+    if moments is not None:
+        m = _dev(moments).reshape(3, ncol).contiguous()
+        mp = _lib.ptr(m)
+    else:
+        _lib.tail_status_reset(c)
+    _lib.call("tprf_fourier_wr_to_tr", c, _stat_code(wmesh), wmesh.beta, wmesh.n_max, ntau, ncol, _lib.ptr(x),
+              _lib.ptr(out), mp)
+    if moments is None:
+        err, m0 = _lib.tail_status(c)
+        if err >= 1e-3:  # fourier_matsubara.cpp:227-230
+            raise _lib.TprfError(-4, f"ERROR: High frequency moments have an error greater than 1e-3.\n  Error = {err}")
+        if err > 1e-6:  # :231-234
+            warnings.warn(f"WARNING: High frequency moments have an error greater than 1e-6.\n Error = {err}")
+        if m0 > 1e-6:  # :238-239
+            warnings.warn(f"WARNING: High frequency tail is not zero: {m0}")
+    tmesh = MeshImTime(wmesh.beta, wmesh.statistic, ntau)
+    return Gf(MeshProduct(tmesh, rmesh), data=out)
+
+
+def fourier_wr_to_tr(g_wr, nt=-1, moments=None):
+    """G(tau, r) from G(i w_n, r); ``moments`` [3][...] = (m1, m2, m3) skips the tail fit
+    (the explicit-moment overload of the reference's _fourier_impl)."""
+    return _w_to_t(g_wr, nt, moments)
+
+
+def chi_tr_from_chi_wr(chi_wr, ntau=-1):
+    return _w_to_t(chi_wr, ntau)
+
+
+# ---------------------------------------------------------------------------
+# a5 imaginary time -> Matsubara -- chi_imtime.cpp:279-282, gf.cpp:257-260 ->
+# fourier.hpp:132-163 -> fourier_matsubara.cpp:42-184
+
+def _t_to_w(g, nw):
+    tmesh, rmesh = g.mesh[0], g.mesh[1]
+    if not isinstance(tmesh, MeshImTime):
+        raise ValueError("expected an imaginary-time mesh first")
+    x = _dev(g.data)
+    c = _lib.ctx(x.device.index)
+    L = tmesh.n_tau - 1
+    nw = L // 6 if nw is None or nw < 0 else int(nw)
+    wmesh = MeshImFreq(tmesh.beta, tmesh.statistic, nw)
+    ncol = x[0].numel()
+    out = torch.empty((wmesh.size,) + tuple(x.shape[1:]), dtype=torch.complex128, device=x.device)
+    _lib.call("tprf_fourier_tr_to_wr", c, _stat_code(tmesh), tmesh.beta, tmesh.n_tau, nw, ncol, _lib.ptr(x),
+              _lib.ptr(out))
+    return Gf(MeshProduct(wmesh, rmesh), data=out)
+
+
+def fourier_tr_to_wr(g_tr, nw=-1):
+    return _t_to_w(g_tr, nw)
+
+
+def chi_wr_from_chi_tr(chi_tr, nw):
+    """chi(i nu, r) from chi(tau, r) (chi_imtime.hpp:87)."""
+    return _t_to_w(chi_tr, nw)
+
+
+# ---------------------------------------------------------------------------
+# a4 bubble -- chi_imtime.cpp:88-228
+
+def _g_tr_args(g_tr):
+    tmesh, rmesh = g_tr.mesh[0], g_tr.mesh[1]
+    if not isinstance(tmesh, MeshImTime) or not isinstance(rmesh, MeshCycLat):
+        raise ValueError("expected G(tau, r)")
+    x = _dev(g_tr.data)
+    return tmesh, rmesh, x, g_tr.target_shape[0]
+
+
+def chi0_tr_from_grt_PH(g_tr):
+    """chi0_abcd(tau, r) = G_da(tau, r) G_bc(beta - tau, -r) (chi_imtime.cpp:88-136)."""
+    tmesh, rmesh, x, n = _g_tr_args(g_tr)
+    c = _lib.ctx(x.device.index)
+    out = torch.empty((tmesh.n_tau, rmesh.size, n, n, n, n), dtype=torch.complex128, device=x.device)
+    _lib.call("tprf_chi0_tr_from_grt_PH", c, tmesh.n_tau, _lib.dims3(rmesh.dims), n, _lib.ptr(x), _lib.ptr(out))
+    return Gf(MeshProduct(MeshImTime(tmesh.beta, BOSON, tmesh.n_tau), rmesh), data=out)
+
+
+def chi0_wr_from_grt_PH(g_tr, nw=1):
+    """Fused product + tau -> nu (chi_imtime.cpp:139-182); chi0(tau, r) is never stored."""
+    tmesh, rmesh, x, n = _g_tr_args(g_tr)
+    c = _lib.ctx(x.device.index)
+    wmesh = MeshImFreq(tmesh.beta, BOSON, nw)
+    out = torch.empty((wmesh.size, rmesh.size, n, n, n, n), dtype=torch.complex128, device=x.device)
+    _lib.call("tprf_chi0_wr_from_grt_PH", c, tmesh.beta, tmesh.n_tau, _lib.dims3(rmesh.dims), n, int(nw),
+              _lib.ptr(x), _lib.ptr(out))
+    return Gf(MeshProduct(wmesh, rmesh), data=out)
+
+
+def chi0_w0r_from_grt_PH(g_tr):
+    """Static bubble by trapezoid integration in tau (chi_imtime.cpp:185-247)."""
+    tmesh, rmesh, x, n = _g_tr_args(g_tr)
+    c = _lib.ctx(x.device.index)
+    out = torch.empty((1, rmesh.size, n, n, n, n), dtype=torch.complex128, device=x.device)
+    _lib.call("tprf_chi0_w0r_from_grt_PH", c, tmesh.beta, tmesh.n_tau, _lib.dims3(rmesh.dims), n, _lib.ptr(x),
+              _lib.ptr(out))
+    return Gf(MeshProduct(MeshImFreq(tmesh.beta, BOSON, 1), rmesh), data=out)
+
+
+def chi_w0r_from_chi_tr(chi_tr):
+    """Static chi(nu=0, r) by trapezoid integration (chi_imtime.cpp:250-277)."""
+    tmesh, rmesh = chi_tr.mesh[0], chi_tr.mesh[1]
+    x = _dev(chi_tr.data)
+    n = chi_tr.target_shape[0]
+    c = _lib.ctx(x.device.index)
+    out = torch.empty((1,) + tuple(x.shape[1:]), dtype=torch.complex128, device=x.device)
+    _lib.call("tprf_chi_w0r_from_chi_tr", c, tmesh.beta, tmesh.n_tau, rmesh.size, n, _lib.ptr(x), _lib.ptr(out))
+    return Gf(MeshProduct(MeshImFreq(tmesh.beta, BOSON, 1), rmesh), data=out)
+
+
+# ---------------------------------------------------------------------------
+# a7 RPA -- lattice_desc.py:547 -> rpa.cpp:29-68
+
+def solve_rpa_PH(chi0, U):
+    """chi = (1 - chi0 U)^-1 chi0 in the PH grouping (ab),(dc) per mesh point."""
+    x = _dev(chi0.data)
+    n = chi0.target_shape[0]
+    u = _dev(U if not isinstance(U, Gf) else U.data)
+    if tuple(u.shape) != (n, n, n, n):
+        raise ValueError("U must be a rank-4 tensor matching chi0's target")
+    c = _lib.ctx(x.device.index)
+    out = torch.empty_like(x)
+    npts = x.numel() // n ** 4
+    _lib.tail_status_reset(c)
+    _lib.call("tprf_solve_rpa_PH", c, npts, n, _lib.ptr(x), _lib.ptr(u), _lib.ptr(out))
+    _lib.tail_status(c)
+    return Gf(chi0.mesh, data=out)
