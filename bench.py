@@ -1,0 +1,189 @@
+"""Benchmark of the lattice bubble + RPA forward chain on one MI355X.
+
+One step = one eval: e_k -> G(iw,k) -> G(iw,r) -> G(tau,r) -> chi0(i nu,r) (fused
+product + tau->nu) -> chi0(i nu,q) -> chi_RPA(i nu,q) for the 3-orbital
+simple-cubic workload (16^3 k, beta 40, 256 positive fermionic frequencies,
+ntau 1537, 127 bosonic frequencies, Kanamori U = 2.3, J = 0.4), all complex
+FP64, inputs resident in HBM.  Prints one JSON line.
+
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+
+Per-stage durations come from HIP events recorded on the stream the kernels
+run on (torch's current stream, which the library context uses).  The CPU
+baseline times the oracle (C/OpenMP restatement of the reference algorithm)
+on one eval of the same workload on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "χ0+RPA evals/s (3-orb 16³k, 256 iω) at 1e-10 rel; grad/fwd time; batched/plain"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def stage_model(wl):
+    """Algorithmic bytes (each materialised API tensor written once and read once,
+    chi0(tau, r) never stored) and FLOPs (5 N log2 N per complex FFT, 6 per complex
+    multiply) per eval and stage."""
+    import math
+    n, nk = wl.norb, wl.nk
+    c = 16
+    nw = 2 * wl.n_iw
+    L = wl.ntau - 1
+    nnu = wl.n_nu
+    g_wk = nw * nk * n * n * c
+    g_tr = wl.ntau * nk * n * n * c
+    chi = nnu * nk * n ** 4 * c
+    ek = nk * n * n * c
+    fft3 = 5 * nk * math.log2(nk)
+    return {
+        "dyson": (ek + g_wk, 0.0),
+        "wk_to_wr": (2 * g_wk, nw * n * n * fft3),
+        "wr_to_tr": (g_wk + g_tr, nk * n * n * 5 * L * math.log2(L)),
+        "chi0_wr": (g_tr + chi, nk * n ** 4 * (5 * L * math.log2(L) + 6 * wl.ntau)),
+        "wr_to_wk": (2 * chi, nnu * n ** 4 * fft3),
+        "rpa": (2 * chi, nnu * nk * 8 * ((n * n) ** 3 * 2 + (n * n) ** 3 // 3)),
+    }
+
+
+def cpu_baseline(wl, chain_result):
+    """One eval of the oracle on the host (OpenMP threads), and the GPU result's
+    relative error against it."""
+    import numpy as np
+    from oracle import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    O.set_num_threads(threads)
+    t0 = time.perf_counter()
+    g_wk = O.lattice_dyson_g_wk(wl.beta, wl.n_iw, wl.mu, wl.e_k)
+    g_wr = O.fourier_lattice(g_wk, wl.dims, -1)
+    del g_wk
+    g_tr = O.fourier_w_to_t(g_wr, wl.beta)
+    del g_wr
+    chi0_wr = O.chi0_wr_from_grt_PH(g_tr, wl.dims, wl.beta, wl.nb)
+    del g_tr
+    chi0_wk = O.fourier_lattice(chi0_wr, wl.dims, +1)
+    del chi0_wr
+    chi = O.solve_rpa_PH(chi0_wk, wl.U)
+    dt = time.perf_counter() - t0
+    rel = lambda a, b: float(np.abs(a - b).max() / np.abs(b).max())
+    n = wl.norb
+    X = chi0_wk.reshape(-1, n, n, n, n).transpose(0, 1, 2, 4, 3).reshape(-1, n * n, n * n)
+    MU = wl.U.transpose(0, 1, 3, 2).reshape(n * n, n * n)
+    s = np.linalg.svd(np.eye(n * n)[None] - X @ MU, compute_uv=False)
+    kappa = s[:, 0] / s[:, -1]
+    good = kappa <= 100
+    n4 = n ** 4
+    got_chi0, got_chi = chain_result
+    parity = {
+        "chi0_wk_rel": rel(got_chi0, chi0_wk),
+        "chi_rpa_rel_all": rel(got_chi, chi),
+        "chi_rpa_rel_cond_le_100": rel(got_chi.reshape(-1, n4)[good], chi.reshape(-1, n4)[good]),
+        "rpa_min_sigma": float(s[:, -1].min()),
+        "rpa_max_cond": float(kappa.max()),
+    }
+    try:
+        cpu = open("/proc/cpuinfo").read().split("model name")[1].split("\n")[0].strip(" :\t")
+    except Exception:
+        cpu = "unknown"
+    base = {"value": 1.0 / dt, "unit": "evals/s", "cores": threads, "kind": "port",
+            "sample": f"one full eval of the same workload (oracle C/OpenMP restatement, {threads} threads, {cpu}), "
+                      f"{dt:.1f} s"}
+    return base, parity
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--seed", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this path shards by the "
+              "design in DESIGN.md but is built and measured on one GPU)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from tprf_amd import chain, workloads
+
+    torch.cuda.set_device(0)
+    wl = workloads.cubic_3orb(seed=args.seed)
+    ch = chain.BubbleRPA(wl, device="cuda:0")
+    for _ in range(args.warmup):
+        ch.run()
+    fit_err, _ = ch.check_status()
+    torch.cuda.synchronize()
+
+    stages = chain.BubbleRPA.STAGES
+    ev = {s: [] for s in stages}
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0.record()
+    for _ in range(args.steps):
+        ch.stage_reset()
+        for s in stages:
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            ch.stage(s)
+            b.record()
+            ev[s].append((a, b))
+    t1.record()
+    torch.cuda.synchronize()
+    total_ms = t0.elapsed_time(t1)
+    ms_per_step = total_ms / args.steps
+    stage_ms = {s: sum(a.elapsed_time(b) for a, b in ev[s]) / args.steps for s in stages}
+    fit_err, _ = ch.check_status()
+
+    model = stage_model(wl)
+    dom = max(stage_ms, key=stage_ms.get)
+    dom_bytes, dom_flops = model[dom]
+    achieved = dom_bytes / (stage_ms[dom] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                "algorithmic_bytes": dom_bytes, "avg_ms": stage_ms[dom]}
+    fp64 = {"kernel": dom, "achieved": dom_flops / (stage_ms[dom] * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s"}
+    fp64["frac"] = fp64["achieved"] / FP64_PEAK_TFLOPS
+    chain_bytes = sum(v[0] for v in model.values())
+    out = {
+        "metric": METRIC,
+        "value": 1000.0 / ms_per_step,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic (seeded hoppings, no external data)",
+        "config": {"workload": "cubic_3orb: 3-orbital simple cubic, 16^3 k, beta 40, 256 fermionic freqs "
+                               "(512 stored), ntau 1537, 127 bosonic freqs, Kanamori U 2.3 J 0.4 (9x9 RPA solves)",
+                   "seed": args.seed, "mu": wl.mu, "tail_fit_residual": fit_err},
+        "stage_ms": stage_ms,
+        "chain_hbm_frac": chain_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "roofline": roofline,
+        "roofline_fp64": fp64,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        torch.cuda.synchronize()
+        got = (ch.chi0_wk.cpu().numpy(), ch.chi_wk.cpu().numpy())
+        base, parity = cpu_baseline(wl, got)
+        out["cpu_baseline"] = base
+        out["parity_vs_oracle"] = parity
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -56,6 +56,11 @@ typedef struct tprf_ctx tprf_ctx;
 int tprf_ctx_create(tprf_ctx **ctx, int device);
 int tprf_ctx_destroy(tprf_ctx *ctx);
 int tprf_ctx_set_stream(tprf_ctx *ctx, void *hip_stream); /* hipStream_t; NULL = default stream */
+/* Options (no counterpart in the reference).  TPRF_OPT_GENERIC_KERNELS = 1
+ * routes every call through the generic full-length kernels instead of the
+ * pruned specialisations (same results to rounding; used to cross-check). */
+#define TPRF_OPT_GENERIC_KERNELS 1
+int tprf_ctx_set_option(tprf_ctx *ctx, int option, int value);
 const char *tprf_ctx_last_error(const tprf_ctx *ctx);
 /* Largest RMS tail-fit residual of the fitted omega->tau calls since the last
  * reset (blocking: synchronises the stream).  The reference asserts < 1e-3

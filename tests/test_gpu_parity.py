@@ -215,6 +215,27 @@ def test_chi0_wr_fused(L, oracle, gtr_case, nw):
     assert relerr(host(got.data), chain) <= TOL
 
 
+@pytest.mark.parametrize("n,n_iw,nb,dims", [(3, 64, 64, (4, 2, 1)), (3, 128, 32, (3, 3, 1)), (2, 64, 17, (2, 2, 2)),
+                                            (2, 256, 64, (5, 3, 1)), (3, 256, 64, (2, 1, 1)), (2, 128, 9, (6, 1, 1))])
+def test_chi0_wr_pruned_pair_kernel(L, oracle, n, n_iw, nb, dims):
+    """The pruned (r, -r) pair kernel and the generic kernel both match the oracle;
+    dims include self-mirrored r (r = -r) and odd extents."""
+    from tprf_amd import _lib
+    rng = np.random.default_rng(n * 1000 + n_iw + nb)
+    beta = 15.0
+    gw = g0_wr(oracle, rng, dims, n, beta, n_iw)
+    gt = oracle.fourier_w_to_t(gw, beta)
+    ref = oracle.chi0_wr_from_grt_PH(gt, dims, beta, nb)
+    c = _lib.ctx()
+    for generic in (0, 1):
+        _lib.call("tprf_ctx_set_option", c, 1, generic)
+        try:
+            got = L.chi0_wr_from_grt_PH(gtr_gf(dims, beta, gt), nw=nb)
+        finally:
+            _lib.call("tprf_ctx_set_option", c, 1, 0)
+        assert relerr(host(got.data), ref) <= TOL, f"generic={generic}"
+
+
 def test_chi0_w0r_trapz(L, oracle, gtr_case):
     dims, n, beta, gt = gtr_case
     got = L.chi0_w0r_from_grt_PH(gtr_gf(dims, beta, gt))

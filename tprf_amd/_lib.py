@@ -36,6 +36,7 @@ _SIGNATURES = {
     "tprf_ctx_create": [ctypes.POINTER(ctypes.c_void_p), _i],
     "tprf_ctx_destroy": [_p],
     "tprf_ctx_set_stream": [_p, _p],
+    "tprf_ctx_set_option": [_p, _i, _i],
     "tprf_ctx_last_error": [_p],
     "tprf_ctx_tail_status": [_p, ctypes.POINTER(_d), ctypes.POINTER(_d)],
     "tprf_ctx_tail_status_reset": [_p],

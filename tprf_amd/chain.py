@@ -55,8 +55,12 @@ class BubbleRPA:
         else:
             raise KeyError(name)
 
-    def run(self):
+    def stage_reset(self):
+        """Clear the tail-fit / singularity status words (one small memset)."""
         _lib.tail_status_reset(self.ctx)
+
+    def run(self):
+        self.stage_reset()
         for s in self.STAGES:
             self.stage(s)
 

@@ -41,8 +41,15 @@ struct MatsEntry {
   c2 *phase = nullptr;
 };
 
+struct BubbleEntry {
+  tprf::BubbleDev dev;
+  c2 *tw1 = nullptr, *twp = nullptr, *phi = nullptr;
+  int *rcanon = nullptr;
+};
+
 struct tprf_ctx {
   int device = 0;
+  bool generic_kernels = false;
   hipStream_t stream = nullptr;
   std::string err;
   double *d_status = nullptr;
@@ -50,6 +57,7 @@ struct tprf_ctx {
   std::map<std::tuple<int, double, int>, TailEntry> fits;
   std::map<std::tuple<int, double, int>, MatsEntry> mats;
   tprf::Vinv8 vinv;
+  std::map<std::tuple<double, int, int, int, int, int>, BubbleEntry> bubbles;
 };
 
 namespace {
@@ -259,6 +267,75 @@ void vandermonde_inverse8(double *vinv) {
   }
 }
 
+// Tables of the pruned bubble transform for (beta, L, nb, dims) and inner DFT
+// length P (see bubble.hip).  Twiddles and the model transforms Phi_q(m) are
+// evaluated in long double from exactly reduced angles.
+int get_bubble(tprf_ctx *ctx, double beta, int L, int nb, const int dims[3], int P, const tprf::BubbleDev **out) {
+  auto key = std::make_tuple(beta, L, nb, dims[0], dims[1], dims[2]);
+  auto it = ctx->bubbles.find(key);
+  if (it != ctx->bubbles.end() && it->second.dev.P == P) { *out = &it->second.dev; return TPRF_OK; }
+  const int N1 = L / P, nnu = 2 * nb - 1;
+  std::vector<long double> cs(L), sn(L);
+  for (int j = 0; j < L; ++j) {
+    long double a = 2.0L * kPiL * (long double)j / (long double)L;
+    cs[j] = cosl(a);
+    sn[j] = sinl(a);
+  }
+  auto e = [&](long jm) {  // exp(+2 pi i jm / L)
+    long r = ((jm % L) + L) % L;
+    return std::complex<long double>(cs[r], sn[r]);
+  };
+  std::vector<c2> tw1((size_t)N1 * P), twp(P), phi(3 * (size_t)nnu);
+  for (int j1 = 0; j1 < N1; ++j1)
+    for (int k = 0; k < P; ++k) {
+      const long m = k < P / 2 ? k : k - P;
+      auto w = e((long)j1 * m);
+      tw1[(size_t)j1 * P + k] = c2{(double)w.real(), (double)w.imag()};
+    }
+  for (int p = 0; p < P; ++p) {
+    long double a = 2.0L * kPiL * (long double)p / (long double)P;
+    twp[p] = c2{(double)cosl(a), (double)sinl(a)};
+  }
+  const double bb[3] = {-0.5, -1.0, 1.0};
+  for (int q = 0; q < 3; ++q) {
+    std::vector<double> ps(L);
+    for (int j = 0; j < L; ++j) ps[j] = one_boson(bb[q], (double)j * (beta / (double)L), beta);
+    for (int im = 0; im < nnu; ++im) {
+      const long m = im - (nb - 1);
+      std::complex<long double> s = 0;
+      for (int j = 0; j < L; ++j) s += (long double)ps[j] * e((long)j * m);
+      phi[(size_t)q * nnu + im] = c2{(double)s.real(), (double)s.imag()};
+    }
+  }
+  const long nr = (long)dims[0] * dims[1] * dims[2];
+  std::vector<int> canon;
+  for (long r = 0; r < nr; ++r) {
+    long i2 = r % dims[2], i1 = (r / dims[2]) % dims[1], i0 = r / ((long)dims[1] * dims[2]);
+    long mr = (((dims[0] - i0) % dims[0]) * dims[1] + (dims[1] - i1) % dims[1]) * dims[2] + (dims[2] - i2) % dims[2];
+    if (r <= mr) canon.push_back((int)r);
+  }
+  if (it != ctx->bubbles.end()) {
+    (void)hipFree(it->second.tw1); (void)hipFree(it->second.twp); (void)hipFree(it->second.phi);
+    (void)hipFree(it->second.rcanon);
+    ctx->bubbles.erase(it);
+  }
+  BubbleEntry b;
+  int st;
+  if ((st = upload(ctx, &b.tw1, tw1))) return st;
+  if ((st = upload(ctx, &b.twp, twp))) return st;
+  if ((st = upload(ctx, &b.phi, phi))) return st;
+  if ((st = upload(ctx, &b.rcanon, canon))) return st;
+  b.dev.P = P;
+  b.dev.ncanon = (int)canon.size();
+  b.dev.tw1 = b.tw1;
+  b.dev.twp = b.twp;
+  b.dev.phi = b.phi;
+  b.dev.rcanon = b.rcanon;
+  auto res = ctx->bubbles.emplace(key, b);
+  *out = &res.first->second.dev;
+  return TPRF_OK;
+}
+
 bool dims_ok(const int dims[3]) {
   if (!dims) return false;
   for (int d = 0; d < 3; ++d)
@@ -335,9 +412,22 @@ int tprf_ctx_destroy(tprf_ctx *ctx) {
   for (auto &kv : ctx->fft) (void)hipFree(kv.second.tw);
   for (auto &kv : ctx->fits) { (void)hipFree(kv.second.idx); (void)hipFree(kv.second.pinv); (void)hipFree(kv.second.vander); }
   for (auto &kv : ctx->mats) { (void)hipFree(kv.second.psi); (void)hipFree(kv.second.phase); }
+  for (auto &kv : ctx->bubbles) {
+    (void)hipFree(kv.second.tw1); (void)hipFree(kv.second.twp); (void)hipFree(kv.second.phi);
+    (void)hipFree(kv.second.rcanon);
+  }
   (void)hipFree(ctx->d_status);
   delete ctx;
   return TPRF_OK;
+}
+
+int tprf_ctx_set_option(tprf_ctx *ctx, int option, int value) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (option == TPRF_OPT_GENERIC_KERNELS) {
+    ctx->generic_kernels = value != 0;
+    return TPRF_OK;
+  }
+  return fail(ctx, TPRF_ERR_ARG, "unknown option");
 }
 
 int tprf_ctx_set_stream(tprf_ctx *ctx, void *stream) {
@@ -460,12 +550,24 @@ int tprf_chi0_wr_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dim
   if (L < 2 * nw)
     return fail(ctx, TPRF_ERR_MESH_TOO_SHORT,
                 "Fourier: The time mesh mush be at least twice as long as the number of positive frequencies");
+  int st;
+  // pruned pair kernel: inner DFT length P >= 2 nw dividing L
+  int P = 0;
+  for (int cand : {64, 128})
+    if (cand >= 2 * nw && L % cand == 0) { P = cand; break; }
+  if (!ctx->generic_kernels && P && tprf::bubble_pair_supported(norb, P)) {
+    const tprf::BubbleDev *bd;
+    if ((st = get_bubble(ctx, beta, L, nw, dims, P, &bd))) return st;
+    return check_launch(ctx,
+                        tprf::launch_bubble_pair(ctx->stream, beta, L, nw, dims, norb, (const c2 *)g_tr,
+                                                 (c2 *)chi0_wr, *bd, ctx->vinv),
+                        "bubble pair kernel");
+  }
   const int ld = (L + 1) | 1;
   if (2L * ld * (long)sizeof(c2) > tprf::kMatsLdsBytes)
     return fail(ctx, TPRF_ERR_UNSUPPORTED, "chi0_wr_from_grt_PH: ntau too large for one workgroup");
   const tprf::FftDev *p;
   const tprf::MatsDev *mt;
-  int st;
   if ((st = get_fft(ctx, L, &p))) return st;
   if ((st = get_mats(ctx, TPRF_BOSON, beta, L, &mt))) return st;
   return check_launch(ctx,

@@ -63,6 +63,17 @@ struct Vinv8 {
   double v[64];
 };
 
+// Tables of the pruned bubble transform (bubble.hip): tau_j with j = N1*j2 + j1,
+// inner DFT of length P over j2, outer twiddles over j1.
+struct BubbleDev {
+  int P;                 // inner DFT length (power of two, >= 2 nb)
+  int ncanon;            // number of canonical r (r <= -r)
+  const c2 *tw1;         // [N1][P] exp(+2 pi i j1 m(k) / L), m(k) = k or k - P
+  const c2 *twp;         // [P] exp(+2 pi i p / P)
+  const c2 *phi;         // [3][2nb-1] sum_{j<L} psi_q(tau_j) exp(+2 pi i j m / L)
+  const int *rcanon;     // [ncanon] canonical r in increasing order
+};
+
 // Status words written by kernels (device memory, 4 doubles):
 //   [0] max RMS tail-fit residual, [1] max |m0|, [2] singular-matrix flag.
 struct Status {
@@ -84,6 +95,10 @@ hipError_t launch_chi0_tr(hipStream_t s, int ntau, const int dims[3], int n, con
 hipError_t launch_chi0_w0r(hipStream_t s, double beta, int ntau, const int dims[3], int n, const c2 *gtr, c2 *out);
 hipError_t launch_chi_w0r(hipStream_t s, double beta, int ntau, long nr, int n, const c2 *chi_tr, c2 *out);
 hipError_t launch_rpa(hipStream_t s, long npts, int n, const c2 *chi0, const c2 *U, c2 *chi, Status st);
+// bubble.hip: returns hipErrorNotSupported when (n, P) has no specialisation
+hipError_t launch_bubble_pair(hipStream_t s, double beta, int L, int nb, const int dims[3], int n, const c2 *gtr,
+                              c2 *out, const BubbleDev &bd, const Vinv8 &vinv);
+bool bubble_pair_supported(int n, int P);
 
 // largest LDS footprint (bytes) a Matsubara-axis kernel may request
 constexpr int kMatsLdsBytes = 128 * 1024;

@@ -103,6 +103,41 @@ struct SmallDft<8, S> {
   }
 };
 
+// 16 = 4 x 4: DFT-4 over n1 for each n2, twiddle W16^{n2 k1}, DFT-4 over n2
+template <int S>
+struct SmallDft<16, S> {
+  __device__ __forceinline__ static void run(c2 *v, const c2 *, int) {
+    const double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173, h = 0.70710678118654752440;
+    c2 y[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+      c2 t[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+      SmallDft<4, S>::run(t, nullptr, 0);
+#pragma unroll
+      for (int k1 = 0; k1 < 4; ++k1) y[n2][k1] = t[k1];
+    }
+    // W16^p = exp(S 2 pi i p / 16) for p = n2 * k1
+    const c2 w1 = c2{c1, S * s1}, w2 = c2{h, S * h}, w3 = c2{s1, S * c1};
+    const c2 w6 = c2{-h, S * h}, w9 = c2{-c1, -S * s1};
+    y[1][1] = cmul(y[1][1], w1);
+    y[1][2] = cmul(y[1][2], w2);
+    y[1][3] = cmul(y[1][3], w3);
+    y[2][1] = cmul(y[2][1], w2);
+    y[2][2] = cmul_i(y[2][2], (double)S);
+    y[2][3] = cmul(y[2][3], w6);
+    y[3][1] = cmul(y[3][1], w3);
+    y[3][2] = cmul(y[3][2], w6);
+    y[3][3] = cmul(y[3][3], w9);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      c2 t[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+      SmallDft<4, S>::run(t, nullptr, 0);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+    }
+  }
+};
+
 // generic odd prime radix via the length-n twiddle table (R divides n)
 template <int R, int S>
 struct SmallDftGeneric {

@@ -5,15 +5,9 @@
 // accesses are contiguous along the innermost (orbital / r) index.
 #include "common.hpp"
 #include "fft_device.hpp"
+#include "mats_device.hpp"
 
 namespace tprf {
-
-constexpr double kPi = 3.14159265358979323846;
-
-__device__ __forceinline__ void atomic_max_nonneg(double *addr, double v) {
-  // non-negative doubles order like their bit patterns
-  atomicMax(reinterpret_cast<unsigned long long *>(addr), (unsigned long long)__double_as_longlong(v));
-}
 
 // ------------------------------------------------------------------------
 // a1 Dyson: G[w,k] = inverse((i w_n + mu) 1 - e_k - Sigma), one thread per
@@ -158,44 +152,7 @@ hipError_t launch_lattice_axis(hipStream_t s, int sign, const c2 *in, c2 *out, l
 }
 
 // ------------------------------------------------------------------------
-// Matsubara transforms.  Pole model of fourier_matsubara.cpp:150-171,274-288:
-// fermions b = (0, 1, -1), bosons b = (-1/2, -1, 1).
-template <int STAT>
-struct Poles;
-template <>
-struct Poles<0> {
-  static constexpr double b0 = 0.0, b1 = 1.0, b2 = -1.0;
-  __device__ static void coef(c2 m1, c2 m2, c2 m3, c2 *a) {
-    a[0] = csub(m1, m3);
-    a[1] = cscale(cadd(m2, m3), 0.5);
-    a[2] = cscale(csub(m3, m2), 0.5);
-  }
-};
-template <>
-struct Poles<1> {
-  static constexpr double b0 = -0.5, b1 = -1.0, b2 = 1.0;
-  __device__ static void coef(c2 m1, c2 m2, c2 m3, c2 *a) {
-    a[0] = cscale(csub(m1, m3), 4.0 / 3.0);
-    a[1] = csub(m3, cscale(cadd(m1, m2), 0.5));
-    a[2] = cadd(cadd(cscale(m1, 1.0 / 6.0), cscale(m2, 0.5)), cscale(m3, 1.0 / 3.0));
-  }
-};
-
-// sum_i a_i / (i w - b_i)
-template <int STAT>
-__device__ __forceinline__ c2 pole_sum(const c2 *a, double w) {
-  c2 iw = c2{0.0, w};
-  c2 s = cdiv(a[0], c2{-Poles<STAT>::b0, w});
-  s = cadd(s, cdiv(a[1], c2{-Poles<STAT>::b1, w}));
-  s = cadd(s, cdiv(a[2], c2{-Poles<STAT>::b2, w}));
-  (void)iw;
-  return s;
-}
-
-template <int STAT>
-__device__ __forceinline__ double matsubara_w(long n, double beta) {
-  return STAT == 0 ? kPi * (double)(2 * n + 1) / beta : 2.0 * kPi * (double)n / beta;
-}
+// Matsubara transforms (pole model and helpers in mats_device.hpp).
 
 int mats_cols_per_block(int L) {
   // two LDS buffers of C columns of (L+1) rounded to odd
@@ -318,22 +275,8 @@ __device__ void t2w_core(c2 *a, c2 *b, int ld, int cn, double beta, int nw, cons
   if (threadIdx.x < cn) {
     const int c = threadIdx.x;
     const c2 *g = a + c * ld;
-    c2 gl0 = c2{0, 0}, gl1 = c2{0, 0}, gr0 = c2{0, 0}, gr1 = c2{0, 0};
-#pragma unroll
-    for (int m = 1; m <= 8; ++m) {
-      const double tm = (double)m * dtau;
-      const c2 dl = c2{(g[m].x - g[0].x) / tm, (g[m].y - g[0].y) / tm};
-      const c2 dr = c2{(g[L].x - g[L - m].x) / tm, (g[L].y - g[L - m].y) / tm};
-      gl0 = cadd(gl0, cscale(dl, vinv.v[0 * 8 + m - 1]));
-      gl1 = cadd(gl1, cscale(dl, vinv.v[1 * 8 + m - 1]));
-      gr0 = cadd(gr0, cscale(dr, vinv.v[0 * 8 + m - 1]));
-      gr1 = cadd(gr1, cscale(dr, vinv.v[1 * 8 + m - 1]));
-    }
-    const double sign = (STAT == 0) ? -1.0 : 1.0;
-    c2 m2 = csub(gl0, cscale(gr0, sign));
-    c2 m3 = cscale(cadd(gl1, cscale(gr1, sign)), -2.0 / dtau);
-    c2 m1 = (STAT == 0) ? cscale(cadd(g[0], g[L]), -1.0) : cscale(csub(g[0], g[L]), -1.0);
-    Poles<STAT>::coef(m1, m2, m3, s_a[c]);
+    boundary_poles<STAT>([&](int m) { return g[m]; }, [&](int m) { return g[L - m]; }, dtau, vinv, s_a[c],
+                         (c2 *)nullptr);
   }
   __syncthreads();
   const double fact = beta / (double)L;
@@ -399,11 +342,6 @@ hipError_t launch_t2w(hipStream_t s, int statistic, double beta, int nw, long nc
 // ------------------------------------------------------------------------
 // a4 / a4': bubble product chi0_abcd(tau_j, r) = G_da(tau_j, r) G_bc(tau_{L-j}, -r)
 // (chi_imtime.cpp:127).
-__device__ __forceinline__ long minus_r(long r, int d0, int d1, int d2) {
-  const long i2 = r % d2, i1 = (r / d2) % d1, i0 = r / ((long)d1 * d2);
-  const long j0 = (d0 - i0) % d0, j1 = (d1 - i1) % d1, j2 = (d2 - i2) % d2;
-  return (j0 * d1 + j1) * d2 + j2;
-}
 
 // fused product + bosonic tau->nu: one workgroup per (r, C consecutive abcd)
 __global__ void __launch_bounds__(256) k_chi0_wr(double beta, int nw, int d0, int d1, int d2, int n, int C,
