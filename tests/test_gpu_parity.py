@@ -146,6 +146,36 @@ def test_chi_tr_from_chi_wr_boson(L, oracle):
     assert relerr(host(got.data), ref) <= TOL
 
 
+@pytest.mark.parametrize("stat,n_max,nt", [("Fermion", 64, -1), ("Fermion", 128, -1), ("Fermion", 32, 641),
+                                           ("Boson", 64, -1), ("Boson", 32, -1), ("Fermion", 512, -1)])
+def test_wr_to_tr_pruned_and_generic(L, oracle, stat, n_max, nt):
+    """The input-pruned kernel (L = Q M) and the generic full-length kernel both
+    match the oracle, for both statistics."""
+    from tprf_amd import _lib
+    from tprf_amd.gf import Gf, MeshImFreq, MeshProduct, MeshCycLat
+    rng = np.random.default_rng(n_max + (7 if stat == "Boson" else 0))
+    beta, dims = 9.0, (3, 2, 1)
+    mesh = MeshImFreq(beta, stat, n_max)
+    w = mesh.values()
+    eps = rng.uniform(-2.0, 2.0, size=(6, 2))
+    if stat == "Fermion":
+        data = 1.0 / (w[:, None, None] - eps[None])
+    else:
+        data = 1.0 / (w[:, None, None] - eps[None]) - 1.0 / (w[:, None, None] + eps[None])
+    data = data.reshape(mesh.size, 6, 1, 2) * (1 + 0.1j)
+    ref = oracle.fourier_w_to_t(data, beta, statistic=oracle.FERMION if stat == "Fermion" else oracle.BOSON,
+                                ntau=None if nt < 0 else nt)
+    g = Gf(MeshProduct(mesh, MeshCycLat(dims)), data=gpu(data))
+    c = _lib.ctx()
+    for generic in (0, 1):
+        _lib.call("tprf_ctx_set_option", c, 1, generic)
+        try:
+            got = L.fourier_wr_to_tr(g, nt=nt)
+        finally:
+            _lib.call("tprf_ctx_set_option", c, 1, 0)
+        assert relerr(host(got.data), ref) <= TOL, f"generic={generic}"
+
+
 def test_wr_to_tr_errors(L):
     from tprf_amd.gf import Gf, MeshImFreq, MeshProduct, MeshCycLat
     from tprf_amd._lib import TprfError

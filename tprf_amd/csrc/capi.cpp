@@ -58,6 +58,7 @@ struct tprf_ctx {
   std::map<std::tuple<int, double, int>, MatsEntry> mats;
   tprf::Vinv8 vinv;
   std::map<std::tuple<double, int, int, int, int, int>, BubbleEntry> bubbles;
+  std::map<std::tuple<int, double, int>, c2 *> pole_inv;
 };
 
 namespace {
@@ -267,6 +268,29 @@ void vandermonde_inverse8(double *vinv) {
   }
 }
 
+// 1 / (i w_n - b_q) on the mesh (statistic, beta, n_max), q = 0..2, for the
+// pole-model subtraction of the pruned Matsubara -> tau kernel
+int get_pole_inv(tprf_ctx *ctx, int statistic, double beta, int n_max, int nmesh, const c2 **out) {
+  auto key = std::make_tuple(statistic, beta, n_max);
+  auto it = ctx->pole_inv.find(key);
+  if (it != ctx->pole_inv.end()) { *out = it->second; return TPRF_OK; }
+  const double bf[3] = {0.0, 1.0, -1.0}, bb[3] = {-0.5, -1.0, 1.0};
+  const double *b = statistic == TPRF_FERMION ? bf : bb;
+  const int first = statistic == TPRF_FERMION ? -n_max : -(n_max - 1);
+  std::vector<c2> v(3 * (size_t)nmesh);
+  for (int q = 0; q < 3; ++q)
+    for (int i = 0; i < nmesh; ++i) {
+      std::complex<double> z = 1.0 / std::complex<double>(-b[q], matsubara(statistic, first + i, beta));
+      v[(size_t)q * nmesh + i] = c2{z.real(), z.imag()};
+    }
+  c2 *d;
+  int st = upload(ctx, &d, v);
+  if (st) return st;
+  ctx->pole_inv.emplace(key, d);
+  *out = d;
+  return TPRF_OK;
+}
+
 // Tables of the pruned bubble transform for (beta, L, nb, dims) and inner DFT
 // length P (see bubble.hip).  Twiddles and the model transforms Phi_q(m) are
 // evaluated in long double from exactly reduced angles.
@@ -361,6 +385,11 @@ int lattice_fft(tprf_ctx *ctx, long nw, const int dims[3], long ntarget, const c
                  "hipMemcpyAsync");
     return TPRF_OK;
   }
+  if (!ctx->generic_kernels) {
+    hipError_t e = tprf::launch_lattice3d(ctx->stream, sign, in, out, nw, dims, ntarget, scale);
+    if (e != hipErrorNotSupported) return check_launch(ctx, e, "lattice 3-d kernel");
+    (void)hipGetLastError();
+  }
   const c2 *src = in;
   for (int d = 2; d >= 0; --d) {
     if (dims[d] == 1) continue;
@@ -412,6 +441,7 @@ int tprf_ctx_destroy(tprf_ctx *ctx) {
   for (auto &kv : ctx->fft) (void)hipFree(kv.second.tw);
   for (auto &kv : ctx->fits) { (void)hipFree(kv.second.idx); (void)hipFree(kv.second.pinv); (void)hipFree(kv.second.vander); }
   for (auto &kv : ctx->mats) { (void)hipFree(kv.second.psi); (void)hipFree(kv.second.phase); }
+  for (auto &kv : ctx->pole_inv) (void)hipFree(kv.second);
   for (auto &kv : ctx->bubbles) {
     (void)hipFree(kv.second.tw1); (void)hipFree(kv.second.twp); (void)hipFree(kv.second.phi);
     (void)hipFree(kv.second.rcanon);
@@ -488,20 +518,37 @@ int tprf_fourier_wr_to_tr(tprf_ctx *ctx, int statistic, double beta, int n_max, 
   if (L < 2 * n_max)
     return fail(ctx, TPRF_ERR_MESH_TOO_SHORT,
                 "Inverse Fourier: The time mesh mush be at least twice as long as the freq mesh");
-  const int ld = (L + 1) | 1;
-  if (2L * ld * (long)sizeof(c2) > tprf::kMatsLdsBytes)
-    return fail(ctx, TPRF_ERR_UNSUPPORTED, "fourier_wr_to_tr: ntau too large for one workgroup");
   const tprf::FftDev *p;
   const tprf::MatsDev *mt;
   tprf::TailFitDev fit{};
   int st;
-  if ((st = get_fft(ctx, L, &p))) return st;
-  if ((st = get_mats(ctx, statistic, beta, L, &mt))) return st;
   if (!moments) {
     const tprf::TailFitDev *f;
     if ((st = get_tail(ctx, statistic, beta, n_max, &f))) return st;
     fit = *f;
   }
+  // input-pruned kernel: L = Q M with M a power of two holding the whole mesh
+  const int nmesh = statistic == TPRF_FERMION ? 2 * n_max : 2 * n_max - 1;
+  int M = 64;
+  while (M < 2 * n_max) M *= 2;
+  if (!ctx->generic_kernels && L % M == 0 && tprf::w2t_pruned_supported(M) && L / M <= 64) {
+    const tprf::FftDev *pL, *pM;
+    if ((st = get_fft(ctx, L, &pL))) return st;
+    if ((st = get_fft(ctx, M, &pM))) return st;
+    if ((st = get_mats(ctx, statistic, beta, L, &mt))) return st;
+    const c2 *pinv;
+    if ((st = get_pole_inv(ctx, statistic, beta, n_max, nmesh, &pinv))) return st;
+    tprf::W2tDev wd{M, L / M, pL->tw, pM->tw, pinv};
+    return check_launch(ctx,
+                        tprf::launch_w2t_pruned(ctx->stream, statistic, beta, n_max, ncol, (const c2 *)in, (c2 *)out,
+                                                (const c2 *)moments, fit, *mt, wd, tprf::Status{ctx->d_status}),
+                        "w2t pruned kernel");
+  }
+  const int ld = (L + 1) | 1;
+  if (2L * ld * (long)sizeof(c2) > tprf::kMatsLdsBytes)
+    return fail(ctx, TPRF_ERR_UNSUPPORTED, "fourier_wr_to_tr: ntau too large for one workgroup");
+  if ((st = get_fft(ctx, L, &p))) return st;
+  if ((st = get_mats(ctx, statistic, beta, L, &mt))) return st;
   return check_launch(ctx,
                       tprf::launch_w2t(ctx->stream, statistic, beta, n_max, ncol, (const c2 *)in, (c2 *)out,
                                        (const c2 *)moments, fit, *mt, *p, tprf::Status{ctx->d_status}),

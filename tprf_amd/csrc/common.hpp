@@ -74,6 +74,15 @@ struct BubbleDev {
   const int *rcanon;     // [ncanon] canonical r in increasing order
 };
 
+// Tables of the input-pruned Matsubara -> tau transform (w2t.hip): L = Q M,
+// the 2 n_max nonzero inputs fit one length-M DFT per output residue t < Q.
+struct W2tDev {
+  int M, Q;
+  const c2 *twL;         // [L] exp(-2 pi i k / L)
+  const c2 *twM;         // [M] exp(-2 pi i k / M)
+  const c2 *pole_inv;    // [3][nmesh] 1 / (i w_n - b_q)
+};
+
 // Status words written by kernels (device memory, 4 doubles):
 //   [0] max RMS tail-fit residual, [1] max |m0|, [2] singular-matrix flag.
 struct Status {
@@ -99,6 +108,14 @@ hipError_t launch_rpa(hipStream_t s, long npts, int n, const c2 *chi0, const c2 
 hipError_t launch_bubble_pair(hipStream_t s, double beta, int L, int nb, const int dims[3], int n, const c2 *gtr,
                               c2 *out, const BubbleDev &bd, const Vinv8 &vinv);
 bool bubble_pair_supported(int n, int P);
+// w2t.hip: returns hipErrorNotSupported when M has no specialisation
+hipError_t launch_w2t_pruned(hipStream_t s, int statistic, double beta, int n_max, long ncol, const c2 *in, c2 *out,
+                             const c2 *moments, const TailFitDev &fit, const MatsDev &mt, const W2tDev &wd,
+                             Status st);
+bool w2t_pruned_supported(int M);
+// kernels.hip: single-pass 3-D lattice FFT; hipErrorNotSupported for other dims
+hipError_t launch_lattice3d(hipStream_t s, int sign, const c2 *in, c2 *out, long nw, const int dims[3], long ntarget,
+                            double scale);
 
 // largest LDS footprint (bytes) a Matsubara-axis kernel may request
 constexpr int kMatsLdsBytes = 128 * 1024;

@@ -584,3 +584,98 @@ hipError_t set_kernel_attributes() {
 }
 
 }  // namespace tprf
+
+namespace tprf {
+
+// ------------------------------------------------------------------------
+// a2 / a6 single pass: the whole 3-D transform of C target components of one
+// outer index in LDS (layout [k][c], C = 4096 / N_k), each axis as register
+// DFTs of length D <= 16 along the lines of that axis.
+template <int D>
+__device__ __forceinline__ void line_dft(c2 *sm, int base, int stride, int sign) {
+  c2 v[D];
+#pragma unroll
+  for (int i = 0; i < D; ++i) v[i] = sm[base + i * stride];
+  if (sign < 0) SmallDft<D, -1>::run(v, nullptr, 0);
+  else SmallDft<D, 1>::run(v, nullptr, 0);
+#pragma unroll
+  for (int i = 0; i < D; ++i) sm[base + i * stride] = v[i];
+}
+
+template <int D0, int D1, int D2>
+__global__ void __launch_bounds__(256) k_lattice3d(const c2 *__restrict__ in, c2 *__restrict__ out, long ntarget,
+                                                   int sign, double scale) {
+  constexpr int NK = D0 * D1 * D2;
+  constexpr int C = NK >= 4096 ? 1 : 4096 / NK;
+  __shared__ c2 sm[NK * C];
+  const long ntc = (ntarget + C - 1) / C;
+  const long tile = xcd_remap(blockIdx.x, gridDim.x);
+  const long w = tile / ntc;
+  const long c0 = (tile - w * ntc) * C;
+  const int cn = (ntarget - c0) < C ? (int)(ntarget - c0) : C;
+  const c2 *src = in + w * NK * ntarget + c0;
+  for (int t = threadIdx.x; t < NK * cn; t += blockDim.x) {
+    const int k = t / cn, c = t - k * cn;
+    sm[k * C + c] = src[(long)k * ntarget + c];
+  }
+  __syncthreads();
+  if constexpr (D2 > 1) {
+    for (int l = threadIdx.x; l < C * D0 * D1; l += blockDim.x) {
+      const int c = l % C, rest = l / C;  // rest = i0 * D1 + i1
+      line_dft<D2>(sm, rest * D2 * C + c, C, sign);
+    }
+    __syncthreads();
+  }
+  if constexpr (D1 > 1) {
+    for (int l = threadIdx.x; l < C * D0 * D2; l += blockDim.x) {
+      const int c = l % C, rest = l / C;
+      const int i2 = rest % D2, i0 = rest / D2;
+      line_dft<D1>(sm, (i0 * D1 * D2 + i2) * C + c, D2 * C, sign);
+    }
+    __syncthreads();
+  }
+  if constexpr (D0 > 1) {
+    for (int l = threadIdx.x; l < C * D1 * D2; l += blockDim.x) {
+      const int c = l % C, rest = l / C;  // rest = i1 * D2 + i2
+      line_dft<D0>(sm, rest * C + c, D1 * D2 * C, sign);
+    }
+    __syncthreads();
+  }
+  c2 *dst = out + w * NK * ntarget + c0;
+  for (int t = threadIdx.x; t < NK * cn; t += blockDim.x) {
+    const int k = t / cn, c = t - k * cn;
+    dst[(long)k * ntarget + c] = cscale(sm[k * C + c], scale);
+  }
+}
+
+template <int D0, int D1, int D2>
+static hipError_t launch_l3(hipStream_t s, const c2 *in, c2 *out, long nw, long ntarget, int sign, double scale) {
+  constexpr int NK = D0 * D1 * D2;
+  constexpr int C = NK >= 4096 ? 1 : 4096 / NK;
+  const long nb = nw * ((ntarget + C - 1) / C);
+  hipLaunchKernelGGL((k_lattice3d<D0, D1, D2>), dim3(nb), dim3(256), 0, s, in, out, ntarget, sign, scale);
+  return hipGetLastError();
+}
+
+// dims with every extent in {1, 2, 4, 8, 16}; hipErrorNotSupported otherwise
+hipError_t launch_lattice3d(hipStream_t s, int sign, const c2 *in, c2 *out, long nw, const int dims[3], long ntarget,
+                            double scale) {
+  const int a = dims[0], b = dims[1], c = dims[2];
+#define TPRF_L3(A, B, C_)                                                              \
+  if (a == A && b == B && c == C_) return launch_l3<A, B, C_>(s, in, out, nw, ntarget, sign, scale);
+  TPRF_L3(16, 16, 16)
+  TPRF_L3(8, 8, 8)
+  TPRF_L3(16, 16, 1)
+  TPRF_L3(8, 8, 1)
+  TPRF_L3(4, 4, 4)
+  TPRF_L3(4, 4, 1)
+  TPRF_L3(16, 1, 1)
+  TPRF_L3(8, 1, 1)
+  TPRF_L3(4, 4, 2)
+  TPRF_L3(2, 2, 2)
+  TPRF_L3(2, 2, 1)
+#undef TPRF_L3
+  return hipErrorNotSupported;
+}
+
+}  // namespace tprf

@@ -1,0 +1,225 @@
+// w2t.hip -- Matsubara -> imaginary-time transform with a pruned input
+// (fourier_wr_to_tr, fourier_matsubara.cpp:218-308).
+//
+// The length-L input of the reference's FFT is nonzero only at the 2 n_max
+// positions (n + L) % L of the mesh.  With L = Q M and M >= 2 n_max a power
+// of two, the outputs split by residue t = j mod Q:
+//   y_{Q j' + t} = sum_n [x_n exp(-2 pi i t n / L)] exp(-2 pi i j' n / M),
+// one length-M DFT per t over the mesh (n taken mod M).  A workgroup holds C
+// columns (C * 16 consecutive bytes per row); each column's DFT runs on M/8
+// threads, eight values per thread, radix-8 Stockham passes exchanged through
+// LDS.
+#include "common.hpp"
+#include "fft_device.hpp"
+#include "mats_device.hpp"
+
+namespace tprf {
+
+template <int M>
+struct W2tPlan {
+  static constexpr int n8 = (M >= 512) ? 3 : (M >= 64 ? 2 : 1);  // radix-8 passes
+  static constexpr int last = M / (n8 == 3 ? 512 : (n8 == 2 ? 64 : 8));  // 1, 2, 4 or 8: final pass radix
+  static constexpr int npass = n8 + (last > 1 ? 1 : 0);
+};
+
+// Twiddle by W_M^{k r (M / (Ns R))}, then the radix-R DFT, for B = 8/R butterflies held in v.
+template <int M, int R, int S>
+__device__ __forceinline__ void w2t_butterflies(c2 *v, int j, int Ns, const c2 *__restrict__ twM) {
+  constexpr int B = 8 / R;
+#pragma unroll
+  for (int u = 0; u < B; ++u) {
+    const int b = j * B + u;
+    const int k = b % Ns;
+    if (k) {
+      const int step = k * (M / (Ns * R));
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[u * R + r] = cmul(v[u * R + r], twiddle<S>(twM, step * r));
+    }
+    SmallDft<R, S>::run(v + u * R, nullptr, 0);
+  }
+}
+
+template <int STAT, int M>
+__global__ void __launch_bounds__(512) k_w2t_pruned(double beta, int n_max, long ncol, const c2 *__restrict__ in,
+                                                    c2 *__restrict__ out, const c2 *__restrict__ moments,
+                                                    TailFitDev fit, MatsDev mt, W2tDev wd, double *status) {
+  constexpr int NT = 512;
+  constexpr int TPC = M / 8;  // threads per column
+  constexpr int C = NT / TPC;  // columns per workgroup
+  using Plan = W2tPlan<M>;
+  __shared__ c2 buf[M * C];
+  __shared__ c2 s_coef[C][kMaxOrder];
+  __shared__ c2 s_a[C][3];
+  __shared__ c2 s_m1[C];
+
+  const int tid = threadIdx.x;
+  const int c = tid % C, j = tid / C;
+  const long tile = xcd_remap(blockIdx.x, gridDim.x);
+  const long c0 = tile * C;
+  const int cn = (ncol - c0) < C ? (int)(ncol - c0) : C;
+  const long col = c0 + (c < cn ? c : 0);
+  const int L = mt.L, Q = wd.Q;
+  const int nmesh = (STAT == 0) ? 2 * n_max : 2 * n_max - 1;
+  const int first = (STAT == 0) ? -n_max : -(n_max - 1);
+
+  // 1. this thread's eight inputs (the same FFT positions e = j + TPC r for every t),
+  //    staged once in registers and once in LDS for the tail fit
+  c2 raw[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int e = j + TPC * r;
+    const int n = e < M / 2 ? e : e - M;
+    raw[r] = (n >= first && n < first + nmesh) ? in[(long)(n - first) * ncol + col] : c2{0.0, 0.0};
+    buf[e * C + c] = raw[r];
+  }
+  __syncthreads();
+  // data index i <-> FFT position ((first + i) mod M)
+  auto fitval = [&](int f, int cc) {
+    const int e = ((first + fit.idx[f]) % M + M) % M;
+    return buf[e * C + cc];
+  };
+  // 2. tail moments (least-squares fit, or given)
+  if (moments == nullptr) {
+    for (int t = tid; t < cn * fit.ncoef; t += NT) {
+      const int cc = t / fit.ncoef, q = t - cc * fit.ncoef;
+      c2 s = c2{0.0, 0.0};
+      for (int f = 0; f < fit.n_fit; ++f) s = cadd(s, cmul(fit.pinv[q * fit.n_fit + f], fitval(f, cc)));
+      s_coef[cc][q] = s;
+    }
+    __syncthreads();
+    if (tid < cn) {
+      double r2 = 0.0;
+      for (int f = 0; f < fit.n_fit; ++f) {
+        c2 v = fitval(f, tid);
+        for (int q = 0; q < fit.ncoef; ++q) v = csub(v, cmul(fit.vander[f * fit.ncoef + q], s_coef[tid][q]));
+        r2 += cabs2(v);
+      }
+      const c2 m1 = cscale(s_coef[tid][1], fit.wpow[1]);
+      Poles<STAT>::coef(m1, cscale(s_coef[tid][2], fit.wpow[2]), cscale(s_coef[tid][3], fit.wpow[3]), s_a[tid]);
+      s_m1[tid] = m1;
+      atomic_max_nonneg(&status[0], sqrt(r2 / fit.n_fit));
+      atomic_max_nonneg(&status[1], sqrt(cabs2(s_coef[tid][0])));
+    }
+  } else if (tid < cn) {
+    const c2 m1 = moments[c0 + tid];
+    Poles<STAT>::coef(m1, moments[ncol + c0 + tid], moments[2 * ncol + c0 + tid], s_a[tid]);
+    s_m1[tid] = m1;
+  }
+  __syncthreads();
+  const c2 a0 = s_a[c][0], a1 = s_a[c][1], a2 = s_a[c][2];
+  const double ib = 1.0 / beta;
+  const double pm = (STAT == 0) ? -1.0 : 1.0;
+
+  // 3. subtract the pole model once: raw <- (G - sum_q a_q / (i w_n - b_q)) / beta
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int e = j + TPC * r;
+    const int n = e < M / 2 ? e : e - M;
+    if (n >= first && n < first + nmesh) {
+      const int i = n - first;
+      c2 x = raw[r];
+      x = csub(x, cmul(a0, wd.pole_inv[i]));
+      x = csub(x, cmul(a1, wd.pole_inv[nmesh + i]));
+      x = csub(x, cmul(a2, wd.pole_inv[2 * nmesh + i]));
+      raw[r] = cscale(x, ib);
+    }
+  }
+
+  for (int t = 0; t < Q; ++t) {
+    // pass 1 inputs: FFT index e = j + TPC r  <->  n = e or e - M, twiddled by exp(-2 pi i t n / L)
+    c2 v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int e = j + TPC * r;
+      const int n = e < M / 2 ? e : e - M;
+      c2 x = raw[r];
+      if (t && n >= first && n < first + nmesh) {
+        long tn = ((long)t * n) % L;
+        if (tn < 0) tn += L;
+        x = cmul(x, wd.twL[tn]);
+      }
+      v[r] = x;
+    }
+    int Ns = 1;
+#pragma unroll
+    for (int p = 0; p < Plan::npass; ++p) {
+      const int R = (p < Plan::n8) ? 8 : Plan::last;
+      if (R == 8) w2t_butterflies<M, 8, -1>(v, j, Ns, wd.twM);
+      else if (R == 4) w2t_butterflies<M, 4, -1>(v, j, Ns, wd.twM);
+      else w2t_butterflies<M, 2, -1>(v, j, Ns, wd.twM);
+      const int B = 8 / R;
+      if (p + 1 < Plan::npass) {
+        // write this pass's outputs, read the next pass's inputs
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int uu = u / R, r = u % R;
+          const int b = j * B + uu;
+          const int k = b % Ns;
+          buf[((b - k) * R + k + r * Ns) * C + c] = v[u];
+        }
+        __syncthreads();
+        Ns *= R;
+        const int Rn = (p + 1 < Plan::n8) ? 8 : Plan::last;
+        const int Bn = 8 / Rn;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int uu = u / Rn, r = u % Rn;
+          v[u] = buf[(j * Bn + uu + r * (M / Rn)) * C + c];
+        }
+        __syncthreads();
+      } else if (c < cn) {
+        // outputs: bin j' = o + r Ns  ->  tau index Q j' + t
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int uu = u / R, r = u % R;
+          const int b = j * B + uu;
+          const int k = b % Ns;
+          const int jp = (b - k) * R + k + r * Ns;
+          const int jt = Q * jp + t;
+          c2 y = v[u];
+          if (STAT == 0) y = cmul(y, mt.phase[jt]);
+          const double p0 = mt.psi[jt], p1 = mt.psi[L + jt], p2 = mt.psi[2 * L + jt];
+          y.x += a0.x * p0 + a1.x * p1 + a2.x * p2;
+          y.y += a0.y * p0 + a1.y * p1 + a2.y * p2;
+          out[(long)jt * ncol + col] = y;
+          if (jt == 0) out[(long)L * ncol + col] = cscale(cadd(y, s_m1[c]), pm);
+        }
+      }
+    }
+  }
+}
+
+bool w2t_pruned_supported(int M) { return M == 64 || M == 128 || M == 256 || M == 512 || M == 1024; }
+
+template <int STAT, int M>
+static hipError_t launch_m(hipStream_t s, double beta, int n_max, long ncol, const c2 *in, c2 *out, const c2 *moments,
+                           const TailFitDev &fit, const MatsDev &mt, const W2tDev &wd, Status st) {
+  constexpr int C = 512 / (M / 8);
+  const long nb = (ncol + C - 1) / C;
+  hipLaunchKernelGGL((k_w2t_pruned<STAT, M>), dim3(nb), dim3(512), 0, s, beta, n_max, ncol, in, out, moments, fit, mt,
+                     wd, st.d);
+  return hipGetLastError();
+}
+
+template <int STAT>
+static hipError_t launch_stat(hipStream_t s, double beta, int n_max, long ncol, const c2 *in, c2 *out,
+                              const c2 *moments, const TailFitDev &fit, const MatsDev &mt, const W2tDev &wd,
+                              Status st) {
+  switch (wd.M) {
+    case 64: return launch_m<STAT, 64>(s, beta, n_max, ncol, in, out, moments, fit, mt, wd, st);
+    case 128: return launch_m<STAT, 128>(s, beta, n_max, ncol, in, out, moments, fit, mt, wd, st);
+    case 256: return launch_m<STAT, 256>(s, beta, n_max, ncol, in, out, moments, fit, mt, wd, st);
+    case 512: return launch_m<STAT, 512>(s, beta, n_max, ncol, in, out, moments, fit, mt, wd, st);
+    case 1024: return launch_m<STAT, 1024>(s, beta, n_max, ncol, in, out, moments, fit, mt, wd, st);
+    default: return hipErrorNotSupported;
+  }
+}
+
+hipError_t launch_w2t_pruned(hipStream_t s, int statistic, double beta, int n_max, long ncol, const c2 *in, c2 *out,
+                             const c2 *moments, const TailFitDev &fit, const MatsDev &mt, const W2tDev &wd,
+                             Status st) {
+  if (statistic == 0) return launch_stat<0>(s, beta, n_max, ncol, in, out, moments, fit, mt, wd, st);
+  return launch_stat<1>(s, beta, n_max, ncol, in, out, moments, fit, mt, wd, st);
+}
+
+}  // namespace tprf
