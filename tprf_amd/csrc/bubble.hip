@@ -42,6 +42,8 @@ __global__ void __launch_bounds__(256) k_bubble_pair(double beta, int L, int nb,
   __shared__ c2 sT[CG * R1 * TLD];
   __shared__ c2 sE[CG][18];
   __shared__ c2 sPole[CG][2][3];
+  __shared__ c2 sTwp[P];   // exp(2 pi i p / P)
+  __shared__ c2 sTw1[P];   // exp(2 pi i j1 m(k) / L) of the current j1
 
   const int tid = threadIdx.x;
   const long nr = (long)d0 * d1 * d2;
@@ -72,8 +74,10 @@ __global__ void __launch_bounds__(256) k_bubble_pair(double beta, int L, int nb,
                       (c2 *)nullptr);
   }
 
-  c2 pa[PF], pb[PF];
+  for (int t = tid; t < P; t += NT) sTwp[t] = bd.twp[t];
+  c2 pa[PF], pb[PF], pw = c2{0.0, 0.0};
   auto fetch = [&](int j1) {
+    if (tid < P) pw = bd.tw1[(long)j1 * P + tid];
 #pragma unroll
     for (int i = 0; i < PF; ++i) {
       const int t = tid + i * NT;
@@ -103,6 +107,7 @@ __global__ void __launch_bounds__(256) k_bubble_pair(double beta, int L, int nb,
         sB[t] = pb[i];
       }
     }
+    if (tid < P) sTw1[tid] = pw;
     __syncthreads();
     if (j1 + 1 < N1) fetch(j1 + 1);
     // stage 1: unit (c, s): DFT-R1 over k of x at j2 = R2 k + s, twiddled by exp(2 pi i s k1 / P)
@@ -118,13 +123,13 @@ __global__ void __launch_bounds__(256) k_bubble_pair(double beta, int L, int nb,
       }
       SmallDft<R1, 1>::run(v, nullptr, 0);
 #pragma unroll
-      for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmul(v[k1], bd.twp[s * k1]);
+      for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmul(v[k1], sTwp[s * k1]);
 #pragma unroll
       for (int k1 = 0; k1 < R1; ++k1) sT[(c * R1 + k1) * TLD + s] = v[k1];
     }
     __syncthreads();
     // stage 2: unit (c, k1): DFT-R2 over s gives bins k = k1 + R1 k2; accumulate with exp(2 pi i j1 m / L)
-    const c2 *w = bd.tw1 + (long)j1 * P;
+    const c2 *w = sTw1;
 #pragma unroll
     for (int i = 0; i < U2; ++i) {
       const int u = tid + i * NT;
@@ -170,15 +175,16 @@ __global__ void __launch_bounds__(256) k_bubble_pair(double beta, int L, int nb,
         const int m = k < P / 2 ? k : k - P;
         if (m <= -nb || m >= nb) continue;
         const int im = m + nb - 1;
-        const double wm = 2.0 * kPi * (double)m / beta;
         const c2 f0 = bd.phi[im], f1 = bd.phi[nnu + im], f2 = bd.phi[2 * nnu + im];
         c2 model = cadd(cadd(cmul(a0[0], f0), cmul(a0[1], f1)), cmul(a0[2], f2));
-        c2 v = cadd(cscale(csub(acc[i][k2], model), fact), pole_sum<1>(a0, wm));
+        const c2 g0 = bd.pinv_nu[im], g1 = bd.pinv_nu[nnu + im], g2 = bd.pinv_nu[2 * nnu + im];
+        c2 v = cadd(cscale(csub(acc[i][k2], model), fact),
+                    cadd(cadd(cmul(a0[0], g0), cmul(a0[1], g1)), cmul(a0[2], g2)));
         out[((long)im * nr + r) * n4 + q] = v;
         if (mirror) {
           const c2 sm = cadd(sS[c * P + ((P - k) & (P - 1))], xl_minus_x0);
           model = cadd(cadd(cmul(a1[0], f0), cmul(a1[1], f1)), cmul(a1[2], f2));
-          v = cadd(cscale(csub(sm, model), fact), pole_sum<1>(a1, wm));
+          v = cadd(cscale(csub(sm, model), fact), cadd(cadd(cmul(a1[0], g0), cmul(a1[1], g1)), cmul(a1[2], g2)));
           out[((long)im * nr + mr) * n4 + qm] = v;
         }
       }

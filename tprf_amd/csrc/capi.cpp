@@ -43,7 +43,7 @@ struct MatsEntry {
 
 struct BubbleEntry {
   tprf::BubbleDev dev;
-  c2 *tw1 = nullptr, *twp = nullptr, *phi = nullptr;
+  c2 *tw1 = nullptr, *twp = nullptr, *phi = nullptr, *pinv_nu = nullptr;
   int *rcanon = nullptr;
 };
 
@@ -309,7 +309,7 @@ int get_bubble(tprf_ctx *ctx, double beta, int L, int nb, const int dims[3], int
     long r = ((jm % L) + L) % L;
     return std::complex<long double>(cs[r], sn[r]);
   };
-  std::vector<c2> tw1((size_t)N1 * P), twp(P), phi(3 * (size_t)nnu);
+  std::vector<c2> tw1((size_t)N1 * P), twp(P), phi(3 * (size_t)nnu), pinv_nu(3 * (size_t)nnu);
   for (int j1 = 0; j1 < N1; ++j1)
     for (int k = 0; k < P; ++k) {
       const long m = k < P / 2 ? k : k - P;
@@ -329,6 +329,8 @@ int get_bubble(tprf_ctx *ctx, double beta, int L, int nb, const int dims[3], int
       std::complex<long double> s = 0;
       for (int j = 0; j < L; ++j) s += (long double)ps[j] * e((long)j * m);
       phi[(size_t)q * nnu + im] = c2{(double)s.real(), (double)s.imag()};
+      std::complex<double> z = 1.0 / std::complex<double>(-bb[q], 2.0 * M_PI * (double)m / beta);
+      pinv_nu[(size_t)q * nnu + im] = c2{z.real(), z.imag()};
     }
   }
   const long nr = (long)dims[0] * dims[1] * dims[2];
@@ -340,7 +342,7 @@ int get_bubble(tprf_ctx *ctx, double beta, int L, int nb, const int dims[3], int
   }
   if (it != ctx->bubbles.end()) {
     (void)hipFree(it->second.tw1); (void)hipFree(it->second.twp); (void)hipFree(it->second.phi);
-    (void)hipFree(it->second.rcanon);
+    (void)hipFree(it->second.pinv_nu); (void)hipFree(it->second.rcanon);
     ctx->bubbles.erase(it);
   }
   BubbleEntry b;
@@ -348,12 +350,14 @@ int get_bubble(tprf_ctx *ctx, double beta, int L, int nb, const int dims[3], int
   if ((st = upload(ctx, &b.tw1, tw1))) return st;
   if ((st = upload(ctx, &b.twp, twp))) return st;
   if ((st = upload(ctx, &b.phi, phi))) return st;
+  if ((st = upload(ctx, &b.pinv_nu, pinv_nu))) return st;
   if ((st = upload(ctx, &b.rcanon, canon))) return st;
   b.dev.P = P;
   b.dev.ncanon = (int)canon.size();
   b.dev.tw1 = b.tw1;
   b.dev.twp = b.twp;
   b.dev.phi = b.phi;
+  b.dev.pinv_nu = b.pinv_nu;
   b.dev.rcanon = b.rcanon;
   auto res = ctx->bubbles.emplace(key, b);
   *out = &res.first->second.dev;
@@ -444,7 +448,7 @@ int tprf_ctx_destroy(tprf_ctx *ctx) {
   for (auto &kv : ctx->pole_inv) (void)hipFree(kv.second);
   for (auto &kv : ctx->bubbles) {
     (void)hipFree(kv.second.tw1); (void)hipFree(kv.second.twp); (void)hipFree(kv.second.phi);
-    (void)hipFree(kv.second.rcanon);
+    (void)hipFree(kv.second.pinv_nu); (void)hipFree(kv.second.rcanon);
   }
   (void)hipFree(ctx->d_status);
   delete ctx;

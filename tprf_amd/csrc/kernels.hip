@@ -462,6 +462,14 @@ hipError_t launch_chi_w0r(hipStream_t s, double beta, int ntau, long nr, int n, 
   return hipGetLastError();
 }
 
+// Order the LDS traffic of one wave: writes issued before are complete and
+// visible to every lane of the wave after it.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ------------------------------------------------------------------------
 // a7 RPA: for each point, M_chi = inverse(1 - M_chi0 M_U) M_chi0 (rpa.cpp:57),
 // solved as (1 - X U) Y = X by Gauss-Jordan with partial pivoting.  A group
@@ -526,7 +534,7 @@ __global__ void __launch_bounds__(256) k_rpa(long npts, const c2 *__restrict__ c
 #pragma unroll
       for (int j = 0; j < N2; ++j) { slot[slot_id][j] = A[j]; slot[slot_id][N2 + j] = X[j]; }
     }
-    __syncthreads();
+    wave_lds_sync();  // slots are private to a wave
     if (!is_piv && gidx < P) {
       const c2 f = cdiv(A[k], slot[slot_id][k]);
 #pragma unroll
@@ -535,7 +543,7 @@ __global__ void __launch_bounds__(256) k_rpa(long npts, const c2 *__restrict__ c
 #pragma unroll
       for (int j = 0; j < N2; ++j) X[j] = csub(X[j], cmul(f, slot[slot_id][N2 + j]));
     }
-    __syncthreads();
+    wave_lds_sync();
   }
   if (singular) status[2] = 1.0;
   if (valid) {
@@ -591,15 +599,19 @@ namespace tprf {
 // a2 / a6 single pass: the whole 3-D transform of C target components of one
 // outer index in LDS (layout [k][c], C = 4096 / N_k), each axis as register
 // DFTs of length D <= 16 along the lines of that axis.
+// LDS index of element (k, c): one pad slot per 16 k keeps the lanes of a
+// contiguous line (stride 16 elements) on distinct banks
+__device__ __forceinline__ int l3_idx(int k, int c, int C) { return (k + (k >> 4)) * C + c; }
+
 template <int D>
-__device__ __forceinline__ void line_dft(c2 *sm, int base, int stride, int sign) {
+__device__ __forceinline__ void line_dft(c2 *sm, int kbase, int kstride, int c, int C, int sign) {
   c2 v[D];
 #pragma unroll
-  for (int i = 0; i < D; ++i) v[i] = sm[base + i * stride];
+  for (int i = 0; i < D; ++i) v[i] = sm[l3_idx(kbase + i * kstride, c, C)];
   if (sign < 0) SmallDft<D, -1>::run(v, nullptr, 0);
   else SmallDft<D, 1>::run(v, nullptr, 0);
 #pragma unroll
-  for (int i = 0; i < D; ++i) sm[base + i * stride] = v[i];
+  for (int i = 0; i < D; ++i) sm[l3_idx(kbase + i * kstride, c, C)] = v[i];
 }
 
 template <int D0, int D1, int D2>
@@ -607,7 +619,7 @@ __global__ void __launch_bounds__(256) k_lattice3d(const c2 *__restrict__ in, c2
                                                    int sign, double scale) {
   constexpr int NK = D0 * D1 * D2;
   constexpr int C = NK >= 4096 ? 1 : 4096 / NK;
-  __shared__ c2 sm[NK * C];
+  __shared__ c2 sm[(NK + NK / 16 + 1) * C];
   const long ntc = (ntarget + C - 1) / C;
   const long tile = xcd_remap(blockIdx.x, gridDim.x);
   const long w = tile / ntc;
@@ -616,13 +628,13 @@ __global__ void __launch_bounds__(256) k_lattice3d(const c2 *__restrict__ in, c2
   const c2 *src = in + w * NK * ntarget + c0;
   for (int t = threadIdx.x; t < NK * cn; t += blockDim.x) {
     const int k = t / cn, c = t - k * cn;
-    sm[k * C + c] = src[(long)k * ntarget + c];
+    sm[l3_idx(k, c, C)] = src[(long)k * ntarget + c];
   }
   __syncthreads();
   if constexpr (D2 > 1) {
     for (int l = threadIdx.x; l < C * D0 * D1; l += blockDim.x) {
       const int c = l % C, rest = l / C;  // rest = i0 * D1 + i1
-      line_dft<D2>(sm, rest * D2 * C + c, C, sign);
+      line_dft<D2>(sm, rest * D2, 1, c, C, sign);
     }
     __syncthreads();
   }
@@ -630,21 +642,21 @@ __global__ void __launch_bounds__(256) k_lattice3d(const c2 *__restrict__ in, c2
     for (int l = threadIdx.x; l < C * D0 * D2; l += blockDim.x) {
       const int c = l % C, rest = l / C;
       const int i2 = rest % D2, i0 = rest / D2;
-      line_dft<D1>(sm, (i0 * D1 * D2 + i2) * C + c, D2 * C, sign);
+      line_dft<D1>(sm, i0 * D1 * D2 + i2, D2, c, C, sign);
     }
     __syncthreads();
   }
   if constexpr (D0 > 1) {
     for (int l = threadIdx.x; l < C * D1 * D2; l += blockDim.x) {
       const int c = l % C, rest = l / C;  // rest = i1 * D2 + i2
-      line_dft<D0>(sm, rest * C + c, D1 * D2 * C, sign);
+      line_dft<D0>(sm, rest, D1 * D2, c, C, sign);
     }
     __syncthreads();
   }
   c2 *dst = out + w * NK * ntarget + c0;
   for (int t = threadIdx.x; t < NK * cn; t += blockDim.x) {
     const int k = t / cn, c = t - k * cn;
-    dst[(long)k * ntarget + c] = cscale(sm[k * C + c], scale);
+    dst[(long)k * ntarget + c] = cscale(sm[l3_idx(k, c, C)], scale);
   }
 }
 

@@ -39,11 +39,13 @@ __device__ __forceinline__ void w2t_butterflies(c2 *v, int j, int Ns, const c2 *
   }
 }
 
+constexpr int kW2tThreads = 256;
+
 template <int STAT, int M>
-__global__ void __launch_bounds__(512) k_w2t_pruned(double beta, int n_max, long ncol, const c2 *__restrict__ in,
+__global__ void __launch_bounds__(kW2tThreads, 2) k_w2t_pruned(double beta, int n_max, long ncol, const c2 *__restrict__ in,
                                                     c2 *__restrict__ out, const c2 *__restrict__ moments,
                                                     TailFitDev fit, MatsDev mt, W2tDev wd, double *status) {
-  constexpr int NT = 512;
+  constexpr int NT = kW2tThreads;
   constexpr int TPC = M / 8;  // threads per column
   constexpr int C = NT / TPC;  // columns per workgroup
   using Plan = W2tPlan<M>;
@@ -51,6 +53,7 @@ __global__ void __launch_bounds__(512) k_w2t_pruned(double beta, int n_max, long
   __shared__ c2 s_coef[C][kMaxOrder];
   __shared__ c2 s_a[C][3];
   __shared__ c2 s_m1[C];
+  __shared__ c2 s_tw[M];
 
   const int tid = threadIdx.x;
   const int c = tid % C, j = tid / C;
@@ -62,6 +65,7 @@ __global__ void __launch_bounds__(512) k_w2t_pruned(double beta, int n_max, long
   const int nmesh = (STAT == 0) ? 2 * n_max : 2 * n_max - 1;
   const int first = (STAT == 0) ? -n_max : -(n_max - 1);
 
+  for (int t = tid; t < M; t += NT) s_tw[t] = wd.twM[t];
   // 1. this thread's eight inputs (the same FFT positions e = j + TPC r for every t),
   //    staged once in registers and once in LDS for the tail fit
   c2 raw[8];
@@ -75,7 +79,8 @@ __global__ void __launch_bounds__(512) k_w2t_pruned(double beta, int n_max, long
   __syncthreads();
   // data index i <-> FFT position ((first + i) mod M)
   auto fitval = [&](int f, int cc) {
-    const int e = ((first + fit.idx[f]) % M + M) % M;
+    int e = first + fit.idx[f];  // in (-M/2, M/2)
+    if (e < 0) e += M;
     return buf[e * C + cc];
   };
   // 2. tail moments (least-squares fit, or given)
@@ -134,7 +139,7 @@ __global__ void __launch_bounds__(512) k_w2t_pruned(double beta, int n_max, long
       const int n = e < M / 2 ? e : e - M;
       c2 x = raw[r];
       if (t && n >= first && n < first + nmesh) {
-        long tn = ((long)t * n) % L;
+        int tn = t * n;  // |t n| < Q M / 2 <= L
         if (tn < 0) tn += L;
         x = cmul(x, wd.twL[tn]);
       }
@@ -144,9 +149,9 @@ __global__ void __launch_bounds__(512) k_w2t_pruned(double beta, int n_max, long
 #pragma unroll
     for (int p = 0; p < Plan::npass; ++p) {
       const int R = (p < Plan::n8) ? 8 : Plan::last;
-      if (R == 8) w2t_butterflies<M, 8, -1>(v, j, Ns, wd.twM);
-      else if (R == 4) w2t_butterflies<M, 4, -1>(v, j, Ns, wd.twM);
-      else w2t_butterflies<M, 2, -1>(v, j, Ns, wd.twM);
+      if (R == 8) w2t_butterflies<M, 8, -1>(v, j, Ns, s_tw);
+      else if (R == 4) w2t_butterflies<M, 4, -1>(v, j, Ns, s_tw);
+      else w2t_butterflies<M, 2, -1>(v, j, Ns, s_tw);
       const int B = 8 / R;
       if (p + 1 < Plan::npass) {
         // write this pass's outputs, read the next pass's inputs
@@ -194,9 +199,9 @@ bool w2t_pruned_supported(int M) { return M == 64 || M == 128 || M == 256 || M =
 template <int STAT, int M>
 static hipError_t launch_m(hipStream_t s, double beta, int n_max, long ncol, const c2 *in, c2 *out, const c2 *moments,
                            const TailFitDev &fit, const MatsDev &mt, const W2tDev &wd, Status st) {
-  constexpr int C = 512 / (M / 8);
+  constexpr int C = kW2tThreads / (M / 8);
   const long nb = (ncol + C - 1) / C;
-  hipLaunchKernelGGL((k_w2t_pruned<STAT, M>), dim3(nb), dim3(512), 0, s, beta, n_max, ncol, in, out, moments, fit, mt,
+  hipLaunchKernelGGL((k_w2t_pruned<STAT, M>), dim3(nb), dim3(kW2tThreads), 0, s, beta, n_max, ncol, in, out, moments, fit, mt,
                      wd, st.d);
   return hipGetLastError();
 }
