@@ -131,6 +131,55 @@ int tprf_chi_w0r_from_chi_tr(tprf_ctx *ctx, double beta, int ntau, long nr, int 
 int tprf_solve_rpa_PH(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *chi0, const tprf_c128 *U,
                       tprf_c128 *chi);
 
+/* ---- differentiation: static chi_RPA(Q, i nu = 0) and its adjoints ------
+ * No counterpart in the reference (TPRF has no derivatives).  Each adjoint is
+ * the conjugate transpose of the complex-linear map named, or the reverse-mode
+ * derivative of a nonlinear one, in the convention dL = Re sum conj(xbar) dx
+ * (xbar = dL/dRe x + i dL/dIm x, torch's complex gradient).  All arrays are
+ * device memory; scalars returned through device pointers.                  */
+
+/* Adjoint of lattice_dyson_g_wk(mu, e_k) (gf.cpp:91-132) given G and Gbar:
+ * ebar_k[k] = sum_w G^H Gbar G^H, *mubar = -Re sum_{w,k} tr(G^H Gbar G^H).
+ * g_wk, gbar_wk [2 n_iw][nk][n][n]; ebar_k [nk][n][n]; mubar one double.
+ * norb <= 3.                                                                 */
+int tprf_lattice_dyson_g_wk_adjoint(tprf_ctx *ctx, int n_iw, long nk, int norb, const tprf_c128 *g_wk,
+                                    const tprf_c128 *gbar_wk, tprf_c128 *ebar_k, double *mubar);
+/* Adjoint of fourier_wk_to_wr (fourier_lattice.cpp:49): gbar_wk =
+ * FFT_{+}(gbar_wr) / N_k.                                                    */
+int tprf_fourier_wk_to_wr_adjoint(tprf_ctx *ctx, long nw, const int dims[3], long ntarget, const tprf_c128 *gbar_wr,
+                                  tprf_c128 *gbar_wk);
+/* Adjoint of fourier_wr_to_tr (fourier_matsubara.cpp:218-308) with fitted
+ * moments (fitted = 1: the least-squares tail fit is differentiated too) or
+ * given moments (fitted = 0: the moments are constants).  gbar_tr[ntau][ncol]
+ * -> gbar_wr[nmesh][ncol].  TPRF_ERR_UNSUPPORTED unless ntau - 1 is a multiple
+ * of the smallest power of two M >= 2 n_max, 64 <= M <= 1024.                */
+int tprf_fourier_wr_to_tr_adjoint(tprf_ctx *ctx, int statistic, double beta, int n_max, int ntau, long ncol,
+                                  const tprf_c128 *gbar_tr, tprf_c128 *gbar_wr, int fitted);
+/* nu = 0 row of chi0_wr_from_grt_PH (chi_imtime.cpp:139-182): the bosonic
+ * tau -> nu transform at nu = 0 is a real linear functional
+ * chi0_abcd(r) = sum_j c_j G_da(tau_j, r) G_bc(tau_{L-j}, -r); out[nr][n^4].
+ * norb <= 3.                                                                 */
+int tprf_chi0_static_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb,
+                                 const tprf_c128 *g_tr, tprf_c128 *chi0_r);
+/* Adjoint of tprf_chi0_static_from_grt_PH with respect to g_tr:
+ * chibar_r[nr][n^4] -> gbar_tr[ntau][nr][n^2] (g_tr is the forward input).  */
+int tprf_chi0_static_from_grt_PH_adjoint(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb,
+                                         const tprf_c128 *g_tr, const tprf_c128 *chibar_r, tprf_c128 *gbar_tr);
+/* One momentum of fourier_wr_to_wk (fourier_lattice.cpp:55-58):
+ * out[t] = sum_r exp(+2 pi i Q.r / dims) in[r][t], Q = q[0..2] (integers),
+ * and its adjoint gbar_r[r][t] = exp(-2 pi i Q.r / dims) gbar_q[t].          */
+int tprf_fourier_r_to_q(tprf_ctx *ctx, const int dims[3], const int q[3], long ntarget, const tprf_c128 *in_r,
+                        tprf_c128 *out_q);
+int tprf_fourier_r_to_q_adjoint(tprf_ctx *ctx, const int dims[3], const int q[3], long ntarget,
+                                const tprf_c128 *gbar_q, tprf_c128 *gbar_r);
+/* solve_rpa_PH (rpa.cpp:29-64) at npts points with a U per point, and its
+ * reverse-mode derivative: with chibar given (may be NULL for forward only),
+ * chi0bar = A^-H chibar (1 + U chi)^H and Ubar = chi0^H A^-H chibar chi^H in
+ * the PH matrix grouping, A = 1 - chi0 U.  chi0, U, chibar, chi, chi0bar,
+ * Ubar [npts][n^4].  norb <= 2.                                              */
+int tprf_solve_rpa_PH_adjoint(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *chi0, const tprf_c128 *U,
+                              const tprf_c128 *chibar, tprf_c128 *chi, tprf_c128 *chi0bar, tprf_c128 *Ubar);
+
 #ifdef __cplusplus
 }
 #endif

@@ -51,6 +51,7 @@ def lib():
             "or_fermi": [d],
             "or_num_threads": [],
             "or_set_num_threads": [i],
+            "or_set_fit_check": [i],
         }
         for name, args in sig.items():
             f = getattr(_lib, name)
@@ -59,6 +60,7 @@ def lib():
         _lib.or_fermi.restype = ctypes.c_double
         _lib.or_vandermonde_inverse8.restype = None
         _lib.or_set_num_threads.restype = None
+        _lib.or_set_fit_check.restype = None
     return _lib
 
 
@@ -172,6 +174,44 @@ def fourier_t_to_w(gt, beta, nw, statistic=BOSON):
     out = np.empty((nout,) + gt.shape[1:], dtype=np.complex128)
     _check(lib().or_fourier_t_to_w(statistic, beta, ntau, nw, ncol, _p(gt), _p(out)), "fourier_t_to_w")
     return out
+
+
+def w_to_t_matrix(beta, n_max, statistic=FERMION, ntau=None):
+    """Matrix T[ntau, nmesh] of the omega->tau transform with fitted tail moments.
+
+    The transform (fourier_matsubara.cpp:218-308) is complex-linear in the
+    Matsubara data once the least-squares tail fit is linear, so T is read off
+    column by column from unit inputs (the fit-error assertion is switched off
+    for that, since a unit input is no smooth Green's function).
+    """
+    nmesh = 2 * n_max if statistic == FERMION else 2 * n_max - 1
+    lib().or_set_fit_check(0)
+    try:
+        return fourier_w_to_t(np.eye(nmesh, dtype=np.complex128), beta, statistic, ntau)
+    finally:
+        lib().or_set_fit_check(1)
+
+
+def w_to_t_moments_matrix(beta, n_max, statistic=FERMION, ntau=None):
+    """Matrix Mop[3, nmesh] of the fitted moments (m1, m2, m3) as a linear map of
+    the Matsubara data (the tail fit of fourier_matsubara.cpp:225-246)."""
+    nmesh = 2 * n_max if statistic == FERMION else 2 * n_max - 1
+    lib().or_set_fit_check(0)
+    try:
+        _, m, _ = fourier_w_to_t(np.eye(nmesh, dtype=np.complex128), beta, statistic, ntau, return_moments=True)
+    finally:
+        lib().or_set_fit_check(1)
+    return m
+
+
+def t_to_w_matrix(beta, ntau, nw, statistic=BOSON, chunk=1024):
+    """Matrix W[nmesh(nw), ntau] of the tau->omega transform (fourier_matsubara.cpp:42-184)."""
+    cols = []
+    for j0 in range(0, ntau, chunk):
+        e = np.zeros((ntau, min(chunk, ntau - j0)), dtype=np.complex128)
+        e[j0 + np.arange(e.shape[1]), np.arange(e.shape[1])] = 1.0
+        cols.append(fourier_t_to_w(e, beta, nw, statistic))
+    return np.concatenate(cols, axis=1)
 
 
 def chi0_tr_from_grt_PH(g_tr, dims):

@@ -36,6 +36,12 @@ typedef long double _Complex lcplx;
 #define M_PI 3.14159265358979323846
 #endif
 
+/* The fit-error assertion of the omega->tau transform can be switched off so
+ * that the transform's matrix can be read column by column from unit inputs
+ * (the map is complex-linear; the assertion is a check on the data only). */
+static int g_fit_check = 1;
+void or_set_fit_check(int on) { g_fit_check = on != 0; }
+
 int or_num_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();
@@ -464,7 +470,7 @@ int or_fourier_w_to_t(int statistic, double beta, int n_max, int ntau, long ncol
   fft_plan_free(&plan);
   free(qr);
   if (fit_error) *fit_error = max_err;
-  if (!moments && max_err >= 1e-3) return OR_ERR_TAIL_FIT; /* fourier_matsubara.cpp:227-230 */
+  if (!moments && max_err >= 1e-3 && g_fit_check) return OR_ERR_TAIL_FIT; /* fourier_matsubara.cpp:227-230 */
   return OR_OK;
 }
 

@@ -74,6 +74,7 @@ void or_vandermonde_inverse8(double *vinv);
 double or_fermi(double e);
 int or_num_threads(void);
 void or_set_num_threads(int n);
+void or_set_fit_check(int on); /* 0: no TAIL_FIT error on bad fits (for reading off the linear map) */
 
 #ifdef __cplusplus
 }

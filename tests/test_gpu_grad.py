@@ -1,0 +1,342 @@
+"""GPU checks of the differentiation path: chi_RPA(Q, 0) and its hand-written adjoints.
+
+* every linear adjoint passes the dot-product test Re<A x, y> = Re<x, A^H y>
+  (complex inner products; the quadratic bubble through its exact
+  linearisation (chi(G + d) - chi(G - d)) / 2);
+* the nonlinear ones (Dyson, RPA) match torch autograd on the CPU;
+* the static forward matches the oracle's chi0_wr_from_grt_PH at nu = 0;
+* the full gradient dL/d(t, t', mu, U) of L = Re tr chi_RPA(Q, 0) matches the
+  FP64 torch reference (oracle/torch_ref.py) and central finite differences
+  (h = 1e-6) of the GPU forward.
+Tolerances are written per test.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def T():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("no GPU visible")
+    return torch
+
+
+def rnd(rng, *shape):
+    return rng.normal(size=shape) + 1j * rng.normal(size=shape)
+
+
+_HOLD = []
+
+
+@pytest.fixture(autouse=True)
+def _hold_device_copies():
+    """Keep every gpu() copy alive until the test ends: a pointer taken from a
+    temporary (ptr(gpu(x))) would otherwise be freed back to torch's caching
+    allocator at once and could be handed to the next copy in the same call."""
+    yield
+    _HOLD.clear()
+
+
+def gpu(x):
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(x, dtype=np.complex128)).cuda()
+    _HOLD.append(t)
+    return t
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+def vdot(a, b):
+    return np.vdot(np.asarray(a).ravel(), np.asarray(b).ravel())
+
+
+def lib():
+    from tprf_amd import _lib
+    return _lib
+
+
+def dot_check(Ax, y, x, AHy, tol=1e-12):
+    lhs, rhs = vdot(Ax, y), vdot(x, AHy)
+    assert abs(lhs - rhs) <= tol * max(abs(lhs), 1e-300) * 10 + tol * np.abs(Ax).max() * np.abs(y).max(), (lhs, rhs)
+    return abs(lhs - rhs) / abs(lhs)
+
+
+# ---------------------------------------------------------------- lattice FFT
+@pytest.mark.parametrize("dims", [(8, 8, 1), (16, 16, 16), (6, 4, 1), (64, 64, 1)])
+def test_lattice_fft_adjoint_dot(T, dims):
+    L = lib()
+    rng = np.random.default_rng(1)
+    nk, nw, nt = int(np.prod(dims)), 3, 4
+    x, y = rnd(rng, nw, nk, nt), rnd(rng, nw, nk, nt)
+    Ax, AHy = T.empty_like(gpu(x)), T.empty_like(gpu(y))
+    c = L.ctx()
+    L.call("tprf_fourier_wk_to_wr", c, nw, L.dims3(dims), nt, L.ptr(gpu(x)), L.ptr(Ax))
+    L.call("tprf_fourier_wk_to_wr_adjoint", c, nw, L.dims3(dims), nt, L.ptr(gpu(y)), L.ptr(AHy))
+    dot_check(host(Ax), y, x, host(AHy))
+
+
+# ---------------------------------------------------------------- omega -> tau
+@pytest.mark.parametrize("stat", [0, 1])
+@pytest.mark.parametrize("n_max", [32, 64, 512])
+@pytest.mark.parametrize("fitted", [1, 0])
+def test_w2t_adjoint_dot(T, stat, n_max, fitted):
+    L = lib()
+    rng = np.random.default_rng(2 + n_max + stat)
+    beta, ncol = 20.0, 13
+    nmesh = 2 * n_max if stat == 0 else 2 * n_max - 1
+    ntau = 6 * n_max + 1
+    x, y = rnd(rng, nmesh, ncol), rnd(rng, ntau, ncol)
+    Ax, AHy = T.empty((ntau, ncol), dtype=T.complex128, device="cuda"), T.empty_like(gpu(x))
+    c = L.ctx()
+    mom = None if fitted else T.zeros((3, ncol), dtype=T.complex128, device="cuda")
+    L.call("tprf_fourier_wr_to_tr", c, stat, beta, n_max, ntau, ncol, L.ptr(gpu(x)), L.ptr(Ax), L.ptr(mom))
+    L.call("tprf_fourier_wr_to_tr_adjoint", c, stat, beta, n_max, ntau, ncol, L.ptr(gpu(y)), L.ptr(AHy), fitted)
+    lhs, rhs = vdot(host(Ax), y), vdot(x, host(AHy))
+    # With fitted moments and random data, the moments m_p = omega_max^p (P g)_p reach
+    # omega_max^3 |P| ~ 1e10 at n_max = 512 (omega_max = 161, an ill-conditioned
+    # order-7 fit on z in [1, 1.2]) and cancel to O(1e4) outputs: rounding of
+    # ~eps * 1e10 per entry.  Given moments (fitted = 0) have no such term.
+    tol = 5e-9 if (fitted and n_max >= 512) else 1e-10
+    assert abs(lhs - rhs) <= tol * abs(lhs), (lhs, rhs)
+
+
+@pytest.mark.parametrize("stat", [0, 1])
+def test_w2t_adjoint_dot_outside_fit_window(T, stat):
+    """n_max = 512 with the data zero on the fit window (so every moment is
+    exactly zero): the rest of the map and its adjoint to 1e-12."""
+    L = lib()
+    rng = np.random.default_rng(31 + stat)
+    beta, ncol, n_max = 20.0, 7, 512
+    nmesh = 2 * n_max if stat == 0 else 2 * n_max - 1
+    ntau = 6 * n_max + 1
+    x, y = rnd(rng, nmesh, ncol), rnd(rng, ntau, ncol)
+    x[:100] = 0.0  # the window is the 30 + 30 outermost points, step 3
+    x[-100:] = 0.0
+    Ax, AHy = T.empty((ntau, ncol), dtype=T.complex128, device="cuda"), T.empty_like(gpu(x))
+    c = L.ctx()
+    L.call("tprf_fourier_wr_to_tr", c, stat, beta, n_max, ntau, ncol, L.ptr(gpu(x)), L.ptr(Ax), None)
+    L.call("tprf_fourier_wr_to_tr_adjoint", c, stat, beta, n_max, ntau, ncol, L.ptr(gpu(y)), L.ptr(AHy), 1)
+    lhs, rhs = vdot(host(Ax), y), vdot(x, host(AHy))
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs), (lhs, rhs)
+
+
+def test_w2t_adjoint_unsupported_size(T):
+    L = lib()
+    c = L.ctx()
+    x = T.zeros((6 * 48 + 1, 2), dtype=T.complex128, device="cuda")
+    y = T.zeros((96, 2), dtype=T.complex128, device="cuda")
+    st = L.lib().tprf_fourier_wr_to_tr_adjoint(c, 0, 5.0, 48, 6 * 48 + 1, 2, L.ptr(x), L.ptr(y), 1)
+    assert st == -9
+
+
+# ---------------------------------------------------------------- static bubble
+def _gtr_oracle(oracle, rng, n, dims, beta, n_iw):
+    nk = int(np.prod(dims))
+    a = rnd(rng, nk, n, n) * 0.7
+    e = 0.5 * (a + np.conj(np.swapaxes(a, 1, 2)))
+    g_wk = oracle.lattice_dyson_g_wk(beta, n_iw, 0.1, e)
+    g_wr = oracle.fourier_lattice(g_wk, dims, -1)
+    return oracle.fourier_w_to_t(g_wr, beta, oracle.FERMION)
+
+
+@pytest.mark.parametrize("n,dims", [(1, (8, 8, 1)), (2, (4, 6, 1)), (3, (4, 4, 2))])
+def test_static_bubble_forward_vs_oracle(T, oracle, n, dims):
+    L = lib()
+    rng = np.random.default_rng(5 + n)
+    beta, n_iw = 5.0, 32
+    g_tr = _gtr_oracle(oracle, rng, n, dims, beta, n_iw)
+    ref = oracle.chi0_wr_from_grt_PH(g_tr, dims, beta, 1)[0]
+    out = T.empty(ref.shape, dtype=T.complex128, device="cuda")
+    L.call("tprf_chi0_static_from_grt_PH", L.ctx(), beta, g_tr.shape[0], L.dims3(dims), n, L.ptr(gpu(g_tr)),
+           L.ptr(out))
+    assert relerr(host(out), ref) <= 1e-10
+
+
+@pytest.mark.parametrize("n,dims", [(1, (8, 8, 1)), (2, (4, 6, 1)), (3, (4, 4, 2))])
+def test_static_bubble_adjoint_dot(T, oracle, n, dims):
+    L = lib()
+    rng = np.random.default_rng(7 + n)
+    beta, n_iw = 5.0, 32
+    g_tr = _gtr_oracle(oracle, rng, n, dims, beta, n_iw)
+    ntau, nr = g_tr.shape[0], int(np.prod(dims))
+    d = rnd(rng, *g_tr.shape) * 0.1
+    ybar = rnd(rng, nr, n, n, n, n)
+    c = L.ctx()
+
+    def chi(g):
+        o = T.empty((nr, n, n, n, n), dtype=T.complex128, device="cuda")
+        L.call("tprf_chi0_static_from_grt_PH", c, beta, ntau, L.dims3(dims), n, L.ptr(gpu(g)), L.ptr(o))
+        return host(o)
+
+    Jd = 0.5 * (chi(g_tr + d) - chi(g_tr - d))  # exact: chi is quadratic in G
+    gbar = T.empty(g_tr.shape, dtype=T.complex128, device="cuda")
+    L.call("tprf_chi0_static_from_grt_PH_adjoint", c, beta, ntau, L.dims3(dims), n, L.ptr(gpu(g_tr)),
+           L.ptr(gpu(ybar)), L.ptr(gbar))
+    lhs, rhs = vdot(Jd, ybar), vdot(d, host(gbar))
+    assert abs(lhs - rhs) <= 1e-11 * abs(lhs), (lhs, rhs)
+
+
+# ---------------------------------------------------------------- one momentum
+@pytest.mark.parametrize("dims,q", [((8, 8, 1), (4, 4, 0)), ((6, 4, 2), (1, 3, 1)), ((64, 64, 1), (32, 32, 0))])
+def test_r_to_q_and_adjoint(T, oracle, dims, q):
+    L = lib()
+    rng = np.random.default_rng(11)
+    nr, nt = int(np.prod(dims)), 16
+    x = rnd(rng, nr, nt)
+    out = T.empty((nt,), dtype=T.complex128, device="cuda")
+    c = L.ctx()
+    L.call("tprf_fourier_r_to_q", c, L.dims3(dims), L.dims3(q), nt, L.ptr(gpu(x)), L.ptr(out))
+    full = oracle.fourier_lattice(x[None], dims, +1)[0]
+    iq = (q[0] * dims[1] + q[1]) * dims[2] + q[2]
+    assert relerr(host(out), full[iq]) <= 1e-12
+    y = rnd(rng, nt)
+    back = T.empty((nr, nt), dtype=T.complex128, device="cuda")
+    L.call("tprf_fourier_r_to_q_adjoint", c, L.dims3(dims), L.dims3(q), nt, L.ptr(gpu(y)), L.ptr(back))
+    dot_check(host(out), y, x, host(back))
+
+
+# ---------------------------------------------------------------- Dyson adjoint
+@pytest.mark.parametrize("n", [1, 2, 3])
+def test_dyson_adjoint_vs_torch(T, n):
+    L = lib()
+    rng = np.random.default_rng(13 + n)
+    beta, n_iw, mu, nk = 6.0, 40, 0.3, 37
+    a = rnd(rng, nk, n, n)
+    e = 0.5 * (a + np.conj(np.swapaxes(a, 1, 2)))
+    gbar = rnd(rng, 2 * n_iw, nk, n, n)
+    # torch reference on the CPU: L = Re sum conj(gbar) G
+    et = T.tensor(e, requires_grad=True)
+    mut = T.tensor(mu, dtype=T.float64, requires_grad=True)
+    w = T.tensor(np.pi * (2 * np.arange(-n_iw, n_iw) + 1) / beta)
+    G = T.linalg.inv((1j * w[:, None, None, None] + mut) * T.eye(n, dtype=T.complex128) - et[None])
+    (T.conj(T.tensor(gbar)) * G).sum().real.backward()
+    c = L.ctx()
+    g = T.empty((2 * n_iw, nk, n, n), dtype=T.complex128, device="cuda")
+    L.call("tprf_lattice_dyson_g_wk", c, beta, n_iw, mu, nk, n, L.ptr(gpu(e)), 0, None, L.ptr(g))
+    ebar = T.empty((nk, n, n), dtype=T.complex128, device="cuda")
+    mubar = T.empty((), dtype=T.float64, device="cuda")
+    L.call("tprf_lattice_dyson_g_wk_adjoint", c, n_iw, nk, n, L.ptr(g), L.ptr(gpu(gbar)), L.ptr(ebar),
+           L.dptr(mubar))
+    assert relerr(host(ebar), et.grad.numpy()) <= 1e-12
+    assert abs(float(mubar) - float(mut.grad)) <= 1e-12 * abs(float(mut.grad))
+
+
+# ---------------------------------------------------------------- RPA adjoint
+@pytest.mark.parametrize("n", [1, 2])
+def test_rpa_point_adjoint_vs_torch(T, n):
+    L = lib()
+    rng = np.random.default_rng(17 + n)
+    npts = 5
+    chi0 = rnd(rng, npts, n, n, n, n) * 0.3
+    U = rnd(rng, npts, n, n, n, n) * 0.5
+    cbar = rnd(rng, npts, n, n, n, n)
+    c0 = T.tensor(chi0, requires_grad=True)
+    Ut = T.tensor(U, requires_grad=True)
+    X = c0.permute(0, 1, 2, 4, 3).reshape(npts, n * n, n * n)
+    MU = Ut.permute(0, 1, 2, 4, 3).reshape(npts, n * n, n * n)
+    chim = T.linalg.solve(T.eye(n * n, dtype=T.complex128) - X @ MU, X)
+    chi_ref = chim.reshape(npts, n, n, n, n).permute(0, 1, 2, 4, 3)
+    (T.conj(T.tensor(cbar)) * chi_ref).sum().real.backward()
+    c = L.ctx()
+    chi, c0bar, Ubar = (T.empty((npts, n, n, n, n), dtype=T.complex128, device="cuda") for _ in range(3))
+    L.call("tprf_solve_rpa_PH_adjoint", c, npts, n, L.ptr(gpu(chi0)), L.ptr(gpu(U)), L.ptr(gpu(cbar)), L.ptr(chi),
+           L.ptr(c0bar), L.ptr(Ubar))
+    assert relerr(host(chi), chi_ref.detach().numpy()) <= 1e-13
+    assert relerr(host(c0bar), c0.grad.numpy()) <= 1e-12
+    assert relerr(host(Ubar), Ut.grad.numpy()) <= 1e-12
+
+
+# ---------------------------------------------------------------- full gradient
+def _grads_gpu(T, spec, dims, x0):
+    from tprf_amd import autograd as A
+    leaves = {k: T.tensor(v, dtype=T.float64, device="cuda", requires_grad=True) for k, v in x0.items()}
+    chi = A.chi_rpa_static(A.square_e_k(leaves["t"], leaves["tp"], dims), leaves["mu"], A.hubbard_U(leaves["U"]),
+                           spec)
+    loss = A.trace_loss(chi)
+    loss.backward()
+    return float(loss), {k: float(v.grad) for k, v in leaves.items()}
+
+
+def _grads_ref(T, ref, dims, x0):
+    from tprf_amd import autograd as A
+    leaves = {k: T.tensor(v, dtype=T.float64, requires_grad=True) for k, v in x0.items()}
+    chi = ref.chi(A.square_e_k(leaves["t"], leaves["tp"], dims), leaves["mu"], A.hubbard_U(leaves["U"]))
+    loss = A.trace_loss(chi)
+    loss.backward()
+    return float(loss), {k: float(v.grad) for k, v in leaves.items()}
+
+
+def _fd_gpu(T, spec, dims, x0, h=1e-6):
+    from tprf_amd import autograd as A
+
+    def f(v):
+        with T.no_grad():
+            t = {k: T.tensor(x, dtype=T.float64, device="cuda") for k, x in v.items()}
+            chi = A.chi_rpa_static(A.square_e_k(t["t"], t["tp"], dims), t["mu"], A.hubbard_U(t["U"]), spec)
+            return float(A.trace_loss(chi))
+
+    out = {}
+    for k in x0:
+        vp, vm = dict(x0), dict(x0)
+        vp[k] += h
+        vm[k] -= h
+        out[k] = (f(vp) - f(vm)) / (2 * h)
+    return out
+
+
+@pytest.mark.parametrize("case", ["small", "cfg_64x64_b20_niw512"])
+def test_full_gradient(T, oracle, case):
+    from oracle.torch_ref import StaticChainRef
+    from tprf_amd import autograd as A
+    if case == "small":
+        beta, n_iw, dims, q = 5.0, 32, (8, 8, 1), (4, 4, 0)
+    else:  # single band, 64x64 k, beta = 20, 512 fermionic frequencies, Q = (pi, pi)
+        beta, n_iw, dims, q = 20.0, 512, (64, 64, 1), (32, 32, 0)
+    x0 = {"t": 1.0, "tp": -0.3, "mu": -0.5, "U": 2.0}
+    spec = A.StaticRPA(beta, n_iw, dims, q, norb=1)
+    loss, g = _grads_gpu(T, spec, dims, x0)
+    ref = StaticChainRef(beta, n_iw, dims, q)
+    loss_ref, g_ref = _grads_ref(T, ref, dims, x0)
+    fd = _fd_gpu(T, spec, dims, x0)
+    print(f"[{case}] L = {loss!r} (ref {loss_ref!r}); grad {g}; ref {g_ref}; fd {fd}", flush=True)
+    assert abs(loss - loss_ref) <= 1e-10 * abs(loss_ref)
+    scale = max(abs(v) for v in g_ref.values())
+    for k in x0:
+        assert abs(g[k] - g_ref[k]) <= 1e-9 * scale, (k, g[k], g_ref[k])
+        assert abs(g[k] - fd[k]) <= 1e-6 * scale, (k, g[k], fd[k])
+
+
+def test_gradient_two_orbital_e_k(T, oracle):
+    """Complex e_k and U gradients of a two-orbital lattice against the torch reference."""
+    from oracle.torch_ref import StaticChainRef
+    from tprf_amd import autograd as A
+    rng = np.random.default_rng(23)
+    beta, n_iw, dims, q, n = 4.0, 32, (6, 6, 1), (3, 0, 0), 2
+    kx, ky = np.meshgrid(2 * np.pi * np.arange(6) / 6, 2 * np.pi * np.arange(6) / 6, indexing="ij")
+    h = rnd(rng, 2, 2) * 0.3
+    h = (h + h.conj().T) / 2
+    e = (-2 * (np.cos(kx) + np.cos(ky))).ravel()[:, None, None] * np.eye(2) + \
+        (np.cos(kx) - np.cos(ky)).ravel()[:, None, None] * h
+    U = rnd(rng, 2, 2, 2, 2) * 0.2
+    spec = A.StaticRPA(beta, n_iw, dims, q, norb=n)
+    e_g = T.tensor(e, device="cuda", requires_grad=True)
+    U_g = T.tensor(U, device="cuda", requires_grad=True)
+    mu_g = T.tensor(0.2, dtype=T.float64, device="cuda", requires_grad=True)
+    A.trace_loss(A.chi_rpa_static(e_g, mu_g, U_g, spec)).backward()
+    ref = StaticChainRef(beta, n_iw, dims, q)
+    e_r = T.tensor(e, requires_grad=True)
+    U_r = T.tensor(U, requires_grad=True)
+    mu_r = T.tensor(0.2, dtype=T.float64, requires_grad=True)
+    A.trace_loss(ref.chi(e_r, mu_r, U_r)).backward()
+    assert relerr(host(e_g.grad), e_r.grad.numpy()) <= 1e-9
+    assert relerr(host(U_g.grad), U_r.grad.numpy()) <= 1e-9
+    assert abs(float(mu_g.grad) - float(mu_r.grad)) <= 1e-9 * abs(float(mu_r.grad))

@@ -1,0 +1,423 @@
+// adjoint.hip -- reverse-mode kernels of the static (nu = 0) bubble + RPA chain
+// at one lattice momentum Q:
+//   L = Re tr chi_RPA(Q, 0),  chi_RPA = (1 - chi0 U)^-1 chi0 (PH grouping),
+//   chi0(Q, 0) = sum_r exp(i Q.r) chi0(r, 0),  chi0(r, 0) = sum_j c_j x(tau_j, r),
+//   x_abcd(tau_j, r) = G_da(tau_j, r) G_bc(tau_{L-j}, -r).
+// The weights c_j are the nu = 0 row of the tau -> nu map (real).  All
+// adjoints follow dL = Re sum conj(xbar) dx, i.e. xbar = dL/dRe x + i dL/dIm x.
+#include "common.hpp"
+#include "fft_device.hpp"
+#include "mats_device.hpp"
+
+namespace tprf {
+
+// chi0(Q) = sum_r exp(2 pi i Q.r / dims) chi0(r): one workgroup per target
+// component, deterministic tree reduction over r.
+__global__ void __launch_bounds__(256) k_q_project(const c2 *__restrict__ chi_r, c2 *__restrict__ chi_q, int d0, int d1,
+                                                   int d2, int q0, int q1, int q2, int ntarget) {
+  __shared__ c2 red[256];
+  const int comp = blockIdx.x;
+  const long nr = (long)d0 * d1 * d2;
+  c2 s = c2{0.0, 0.0};
+  for (long r = threadIdx.x; r < nr; r += blockDim.x) {
+    const long i2 = r % d2, i1 = (r / d2) % d1, i0 = r / ((long)d1 * d2);
+    // phase fraction in [0, 1) computed exactly in integers per dimension
+    const double f = (double)((q0 * i0) % d0) / d0 + (double)((q1 * i1) % d1) / d1 + (double)((q2 * i2) % d2) / d2;
+    double sn, cs;
+    sincospi(2.0 * f, &sn, &cs);
+    s = cadd(s, cmul(c2{cs, sn}, chi_r[r * ntarget + comp]));
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = cadd(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) chi_q[comp] = red[0];
+}
+
+// chibar(r) = exp(-2 pi i Q.r / dims) chibar(Q)
+__global__ void __launch_bounds__(256) k_q_project_adjoint(const c2 *__restrict__ chibar_q, c2 *__restrict__ chibar_r,
+                                                           int d0, int d1, int d2, int q0, int q1, int q2,
+                                                           int ntarget) {
+  const long nr = (long)d0 * d1 * d2;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nr * ntarget) return;
+  const long r = idx / ntarget;
+  const int comp = (int)(idx - r * ntarget);
+  const long i2 = r % d2, i1 = (r / d2) % d1, i0 = r / ((long)d1 * d2);
+  const double f = (double)((q0 * i0) % d0) / d0 + (double)((q1 * i1) % d1) / d1 + (double)((q2 * i2) % d2) / d2;
+  double sn, cs;
+  sincospi(2.0 * f, &sn, &cs);
+  chibar_r[idx] = cmul(c2{cs, -sn}, chibar_q[comp]);
+}
+
+// Static bubble adjoint: gbar_xy(tau_j, r) =
+//   conj(G(L-j, -r)) contracted with c_j chibar_{y b c x}(r) over (b, c)
+//   + c_{L-j} chibar_{a x y d}(-r) over (a, d)  [with G_bc / G_da of (L-j, -r)].
+template <int N>
+__global__ void __launch_bounds__(256) k_bubble_static_adjoint(int ntau, int d0, int d1, int d2,
+                                                               const c2 *__restrict__ gtr,
+                                                               const c2 *__restrict__ chibar_r,
+                                                               const double *__restrict__ cw, c2 *__restrict__ gbar) {
+  constexpr int n2 = N * N, n4 = n2 * n2;
+  const long nr = (long)d0 * d1 * d2;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)ntau * nr) return;
+  const long j = idx / nr, r = idx - j * nr;
+  const int L = ntau - 1;
+  const long mr = minus_r(r, d0, d1, d2);
+  const c2 *gm = gtr + ((long)(L - j) * nr + mr) * n2;  // G(tau_{L-j}, -r)
+  c2 g[n2];
+#pragma unroll
+  for (int i = 0; i < n2; ++i) g[i] = cconj(gm[i]);
+  const c2 *cb = chibar_r + r * n4, *cbm = chibar_r + mr * n4;
+  const double cj = cw[j], cl = cw[L - j];
+  c2 *o = gbar + idx * n2;
+#pragma unroll
+  for (int x = 0; x < N; ++x)
+#pragma unroll
+    for (int y = 0; y < N; ++y) {
+      c2 s = c2{0.0, 0.0};
+#pragma unroll
+      for (int b = 0; b < N; ++b)
+#pragma unroll
+        for (int c = 0; c < N; ++c) {
+          // first factor: x = d, y = a of chi_{y b c x}(r), partner G_bc(L-j, -r)
+          s = cadd(s, cscale(cmul(cb[((y * N + b) * N + c) * N + x], g[b * N + c]), cj));
+          // second factor: (b, c) = (x, y) of chi_{a x y d}(-r), partner G_da(L-j, -r) with (a, d) = (b, c)
+          s = cadd(s, cscale(cmul(cbm[((b * N + x) * N + y) * N + c], g[c * N + b]), cl));
+        }
+      o[x * N + y] = s;
+    }
+}
+
+// Dyson adjoint: Mbar = -G^H Gbar G^H for every (w, k); ebar_k = -sum_w Mbar,
+// mubar = Re sum tr Mbar.  Thread (k, chunk) sums its chunk of frequencies into
+// part_e[chunk][k][n^2]; per-block sums of Re tr Mbar go to mu_part.
+template <int N>
+__global__ void __launch_bounds__(256) k_dyson_adjoint(int nw, long nk, int wchunk, const c2 *__restrict__ g,
+                                                       const c2 *__restrict__ gbar, c2 *__restrict__ part_e,
+                                                       double *__restrict__ mu_part) {
+  __shared__ double red[256];
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int w0 = blockIdx.y * wchunk;
+  const int w1 = (w0 + wchunk) < nw ? (w0 + wchunk) : nw;
+  double mu = 0.0;
+  if (k < nk) {
+    c2 e[N][N];
+#pragma unroll
+    for (int a = 0; a < N; ++a)
+#pragma unroll
+      for (int b = 0; b < N; ++b) e[a][b] = c2{0.0, 0.0};
+    for (int w = w0; w < w1; ++w) {
+      const c2 *G = g + ((long)w * nk + k) * N * N;
+      const c2 *B = gbar + ((long)w * nk + k) * N * N;
+      c2 Gv[N][N], Bv[N][N];
+#pragma unroll
+      for (int a = 0; a < N; ++a)
+#pragma unroll
+        for (int b = 0; b < N; ++b) { Gv[a][b] = G[a * N + b]; Bv[a][b] = B[a * N + b]; }
+      // T = G^H Gbar ; S = T G^H = -Mbar
+      c2 T[N][N];
+#pragma unroll
+      for (int a = 0; a < N; ++a)
+#pragma unroll
+        for (int b = 0; b < N; ++b) {
+          c2 s = c2{0.0, 0.0};
+#pragma unroll
+          for (int l = 0; l < N; ++l) s = cadd(s, cmul(cconj(Gv[l][a]), Bv[l][b]));
+          T[a][b] = s;
+        }
+#pragma unroll
+      for (int a = 0; a < N; ++a)
+#pragma unroll
+        for (int b = 0; b < N; ++b) {
+          c2 s = c2{0.0, 0.0};
+#pragma unroll
+          for (int l = 0; l < N; ++l) s = cadd(s, cmul(T[a][l], cconj(Gv[b][l])));
+          e[a][b] = cadd(e[a][b], s);
+          if (a == b) mu -= s.x;
+        }
+    }
+    c2 *o = part_e + ((long)blockIdx.y * nk + k) * N * N;
+#pragma unroll
+    for (int a = 0; a < N; ++a)
+#pragma unroll
+      for (int b = 0; b < N; ++b) o[a * N + b] = e[a][b];
+  }
+  red[threadIdx.x] = mu;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) mu_part[(long)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
+// out[i] = sum_c part[c * stride + i], i < stride (fixed order: deterministic)
+__global__ void __launch_bounds__(256) k_sum_chunks(const c2 *__restrict__ part, int nchunk, long stride,
+                                                    c2 *__restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= stride) return;
+  c2 s = c2{0.0, 0.0};
+  for (int c = 0; c < nchunk; ++c) s = cadd(s, part[(long)c * stride + i]);
+  out[i] = s;
+}
+
+// Static bubble: chi0_abcd(r) = sum_j c_j G_da(tau_j, r) G_bc(tau_{L-j}, -r).
+// Thread (r, d, a) of tau chunk y accumulates the n^2 entries (b, c) into
+// part[y][r][abcd].
+template <int N>
+__global__ void __launch_bounds__(256) k_bubble_static(int ntau, int d0, int d1, int d2, int jchunk,
+                                                       const c2 *__restrict__ gtr, const double *__restrict__ cw,
+                                                       c2 *__restrict__ part) {
+  constexpr int n2 = N * N, n4 = n2 * n2;
+  const long nr = (long)d0 * d1 * d2;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nr * n2) return;
+  const long r = idx / n2;
+  const int da = (int)(idx - r * n2), d = da / N, a = da % N;
+  const int L = ntau - 1;
+  const long mr = minus_r(r, d0, d1, d2);
+  const int j0 = blockIdx.y * jchunk;
+  const int j1 = (j0 + jchunk) < ntau ? (j0 + jchunk) : ntau;
+  c2 acc[n2];
+#pragma unroll
+  for (int i = 0; i < n2; ++i) acc[i] = c2{0.0, 0.0};
+  for (int j = j0; j < j1; ++j) {
+    const c2 x = cscale(gtr[((long)j * nr + r) * n2 + da], cw[j]);
+    const c2 *gm = gtr + ((long)(L - j) * nr + mr) * n2;
+#pragma unroll
+    for (int bc = 0; bc < n2; ++bc) acc[bc] = cadd(acc[bc], cmul(x, gm[bc]));
+  }
+  c2 *o = part + ((long)blockIdx.y * nr + r) * n4;
+#pragma unroll
+  for (int b = 0; b < N; ++b)
+#pragma unroll
+    for (int c = 0; c < N; ++c) o[((a * N + b) * N + c) * N + d] = acc[b * N + c];
+}
+
+// sum of the per-block partials (one block, deterministic)
+__global__ void __launch_bounds__(256) k_sum_partials(const double *__restrict__ part, int n, double *__restrict__ out) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += part[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+// RPA at single points, forward and reverse, one thread per point (n <= 2):
+// X = M(chi0), A = 1 - X U, chi = A^-1 X; with chibar = M(dL/dchi):
+// Xbar = A^-H chibar (1 + U chi)^H, Ubar = X^H A^-H chibar chi^H.
+template <int N>
+__global__ void __launch_bounds__(64) k_rpa_point(long npts, const c2 *__restrict__ chi0, const c2 *__restrict__ U,
+                                                  const c2 *__restrict__ chibar_t, c2 *__restrict__ chi,
+                                                  c2 *__restrict__ xbar_t, c2 *__restrict__ ubar_t, double *status) {
+  constexpr int M = N * N, n4 = M * M;
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npts) return;
+  auto tidx = [](int row, int col) {  // M[(a,b),(d,c)] <-> tensor abcd
+    const int a = row / N, b = row % N, d = col / N, c = col % N;
+    return ((a * N + b) * N + c) * N + d;
+  };
+  c2 X[M][M], MU[M][M], A[M][M], Ai[M][M];
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+#pragma unroll
+    for (int jj = 0; jj < M; ++jj) {
+      X[i][jj] = chi0[p * n4 + tidx(i, jj)];
+      MU[i][jj] = U[p * n4 + tidx(i, jj)];
+    }
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+#pragma unroll
+    for (int jj = 0; jj < M; ++jj) {
+      c2 s = (i == jj) ? c2{1.0, 0.0} : c2{0.0, 0.0};
+#pragma unroll
+      for (int l = 0; l < M; ++l) s = csub(s, cmul(X[i][l], MU[l][jj]));
+      A[i][jj] = s;
+      Ai[i][jj] = (i == jj) ? c2{1.0, 0.0} : c2{0.0, 0.0};
+    }
+  // Gauss-Jordan inverse with partial pivoting
+  bool singular = false;
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    int pv = k;
+    double best = cabs2(A[k][k]);
+#pragma unroll
+    for (int i = k + 1; i < M; ++i)
+      if (cabs2(A[i][k]) > best) { best = cabs2(A[i][k]); pv = i; }
+    if (best == 0.0) singular = true;
+#pragma unroll
+    for (int i = k + 1; i < M; ++i)
+      if (pv == i) {
+#pragma unroll
+        for (int jj = 0; jj < M; ++jj) {
+          c2 t = A[k][jj]; A[k][jj] = A[i][jj]; A[i][jj] = t;
+          t = Ai[k][jj]; Ai[k][jj] = Ai[i][jj]; Ai[i][jj] = t;
+        }
+      }
+    const c2 inv = cdiv(c2{1.0, 0.0}, A[k][k]);
+#pragma unroll
+    for (int jj = 0; jj < M; ++jj) { A[k][jj] = cmul(A[k][jj], inv); Ai[k][jj] = cmul(Ai[k][jj], inv); }
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+      if (i == k) continue;
+      const c2 f = A[i][k];
+#pragma unroll
+      for (int jj = 0; jj < M; ++jj) { A[i][jj] = csub(A[i][jj], cmul(f, A[k][jj])); Ai[i][jj] = csub(Ai[i][jj], cmul(f, Ai[k][jj])); }
+    }
+  }
+  if (singular && status) status[2] = 1.0;
+  c2 Y[M][M];  // chi
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+#pragma unroll
+    for (int jj = 0; jj < M; ++jj) {
+      c2 s = c2{0.0, 0.0};
+#pragma unroll
+      for (int l = 0; l < M; ++l) s = cadd(s, cmul(Ai[i][l], X[l][jj]));
+      Y[i][jj] = s;
+      chi[p * n4 + tidx(i, jj)] = s;
+    }
+  if (!chibar_t) return;
+  c2 Cb[M][M], Z[M][M], W[M][M];
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+#pragma unroll
+    for (int jj = 0; jj < M; ++jj) Cb[i][jj] = chibar_t[p * n4 + tidx(i, jj)];
+  // Z = A^-H Cb
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+#pragma unroll
+    for (int jj = 0; jj < M; ++jj) {
+      c2 s = c2{0.0, 0.0};
+#pragma unroll
+      for (int l = 0; l < M; ++l) s = cadd(s, cmul(cconj(Ai[l][i]), Cb[l][jj]));
+      Z[i][jj] = s;
+    }
+  // W = 1 + U chi
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+#pragma unroll
+    for (int jj = 0; jj < M; ++jj) {
+      c2 s = (i == jj) ? c2{1.0, 0.0} : c2{0.0, 0.0};
+#pragma unroll
+      for (int l = 0; l < M; ++l) s = cadd(s, cmul(MU[i][l], Y[l][jj]));
+      W[i][jj] = s;
+    }
+  // Xbar = Z W^H ; Ubar = X^H Z chi^H
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+#pragma unroll
+    for (int jj = 0; jj < M; ++jj) {
+      c2 s = c2{0.0, 0.0};
+#pragma unroll
+      for (int l = 0; l < M; ++l) s = cadd(s, cmul(Z[i][l], cconj(W[jj][l])));
+      xbar_t[p * n4 + tidx(i, jj)] = s;
+    }
+  c2 T[M][M];
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+#pragma unroll
+    for (int jj = 0; jj < M; ++jj) {
+      c2 s = c2{0.0, 0.0};
+#pragma unroll
+      for (int l = 0; l < M; ++l) s = cadd(s, cmul(cconj(X[l][i]), Z[l][jj]));
+      T[i][jj] = s;
+    }
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+#pragma unroll
+    for (int jj = 0; jj < M; ++jj) {
+      c2 s = c2{0.0, 0.0};
+#pragma unroll
+      for (int l = 0; l < M; ++l) s = cadd(s, cmul(T[i][l], cconj(Y[jj][l])));
+      ubar_t[p * n4 + tidx(i, jj)] = s;
+    }
+}
+
+// ---- launchers ---------------------------------------------------------
+hipError_t launch_q_project(hipStream_t s, const c2 *chi_r, c2 *chi_q, const int dims[3], const int q[3], int ntarget) {
+  hipLaunchKernelGGL(k_q_project, dim3(ntarget), dim3(256), 0, s, chi_r, chi_q, dims[0], dims[1], dims[2], q[0], q[1],
+                     q[2], ntarget);
+  return hipGetLastError();
+}
+
+hipError_t launch_q_project_adjoint(hipStream_t s, const c2 *chibar_q, c2 *chibar_r, const int dims[3], const int q[3],
+                                    int ntarget) {
+  const long total = (long)dims[0] * dims[1] * dims[2] * ntarget;
+  hipLaunchKernelGGL(k_q_project_adjoint, dim3((total + 255) / 256), dim3(256), 0, s, chibar_q, chibar_r, dims[0],
+                     dims[1], dims[2], q[0], q[1], q[2], ntarget);
+  return hipGetLastError();
+}
+
+hipError_t launch_bubble_static_adjoint(hipStream_t s, int ntau, const int dims[3], int n, const c2 *gtr,
+                                        const c2 *chibar_r, const double *cw, c2 *gbar) {
+  const long total = (long)ntau * dims[0] * dims[1] * dims[2];
+  const long nb = (total + 255) / 256;
+  switch (n) {
+    case 1: hipLaunchKernelGGL(k_bubble_static_adjoint<1>, dim3(nb), dim3(256), 0, s, ntau, dims[0], dims[1], dims[2], gtr, chibar_r, cw, gbar); break;
+    case 2: hipLaunchKernelGGL(k_bubble_static_adjoint<2>, dim3(nb), dim3(256), 0, s, ntau, dims[0], dims[1], dims[2], gtr, chibar_r, cw, gbar); break;
+    case 3: hipLaunchKernelGGL(k_bubble_static_adjoint<3>, dim3(nb), dim3(256), 0, s, ntau, dims[0], dims[1], dims[2], gtr, chibar_r, cw, gbar); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_dyson_adjoint(hipStream_t s, int nw, long nk, int n, const c2 *g, const c2 *gbar, c2 *ebar,
+                                c2 *part_e, double *mu_part, double *mubar, int nchunk) {
+  const long nb = (nk + 255) / 256;
+  const int wchunk = (nw + nchunk - 1) / nchunk;
+  const dim3 grid(nb, nchunk);
+  switch (n) {
+    case 1: hipLaunchKernelGGL(k_dyson_adjoint<1>, grid, dim3(256), 0, s, nw, nk, wchunk, g, gbar, part_e, mu_part); break;
+    case 2: hipLaunchKernelGGL(k_dyson_adjoint<2>, grid, dim3(256), 0, s, nw, nk, wchunk, g, gbar, part_e, mu_part); break;
+    case 3: hipLaunchKernelGGL(k_dyson_adjoint<3>, grid, dim3(256), 0, s, nw, nk, wchunk, g, gbar, part_e, mu_part); break;
+    default: return hipErrorInvalidValue;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const long stride = nk * n * n;
+  hipLaunchKernelGGL(k_sum_chunks, dim3((stride + 255) / 256), dim3(256), 0, s, part_e, nchunk, stride, ebar);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, mu_part, (int)(nb * nchunk), mubar);
+  return hipGetLastError();
+}
+
+hipError_t launch_bubble_static(hipStream_t s, int ntau, const int dims[3], int n, const c2 *gtr, const double *cw,
+                                c2 *part, int nchunk, c2 *chi_r) {
+  const long nr = (long)dims[0] * dims[1] * dims[2];
+  const long nb = (nr * n * n + 255) / 256;
+  const int jchunk = (ntau + nchunk - 1) / nchunk;
+  const dim3 grid(nb, nchunk);
+  switch (n) {
+    case 1: hipLaunchKernelGGL(k_bubble_static<1>, grid, dim3(256), 0, s, ntau, dims[0], dims[1], dims[2], jchunk, gtr, cw, part); break;
+    case 2: hipLaunchKernelGGL(k_bubble_static<2>, grid, dim3(256), 0, s, ntau, dims[0], dims[1], dims[2], jchunk, gtr, cw, part); break;
+    case 3: hipLaunchKernelGGL(k_bubble_static<3>, grid, dim3(256), 0, s, ntau, dims[0], dims[1], dims[2], jchunk, gtr, cw, part); break;
+    default: return hipErrorInvalidValue;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const long stride = nr * n * n * n * n;
+  hipLaunchKernelGGL(k_sum_chunks, dim3((stride + 255) / 256), dim3(256), 0, s, part, nchunk, stride, chi_r);
+  return hipGetLastError();
+}
+
+hipError_t launch_rpa_point(hipStream_t s, long npts, int n, const c2 *chi0, const c2 *U, const c2 *chibar, c2 *chi,
+                            c2 *xbar, c2 *ubar, Status st) {
+  const long nb = (npts + 63) / 64;
+  switch (n) {
+    case 1: hipLaunchKernelGGL(k_rpa_point<1>, dim3(nb), dim3(64), 0, s, npts, chi0, U, chibar, chi, xbar, ubar, st.d); break;
+    case 2: hipLaunchKernelGGL(k_rpa_point<2>, dim3(nb), dim3(64), 0, s, npts, chi0, U, chibar, chi, xbar, ubar, st.d); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace tprf

@@ -59,6 +59,10 @@ struct tprf_ctx {
   tprf::Vinv8 vinv;
   std::map<std::tuple<double, int, int, int, int, int>, BubbleEntry> bubbles;
   std::map<std::tuple<int, double, int>, c2 *> pole_inv;
+  std::map<std::tuple<double, int>, double *> static_w;  // nu = 0 weights c_j
+  std::map<std::tuple<int, double, int, int>, c2 *> dtail;  // out-of-mesh pole model in tau
+  void *work = nullptr;                                   // grow-only scratch
+  size_t work_bytes = 0;
 };
 
 namespace {
@@ -291,6 +295,66 @@ int get_pole_inv(tprf_ctx *ctx, int statistic, double beta, int n_max, int nmesh
   return TPRF_OK;
 }
 
+// D_q(tau_j) = psi_q(tau_j) - (1/beta) sum_{n in mesh} exp(-i w_n tau_j) / (i w_n - b_q),
+// j = 0..L-1: what the pole model adds in tau beyond its own in-mesh transform.
+// The omega -> tau map adds sum_q a_q D_q(tau_j) to its FFT part, so its adjoint
+// needs a_q-bar = sum_j conj(D_q(tau_j)) gbar_j.  D is small (the pole tail beyond
+// the mesh) while both of its terms are O(1); it is formed here in long double
+// so that the adjoint does not take the difference of two large sums.
+int get_dtail(tprf_ctx *ctx, int statistic, double beta, int n_max, int L, const c2 **out) {
+  auto key = std::make_tuple(statistic, beta, n_max, L);
+  auto it = ctx->dtail.find(key);
+  if (it != ctx->dtail.end()) { *out = it->second; return TPRF_OK; }
+  typedef std::complex<long double> lc;
+  const long double bf[3] = {0.0L, 1.0L, -1.0L}, bb[3] = {-0.5L, -1.0L, 1.0L};
+  const long double *b = statistic == TPRF_FERMION ? bf : bb;
+  const int first = statistic == TPRF_FERMION ? -n_max : -(n_max - 1);
+  const int nmesh = statistic == TPRF_FERMION ? 2 * n_max : 2 * n_max - 1;
+  const long double lb = beta;
+  std::vector<c2> d(3 * (size_t)L);
+  std::vector<lc> pole(3 * (size_t)nmesh);
+  for (int q = 0; q < 3; ++q)
+    for (int i = 0; i < nmesh; ++i) {
+      const long n = first + i;
+      const long double w = statistic == TPRF_FERMION ? kPiL * (2 * n + 1) / lb : 2 * kPiL * n / lb;
+      pole[(size_t)q * nmesh + i] = 1.0L / lc(-b[q], w);
+    }
+  std::vector<lc> ph(2 * (size_t)L);  // exp(-i pi k / L), k < 2L
+  for (long k = 0; k < 2L * L; ++k) {
+    const long double ang = kPiL * (long double)k / L;
+    ph[k] = lc(cosl(ang), -sinl(ang));
+  }
+  for (int j = 0; j < L; ++j) {
+    const long double tau = lb * j / L;
+    lc s[3] = {0, 0, 0};
+    for (int i = 0; i < nmesh; ++i) {
+      const long n = first + i;
+      // w_n tau_j = pi (2n+1) j / L (fermions) or 2 pi n j / L (bosons), reduced exactly mod 2 L
+      long num = statistic == TPRF_FERMION ? (2 * n + 1) * (long)j : 2 * n * (long)j;
+      num %= 2L * L;
+      if (num < 0) num += 2L * L;
+      const lc e = ph[num];
+      for (int q = 0; q < 3; ++q) s[q] += e * pole[(size_t)q * nmesh + i];
+    }
+    for (int q = 0; q < 3; ++q) {
+      long double psi;
+      const long double bq = b[q];
+      if (statistic == TPRF_FERMION)
+        psi = bq >= 0 ? -expl(-bq * tau) / (1 + expl(-lb * bq)) : -expl(bq * (lb - tau)) / (1 + expl(lb * bq));
+      else
+        psi = bq >= 0 ? expl(-bq * tau) / (expl(-lb * bq) - 1) : expl(bq * (lb - tau)) / (1 - expl(bq * lb));
+      const lc v = lc(psi, 0) - s[q] / lb;
+      d[(size_t)q * L + j] = c2{(double)v.real(), (double)v.imag()};
+    }
+  }
+  c2 *dd;
+  int st = upload(ctx, &dd, d);
+  if (st) return st;
+  ctx->dtail.emplace(key, dd);
+  *out = dd;
+  return TPRF_OK;
+}
+
 // Tables of the pruned bubble transform for (beta, L, nb, dims) and inner DFT
 // length P (see bubble.hip).  Twiddles and the model transforms Phi_q(m) are
 // evaluated in long double from exactly reduced angles.
@@ -364,6 +428,73 @@ int get_bubble(tprf_ctx *ctx, double beta, int L, int nb, const int dims[3], int
   return TPRF_OK;
 }
 
+// Context scratch of at least `bytes` (grow-only; the old buffer is freed
+// after the stream drains).
+int get_work(tprf_ctx *ctx, size_t bytes, void **out) {
+  if (ctx->work_bytes < bytes) {
+    if (ctx->work) {
+      TPRF_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+      (void)hipFree(ctx->work);
+      ctx->work = nullptr;
+      ctx->work_bytes = 0;
+    }
+    if (hipMalloc(&ctx->work, bytes) != hipSuccess) return fail(ctx, TPRF_ERR_NOMEM, "scratch allocation failed");
+    ctx->work_bytes = bytes;
+  }
+  *out = ctx->work;
+  return TPRF_OK;
+}
+
+// Weights c_j, j = 0..L, of the bosonic tau -> nu transform at nu = 0
+// (fourier_matsubara.cpp:42-184 restated by t2w_core in kernels.hip):
+//   chi(0) = (beta/L) sum_{j<L} (x_j - sum_q a_q psi_q(tau_j)) + sum_q a_q / (-b_q)
+// with a = A m the pole coefficients of the boundary moments
+//   m1 = x_L - x_0,  m2 = gl0 - gr0,  m3 = -(2/dtau)(gl1 + gr1),
+//   gl_i = sum_m V_{i,m} (x_m - x_0)/(m dtau),  gr_i = sum_m V_{i,m} (x_L - x_{L-m})/(m dtau),
+// all linear in x with real coefficients, so chi(0) = sum_j c_j x_j.
+int get_static_weights(tprf_ctx *ctx, double beta, int L, const double **out) {
+  auto key = std::make_tuple(beta, L);
+  auto it = ctx->static_w.find(key);
+  if (it != ctx->static_w.end()) { *out = it->second; return TPRF_OK; }
+  const double bb[3] = {-0.5, -1.0, 1.0};
+  // a_q = sum_p A[q][p] m_p (bosonic three-pole model)
+  const long double A[3][3] = {{4.0L / 3, 0, -4.0L / 3}, {-0.5L, -0.5L, 1}, {1.0L / 6, 0.5L, 1.0L / 3}};
+  const long double fact = (long double)beta / L;
+  const double dtau = beta / (double)L;
+  long double phi[3];
+  for (int q = 0; q < 3; ++q) {
+    long double sp = 0;
+    for (int j = 0; j < L; ++j) sp += one_boson(bb[q], (double)j * dtau, beta);
+    phi[q] = -fact * sp + 1.0L / (-(long double)bb[q]);
+  }
+  long double kap[3];
+  for (int p = 0; p < 3; ++p) kap[p] = phi[0] * A[0][p] + phi[1] * A[1][p] + phi[2] * A[2][p];
+  std::vector<long double> c(L + 1, 0.0L);
+  for (int j = 0; j < L; ++j) c[j] = fact;
+  // m1
+  c[0] -= kap[0];
+  c[L] += kap[0];
+  // m2, m3 through the boundary derivatives
+  for (int m = 1; m <= 8; ++m) {
+    const long double tm = (long double)m * dtau;
+    const long double v0 = ctx->vinv.v[0 * 8 + m - 1] / tm, v1 = ctx->vinv.v[1 * 8 + m - 1] / tm;
+    const long double k2 = kap[1], k3 = kap[2] * (-2.0L / dtau);
+    // gl: +x_m - x_0 ; gr: +x_L - x_{L-m} ; m2 = gl0 - gr0 ; m3 = -(2/dtau)(gl1 + gr1)
+    c[m] += k2 * v0 + k3 * v1;
+    c[0] -= k2 * v0 + k3 * v1;
+    c[L] += -k2 * v0 + k3 * v1;
+    c[L - m] -= -k2 * v0 + k3 * v1;
+  }
+  std::vector<double> cd(L + 1);
+  for (int j = 0; j <= L; ++j) cd[j] = (double)c[j];
+  double *d;
+  int st = upload(ctx, &d, cd);
+  if (st) return st;
+  ctx->static_w.emplace(key, d);
+  *out = d;
+  return TPRF_OK;
+}
+
 bool dims_ok(const int dims[3]) {
   if (!dims) return false;
   for (int d = 0; d < 3; ++d)
@@ -376,15 +507,16 @@ int check_launch(tprf_ctx *ctx, hipError_t e, const char *what) {
   return TPRF_OK;
 }
 
-int lattice_fft(tprf_ctx *ctx, long nw, const int dims[3], long ntarget, const c2 *in, c2 *out, int sign) {
+int lattice_fft(tprf_ctx *ctx, long nw, const int dims[3], long ntarget, const c2 *in, c2 *out, int sign,
+                bool scaled) {
   if (!ctx) return TPRF_ERR_ARG;
   if (nw < 1 || ntarget < 1 || !dims_ok(dims) || !in || !out) return fail(ctx, TPRF_ERR_ARG, "lattice fft: bad shape");
   const long nk = (long)dims[0] * dims[1] * dims[2];
-  const double scale = sign < 0 ? 1.0 / (double)nk : 1.0;
+  const double scale = scaled ? 1.0 / (double)nk : 1.0;
   int last = -1;
   for (int d = 0; d < 3; ++d)
     if (dims[d] > 1) last = d;
-  if (last < 0) {  // a single k-point: copy (scale is 1)
+  if (last < 0) {  // a single k-point: copy (N_k = 1, so the scale is 1)
     TPRF_HIP_TRY(ctx, hipMemcpyAsync(out, in, sizeof(c2) * nw * ntarget, hipMemcpyDeviceToDevice, ctx->stream),
                  "hipMemcpyAsync");
     return TPRF_OK;
@@ -446,6 +578,9 @@ int tprf_ctx_destroy(tprf_ctx *ctx) {
   for (auto &kv : ctx->fits) { (void)hipFree(kv.second.idx); (void)hipFree(kv.second.pinv); (void)hipFree(kv.second.vander); }
   for (auto &kv : ctx->mats) { (void)hipFree(kv.second.psi); (void)hipFree(kv.second.phase); }
   for (auto &kv : ctx->pole_inv) (void)hipFree(kv.second);
+  for (auto &kv : ctx->static_w) (void)hipFree(kv.second);
+  for (auto &kv : ctx->dtail) (void)hipFree(kv.second);
+  if (ctx->work) (void)hipFree(ctx->work);
   for (auto &kv : ctx->bubbles) {
     (void)hipFree(kv.second.tw1); (void)hipFree(kv.second.twp); (void)hipFree(kv.second.phi);
     (void)hipFree(kv.second.pinv_nu); (void)hipFree(kv.second.rcanon);
@@ -504,12 +639,12 @@ int tprf_lattice_dyson_g_wk(tprf_ctx *ctx, double beta, int n_iw, double mu, lon
 
 int tprf_fourier_wk_to_wr(tprf_ctx *ctx, long nw, const int dims[3], long ntarget, const tprf_c128 *in,
                           tprf_c128 *out) {
-  return lattice_fft(ctx, nw, dims, ntarget, (const c2 *)in, (c2 *)out, -1);
+  return lattice_fft(ctx, nw, dims, ntarget, (const c2 *)in, (c2 *)out, -1, true);
 }
 
 int tprf_fourier_wr_to_wk(tprf_ctx *ctx, long nw, const int dims[3], long ntarget, const tprf_c128 *in,
                           tprf_c128 *out) {
-  return lattice_fft(ctx, nw, dims, ntarget, (const c2 *)in, (c2 *)out, +1);
+  return lattice_fft(ctx, nw, dims, ntarget, (const c2 *)in, (c2 *)out, +1, false);
 }
 
 int tprf_fourier_wr_to_tr(tprf_ctx *ctx, int statistic, double beta, int n_max, int ntau, long ncol,
@@ -542,7 +677,7 @@ int tprf_fourier_wr_to_tr(tprf_ctx *ctx, int statistic, double beta, int n_max, 
     if ((st = get_mats(ctx, statistic, beta, L, &mt))) return st;
     const c2 *pinv;
     if ((st = get_pole_inv(ctx, statistic, beta, n_max, nmesh, &pinv))) return st;
-    tprf::W2tDev wd{M, L / M, pL->tw, pM->tw, pinv};
+    tprf::W2tDev wd{M, L / M, pL->tw, pM->tw, pinv, nullptr};
     return check_launch(ctx,
                         tprf::launch_w2t_pruned(ctx->stream, statistic, beta, n_max, ncol, (const c2 *)in, (c2 *)out,
                                                 (const c2 *)moments, fit, *mt, wd, tprf::Status{ctx->d_status}),
@@ -653,6 +788,155 @@ int tprf_solve_rpa_PH(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *chi0,
                       tprf::launch_rpa(ctx->stream, npts, norb, (const c2 *)chi0, (const c2 *)U, (c2 *)chi,
                                        tprf::Status{ctx->d_status}),
                       "rpa kernel");
+}
+
+// ---- differentiation -----------------------------------------------------
+
+int tprf_lattice_dyson_g_wk_adjoint(tprf_ctx *ctx, int n_iw, long nk, int norb, const tprf_c128 *g_wk,
+                                    const tprf_c128 *gbar_wk, tprf_c128 *ebar_k, double *mubar) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (n_iw < 1 || nk < 1 || !g_wk || !gbar_wk || !ebar_k || !mubar)
+    return fail(ctx, TPRF_ERR_ARG, "lattice_dyson_g_wk_adjoint: bad argument");
+  if (norb < 1 || norb > 3) return fail(ctx, TPRF_ERR_UNSUPPORTED, "lattice_dyson_g_wk_adjoint: norb must be 1..3");
+  const int nw = 2 * n_iw;
+  // frequency chunks: enough (k, chunk) threads to fill the chip
+  int nchunk = (int)((1L << 20) / (nk > 0 ? nk : 1));
+  if (nchunk < 1) nchunk = 1;
+  if (nchunk > nw) nchunk = nw;
+  if (nchunk > 256) nchunk = 256;
+  const long nb = (nk + 255) / 256;
+  const size_t part_bytes = sizeof(c2) * (size_t)nchunk * nk * norb * norb;
+  const size_t mu_bytes = sizeof(double) * (size_t)nchunk * nb;
+  void *w;
+  int st = get_work(ctx, part_bytes + mu_bytes, &w);
+  if (st) return st;
+  c2 *part_e = (c2 *)w;
+  double *mu_part = (double *)((char *)w + part_bytes);
+  return check_launch(ctx,
+                      tprf::launch_dyson_adjoint(ctx->stream, nw, nk, norb, (const c2 *)g_wk, (const c2 *)gbar_wk,
+                                                 (c2 *)ebar_k, part_e, mu_part, mubar, nchunk),
+                      "dyson adjoint kernel");
+}
+
+int tprf_fourier_wk_to_wr_adjoint(tprf_ctx *ctx, long nw, const int dims[3], long ntarget, const tprf_c128 *gbar_wr,
+                                  tprf_c128 *gbar_wk) {
+  return lattice_fft(ctx, nw, dims, ntarget, (const c2 *)gbar_wr, (c2 *)gbar_wk, +1, true);
+}
+
+int tprf_fourier_wr_to_tr_adjoint(tprf_ctx *ctx, int statistic, double beta, int n_max, int ntau, long ncol,
+                                  const tprf_c128 *gbar_tr, tprf_c128 *gbar_wr, int fitted) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if ((statistic != TPRF_FERMION && statistic != TPRF_BOSON) || !(beta > 0) || n_max < 1 || ncol < 1 || !gbar_tr ||
+      !gbar_wr)
+    return fail(ctx, TPRF_ERR_ARG, "fourier_wr_to_tr_adjoint: bad argument");
+  if (ntau <= 0) ntau = 6 * n_max + 1;
+  const int L = ntau - 1;
+  if (L < 2 * n_max)
+    return fail(ctx, TPRF_ERR_MESH_TOO_SHORT,
+                "Inverse Fourier: The time mesh mush be at least twice as long as the freq mesh");
+  const int nmesh = statistic == TPRF_FERMION ? 2 * n_max : 2 * n_max - 1;
+  int M = 64;
+  while (M < 2 * n_max) M *= 2;
+  if (L % M != 0 || !tprf::w2t_pruned_supported(M) || L / M > 64)
+    return fail(ctx, TPRF_ERR_UNSUPPORTED, "fourier_wr_to_tr_adjoint: ntau-1 must be a multiple of M in 64..1024");
+  int st;
+  tprf::TailFitDev fit{};
+  if (fitted) {
+    const tprf::TailFitDev *f;
+    if ((st = get_tail(ctx, statistic, beta, n_max, &f))) return st;
+    fit = *f;
+  }
+  const tprf::FftDev *pL, *pM;
+  const tprf::MatsDev *mt;
+  const c2 *pinv;
+  if ((st = get_fft(ctx, L, &pL))) return st;
+  if ((st = get_fft(ctx, M, &pM))) return st;
+  if ((st = get_mats(ctx, statistic, beta, L, &mt))) return st;
+  if ((st = get_pole_inv(ctx, statistic, beta, n_max, nmesh, &pinv))) return st;
+  const c2 *dtail;
+  if ((st = get_dtail(ctx, statistic, beta, n_max, L, &dtail))) return st;
+  tprf::W2tDev wd{M, L / M, pL->tw, pM->tw, pinv, dtail};
+  return check_launch(ctx,
+                      tprf::launch_w2t_adjoint(ctx->stream, statistic, beta, n_max, ncol, (const c2 *)gbar_tr,
+                                               (c2 *)gbar_wr, fitted ? 1 : 0, fit, *mt, wd),
+                      "w2t adjoint kernel");
+}
+
+int tprf_chi0_static_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb,
+                                 const tprf_c128 *g_tr, tprf_c128 *chi0_r) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (!(beta > 0) || ntau < 18 || !dims_ok(dims) || !g_tr || !chi0_r)
+    return fail(ctx, TPRF_ERR_ARG, "chi0_static_from_grt_PH: bad argument");
+  if (norb < 1 || norb > 3) return fail(ctx, TPRF_ERR_UNSUPPORTED, "chi0_static_from_grt_PH: norb must be 1..3");
+  const int L = ntau - 1;
+  const double *cw;
+  int st;
+  if ((st = get_static_weights(ctx, beta, L, &cw))) return st;
+  const long nr = (long)dims[0] * dims[1] * dims[2];
+  const long n4 = (long)norb * norb * norb * norb;
+  // tau chunks: enough (r, chunk) threads to fill the chip
+  int nchunk = (int)((1L << 20) / (nr * norb * norb));
+  if (nchunk < 1) nchunk = 1;
+  if (nchunk > 64) nchunk = 64;
+  void *w;
+  if ((st = get_work(ctx, sizeof(c2) * (size_t)nchunk * nr * n4, &w))) return st;
+  return check_launch(ctx,
+                      tprf::launch_bubble_static(ctx->stream, ntau, dims, norb, (const c2 *)g_tr, cw, (c2 *)w, nchunk,
+                                                 (c2 *)chi0_r),
+                      "static bubble kernel");
+}
+
+int tprf_chi0_static_from_grt_PH_adjoint(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb,
+                                         const tprf_c128 *g_tr, const tprf_c128 *chibar_r, tprf_c128 *gbar_tr) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (!(beta > 0) || ntau < 18 || !dims_ok(dims) || !g_tr || !chibar_r || !gbar_tr)
+    return fail(ctx, TPRF_ERR_ARG, "chi0_static_from_grt_PH_adjoint: bad argument");
+  if (norb < 1 || norb > 3)
+    return fail(ctx, TPRF_ERR_UNSUPPORTED, "chi0_static_from_grt_PH_adjoint: norb must be 1..3");
+  const double *cw;
+  int st;
+  if ((st = get_static_weights(ctx, beta, ntau - 1, &cw))) return st;
+  return check_launch(ctx,
+                      tprf::launch_bubble_static_adjoint(ctx->stream, ntau, dims, norb, (const c2 *)g_tr,
+                                                         (const c2 *)chibar_r, cw, (c2 *)gbar_tr),
+                      "static bubble adjoint kernel");
+}
+
+int tprf_fourier_r_to_q(tprf_ctx *ctx, const int dims[3], const int q[3], long ntarget, const tprf_c128 *in_r,
+                        tprf_c128 *out_q) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (!dims_ok(dims) || !q || ntarget < 1 || ntarget > (1L << 20) || !in_r || !out_q)
+    return fail(ctx, TPRF_ERR_ARG, "fourier_r_to_q: bad argument");
+  int qq[3];
+  for (int d = 0; d < 3; ++d) qq[d] = ((q[d] % dims[d]) + dims[d]) % dims[d];
+  return check_launch(ctx, tprf::launch_q_project(ctx->stream, (const c2 *)in_r, (c2 *)out_q, dims, qq, (int)ntarget),
+                      "r->q kernel");
+}
+
+int tprf_fourier_r_to_q_adjoint(tprf_ctx *ctx, const int dims[3], const int q[3], long ntarget,
+                                const tprf_c128 *gbar_q, tprf_c128 *gbar_r) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (!dims_ok(dims) || !q || ntarget < 1 || ntarget > (1L << 20) || !gbar_q || !gbar_r)
+    return fail(ctx, TPRF_ERR_ARG, "fourier_r_to_q_adjoint: bad argument");
+  int qq[3];
+  for (int d = 0; d < 3; ++d) qq[d] = ((q[d] % dims[d]) + dims[d]) % dims[d];
+  return check_launch(ctx,
+                      tprf::launch_q_project_adjoint(ctx->stream, (const c2 *)gbar_q, (c2 *)gbar_r, dims, qq,
+                                                     (int)ntarget),
+                      "r->q adjoint kernel");
+}
+
+int tprf_solve_rpa_PH_adjoint(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *chi0, const tprf_c128 *U,
+                              const tprf_c128 *chibar, tprf_c128 *chi, tprf_c128 *chi0bar, tprf_c128 *Ubar) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (npts < 1 || !chi0 || !U || !chi || (chibar && (!chi0bar || !Ubar)))
+    return fail(ctx, TPRF_ERR_ARG, "solve_rpa_PH_adjoint: bad argument");
+  if (norb < 1 || norb > 2) return fail(ctx, TPRF_ERR_UNSUPPORTED, "solve_rpa_PH_adjoint: norb must be 1..2");
+  return check_launch(ctx,
+                      tprf::launch_rpa_point(ctx->stream, npts, norb, (const c2 *)chi0, (const c2 *)U,
+                                             (const c2 *)chibar, (c2 *)chi, (c2 *)chi0bar, (c2 *)Ubar,
+                                             tprf::Status{ctx->d_status}),
+                      "rpa point kernel");
 }
 
 }  // extern "C"

@@ -82,6 +82,8 @@ struct W2tDev {
   const c2 *twL;         // [L] exp(-2 pi i k / L)
   const c2 *twM;         // [M] exp(-2 pi i k / M)
   const c2 *pole_inv;    // [3][nmesh] 1 / (i w_n - b_q)
+  const c2 *dtail;       // [3][L] psi_q(tau_j) - (1/beta) sum_{n in mesh} exp(-i w_n tau_j) / (i w_n - b_q)
+                         // (adjoint only; the part of the pole model beyond the mesh)
 };
 
 // Status words written by kernels (device memory, 4 doubles):
@@ -114,6 +116,23 @@ hipError_t launch_w2t_pruned(hipStream_t s, int statistic, double beta, int n_ma
                              const c2 *moments, const TailFitDev &fit, const MatsDev &mt, const W2tDev &wd,
                              Status st);
 bool w2t_pruned_supported(int M);
+// w2t.hip: transpose of the pruned w -> tau map (same M specialisations)
+hipError_t launch_w2t_adjoint(hipStream_t s, int statistic, double beta, int n_max, long ncol, const c2 *gbar_t,
+                              c2 *gbar_w, int fitted, const TailFitDev &fit, const MatsDev &mt, const W2tDev &wd);
+// adjoint.hip: reverse-mode kernels of the static bubble + RPA at one momentum
+hipError_t launch_q_project(hipStream_t s, const c2 *chi_r, c2 *chi_q, const int dims[3], const int q[3], int ntarget);
+hipError_t launch_q_project_adjoint(hipStream_t s, const c2 *chibar_q, c2 *chibar_r, const int dims[3], const int q[3],
+                                    int ntarget);
+hipError_t launch_bubble_static_adjoint(hipStream_t s, int ntau, const int dims[3], int n, const c2 *gtr,
+                                        const c2 *chibar_r, const double *cw, c2 *gbar);
+// part_e: nchunk * nk * n^2 values; mu_part: nchunk * ceil(nk / 256) doubles
+hipError_t launch_dyson_adjoint(hipStream_t s, int nw, long nk, int n, const c2 *g, const c2 *gbar, c2 *ebar,
+                                c2 *part_e, double *mu_part, double *mubar, int nchunk);
+// part: nchunk * nr * n^4 values
+hipError_t launch_bubble_static(hipStream_t s, int ntau, const int dims[3], int n, const c2 *gtr, const double *cw,
+                                c2 *part, int nchunk, c2 *chi_r);
+hipError_t launch_rpa_point(hipStream_t s, long npts, int n, const c2 *chi0, const c2 *U, const c2 *chibar, c2 *chi,
+                            c2 *xbar, c2 *ubar, Status st);
 // kernels.hip: single-pass 3-D lattice FFT; hipErrorNotSupported for other dims
 hipError_t launch_lattice3d(hipStream_t s, int sign, const c2 *in, c2 *out, long nw, const int dims[3], long ntarget,
                             double scale);

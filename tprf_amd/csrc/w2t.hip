@@ -228,3 +228,205 @@ hipError_t launch_w2t_pruned(hipStream_t s, int statistic, double beta, int n_ma
 }
 
 }  // namespace tprf
+
+namespace tprf {
+
+// ------------------------------------------------------------------------
+// Adjoint of the Matsubara -> tau transform above: given gbar(tau_j), j = 0..L,
+// computes gbar(i w_n) = T^H gbar(tau) for the complex-linear map T of
+// k_w2t_pruned (moments fitted, or given when `fitted` is 0).  Transposed
+// steps in reverse order: the endpoint G(tau_L) = pm (G(tau_0) + m1), the
+// phase, the length-M DFTs per residue t (conjugate sign), the pole model
+// (added in tau, subtracted in frequency: together sum_q a_q D_q(tau_j)), the
+// moment -> pole map and the least-squares fit.
+template <int STAT, int M>
+__global__ void __launch_bounds__(kW2tThreads, 2) k_w2t_adjoint(double beta, int n_max, long ncol,
+                                                          const c2 *__restrict__ gbar_t, c2 *__restrict__ gbar_w,
+                                                          int fitted, TailFitDev fit, MatsDev mt, W2tDev wd) {
+  constexpr int NT = kW2tThreads;
+  constexpr int TPC = M / 8;
+  constexpr int C = NT / TPC;
+  using Plan = W2tPlan<M>;
+  __shared__ c2 buf[M * C];
+  __shared__ c2 s_tw[M];
+  __shared__ c2 s_part[NT][3];
+  __shared__ c2 s_mbar[C][3];
+
+  const int tid = threadIdx.x;
+  const int c = tid % C, j = tid / C;
+  const long tile = xcd_remap(blockIdx.x, gridDim.x);
+  const long c0 = tile * C;
+  const int cn = (ncol - c0) < C ? (int)(ncol - c0) : C;
+  const long col = c0 + (c < cn ? c : 0);
+  const int L = mt.L, Q = wd.Q;
+  const int nmesh = (STAT == 0) ? 2 * n_max : 2 * n_max - 1;
+  const int first = (STAT == 0) ? -n_max : -(n_max - 1);
+  const double pm = (STAT == 0) ? -1.0 : 1.0;
+  for (int t = tid; t < M; t += NT) s_tw[t] = wd.twM[t];
+  __syncthreads();
+
+  const c2 gL = gbar_t[(long)L * ncol + col];
+  c2 xbar[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) xbar[u] = c2{0.0, 0.0};
+  c2 ab0 = c2{0.0, 0.0}, ab1 = c2{0.0, 0.0}, ab2 = c2{0.0, 0.0};
+
+  for (int t = 0; t < Q; ++t) {
+    c2 v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int jt = Q * (j + TPC * r) + t;
+      c2 g = gbar_t[(long)jt * ncol + col];
+      if (jt == 0) g = cadd(g, cscale(gL, pm));
+      // pole model net of its in-mesh part: abar_q += conj(D_q(tau_j)) gbar_j
+      ab0 = cadd(ab0, cmul(cconj(wd.dtail[jt]), g));
+      ab1 = cadd(ab1, cmul(cconj(wd.dtail[L + jt]), g));
+      ab2 = cadd(ab2, cmul(cconj(wd.dtail[2 * L + jt]), g));
+      v[r] = (STAT == 0) ? cmul(cconj(mt.phase[jt]), g) : g;
+    }
+    int Ns = 1;
+#pragma unroll
+    for (int p = 0; p < Plan::npass; ++p) {
+      const int R = (p < Plan::n8) ? 8 : Plan::last;
+      if (R == 8) w2t_butterflies<M, 8, 1>(v, j, Ns, s_tw);
+      else if (R == 4) w2t_butterflies<M, 4, 1>(v, j, Ns, s_tw);
+      else w2t_butterflies<M, 2, 1>(v, j, Ns, s_tw);
+      const int B = 8 / R;
+      if (p + 1 < Plan::npass) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int uu = u / R, r = u % R;
+          const int b = j * B + uu;
+          const int k = b % Ns;
+          buf[((b - k) * R + k + r * Ns) * C + c] = v[u];
+        }
+        __syncthreads();
+        Ns *= R;
+        const int Rn = (p + 1 < Plan::n8) ? 8 : Plan::last;
+        const int Bn = 8 / Rn;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int uu = u / Rn, r = u % Rn;
+          v[u] = buf[(j * Bn + uu + r * (M / Rn)) * C + c];
+        }
+        __syncthreads();
+      } else {
+        // bins e -> mesh n = e or e - M, twiddled by exp(+2 pi i t n / L) and summed over t
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int uu = u / R, r = u % R;
+          const int b = j * B + uu;
+          const int k = b % Ns;
+          const int e = (b - k) * R + k + r * Ns;
+          const int n = e < M / 2 ? e : e - M;
+          c2 z = v[u];
+          if (t) {
+            int tn = t * n;
+            if (tn < 0) tn += L;
+            z = cmul(z, cconj(wd.twL[tn]));
+          }
+          xbar[u] = cadd(xbar[u], z);
+        }
+      }
+    }
+  }
+  // gbar(i w_n) = xbar_n / beta (the in-mesh part of the pole model is already in D_q)
+  const double ib = 1.0 / beta;
+  {
+    const int R = Plan::npass > Plan::n8 ? Plan::last : 8;
+    const int B = 8 / R;
+    int Ns = M / R;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int uu = u / R, r = u % R;
+      const int b = j * B + uu;
+      const int k = b % Ns;
+      const int e = (b - k) * R + k + r * Ns;
+      const int n = e < M / 2 ? e : e - M;
+      c2 g = c2{0.0, 0.0};
+      if (n >= first && n < first + nmesh) g = cscale(xbar[u], ib);
+      buf[e * C + c] = g;
+    }
+  }
+  s_part[tid][0] = ab0;
+  s_part[tid][1] = ab1;
+  s_part[tid][2] = ab2;
+  __syncthreads();
+  if (tid < cn) {
+    c2 a0 = c2{0, 0}, a1 = c2{0, 0}, a2 = c2{0, 0};
+    for (int jj = 0; jj < TPC; ++jj) {
+      const int src = jj * C + tid;
+      a0 = cadd(a0, s_part[src][0]);
+      a1 = cadd(a1, s_part[src][1]);
+      a2 = cadd(a2, s_part[src][2]);
+    }
+    c2 m1, m2, m3;
+    if (STAT == 0) {  // a = (m1 - m3, (m2 + m3)/2, (m3 - m2)/2)
+      m1 = a0;
+      m2 = cscale(csub(a1, a2), 0.5);
+      m3 = cadd(cscale(a0, -1.0), cscale(cadd(a1, a2), 0.5));
+    } else {  // a = (4/3 (m1 - m3), m3 - (m1 + m2)/2, m1/6 + m2/2 + m3/3)
+      m1 = cadd(cadd(cscale(a0, 4.0 / 3.0), cscale(a1, -0.5)), cscale(a2, 1.0 / 6.0));
+      m2 = cadd(cscale(a1, -0.5), cscale(a2, 0.5));
+      m3 = cadd(cadd(cscale(a0, -4.0 / 3.0), a1), cscale(a2, 1.0 / 3.0));
+    }
+    const c2 gLc = gbar_t[(long)L * ncol + c0 + tid];
+    m1 = cadd(m1, cscale(gLc, pm));
+    s_mbar[tid][0] = m1;
+    s_mbar[tid][1] = m2;
+    s_mbar[tid][2] = m3;
+  }
+  __syncthreads();
+  if (fitted) {
+    // moments m_p = w^p sum_f P[p][f] g_f  ->  gbar_f += w^p conj(P[p][f]) mbar_p
+    for (int t = tid; t < cn * fit.n_fit; t += NT) {
+      const int cc = t / fit.n_fit, f = t - cc * fit.n_fit;
+      int e = first + fit.idx[f];
+      if (e < 0) e += M;
+      c2 add = c2{0.0, 0.0};
+#pragma unroll
+      for (int p = 1; p <= 3; ++p)
+        add = cadd(add, cscale(cmul(cconj(fit.pinv[p * fit.n_fit + f]), s_mbar[cc][p - 1]), fit.wpow[p]));
+      buf[e * C + cc] = cadd(buf[e * C + cc], add);
+    }
+    __syncthreads();
+  }
+  for (int t = tid; t < nmesh * C; t += NT) {
+    const int i = t / C, cc = t - i * C;
+    if (cc >= cn) continue;
+    int e = first + i;
+    if (e < 0) e += M;
+    gbar_w[(long)i * ncol + c0 + cc] = buf[e * C + cc];
+  }
+}
+
+template <int STAT, int M>
+static hipError_t launch_adj_m(hipStream_t s, double beta, int n_max, long ncol, const c2 *gt, c2 *gw, int fitted,
+                               const TailFitDev &fit, const MatsDev &mt, const W2tDev &wd) {
+  constexpr int C = kW2tThreads / (M / 8);
+  const long nb = (ncol + C - 1) / C;
+  hipLaunchKernelGGL((k_w2t_adjoint<STAT, M>), dim3(nb), dim3(kW2tThreads), 0, s, beta, n_max, ncol, gt, gw, fitted,
+                     fit, mt, wd);
+  return hipGetLastError();
+}
+
+template <int STAT>
+static hipError_t launch_adj_stat(hipStream_t s, double beta, int n_max, long ncol, const c2 *gt, c2 *gw, int fitted,
+                                  const TailFitDev &fit, const MatsDev &mt, const W2tDev &wd) {
+  switch (wd.M) {
+    case 64: return launch_adj_m<STAT, 64>(s, beta, n_max, ncol, gt, gw, fitted, fit, mt, wd);
+    case 128: return launch_adj_m<STAT, 128>(s, beta, n_max, ncol, gt, gw, fitted, fit, mt, wd);
+    case 256: return launch_adj_m<STAT, 256>(s, beta, n_max, ncol, gt, gw, fitted, fit, mt, wd);
+    case 512: return launch_adj_m<STAT, 512>(s, beta, n_max, ncol, gt, gw, fitted, fit, mt, wd);
+    case 1024: return launch_adj_m<STAT, 1024>(s, beta, n_max, ncol, gt, gw, fitted, fit, mt, wd);
+    default: return hipErrorNotSupported;
+  }
+}
+
+hipError_t launch_w2t_adjoint(hipStream_t s, int statistic, double beta, int n_max, long ncol, const c2 *gbar_t,
+                              c2 *gbar_w, int fitted, const TailFitDev &fit, const MatsDev &mt, const W2tDev &wd) {
+  if (statistic == 0) return launch_adj_stat<0>(s, beta, n_max, ncol, gbar_t, gbar_w, fitted, fit, mt, wd);
+  return launch_adj_stat<1>(s, beta, n_max, ncol, gbar_t, gbar_w, fitted, fit, mt, wd);
+}
+
+}  // namespace tprf
