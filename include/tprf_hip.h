@@ -138,11 +138,23 @@ int tprf_solve_rpa_PH(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *chi0,
  * (xbar = dL/dRe x + i dL/dIm x, torch's complex gradient).  All arrays are
  * device memory; scalars returned through device pointers.                  */
 
+/* Batches.  nbatch lattices with the same mesh (beta, n_iw, dims, norb) are
+ * folded into the k (or r) index, k = inst * nk + k_local, with the mesh
+ * index outermost: G[w][inst][k][n][n], G(tau)[j][inst][r][n][n],
+ * chi0_r[inst][r][n^4], chi_q[inst][n^4].  Every lattice transform and the
+ * omega <-> tau transforms then treat the batch as nbatch times more columns
+ * (tprf_fourier_wk_to_wr with nw * nbatch; tprf_fourier_wr_to_tr with
+ * ncol = nbatch * nk * n^2).                                                 */
+
+/* lattice_dyson_g_wk (gf.cpp:91-132) for a batch, one mu per lattice:
+ * mu[nbatch] (device), e_k[nbatch][nk][n][n] -> g_wk[2 n_iw][nbatch][nk][n][n]. */
+int tprf_lattice_dyson_g_wk_batched(tprf_ctx *ctx, double beta, int n_iw, const double *mu, long nbatch, long nk,
+                                    int norb, const tprf_c128 *e_k, tprf_c128 *g_wk);
 /* Adjoint of lattice_dyson_g_wk(mu, e_k) (gf.cpp:91-132) given G and Gbar:
- * ebar_k[k] = sum_w G^H Gbar G^H, *mubar = -Re sum_{w,k} tr(G^H Gbar G^H).
- * g_wk, gbar_wk [2 n_iw][nk][n][n]; ebar_k [nk][n][n]; mubar one double.
- * norb <= 3.                                                                 */
-int tprf_lattice_dyson_g_wk_adjoint(tprf_ctx *ctx, int n_iw, long nk, int norb, const tprf_c128 *g_wk,
+ * ebar_k[k] = sum_w G^H Gbar G^H, mubar[inst] = -Re sum_{w,k} tr(G^H Gbar G^H).
+ * g_wk, gbar_wk [2 n_iw][nbatch][nk][n][n]; ebar_k [nbatch][nk][n][n];
+ * mubar [nbatch] doubles.  norb <= 3.                                       */
+int tprf_lattice_dyson_g_wk_adjoint(tprf_ctx *ctx, int n_iw, long nk, int norb, long nbatch, const tprf_c128 *g_wk,
                                     const tprf_c128 *gbar_wk, tprf_c128 *ebar_k, double *mubar);
 /* Adjoint of fourier_wk_to_wr (fourier_lattice.cpp:49): gbar_wk =
  * FFT_{+}(gbar_wr) / N_k.                                                    */
@@ -157,20 +169,23 @@ int tprf_fourier_wr_to_tr_adjoint(tprf_ctx *ctx, int statistic, double beta, int
                                   const tprf_c128 *gbar_tr, tprf_c128 *gbar_wr, int fitted);
 /* nu = 0 row of chi0_wr_from_grt_PH (chi_imtime.cpp:139-182): the bosonic
  * tau -> nu transform at nu = 0 is a real linear functional
- * chi0_abcd(r) = sum_j c_j G_da(tau_j, r) G_bc(tau_{L-j}, -r); out[nr][n^4].
- * norb <= 3.                                                                 */
-int tprf_chi0_static_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb,
+ * chi0_abcd(r) = sum_j c_j G_da(tau_j, r) G_bc(tau_{L-j}, -r);
+ * g_tr[ntau][nbatch][nr][n][n] -> chi0_r[nbatch][nr][n^4].  norb <= 3.       */
+int tprf_chi0_static_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb, long nbatch,
                                  const tprf_c128 *g_tr, tprf_c128 *chi0_r);
 /* Adjoint of tprf_chi0_static_from_grt_PH with respect to g_tr:
- * chibar_r[nr][n^4] -> gbar_tr[ntau][nr][n^2] (g_tr is the forward input).  */
+ * chibar_r[nbatch][nr][n^4] -> gbar_tr[ntau][nbatch][nr][n^2] (g_tr is the
+ * forward input).                                                            */
 int tprf_chi0_static_from_grt_PH_adjoint(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb,
-                                         const tprf_c128 *g_tr, const tprf_c128 *chibar_r, tprf_c128 *gbar_tr);
+                                         long nbatch, const tprf_c128 *g_tr, const tprf_c128 *chibar_r,
+                                         tprf_c128 *gbar_tr);
 /* One momentum of fourier_wr_to_wk (fourier_lattice.cpp:55-58):
- * out[t] = sum_r exp(+2 pi i Q.r / dims) in[r][t], Q = q[0..2] (integers),
- * and its adjoint gbar_r[r][t] = exp(-2 pi i Q.r / dims) gbar_q[t].          */
-int tprf_fourier_r_to_q(tprf_ctx *ctx, const int dims[3], const int q[3], long ntarget, const tprf_c128 *in_r,
-                        tprf_c128 *out_q);
-int tprf_fourier_r_to_q_adjoint(tprf_ctx *ctx, const int dims[3], const int q[3], long ntarget,
+ * out[inst][t] = sum_r exp(+2 pi i Q.r / dims) in[inst][r][t], Q = q[0..2]
+ * (integers), and its adjoint gbar_r[inst][r][t] = exp(-2 pi i Q.r / dims)
+ * gbar_q[inst][t].  ntarget <= 65535.                                        */
+int tprf_fourier_r_to_q(tprf_ctx *ctx, const int dims[3], const int q[3], long ntarget, long nbatch,
+                        const tprf_c128 *in_r, tprf_c128 *out_q);
+int tprf_fourier_r_to_q_adjoint(tprf_ctx *ctx, const int dims[3], const int q[3], long ntarget, long nbatch,
                                 const tprf_c128 *gbar_q, tprf_c128 *gbar_r);
 /* solve_rpa_PH (rpa.cpp:29-64) at npts points with a U per point, and its
  * reverse-mode derivative: with chibar given (may be NULL for forward only),

@@ -156,7 +156,7 @@ def test_static_bubble_forward_vs_oracle(T, oracle, n, dims):
     g_tr = _gtr_oracle(oracle, rng, n, dims, beta, n_iw)
     ref = oracle.chi0_wr_from_grt_PH(g_tr, dims, beta, 1)[0]
     out = T.empty(ref.shape, dtype=T.complex128, device="cuda")
-    L.call("tprf_chi0_static_from_grt_PH", L.ctx(), beta, g_tr.shape[0], L.dims3(dims), n, L.ptr(gpu(g_tr)),
+    L.call("tprf_chi0_static_from_grt_PH", L.ctx(), beta, g_tr.shape[0], L.dims3(dims), n, 1, L.ptr(gpu(g_tr)),
            L.ptr(out))
     assert relerr(host(out), ref) <= 1e-10
 
@@ -174,12 +174,12 @@ def test_static_bubble_adjoint_dot(T, oracle, n, dims):
 
     def chi(g):
         o = T.empty((nr, n, n, n, n), dtype=T.complex128, device="cuda")
-        L.call("tprf_chi0_static_from_grt_PH", c, beta, ntau, L.dims3(dims), n, L.ptr(gpu(g)), L.ptr(o))
+        L.call("tprf_chi0_static_from_grt_PH", c, beta, ntau, L.dims3(dims), n, 1, L.ptr(gpu(g)), L.ptr(o))
         return host(o)
 
     Jd = 0.5 * (chi(g_tr + d) - chi(g_tr - d))  # exact: chi is quadratic in G
     gbar = T.empty(g_tr.shape, dtype=T.complex128, device="cuda")
-    L.call("tprf_chi0_static_from_grt_PH_adjoint", c, beta, ntau, L.dims3(dims), n, L.ptr(gpu(g_tr)),
+    L.call("tprf_chi0_static_from_grt_PH_adjoint", c, beta, ntau, L.dims3(dims), n, 1, L.ptr(gpu(g_tr)),
            L.ptr(gpu(ybar)), L.ptr(gbar))
     lhs, rhs = vdot(Jd, ybar), vdot(d, host(gbar))
     assert abs(lhs - rhs) <= 1e-11 * abs(lhs), (lhs, rhs)
@@ -194,13 +194,13 @@ def test_r_to_q_and_adjoint(T, oracle, dims, q):
     x = rnd(rng, nr, nt)
     out = T.empty((nt,), dtype=T.complex128, device="cuda")
     c = L.ctx()
-    L.call("tprf_fourier_r_to_q", c, L.dims3(dims), L.dims3(q), nt, L.ptr(gpu(x)), L.ptr(out))
+    L.call("tprf_fourier_r_to_q", c, L.dims3(dims), L.dims3(q), nt, 1, L.ptr(gpu(x)), L.ptr(out))
     full = oracle.fourier_lattice(x[None], dims, +1)[0]
     iq = (q[0] * dims[1] + q[1]) * dims[2] + q[2]
     assert relerr(host(out), full[iq]) <= 1e-12
     y = rnd(rng, nt)
     back = T.empty((nr, nt), dtype=T.complex128, device="cuda")
-    L.call("tprf_fourier_r_to_q_adjoint", c, L.dims3(dims), L.dims3(q), nt, L.ptr(gpu(y)), L.ptr(back))
+    L.call("tprf_fourier_r_to_q_adjoint", c, L.dims3(dims), L.dims3(q), nt, 1, L.ptr(gpu(y)), L.ptr(back))
     dot_check(host(out), y, x, host(back))
 
 
@@ -223,8 +223,8 @@ def test_dyson_adjoint_vs_torch(T, n):
     g = T.empty((2 * n_iw, nk, n, n), dtype=T.complex128, device="cuda")
     L.call("tprf_lattice_dyson_g_wk", c, beta, n_iw, mu, nk, n, L.ptr(gpu(e)), 0, None, L.ptr(g))
     ebar = T.empty((nk, n, n), dtype=T.complex128, device="cuda")
-    mubar = T.empty((), dtype=T.float64, device="cuda")
-    L.call("tprf_lattice_dyson_g_wk_adjoint", c, n_iw, nk, n, L.ptr(g), L.ptr(gpu(gbar)), L.ptr(ebar),
+    mubar = T.empty((1,), dtype=T.float64, device="cuda")
+    L.call("tprf_lattice_dyson_g_wk_adjoint", c, n_iw, nk, n, 1, L.ptr(g), L.ptr(gpu(gbar)), L.ptr(ebar),
            L.dptr(mubar))
     assert relerr(host(ebar), et.grad.numpy()) <= 1e-12
     assert abs(float(mubar) - float(mut.grad)) <= 1e-12 * abs(float(mut.grad))
@@ -340,3 +340,49 @@ def test_gradient_two_orbital_e_k(T, oracle):
     assert relerr(host(e_g.grad), e_r.grad.numpy()) <= 1e-9
     assert relerr(host(U_g.grad), U_r.grad.numpy()) <= 1e-9
     assert abs(float(mu_g.grad) - float(mu_r.grad)) <= 1e-9 * abs(float(mu_r.grad))
+
+
+# ---------------------------------------------------------------- batches
+def _two_orbital_sets(T, B, seed):
+    g = np.random.default_rng(seed)
+    tp = T.tensor(g.uniform(-0.4, 0.0, B), dtype=T.float64, device="cuda")
+    mu = T.tensor(g.uniform(-1.0, 1.0, B), dtype=T.float64, device="cuda")
+    v = T.tensor(g.uniform(0.0, 0.3, B), dtype=T.float64, device="cuda")
+    U = T.tensor(g.uniform(1.0, 3.0, B), dtype=T.float64, device="cuda")
+    return tp, mu, v, U
+
+
+def test_batched_grad_equals_per_lattice(T):
+    """A batch of 5 two-orbital lattices (eager and graph-captured) against the
+    same lattices one at a time: chi and every gradient to 1e-12."""
+    from tprf_amd import autograd as A
+    B, beta, n_iw, dims, q = 5, 10.0, 32, (12, 12, 1), (6, 6, 0)
+    tp, mu, v, U = _two_orbital_sets(T, B, 41)
+    e_k = A.two_orbital_square_e_k(tp, v, dims)
+    Uk = A.kanamori_spin_U(U, U / 6)
+    batch = A.StaticRPA(beta, n_iw, dims, q, norb=2, nbatch=B)
+    e_g = e_k.clone().requires_grad_(True)
+    mu_g = mu.clone().requires_grad_(True)
+    U_g = Uk.clone().requires_grad_(True)
+    chi_b = A.chi_rpa_static(e_g, mu_g, U_g, batch)
+    A.trace_loss(chi_b).backward()
+    one = A.StaticRPA(beta, n_iw, dims, q, norb=2)
+    for i in range(B):
+        e1 = e_k[i].clone().requires_grad_(True)
+        m1 = mu[i].clone().requires_grad_(True)
+        U1 = Uk[i].clone().requires_grad_(True)
+        chi1 = A.chi_rpa_static(e1, m1, U1, one)
+        A.trace_loss(chi1).backward()
+        assert relerr(host(chi_b[i]), host(chi1)) <= 1e-12
+        assert relerr(host(e_g.grad[i]), host(e1.grad)) <= 1e-12
+        assert relerr(host(U_g.grad[i]), host(U1.grad)) <= 1e-12
+        assert abs(float(mu_g.grad[i]) - float(m1.grad)) <= 1e-12 * abs(float(m1.grad))
+    chi_gr, ebar, mubar, Ubar = batch.graphed_grad(e_k, mu, Uk)
+    for rep in range(3):  # replays with eager work in between
+        assert relerr(host(chi_gr), host(chi_b)) <= 1e-13
+        assert relerr(host(ebar), host(e_g.grad)) <= 1e-13
+        assert relerr(host(Ubar), host(U_g.grad)) <= 1e-13
+        assert relerr(host(mubar), host(mu_g.grad)) <= 1e-13
+        _ = A.chi_rpa_static(e_k, mu, Uk, batch)
+        T.cuda.synchronize()
+        chi_gr, ebar, mubar, Ubar = batch.graphed_grad(e_k, mu, Uk)

@@ -26,64 +26,98 @@ def _dims3(dims):
 
 
 class StaticRPA:
-    """chi_RPA(Q, 0) as a function of (e_k[nk, n, n], mu, U[n, n, n, n]) for a fixed
-    mesh: beta, n_iw fermionic frequencies (ntau = 6 n_iw + 1 by default),
-    lattice dims and integer momentum q (Q = 2 pi q / dims)."""
+    """chi_RPA(Q, 0) as a function of (e_k, mu, U) for a fixed mesh: beta, n_iw
+    fermionic frequencies (ntau = 6 n_iw + 1 by default), lattice dims and integer
+    momentum q (Q = 2 pi q / dims).
 
-    def __init__(self, beta, n_iw, dims, q, norb, ntau=None, device=None):
+    With nbatch > 1 the object evaluates nbatch lattices of that mesh at once
+    (e_k[nbatch, nk, n, n], mu[nbatch], U[nbatch, n, n, n, n]), each stage one
+    launch over the whole batch; ``graphed_grad`` captures forward + backward of
+    L = sum_i Re tr chi_i in one HIP graph."""
+
+    def __init__(self, beta, n_iw, dims, q, norb, ntau=None, nbatch=1, device=None):
         self.beta = float(beta)
         self.n_iw = int(n_iw)
         self.dims = _dims3(dims)
         self.q = _dims3(q)
         self.n = int(norb)
+        self.nb = int(nbatch)
         self.nk = self.dims[0] * self.dims[1] * self.dims[2]
         self.nw = 2 * self.n_iw
         self.ntau = 6 * self.n_iw + 1 if (ntau is None or ntau <= 0) else int(ntau)
         dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
         self.dev = dev
-        n, nk, c128 = self.n, self.nk, torch.complex128
-        self._g_wr = torch.empty((self.nw, nk, n, n), dtype=c128, device=dev)
-        self._g_tr = torch.empty((self.ntau, nk, n, n), dtype=c128, device=dev)
-        self._x_tr = torch.empty((self.ntau, nk, n, n), dtype=c128, device=dev)
-        self._chi_r = torch.empty((nk, n, n, n, n), dtype=c128, device=dev)
+        n, nk, B, c128 = self.n, self.nk, self.nb, torch.complex128
+        self._g_wr = torch.empty((self.nw, B, nk, n, n), dtype=c128, device=dev)
+        self._g_tr = torch.empty((self.ntau, B, nk, n, n), dtype=c128, device=dev)
+        self._x_tr = torch.empty((self.ntau, B, nk, n, n), dtype=c128, device=dev)
+        self._chi_r = torch.empty((B, nk, n, n, n, n), dtype=c128, device=dev)
         self._cdims = _lib.dims3(self.dims)
         self._cq = _lib.dims3(self.q)
+        self._graph = None
 
     # -- pieces ---------------------------------------------------------------
     def _ctx(self):
         return _lib.ctx(self.dev.index)
 
-    def _check_input(self, e_k, U):
-        n, nk = self.n, self.nk
-        if tuple(e_k.shape) != (nk, n, n) or tuple(U.shape) != (n, n, n, n):
-            raise ValueError(f"expected e_k[{nk},{n},{n}] and U[{n},{n},{n},{n}]")
+    def _inputs(self, e_k, mu, U):
+        n, nk, B = self.n, self.nk, self.nb
+        e_k = e_k.detach().to(self.dev, torch.complex128).reshape(B, nk, n, n).contiguous()
+        U = U.detach().to(self.dev, torch.complex128).reshape(B, n, n, n, n).contiguous()
+        mu = torch.as_tensor(mu, dtype=torch.float64).detach().to(self.dev).reshape(B).contiguous()
+        return e_k, mu, U
 
     def _g_tr_from_g_wk(self, c, g_wk):
         p = _lib.ptr
-        n, nk = self.n, self.nk
-        _lib.call("tprf_fourier_wk_to_wr", c, self.nw, self._cdims, n * n, p(g_wk), p(self._g_wr))
-        _lib.call("tprf_fourier_wr_to_tr", c, _lib.FERMION, self.beta, self.n_iw, self.ntau, nk * n * n,
+        n, nk, B = self.n, self.nk, self.nb
+        _lib.call("tprf_fourier_wk_to_wr", c, self.nw * B, self._cdims, n * n, p(g_wk), p(self._g_wr))
+        _lib.call("tprf_fourier_wr_to_tr", c, _lib.FERMION, self.beta, self.n_iw, self.ntau, B * nk * n * n,
                   p(self._g_wr), p(self._g_tr), None)
 
+    def _forward_into(self, c, e_k, mu, U, g_wk, chi0_q, chi):
+        p = _lib.ptr
+        n, nk, B = self.n, self.nk, self.nb
+        _lib.call("tprf_lattice_dyson_g_wk_batched", c, self.beta, self.n_iw, _lib.dptr(mu), B, nk, n, p(e_k),
+                  p(g_wk))
+        self._g_tr_from_g_wk(c, g_wk)
+        _lib.call("tprf_chi0_static_from_grt_PH", c, self.beta, self.ntau, self._cdims, n, B, p(self._g_tr),
+                  p(self._chi_r))
+        _lib.call("tprf_fourier_r_to_q", c, self._cdims, self._cq, n ** 4, B, p(self._chi_r), p(chi0_q))
+        _lib.call("tprf_solve_rpa_PH_adjoint", c, B, n, p(chi0_q), p(U), None, p(chi), None, None)
+
+    def _backward_into(self, c, g_wk, chi0_q, U, chibar, chi, chi0bar, Ubar, ebar, mubar):
+        p = _lib.ptr
+        n, nk, B = self.n, self.nk, self.nb
+        _lib.call("tprf_solve_rpa_PH_adjoint", c, B, n, p(chi0_q), p(U), p(chibar), p(chi), p(chi0bar), p(Ubar))
+        chibar_r = self._chi_r
+        _lib.call("tprf_fourier_r_to_q_adjoint", c, self._cdims, self._cq, n ** 4, B, p(chi0bar), p(chibar_r))
+        # recompute G(tau, r) from the kept G(iw, k)
+        self._g_tr_from_g_wk(c, g_wk)
+        gbar_tr = self._x_tr
+        _lib.call("tprf_chi0_static_from_grt_PH_adjoint", c, self.beta, self.ntau, self._cdims, n, B,
+                  p(self._g_tr), p(chibar_r), p(gbar_tr))
+        gbar_wr = self._g_wr
+        _lib.call("tprf_fourier_wr_to_tr_adjoint", c, _lib.FERMION, self.beta, self.n_iw, self.ntau, B * nk * n * n,
+                  p(gbar_tr), p(gbar_wr), 1)
+        gbar_wk = self._g_tr.view(-1)[: self.nw * B * nk * n * n].view(self.nw, B, nk, n, n)
+        _lib.call("tprf_fourier_wk_to_wr_adjoint", c, self.nw * B, self._cdims, n * n, p(gbar_wr), p(gbar_wk))
+        _lib.call("tprf_lattice_dyson_g_wk_adjoint", c, self.n_iw, nk, n, B, p(g_wk), p(gbar_wk), p(ebar),
+                  _lib.dptr(mubar))
+
+    def _new(self, *shape, dtype=torch.complex128):
+        return torch.empty(shape, dtype=dtype, device=self.dev)
+
+    # -- eager API --------------------------------------------------------------
     def forward(self, e_k, mu, U, check_tail=True):
-        """Returns (chi[n,n,n,n], chi0_q[n,n,n,n], g_wk)."""
-        self._check_input(e_k, U)
-        c, p = self._ctx(), _lib.ptr
-        n, nk = self.n, self.nk
-        e_k = e_k.detach().to(self.dev, torch.complex128).contiguous()
-        U = U.detach().to(self.dev, torch.complex128).contiguous()
-        g_wk = torch.empty((self.nw, nk, n, n), dtype=torch.complex128, device=self.dev)
-        chi0_q = torch.empty((n, n, n, n), dtype=torch.complex128, device=self.dev)
-        chi = torch.empty_like(chi0_q)
+        """Returns (chi[nbatch, n,n,n,n], chi0_q, g_wk); g_wk is what backward needs."""
+        n, nk, B = self.n, self.nk, self.nb
+        e_k, mu, U = self._inputs(e_k, mu, U)
+        c = self._ctx()
+        g_wk = self._new(self.nw, B, nk, n, n)
+        chi0_q, chi = self._new(B, n, n, n, n), self._new(B, n, n, n, n)
         if check_tail:
             _lib.tail_status_reset(c)
-        _lib.call("tprf_lattice_dyson_g_wk", c, self.beta, self.n_iw, float(mu), nk, n, p(e_k), _lib.SIGMA_NONE,
-                  None, p(g_wk))
-        self._g_tr_from_g_wk(c, g_wk)
-        _lib.call("tprf_chi0_static_from_grt_PH", c, self.beta, self.ntau, self._cdims, n, p(self._g_tr),
-                  p(self._chi_r))
-        _lib.call("tprf_fourier_r_to_q", c, self._cdims, self._cq, n ** 4, p(self._chi_r), p(chi0_q))
-        _lib.call("tprf_solve_rpa_PH_adjoint", c, 1, n, p(chi0_q), p(U), None, p(chi), None, None)
+        self._forward_into(c, e_k, mu, U, g_wk, chi0_q, chi)
         if check_tail:
             err, _ = _lib.tail_status(c)
             if err >= 1e-3:
@@ -91,71 +125,95 @@ class StaticRPA:
         return chi, chi0_q, g_wk
 
     def backward(self, g_wk, chi0_q, U, chibar):
-        """Returns (ebar_k[nk,n,n], mubar (0-dim float64), Ubar[n,n,n,n])."""
-        c, p = self._ctx(), _lib.ptr
-        n, nk = self.n, self.nk
-        U = U.detach().to(self.dev, torch.complex128).contiguous()
-        chibar = chibar.detach().to(self.dev, torch.complex128).contiguous()
-        chi = torch.empty_like(chi0_q)
-        chi0bar = torch.empty_like(chi0_q)
-        Ubar = torch.empty_like(chi0_q)
-        ebar = torch.empty((nk, n, n), dtype=torch.complex128, device=self.dev)
-        mubar = torch.empty((), dtype=torch.float64, device=self.dev)
-        # RPA and momentum projection
-        _lib.call("tprf_solve_rpa_PH_adjoint", c, 1, n, p(chi0_q), p(U), p(chibar), p(chi), p(chi0bar), p(Ubar))
-        chibar_r = self._chi_r
-        _lib.call("tprf_fourier_r_to_q_adjoint", c, self._cdims, self._cq, n ** 4, p(chi0bar), p(chibar_r))
-        # recompute G(tau, r) from the kept G(iw, k)
-        self._g_tr_from_g_wk(c, g_wk)
-        gbar_tr = self._x_tr
-        _lib.call("tprf_chi0_static_from_grt_PH_adjoint", c, self.beta, self.ntau, self._cdims, n, p(self._g_tr),
-                  p(chibar_r), p(gbar_tr))
-        gbar_wr = self._g_wr
-        _lib.call("tprf_fourier_wr_to_tr_adjoint", c, _lib.FERMION, self.beta, self.n_iw, self.ntau, nk * n * n,
-                  p(gbar_tr), p(gbar_wr), 1)
-        gbar_wk = self._g_tr.view(-1)[: self.nw * nk * n * n].view(self.nw, nk, n, n)
-        _lib.call("tprf_fourier_wk_to_wr_adjoint", c, self.nw, self._cdims, n * n, p(gbar_wr), p(gbar_wk))
-        _lib.call("tprf_lattice_dyson_g_wk_adjoint", c, self.n_iw, nk, n, p(g_wk), p(gbar_wk), p(ebar),
-                  _lib.dptr(mubar))
+        """Returns (ebar_k[nbatch, nk, n, n], mubar[nbatch], Ubar[nbatch, n, n, n, n])."""
+        n, nk, B = self.n, self.nk, self.nb
+        U = U.detach().to(self.dev, torch.complex128).reshape(B, n, n, n, n).contiguous()
+        chibar = chibar.detach().to(self.dev, torch.complex128).reshape(B, n, n, n, n).contiguous()
+        chi, chi0bar, Ubar = (self._new(B, n, n, n, n) for _ in range(3))
+        ebar = self._new(B, nk, n, n)
+        mubar = self._new(B, dtype=torch.float64)
+        self._backward_into(self._ctx(), g_wk, chi0_q, U, chibar, chi, chi0bar, Ubar, ebar, mubar)
         return ebar, mubar, Ubar
+
+    # -- one HIP graph: forward + backward of L = sum_i Re tr chi_i ------------
+    def graphed_grad(self, e_k, mu, U):
+        """Copies the inputs into static buffers and replays (capturing on the first
+        call) forward + backward for the whole batch.  Returns (chi, ebar, mubar, Ubar)
+        as views of static buffers, valid until the next call."""
+        n, nk, B = self.n, self.nk, self.nb
+        e_k, mu, U = self._inputs(e_k, mu, U)
+        if self._graph is None:
+            st = {"e_k": e_k.clone(), "mu": mu.clone(), "U": U.clone(), "g_wk": self._new(self.nw, B, nk, n, n),
+                  "chi0_q": self._new(B, n, n, n, n), "chi": self._new(B, n, n, n, n),
+                  "chi2": self._new(B, n, n, n, n), "chi0bar": self._new(B, n, n, n, n),
+                  "Ubar": self._new(B, n, n, n, n), "ebar": self._new(B, nk, n, n),
+                  "mubar": self._new(B, dtype=torch.float64)}
+            cb = torch.zeros((B, n, n, n, n), dtype=torch.complex128)
+            for a in range(n):
+                for b in range(n):
+                    cb[:, a, a, b, b] = 1.0  # d(Re sum_ab chi_aabb) in the complex-gradient convention
+            st["chibar"] = cb.to(self.dev)
+
+            def run():
+                c = self._ctx()
+                self._forward_into(c, st["e_k"], st["mu"], st["U"], st["g_wk"], st["chi0_q"], st["chi"])
+                self._backward_into(c, st["g_wk"], st["chi0_q"], st["U"], st["chibar"], st["chi2"],
+                                    st["chi0bar"], st["Ubar"], st["ebar"], st["mubar"])
+
+            # warm-up outside the graph: plans, tables and scratch are allocated here
+            side = torch.cuda.Stream(device=self.dev)
+            side.wait_stream(torch.cuda.current_stream(self.dev))
+            with torch.cuda.stream(side):
+                run()
+            torch.cuda.current_stream(self.dev).wait_stream(side)
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                run()
+            self._graph, self._static = g, st
+        st = self._static
+        st["e_k"].copy_(e_k)
+        st["mu"].copy_(mu)
+        st["U"].copy_(U)
+        self._graph.replay()
+        return st["chi"], st["ebar"], st["mubar"], st["Ubar"]
 
 
 class _StaticRPAFunction(torch.autograd.Function):
     @staticmethod
     def forward(fctx, e_k, mu, U, spec):
-        mu_val = float(mu)
-        chi, chi0_q, g_wk = spec.forward(e_k, mu_val, U)
+        chi, chi0_q, g_wk = spec.forward(e_k, mu, U)
         fctx.spec = spec
         fctx.mu_like = (mu.device, mu.dtype, mu.shape) if torch.is_tensor(mu) else None
-        fctx.e_meta = (e_k.device, e_k.dtype)
-        fctx.u_meta = (U.device, U.dtype)
+        fctx.e_meta = (e_k.device, e_k.dtype, e_k.shape)
+        fctx.u_meta = (U.device, U.dtype, U.shape)
         fctx.save_for_backward(g_wk, chi0_q, U.detach())
-        return chi
+        return chi.reshape(U.shape)
 
     @staticmethod
     def backward(fctx, chibar):
         g_wk, chi0_q, U = fctx.saved_tensors
         ebar, mubar, Ubar = fctx.spec.backward(g_wk, chi0_q, U, chibar)
-        ge = ebar.to(*fctx.e_meta) if fctx.needs_input_grad[0] else None
+        ge = ebar.to(*fctx.e_meta[:2]).reshape(fctx.e_meta[2]) if fctx.needs_input_grad[0] else None
         gm = None
         if fctx.needs_input_grad[1] and fctx.mu_like is not None:
             dev, dt, shape = fctx.mu_like
             gm = mubar.to(dev, dt).reshape(shape)
-        gU = Ubar.to(*fctx.u_meta) if fctx.needs_input_grad[2] else None
+        gU = Ubar.to(*fctx.u_meta[:2]).reshape(fctx.u_meta[2]) if fctx.needs_input_grad[2] else None
         return ge, gm, gU, None
 
 
 def chi_rpa_static(e_k, mu, U, spec):
-    """chi_RPA(Q, 0)[a,b,c,d] for the mesh of `spec` (a StaticRPA); differentiable in
-    e_k (complex [nk,n,n]), mu (float64 tensor) and U (complex [n,n,n,n])."""
+    """chi_RPA(Q, 0)[..., a,b,c,d] for the mesh of `spec` (a StaticRPA), shaped like U;
+    differentiable in e_k (complex [(nbatch,) nk, n, n]), mu (float64, 0-dim or
+    [nbatch]) and U (complex [(nbatch,) n, n, n, n])."""
     return _StaticRPAFunction.apply(e_k, mu, U, spec)
 
 
 def trace_loss(chi):
-    """L = Re sum_{a,b} chi_{aabb}."""
-    n = chi.shape[0]
+    """L = Re sum_{a,b} chi_{aabb} (summed over a leading batch axis, if any)."""
+    n = chi.shape[-1]
     idx = torch.arange(n, device=chi.device)
-    return chi[idx[:, None], idx[:, None], idx[None, :], idx[None, :]].sum().real
+    return chi[..., idx[:, None], idx[:, None], idx[None, :], idx[None, :]].sum().real
 
 
 def square_e_k(t, tp, dims):
@@ -173,3 +231,37 @@ def square_e_k(t, tp, dims):
 def hubbard_U(U):
     """1x1x1x1 onsite interaction tensor from a real U tensor (single band)."""
     return U.reshape(1, 1, 1, 1).to(torch.complex128)
+
+
+def two_orbital_square_e_k(tp, v, dims, t=1.0):
+    """Two-orbital square lattice: both orbitals e(k) = -2t(cos kx + cos ky) - 4t' cos kx cos ky,
+    inter-orbital nearest-neighbour hopping v: e_12(k) = -2v(cos kx + cos ky).
+    tp, v: real tensors of shape [B]; returns complex [B, nk, 2, 2]."""
+    d0, d1 = int(dims[0]), int(dims[1])
+    dev = tp.device
+    kx = 2 * torch.pi * torch.arange(d0, dtype=torch.float64, device=dev) / d0
+    ky = 2 * torch.pi * torch.arange(d1, dtype=torch.float64, device=dev) / d1
+    cx, cy = torch.cos(kx)[:, None].expand(d0, d1).reshape(-1), torch.cos(ky)[None, :].expand(d0, d1).reshape(-1)
+    eps = -2 * t * (cx + cy)[None, :] - 4 * tp[:, None] * (cx * cy)[None, :]
+    off = -2 * v[:, None] * (cx + cy)[None, :]
+    row0 = torch.stack([eps, off], dim=-1)
+    row1 = torch.stack([off, eps], dim=-1)
+    return torch.stack([row0, row1], dim=-2).to(torch.complex128)
+
+
+def kanamori_spin_U(U, J):
+    """Spin-channel Kanamori tensor of two orbitals (U' = U - 2J, J' = J) in TPRF's
+    [a, a', b, b'] order (rpa_tensor.py:94-118) as a differentiable function of the
+    real tensors U, J of shape [B]; returns complex [B, 2, 2, 2, 2]."""
+    B = U.shape[0]
+    out = torch.zeros((B, 2, 2, 2, 2), dtype=torch.float64, device=U.device)
+    Up = U - 2 * J
+    for a in range(2):
+        for b in range(2):
+            if a == b:
+                out[:, a, a, a, a] = U
+            else:
+                out[:, a, b, b, a] = Up      # a = b' != b = a'
+                out[:, a, a, b, b] = J       # a = a' != b = b'
+                out[:, a, b, a, b] = J       # a = b != a' = b'
+    return out.to(torch.complex128)

@@ -11,13 +11,14 @@
 
 namespace tprf {
 
-// chi0(Q) = sum_r exp(2 pi i Q.r / dims) chi0(r): one workgroup per target
-// component, deterministic tree reduction over r.
+// chi0(Q) = sum_r exp(2 pi i Q.r / dims) chi0(r): one workgroup per (target
+// component, lattice of the batch), deterministic tree reduction over r.
 __global__ void __launch_bounds__(256) k_q_project(const c2 *__restrict__ chi_r, c2 *__restrict__ chi_q, int d0, int d1,
                                                    int d2, int q0, int q1, int q2, int ntarget) {
   __shared__ c2 red[256];
   const int comp = blockIdx.x;
   const long nr = (long)d0 * d1 * d2;
+  const c2 *src = chi_r + (long)blockIdx.y * nr * ntarget;
   c2 s = c2{0.0, 0.0};
   for (long r = threadIdx.x; r < nr; r += blockDim.x) {
     const long i2 = r % d2, i1 = (r / d2) % d1, i0 = r / ((long)d1 * d2);
@@ -25,7 +26,7 @@ __global__ void __launch_bounds__(256) k_q_project(const c2 *__restrict__ chi_r,
     const double f = (double)((q0 * i0) % d0) / d0 + (double)((q1 * i1) % d1) / d1 + (double)((q2 * i2) % d2) / d2;
     double sn, cs;
     sincospi(2.0 * f, &sn, &cs);
-    s = cadd(s, cmul(c2{cs, sn}, chi_r[r * ntarget + comp]));
+    s = cadd(s, cmul(c2{cs, sn}, src[r * ntarget + comp]));
   }
   red[threadIdx.x] = s;
   __syncthreads();
@@ -33,86 +34,100 @@ __global__ void __launch_bounds__(256) k_q_project(const c2 *__restrict__ chi_r,
     if ((int)threadIdx.x < w) red[threadIdx.x] = cadd(red[threadIdx.x], red[threadIdx.x + w]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) chi_q[comp] = red[0];
+  if (threadIdx.x == 0) chi_q[(long)blockIdx.y * ntarget + comp] = red[0];
 }
 
-// chibar(r) = exp(-2 pi i Q.r / dims) chibar(Q)
+// chibar(r) = exp(-2 pi i Q.r / dims) chibar(Q), per lattice of the batch
 __global__ void __launch_bounds__(256) k_q_project_adjoint(const c2 *__restrict__ chibar_q, c2 *__restrict__ chibar_r,
                                                            int d0, int d1, int d2, int q0, int q1, int q2,
-                                                           int ntarget) {
+                                                           int ntarget, long total) {
   const long nr = (long)d0 * d1 * d2;
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= nr * ntarget) return;
-  const long r = idx / ntarget;
-  const int comp = (int)(idx - r * ntarget);
+  if (idx >= total) return;
+  const long rt = idx / ntarget;
+  const int comp = (int)(idx - rt * ntarget);
+  const long inst = rt / nr, r = rt - inst * nr;
   const long i2 = r % d2, i1 = (r / d2) % d1, i0 = r / ((long)d1 * d2);
   const double f = (double)((q0 * i0) % d0) / d0 + (double)((q1 * i1) % d1) / d1 + (double)((q2 * i2) % d2) / d2;
   double sn, cs;
   sincospi(2.0 * f, &sn, &cs);
-  chibar_r[idx] = cmul(c2{cs, -sn}, chibar_q[comp]);
+  chibar_r[idx] = cmul(c2{cs, -sn}, chibar_q[inst * ntarget + comp]);
 }
 
 // Static bubble adjoint: gbar_xy(tau_j, r) =
-//   conj(G(L-j, -r)) contracted with c_j chibar_{y b c x}(r) over (b, c)
-//   + c_{L-j} chibar_{a x y d}(-r) over (a, d)  [with G_bc / G_da of (L-j, -r)].
+//   c_j     sum_{b,c} chibar_{y b c x}(r)  conj(G_bc(tau_{L-j}, -r))
+// + c_{L-j} sum_{a,d} chibar_{a x y d}(-r) conj(G_da(tau_{L-j}, -r)).
+// Thread (r, x, y) of tau chunk blockIdx.y keeps its 2 n^2 chibar weights in
+// registers and walks its chunk of tau; consecutive threads write consecutive
+// gbar entries.  A batch of lattices (same dims) is folded into r:
+// r = inst * nr + r_local.
 template <int N>
-__global__ void __launch_bounds__(256) k_bubble_static_adjoint(int ntau, int d0, int d1, int d2,
-                                                               const c2 *__restrict__ gtr,
+__global__ void __launch_bounds__(256) k_bubble_static_adjoint(int ntau, int d0, int d1, int d2, long nbatch,
+                                                               int jchunk, const c2 *__restrict__ gtr,
                                                                const c2 *__restrict__ chibar_r,
                                                                const double *__restrict__ cw, c2 *__restrict__ gbar) {
   constexpr int n2 = N * N, n4 = n2 * n2;
-  const long nr = (long)d0 * d1 * d2;
+  const long nri = (long)d0 * d1 * d2;
+  const long nr = nri * nbatch;
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long)ntau * nr) return;
-  const long j = idx / nr, r = idx - j * nr;
+  if (idx >= nr * n2) return;
+  const long r = idx / n2;
+  const int xy = (int)(idx - r * n2), x = xy / N, y = xy % N;
   const int L = ntau - 1;
-  const long mr = minus_r(r, d0, d1, d2);
-  const c2 *gm = gtr + ((long)(L - j) * nr + mr) * n2;  // G(tau_{L-j}, -r)
-  c2 g[n2];
-#pragma unroll
-  for (int i = 0; i < n2; ++i) g[i] = cconj(gm[i]);
+  const long inst = r / nri;
+  const long mr = inst * nri + minus_r(r - inst * nri, d0, d1, d2);
   const c2 *cb = chibar_r + r * n4, *cbm = chibar_r + mr * n4;
-  const double cj = cw[j], cl = cw[L - j];
-  c2 *o = gbar + idx * n2;
+  c2 A[n2], B[n2];
 #pragma unroll
-  for (int x = 0; x < N; ++x)
+  for (int b = 0; b < N; ++b)
 #pragma unroll
-    for (int y = 0; y < N; ++y) {
-      c2 s = c2{0.0, 0.0};
-#pragma unroll
-      for (int b = 0; b < N; ++b)
-#pragma unroll
-        for (int c = 0; c < N; ++c) {
-          // first factor: x = d, y = a of chi_{y b c x}(r), partner G_bc(L-j, -r)
-          s = cadd(s, cscale(cmul(cb[((y * N + b) * N + c) * N + x], g[b * N + c]), cj));
-          // second factor: (b, c) = (x, y) of chi_{a x y d}(-r), partner G_da(L-j, -r) with (a, d) = (b, c)
-          s = cadd(s, cscale(cmul(cbm[((b * N + x) * N + y) * N + c], g[c * N + b]), cl));
-        }
-      o[x * N + y] = s;
+    for (int c = 0; c < N; ++c) {
+      A[b * N + c] = cb[((y * N + b) * N + c) * N + x];   // pairs with G_bc
+      B[c * N + b] = cbm[((b * N + x) * N + y) * N + c];  // (a, d) = (b, c): pairs with G_cb
     }
+  const int j0 = blockIdx.y * jchunk;
+  const int j1 = (j0 + jchunk) < ntau ? (j0 + jchunk) : ntau;
+  for (int j = j0; j < j1; ++j) {
+    const c2 *gm = gtr + ((long)(L - j) * nr + mr) * n2;  // G(tau_{L-j}, -r)
+    c2 s1 = c2{0.0, 0.0}, s2 = c2{0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < n2; ++i) {
+      const c2 g = cconj(gm[i]);
+      s1 = cadd(s1, cmul(A[i], g));
+      s2 = cadd(s2, cmul(B[i], g));
+    }
+    gbar[((long)j * nr + r) * n2 + xy] = cadd(cscale(s1, cw[j]), cscale(s2, cw[L - j]));
+  }
 }
 
 // Dyson adjoint: Mbar = -G^H Gbar G^H for every (w, k); ebar_k = -sum_w Mbar,
-// mubar = Re sum tr Mbar.  Thread (k, chunk) sums its chunk of frequencies into
-// part_e[chunk][k][n^2]; per-block sums of Re tr Mbar go to mu_part.
+// mubar = Re sum tr Mbar.  A batch of lattices is folded into k (k = inst * nk
+// + k_local); blockIdx.x = inst * nbk + block of k_local, so no block straddles
+// two lattices.  Thread (k, chunk) sums its chunk of frequencies into
+// part_e[chunk][k][n^2]; per-block sums of Re tr Mbar go to
+// mu_part[(inst * nchunk + chunk) * nbk + block].
 template <int N>
-__global__ void __launch_bounds__(256) k_dyson_adjoint(int nw, long nk, int wchunk, const c2 *__restrict__ g,
-                                                       const c2 *__restrict__ gbar, c2 *__restrict__ part_e,
-                                                       double *__restrict__ mu_part) {
+__global__ void __launch_bounds__(256) k_dyson_adjoint(int nw, long nk, long nbatch, int nbk, int wchunk,
+                                                       const c2 *__restrict__ g, const c2 *__restrict__ gbar,
+                                                       c2 *__restrict__ part_e, double *__restrict__ mu_part) {
   __shared__ double red[256];
-  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long inst = blockIdx.x / nbk;
+  const int kb = (int)(blockIdx.x - inst * nbk);
+  const long kl = (long)kb * blockDim.x + threadIdx.x;
+  const long ktot = nk * nbatch;
+  const long k = inst * nk + kl;
   const int w0 = blockIdx.y * wchunk;
   const int w1 = (w0 + wchunk) < nw ? (w0 + wchunk) : nw;
   double mu = 0.0;
-  if (k < nk) {
+  if (kl < nk) {
     c2 e[N][N];
 #pragma unroll
     for (int a = 0; a < N; ++a)
 #pragma unroll
       for (int b = 0; b < N; ++b) e[a][b] = c2{0.0, 0.0};
     for (int w = w0; w < w1; ++w) {
-      const c2 *G = g + ((long)w * nk + k) * N * N;
-      const c2 *B = gbar + ((long)w * nk + k) * N * N;
+      const c2 *G = g + ((long)w * ktot + k) * N * N;
+      const c2 *B = gbar + ((long)w * ktot + k) * N * N;
       c2 Gv[N][N], Bv[N][N];
 #pragma unroll
       for (int a = 0; a < N; ++a)
@@ -140,7 +155,7 @@ __global__ void __launch_bounds__(256) k_dyson_adjoint(int nw, long nk, int wchu
           if (a == b) mu -= s.x;
         }
     }
-    c2 *o = part_e + ((long)blockIdx.y * nk + k) * N * N;
+    c2 *o = part_e + ((long)blockIdx.y * ktot + k) * N * N;
 #pragma unroll
     for (int a = 0; a < N; ++a)
 #pragma unroll
@@ -152,7 +167,7 @@ __global__ void __launch_bounds__(256) k_dyson_adjoint(int nw, long nk, int wchu
     if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) mu_part[(long)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+  if (threadIdx.x == 0) mu_part[(inst * gridDim.y + blockIdx.y) * nbk + kb] = red[0];
 }
 
 // out[i] = sum_c part[c * stride + i], i < stride (fixed order: deterministic)
@@ -169,17 +184,19 @@ __global__ void __launch_bounds__(256) k_sum_chunks(const c2 *__restrict__ part,
 // Thread (r, d, a) of tau chunk y accumulates the n^2 entries (b, c) into
 // part[y][r][abcd].
 template <int N>
-__global__ void __launch_bounds__(256) k_bubble_static(int ntau, int d0, int d1, int d2, int jchunk,
+__global__ void __launch_bounds__(256) k_bubble_static(int ntau, int d0, int d1, int d2, long nbatch, int jchunk,
                                                        const c2 *__restrict__ gtr, const double *__restrict__ cw,
                                                        c2 *__restrict__ part) {
   constexpr int n2 = N * N, n4 = n2 * n2;
-  const long nr = (long)d0 * d1 * d2;
+  const long nri = (long)d0 * d1 * d2;
+  const long nr = nri * nbatch;
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= nr * n2) return;
   const long r = idx / n2;
   const int da = (int)(idx - r * n2), d = da / N, a = da % N;
   const int L = ntau - 1;
-  const long mr = minus_r(r, d0, d1, d2);
+  const long inst = r / nri;
+  const long mr = inst * nri + minus_r(r - inst * nri, d0, d1, d2);
   const int j0 = blockIdx.y * jchunk;
   const int j1 = (j0 + jchunk) < ntau ? (j0 + jchunk) : ntau;
   c2 acc[n2];
@@ -198,18 +215,19 @@ __global__ void __launch_bounds__(256) k_bubble_static(int ntau, int d0, int d1,
     for (int c = 0; c < N; ++c) o[((a * N + b) * N + c) * N + d] = acc[b * N + c];
 }
 
-// sum of the per-block partials (one block, deterministic)
+// out[b] = sum of part[b * n .. b * n + n) (one workgroup per segment, deterministic)
 __global__ void __launch_bounds__(256) k_sum_partials(const double *__restrict__ part, int n, double *__restrict__ out) {
   __shared__ double red[256];
+  const double *p = part + (long)blockIdx.x * n;
   double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) s += part[i];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
   red[threadIdx.x] = s;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = red[0];
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
 // RPA at single points, forward and reverse, one thread per point (n <= 2):
@@ -344,64 +362,81 @@ __global__ void __launch_bounds__(64) k_rpa_point(long npts, const c2 *__restric
 }
 
 // ---- launchers ---------------------------------------------------------
-hipError_t launch_q_project(hipStream_t s, const c2 *chi_r, c2 *chi_q, const int dims[3], const int q[3], int ntarget) {
-  hipLaunchKernelGGL(k_q_project, dim3(ntarget), dim3(256), 0, s, chi_r, chi_q, dims[0], dims[1], dims[2], q[0], q[1],
-                     q[2], ntarget);
+hipError_t launch_q_project(hipStream_t s, const c2 *chi_r, c2 *chi_q, const int dims[3], const int q[3], int ntarget,
+                            long nbatch) {
+  hipLaunchKernelGGL(k_q_project, dim3(ntarget, nbatch), dim3(256), 0, s, chi_r, chi_q, dims[0], dims[1], dims[2],
+                     q[0], q[1], q[2], ntarget);
   return hipGetLastError();
 }
 
 hipError_t launch_q_project_adjoint(hipStream_t s, const c2 *chibar_q, c2 *chibar_r, const int dims[3], const int q[3],
-                                    int ntarget) {
-  const long total = (long)dims[0] * dims[1] * dims[2] * ntarget;
+                                    int ntarget, long nbatch) {
+  const long total = (long)dims[0] * dims[1] * dims[2] * ntarget * nbatch;
   hipLaunchKernelGGL(k_q_project_adjoint, dim3((total + 255) / 256), dim3(256), 0, s, chibar_q, chibar_r, dims[0],
-                     dims[1], dims[2], q[0], q[1], q[2], ntarget);
+                     dims[1], dims[2], q[0], q[1], q[2], ntarget, total);
   return hipGetLastError();
 }
 
-hipError_t launch_bubble_static_adjoint(hipStream_t s, int ntau, const int dims[3], int n, const c2 *gtr,
+hipError_t launch_bubble_static_adjoint(hipStream_t s, int ntau, const int dims[3], int n, long nbatch, const c2 *gtr,
                                         const c2 *chibar_r, const double *cw, c2 *gbar) {
-  const long total = (long)ntau * dims[0] * dims[1] * dims[2];
-  const long nb = (total + 255) / 256;
+  const long nthr = (long)dims[0] * dims[1] * dims[2] * nbatch * n * n;
+  const long nb = (nthr + 255) / 256;
+  // tau chunks: about 2^20 threads in flight, each chunk at least 16 tau points
+  long nchunk = (1L << 20) / nthr;
+  if (nchunk < 1) nchunk = 1;
+  if (nchunk > ntau / 16) nchunk = ntau / 16 > 0 ? ntau / 16 : 1;
+  const int jchunk = (int)((ntau + nchunk - 1) / nchunk);
+  nchunk = (ntau + jchunk - 1) / jchunk;
+  const dim3 grid(nb, nchunk);
+#define TPRF_SA(N) hipLaunchKernelGGL(k_bubble_static_adjoint<N>, grid, dim3(256), 0, s, ntau, dims[0], dims[1], \
+                                      dims[2], nbatch, jchunk, gtr, chibar_r, cw, gbar)
   switch (n) {
-    case 1: hipLaunchKernelGGL(k_bubble_static_adjoint<1>, dim3(nb), dim3(256), 0, s, ntau, dims[0], dims[1], dims[2], gtr, chibar_r, cw, gbar); break;
-    case 2: hipLaunchKernelGGL(k_bubble_static_adjoint<2>, dim3(nb), dim3(256), 0, s, ntau, dims[0], dims[1], dims[2], gtr, chibar_r, cw, gbar); break;
-    case 3: hipLaunchKernelGGL(k_bubble_static_adjoint<3>, dim3(nb), dim3(256), 0, s, ntau, dims[0], dims[1], dims[2], gtr, chibar_r, cw, gbar); break;
+    case 1: TPRF_SA(1); break;
+    case 2: TPRF_SA(2); break;
+    case 3: TPRF_SA(3); break;
     default: return hipErrorInvalidValue;
   }
+#undef TPRF_SA
   return hipGetLastError();
 }
 
-hipError_t launch_dyson_adjoint(hipStream_t s, int nw, long nk, int n, const c2 *g, const c2 *gbar, c2 *ebar,
-                                c2 *part_e, double *mu_part, double *mubar, int nchunk) {
-  const long nb = (nk + 255) / 256;
+hipError_t launch_dyson_adjoint(hipStream_t s, int nw, long nk, int n, long nbatch, const c2 *g, const c2 *gbar,
+                                c2 *ebar, c2 *part_e, double *mu_part, double *mubar, int nchunk) {
+  const int nbk = (int)((nk + 255) / 256);
   const int wchunk = (nw + nchunk - 1) / nchunk;
-  const dim3 grid(nb, nchunk);
+  const dim3 grid(nbk * nbatch, nchunk);
+#define TPRF_DA(N) hipLaunchKernelGGL(k_dyson_adjoint<N>, grid, dim3(256), 0, s, nw, nk, nbatch, nbk, wchunk, g, gbar, \
+                                      part_e, mu_part)
   switch (n) {
-    case 1: hipLaunchKernelGGL(k_dyson_adjoint<1>, grid, dim3(256), 0, s, nw, nk, wchunk, g, gbar, part_e, mu_part); break;
-    case 2: hipLaunchKernelGGL(k_dyson_adjoint<2>, grid, dim3(256), 0, s, nw, nk, wchunk, g, gbar, part_e, mu_part); break;
-    case 3: hipLaunchKernelGGL(k_dyson_adjoint<3>, grid, dim3(256), 0, s, nw, nk, wchunk, g, gbar, part_e, mu_part); break;
+    case 1: TPRF_DA(1); break;
+    case 2: TPRF_DA(2); break;
+    case 3: TPRF_DA(3); break;
     default: return hipErrorInvalidValue;
   }
+#undef TPRF_DA
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const long stride = nk * n * n;
+  const long stride = nk * nbatch * n * n;
   hipLaunchKernelGGL(k_sum_chunks, dim3((stride + 255) / 256), dim3(256), 0, s, part_e, nchunk, stride, ebar);
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, mu_part, (int)(nb * nchunk), mubar);
+  hipLaunchKernelGGL(k_sum_partials, dim3(nbatch), dim3(256), 0, s, mu_part, nbk * nchunk, mubar);
   return hipGetLastError();
 }
 
-hipError_t launch_bubble_static(hipStream_t s, int ntau, const int dims[3], int n, const c2 *gtr, const double *cw,
-                                c2 *part, int nchunk, c2 *chi_r) {
-  const long nr = (long)dims[0] * dims[1] * dims[2];
+hipError_t launch_bubble_static(hipStream_t s, int ntau, const int dims[3], int n, long nbatch, const c2 *gtr,
+                                const double *cw, c2 *part, int nchunk, c2 *chi_r) {
+  const long nr = (long)dims[0] * dims[1] * dims[2] * nbatch;
   const long nb = (nr * n * n + 255) / 256;
   const int jchunk = (ntau + nchunk - 1) / nchunk;
   const dim3 grid(nb, nchunk);
+#define TPRF_SF(N) hipLaunchKernelGGL(k_bubble_static<N>, grid, dim3(256), 0, s, ntau, dims[0], dims[1], dims[2], \
+                                      nbatch, jchunk, gtr, cw, part)
   switch (n) {
-    case 1: hipLaunchKernelGGL(k_bubble_static<1>, grid, dim3(256), 0, s, ntau, dims[0], dims[1], dims[2], jchunk, gtr, cw, part); break;
-    case 2: hipLaunchKernelGGL(k_bubble_static<2>, grid, dim3(256), 0, s, ntau, dims[0], dims[1], dims[2], jchunk, gtr, cw, part); break;
-    case 3: hipLaunchKernelGGL(k_bubble_static<3>, grid, dim3(256), 0, s, ntau, dims[0], dims[1], dims[2], jchunk, gtr, cw, part); break;
+    case 1: TPRF_SF(1); break;
+    case 2: TPRF_SF(2); break;
+    case 3: TPRF_SF(3); break;
     default: return hipErrorInvalidValue;
   }
+#undef TPRF_SF
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const long stride = nr * n * n * n * n;

@@ -792,29 +792,42 @@ int tprf_solve_rpa_PH(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *chi0,
 
 // ---- differentiation -----------------------------------------------------
 
-int tprf_lattice_dyson_g_wk_adjoint(tprf_ctx *ctx, int n_iw, long nk, int norb, const tprf_c128 *g_wk,
+int tprf_lattice_dyson_g_wk_batched(tprf_ctx *ctx, double beta, int n_iw, const double *mu, long nbatch, long nk,
+                                    int norb, const tprf_c128 *e_k, tprf_c128 *g_wk) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (!(beta > 0) || n_iw < 1 || nk < 1 || nbatch < 1 || !mu || !e_k || !g_wk)
+    return fail(ctx, TPRF_ERR_ARG, "lattice_dyson_g_wk_batched: bad argument");
+  if (norb < 1 || norb > 6) return fail(ctx, TPRF_ERR_UNSUPPORTED, "lattice_dyson_g_wk_batched: norb must be 1..6");
+  return check_launch(ctx,
+                      tprf::launch_dyson(ctx->stream, beta, n_iw, 0.0, nk * nbatch, norb, (const c2 *)e_k, 0, nullptr,
+                                         (c2 *)g_wk, tprf::Status{ctx->d_status}, mu, nk),
+                      "dyson kernel");
+}
+
+int tprf_lattice_dyson_g_wk_adjoint(tprf_ctx *ctx, int n_iw, long nk, int norb, long nbatch, const tprf_c128 *g_wk,
                                     const tprf_c128 *gbar_wk, tprf_c128 *ebar_k, double *mubar) {
   if (!ctx) return TPRF_ERR_ARG;
-  if (n_iw < 1 || nk < 1 || !g_wk || !gbar_wk || !ebar_k || !mubar)
+  if (n_iw < 1 || nk < 1 || nbatch < 1 || !g_wk || !gbar_wk || !ebar_k || !mubar)
     return fail(ctx, TPRF_ERR_ARG, "lattice_dyson_g_wk_adjoint: bad argument");
   if (norb < 1 || norb > 3) return fail(ctx, TPRF_ERR_UNSUPPORTED, "lattice_dyson_g_wk_adjoint: norb must be 1..3");
   const int nw = 2 * n_iw;
   // frequency chunks: enough (k, chunk) threads to fill the chip
-  int nchunk = (int)((1L << 20) / (nk > 0 ? nk : 1));
+  const long ktot = nk * nbatch;
+  int nchunk = (int)((1L << 20) / ktot);
   if (nchunk < 1) nchunk = 1;
   if (nchunk > nw) nchunk = nw;
   if (nchunk > 256) nchunk = 256;
-  const long nb = (nk + 255) / 256;
-  const size_t part_bytes = sizeof(c2) * (size_t)nchunk * nk * norb * norb;
-  const size_t mu_bytes = sizeof(double) * (size_t)nchunk * nb;
+  const long nbk = (nk + 255) / 256;
+  const size_t part_bytes = sizeof(c2) * (size_t)nchunk * ktot * norb * norb;
+  const size_t mu_bytes = sizeof(double) * (size_t)nchunk * nbk * nbatch;
   void *w;
   int st = get_work(ctx, part_bytes + mu_bytes, &w);
   if (st) return st;
   c2 *part_e = (c2 *)w;
   double *mu_part = (double *)((char *)w + part_bytes);
   return check_launch(ctx,
-                      tprf::launch_dyson_adjoint(ctx->stream, nw, nk, norb, (const c2 *)g_wk, (const c2 *)gbar_wk,
-                                                 (c2 *)ebar_k, part_e, mu_part, mubar, nchunk),
+                      tprf::launch_dyson_adjoint(ctx->stream, nw, nk, norb, nbatch, (const c2 *)g_wk,
+                                                 (const c2 *)gbar_wk, (c2 *)ebar_k, part_e, mu_part, mubar, nchunk),
                       "dyson adjoint kernel");
 }
 
@@ -862,17 +875,17 @@ int tprf_fourier_wr_to_tr_adjoint(tprf_ctx *ctx, int statistic, double beta, int
                       "w2t adjoint kernel");
 }
 
-int tprf_chi0_static_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb,
+int tprf_chi0_static_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb, long nbatch,
                                  const tprf_c128 *g_tr, tprf_c128 *chi0_r) {
   if (!ctx) return TPRF_ERR_ARG;
-  if (!(beta > 0) || ntau < 18 || !dims_ok(dims) || !g_tr || !chi0_r)
+  if (!(beta > 0) || ntau < 18 || !dims_ok(dims) || nbatch < 1 || !g_tr || !chi0_r)
     return fail(ctx, TPRF_ERR_ARG, "chi0_static_from_grt_PH: bad argument");
   if (norb < 1 || norb > 3) return fail(ctx, TPRF_ERR_UNSUPPORTED, "chi0_static_from_grt_PH: norb must be 1..3");
   const int L = ntau - 1;
   const double *cw;
   int st;
   if ((st = get_static_weights(ctx, beta, L, &cw))) return st;
-  const long nr = (long)dims[0] * dims[1] * dims[2];
+  const long nr = (long)dims[0] * dims[1] * dims[2] * nbatch;
   const long n4 = (long)norb * norb * norb * norb;
   // tau chunks: enough (r, chunk) threads to fill the chip
   int nchunk = (int)((1L << 20) / (nr * norb * norb));
@@ -881,15 +894,16 @@ int tprf_chi0_static_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int
   void *w;
   if ((st = get_work(ctx, sizeof(c2) * (size_t)nchunk * nr * n4, &w))) return st;
   return check_launch(ctx,
-                      tprf::launch_bubble_static(ctx->stream, ntau, dims, norb, (const c2 *)g_tr, cw, (c2 *)w, nchunk,
-                                                 (c2 *)chi0_r),
+                      tprf::launch_bubble_static(ctx->stream, ntau, dims, norb, nbatch, (const c2 *)g_tr, cw, (c2 *)w,
+                                                 nchunk, (c2 *)chi0_r),
                       "static bubble kernel");
 }
 
 int tprf_chi0_static_from_grt_PH_adjoint(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb,
-                                         const tprf_c128 *g_tr, const tprf_c128 *chibar_r, tprf_c128 *gbar_tr) {
+                                         long nbatch, const tprf_c128 *g_tr, const tprf_c128 *chibar_r,
+                                         tprf_c128 *gbar_tr) {
   if (!ctx) return TPRF_ERR_ARG;
-  if (!(beta > 0) || ntau < 18 || !dims_ok(dims) || !g_tr || !chibar_r || !gbar_tr)
+  if (!(beta > 0) || ntau < 18 || !dims_ok(dims) || nbatch < 1 || !g_tr || !chibar_r || !gbar_tr)
     return fail(ctx, TPRF_ERR_ARG, "chi0_static_from_grt_PH_adjoint: bad argument");
   if (norb < 1 || norb > 3)
     return fail(ctx, TPRF_ERR_UNSUPPORTED, "chi0_static_from_grt_PH_adjoint: norb must be 1..3");
@@ -897,32 +911,34 @@ int tprf_chi0_static_from_grt_PH_adjoint(tprf_ctx *ctx, double beta, int ntau, c
   int st;
   if ((st = get_static_weights(ctx, beta, ntau - 1, &cw))) return st;
   return check_launch(ctx,
-                      tprf::launch_bubble_static_adjoint(ctx->stream, ntau, dims, norb, (const c2 *)g_tr,
+                      tprf::launch_bubble_static_adjoint(ctx->stream, ntau, dims, norb, nbatch, (const c2 *)g_tr,
                                                          (const c2 *)chibar_r, cw, (c2 *)gbar_tr),
                       "static bubble adjoint kernel");
 }
 
-int tprf_fourier_r_to_q(tprf_ctx *ctx, const int dims[3], const int q[3], long ntarget, const tprf_c128 *in_r,
-                        tprf_c128 *out_q) {
+int tprf_fourier_r_to_q(tprf_ctx *ctx, const int dims[3], const int q[3], long ntarget, long nbatch,
+                        const tprf_c128 *in_r, tprf_c128 *out_q) {
   if (!ctx) return TPRF_ERR_ARG;
-  if (!dims_ok(dims) || !q || ntarget < 1 || ntarget > (1L << 20) || !in_r || !out_q)
+  if (!dims_ok(dims) || !q || ntarget < 1 || ntarget > 65535 || nbatch < 1 || nbatch > 65535 || !in_r || !out_q)
     return fail(ctx, TPRF_ERR_ARG, "fourier_r_to_q: bad argument");
   int qq[3];
   for (int d = 0; d < 3; ++d) qq[d] = ((q[d] % dims[d]) + dims[d]) % dims[d];
-  return check_launch(ctx, tprf::launch_q_project(ctx->stream, (const c2 *)in_r, (c2 *)out_q, dims, qq, (int)ntarget),
+  return check_launch(ctx,
+                      tprf::launch_q_project(ctx->stream, (const c2 *)in_r, (c2 *)out_q, dims, qq, (int)ntarget,
+                                             nbatch),
                       "r->q kernel");
 }
 
-int tprf_fourier_r_to_q_adjoint(tprf_ctx *ctx, const int dims[3], const int q[3], long ntarget,
+int tprf_fourier_r_to_q_adjoint(tprf_ctx *ctx, const int dims[3], const int q[3], long ntarget, long nbatch,
                                 const tprf_c128 *gbar_q, tprf_c128 *gbar_r) {
   if (!ctx) return TPRF_ERR_ARG;
-  if (!dims_ok(dims) || !q || ntarget < 1 || ntarget > (1L << 20) || !gbar_q || !gbar_r)
+  if (!dims_ok(dims) || !q || ntarget < 1 || ntarget > 65535 || nbatch < 1 || !gbar_q || !gbar_r)
     return fail(ctx, TPRF_ERR_ARG, "fourier_r_to_q_adjoint: bad argument");
   int qq[3];
   for (int d = 0; d < 3; ++d) qq[d] = ((q[d] % dims[d]) + dims[d]) % dims[d];
   return check_launch(ctx,
                       tprf::launch_q_project_adjoint(ctx->stream, (const c2 *)gbar_q, (c2 *)gbar_r, dims, qq,
-                                                     (int)ntarget),
+                                                     (int)ntarget, nbatch),
                       "r->q adjoint kernel");
 }
 

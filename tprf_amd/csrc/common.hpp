@@ -93,8 +93,10 @@ struct Status {
 };
 
 // ---- launchers (kernels.hip) -------------------------------------------
+// mu_arr (optional, device): one mu per lattice of a batch folded into k, nk_inst
+// k-points each (k = inst * nk_inst + k_local); mu is then ignored
 hipError_t launch_dyson(hipStream_t s, double beta, int n_iw, double mu, long nk, int n, const c2 *ek, int sigma_kind,
-                        const c2 *sigma, c2 *g, Status st);
+                        const c2 *sigma, c2 *g, Status st, const double *mu_arr = nullptr, long nk_inst = 1);
 hipError_t launch_lattice_axis(hipStream_t s, int sign, const c2 *in, c2 *out, long outer, int len, long inner,
                                const FftDev &p, double scale);
 hipError_t launch_w2t(hipStream_t s, int statistic, double beta, int n_max, long ncol, const c2 *in, c2 *out,
@@ -119,18 +121,21 @@ bool w2t_pruned_supported(int M);
 // w2t.hip: transpose of the pruned w -> tau map (same M specialisations)
 hipError_t launch_w2t_adjoint(hipStream_t s, int statistic, double beta, int n_max, long ncol, const c2 *gbar_t,
                               c2 *gbar_w, int fitted, const TailFitDev &fit, const MatsDev &mt, const W2tDev &wd);
-// adjoint.hip: reverse-mode kernels of the static bubble + RPA at one momentum
-hipError_t launch_q_project(hipStream_t s, const c2 *chi_r, c2 *chi_q, const int dims[3], const int q[3], int ntarget);
+// adjoint.hip: reverse-mode kernels of the static bubble + RPA at one momentum.
+// nbatch lattices of equal dims are folded into the k / r index
+// (k = inst * nk + k_local); per-lattice outputs (chi_q, mubar) are [nbatch][...].
+hipError_t launch_q_project(hipStream_t s, const c2 *chi_r, c2 *chi_q, const int dims[3], const int q[3], int ntarget,
+                            long nbatch);
 hipError_t launch_q_project_adjoint(hipStream_t s, const c2 *chibar_q, c2 *chibar_r, const int dims[3], const int q[3],
-                                    int ntarget);
-hipError_t launch_bubble_static_adjoint(hipStream_t s, int ntau, const int dims[3], int n, const c2 *gtr,
+                                    int ntarget, long nbatch);
+hipError_t launch_bubble_static_adjoint(hipStream_t s, int ntau, const int dims[3], int n, long nbatch, const c2 *gtr,
                                         const c2 *chibar_r, const double *cw, c2 *gbar);
-// part_e: nchunk * nk * n^2 values; mu_part: nchunk * ceil(nk / 256) doubles
-hipError_t launch_dyson_adjoint(hipStream_t s, int nw, long nk, int n, const c2 *g, const c2 *gbar, c2 *ebar,
-                                c2 *part_e, double *mu_part, double *mubar, int nchunk);
-// part: nchunk * nr * n^4 values
-hipError_t launch_bubble_static(hipStream_t s, int ntau, const int dims[3], int n, const c2 *gtr, const double *cw,
-                                c2 *part, int nchunk, c2 *chi_r);
+// part_e: nchunk * nbatch * nk * n^2 values; mu_part: nbatch * nchunk * ceil(nk / 256) doubles
+hipError_t launch_dyson_adjoint(hipStream_t s, int nw, long nk, int n, long nbatch, const c2 *g, const c2 *gbar,
+                                c2 *ebar, c2 *part_e, double *mu_part, double *mubar, int nchunk);
+// part: nchunk * nbatch * nr * n^4 values
+hipError_t launch_bubble_static(hipStream_t s, int ntau, const int dims[3], int n, long nbatch, const c2 *gtr,
+                                const double *cw, c2 *part, int nchunk, c2 *chi_r);
 hipError_t launch_rpa_point(hipStream_t s, long npts, int n, const c2 *chi0, const c2 *U, const c2 *chibar, c2 *chi,
                             c2 *xbar, c2 *ubar, Status st);
 // kernels.hip: single-pass 3-D lattice FFT; hipErrorNotSupported for other dims

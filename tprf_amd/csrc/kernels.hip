@@ -13,12 +13,14 @@ namespace tprf {
 // a1 Dyson: G[w,k] = inverse((i w_n + mu) 1 - e_k - Sigma), one thread per
 // (w, k); Gauss-Jordan with partial pivoting on [M | 1] in registers.
 template <int N>
-__global__ void __launch_bounds__(256) k_dyson(double beta, int n_iw, double mu, long nk, const c2 *__restrict__ ek,
-                                               int sigma_kind, const c2 *__restrict__ sigma, c2 *__restrict__ g,
-                                               long npts, double *status) {
+__global__ void __launch_bounds__(256) k_dyson(double beta, int n_iw, double mu, const double *__restrict__ mu_arr,
+                                               long nk_inst, long nk, const c2 *__restrict__ ek, int sigma_kind,
+                                               const c2 *__restrict__ sigma, c2 *__restrict__ g, long npts,
+                                               double *status) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= npts) return;
   const long iw = idx / nk, k = idx - iw * nk;
+  if (mu_arr) mu = mu_arr[k / nk_inst];  // a batch of lattices folded into k
   const double wn = kPi * (double)(2 * (iw - n_iw) + 1) / beta;
   c2 A[N][N], B[N][N];
   const c2 *e = ek + k * N * N;
@@ -82,14 +84,15 @@ __global__ void __launch_bounds__(256) k_dyson(double beta, int n_iw, double mu,
 }
 
 hipError_t launch_dyson(hipStream_t s, double beta, int n_iw, double mu, long nk, int n, const c2 *ek, int sigma_kind,
-                        const c2 *sigma, c2 *g, Status st) {
+                        const c2 *sigma, c2 *g, Status st, const double *mu_arr, long nk_inst) {
   const long npts = 2L * n_iw * nk;
   const int bs = 256;
   const long nb = (npts + bs - 1) / bs;
   switch (n) {
 #define TPRF_DYSON_CASE(N)                                                                               \
   case N:                                                                                                \
-    hipLaunchKernelGGL(k_dyson<N>, dim3(nb), dim3(bs), 0, s, beta, n_iw, mu, nk, ek, sigma_kind, sigma, g, \
+    hipLaunchKernelGGL(k_dyson<N>, dim3(nb), dim3(bs), 0, s, beta, n_iw, mu, mu_arr, nk_inst, nk, ek, sigma_kind, \
+                       sigma, g,                                                                         \
                        npts, st.d);                                                                      \
     break;
     TPRF_DYSON_CASE(1)

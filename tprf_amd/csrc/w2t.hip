@@ -40,6 +40,7 @@ __device__ __forceinline__ void w2t_butterflies(c2 *v, int j, int Ns, const c2 *
 }
 
 constexpr int kW2tThreads = 256;
+constexpr int kFitSplit = 4;  // partial sums per fitted coefficient
 
 template <int STAT, int M>
 __global__ void __launch_bounds__(kW2tThreads, 2) k_w2t_pruned(double beta, int n_max, long ncol, const c2 *__restrict__ in,
@@ -51,6 +52,8 @@ __global__ void __launch_bounds__(kW2tThreads, 2) k_w2t_pruned(double beta, int 
   using Plan = W2tPlan<M>;
   __shared__ c2 buf[M * C];
   __shared__ c2 s_coef[C][kMaxOrder];
+  __shared__ c2 s_cpart[C][kMaxOrder][kFitSplit];
+  __shared__ double s_res[C][kMaxFit];
   __shared__ c2 s_a[C][3];
   __shared__ c2 s_m1[C];
   __shared__ c2 s_tw[M];
@@ -83,22 +86,40 @@ __global__ void __launch_bounds__(kW2tThreads, 2) k_w2t_pruned(double beta, int 
     if (e < 0) e += M;
     return buf[e * C + cc];
   };
-  // 2. tail moments (least-squares fit, or given)
+  // 2. tail moments (least-squares fit, or given).  The fit is spread over
+  //    (column, coefficient, quarter of the fitted points) threads and the
+  //    residual over (column, fitted point) threads, so no thread walks a long
+  //    chain of table loads.
   if (moments == nullptr) {
-    for (int t = tid; t < cn * fit.ncoef; t += NT) {
-      const int cc = t / fit.ncoef, q = t - cc * fit.ncoef;
+    const int nc = fit.ncoef, nf = fit.n_fit;
+    const int fq = (nf + kFitSplit - 1) / kFitSplit;
+    for (int t = tid; t < cn * nc * kFitSplit; t += NT) {
+      const int part = t % kFitSplit, cq = t / kFitSplit;
+      const int cc = cq / nc, q = cq - cc * nc;
+      const int f0 = part * fq, f1 = (f0 + fq) < nf ? (f0 + fq) : nf;
       c2 s = c2{0.0, 0.0};
-      for (int f = 0; f < fit.n_fit; ++f) s = cadd(s, cmul(fit.pinv[q * fit.n_fit + f], fitval(f, cc)));
+      for (int f = f0; f < f1; ++f) s = cadd(s, cmul(fit.pinv[q * nf + f], fitval(f, cc)));
+      s_cpart[cc][q][part] = s;
+    }
+    __syncthreads();
+    for (int t = tid; t < cn * nc; t += NT) {
+      const int cc = t / nc, q = t - cc * nc;
+      c2 s = s_cpart[cc][q][0];
+#pragma unroll
+      for (int part = 1; part < kFitSplit; ++part) s = cadd(s, s_cpart[cc][q][part]);
       s_coef[cc][q] = s;
+    }
+    __syncthreads();
+    for (int t = tid; t < cn * nf; t += NT) {
+      const int cc = t / nf, f = t - cc * nf;
+      c2 v = fitval(f, cc);
+      for (int q = 0; q < nc; ++q) v = csub(v, cmul(fit.vander[f * nc + q], s_coef[cc][q]));
+      s_res[cc][f] = cabs2(v);
     }
     __syncthreads();
     if (tid < cn) {
       double r2 = 0.0;
-      for (int f = 0; f < fit.n_fit; ++f) {
-        c2 v = fitval(f, tid);
-        for (int q = 0; q < fit.ncoef; ++q) v = csub(v, cmul(fit.vander[f * fit.ncoef + q], s_coef[tid][q]));
-        r2 += cabs2(v);
-      }
+      for (int f = 0; f < nf; ++f) r2 += s_res[tid][f];
       const c2 m1 = cscale(s_coef[tid][1], fit.wpow[1]);
       Poles<STAT>::coef(m1, cscale(s_coef[tid][2], fit.wpow[2]), cscale(s_coef[tid][3], fit.wpow[3]), s_a[tid]);
       s_m1[tid] = m1;
