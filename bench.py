@@ -97,6 +97,97 @@ def cpu_baseline(wl, chain_result):
     return base, parity
 
 
+def _timed(f, reps, warm=2):
+    """Mean wall time (ms) of f() over reps calls, the stream drained on both sides."""
+    import torch
+    for _ in range(warm):
+        f()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        f()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t) * 1e3 / reps
+
+
+def grad_leg(reps=20):
+    """grad/fwd at the differentiation workload: single-band square lattice 64x64 k,
+    beta 20, 512 fermionic frequencies, t = 1, t' = -0.3, mu = -0.5, U = 2.0,
+    L = Re tr chi_RPA(Q = (pi, pi), i nu = 0); wall(forward + backward) / wall(forward)."""
+    import torch
+    from tprf_amd import autograd as A
+    dims, q = (64, 64), (32, 32, 0)
+    spec = A.StaticRPA(20.0, 512, dims, q, norb=1)
+    x0 = {"t": 1.0, "tp": -0.3, "mu": -0.5, "U": 2.0}
+    leaves = {k: torch.tensor(v, dtype=torch.float64, device="cuda", requires_grad=True) for k, v in x0.items()}
+
+    def chi():
+        return A.chi_rpa_static(A.square_e_k(leaves["t"], leaves["tp"], dims), leaves["mu"],
+                                A.hubbard_U(leaves["U"]), spec)
+
+    def fwd():
+        with torch.no_grad():
+            return A.trace_loss(chi())
+
+    def fwd_bwd():
+        for v in leaves.values():
+            v.grad = None
+        A.trace_loss(chi()).backward()
+
+    tf = _timed(fwd, reps)
+    tfb = _timed(fwd_bwd, reps)
+    return {"workload": "square 64x64 k, beta 20, 512 fermionic freqs, t 1, t' -0.3, mu -0.5, U 2.0, "
+                        "L = Re tr chi_RPA((pi,pi), 0)",
+            "fwd_ms": tf, "fwd_bwd_ms": tfb, "grad_over_fwd": tfb / tf,
+            "grad": {k: float(v.grad) for k, v in leaves.items()},
+            "L": float(fwd())}
+
+
+def batched_leg(reps=3):
+    """batched/plain at the batched forward+adjoint workload: 48 two-orbital square
+    lattices, 48x48 k, 256 fermionic frequencies, beta 20, t' ~ U[-0.4, 0],
+    mu ~ U[-1, 1], inter-orbital hopping ~ U[0, 0.3], U ~ U[1, 3], J = U/6,
+    chi_RPA(Q = (pi, pi), 0) and dL/d(e_k, mu, U) for every set.  Plain: a loop of
+    the differentiable per-lattice call; batched: one HIP-graph replay."""
+    import numpy as np
+    import torch
+    from tprf_amd import autograd as A
+    B, dims, q = 48, (48, 48), (24, 24, 0)
+    g = np.random.default_rng(5)
+    tp = torch.tensor(g.uniform(-0.4, 0.0, B), dtype=torch.float64, device="cuda")
+    mu = torch.tensor(g.uniform(-1.0, 1.0, B), dtype=torch.float64, device="cuda")
+    v = torch.tensor(g.uniform(0.0, 0.3, B), dtype=torch.float64, device="cuda")
+    U = torch.tensor(g.uniform(1.0, 3.0, B), dtype=torch.float64, device="cuda")
+    e_k = A.two_orbital_square_e_k(tp, v, dims)
+    Uk = A.kanamori_spin_U(U, U / 6)
+    one = A.StaticRPA(20.0, 256, dims, q, norb=2)
+    res = []
+
+    def plain():
+        res.clear()
+        for i in range(B):
+            e1 = e_k[i].clone().requires_grad_(True)
+            m1 = mu[i].clone().requires_grad_(True)
+            U1 = Uk[i].clone().requires_grad_(True)
+            c = A.chi_rpa_static(e1, m1, U1, one)
+            A.trace_loss(c).backward()
+            res.append((c.detach(), e1.grad, m1.grad, U1.grad))
+
+    batch = A.StaticRPA(20.0, 256, dims, q, norb=2, nbatch=B)
+    tp_ms = _timed(plain, reps, warm=1)
+    tb_ms = _timed(lambda: batch.graphed_grad(e_k, mu, Uk), reps + 2, warm=1)
+    chi, ebar, mubar, Ubar = batch.graphed_grad(e_k, mu, Uk)
+    torch.cuda.synchronize()
+    rel = lambda a, b: float((a - b).abs().max() / b.abs().max())
+    diff = max(max(rel(chi[i], res[i][0]), rel(ebar[i], res[i][1]), rel(Ubar[i], res[i][3]),
+                   abs(float(mubar[i] - res[i][2])) / abs(float(res[i][2]))) for i in range(B))
+    del batch
+    return {"workload": "48 two-orbital square lattices 48x48 k, 256 fermionic freqs, beta 20, forward + adjoint "
+                        "of chi_RPA((pi,pi), 0)",
+            "plain_ms": tp_ms, "batched_ms": tb_ms, "batched_over_plain": tp_ms / tb_ms,
+            "max_rel_diff_batched_vs_plain": diff}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +195,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-extra", action="store_true", help="skip the grad/fwd and batched/plain legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this path shards by the "
@@ -182,6 +274,12 @@ def main():
         base, parity = cpu_baseline(wl, got)
         out["cpu_baseline"] = base
         out["parity_vs_oracle"] = parity
+    if not args.no_extra:
+        del ch
+        torch.cuda.empty_cache()
+        out["grad_fwd"] = grad_leg()
+        torch.cuda.empty_cache()
+        out["batched"] = batched_leg()
     print(json.dumps(out))
 
 

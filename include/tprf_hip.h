@@ -60,6 +60,14 @@ int tprf_ctx_set_stream(tprf_ctx *ctx, void *hip_stream); /* hipStream_t; NULL =
  * routes every call through the generic full-length kernels instead of the
  * pruned specialisations (same results to rounding; used to cross-check). */
 #define TPRF_OPT_GENERIC_KERNELS 1
+/* TPRF_OPT_BUBBLE_VARIANT = v >= 0 fixes the workgroup shape of the fused
+ * bubble kernel (same results); -1 (default) autotunes it: the first call per
+ * shape times every shape and keeps the fastest (not while the stream is being
+ * captured into a graph).                                                   */
+#define TPRF_OPT_BUBBLE_VARIANT 2
+/* TPRF_OPT_W2T_VARIANT: the same for the workgroup width of the pruned
+ * omega -> tau kernel.                                                      */
+#define TPRF_OPT_W2T_VARIANT 3
 int tprf_ctx_set_option(tprf_ctx *ctx, int option, int value);
 const char *tprf_ctx_last_error(const tprf_ctx *ctx);
 /* Largest RMS tail-fit residual of the fitted omega->tau calls since the last

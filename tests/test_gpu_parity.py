@@ -174,6 +174,14 @@ def test_wr_to_tr_pruned_and_generic(L, oracle, stat, n_max, nt):
         finally:
             _lib.call("tprf_ctx_set_option", c, 1, 0)
         assert relerr(host(got.data), ref) <= TOL, f"generic={generic}"
+    # both workgroup widths of the pruned kernel (option 3 fixes it, -1 restores tuning)
+    for variant in (0, 1):
+        _lib.call("tprf_ctx_set_option", c, 3, variant)
+        try:
+            got = L.fourier_wr_to_tr(g, nt=nt)
+        finally:
+            _lib.call("tprf_ctx_set_option", c, 3, -1)
+        assert relerr(host(got.data), ref) <= TOL, f"variant={variant}"
 
 
 def test_wr_to_tr_errors(L):
@@ -264,6 +272,14 @@ def test_chi0_wr_pruned_pair_kernel(L, oracle, n, n_iw, nb, dims):
         finally:
             _lib.call("tprf_ctx_set_option", c, 1, 0)
         assert relerr(host(got.data), ref) <= TOL, f"generic={generic}"
+    # every workgroup shape the autotuner may pick (option 2 fixes it, -1 restores tuning)
+    for variant in range(3):
+        _lib.call("tprf_ctx_set_option", c, 2, variant)
+        try:
+            got = L.chi0_wr_from_grt_PH(gtr_gf(dims, beta, gt), nw=nb)
+        finally:
+            _lib.call("tprf_ctx_set_option", c, 2, -1)
+        assert relerr(host(got.data), ref) <= TOL, f"variant={variant}"
 
 
 def test_chi0_w0r_trapz(L, oracle, gtr_case):

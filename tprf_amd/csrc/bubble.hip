@@ -24,17 +24,23 @@
 
 namespace tprf {
 
-template <int N, int R1, int R2, int CG>
-__global__ void __launch_bounds__(256) k_bubble_pair(double beta, int L, int nb, int d0, int d1, int d2,
-                                                     const c2 *__restrict__ gtr, c2 *__restrict__ out, BubbleDev bd,
-                                                     Vinv8 vinv) {
-  constexpr int NT = 256;
+// AB = true: a workgroup owns the CG = n^2 columns of one orbital pair (a, b)
+// (abcd with c, d free) and stages only the n entries G_da(r), d < n, and
+// G_bc(-r), c < n, per tau point; AB = false: CG consecutive columns, all n^2
+// entries of G(r) and G(-r) staged.
+template <int N, int R1, int R2, int CG, int NT, bool AB>
+__global__ void __launch_bounds__(NT) k_bubble_pair(double beta, int L, int nb, int d0, int d1, int d2,
+                                                    const c2 *__restrict__ gtr, c2 *__restrict__ out, BubbleDev bd,
+                                                    Vinv8 vinv) {
   constexpr int P = R1 * R2;
   constexpr int n2 = N * N, n4 = n2 * n2;
+  static_assert(!AB || CG == n2, "one orbital pair per workgroup");
+  static_assert(NT >= P, "the outer twiddles of a step are staged one per thread");
   constexpr int NG = (n4 + CG - 1) / CG;
   constexpr int U2 = (CG * R1 + NT - 1) / NT;
   constexpr int TLD = R2 + 1;  // padded LDS row of the transpose
-  constexpr int NSTAGE = P * n2;
+  constexpr int NE = AB ? N : n2;  // staged entries of G per tau point and side
+  constexpr int NSTAGE = P * NE;
   constexpr int PF = (NSTAGE + NT - 1) / NT;
   static_assert(CG * R1 * TLD >= CG * P, "transpose buffer reused for the final exchange");
 
@@ -54,6 +60,7 @@ __global__ void __launch_bounds__(256) k_bubble_pair(double beta, int L, int nb,
   const long mr = minus_r(r, d0, d1, d2);
   const int q0 = grp * CG;
   const int cn = (n4 - q0) < CG ? (n4 - q0) : CG;
+  const int a0 = q0 / (n2 * N), b0 = (q0 / n2) % N;  // the orbital pair of an AB workgroup
   const int N1 = L / P;
   const double dtau = beta / (double)L;
 
@@ -76,16 +83,19 @@ __global__ void __launch_bounds__(256) k_bubble_pair(double beta, int L, int nb,
 
   for (int t = tid; t < P; t += NT) sTwp[t] = bd.twp[t];
   c2 pa[PF], pb[PF], pw = c2{0.0, 0.0};
+  // loads of step j1 + 1 are issued right after step j1's tile is staged
   auto fetch = [&](int j1) {
     if (tid < P) pw = bd.tw1[(long)j1 * P + tid];
 #pragma unroll
     for (int i = 0; i < PF; ++i) {
       const int t = tid + i * NT;
       if (t < NSTAGE) {
-        const int j2 = t / n2, ab = t - j2 * n2;
+        const int j2 = t / NE, e = t - j2 * NE;
         const long tau = (long)N1 * j2 + j1;
-        pa[i] = gtr[(tau * nr + r) * n2 + ab];
-        pb[i] = gtr[((L - tau) * nr + mr) * n2 + ab];
+        // AB: e = d of G_da(r) (a = a0) and e = c of G_bc(-r) (b = b0)
+        const int ea = AB ? e * N + a0 : e, eb = AB ? b0 * N + e : e;
+        pa[i] = gtr[(tau * nr + r) * n2 + ea];
+        pb[i] = gtr[((L - tau) * nr + mr) * n2 + eb];
       }
     }
   };
@@ -119,7 +129,8 @@ __global__ void __launch_bounds__(256) k_bubble_pair(double beta, int L, int nb,
 #pragma unroll
       for (int k = 0; k < R1; ++k) {
         const int j2 = R2 * k + s;
-        v[k] = cmul(sA[j2 * n2 + dd * N + aa], sB[j2 * n2 + bb * N + cc]);
+        v[k] = AB ? cmul(sA[j2 * NE + dd], sB[j2 * NE + cc])
+                  : cmul(sA[j2 * n2 + dd * N + aa], sB[j2 * n2 + bb * N + cc]);
       }
       SmallDft<R1, 1>::run(v, nullptr, 0);
 #pragma unroll
@@ -192,25 +203,49 @@ __global__ void __launch_bounds__(256) k_bubble_pair(double beta, int L, int nb,
   }
 }
 
-template <int N, int R1, int R2, int CG>
+template <int N, int R1, int R2, int CG, int NT, bool AB>
 static hipError_t launch_one(hipStream_t s, double beta, int L, int nb, const int dims[3], const c2 *gtr, c2 *out,
                              const BubbleDev &bd, const Vinv8 &vinv) {
   constexpr int n4 = N * N * N * N;
   constexpr int NG = (n4 + CG - 1) / CG;
   const long nb_blocks = (long)bd.ncanon * NG;
-  hipLaunchKernelGGL((k_bubble_pair<N, R1, R2, CG>), dim3(nb_blocks), dim3(256), 0, s, beta, L, nb, dims[0], dims[1],
-                     dims[2], gtr, out, bd, vinv);
+  hipLaunchKernelGGL((k_bubble_pair<N, R1, R2, CG, NT, AB>), dim3(nb_blocks), dim3(NT), 0, s, beta, L, nb, dims[0],
+                     dims[1], dims[2], gtr, out, bd, vinv);
   return hipGetLastError();
 }
 
 bool bubble_pair_supported(int n, int P) { return (n == 2 || n == 3) && (P == 64 || P == 128); }
 
+int bubble_pair_variants() { return 3; }
+
+// Workgroup shapes (same results, picked by the autotuner): variant 0 owns one
+// orbital pair; 1 is a narrower or differently grouped shape; 2 stages every G
+// entry for CG = n^4 / 3 (n = 3) or n^4 (n = 2) columns.
 hipError_t launch_bubble_pair(hipStream_t s, double beta, int L, int nb, const int dims[3], int n, const c2 *gtr,
-                              c2 *out, const BubbleDev &bd, const Vinv8 &vinv) {
-  if (n == 3 && bd.P == 128) return launch_one<3, 16, 8, 27>(s, beta, L, nb, dims, gtr, out, bd, vinv);
-  if (n == 3 && bd.P == 64) return launch_one<3, 8, 8, 27>(s, beta, L, nb, dims, gtr, out, bd, vinv);
-  if (n == 2 && bd.P == 128) return launch_one<2, 16, 8, 16>(s, beta, L, nb, dims, gtr, out, bd, vinv);
-  if (n == 2 && bd.P == 64) return launch_one<2, 8, 8, 16>(s, beta, L, nb, dims, gtr, out, bd, vinv);
+                              c2 *out, const BubbleDev &bd, const Vinv8 &vinv, int variant) {
+#define TPRF_BP(N_, R1_, R2_, CG_, NT_, AB_) \
+  return launch_one<N_, R1_, R2_, CG_, NT_, AB_>(s, beta, L, nb, dims, gtr, out, bd, vinv)
+  if (n == 3 && bd.P == 128) {
+    if (variant == 0) TPRF_BP(3, 16, 8, 9, 256, true);
+    if (variant == 1) TPRF_BP(3, 16, 8, 9, 128, true);
+    TPRF_BP(3, 16, 8, 27, 256, false);
+  }
+  if (n == 3 && bd.P == 64) {
+    if (variant == 0) TPRF_BP(3, 8, 8, 9, 256, true);
+    if (variant == 1) TPRF_BP(3, 8, 8, 9, 128, true);
+    TPRF_BP(3, 8, 8, 27, 256, false);
+  }
+  if (n == 2 && bd.P == 128) {
+    if (variant == 0) TPRF_BP(2, 16, 8, 4, 128, true);
+    if (variant == 1) TPRF_BP(2, 16, 8, 8, 128, false);
+    TPRF_BP(2, 16, 8, 16, 256, false);
+  }
+  if (n == 2 && bd.P == 64) {
+    if (variant == 0) TPRF_BP(2, 8, 8, 4, 64, true);
+    if (variant == 1) TPRF_BP(2, 8, 8, 4, 128, true);
+    TPRF_BP(2, 8, 8, 16, 256, false);
+  }
+#undef TPRF_BP
   return hipErrorNotSupported;
 }
 

@@ -63,6 +63,10 @@ struct tprf_ctx {
   std::map<std::tuple<int, double, int, int>, c2 *> dtail;  // out-of-mesh pole model in tau
   void *work = nullptr;                                   // grow-only scratch
   size_t work_bytes = 0;
+  int bubble_variant = -1;                                // -1: autotuned per shape
+  std::map<std::tuple<int, int, int, int, long>, int> bubble_tuned;
+  int w2t_variant = -1;
+  std::map<std::tuple<int, int, int, int, long>, int> w2t_tuned;
 };
 
 namespace {
@@ -507,6 +511,51 @@ int check_launch(tprf_ctx *ctx, hipError_t e, const char *what) {
   return TPRF_OK;
 }
 
+// Kernel-variant autotuner.  Every variant of a launch writes the same output;
+// on the first call per shape key all variants are timed with HIP events on
+// the context's stream (two launches each, the second timed) and the fastest
+// is cached.  While the stream is being captured into a graph nothing can be
+// timed, so variant 0 is used for that call and nothing is cached.
+template <class Key, class F>
+int tuned_launch(tprf_ctx *ctx, std::map<Key, int> &cache, const Key &key, int nvar, int forced, F launch,
+                 const char *what) {
+  int variant = forced;
+  if (variant < 0) {
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+      variant = it->second;
+    } else {
+      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+      TPRF_HIP_TRY(ctx, hipStreamIsCapturing(ctx->stream, &cs), "hipStreamIsCapturing");
+      if (cs != hipStreamCaptureStatusNone || nvar == 1) {
+        variant = 0;
+      } else {
+        hipEvent_t e0, e1;
+        TPRF_HIP_TRY(ctx, hipEventCreate(&e0), "hipEventCreate");
+        TPRF_HIP_TRY(ctx, hipEventCreate(&e1), "hipEventCreate");
+        float best = 1e30f;
+        int st = TPRF_OK;
+        for (int v = 0; v < nvar && st == TPRF_OK; ++v) {
+          if ((st = check_launch(ctx, launch(v), what))) break;  // warm-up
+          hipError_t e = hipEventRecord(e0, ctx->stream);
+          if (e == hipSuccess) e = launch(v);
+          if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+          if (e == hipSuccess) e = hipEventSynchronize(e1);
+          float ms = 0;
+          if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+          if (e != hipSuccess) { st = hip_fail(ctx, e, what); break; }
+          if (ms < best) { best = ms; variant = v; }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (st) return st;
+        cache.emplace(key, variant);
+      }
+    }
+  }
+  return check_launch(ctx, launch(variant), what);
+}
+
 int lattice_fft(tprf_ctx *ctx, long nw, const int dims[3], long ntarget, const c2 *in, c2 *out, int sign,
                 bool scaled) {
   if (!ctx) return TPRF_ERR_ARG;
@@ -596,6 +645,16 @@ int tprf_ctx_set_option(tprf_ctx *ctx, int option, int value) {
     ctx->generic_kernels = value != 0;
     return TPRF_OK;
   }
+  if (option == TPRF_OPT_W2T_VARIANT) {
+    if (value < -1 || value >= tprf::w2t_pruned_variants()) return fail(ctx, TPRF_ERR_ARG, "bad w2t variant");
+    ctx->w2t_variant = value;
+    return TPRF_OK;
+  }
+  if (option == TPRF_OPT_BUBBLE_VARIANT) {
+    if (value < -1 || value >= tprf::bubble_pair_variants()) return fail(ctx, TPRF_ERR_ARG, "bad bubble variant");
+    ctx->bubble_variant = value;
+    return TPRF_OK;
+  }
   return fail(ctx, TPRF_ERR_ARG, "unknown option");
 }
 
@@ -678,9 +737,12 @@ int tprf_fourier_wr_to_tr(tprf_ctx *ctx, int statistic, double beta, int n_max, 
     const c2 *pinv;
     if ((st = get_pole_inv(ctx, statistic, beta, n_max, nmesh, &pinv))) return st;
     tprf::W2tDev wd{M, L / M, pL->tw, pM->tw, pinv, nullptr};
-    return check_launch(ctx,
-                        tprf::launch_w2t_pruned(ctx->stream, statistic, beta, n_max, ncol, (const c2 *)in, (c2 *)out,
-                                                (const c2 *)moments, fit, *mt, wd, tprf::Status{ctx->d_status}),
+    auto launch = [&](int v) {
+      return tprf::launch_w2t_pruned(ctx->stream, statistic, beta, n_max, ncol, (const c2 *)in, (c2 *)out,
+                                     (const c2 *)moments, fit, *mt, wd, tprf::Status{ctx->d_status}, v);
+    };
+    const auto key = std::make_tuple(statistic, M, L, moments ? 1 : 0, ncol);
+    return tuned_launch(ctx, ctx->w2t_tuned, key, tprf::w2t_pruned_variants(), ctx->w2t_variant, launch,
                         "w2t pruned kernel");
   }
   const int ld = (L + 1) | 1;
@@ -744,9 +806,12 @@ int tprf_chi0_wr_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dim
   if (!ctx->generic_kernels && P && tprf::bubble_pair_supported(norb, P)) {
     const tprf::BubbleDev *bd;
     if ((st = get_bubble(ctx, beta, L, nw, dims, P, &bd))) return st;
-    return check_launch(ctx,
-                        tprf::launch_bubble_pair(ctx->stream, beta, L, nw, dims, norb, (const c2 *)g_tr,
-                                                 (c2 *)chi0_wr, *bd, ctx->vinv),
+    auto launch = [&](int v) {
+      return tprf::launch_bubble_pair(ctx->stream, beta, L, nw, dims, norb, (const c2 *)g_tr, (c2 *)chi0_wr, *bd,
+                                      ctx->vinv, v);
+    };
+    const auto key = std::make_tuple(norb, P, L, nw, (long)dims[0] * dims[1] * dims[2]);
+    return tuned_launch(ctx, ctx->bubble_tuned, key, tprf::bubble_pair_variants(), ctx->bubble_variant, launch,
                         "bubble pair kernel");
   }
   const int ld = (L + 1) | 1;

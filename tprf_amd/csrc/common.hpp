@@ -110,14 +110,18 @@ hipError_t launch_chi0_w0r(hipStream_t s, double beta, int ntau, const int dims[
 hipError_t launch_chi_w0r(hipStream_t s, double beta, int ntau, long nr, int n, const c2 *chi_tr, c2 *out);
 hipError_t launch_rpa(hipStream_t s, long npts, int n, const c2 *chi0, const c2 *U, c2 *chi, Status st);
 // bubble.hip: returns hipErrorNotSupported when (n, P) has no specialisation
+// variant < bubble_pair_variants() selects the workgroup shape (same results)
 hipError_t launch_bubble_pair(hipStream_t s, double beta, int L, int nb, const int dims[3], int n, const c2 *gtr,
-                              c2 *out, const BubbleDev &bd, const Vinv8 &vinv);
+                              c2 *out, const BubbleDev &bd, const Vinv8 &vinv, int variant);
 bool bubble_pair_supported(int n, int P);
+int bubble_pair_variants();
 // w2t.hip: returns hipErrorNotSupported when M has no specialisation
+// variant < w2t_pruned_variants() selects the workgroup width (same results)
 hipError_t launch_w2t_pruned(hipStream_t s, int statistic, double beta, int n_max, long ncol, const c2 *in, c2 *out,
                              const c2 *moments, const TailFitDev &fit, const MatsDev &mt, const W2tDev &wd,
-                             Status st);
+                             Status st, int variant);
 bool w2t_pruned_supported(int M);
+int w2t_pruned_variants();
 // w2t.hip: transpose of the pruned w -> tau map (same M specialisations)
 hipError_t launch_w2t_adjoint(hipStream_t s, int statistic, double beta, int n_max, long ncol, const c2 *gbar_t,
                               c2 *gbar_w, int fitted, const TailFitDev &fit, const MatsDev &mt, const W2tDev &wd);

@@ -42,11 +42,13 @@ __device__ __forceinline__ void w2t_butterflies(c2 *v, int j, int Ns, const c2 *
 constexpr int kW2tThreads = 256;
 constexpr int kFitSplit = 4;  // partial sums per fitted coefficient
 
-template <int STAT, int M>
-__global__ void __launch_bounds__(kW2tThreads, 2) k_w2t_pruned(double beta, int n_max, long ncol, const c2 *__restrict__ in,
-                                                    c2 *__restrict__ out, const c2 *__restrict__ moments,
-                                                    TailFitDev fit, MatsDev mt, W2tDev wd, double *status) {
-  constexpr int NT = kW2tThreads;
+// NT threads per workgroup: C = NT / (M / 8) columns, i.e. C * 16 contiguous
+// bytes of every row written per workgroup.
+template <int STAT, int M, int NT>
+__global__ void __launch_bounds__(NT, 512 / NT) k_w2t_pruned(double beta, int n_max, long ncol,
+                                                             const c2 *__restrict__ in, c2 *__restrict__ out,
+                                                             const c2 *__restrict__ moments, TailFitDev fit,
+                                                             MatsDev mt, W2tDev wd, double *status) {
   constexpr int TPC = M / 8;  // threads per column
   constexpr int C = NT / TPC;  // columns per workgroup
   using Plan = W2tPlan<M>;
@@ -217,35 +219,43 @@ __global__ void __launch_bounds__(kW2tThreads, 2) k_w2t_pruned(double beta, int 
 
 bool w2t_pruned_supported(int M) { return M == 64 || M == 128 || M == 256 || M == 512 || M == 1024; }
 
-template <int STAT, int M>
+template <int STAT, int M, int NT>
 static hipError_t launch_m(hipStream_t s, double beta, int n_max, long ncol, const c2 *in, c2 *out, const c2 *moments,
                            const TailFitDev &fit, const MatsDev &mt, const W2tDev &wd, Status st) {
-  constexpr int C = kW2tThreads / (M / 8);
+  constexpr int C = NT / (M / 8);
   const long nb = (ncol + C - 1) / C;
-  hipLaunchKernelGGL((k_w2t_pruned<STAT, M>), dim3(nb), dim3(kW2tThreads), 0, s, beta, n_max, ncol, in, out, moments, fit, mt,
-                     wd, st.d);
+  hipLaunchKernelGGL((k_w2t_pruned<STAT, M, NT>), dim3(nb), dim3(NT), 0, s, beta, n_max, ncol, in, out, moments, fit,
+                     mt, wd, st.d);
   return hipGetLastError();
 }
 
-template <int STAT>
+template <int STAT, int NT>
 static hipError_t launch_stat(hipStream_t s, double beta, int n_max, long ncol, const c2 *in, c2 *out,
                               const c2 *moments, const TailFitDev &fit, const MatsDev &mt, const W2tDev &wd,
                               Status st) {
   switch (wd.M) {
-    case 64: return launch_m<STAT, 64>(s, beta, n_max, ncol, in, out, moments, fit, mt, wd, st);
-    case 128: return launch_m<STAT, 128>(s, beta, n_max, ncol, in, out, moments, fit, mt, wd, st);
-    case 256: return launch_m<STAT, 256>(s, beta, n_max, ncol, in, out, moments, fit, mt, wd, st);
-    case 512: return launch_m<STAT, 512>(s, beta, n_max, ncol, in, out, moments, fit, mt, wd, st);
-    case 1024: return launch_m<STAT, 1024>(s, beta, n_max, ncol, in, out, moments, fit, mt, wd, st);
+    case 64: return launch_m<STAT, 64, NT>(s, beta, n_max, ncol, in, out, moments, fit, mt, wd, st);
+    case 128: return launch_m<STAT, 128, NT>(s, beta, n_max, ncol, in, out, moments, fit, mt, wd, st);
+    case 256: return launch_m<STAT, 256, NT>(s, beta, n_max, ncol, in, out, moments, fit, mt, wd, st);
+    case 512: return launch_m<STAT, 512, NT>(s, beta, n_max, ncol, in, out, moments, fit, mt, wd, st);
+    case 1024: return launch_m<STAT, 1024, NT>(s, beta, n_max, ncol, in, out, moments, fit, mt, wd, st);
     default: return hipErrorNotSupported;
   }
 }
 
+int w2t_pruned_variants() { return 2; }
+
+// variant 0: 256 threads per workgroup; 1: 512 threads (twice the columns per
+// workgroup, twice the contiguous bytes per written row)
 hipError_t launch_w2t_pruned(hipStream_t s, int statistic, double beta, int n_max, long ncol, const c2 *in, c2 *out,
                              const c2 *moments, const TailFitDev &fit, const MatsDev &mt, const W2tDev &wd,
-                             Status st) {
-  if (statistic == 0) return launch_stat<0>(s, beta, n_max, ncol, in, out, moments, fit, mt, wd, st);
-  return launch_stat<1>(s, beta, n_max, ncol, in, out, moments, fit, mt, wd, st);
+                             Status st, int variant) {
+  if (variant == 1) {
+    if (statistic == 0) return launch_stat<0, 512>(s, beta, n_max, ncol, in, out, moments, fit, mt, wd, st);
+    return launch_stat<1, 512>(s, beta, n_max, ncol, in, out, moments, fit, mt, wd, st);
+  }
+  if (statistic == 0) return launch_stat<0, 256>(s, beta, n_max, ncol, in, out, moments, fit, mt, wd, st);
+  return launch_stat<1, 256>(s, beta, n_max, ncol, in, out, moments, fit, mt, wd, st);
 }
 
 }  // namespace tprf

@@ -99,4 +99,28 @@ def cubic_3orb(seed=0, L=16, beta=40.0, n_iw=256, nb=64, U=2.3, J=0.4, norb=3):
     return Workload("cubic_3orb", (L, L, L), norb, beta, n_iw, nb, mu, e, U_s)
 
 
+def uneven_batch(seed=0, count=48, nb=64, U=2.3, J=0.4):
+    """48 uneven lattices drawn from the seed: simple-cubic L^3 k with
+    L in {8, 12, 16}, norb in {1, 2, 3}, n_iw in {128, 256, 512}, beta in
+    {10, 20, 40}, mu ~ U[-1, 1]; hoppings as in cubic_3orb (per direction
+    (A + A^H)/2 - 0.26 * 1, A entries N(0, 0.25^2)); nb bosonic frequencies;
+    Kanamori spin-channel U (U, U' = U - 2J, J, J' = J; just U for one orbital)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        L = int(rng.choice([8, 12, 16]))
+        norb = int(rng.choice([1, 2, 3]))
+        n_iw = int(rng.choice([128, 256, 512]))
+        beta = float(rng.choice([10.0, 20.0, 40.0]))
+        mu = float(rng.uniform(-1.0, 1.0))
+        T = cubic_hoppings(int(rng.integers(1 << 30)), norb)
+        k = kgrid((L, L, L))
+        e = np.zeros((L ** 3, norb, norb), dtype=np.complex128)
+        for d in range(3):
+            e += 2 * np.cos(k[d])[:, None, None] * T[d][None]
+        _, U_s = kanamori_charge_and_spin_quartic_interaction_tensors(norb, U, U - 2 * J, J, J)
+        out.append(Workload(f"uneven_{i}", (L, L, L), norb, beta, n_iw, nb, mu, e, U_s))
+    return out
+
+
 WORKLOADS = {"plain_square": plain_square, "cubic_3orb": cubic_3orb}
