@@ -26,6 +26,15 @@ METRIC = "χ0+RPA evals/s (3-orb 16³k, 256 iω) at 1e-10 rel; grad/fwd time; ba
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
+# HBM traffic of the dominant kernel per launch, from two rocprofv3 --pmc runs
+# (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) of `bench.py --only calibration`
+# on the MI355X.  The same run's 1 GiB device copy reads FETCH_SIZE = 0.5 GiB and
+# its 1 GiB fill WRITE_SIZE = 1 GiB, so fetches are scaled by 2.0 and writes by 1.0.
+# The kernel is the fused bubble in the workgroup shape the autotuner kept there.
+FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
+MEASURED_TRAFFIC = {"chi0_wr": {"kernel": "k_bubble_pair<3,16,8,9,256,true>", "fetch_kib": 602495,
+                                "write_kib": 2108404}}
+
 
 def stage_model(wl):
     """Algorithmic bytes (each materialised API tensor written once and read once,
@@ -188,6 +197,67 @@ def batched_leg(reps=3):
             "max_rel_diff_batched_vs_plain": diff}
 
 
+def uneven_leg(reps=3):
+    """batched/plain at the uneven batch: 48 lattices drawn from the seed with
+    L^3 k (L in {8, 12, 16}), norb in {1, 2, 3}, n_iw in {128, 256, 512}, beta in
+    {10, 20, 40}, mu ~ U[-1, 1], 64 bosonic frequencies, full forward chi0 + RPA.
+    Plain: the drop-in call sequence per lattice (lattice_dyson_g0_wk,
+    fourier_wk_to_wr, fourier_wr_to_tr, chi0_wr_from_grt_PH, chi_wk_from_chi_wr,
+    solve_rpa_PH), each allocating its output; batched: one HIP-graph replay of
+    every lattice's resident chain, stream count autotuned."""
+    import torch
+    from tprf_amd import workloads, lattice as Lt
+    from tprf_amd.batched import BatchedChain
+    from tprf_amd.gf import Gf, MeshBrZone, MeshImFreq
+    wls = workloads.uneven_batch(seed=0)
+    ins = [(Gf(MeshBrZone(w.dims), data=torch.from_numpy(w.e_k).cuda()), torch.from_numpy(w.U).cuda()) for w in wls]
+
+    def plain_one(w, e_k, U):
+        g_wk = Lt.lattice_dyson_g0_wk(mu=w.mu, e_k=e_k, mesh=MeshImFreq(w.beta, "Fermion", w.n_iw))
+        g_wr = Lt.fourier_wk_to_wr(g_wk)
+        del g_wk
+        g_tr = Lt.fourier_wr_to_tr(g_wr)
+        del g_wr
+        chi0_wr = Lt.chi0_wr_from_grt_PH(g_tr, nw=w.nb)
+        del g_tr
+        chi0_wk = Lt.chi_wk_from_chi_wr(chi0_wr)
+        del chi0_wr
+        return Lt.solve_rpa_PH(chi0_wk, U)
+
+    def plain():
+        for w, (e_k, U) in zip(wls, ins):
+            plain_one(w, e_k, U)
+
+    tp_ms = _timed(plain, reps, warm=1)
+    bc = BatchedChain(wls)
+    per_streams = bc.autotune((1, 2, 4, 8))
+    tb_ms = _timed(bc.replay, reps, warm=1)
+    ref = plain_one(wls[0], *ins[0]).data
+    diff = float((bc.chains[0].chi_wk - ref).abs().max() / ref.abs().max())
+    out = {"workload": "48 uneven cubic lattices (8^3/12^3/16^3 k, 1-3 orbitals, 128/256/512 freqs, beta 10/20/40), "
+                       "forward chi0 + RPA over 127 bosonic freqs",
+           "plain_ms": tp_ms, "batched_ms": tb_ms, "batched_over_plain": tp_ms / tb_ms,
+           "graph_ms_by_streams": per_streams, "streams": bc.nstreams, "max_rel_diff_first_lattice": diff}
+    del bc
+    return out
+
+
+def calibration_run():
+    """Two chain evals and one plain device copy of exactly 1 GiB (the reference
+    figure for the FETCH_SIZE / WRITE_SIZE counters)."""
+    import torch
+    from tprf_amd import chain, workloads
+    ch = chain.BubbleRPA(workloads.cubic_3orb(seed=0))
+    for _ in range(2):
+        ch.run()
+    a = torch.empty(64 * 1024 * 1024, dtype=torch.complex128, device="cuda")
+    a.fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,6 +266,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-extra", action="store_true", help="skip the grad/fwd and batched/plain legs")
+    ap.add_argument("--only", choices=["grad", "batched", "uneven", "calibration"],
+                    help="run one leg alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this path shards by the "
@@ -206,6 +278,13 @@ def main():
     from tprf_amd import chain, workloads
 
     torch.cuda.set_device(0)
+    if args.only:
+        if args.only == "calibration":
+            calibration_run()
+            print(json.dumps({"calibration": "done"}))
+        else:
+            print(json.dumps({"grad": grad_leg, "batched": batched_leg, "uneven": uneven_leg}[args.only]()))
+        return
     wl = workloads.cubic_3orb(seed=args.seed)
     ch = chain.BubbleRPA(wl, device="cuda:0")
     for _ in range(args.warmup):
@@ -239,9 +318,15 @@ def main():
     dom = max(stage_ms, key=stage_ms.get)
     dom_bytes, dom_flops = model[dom]
     achieved = dom_bytes / (stage_ms[dom] * 1e-3) / 1e9
+    meas = MEASURED_TRAFFIC.get(dom)
+    traffic = None
+    if meas:
+        traffic = 1024.0 * (FETCH_SCALE * meas["fetch_kib"] + WRITE_SCALE * meas["write_kib"])
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
-                "algorithmic_bytes": dom_bytes, "avg_ms": stage_ms[dom]}
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
+                "algorithmic_bytes": dom_bytes, "avg_ms": stage_ms[dom],
+                "traffic_note": (f"rocprofv3 FETCH_SIZE x{FETCH_SCALE} + WRITE_SIZE x{WRITE_SCALE} of "
+                                 f"{meas['kernel']}, profiles/pmc_write_fetch_chain_cubic_3orb.md") if meas else None}
     fp64 = {"kernel": dom, "achieved": dom_flops / (stage_ms[dom] * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s"}
     fp64["frac"] = fp64["achieved"] / FP64_PEAK_TFLOPS
@@ -280,6 +365,8 @@ def main():
         out["grad_fwd"] = grad_leg()
         torch.cuda.empty_cache()
         out["batched"] = batched_leg()
+        torch.cuda.empty_cache()
+        out["batched_uneven"] = uneven_leg()
     print(json.dumps(out))
 
 
