@@ -71,7 +71,9 @@ def test_dyson(L, oracle, n, sigma):
 # ---------------------------------------------------------------- a2/a6 lattice FFT
 @pytest.mark.parametrize("dims,ntarget", [((4, 3, 2), 4), ((16, 16, 16), 9), ((32, 32, 1), 1), ((5, 7, 1), 3),
                                           ((12, 12, 12), 1), ((48, 48, 1), 4), ((64, 64, 1), 1), ((11, 13, 1), 2),
-                                          ((1, 1, 1), 5), ((8, 1, 8), 81)])
+                                          ((1, 1, 1), 5), ((8, 1, 8), 81), ((12, 12, 12), 3), ((6, 6, 6), 16),
+                                          ((24, 24, 1), 5), ((64, 64, 1), 4), ((8, 8, 4), 2), ((6, 6, 1), 81),
+                                          ((16, 16, 16), 81), ((8, 8, 8), 7)])
 def test_lattice_fft(L, oracle, dims, ntarget):
     from tprf_amd import _lib
     import torch

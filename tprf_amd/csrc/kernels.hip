@@ -606,89 +606,166 @@ namespace tprf {
 // contiguous line (stride 16 elements) on distinct banks
 __device__ __forceinline__ int l3_idx(int k, int c, int C) { return (k + (k >> 4)) * C + c; }
 
+// Radix plan of one lattice axis of length D: in-place decimation-in-frequency
+// stages of radix R0 then R1 (R1 = 1: one stage).  Lengths whose factors are
+// not all in {2, 3, 4, 5, 8, 16} have no plan (R0 = 0).
 template <int D>
-__device__ __forceinline__ void line_dft(c2 *sm, int kbase, int kstride, int c, int C, int sign) {
-  c2 v[D];
+struct AxisPlan {
+  static constexpr int R0 = (D == 1) ? 1
+                            : (D == 2 || D == 3 || D == 4 || D == 5 || D == 8 || D == 16) ? D
+                            : (D == 6) ? 3 : (D == 10) ? 5 : (D == 12) ? 4 : (D == 20) ? 5 : (D == 24) ? 8
+                            : (D == 32) ? 8 : (D == 40) ? 8 : (D == 48) ? 16 : (D == 64) ? 8 : (D == 80) ? 16 : 0;
+  static constexpr int R1 = (R0 == 0 || R0 == D) ? 1 : D / R0;
+  static constexpr bool ok = R0 != 0 && (R1 == 1 || R1 == 2 || R1 == 3 || R1 == 4 || R1 == 5 || R1 == 8 ||
+                                         R1 == 16);
+};
+
+// One DIF stage over every line of an axis held in LDS: span S (= D for the
+// first stage), radix R, sub-length m = S / R; the R points b*S + j + r*m of a
+// butterfly are transformed in place and twiddled by W_S^{j r'} = W_D^{(D/S) j r'}.
+template <int R, int D, int SIGN>
+__device__ __forceinline__ void lds_axis_stage(c2 *sm, const c2 *__restrict__ tw, int S, int stride, int nlines,
+                                               int line_mul0, int line_div, int line_mul1, int C) {
+  const int m = S / R;
+  const int nbf = D / R;
+  const int items = C * nlines * nbf;
+  for (int it = threadIdx.x; it < items; it += blockDim.x) {
+    const int c = it % C;
+    const int rest = it / C;
+    const int line = rest / nbf, b = rest - (rest / nbf) * nbf;
+    // base offset of the line: (line / line_div) * line_mul0 + (line % line_div) * line_mul1
+    const int base = (line / line_div) * line_mul0 + (line % line_div) * line_mul1;
+    const int blk = b / m, j = b - blk * m;
+    const int a0 = blk * S + j;
+    c2 v[R];
 #pragma unroll
-  for (int i = 0; i < D; ++i) v[i] = sm[l3_idx(kbase + i * kstride, c, C)];
-  if (sign < 0) SmallDft<D, -1>::run(v, nullptr, 0);
-  else SmallDft<D, 1>::run(v, nullptr, 0);
+    for (int r = 0; r < R; ++r) v[r] = sm[l3_idx(base + (a0 + r * m) * stride, c, C)];
+    SmallDft<R, SIGN>::run(v, nullptr, 0);
+    if (m > 1) {
+      const int step = (D / S) * j;
 #pragma unroll
-  for (int i = 0; i < D; ++i) sm[l3_idx(kbase + i * kstride, c, C)] = v[i];
+      for (int r = 1; r < R; ++r) {
+        const c2 w = tw[(step * r) % D];  // exp(+2 pi i e / D)
+        v[r] = cmul(v[r], SIGN > 0 ? w : cconj(w));
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) sm[l3_idx(base + (a0 + r * m) * stride, c, C)] = v[r];
+  }
 }
 
-template <int D0, int D1, int D2>
+// frequency k of an axis -> its LDS position after the DIF stages (digit reversal)
+template <int D>
+__device__ __forceinline__ int axis_pos(int k) {
+  using Pl = AxisPlan<D>;
+  if (Pl::R1 == 1) return k;
+  const int r0 = k % Pl::R0, r1 = k / Pl::R0;  // k = r0 + R0 r1
+  return r0 * (D / Pl::R0) + r1;                // position r0 m0 + r1, m0 = D / R0
+}
+
+template <int D, int SIGN>
+__device__ __forceinline__ void lds_axis(c2 *sm, const c2 *tw, int stride, int nlines, int line_mul0, int line_div,
+                                         int line_mul1, int C) {
+  using Pl = AxisPlan<D>;
+  lds_axis_stage<Pl::R0, D, SIGN>(sm, tw, D, stride, nlines, line_mul0, line_div, line_mul1, C);
+  if constexpr (Pl::R1 > 1) {
+    __syncthreads();
+    lds_axis_stage<Pl::R1, D, SIGN>(sm, tw, D / Pl::R0, stride, nlines, line_mul0, line_div, line_mul1, C);
+  }
+}
+
+// Single-pass 3-D lattice FFT: a workgroup holds the whole D0 x D1 x D2 grid of C
+// target columns of one outer index in LDS (padded index k + k / 16), transforms
+// every axis in place with its radix plan and writes the result back in natural
+// order, scaled.
+template <int D0, int D1, int D2, int SIGN>
 __global__ void __launch_bounds__(256) k_lattice3d(const c2 *__restrict__ in, c2 *__restrict__ out, long ntarget,
-                                                   int sign, double scale) {
+                                                   double scale) {
   constexpr int NK = D0 * D1 * D2;
   constexpr int C = NK >= 4096 ? 1 : 4096 / NK;
   __shared__ c2 sm[(NK + NK / 16 + 1) * C];
+  __shared__ c2 tw0[D0], tw1[D1], tw2[D2];
   const long ntc = (ntarget + C - 1) / C;
   const long tile = xcd_remap(blockIdx.x, gridDim.x);
   const long w = tile / ntc;
   const long c0 = (tile - w * ntc) * C;
   const int cn = (ntarget - c0) < C ? (int)(ntarget - c0) : C;
+  for (int t = threadIdx.x; t < D0 + D1 + D2; t += blockDim.x) {
+    // exp(+2 pi i e / D): sincospi of an exactly reduced fraction
+    double sn, cs;
+    if (t < D0) { sincospi(2.0 * t / D0, &sn, &cs); tw0[t] = c2{cs, sn}; }
+    else if (t < D0 + D1) { const int e = t - D0; sincospi(2.0 * e / D1, &sn, &cs); tw1[e] = c2{cs, sn}; }
+    else { const int e = t - D0 - D1; sincospi(2.0 * e / D2, &sn, &cs); tw2[e] = c2{cs, sn}; }
+  }
   const c2 *src = in + w * NK * ntarget + c0;
-  for (int t = threadIdx.x; t < NK * cn; t += blockDim.x) {
-    const int k = t / cn, c = t - k * cn;
-    sm[l3_idx(k, c, C)] = src[(long)k * ntarget + c];
+  for (int t = threadIdx.x; t < NK * C; t += blockDim.x) {
+    const int k = t / C, c = t - k * C;
+    sm[l3_idx(k, c, C)] = c < cn ? src[(long)k * ntarget + c] : c2{0.0, 0.0};
   }
   __syncthreads();
+  // axis 2 (stride 1): line = i0 * D1 + i1, base = line * D2
   if constexpr (D2 > 1) {
-    for (int l = threadIdx.x; l < C * D0 * D1; l += blockDim.x) {
-      const int c = l % C, rest = l / C;  // rest = i0 * D1 + i1
-      line_dft<D2>(sm, rest * D2, 1, c, C, sign);
-    }
+    lds_axis<D2, SIGN>(sm, tw2, 1, D0 * D1, D2, 1, 0, C);
     __syncthreads();
   }
+  // axis 1 (stride D2): line = i0 * D2 + i2, base = i0 * D1 * D2 + i2
   if constexpr (D1 > 1) {
-    for (int l = threadIdx.x; l < C * D0 * D2; l += blockDim.x) {
-      const int c = l % C, rest = l / C;
-      const int i2 = rest % D2, i0 = rest / D2;
-      line_dft<D1>(sm, i0 * D1 * D2 + i2, D2, c, C, sign);
-    }
+    lds_axis<D1, SIGN>(sm, tw1, D2, D0 * D2, D1 * D2, D2, 1, C);
     __syncthreads();
   }
+  // axis 0 (stride D1 D2): line = i1 * D2 + i2, base = line
   if constexpr (D0 > 1) {
-    for (int l = threadIdx.x; l < C * D1 * D2; l += blockDim.x) {
-      const int c = l % C, rest = l / C;  // rest = i1 * D2 + i2
-      line_dft<D0>(sm, rest, D1 * D2, c, C, sign);
-    }
+    lds_axis<D0, SIGN>(sm, tw0, D1 * D2, D1 * D2, 1, 1, 0, C);
     __syncthreads();
   }
   c2 *dst = out + w * NK * ntarget + c0;
   for (int t = threadIdx.x; t < NK * cn; t += blockDim.x) {
     const int k = t / cn, c = t - k * cn;
-    dst[(long)k * ntarget + c] = cscale(sm[l3_idx(k, c, C)], scale);
+    const int k2 = k % D2, k1 = (k / D2) % D1, k0 = k / (D1 * D2);
+    const int p = (axis_pos<D0>(k0) * D1 + axis_pos<D1>(k1)) * D2 + axis_pos<D2>(k2);
+    dst[(long)k * ntarget + c] = cscale(sm[l3_idx(p, c, C)], scale);
   }
 }
 
 template <int D0, int D1, int D2>
 static hipError_t launch_l3(hipStream_t s, const c2 *in, c2 *out, long nw, long ntarget, int sign, double scale) {
+  static_assert(AxisPlan<D0>::ok && AxisPlan<D1>::ok && AxisPlan<D2>::ok, "no radix plan");
   constexpr int NK = D0 * D1 * D2;
   constexpr int C = NK >= 4096 ? 1 : 4096 / NK;
   const long nb = nw * ((ntarget + C - 1) / C);
-  hipLaunchKernelGGL((k_lattice3d<D0, D1, D2>), dim3(nb), dim3(256), 0, s, in, out, ntarget, sign, scale);
+  if (sign < 0)
+    hipLaunchKernelGGL((k_lattice3d<D0, D1, D2, -1>), dim3(nb), dim3(256), 0, s, in, out, ntarget, scale);
+  else
+    hipLaunchKernelGGL((k_lattice3d<D0, D1, D2, 1>), dim3(nb), dim3(256), 0, s, in, out, ntarget, scale);
   return hipGetLastError();
 }
 
-// dims with every extent in {1, 2, 4, 8, 16}; hipErrorNotSupported otherwise
+// the lattice shapes with a single-pass kernel; hipErrorNotSupported otherwise
 hipError_t launch_lattice3d(hipStream_t s, int sign, const c2 *in, c2 *out, long nw, const int dims[3], long ntarget,
                             double scale) {
   const int a = dims[0], b = dims[1], c = dims[2];
 #define TPRF_L3(A, B, C_)                                                              \
   if (a == A && b == B && c == C_) return launch_l3<A, B, C_>(s, in, out, nw, ntarget, sign, scale);
   TPRF_L3(16, 16, 16)
+  TPRF_L3(12, 12, 12)
   TPRF_L3(8, 8, 8)
-  TPRF_L3(16, 16, 1)
-  TPRF_L3(8, 8, 1)
+  TPRF_L3(6, 6, 6)
   TPRF_L3(4, 4, 4)
+  TPRF_L3(2, 2, 2)
+  TPRF_L3(64, 64, 1)
+  TPRF_L3(48, 48, 1)
+  TPRF_L3(32, 32, 1)
+  TPRF_L3(24, 24, 1)
+  TPRF_L3(16, 16, 1)
+  TPRF_L3(12, 12, 1)
+  TPRF_L3(8, 8, 1)
+  TPRF_L3(6, 6, 1)
   TPRF_L3(4, 4, 1)
+  TPRF_L3(2, 2, 1)
   TPRF_L3(16, 1, 1)
   TPRF_L3(8, 1, 1)
   TPRF_L3(4, 4, 2)
-  TPRF_L3(2, 2, 2)
-  TPRF_L3(2, 2, 1)
+  TPRF_L3(8, 8, 4)
 #undef TPRF_L3
   return hipErrorNotSupported;
 }
