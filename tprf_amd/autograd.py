@@ -85,14 +85,15 @@ class StaticRPA:
         _lib.call("tprf_fourier_r_to_q", c, self._cdims, self._cq, n ** 4, B, p(self._chi_r), p(chi0_q))
         _lib.call("tprf_solve_rpa_PH_adjoint", c, B, n, p(chi0_q), p(U), None, p(chi), None, None)
 
-    def _backward_into(self, c, g_wk, chi0_q, U, chibar, chi, chi0bar, Ubar, ebar, mubar):
+    def _backward_into(self, c, g_wk, chi0_q, U, chibar, chi, chi0bar, Ubar, ebar, mubar, recompute=True):
         p = _lib.ptr
         n, nk, B = self.n, self.nk, self.nb
         _lib.call("tprf_solve_rpa_PH_adjoint", c, B, n, p(chi0_q), p(U), p(chibar), p(chi), p(chi0bar), p(Ubar))
         chibar_r = self._chi_r
         _lib.call("tprf_fourier_r_to_q_adjoint", c, self._cdims, self._cq, n ** 4, B, p(chi0bar), p(chibar_r))
-        # recompute G(tau, r) from the kept G(iw, k)
-        self._g_tr_from_g_wk(c, g_wk)
+        if recompute:
+            # recompute G(tau, r) from the kept G(iw, k)
+            self._g_tr_from_g_wk(c, g_wk)
         gbar_tr = self._x_tr
         _lib.call("tprf_chi0_static_from_grt_PH_adjoint", c, self.beta, self.ntau, self._cdims, n, B,
                   p(self._g_tr), p(chibar_r), p(gbar_tr))
@@ -139,7 +140,10 @@ class StaticRPA:
     def graphed_grad(self, e_k, mu, U):
         """Copies the inputs into static buffers and replays (capturing on the first
         call) forward + backward for the whole batch.  Returns (chi, ebar, mubar, Ubar)
-        as views of static buffers, valid until the next call."""
+        as views of static buffers, valid until the next call.  Forward and backward
+        run back to back in one launch, so the backward uses G(tau, r) from the
+        forward's transient work buffer instead of recomputing it (the per-call
+        autograd path recomputes it, since it keeps only G(i w, k) between calls)."""
         n, nk, B = self.n, self.nk, self.nb
         e_k, mu, U = self._inputs(e_k, mu, U)
         if self._graph is None:
@@ -158,7 +162,7 @@ class StaticRPA:
                 c = self._ctx()
                 self._forward_into(c, st["e_k"], st["mu"], st["U"], st["g_wk"], st["chi0_q"], st["chi"])
                 self._backward_into(c, st["g_wk"], st["chi0_q"], st["U"], st["chibar"], st["chi2"],
-                                    st["chi0bar"], st["Ubar"], st["ebar"], st["mubar"])
+                                    st["chi0bar"], st["Ubar"], st["ebar"], st["mubar"], recompute=False)
 
             # warm-up outside the graph: plans, tables and scratch are allocated here
             side = torch.cuda.Stream(device=self.dev)
