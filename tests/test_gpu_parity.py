@@ -255,16 +255,21 @@ def test_chi0_wr_fused(L, oracle, gtr_case, nw):
     assert relerr(host(got.data), chain) <= TOL
 
 
-@pytest.mark.parametrize("n,n_iw,nb,dims", [(3, 64, 64, (4, 2, 1)), (3, 128, 32, (3, 3, 1)), (2, 64, 17, (2, 2, 2)),
-                                            (2, 256, 64, (5, 3, 1)), (3, 256, 64, (2, 1, 1)), (2, 128, 9, (6, 1, 1))])
-def test_chi0_wr_pruned_pair_kernel(L, oracle, n, n_iw, nb, dims):
+# ntau sets the number of tau tiles L / P of the pair kernel: 3 (odd) for
+# (3, 64, 64), 1, 2 and 3 for the explicit ntau cases, 12 elsewhere
+@pytest.mark.parametrize("n,n_iw,nb,dims,ntau", [(3, 64, 64, (4, 2, 1), None), (3, 128, 32, (3, 3, 1), None),
+                                                 (2, 64, 17, (2, 2, 2), None), (2, 256, 64, (5, 3, 1), None),
+                                                 (3, 256, 64, (2, 1, 1), None), (2, 128, 9, (6, 1, 1), None),
+                                                 (3, 64, 64, (3, 2, 1), 257), (2, 64, 64, (2, 2, 1), 129),
+                                                 (3, 32, 32, (2, 2, 1), 129), (2, 32, 32, (3, 1, 1), 193)])
+def test_chi0_wr_pruned_pair_kernel(L, oracle, n, n_iw, nb, dims, ntau):
     """The pruned (r, -r) pair kernel and the generic kernel both match the oracle;
     dims include self-mirrored r (r = -r) and odd extents."""
     from tprf_amd import _lib
     rng = np.random.default_rng(n * 1000 + n_iw + nb)
     beta = 15.0
     gw = g0_wr(oracle, rng, dims, n, beta, n_iw)
-    gt = oracle.fourier_w_to_t(gw, beta)
+    gt = oracle.fourier_w_to_t(gw, beta, ntau=ntau)
     ref = oracle.chi0_wr_from_grt_PH(gt, dims, beta, nb)
     c = _lib.ctx()
     for generic in (0, 1):
@@ -275,7 +280,7 @@ def test_chi0_wr_pruned_pair_kernel(L, oracle, n, n_iw, nb, dims):
             _lib.call("tprf_ctx_set_option", c, 1, 0)
         assert relerr(host(got.data), ref) <= TOL, f"generic={generic}"
     # every workgroup shape the autotuner may pick (option 2 fixes it, -1 restores tuning)
-    for variant in range(3):
+    for variant in range(5):
         _lib.call("tprf_ctx_set_option", c, 2, variant)
         try:
             got = L.chi0_wr_from_grt_PH(gtr_gf(dims, beta, gt), nw=nb)

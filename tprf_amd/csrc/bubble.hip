@@ -18,18 +18,46 @@
 // (it is linear): y(m) = (beta/L) (S(m) - sum_i a_i Phi_i(m)) + sum_i a_i/(i nu_m - b_i)
 // with Phi_i(m) the transform of the model's tau samples, and a_i from each
 // column's own boundary derivatives.
+#include <type_traits>
+
 #include "common.hpp"
 #include "fft_device.hpp"
 #include "mats_device.hpp"
 
 namespace tprf {
 
+// Direct global -> LDS load (global_load_lds_dwordx4): lane l of a wave writes
+// the 16 bytes at its own `src` to `dst` + 16 l; `dst` must be the same for the
+// whole wave.  Completion is counted by vmcnt like any vector memory load.
+__device__ __forceinline__ void dma16(const c2 *src, c2 *dst) {
+  __builtin_amdgcn_global_load_lds((const void *)src, (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
+}
+// s_waitcnt vmcnt(K): vmcnt in bits 3:0 and 15:14, expcnt (6:4) and lgkmcnt (11:8) at their maxima.
+template <int K>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(K >= 0 && K < 64, "vmcnt holds 6 bits");
+  __builtin_amdgcn_s_waitcnt((K & 15) | ((K >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+// Workgroup barrier that leaves vector memory loads in flight: waits for this
+// wave's LDS reads and writes (lgkmcnt), then s_barrier.  __syncthreads() would
+// also wait for every in-flight global -> LDS load (its release fence covers
+// them), which would stall the prefetch two steps ahead.  The "memory" clobber
+// keeps the compiler from moving LDS accesses across it.
+__device__ __forceinline__ void barrier_keep_vm() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // AB = true: a workgroup owns the CG = n^2 columns of one orbital pair (a, b)
 // (abcd with c, d free) and stages only the n entries G_da(r), d < n, and
 // G_bc(-r), c < n, per tau point; AB = false: CG consecutive columns, all n^2
 // entries of G(r) and G(-r) staged.
-template <int N, int R1, int R2, int CG, int NT, bool AB>
-__global__ void __launch_bounds__(NT) k_bubble_pair(double beta, int L, int nb, int d0, int d1, int d2,
+// DMA = false: the tile of step j1 + 1 is loaded into registers during step j1
+// and written to LDS at the top of step j1 + 1.  DMA = true: tiles go straight
+// from global memory to LDS through two buffers, two steps ahead; each wave
+// issues 64-entry chunks of the tile and waits for its own older chunks with
+// vmcnt before the barrier that publishes them.
+// WPE: waves per SIMD the register allocation must allow (1: no limit).
+template <int N, int R1, int R2, int CG, int NT, bool AB, bool DMA, int WPE>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) k_bubble_pair(double beta, int L, int nb, int d0, int d1, int d2,
                                                     const c2 *__restrict__ gtr, c2 *__restrict__ out, BubbleDev bd,
                                                     Vinv8 vinv) {
   constexpr int P = R1 * R2;
@@ -43,8 +71,14 @@ __global__ void __launch_bounds__(NT) k_bubble_pair(double beta, int L, int nb, 
   constexpr int NSTAGE = P * NE;
   constexpr int PF = (NSTAGE + NT - 1) / NT;
   static_assert(CG * R1 * TLD >= CG * P, "transpose buffer reused for the final exchange");
+  constexpr int NW = NT / 64;                  // waves per workgroup
+  constexpr int NCH = NSTAGE / 64;             // 64-entry chunks of one side of a tile
+  constexpr int KD = (2 * NCH) / NW;           // chunks each wave issues per tile
+  static_assert(!DMA || (NSTAGE % 64 == 0 && NT % 64 == 0 && (2 * NCH) % NW == 0),
+                "whole-wave chunks, the same number for every wave");
 
   __shared__ c2 sA[NSTAGE], sB[NSTAGE];
+  __shared__ c2 sA2[DMA ? NSTAGE : 1], sB2[DMA ? NSTAGE : 1];  // second buffers of the DMA pipeline
   __shared__ c2 sT[CG * R1 * TLD];
   __shared__ c2 sE[CG][18];
   __shared__ c2 sPole[CG][2][3];
@@ -82,24 +116,18 @@ __global__ void __launch_bounds__(NT) k_bubble_pair(double beta, int L, int nb, 
   }
 
   for (int t = tid; t < P; t += NT) sTwp[t] = bd.twp[t];
-  c2 pa[PF], pb[PF], pw = c2{0.0, 0.0};
-  // loads of step j1 + 1 are issued right after step j1's tile is staged
-  auto fetch = [&](int j1) {
-    if (tid < P) pw = bd.tw1[(long)j1 * P + tid];
-#pragma unroll
-    for (int i = 0; i < PF; ++i) {
-      const int t = tid + i * NT;
-      if (t < NSTAGE) {
-        const int j2 = t / NE, e = t - j2 * NE;
-        const long tau = (long)N1 * j2 + j1;
-        // AB: e = d of G_da(r) (a = a0) and e = c of G_bc(-r) (b = b0)
-        const int ea = AB ? e * N + a0 : e, eb = AB ? b0 * N + e : e;
-        pa[i] = gtr[(tau * nr + r) * n2 + ea];
-        pb[i] = gtr[((L - tau) * nr + mr) * n2 + eb];
-      }
-    }
+  // Element offset of staged entry t of side A (G_da(tau, r)) or side B
+  // (G_bc(L - tau, -r)) at tau = j2 N1 (+ j1 for step j1); AB: e = d with
+  // a = a0, and e = c with b = b0.
+  const long stepa = nr * n2;  // one tau point
+  auto off_a = [&](int t) {
+    const int j2 = t / NE, e = t - j2 * NE;
+    return ((long)N1 * j2 * nr + r) * n2 + (AB ? e * N + a0 : e);
   };
-  fetch(0);
+  auto off_b = [&](int t) {
+    const int j2 = t / NE, e = t - j2 * NE;
+    return (((long)L - (long)N1 * j2) * nr + mr) * n2 + (AB ? b0 * N + e : e);
+  };
 
   c2 acc[U2][R2];
 #pragma unroll
@@ -107,20 +135,8 @@ __global__ void __launch_bounds__(NT) k_bubble_pair(double beta, int L, int nb, 
 #pragma unroll
     for (int k = 0; k < R2; ++k) acc[i][k] = c2{0.0, 0.0};
 
-  for (int j1 = 0; j1 < N1; ++j1) {
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < PF; ++i) {
-      const int t = tid + i * NT;
-      if (t < NSTAGE) {
-        sA[t] = pa[i];
-        sB[t] = pb[i];
-      }
-    }
-    if (tid < P) sTw1[tid] = pw;
-    __syncthreads();
-    if (j1 + 1 < N1) fetch(j1 + 1);
-    // stage 1: unit (c, s): DFT-R1 over k of x at j2 = R2 k + s, twiddled by exp(2 pi i s k1 / P)
+  // stage 1: unit (c, s): DFT-R1 over k of x at j2 = R2 k + s, twiddled by exp(2 pi i s k1 / P)
+  auto stage1 = [&](const c2 *tA, const c2 *tB) {
     for (int u = tid; u < cn * R2; u += NT) {
       const int c = u / R2, s = u - c * R2;
       const int q = q0 + c;
@@ -129,8 +145,8 @@ __global__ void __launch_bounds__(NT) k_bubble_pair(double beta, int L, int nb, 
 #pragma unroll
       for (int k = 0; k < R1; ++k) {
         const int j2 = R2 * k + s;
-        v[k] = AB ? cmul(sA[j2 * NE + dd], sB[j2 * NE + cc])
-                  : cmul(sA[j2 * n2 + dd * N + aa], sB[j2 * n2 + bb * N + cc]);
+        v[k] = AB ? cmul(tA[j2 * NE + dd], tB[j2 * NE + cc])
+                  : cmul(tA[j2 * n2 + dd * N + aa], tB[j2 * n2 + bb * N + cc]);
       }
       SmallDft<R1, 1>::run(v, nullptr, 0);
 #pragma unroll
@@ -138,8 +154,9 @@ __global__ void __launch_bounds__(NT) k_bubble_pair(double beta, int L, int nb, 
 #pragma unroll
       for (int k1 = 0; k1 < R1; ++k1) sT[(c * R1 + k1) * TLD + s] = v[k1];
     }
-    __syncthreads();
-    // stage 2: unit (c, k1): DFT-R2 over s gives bins k = k1 + R1 k2; accumulate with exp(2 pi i j1 m / L)
+  };
+  // stage 2: unit (c, k1): DFT-R2 over s gives bins k = k1 + R1 k2; accumulate with exp(2 pi i j1 m / L)
+  auto stage2 = [&]() {
     const c2 *w = sTw1;
 #pragma unroll
     for (int i = 0; i < U2; ++i) {
@@ -153,6 +170,112 @@ __global__ void __launch_bounds__(NT) k_bubble_pair(double beta, int L, int nb, 
 #pragma unroll
         for (int k2 = 0; k2 < R2; ++k2) acc[i][k2] = cadd(acc[i][k2], cmul(w[k1 + R1 * k2], v[k2]));
       }
+    }
+  };
+
+  // The twiddles of step j1 + 1 travel in a register (v2d: a double2 copy
+  // assigned under a branch in a loop can be left in private memory).
+  v2d pw = v2d{0.0, 0.0};
+  if constexpr (DMA) {
+    // chunk c = wave + NW i (i < KD) of a tile: side A for c < NCH, side B
+    // after.  Selects rather than branches, so that every wave issues exactly
+    // KD loads per tile on every path.
+    const int wave = tid >> 6, lane = tid & 63;
+    long offd[KD];
+#pragma unroll
+    for (int i = 0; i < KD; ++i) {
+      const int c = wave + NW * i;
+      offd[i] = c < NCH ? off_a(c * 64 + lane) : off_b((c - NCH) * 64 + lane);
+    }
+    auto issue = [&](int j1, c2 *tA, c2 *tB) {
+      const long da = (long)j1 * stepa;
+#pragma unroll
+      for (int i = 0; i < KD; ++i) {
+        const int c = wave + NW * i;
+        const bool sb = c >= NCH;
+        dma16(gtr + offd[i] + (sb ? -da : da), sb ? tB + (c - NCH) * 64 : tA + c * 64);
+      }
+    };
+    // Step j1 reads tile j1 from (tA, tB).  ISSUE: tile j1 + 2 goes into the
+    // same buffers once every wave is past stage 1, behind the twiddles of
+    // step j1 + 1, so each wave's newest KD or more loads are always tile
+    // j1 + 2's and vmcnt(KD) at the next step waits for nothing younger than
+    // that step's own tile and twiddles.  The last two steps issue nothing
+    // and are peeled off the loop.
+    auto step = [&](auto issue_tag, int j1, c2 *tA, c2 *tB) {
+      constexpr bool ISSUE = decltype(issue_tag)::value;
+      if (ISSUE || j1 + 1 < N1)
+        wait_vmcnt<KD>();
+      else
+        wait_vmcnt<0>();
+      barrier_keep_vm();
+      if (tid < P) st_v2d(sTw1 + tid, pw);
+      stage1(tA, tB);
+      barrier_keep_vm();
+      if ((ISSUE || j1 + 1 < N1) && tid < P) pw = ld_v2d(bd.tw1 + (long)(j1 + 1) * P + tid);
+      if constexpr (ISSUE) issue(j1 + 2, tA, tB);
+      stage2();
+    };
+    using Issue = std::integral_constant<bool, true>;
+    using NoIssue = std::integral_constant<bool, false>;
+    // N1 >= 2 (the launcher sends N1 = 1 to the register variant)
+    issue(0, sA, sB);
+    if (tid < P) pw = ld_v2d(bd.tw1 + tid);
+    issue(1, sA2, sB2);
+    int j1 = 0;
+    for (; j1 + 3 < N1; j1 += 2) {
+      step(Issue{}, j1, sA, sB);
+      step(Issue{}, j1 + 1, sA2, sB2);
+    }
+    if (N1 - j1 == 3) {
+      step(Issue{}, j1, sA, sB);
+      step(NoIssue{}, j1 + 1, sA2, sB2);
+      step(NoIssue{}, j1 + 2, sA, sB);
+    } else {
+      step(NoIssue{}, j1, sA, sB);
+      step(NoIssue{}, j1 + 1, sA2, sB2);
+    }
+  } else {
+    // Loads of step j1 + 1 are issued right after step j1's tile is staged and
+    // stay in flight in registers through step j1's DFTs.
+    long offa[PF], offb[PF];
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      const int t = tid + i * NT;
+      offa[i] = off_a(t < NSTAGE ? t : 0);
+      offb[i] = off_b(t < NSTAGE ? t : 0);
+    }
+    v2d pa[PF], pb[PF];
+    if (tid < P) pw = ld_v2d(bd.tw1 + tid);
+#pragma unroll
+    for (int i = 0; i < PF; ++i)
+      if (tid + i * NT < NSTAGE) {
+        pa[i] = ld_v2d(gtr + offa[i]);
+        pb[i] = ld_v2d(gtr + offb[i]);
+      }
+    for (int j1 = 0; j1 < N1; ++j1) {
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < PF; ++i)
+        if (tid + i * NT < NSTAGE) {
+          st_v2d(sA + tid + i * NT, pa[i]);
+          st_v2d(sB + tid + i * NT, pb[i]);
+        }
+      if (tid < P) st_v2d(sTw1 + tid, pw);
+      __syncthreads();
+      if (j1 + 1 < N1) {
+        if (tid < P) pw = ld_v2d(bd.tw1 + (long)(j1 + 1) * P + tid);
+        const long da = (long)(j1 + 1) * stepa;
+#pragma unroll
+        for (int i = 0; i < PF; ++i)
+          if (tid + i * NT < NSTAGE) {
+            pa[i] = ld_v2d(gtr + offa[i] + da);
+            pb[i] = ld_v2d(gtr + offb[i] - da);
+          }
+      }
+      stage1(sA, sB);
+      __syncthreads();
+      stage2();
     }
   }
   __syncthreads();
@@ -203,47 +326,62 @@ __global__ void __launch_bounds__(NT) k_bubble_pair(double beta, int L, int nb, 
   }
 }
 
-template <int N, int R1, int R2, int CG, int NT, bool AB>
+template <int N, int R1, int R2, int CG, int NT, bool AB, bool DMA, int WPE>
 static hipError_t launch_one(hipStream_t s, double beta, int L, int nb, const int dims[3], const c2 *gtr, c2 *out,
                              const BubbleDev &bd, const Vinv8 &vinv) {
   constexpr int n4 = N * N * N * N;
   constexpr int NG = (n4 + CG - 1) / CG;
   const long nb_blocks = (long)bd.ncanon * NG;
-  hipLaunchKernelGGL((k_bubble_pair<N, R1, R2, CG, NT, AB>), dim3(nb_blocks), dim3(NT), 0, s, beta, L, nb, dims[0],
+  if (DMA && L / (R1 * R2) < 2) {  // the global -> LDS pipeline needs two tiles
+    hipLaunchKernelGGL((k_bubble_pair<N, R1, R2, CG, NT, AB, false, 1>), dim3(nb_blocks), dim3(NT), 0, s, beta, L, nb,
+                       dims[0], dims[1], dims[2], gtr, out, bd, vinv);
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL((k_bubble_pair<N, R1, R2, CG, NT, AB, DMA, WPE>), dim3(nb_blocks), dim3(NT), 0, s, beta, L, nb, dims[0],
                      dims[1], dims[2], gtr, out, bd, vinv);
   return hipGetLastError();
 }
 
 bool bubble_pair_supported(int n, int P) { return (n == 2 || n == 3) && (P == 64 || P == 128); }
 
-int bubble_pair_variants() { return 3; }
+int bubble_pair_variants() { return 5; }
 
 // Workgroup shapes (same results, picked by the autotuner): variant 0 owns one
 // orbital pair; 1 is a narrower or differently grouped shape; 2 stages every G
-// entry for CG = n^4 / 3 (n = 3) or n^4 (n = 2) columns.
+// entry for CG = n^4 / 3 (n = 3) or n^4 (n = 2) columns; 3 and 4 load tiles
+// straight into LDS (for n = 3, P = 128 the two differ in the occupancy the
+// register allocation is held to).
 hipError_t launch_bubble_pair(hipStream_t s, double beta, int L, int nb, const int dims[3], int n, const c2 *gtr,
                               c2 *out, const BubbleDev &bd, const Vinv8 &vinv, int variant) {
-#define TPRF_BP(N_, R1_, R2_, CG_, NT_, AB_) \
-  return launch_one<N_, R1_, R2_, CG_, NT_, AB_>(s, beta, L, nb, dims, gtr, out, bd, vinv)
+#define TPRF_BP(N_, R1_, R2_, CG_, NT_, AB_, DMA_, WPE_) \
+  return launch_one<N_, R1_, R2_, CG_, NT_, AB_, DMA_, WPE_>(s, beta, L, nb, dims, gtr, out, bd, vinv)
   if (n == 3 && bd.P == 128) {
-    if (variant == 0) TPRF_BP(3, 16, 8, 9, 256, true);
-    if (variant == 1) TPRF_BP(3, 16, 8, 9, 128, true);
-    TPRF_BP(3, 16, 8, 27, 256, false);
+    if (variant == 0) TPRF_BP(3, 16, 8, 9, 256, true, false, 1);
+    if (variant == 1) TPRF_BP(3, 16, 8, 9, 128, true, false, 1);
+    if (variant == 3) TPRF_BP(3, 16, 8, 9, 256, true, true, 3);
+    if (variant == 4) TPRF_BP(3, 16, 8, 9, 256, true, true, 1);
+    TPRF_BP(3, 16, 8, 27, 256, false, false, 1);
   }
   if (n == 3 && bd.P == 64) {
-    if (variant == 0) TPRF_BP(3, 8, 8, 9, 256, true);
-    if (variant == 1) TPRF_BP(3, 8, 8, 9, 128, true);
-    TPRF_BP(3, 8, 8, 27, 256, false);
+    if (variant == 0) TPRF_BP(3, 8, 8, 9, 256, true, false, 1);
+    if (variant == 1) TPRF_BP(3, 8, 8, 9, 128, true, false, 1);
+    if (variant == 3) TPRF_BP(3, 8, 8, 9, 128, true, true, 1);
+    if (variant == 4) TPRF_BP(3, 8, 8, 9, 192, true, true, 1);
+    TPRF_BP(3, 8, 8, 27, 256, false, false, 1);
   }
   if (n == 2 && bd.P == 128) {
-    if (variant == 0) TPRF_BP(2, 16, 8, 4, 128, true);
-    if (variant == 1) TPRF_BP(2, 16, 8, 8, 128, false);
-    TPRF_BP(2, 16, 8, 16, 256, false);
+    if (variant == 0) TPRF_BP(2, 16, 8, 4, 128, true, false, 1);
+    if (variant == 1) TPRF_BP(2, 16, 8, 8, 128, false, false, 1);
+    if (variant == 3) TPRF_BP(2, 16, 8, 4, 128, true, true, 1);
+    if (variant == 4) TPRF_BP(2, 16, 8, 8, 128, false, true, 1);
+    TPRF_BP(2, 16, 8, 16, 256, false, false, 1);
   }
   if (n == 2 && bd.P == 64) {
-    if (variant == 0) TPRF_BP(2, 8, 8, 4, 64, true);
-    if (variant == 1) TPRF_BP(2, 8, 8, 4, 128, true);
-    TPRF_BP(2, 8, 8, 16, 256, false);
+    if (variant == 0) TPRF_BP(2, 8, 8, 4, 64, true, false, 1);
+    if (variant == 1) TPRF_BP(2, 8, 8, 4, 128, true, false, 1);
+    if (variant == 3) TPRF_BP(2, 8, 8, 4, 64, true, true, 1);
+    if (variant == 4) TPRF_BP(2, 8, 8, 4, 128, true, true, 1);
+    TPRF_BP(2, 8, 8, 16, 256, false, false, 1);
   }
 #undef TPRF_BP
   return hipErrorNotSupported;

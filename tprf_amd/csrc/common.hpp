@@ -26,6 +26,14 @@ __host__ __device__ __forceinline__ c2 cdiv(c2 a, c2 b) {
 // a * (i s): multiply by a purely imaginary number
 __host__ __device__ __forceinline__ c2 cmul_i(c2 a, double s) { return c2{-a.y * s, a.x * s}; }
 
+// Native two-double vector for values held in registers across a loop
+// iteration (prefetches): copies of double2 are struct copies that can be left
+// in private memory when they are assigned under a condition inside a loop,
+// while this type always stays in VGPRs.
+typedef double v2d __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ v2d ld_v2d(const c2 *p) { return *reinterpret_cast<const v2d *>(p); }
+__device__ __forceinline__ void st_v2d(c2 *p, v2d v) { *reinterpret_cast<v2d *>(p) = v; }
+
 constexpr int kMaxPass = 24;
 
 // One FFT length: Stockham passes with the listed radices, twiddle table
