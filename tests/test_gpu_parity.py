@@ -301,13 +301,18 @@ def test_chi0_w0r_trapz(L, oracle, gtr_case):
 
 
 # ---------------------------------------------------------------- a7 RPA
+@pytest.mark.parametrize("sparse", [False, True])
 @pytest.mark.parametrize("n", [1, 2, 3, 4])
-def test_rpa(L, oracle, n):
+def test_rpa(L, oracle, n, sparse):
+    """Dense U, and U with about two thirds of its entries exactly zero (the
+    kernel skips zero entries of the interaction matrix)."""
     from tprf_amd.gf import Gf, MeshImFreq, MeshProduct, MeshBrZone
     rng = np.random.default_rng(40 + n)
     npts_w, nk = 5, 13
     chi0 = rnd(rng, npts_w, nk, n, n, n, n) * 0.2
     U = rnd(rng, n, n, n, n) * 0.5
+    if sparse:
+        U = U * (rng.random(U.shape) < 0.35)
     g = Gf(MeshProduct(MeshImFreq(10.0, "Boson", 3), MeshBrZone((13, 1, 1))), data=gpu(chi0))
     got = L.solve_rpa_PH(g, U)
     ref = oracle.solve_rpa_PH(chi0, U)

@@ -359,7 +359,7 @@ hipError_t launch_bubble_pair(hipStream_t s, double beta, int L, int nb, const i
     if (variant == 0) TPRF_BP(3, 16, 8, 9, 256, true, false, 1);
     if (variant == 1) TPRF_BP(3, 16, 8, 9, 128, true, false, 1);
     if (variant == 3) TPRF_BP(3, 16, 8, 9, 256, true, true, 3);
-    if (variant == 4) TPRF_BP(3, 16, 8, 9, 256, true, true, 1);
+    if (variant == 4) TPRF_BP(3, 16, 8, 9, 256, true, false, 3);
     TPRF_BP(3, 16, 8, 27, 256, false, false, 1);
   }
   if (n == 3 && bd.P == 64) {

@@ -483,14 +483,34 @@ __global__ void __launch_bounds__(256) k_rpa(long npts, const c2 *__restrict__ c
                                              c2 *__restrict__ chi, double *status) {
   constexpr int N2 = N * N;
   constexpr int P = 64 / N2;  // points per wave
+  constexpr int NMW = (N2 * N2 + 63) / 64;  // words of the nonzero mask of M_U
   __shared__ c2 sU[N2 * N2];
   __shared__ c2 slot[4 * P][2 * N2];
+  __shared__ unsigned long long sMask[NMW];
   for (int t = threadIdx.x; t < N2 * N2; t += blockDim.x) {
     // t = ((a*N + b)*N + c)*N + d  ->  M_U[(a,b),(d,c)]
     const int d = t % N, c = (t / N) % N, ab = t / N2;
     sU[ab * N2 + d * N + c] = U[t];
   }
   __syncthreads();
+  // Bit l N2 + j of the mask: M_U[l][j] != 0.  Interactions of physical models
+  // are sparse in the particle-hole grouping (Kanamori: 21 of 81 entries for
+  // three orbitals); zero entries add exact zeros to 1 - X U and are skipped
+  // with uniform branches.
+  if (threadIdx.x < 64 * NMW) {
+    const int t = threadIdx.x;
+    const bool nz = t < N2 * N2 && (sU[t].x != 0.0 || sU[t].y != 0.0);
+    const unsigned long long m = __ballot(nz);
+    if ((t & 63) == 0) sMask[t >> 6] = m;
+  }
+  __syncthreads();
+  unsigned long long umask[NMW];
+#pragma unroll
+  for (int w = 0; w < NMW; ++w) {
+    const unsigned long long m = sMask[w];
+    umask[w] = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(m >> 32)) << 32) |
+               (unsigned)__builtin_amdgcn_readfirstlane((unsigned)m);
+  }
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int gidx = lane / N2, i = lane - gidx * N2;
   const int slot_id = wave * P + gidx;
@@ -511,7 +531,10 @@ __global__ void __launch_bounds__(256) k_rpa(long npts, const c2 *__restrict__ c
   for (int j = 0; j < N2; ++j) {
     c2 s = (i == j) ? c2{1.0, 0.0} : c2{0.0, 0.0};
 #pragma unroll
-    for (int l = 0; l < N2; ++l) s = csub(s, cmul(X[l], sU[l * N2 + j]));
+    for (int l = 0; l < N2; ++l) {
+      const int bit = l * N2 + j;
+      if ((umask[bit >> 6] >> (bit & 63)) & 1ull) s = csub(s, cmul(X[l], sU[l * N2 + j]));
+    }
     A[j] = s;
   }
   bool used = false;
@@ -727,17 +750,103 @@ __global__ void __launch_bounds__(256) k_lattice3d(const c2 *__restrict__ in, c2
   }
 }
 
+// Register-resident 16^3 lattice FFT of one target column of one outer index.
+// Thread t = 16 hi + lo holds a 16-point line in registers and transforms it
+// with one radix-16 register DFT per axis; LDS only carries the two
+// transposes between axes, each in two rounds over half of the grid (34 KB),
+// so four workgroups fit on a CU.  In each round the same 128 threads write
+// all of their 16 values and then read their next line, so no second register
+// array is live.
+//   axis 0: thread (hi, lo) = (i1, i2) holds i0 = 0..15 (coalesced loads)
+//   axis 1: thread (hi, lo) = (k0, i2) holds i1; rounds split i2 = lo in halves
+//   axis 2: thread (hi, lo) = (k0, k1) holds i2; rounds split k0 = hi in halves
+template <int SIGN>
+__global__ void __launch_bounds__(256) k_lattice16r(const c2 *__restrict__ in, c2 *__restrict__ out, long ntarget,
+                                                    double scale) {
+  constexpr int NSM = 2048 + 2048 / 16;  // half grid, one pad slot per 16 entries (the larger of the two paddings)
+  __shared__ c2 sm[NSM];
+  const long tile = xcd_remap(blockIdx.x, gridDim.x);
+  const long w = tile / ntarget;
+  const long c = tile - w * ntarget;
+  const c2 *src = in + w * 4096 * ntarget + c;
+  c2 *dst = out + w * 4096 * ntarget + c;
+  const int t = threadIdx.x, hi = t >> 4, lo = t & 15;
+  c2 v[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) v[i] = src[(long)(i * 256 + t) * ntarget];
+  SmallDft<16, SIGN>::run(v, nullptr, 0);
+  // transpose 1: element (k0, i1, i2) of half h (i2 in [8h, 8h + 8)) at
+  // e = (k0 * 16 + i1) * 8 + (i2 - 8h), one pad slot per 128 entries
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if ((lo >> 3) == h) {
+      const int b = lo - 8 * h;
+#pragma unroll
+      for (int k0 = 0; k0 < 16; ++k0) {
+        const int e = (k0 * 16 + hi) * 8 + b;
+        sm[e + (e >> 7)] = v[k0];
+      }
+    }
+    __syncthreads();
+    if ((lo >> 3) == h) {
+      const int b = lo - 8 * h;
+#pragma unroll
+      for (int i1 = 0; i1 < 16; ++i1) {
+        const int e = (hi * 16 + i1) * 8 + b;
+        v[i1] = sm[e + (e >> 7)];
+      }
+    }
+    __syncthreads();
+  }
+  SmallDft<16, SIGN>::run(v, nullptr, 0);
+  // transpose 2: element (k0, k1, i2) of half h (k0 in [8h, 8h + 8)) at
+  // e = ((k0 - 8h) * 16 + k1) * 16 + i2, one pad slot per 16 entries
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if ((hi >> 3) == h) {
+      const int a = hi - 8 * h;
+#pragma unroll
+      for (int k1 = 0; k1 < 16; ++k1) {
+        const int e = (a * 16 + k1) * 16 + lo;
+        sm[e + (e >> 4)] = v[k1];
+      }
+    }
+    __syncthreads();
+    if ((hi >> 3) == h) {
+      const int a = hi - 8 * h;
+#pragma unroll
+      for (int i2 = 0; i2 < 16; ++i2) {
+        const int e = (a * 16 + lo) * 16 + i2;
+        v[i2] = sm[e + (e >> 4)];
+      }
+    }
+    __syncthreads();
+  }
+  SmallDft<16, SIGN>::run(v, nullptr, 0);
+#pragma unroll
+  for (int k2 = 0; k2 < 16; ++k2) dst[(long)(t * 16 + k2) * ntarget] = cscale(v[k2], scale);
+}
+
 template <int D0, int D1, int D2>
 static hipError_t launch_l3(hipStream_t s, const c2 *in, c2 *out, long nw, long ntarget, int sign, double scale) {
-  static_assert(AxisPlan<D0>::ok && AxisPlan<D1>::ok && AxisPlan<D2>::ok, "no radix plan");
-  constexpr int NK = D0 * D1 * D2;
-  constexpr int C = NK >= 4096 ? 1 : 4096 / NK;
-  const long nb = nw * ((ntarget + C - 1) / C);
-  if (sign < 0)
-    hipLaunchKernelGGL((k_lattice3d<D0, D1, D2, -1>), dim3(nb), dim3(256), 0, s, in, out, ntarget, scale);
-  else
-    hipLaunchKernelGGL((k_lattice3d<D0, D1, D2, 1>), dim3(nb), dim3(256), 0, s, in, out, ntarget, scale);
-  return hipGetLastError();
+  if constexpr (D0 == 16 && D1 == 16 && D2 == 16) {
+    const long nb = nw * ntarget;
+    if (sign < 0)
+      hipLaunchKernelGGL(k_lattice16r<-1>, dim3(nb), dim3(256), 0, s, in, out, ntarget, scale);
+    else
+      hipLaunchKernelGGL(k_lattice16r<1>, dim3(nb), dim3(256), 0, s, in, out, ntarget, scale);
+    return hipGetLastError();
+  } else {
+    static_assert(AxisPlan<D0>::ok && AxisPlan<D1>::ok && AxisPlan<D2>::ok, "no radix plan");
+    constexpr int NK = D0 * D1 * D2;
+    constexpr int C = NK >= 4096 ? 1 : 4096 / NK;
+    const long nb = nw * ((ntarget + C - 1) / C);
+    if (sign < 0)
+      hipLaunchKernelGGL((k_lattice3d<D0, D1, D2, -1>), dim3(nb), dim3(256), 0, s, in, out, ntarget, scale);
+    else
+      hipLaunchKernelGGL((k_lattice3d<D0, D1, D2, 1>), dim3(nb), dim3(256), 0, s, in, out, ntarget, scale);
+    return hipGetLastError();
+  }
 }
 
 // the lattice shapes with a single-pass kernel; hipErrorNotSupported otherwise
