@@ -83,11 +83,22 @@ def test_lattice_fft(L, oracle, dims, ntarget):
     x = rnd(rng, nw, nk, ntarget)
     c = _lib.ctx()
     xin = gpu(x)
-    for name, direction in (("tprf_fourier_wk_to_wr", -1), ("tprf_fourier_wr_to_wk", 1)):
-        out = torch.empty_like(xin)
-        _lib.call(name, c, nw, _lib.dims3(dims), ntarget, _lib.ptr(xin), _lib.ptr(out))
-        ref = oracle.fourier_lattice(x, dims, direction)
-        assert relerr(host(out), ref) <= 1e-13, name
+    # 16^3 has three single-pass variants (1, 2 or 4 target columns per
+    # workgroup, option 4 fixes one, -1 restores tuning); -1 alone elsewhere
+    variants = (0, 1, 2, -1) if tuple(dims) == (16, 16, 16) else (-1,)
+    for variant in variants:
+        _lib.call("tprf_ctx_set_option", c, 4, variant)
+        try:
+            for name, direction in (("tprf_fourier_wk_to_wr", -1), ("tprf_fourier_wr_to_wk", 1)):
+                ref = oracle.fourier_lattice(x, dims, direction)
+                out = torch.empty_like(xin)
+                _lib.call(name, c, nw, _lib.dims3(dims), ntarget, _lib.ptr(xin), _lib.ptr(out))
+                assert relerr(host(out), ref) <= 1e-13, (name, variant)
+                inplace = xin.clone()  # in == out
+                _lib.call(name, c, nw, _lib.dims3(dims), ntarget, _lib.ptr(inplace), _lib.ptr(inplace))
+                assert relerr(host(inplace), ref) <= 1e-13, (name, variant, "in place")
+        finally:
+            _lib.call("tprf_ctx_set_option", c, 4, -1)
 
 
 def test_lattice_fft_gf_roundtrip(L):

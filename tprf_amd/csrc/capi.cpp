@@ -67,6 +67,8 @@ struct tprf_ctx {
   std::map<std::tuple<int, int, int, int, long>, int> bubble_tuned;
   int w2t_variant = -1;
   std::map<std::tuple<int, int, int, int, long>, int> w2t_tuned;
+  int lattice_variant = -1;
+  std::map<std::tuple<long, long, int, int, int, int>, int> lattice_tuned;
 };
 
 namespace {
@@ -570,10 +572,19 @@ int lattice_fft(tprf_ctx *ctx, long nw, const int dims[3], long ntarget, const c
                  "hipMemcpyAsync");
     return TPRF_OK;
   }
-  if (!ctx->generic_kernels) {
-    hipError_t e = tprf::launch_lattice3d(ctx->stream, sign, in, out, nw, dims, ntarget, scale);
-    if (e != hipErrorNotSupported) return check_launch(ctx, e, "lattice 3-d kernel");
-    (void)hipGetLastError();
+  const int nvar = ctx->generic_kernels ? 0 : tprf::lattice3d_variants(dims);
+  if (nvar > 0) {
+    // single-pass kernel; with several variants the first call per shape times
+    // them all, except in place (in == out), where a timing launch would
+    // transform the data again: the cached or default variant runs then
+    auto launch = [&](int v) { return tprf::launch_lattice3d(ctx->stream, sign, in, out, nw, dims, ntarget, scale, v); };
+    const auto key = std::make_tuple(nw, ntarget, dims[0], dims[1], dims[2], sign);
+    int forced = (nvar == 1 || ctx->lattice_variant >= nvar) ? 0 : ctx->lattice_variant;
+    if (forced < 0 && in == out) {
+      auto it = ctx->lattice_tuned.find(key);
+      forced = it != ctx->lattice_tuned.end() ? it->second : tprf::lattice3d_default_variant(dims);
+    }
+    return tuned_launch(ctx, ctx->lattice_tuned, key, nvar, forced, launch, "lattice 3-d kernel");
   }
   const c2 *src = in;
   for (int d = 2; d >= 0; --d) {
@@ -643,6 +654,11 @@ int tprf_ctx_set_option(tprf_ctx *ctx, int option, int value) {
   if (!ctx) return TPRF_ERR_ARG;
   if (option == TPRF_OPT_GENERIC_KERNELS) {
     ctx->generic_kernels = value != 0;
+    return TPRF_OK;
+  }
+  if (option == TPRF_OPT_LATTICE_VARIANT) {
+    if (value < -1 || value > 2) return fail(ctx, TPRF_ERR_ARG, "bad lattice variant");
+    ctx->lattice_variant = value;
     return TPRF_OK;
   }
   if (option == TPRF_OPT_W2T_VARIANT) {

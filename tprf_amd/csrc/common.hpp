@@ -152,7 +152,9 @@ hipError_t launch_rpa_point(hipStream_t s, long npts, int n, const c2 *chi0, con
                             c2 *xbar, c2 *ubar, Status st);
 // kernels.hip: single-pass 3-D lattice FFT; hipErrorNotSupported for other dims
 hipError_t launch_lattice3d(hipStream_t s, int sign, const c2 *in, c2 *out, long nw, const int dims[3], long ntarget,
-                            double scale);
+                            double scale, int variant = 0);
+int lattice3d_variants(const int dims[3]);
+int lattice3d_default_variant(const int dims[3]);
 
 // largest LDS footprint (bytes) a Matsubara-axis kernel may request
 constexpr int kMatsLdsBytes = 128 * 1024;

@@ -750,30 +750,33 @@ __global__ void __launch_bounds__(256) k_lattice3d(const c2 *__restrict__ in, c2
   }
 }
 
-// Register-resident 16^3 lattice FFT of one target column of one outer index.
-// Thread t = 16 hi + lo holds a 16-point line in registers and transforms it
-// with one radix-16 register DFT per axis; LDS only carries the two
-// transposes between axes, each in two rounds over half of the grid (34 KB),
-// so four workgroups fit on a CU.  In each round the same 128 threads write
-// all of their 16 values and then read their next line, so no second register
-// array is live.
-//   axis 0: thread (hi, lo) = (i1, i2) holds i0 = 0..15 (coalesced loads)
-//   axis 1: thread (hi, lo) = (k0, i2) holds i1; rounds split i2 = lo in halves
-//   axis 2: thread (hi, lo) = (k0, k1) holds i2; rounds split k0 = hi in halves
-template <int SIGN>
-__global__ void __launch_bounds__(256) k_lattice16r(const c2 *__restrict__ in, c2 *__restrict__ out, long ntarget,
-                                                    double scale) {
+// Register-resident 16^3 lattice FFT of C adjacent target columns of one
+// outer index.  Thread t = C p + cc (p = 16 hi + lo) holds a 16-point line of
+// column cc in registers and transforms it with one radix-16 register DFT per
+// axis; lanes of adjacent columns are adjacent, so each row of the grid is
+// read and written as C * 16 contiguous bytes.  LDS (columns interleaved)
+// only carries the two transposes between axes, each in two rounds over half
+// of the grid; in each round the same threads write all of their 16 values
+// and then read their next line, so no second register array is live.
+//   axis 0: line p = (hi, lo) = (i1, i2) holds i0 = 0..15 (coalesced loads)
+//   axis 1: line p = (k0, i2) holds i1; rounds split i2 = lo in halves
+//   axis 2: line p = (k0, k1) holds i2; rounds split k0 = hi in halves
+template <int C, int SIGN>
+__global__ void __launch_bounds__(256 * C) k_lattice16r(const c2 *__restrict__ in, c2 *__restrict__ out,
+                                                        long ntarget, double scale) {
   constexpr int NSM = 2048 + 2048 / 16;  // half grid, one pad slot per 16 entries (the larger of the two paddings)
-  __shared__ c2 sm[NSM];
+  __shared__ c2 sm[NSM * C];
+  const long ntc = (ntarget + C - 1) / C;
   const long tile = xcd_remap(blockIdx.x, gridDim.x);
-  const long w = tile / ntarget;
-  const long c = tile - w * ntarget;
-  const c2 *src = in + w * 4096 * ntarget + c;
-  c2 *dst = out + w * 4096 * ntarget + c;
-  const int t = threadIdx.x, hi = t >> 4, lo = t & 15;
+  const long w = tile / ntc;
+  const long c0 = (tile - w * ntc) * C;
+  const int t = threadIdx.x, cc = t % C, p = t / C, hi = p >> 4, lo = p & 15;
+  const bool live = c0 + cc < ntarget;  // columns past the end are neither read nor written
+  const c2 *src = in + w * 4096 * ntarget + c0 + cc;
+  c2 *dst = out + w * 4096 * ntarget + c0 + cc;
   c2 v[16];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) v[i] = src[(long)(i * 256 + t) * ntarget];
+  for (int i = 0; i < 16; ++i) v[i] = live ? src[(long)(i * 256 + p) * ntarget] : c2{0.0, 0.0};
   SmallDft<16, SIGN>::run(v, nullptr, 0);
   // transpose 1: element (k0, i1, i2) of half h (i2 in [8h, 8h + 8)) at
   // e = (k0 * 16 + i1) * 8 + (i2 - 8h), one pad slot per 128 entries
@@ -784,7 +787,7 @@ __global__ void __launch_bounds__(256) k_lattice16r(const c2 *__restrict__ in, c
 #pragma unroll
       for (int k0 = 0; k0 < 16; ++k0) {
         const int e = (k0 * 16 + hi) * 8 + b;
-        sm[e + (e >> 7)] = v[k0];
+        sm[(e + (e >> 7)) * C + cc] = v[k0];
       }
     }
     __syncthreads();
@@ -793,7 +796,7 @@ __global__ void __launch_bounds__(256) k_lattice16r(const c2 *__restrict__ in, c
 #pragma unroll
       for (int i1 = 0; i1 < 16; ++i1) {
         const int e = (hi * 16 + i1) * 8 + b;
-        v[i1] = sm[e + (e >> 7)];
+        v[i1] = sm[(e + (e >> 7)) * C + cc];
       }
     }
     __syncthreads();
@@ -808,7 +811,7 @@ __global__ void __launch_bounds__(256) k_lattice16r(const c2 *__restrict__ in, c
 #pragma unroll
       for (int k1 = 0; k1 < 16; ++k1) {
         const int e = (a * 16 + k1) * 16 + lo;
-        sm[e + (e >> 4)] = v[k1];
+        sm[(e + (e >> 4)) * C + cc] = v[k1];
       }
     }
     __syncthreads();
@@ -817,26 +820,38 @@ __global__ void __launch_bounds__(256) k_lattice16r(const c2 *__restrict__ in, c
 #pragma unroll
       for (int i2 = 0; i2 < 16; ++i2) {
         const int e = (a * 16 + lo) * 16 + i2;
-        v[i2] = sm[e + (e >> 4)];
+        v[i2] = sm[(e + (e >> 4)) * C + cc];
       }
     }
     __syncthreads();
   }
   SmallDft<16, SIGN>::run(v, nullptr, 0);
+  if (live) {
 #pragma unroll
-  for (int k2 = 0; k2 < 16; ++k2) dst[(long)(t * 16 + k2) * ntarget] = cscale(v[k2], scale);
+    for (int k2 = 0; k2 < 16; ++k2) dst[(long)(p * 16 + k2) * ntarget] = cscale(v[k2], scale);
+  }
+}
+
+template <int C>
+static hipError_t launch_l16r(hipStream_t s, const c2 *in, c2 *out, long nw, long ntarget, int sign, double scale) {
+  const long nb = nw * ((ntarget + C - 1) / C);
+  if (sign < 0)
+    hipLaunchKernelGGL((k_lattice16r<C, -1>), dim3(nb), dim3(256 * C), 0, s, in, out, ntarget, scale);
+  else
+    hipLaunchKernelGGL((k_lattice16r<C, 1>), dim3(nb), dim3(256 * C), 0, s, in, out, ntarget, scale);
+  return hipGetLastError();
 }
 
 template <int D0, int D1, int D2>
-static hipError_t launch_l3(hipStream_t s, const c2 *in, c2 *out, long nw, long ntarget, int sign, double scale) {
+static hipError_t launch_l3(hipStream_t s, const c2 *in, c2 *out, long nw, long ntarget, int sign, double scale,
+                            int variant) {
   if constexpr (D0 == 16 && D1 == 16 && D2 == 16) {
-    const long nb = nw * ntarget;
-    if (sign < 0)
-      hipLaunchKernelGGL(k_lattice16r<-1>, dim3(nb), dim3(256), 0, s, in, out, ntarget, scale);
-    else
-      hipLaunchKernelGGL(k_lattice16r<1>, dim3(nb), dim3(256), 0, s, in, out, ntarget, scale);
-    return hipGetLastError();
+    // columns per workgroup 1, 2 or 4 (same results)
+    if (variant == 1) return launch_l16r<2>(s, in, out, nw, ntarget, sign, scale);
+    if (variant == 2) return launch_l16r<4>(s, in, out, nw, ntarget, sign, scale);
+    return launch_l16r<1>(s, in, out, nw, ntarget, sign, scale);
   } else {
+    (void)variant;
     static_assert(AxisPlan<D0>::ok && AxisPlan<D1>::ok && AxisPlan<D2>::ok, "no radix plan");
     constexpr int NK = D0 * D1 * D2;
     constexpr int C = NK >= 4096 ? 1 : 4096 / NK;
@@ -849,12 +864,30 @@ static hipError_t launch_l3(hipStream_t s, const c2 *in, c2 *out, long nw, long 
   }
 }
 
+// lattice shapes with a single-pass kernel
+static const int kLattice3dShapes[][3] = {{16, 16, 16}, {12, 12, 12}, {8, 8, 8}, {6, 6, 6},  {4, 4, 4},
+                                          {2, 2, 2},    {64, 64, 1},  {48, 48, 1}, {32, 32, 1}, {24, 24, 1},
+                                          {16, 16, 1},  {12, 12, 1},  {8, 8, 1},  {6, 6, 1},   {4, 4, 1},
+                                          {2, 2, 1},    {16, 1, 1},   {8, 1, 1},  {4, 4, 2},   {8, 8, 4}};
+
+// number of single-pass kernel variants (same results) for a lattice shape, 0 without one
+int lattice3d_variants(const int dims[3]) {
+  if (dims[0] == 16 && dims[1] == 16 && dims[2] == 16) return 3;
+  for (const auto &d : kLattice3dShapes)
+    if (d[0] == dims[0] && d[1] == dims[1] && d[2] == dims[2]) return 1;
+  return 0;
+}
+// variant used when it cannot be timed (in-place calls before the first tuned call)
+int lattice3d_default_variant(const int dims[3]) {
+  return (dims[0] == 16 && dims[1] == 16 && dims[2] == 16) ? 1 : 0;
+}
+
 // the lattice shapes with a single-pass kernel; hipErrorNotSupported otherwise
 hipError_t launch_lattice3d(hipStream_t s, int sign, const c2 *in, c2 *out, long nw, const int dims[3], long ntarget,
-                            double scale) {
+                            double scale, int variant) {
   const int a = dims[0], b = dims[1], c = dims[2];
 #define TPRF_L3(A, B, C_)                                                              \
-  if (a == A && b == B && c == C_) return launch_l3<A, B, C_>(s, in, out, nw, ntarget, sign, scale);
+  if (a == A && b == B && c == C_) return launch_l3<A, B, C_>(s, in, out, nw, ntarget, sign, scale, variant);
   TPRF_L3(16, 16, 16)
   TPRF_L3(12, 12, 12)
   TPRF_L3(8, 8, 8)
