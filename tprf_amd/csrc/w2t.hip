@@ -44,8 +44,13 @@ constexpr int kFitSplit = 4;  // partial sums per fitted coefficient
 
 // NT threads per workgroup: C = NT / (M / 8) columns, i.e. C * 16 contiguous
 // bytes of every row written per workgroup.
+// Fermionic M = 256, 512 with 256 threads: registers held to the three
+// workgroups per CU that the LDS allows (they fit without spilling); two
+// waves per SIMD otherwise.
 template <int STAT, int M, int NT>
-__global__ void __launch_bounds__(NT, 512 / NT) k_w2t_pruned(double beta, int n_max, long ncol,
+__global__ void __launch_bounds__(NT)
+    __attribute__((amdgpu_waves_per_eu((STAT == 0 && (M == 256 || M == 512) && NT == 256) ? 3 : 2)))
+    k_w2t_pruned(double beta, int n_max, long ncol,
                                                              const c2 *__restrict__ in, c2 *__restrict__ out,
                                                              const c2 *__restrict__ moments, TailFitDev fit,
                                                              MatsDev mt, W2tDev wd, double *status) {
@@ -99,8 +104,25 @@ __global__ void __launch_bounds__(NT, 512 / NT) k_w2t_pruned(double beta, int n_
       const int part = t % kFitSplit, cq = t / kFitSplit;
       const int cc = cq / nc, q = cq - cc * nc;
       const int f0 = part * fq, f1 = (f0 + fq) < nf ? (f0 + fq) : nf;
+      // batches of 8 terms with clamped (always valid) indices, so a batch's
+      // table loads are all issued before the first product waits on one;
+      // the same terms are summed in the same order
+      constexpr int kFq = (kMaxFit + kFitSplit - 1) / kFitSplit, kB = 4;
       c2 s = c2{0.0, 0.0};
-      for (int f = f0; f < f1; ++f) s = cadd(s, cmul(fit.pinv[q * nf + f], fitval(f, cc)));
+#pragma unroll
+      for (int f8 = 0; f8 < kFq; f8 += kB) {
+        if (f0 + f8 >= f1) break;
+        c2 pv[kB], xv[kB];
+#pragma unroll
+        for (int ff = 0; ff < kB; ++ff) {
+          const int f = min(f0 + f8 + ff, nf - 1);
+          pv[ff] = fit.pinv[q * nf + f];
+          xv[ff] = fitval(f, cc);
+        }
+#pragma unroll
+        for (int ff = 0; ff < kB; ++ff)
+          if (f0 + f8 + ff < f1) s = cadd(s, cmul(pv[ff], xv[ff]));
+      }
       s_cpart[cc][q][part] = s;
     }
     __syncthreads();
@@ -115,7 +137,15 @@ __global__ void __launch_bounds__(NT, 512 / NT) k_w2t_pruned(double beta, int n_
     for (int t = tid; t < cn * nf; t += NT) {
       const int cc = t / nf, f = t - cc * nf;
       c2 v = fitval(f, cc);
-      for (int q = 0; q < nc; ++q) v = csub(v, cmul(fit.vander[f * nc + q], s_coef[cc][q]));
+#pragma unroll
+      for (int q5 = 0; q5 < kMaxOrder; q5 += 5) {
+        c2 vd[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) vd[q] = fit.vander[f * nc + min(q5 + q, nc - 1)];
+#pragma unroll
+        for (int q = 0; q < 5; ++q)
+          if (q5 + q < nc) v = csub(v, cmul(vd[q], s_coef[cc][q5 + q]));
+      }
       s_res[cc][f] = cabs2(v);
     }
     __syncthreads();
