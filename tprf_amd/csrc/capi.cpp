@@ -60,7 +60,7 @@ struct tprf_ctx {
   std::map<std::tuple<double, int, int, int, int, int>, BubbleEntry> bubbles;
   std::map<std::tuple<int, double, int>, c2 *> pole_inv;
   std::map<std::tuple<double, int>, double *> static_w;  // nu = 0 weights c_j
-  std::map<std::tuple<int, double, int, int>, c2 *> dtail;  // out-of-mesh pole model in tau
+  std::map<std::tuple<int, double, int, int>, double *> dtail;  // out-of-mesh pole model in tau (real)
   void *work = nullptr;                                   // grow-only scratch
   size_t work_bytes = 0;
   int bubble_variant = -1;                                // -1: autotuned per shape
@@ -307,7 +307,7 @@ int get_pole_inv(tprf_ctx *ctx, int statistic, double beta, int n_max, int nmesh
 // needs a_q-bar = sum_j conj(D_q(tau_j)) gbar_j.  D is small (the pole tail beyond
 // the mesh) while both of its terms are O(1); it is formed here in long double
 // so that the adjoint does not take the difference of two large sums.
-int get_dtail(tprf_ctx *ctx, int statistic, double beta, int n_max, int L, const c2 **out) {
+int get_dtail(tprf_ctx *ctx, int statistic, double beta, int n_max, int L, const double **out) {
   auto key = std::make_tuple(statistic, beta, n_max, L);
   auto it = ctx->dtail.find(key);
   if (it != ctx->dtail.end()) { *out = it->second; return TPRF_OK; }
@@ -317,7 +317,7 @@ int get_dtail(tprf_ctx *ctx, int statistic, double beta, int n_max, int L, const
   const int first = statistic == TPRF_FERMION ? -n_max : -(n_max - 1);
   const int nmesh = statistic == TPRF_FERMION ? 2 * n_max : 2 * n_max - 1;
   const long double lb = beta;
-  std::vector<c2> d(3 * (size_t)L);
+  std::vector<double> d(3 * (size_t)L);
   std::vector<lc> pole(3 * (size_t)nmesh);
   for (int q = 0; q < 3; ++q)
     for (int i = 0; i < nmesh; ++i) {
@@ -349,11 +349,13 @@ int get_dtail(tprf_ctx *ctx, int statistic, double beta, int n_max, int L, const
         psi = bq >= 0 ? -expl(-bq * tau) / (1 + expl(-lb * bq)) : -expl(bq * (lb - tau)) / (1 + expl(lb * bq));
       else
         psi = bq >= 0 ? expl(-bq * tau) / (expl(-lb * bq) - 1) : expl(bq * (lb - tau)) / (1 - expl(bq * lb));
+      // real: the mesh is symmetric under w -> -w and the terms of w and -w are
+      // complex conjugates (b_q real), so only rounding is left in v.imag()
       const lc v = lc(psi, 0) - s[q] / lb;
-      d[(size_t)q * L + j] = c2{(double)v.real(), (double)v.imag()};
+      d[(size_t)q * L + j] = (double)v.real();
     }
   }
-  c2 *dd;
+  double *dd;
   int st = upload(ctx, &dd, d);
   if (st) return st;
   ctx->dtail.emplace(key, dd);
@@ -947,7 +949,7 @@ int tprf_fourier_wr_to_tr_adjoint(tprf_ctx *ctx, int statistic, double beta, int
   if ((st = get_fft(ctx, M, &pM))) return st;
   if ((st = get_mats(ctx, statistic, beta, L, &mt))) return st;
   if ((st = get_pole_inv(ctx, statistic, beta, n_max, nmesh, &pinv))) return st;
-  const c2 *dtail;
+  const double *dtail;
   if ((st = get_dtail(ctx, statistic, beta, n_max, L, &dtail))) return st;
   tprf::W2tDev wd{M, L / M, pL->tw, pM->tw, pinv, dtail};
   return check_launch(ctx,

@@ -339,10 +339,10 @@ __global__ void __launch_bounds__(kW2tThreads, 2) k_w2t_adjoint(double beta, int
       const int jt = Q * (j + TPC * r) + t;
       c2 g = gbar_t[(long)jt * ncol + col];
       if (jt == 0) g = cadd(g, cscale(gL, pm));
-      // pole model net of its in-mesh part: abar_q += conj(D_q(tau_j)) gbar_j
-      ab0 = cadd(ab0, cmul(cconj(wd.dtail[jt]), g));
-      ab1 = cadd(ab1, cmul(cconj(wd.dtail[L + jt]), g));
-      ab2 = cadd(ab2, cmul(cconj(wd.dtail[2 * L + jt]), g));
+      // pole model net of its in-mesh part: abar_q += D_q(tau_j) gbar_j (D_q real)
+      ab0 = cadd(ab0, cscale(g, wd.dtail[jt]));
+      ab1 = cadd(ab1, cscale(g, wd.dtail[L + jt]));
+      ab2 = cadd(ab2, cscale(g, wd.dtail[2 * L + jt]));
       v[r] = (STAT == 0) ? cmul(cconj(mt.phase[jt]), g) : g;
     }
     int Ns = 1;
