@@ -478,8 +478,10 @@ __device__ __forceinline__ void wave_lds_sync() {
 // solved as (1 - X U) Y = X by Gauss-Jordan with partial pivoting.  A group
 // of N2 = n^2 lanes owns one point, lane i holding row i of [A | X]; the pivot
 // row is exchanged through LDS.
+// n = 3: registers held to four waves per SIMD (a few spills outside the
+// pivot loop cost less than the lost wave: 0.51 -> 0.47 ms at 127 x 4096 points)
 template <int N>
-__global__ void __launch_bounds__(256) k_rpa(long npts, const c2 *__restrict__ chi0, const c2 *__restrict__ U,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N == 3 ? 4 : 1))) k_rpa(long npts, const c2 *__restrict__ chi0, const c2 *__restrict__ U,
                                              c2 *__restrict__ chi, double *status) {
   constexpr int N2 = N * N;
   constexpr int P = 64 / N2;  // points per wave
@@ -543,16 +545,19 @@ __global__ void __launch_bounds__(256) k_rpa(long npts, const c2 *__restrict__ c
   bool singular = false;
 #pragma unroll
   for (int k = 0; k < N2; ++k) {
-    const double mag = (!used && valid) ? cabs2(A[k]) : -1.0;
-    int best_i = 0;
-    double best = -2.0;
+    // Pivot key: the upper 32 bits of |A[k]|^2 (sign, exponent and 20 mantissa
+    // bits of a non-negative double order like the value), one 32-bit shuffle
+    // per candidate; -1 for rows that cannot pivot.  Magnitudes within ~1e-6
+    // of each other may pick either row, which only changes rounding.
+    const int key = (!used && valid) ? (int)(__double_as_longlong(cabs2(A[k])) >> 32) : -1;
+    int best_i = 0, best = -2;
 #pragma unroll
     for (int s = 0; s < N2; ++s) {
-      const double m = __shfl(mag, (lane - i) + s, 64);
+      const int m = __shfl(key, (lane - i) + s, 64);
       if (m > best) { best = m; best_i = s; }
     }
     const bool is_piv = (i == best_i);
-    if (valid && best <= 0.0) singular = true;
+    if (valid && best <= 0) singular = true;  // |A[k]|^2 below ~1e-314 in every candidate row
     if (is_piv && gidx < P) {
       used = true;
       myk = k;
