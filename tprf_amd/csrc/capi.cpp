@@ -58,7 +58,6 @@ struct tprf_ctx {
   std::map<std::tuple<int, double, int>, MatsEntry> mats;
   tprf::Vinv8 vinv;
   std::map<std::tuple<double, int, int, int, int, int>, BubbleEntry> bubbles;
-  std::map<std::tuple<int, double, int>, c2 *> pole_inv;
   std::map<std::tuple<double, int>, double *> static_w;  // nu = 0 weights c_j
   std::map<std::tuple<int, double, int, int>, double *> dtail;  // out-of-mesh pole model in tau (real)
   void *work = nullptr;                                   // grow-only scratch
@@ -278,35 +277,13 @@ void vandermonde_inverse8(double *vinv) {
   }
 }
 
-// 1 / (i w_n - b_q) on the mesh (statistic, beta, n_max), q = 0..2, for the
-// pole-model subtraction of the pruned Matsubara -> tau kernel
-int get_pole_inv(tprf_ctx *ctx, int statistic, double beta, int n_max, int nmesh, const c2 **out) {
-  auto key = std::make_tuple(statistic, beta, n_max);
-  auto it = ctx->pole_inv.find(key);
-  if (it != ctx->pole_inv.end()) { *out = it->second; return TPRF_OK; }
-  const double bf[3] = {0.0, 1.0, -1.0}, bb[3] = {-0.5, -1.0, 1.0};
-  const double *b = statistic == TPRF_FERMION ? bf : bb;
-  const int first = statistic == TPRF_FERMION ? -n_max : -(n_max - 1);
-  std::vector<c2> v(3 * (size_t)nmesh);
-  for (int q = 0; q < 3; ++q)
-    for (int i = 0; i < nmesh; ++i) {
-      std::complex<double> z = 1.0 / std::complex<double>(-b[q], matsubara(statistic, first + i, beta));
-      v[(size_t)q * nmesh + i] = c2{z.real(), z.imag()};
-    }
-  c2 *d;
-  int st = upload(ctx, &d, v);
-  if (st) return st;
-  ctx->pole_inv.emplace(key, d);
-  *out = d;
-  return TPRF_OK;
-}
-
 // D_q(tau_j) = psi_q(tau_j) - (1/beta) sum_{n in mesh} exp(-i w_n tau_j) / (i w_n - b_q),
 // j = 0..L-1: what the pole model adds in tau beyond its own in-mesh transform.
-// The omega -> tau map adds sum_q a_q D_q(tau_j) to its FFT part, so its adjoint
-// needs a_q-bar = sum_j conj(D_q(tau_j)) gbar_j.  D is small (the pole tail beyond
-// the mesh) while both of its terms are O(1); it is formed here in long double
-// so that the adjoint does not take the difference of two large sums.
+// The omega -> tau map is its FFT of G / beta plus sum_q a_q D_q(tau_j), so the
+// forward kernel adds this table at its outputs and the adjoint needs
+// a_q-bar = sum_j D_q(tau_j) gbar_j.  D is small (the pole tail beyond the mesh)
+// while both of its terms are O(1); it is formed here in long double so that
+// neither kernel takes the difference of two large sums.
 int get_dtail(tprf_ctx *ctx, int statistic, double beta, int n_max, int L, const double **out) {
   auto key = std::make_tuple(statistic, beta, n_max, L);
   auto it = ctx->dtail.find(key);
@@ -639,7 +616,6 @@ int tprf_ctx_destroy(tprf_ctx *ctx) {
   for (auto &kv : ctx->fft) (void)hipFree(kv.second.tw);
   for (auto &kv : ctx->fits) { (void)hipFree(kv.second.idx); (void)hipFree(kv.second.pinv); (void)hipFree(kv.second.vander); }
   for (auto &kv : ctx->mats) { (void)hipFree(kv.second.psi); (void)hipFree(kv.second.phase); }
-  for (auto &kv : ctx->pole_inv) (void)hipFree(kv.second);
   for (auto &kv : ctx->static_w) (void)hipFree(kv.second);
   for (auto &kv : ctx->dtail) (void)hipFree(kv.second);
   if (ctx->work) (void)hipFree(ctx->work);
@@ -744,7 +720,6 @@ int tprf_fourier_wr_to_tr(tprf_ctx *ctx, int statistic, double beta, int n_max, 
     fit = *f;
   }
   // input-pruned kernel: L = Q M with M a power of two holding the whole mesh
-  const int nmesh = statistic == TPRF_FERMION ? 2 * n_max : 2 * n_max - 1;
   int M = 64;
   while (M < 2 * n_max) M *= 2;
   if (!ctx->generic_kernels && L % M == 0 && tprf::w2t_pruned_supported(M) && L / M <= 64) {
@@ -752,9 +727,9 @@ int tprf_fourier_wr_to_tr(tprf_ctx *ctx, int statistic, double beta, int n_max, 
     if ((st = get_fft(ctx, L, &pL))) return st;
     if ((st = get_fft(ctx, M, &pM))) return st;
     if ((st = get_mats(ctx, statistic, beta, L, &mt))) return st;
-    const c2 *pinv;
-    if ((st = get_pole_inv(ctx, statistic, beta, n_max, nmesh, &pinv))) return st;
-    tprf::W2tDev wd{M, L / M, pL->tw, pM->tw, pinv, nullptr};
+    const double *dtail;
+    if ((st = get_dtail(ctx, statistic, beta, n_max, L, &dtail))) return st;
+    tprf::W2tDev wd{M, L / M, pL->tw, pM->tw, dtail};
     auto launch = [&](int v) {
       return tprf::launch_w2t_pruned(ctx->stream, statistic, beta, n_max, ncol, (const c2 *)in, (c2 *)out,
                                      (const c2 *)moments, fit, *mt, wd, tprf::Status{ctx->d_status}, v);
@@ -930,7 +905,6 @@ int tprf_fourier_wr_to_tr_adjoint(tprf_ctx *ctx, int statistic, double beta, int
   if (L < 2 * n_max)
     return fail(ctx, TPRF_ERR_MESH_TOO_SHORT,
                 "Inverse Fourier: The time mesh mush be at least twice as long as the freq mesh");
-  const int nmesh = statistic == TPRF_FERMION ? 2 * n_max : 2 * n_max - 1;
   int M = 64;
   while (M < 2 * n_max) M *= 2;
   if (L % M != 0 || !tprf::w2t_pruned_supported(M) || L / M > 64)
@@ -944,14 +918,12 @@ int tprf_fourier_wr_to_tr_adjoint(tprf_ctx *ctx, int statistic, double beta, int
   }
   const tprf::FftDev *pL, *pM;
   const tprf::MatsDev *mt;
-  const c2 *pinv;
   if ((st = get_fft(ctx, L, &pL))) return st;
   if ((st = get_fft(ctx, M, &pM))) return st;
   if ((st = get_mats(ctx, statistic, beta, L, &mt))) return st;
-  if ((st = get_pole_inv(ctx, statistic, beta, n_max, nmesh, &pinv))) return st;
   const double *dtail;
   if ((st = get_dtail(ctx, statistic, beta, n_max, L, &dtail))) return st;
-  tprf::W2tDev wd{M, L / M, pL->tw, pM->tw, pinv, dtail};
+  tprf::W2tDev wd{M, L / M, pL->tw, pM->tw, dtail};
   return check_launch(ctx,
                       tprf::launch_w2t_adjoint(ctx->stream, statistic, beta, n_max, ncol, (const c2 *)gbar_tr,
                                                (c2 *)gbar_wr, fitted ? 1 : 0, fit, *mt, wd),

@@ -89,9 +89,8 @@ struct W2tDev {
   int M, Q;
   const c2 *twL;         // [L] exp(-2 pi i k / L)
   const c2 *twM;         // [M] exp(-2 pi i k / M)
-  const c2 *pole_inv;    // [3][nmesh] 1 / (i w_n - b_q)
-  const double *dtail;   // [3][L] psi_q(tau_j) - (1/beta) sum_{n in mesh} exp(-i w_n tau_j) / (i w_n - b_q), real
-                         // (adjoint only; the part of the pole model beyond the mesh)
+  const double *dtail;   // [3][L] psi_q(tau_j) - (1/beta) sum_{n in mesh} exp(-i w_n tau_j) / (i w_n - b_q), real:
+                         // the part of the pole model beyond the mesh, added at the outputs
 };
 
 // Status words written by kernels (device memory, 4 doubles):

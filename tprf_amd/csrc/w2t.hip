@@ -168,20 +168,14 @@ __global__ void __launch_bounds__(NT)
   const double ib = 1.0 / beta;
   const double pm = (STAT == 0) ? -1.0 : 1.0;
 
-  // 3. subtract the pole model once: raw <- (G - sum_q a_q / (i w_n - b_q)) / beta
+  // 3. The pole model is not subtracted from the inputs: by linearity
+  //      phase DFT[(G - sum_q a_q pole_q) / beta] + sum_q a_q psi_q
+  //    = phase DFT[G / beta] + sum_q a_q D_q(tau),
+  //    D_q = psi_q - (1/beta) sum_{n in mesh} exp(-i w_n tau) pole_q(n) (the
+  //    real table wd.dtail, long double on the host), so only the scale is
+  //    applied here and the model enters at the outputs.
 #pragma unroll
-  for (int r = 0; r < 8; ++r) {
-    const int e = j + TPC * r;
-    const int n = e < M / 2 ? e : e - M;
-    if (n >= first && n < first + nmesh) {
-      const int i = n - first;
-      c2 x = raw[r];
-      x = csub(x, cmul(a0, wd.pole_inv[i]));
-      x = csub(x, cmul(a1, wd.pole_inv[nmesh + i]));
-      x = csub(x, cmul(a2, wd.pole_inv[2 * nmesh + i]));
-      raw[r] = cscale(x, ib);
-    }
-  }
+  for (int r = 0; r < 8; ++r) raw[r] = cscale(raw[r], ib);
 
   for (int t = 0; t < Q; ++t) {
     // pass 1 inputs: FFT index e = j + TPC r  <->  n = e or e - M, twiddled by exp(-2 pi i t n / L)
@@ -236,7 +230,7 @@ __global__ void __launch_bounds__(NT)
           const int jt = Q * jp + t;
           c2 y = v[u];
           if (STAT == 0) y = cmul(y, mt.phase[jt]);
-          const double p0 = mt.psi[jt], p1 = mt.psi[L + jt], p2 = mt.psi[2 * L + jt];
+          const double p0 = wd.dtail[jt], p1 = wd.dtail[L + jt], p2 = wd.dtail[2 * L + jt];
           y.x += a0.x * p0 + a1.x * p1 + a2.x * p2;
           y.y += a0.y * p0 + a1.y * p1 + a2.y * p2;
           out[(long)jt * ncol + col] = y;
