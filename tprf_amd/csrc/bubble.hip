@@ -289,10 +289,30 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
       for (int k2 = 0; k2 < R2; ++k2) sS[c * P + k1 + R1 * k2] = acc[i][k2];
     }
   }
+  // The pole-model table H_q(m) (3 x nnu entries) goes into the tile buffers,
+  // free now (sA, then sB): all of a thread's loads are issued before its LDS
+  // stores, and the outputs below read LDS instead of waiting on a global load
+  // before each store.
+  const int nnu = 2 * nb - 1;
+  static_assert(3 * P <= 2 * NSTAGE, "the tile buffers hold the pole-model table");
+  {
+    constexpr int NH = (3 * P + NT - 1) / NT;
+    v2d hv[NH];
+#pragma unroll
+    for (int w = 0; w < NH; ++w) {
+      const int f = tid + w * NT;
+      hv[w] = ld_v2d(bd.hmod + (f < 3 * nnu ? f : 0));
+    }
+#pragma unroll
+    for (int w = 0; w < NH; ++w) {
+      const int f = tid + w * NT;
+      if (f < 3 * nnu) st_v2d(f < NSTAGE ? sA + f : sB + (f - NSTAGE), hv[w]);
+    }
+  }
   __syncthreads();
+  auto hmod = [&](int f) { return f < NSTAGE ? sA[f] : sB[f - NSTAGE]; };
   const double fact = beta / (double)L;
   const bool mirror = (mr != r);
-  const int nnu = 2 * nb - 1;
 #pragma unroll
   for (int i = 0; i < U2; ++i) {
     const int u = tid + i * NT;
@@ -309,16 +329,13 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
         const int m = k < P / 2 ? k : k - P;
         if (m <= -nb || m >= nb) continue;
         const int im = m + nb - 1;
-        const c2 f0 = bd.phi[im], f1 = bd.phi[nnu + im], f2 = bd.phi[2 * nnu + im];
-        c2 model = cadd(cadd(cmul(a0[0], f0), cmul(a0[1], f1)), cmul(a0[2], f2));
-        const c2 g0 = bd.pinv_nu[im], g1 = bd.pinv_nu[nnu + im], g2 = bd.pinv_nu[2 * nnu + im];
-        c2 v = cadd(cscale(csub(acc[i][k2], model), fact),
-                    cadd(cadd(cmul(a0[0], g0), cmul(a0[1], g1)), cmul(a0[2], g2)));
+        // y(m) = fact S(m) + sum_q a_q H_q(m), H_q = 1 / (i nu_m - b_q) - fact Phi_q(m)
+        const c2 h0 = hmod(im), h1 = hmod(nnu + im), h2 = hmod(2 * nnu + im);
+        c2 v = cadd(cscale(acc[i][k2], fact), cadd(cadd(cmul(a0[0], h0), cmul(a0[1], h1)), cmul(a0[2], h2)));
         out[((long)im * nr + r) * n4 + q] = v;
         if (mirror) {
           const c2 sm = cadd(sS[c * P + ((P - k) & (P - 1))], xl_minus_x0);
-          model = cadd(cadd(cmul(a1[0], f0), cmul(a1[1], f1)), cmul(a1[2], f2));
-          v = cadd(cscale(csub(sm, model), fact), cadd(cadd(cmul(a1[0], g0), cmul(a1[1], g1)), cmul(a1[2], g2)));
+          v = cadd(cscale(sm, fact), cadd(cadd(cmul(a1[0], h0), cmul(a1[1], h1)), cmul(a1[2], h2)));
           out[((long)im * nr + mr) * n4 + qm] = v;
         }
       }

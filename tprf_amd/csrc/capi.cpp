@@ -43,7 +43,7 @@ struct MatsEntry {
 
 struct BubbleEntry {
   tprf::BubbleDev dev;
-  c2 *tw1 = nullptr, *twp = nullptr, *phi = nullptr, *pinv_nu = nullptr;
+  c2 *tw1 = nullptr, *twp = nullptr, *hmod = nullptr;
   int *rcanon = nullptr;
 };
 
@@ -358,7 +358,7 @@ int get_bubble(tprf_ctx *ctx, double beta, int L, int nb, const int dims[3], int
     long r = ((jm % L) + L) % L;
     return std::complex<long double>(cs[r], sn[r]);
   };
-  std::vector<c2> tw1((size_t)N1 * P), twp(P), phi(3 * (size_t)nnu), pinv_nu(3 * (size_t)nnu);
+  std::vector<c2> tw1((size_t)N1 * P), twp(P), hmod(3 * (size_t)nnu);
   for (int j1 = 0; j1 < N1; ++j1)
     for (int k = 0; k < P; ++k) {
       const long m = k < P / 2 ? k : k - P;
@@ -369,7 +369,10 @@ int get_bubble(tprf_ctx *ctx, double beta, int L, int nb, const int dims[3], int
     long double a = 2.0L * kPiL * (long double)p / (long double)P;
     twp[p] = c2{(double)cosl(a), (double)sinl(a)};
   }
+  // H_q(m) = 1 / (i nu_m - b_q) - (beta / L) Phi_q(m), Phi_q(m) the transform of
+  // the model's tau samples: the kernel's output is (beta/L) S(m) + sum_q a_q H_q(m)
   const double bb[3] = {-0.5, -1.0, 1.0};
+  const long double fact = (long double)beta / (long double)L;
   for (int q = 0; q < 3; ++q) {
     std::vector<double> ps(L);
     for (int j = 0; j < L; ++j) ps[j] = one_boson(bb[q], (double)j * (beta / (double)L), beta);
@@ -377,9 +380,10 @@ int get_bubble(tprf_ctx *ctx, double beta, int L, int nb, const int dims[3], int
       const long m = im - (nb - 1);
       std::complex<long double> s = 0;
       for (int j = 0; j < L; ++j) s += (long double)ps[j] * e((long)j * m);
-      phi[(size_t)q * nnu + im] = c2{(double)s.real(), (double)s.imag()};
-      std::complex<double> z = 1.0 / std::complex<double>(-bb[q], 2.0 * M_PI * (double)m / beta);
-      pinv_nu[(size_t)q * nnu + im] = c2{z.real(), z.imag()};
+      const std::complex<long double> z =
+          1.0L / std::complex<long double>(-(long double)bb[q], 2.0L * kPiL * (long double)m / (long double)beta);
+      const std::complex<long double> h = z - fact * s;
+      hmod[(size_t)q * nnu + im] = c2{(double)h.real(), (double)h.imag()};
     }
   }
   const long nr = (long)dims[0] * dims[1] * dims[2];
@@ -390,23 +394,21 @@ int get_bubble(tprf_ctx *ctx, double beta, int L, int nb, const int dims[3], int
     if (r <= mr) canon.push_back((int)r);
   }
   if (it != ctx->bubbles.end()) {
-    (void)hipFree(it->second.tw1); (void)hipFree(it->second.twp); (void)hipFree(it->second.phi);
-    (void)hipFree(it->second.pinv_nu); (void)hipFree(it->second.rcanon);
+    (void)hipFree(it->second.tw1); (void)hipFree(it->second.twp); (void)hipFree(it->second.hmod);
+    (void)hipFree(it->second.rcanon);
     ctx->bubbles.erase(it);
   }
   BubbleEntry b;
   int st;
   if ((st = upload(ctx, &b.tw1, tw1))) return st;
   if ((st = upload(ctx, &b.twp, twp))) return st;
-  if ((st = upload(ctx, &b.phi, phi))) return st;
-  if ((st = upload(ctx, &b.pinv_nu, pinv_nu))) return st;
+  if ((st = upload(ctx, &b.hmod, hmod))) return st;
   if ((st = upload(ctx, &b.rcanon, canon))) return st;
   b.dev.P = P;
   b.dev.ncanon = (int)canon.size();
   b.dev.tw1 = b.tw1;
   b.dev.twp = b.twp;
-  b.dev.phi = b.phi;
-  b.dev.pinv_nu = b.pinv_nu;
+  b.dev.hmod = b.hmod;
   b.dev.rcanon = b.rcanon;
   auto res = ctx->bubbles.emplace(key, b);
   *out = &res.first->second.dev;
@@ -620,8 +622,8 @@ int tprf_ctx_destroy(tprf_ctx *ctx) {
   for (auto &kv : ctx->dtail) (void)hipFree(kv.second);
   if (ctx->work) (void)hipFree(ctx->work);
   for (auto &kv : ctx->bubbles) {
-    (void)hipFree(kv.second.tw1); (void)hipFree(kv.second.twp); (void)hipFree(kv.second.phi);
-    (void)hipFree(kv.second.pinv_nu); (void)hipFree(kv.second.rcanon);
+    (void)hipFree(kv.second.tw1); (void)hipFree(kv.second.twp); (void)hipFree(kv.second.hmod);
+    (void)hipFree(kv.second.rcanon);
   }
   (void)hipFree(ctx->d_status);
   delete ctx;

@@ -78,8 +78,7 @@ struct BubbleDev {
   int ncanon;            // number of canonical r (r <= -r)
   const c2 *tw1;         // [N1][P] exp(+2 pi i j1 m(k) / L), m(k) = k or k - P
   const c2 *twp;         // [P] exp(+2 pi i p / P)
-  const c2 *phi;         // [3][2nb-1] sum_{j<L} psi_q(tau_j) exp(+2 pi i j m / L)
-  const c2 *pinv_nu;     // [3][2nb-1] 1 / (i nu_m - b_q)
+  const c2 *hmod;        // [3][2nb-1] 1 / (i nu_m - b_q) - (beta/L) sum_{j<L} psi_q(tau_j) exp(+2 pi i j m / L)
   const int *rcanon;     // [ncanon] canonical r in increasing order
 };
 
