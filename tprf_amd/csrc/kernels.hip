@@ -24,16 +24,22 @@ __global__ void __launch_bounds__(256) k_dyson(double beta, int n_iw, double mu,
   const double wn = kPi * (double)(2 * (iw - n_iw) + 1) / beta;
   c2 A[N][N], B[N][N];
   const c2 *e = ek + k * N * N;
+  // all n^2 entries of e_k (and of Sigma) are loaded before the first is used
+  v2d ev[N * N], sv[N * N];
+#pragma unroll
+  for (int t = 0; t < N * N; ++t) ev[t] = ld_v2d(e + t);
+  if (sigma_kind != 0) {
+    const c2 *sg = sigma + (sigma_kind == 1 ? iw : idx) * N * N;
+#pragma unroll
+    for (int t = 0; t < N * N; ++t) sv[t] = ld_v2d(sg + t);
+  }
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-      c2 v = e[i * N + j];
-      v.x = -v.x;
-      v.y = -v.y;
+      c2 v = c2{-ev[i * N + j].x, -ev[i * N + j].y};
       if (i == j) { v.x += mu; v.y += wn; }
-      if (sigma_kind == 1) v = csub(v, sigma[iw * N * N + i * N + j]);
-      else if (sigma_kind == 2) v = csub(v, sigma[idx * N * N + i * N + j]);
+      if (sigma_kind != 0) v = csub(v, c2{sv[i * N + j].x, sv[i * N + j].y});
       A[i][j] = v;
       B[i][j] = (i == j) ? c2{1.0, 0.0} : c2{0.0, 0.0};
     }
