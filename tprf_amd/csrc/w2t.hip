@@ -75,15 +75,30 @@ __global__ void __launch_bounds__(NT)
   const int nmesh = (STAT == 0) ? 2 * n_max : 2 * n_max - 1;
   const int first = (STAT == 0) ? -n_max : -(n_max - 1);
 
-  for (int t = tid; t < M; t += NT) s_tw[t] = wd.twM[t];
   // 1. this thread's eight inputs (the same FFT positions e = j + TPC r for every t),
-  //    staged once in registers and once in LDS for the tail fit
+  //    staged once in registers and once in LDS for the tail fit.  All loads
+  //    (inputs and the twiddle table) are issued before the first LDS store
+  //    (outside the mesh a valid entry is loaded and discarded).
+  constexpr int NTW = (M + NT - 1) / NT;
+  v2d twv[NTW], rv[8];
+#pragma unroll
+  for (int w = 0; w < NTW; ++w) twv[w] = ld_v2d(wd.twM + min(tid + w * NT, M - 1));
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int e = j + TPC * r;
+    const int n = e < M / 2 ? e : e - M;
+    const bool in_mesh = n >= first && n < first + nmesh;
+    rv[r] = ld_v2d(in + (in_mesh ? (long)(n - first) * ncol + col : col));
+  }
+#pragma unroll
+  for (int w = 0; w < NTW; ++w)
+    if (tid + w * NT < M) st_v2d(s_tw + tid + w * NT, twv[w]);
   c2 raw[8];
 #pragma unroll
   for (int r = 0; r < 8; ++r) {
     const int e = j + TPC * r;
     const int n = e < M / 2 ? e : e - M;
-    raw[r] = (n >= first && n < first + nmesh) ? in[(long)(n - first) * ncol + col] : c2{0.0, 0.0};
+    raw[r] = (n >= first && n < first + nmesh) ? c2{rv[r].x, rv[r].y} : c2{0.0, 0.0};
     buf[e * C + c] = raw[r];
   }
   __syncthreads();
