@@ -60,6 +60,7 @@ struct tprf_ctx {
   std::map<std::tuple<double, int, int, int, int, int>, BubbleEntry> bubbles;
   std::map<std::tuple<double, int>, double *> static_w;  // nu = 0 weights c_j
   std::map<std::tuple<int, double, int, int>, double *> dtail;  // out-of-mesh pole model in tau (real)
+  std::map<std::tuple<int, double, int, int>, c2 *> otab;       // per-tau output records of omega -> tau
   void *work = nullptr;                                   // grow-only scratch
   size_t work_bytes = 0;
   int bubble_variant = -1;                                // -1: autotuned per shape
@@ -284,10 +285,7 @@ void vandermonde_inverse8(double *vinv) {
 // a_q-bar = sum_j D_q(tau_j) gbar_j.  D is small (the pole tail beyond the mesh)
 // while both of its terms are O(1); it is formed here in long double so that
 // neither kernel takes the difference of two large sums.
-int get_dtail(tprf_ctx *ctx, int statistic, double beta, int n_max, int L, const double **out) {
-  auto key = std::make_tuple(statistic, beta, n_max, L);
-  auto it = ctx->dtail.find(key);
-  if (it != ctx->dtail.end()) { *out = it->second; return TPRF_OK; }
+static std::vector<double> dtail_values(int statistic, double beta, int n_max, int L) {
   typedef std::complex<long double> lc;
   const long double bf[3] = {0.0L, 1.0L, -1.0L}, bb[3] = {-0.5L, -1.0L, 1.0L};
   const long double *b = statistic == TPRF_FERMION ? bf : bb;
@@ -332,10 +330,46 @@ int get_dtail(tprf_ctx *ctx, int statistic, double beta, int n_max, int L, const
       d[(size_t)q * L + j] = (double)v.real();
     }
   }
+  return d;
+}
+
+int get_dtail(tprf_ctx *ctx, int statistic, double beta, int n_max, int L, const double **out) {
+  auto key = std::make_tuple(statistic, beta, n_max, L);
+  auto it = ctx->dtail.find(key);
+  if (it != ctx->dtail.end()) { *out = it->second; return TPRF_OK; }
+  const std::vector<double> d = dtail_values(statistic, beta, n_max, L);
   double *dd;
   int st = upload(ctx, &dd, d);
   if (st) return st;
   ctx->dtail.emplace(key, dd);
+  *out = dd;
+  return TPRF_OK;
+}
+
+// Per-tau output table of the pruned omega -> tau kernel, one 48-byte record
+// per tau_j: [phase_j, (D_0, D_1), (D_2, 0)] with phase_j = exp(-i pi j / L)
+// for fermions (1 for bosons) and D_q the table above; the kernel copies the
+// records of one output residue into LDS in one pass.
+int get_otab(tprf_ctx *ctx, int statistic, double beta, int n_max, int L, const c2 **out) {
+  auto key = std::make_tuple(statistic, beta, n_max, L);
+  auto it = ctx->otab.find(key);
+  if (it != ctx->otab.end()) { *out = it->second; return TPRF_OK; }
+  const std::vector<double> d = dtail_values(statistic, beta, n_max, L);
+  std::vector<c2> o(3 * (size_t)L);
+  for (int j = 0; j < L; ++j) {
+    c2 ph = c2{1.0, 0.0};
+    if (statistic == TPRF_FERMION) {
+      const long double a = kPiL * (long double)j / (long double)L;
+      ph = c2{(double)cosl(a), -(double)sinl(a)};
+    }
+    o[3 * (size_t)j] = ph;
+    o[3 * (size_t)j + 1] = c2{d[j], d[(size_t)L + j]};
+    o[3 * (size_t)j + 2] = c2{d[2 * (size_t)L + j], 0.0};
+  }
+  c2 *dd;
+  int st = upload(ctx, &dd, o);
+  if (st) return st;
+  ctx->otab.emplace(key, dd);
   *out = dd;
   return TPRF_OK;
 }
@@ -620,6 +654,7 @@ int tprf_ctx_destroy(tprf_ctx *ctx) {
   for (auto &kv : ctx->mats) { (void)hipFree(kv.second.psi); (void)hipFree(kv.second.phase); }
   for (auto &kv : ctx->static_w) (void)hipFree(kv.second);
   for (auto &kv : ctx->dtail) (void)hipFree(kv.second);
+  for (auto &kv : ctx->otab) (void)hipFree(kv.second);
   if (ctx->work) (void)hipFree(ctx->work);
   for (auto &kv : ctx->bubbles) {
     (void)hipFree(kv.second.tw1); (void)hipFree(kv.second.twp); (void)hipFree(kv.second.hmod);
@@ -731,7 +766,9 @@ int tprf_fourier_wr_to_tr(tprf_ctx *ctx, int statistic, double beta, int n_max, 
     if ((st = get_mats(ctx, statistic, beta, L, &mt))) return st;
     const double *dtail;
     if ((st = get_dtail(ctx, statistic, beta, n_max, L, &dtail))) return st;
-    tprf::W2tDev wd{M, L / M, pL->tw, pM->tw, dtail};
+    const c2 *otab;
+    if ((st = get_otab(ctx, statistic, beta, n_max, L, &otab))) return st;
+    tprf::W2tDev wd{M, L / M, pL->tw, pM->tw, dtail, otab};
     auto launch = [&](int v) {
       return tprf::launch_w2t_pruned(ctx->stream, statistic, beta, n_max, ncol, (const c2 *)in, (c2 *)out,
                                      (const c2 *)moments, fit, *mt, wd, tprf::Status{ctx->d_status}, v);
@@ -925,7 +962,7 @@ int tprf_fourier_wr_to_tr_adjoint(tprf_ctx *ctx, int statistic, double beta, int
   if ((st = get_mats(ctx, statistic, beta, L, &mt))) return st;
   const double *dtail;
   if ((st = get_dtail(ctx, statistic, beta, n_max, L, &dtail))) return st;
-  tprf::W2tDev wd{M, L / M, pL->tw, pM->tw, dtail};
+  tprf::W2tDev wd{M, L / M, pL->tw, pM->tw, dtail, nullptr};
   return check_launch(ctx,
                       tprf::launch_w2t_adjoint(ctx->stream, statistic, beta, n_max, ncol, (const c2 *)gbar_tr,
                                                (c2 *)gbar_wr, fitted ? 1 : 0, fit, *mt, wd),

@@ -90,6 +90,7 @@ struct W2tDev {
   const c2 *twM;         // [M] exp(-2 pi i k / M)
   const double *dtail;   // [3][L] psi_q(tau_j) - (1/beta) sum_{n in mesh} exp(-i w_n tau_j) / (i w_n - b_q), real:
                          // the part of the pole model beyond the mesh, added at the outputs
+  const c2 *otab;        // [L][3] per-tau output records [phase, (D_0, D_1), (D_2, 0)] (forward kernel)
 };
 
 // Status words written by kernels (device memory, 4 doubles):

@@ -10,6 +10,7 @@
 // threads, eight values per thread, radix-8 Stockham passes exchanged through
 // LDS.
 #include "common.hpp"
+#include "lds_dma.hpp"
 #include "fft_device.hpp"
 #include "mats_device.hpp"
 
@@ -56,6 +57,8 @@ __global__ void __launch_bounds__(NT)
                                                              MatsDev mt, W2tDev wd, double *status) {
   constexpr int TPC = M / 8;  // threads per column
   constexpr int C = NT / TPC;  // columns per workgroup
+  // C >= 3: the output records of a residue (3 M entries) fit in buf
+  constexpr bool OTAB = C >= 3;
   using Plan = W2tPlan<M>;
   __shared__ c2 buf[M * C];
   __shared__ c2 s_coef[C][kMaxOrder];
@@ -211,6 +214,16 @@ __global__ void __launch_bounds__(NT)
 #pragma unroll
     for (int p = 0; p < Plan::npass; ++p) {
       const int R = (p < Plan::n8) ? 8 : Plan::last;
+      if (OTAB && p + 1 == Plan::npass) {
+        // buf is free (every thread has read the last exchange): copy this
+        // residue's output records otab[Q jp + t], jp < M, straight into it
+        // while the last pass computes (64-record chunks per wave)
+        const int wave = tid >> 6, lane = tid & 63;
+        for (int ch = wave; ch < 3 * M / 64; ch += NT / 64) {
+          const int e = ch * 64 + lane, jp = e / 3, part = e - 3 * jp;
+          dma16(wd.otab + ((long)(Q * jp + t) * 3 + part), buf + ch * 64);
+        }
+      }
       if (R == 8) w2t_butterflies<M, 8, -1>(v, j, Ns, s_tw);
       else if (R == 4) w2t_butterflies<M, 4, -1>(v, j, Ns, s_tw);
       else w2t_butterflies<M, 2, -1>(v, j, Ns, s_tw);
@@ -234,6 +247,29 @@ __global__ void __launch_bounds__(NT)
           v[u] = buf[(j * Bn + uu + r * (M / Rn)) * C + c];
         }
         __syncthreads();
+      } else if (OTAB) {
+        // outputs: bin j' = o + r Ns  ->  tau index Q j' + t, with the records
+        // [phase, (D_0, D_1), (D_2, 0)] of tau_{Q j' + t} read from LDS
+        wait_vmcnt<0>();
+        __syncthreads();
+        if (c < cn) {
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int uu = u / R, r = u % R;
+            const int b = j * B + uu;
+            const int k = b % Ns;
+            const int jp = (b - k) * R + k + r * Ns;
+            const int jt = Q * jp + t;
+            const c2 ph = buf[3 * jp], d01 = buf[3 * jp + 1], d2 = buf[3 * jp + 2];
+            c2 y = v[u];
+            if (STAT == 0) y = cmul(y, ph);
+            y.x += a0.x * d01.x + a1.x * d01.y + a2.x * d2.x;
+            y.y += a0.y * d01.x + a1.y * d01.y + a2.y * d2.x;
+            out[(long)jt * ncol + col] = y;
+            if (jt == 0) out[(long)L * ncol + col] = cscale(cadd(y, s_m1[c]), pm);
+          }
+        }
+        __syncthreads();  // the next residue's first exchange overwrites buf
       } else if (c < cn) {
         // outputs: bin j' = o + r Ns  ->  tau index Q j' + t
 #pragma unroll
