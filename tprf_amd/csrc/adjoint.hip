@@ -93,8 +93,8 @@ __global__ void __launch_bounds__(256) k_bubble_static_adjoint(int ntau, int d0,
 #pragma unroll
     for (int i = 0; i < n2; ++i) {
       const c2 g = cconj(gm[i]);
-      s1 = cadd(s1, cmul(A[i], g));
-      s2 = cadd(s2, cmul(B[i], g));
+      s1 = cfma(s1, A[i], g);
+      s2 = cfma(s2, B[i], g);
     }
     gbar[((long)j * nr + r) * n2 + xy] = cadd(cscale(s1, cw[j]), cscale(s2, cw[L - j]));
   }
@@ -141,7 +141,7 @@ __global__ void __launch_bounds__(256) k_dyson_adjoint(int nw, long nk, long nba
         for (int b = 0; b < N; ++b) {
           c2 s = c2{0.0, 0.0};
 #pragma unroll
-          for (int l = 0; l < N; ++l) s = cadd(s, cmul(cconj(Gv[l][a]), Bv[l][b]));
+          for (int l = 0; l < N; ++l) s = cfma(s, cconj(Gv[l][a]), Bv[l][b]);
           T[a][b] = s;
         }
 #pragma unroll
@@ -150,7 +150,7 @@ __global__ void __launch_bounds__(256) k_dyson_adjoint(int nw, long nk, long nba
         for (int b = 0; b < N; ++b) {
           c2 s = c2{0.0, 0.0};
 #pragma unroll
-          for (int l = 0; l < N; ++l) s = cadd(s, cmul(T[a][l], cconj(Gv[b][l])));
+          for (int l = 0; l < N; ++l) s = cfma(s, T[a][l], cconj(Gv[b][l]));
           e[a][b] = cadd(e[a][b], s);
           if (a == b) mu -= s.x;
         }
@@ -206,7 +206,7 @@ __global__ void __launch_bounds__(256) k_bubble_static(int ntau, int d0, int d1,
     const c2 x = cscale(gtr[((long)j * nr + r) * n2 + da], cw[j]);
     const c2 *gm = gtr + ((long)(L - j) * nr + mr) * n2;
 #pragma unroll
-    for (int bc = 0; bc < n2; ++bc) acc[bc] = cadd(acc[bc], cmul(x, gm[bc]));
+    for (int bc = 0; bc < n2; ++bc) acc[bc] = cfma(acc[bc], x, gm[bc]);
   }
   c2 *o = part + ((long)blockIdx.y * nr + r) * n4;
 #pragma unroll
@@ -258,7 +258,7 @@ __global__ void __launch_bounds__(64) k_rpa_point(long npts, const c2 *__restric
     for (int jj = 0; jj < M; ++jj) {
       c2 s = (i == jj) ? c2{1.0, 0.0} : c2{0.0, 0.0};
 #pragma unroll
-      for (int l = 0; l < M; ++l) s = csub(s, cmul(X[i][l], MU[l][jj]));
+      for (int l = 0; l < M; ++l) s = cfms(s, X[i][l], MU[l][jj]);
       A[i][jj] = s;
       Ai[i][jj] = (i == jj) ? c2{1.0, 0.0} : c2{0.0, 0.0};
     }
@@ -289,7 +289,7 @@ __global__ void __launch_bounds__(64) k_rpa_point(long npts, const c2 *__restric
       if (i == k) continue;
       const c2 f = A[i][k];
 #pragma unroll
-      for (int jj = 0; jj < M; ++jj) { A[i][jj] = csub(A[i][jj], cmul(f, A[k][jj])); Ai[i][jj] = csub(Ai[i][jj], cmul(f, Ai[k][jj])); }
+      for (int jj = 0; jj < M; ++jj) { A[i][jj] = cfms(A[i][jj], f, A[k][jj]); Ai[i][jj] = cfms(Ai[i][jj], f, Ai[k][jj]); }
     }
   }
   if (singular && status) status[2] = 1.0;
@@ -300,7 +300,7 @@ __global__ void __launch_bounds__(64) k_rpa_point(long npts, const c2 *__restric
     for (int jj = 0; jj < M; ++jj) {
       c2 s = c2{0.0, 0.0};
 #pragma unroll
-      for (int l = 0; l < M; ++l) s = cadd(s, cmul(Ai[i][l], X[l][jj]));
+      for (int l = 0; l < M; ++l) s = cfma(s, Ai[i][l], X[l][jj]);
       Y[i][jj] = s;
       chi[p * n4 + tidx(i, jj)] = s;
     }
@@ -317,7 +317,7 @@ __global__ void __launch_bounds__(64) k_rpa_point(long npts, const c2 *__restric
     for (int jj = 0; jj < M; ++jj) {
       c2 s = c2{0.0, 0.0};
 #pragma unroll
-      for (int l = 0; l < M; ++l) s = cadd(s, cmul(cconj(Ai[l][i]), Cb[l][jj]));
+      for (int l = 0; l < M; ++l) s = cfma(s, cconj(Ai[l][i]), Cb[l][jj]);
       Z[i][jj] = s;
     }
   // W = 1 + U chi
@@ -327,7 +327,7 @@ __global__ void __launch_bounds__(64) k_rpa_point(long npts, const c2 *__restric
     for (int jj = 0; jj < M; ++jj) {
       c2 s = (i == jj) ? c2{1.0, 0.0} : c2{0.0, 0.0};
 #pragma unroll
-      for (int l = 0; l < M; ++l) s = cadd(s, cmul(MU[i][l], Y[l][jj]));
+      for (int l = 0; l < M; ++l) s = cfma(s, MU[i][l], Y[l][jj]);
       W[i][jj] = s;
     }
   // Xbar = Z W^H ; Ubar = X^H Z chi^H
@@ -337,7 +337,7 @@ __global__ void __launch_bounds__(64) k_rpa_point(long npts, const c2 *__restric
     for (int jj = 0; jj < M; ++jj) {
       c2 s = c2{0.0, 0.0};
 #pragma unroll
-      for (int l = 0; l < M; ++l) s = cadd(s, cmul(Z[i][l], cconj(W[jj][l])));
+      for (int l = 0; l < M; ++l) s = cfma(s, Z[i][l], cconj(W[jj][l]));
       xbar_t[p * n4 + tidx(i, jj)] = s;
     }
   c2 T[M][M];
@@ -347,7 +347,7 @@ __global__ void __launch_bounds__(64) k_rpa_point(long npts, const c2 *__restric
     for (int jj = 0; jj < M; ++jj) {
       c2 s = c2{0.0, 0.0};
 #pragma unroll
-      for (int l = 0; l < M; ++l) s = cadd(s, cmul(cconj(X[l][i]), Z[l][jj]));
+      for (int l = 0; l < M; ++l) s = cfma(s, cconj(X[l][i]), Z[l][jj]);
       T[i][jj] = s;
     }
 #pragma unroll
@@ -356,7 +356,7 @@ __global__ void __launch_bounds__(64) k_rpa_point(long npts, const c2 *__restric
     for (int jj = 0; jj < M; ++jj) {
       c2 s = c2{0.0, 0.0};
 #pragma unroll
-      for (int l = 0; l < M; ++l) s = cadd(s, cmul(T[i][l], cconj(Y[jj][l])));
+      for (int l = 0; l < M; ++l) s = cfma(s, T[i][l], cconj(Y[jj][l]));
       ubar_t[p * n4 + tidx(i, jj)] = s;
     }
 }

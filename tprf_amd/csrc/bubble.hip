@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
         for (int s = 0; s < R2; ++s) v[s] = sT[(c * R1 + k1) * TLD + s];
         SmallDft<R2, 1>::run(v, nullptr, 0);
 #pragma unroll
-        for (int k2 = 0; k2 < R2; ++k2) acc[i][k2] = cadd(acc[i][k2], cmul(w[k1 + R1 * k2], v[k2]));
+        for (int k2 = 0; k2 < R2; ++k2) acc[i][k2] = cfma(acc[i][k2], w[k1 + R1 * k2], v[k2]);
       }
     }
   };
@@ -312,11 +312,11 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
         const int im = m + nb - 1;
         // y(m) = fact S(m) + sum_q a_q H_q(m), H_q = 1 / (i nu_m - b_q) - fact Phi_q(m)
         const c2 h0 = hmod(im), h1 = hmod(nnu + im), h2 = hmod(2 * nnu + im);
-        c2 v = cadd(cscale(acc[i][k2], fact), cadd(cadd(cmul(a0[0], h0), cmul(a0[1], h1)), cmul(a0[2], h2)));
+        c2 v = cadd(cscale(acc[i][k2], fact), cfma(cfma(cmul(a0[0], h0), a0[1], h1), a0[2], h2));
         out[((long)im * nr + r) * n4 + q] = v;
         if (mirror) {
           const c2 sm = cadd(sS[c * P + ((P - k) & (P - 1))], xl_minus_x0);
-          v = cadd(cscale(sm, fact), cadd(cadd(cmul(a1[0], h0), cmul(a1[1], h1)), cmul(a1[2], h2)));
+          v = cadd(cscale(sm, fact), cfma(cfma(cmul(a1[0], h0), a1[1], h1), a1[2], h2));
           out[((long)im * nr + mr) * n4 + qm] = v;
         }
       }

@@ -25,6 +25,15 @@ __host__ __device__ __forceinline__ c2 cdiv(c2 a, c2 b) {
 }
 // a * (i s): multiply by a purely imaginary number
 __host__ __device__ __forceinline__ c2 cmul_i(c2 a, double s) { return c2{-a.y * s, a.x * s}; }
+// acc + a b and acc - a b as two chained fused multiply-adds per component
+// (four FP64 instructions instead of a multiply, a fused multiply-add and an
+// add per component)
+__host__ __device__ __forceinline__ c2 cfma(c2 acc, c2 a, c2 b) {
+  return c2{fma(a.x, b.x, fma(-a.y, b.y, acc.x)), fma(a.x, b.y, fma(a.y, b.x, acc.y))};
+}
+__host__ __device__ __forceinline__ c2 cfms(c2 acc, c2 a, c2 b) {
+  return c2{fma(-a.x, b.x, fma(a.y, b.y, acc.x)), fma(-a.x, b.y, fma(-a.y, b.x, acc.y))};
+}
 
 // Native two-double vector for values held in registers across a loop
 // iteration (prefetches): copies of double2 are struct copies that can be left

@@ -76,8 +76,8 @@ __global__ void __launch_bounds__(256) k_dyson(double beta, int n_iw, double mu,
       const c2 f = A[i][kk];
 #pragma unroll
       for (int j = 0; j < N; ++j) {
-        A[i][j] = csub(A[i][j], cmul(f, A[kk][j]));
-        B[i][j] = csub(B[i][j], cmul(f, B[kk][j]));
+        A[i][j] = cfms(A[i][j], f, A[kk][j]);
+        B[i][j] = cfms(B[i][j], f, B[kk][j]);
       }
     }
   }
@@ -197,7 +197,7 @@ __global__ void __launch_bounds__(256) k_w2t(double beta, int n_max, long ncol, 
     for (int t = threadIdx.x; t < cn * fit.ncoef; t += blockDim.x) {
       const int c = t / fit.ncoef, q = t - c * fit.ncoef;
       c2 s = c2{0.0, 0.0};
-      for (int f = 0; f < fit.n_fit; ++f) s = cadd(s, cmul(fit.pinv[q * fit.n_fit + f], in[(long)fit.idx[f] * ncol + c0 + c]));
+      for (int f = 0; f < fit.n_fit; ++f) s = cfma(s, fit.pinv[q * fit.n_fit + f], in[(long)fit.idx[f] * ncol + c0 + c]);
       s_coef[c][q] = s;
     }
     __syncthreads();
@@ -206,7 +206,7 @@ __global__ void __launch_bounds__(256) k_w2t(double beta, int n_max, long ncol, 
       double r2 = 0.0;
       for (int f = 0; f < fit.n_fit; ++f) {
         c2 v = in[(long)fit.idx[f] * ncol + c0 + c];
-        for (int q = 0; q < fit.ncoef; ++q) v = csub(v, cmul(fit.vander[f * fit.ncoef + q], s_coef[c][q]));
+        for (int q = 0; q < fit.ncoef; ++q) v = cfms(v, fit.vander[f * fit.ncoef + q], s_coef[c][q]);
         r2 += cabs2(v);
       }
       s_res[c] = sqrt(r2 / fit.n_fit);
@@ -541,7 +541,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N == 3
 #pragma unroll
     for (int l = 0; l < N2; ++l) {
       const int bit = l * N2 + j;
-      if ((umask[bit >> 6] >> (bit & 63)) & 1ull) s = csub(s, cmul(X[l], sU[l * N2 + j]));
+      if ((umask[bit >> 6] >> (bit & 63)) & 1ull) s = cfms(s, X[l], sU[l * N2 + j]);
     }
     A[j] = s;
   }
@@ -575,10 +575,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N == 3
     if (!is_piv && gidx < P) {
       const c2 f = cdiv(A[k], slot[slot_id][k]);
 #pragma unroll
-      for (int j = k + 1; j < N2; ++j) A[j] = csub(A[j], cmul(f, slot[slot_id][j]));
+      for (int j = k + 1; j < N2; ++j) A[j] = cfms(A[j], f, slot[slot_id][j]);
       A[k] = c2{0.0, 0.0};
 #pragma unroll
-      for (int j = 0; j < N2; ++j) X[j] = csub(X[j], cmul(f, slot[slot_id][N2 + j]));
+      for (int j = 0; j < N2; ++j) X[j] = cfms(X[j], f, slot[slot_id][N2 + j]);
     }
     wave_lds_sync();
   }

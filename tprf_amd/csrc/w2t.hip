@@ -139,7 +139,7 @@ __global__ void __launch_bounds__(NT)
         }
 #pragma unroll
         for (int ff = 0; ff < kB; ++ff)
-          if (f0 + f8 + ff < f1) s = cadd(s, cmul(pv[ff], xv[ff]));
+          if (f0 + f8 + ff < f1) s = cfma(s, pv[ff], xv[ff]);
       }
       s_cpart[cc][q][part] = s;
     }
@@ -162,7 +162,7 @@ __global__ void __launch_bounds__(NT)
         for (int q = 0; q < 5; ++q) vd[q] = fit.vander[f * nc + min(q5 + q, nc - 1)];
 #pragma unroll
         for (int q = 0; q < 5; ++q)
-          if (q5 + q < nc) v = csub(v, cmul(vd[q], s_coef[cc][q5 + q]));
+          if (q5 + q < nc) v = cfms(v, vd[q], s_coef[cc][q5 + q]);
       }
       s_res[cc][f] = cabs2(v);
     }
