@@ -64,6 +64,7 @@ __global__ void __launch_bounds__(NT)
   __shared__ c2 s_coef[C][kMaxOrder];
   __shared__ c2 s_cpart[C][kMaxOrder][kFitSplit];
   __shared__ double s_res[C][kMaxFit];
+  __shared__ double s_rpart[C][8];
   __shared__ c2 s_a[C][3];
   __shared__ c2 s_m1[C];
   __shared__ c2 s_tw[M];
@@ -167,9 +168,21 @@ __global__ void __launch_bounds__(NT)
       s_res[cc][f] = cabs2(v);
     }
     __syncthreads();
+    // residual sums in two levels (8 partial sums per column, then their
+    // total) rather than one chain of nf dependent LDS reads
+    for (int t = tid; t < cn * 8; t += NT) {
+      const int cc = t >> 3, part = t & 7;
+      double r = 0.0;
+#pragma unroll
+      for (int w = 0; w < kMaxFit / 8; ++w)
+        if (part + 8 * w < nf) r += s_res[cc][part + 8 * w];
+      s_rpart[cc][part] = r;
+    }
+    __syncthreads();
     if (tid < cn) {
       double r2 = 0.0;
-      for (int f = 0; f < nf; ++f) r2 += s_res[tid][f];
+#pragma unroll
+      for (int part = 0; part < 8; ++part) r2 += s_rpart[tid][part];
       const c2 m1 = cscale(s_coef[tid][1], fit.wpow[1]);
       Poles<STAT>::coef(m1, cscale(s_coef[tid][2], fit.wpow[2]), cscale(s_coef[tid][3], fit.wpow[3]), s_a[tid]);
       s_m1[tid] = m1;
@@ -356,6 +369,7 @@ __global__ void __launch_bounds__(kW2tThreads, 2) k_w2t_adjoint(double beta, int
   __shared__ c2 buf[M * C];
   __shared__ c2 s_tw[M];
   __shared__ c2 s_part[NT][3];
+  __shared__ c2 s_part2[C][8][3];
   __shared__ c2 s_mbar[C][3];
 
   const int tid = threadIdx.x;
@@ -458,13 +472,30 @@ __global__ void __launch_bounds__(kW2tThreads, 2) k_w2t_adjoint(double beta, int
   s_part[tid][1] = ab1;
   s_part[tid][2] = ab2;
   __syncthreads();
+  // the TPC partial sums of each column in two levels (8 groups, then their
+  // total) rather than one chain of TPC dependent LDS reads
+  for (int t = tid; t < cn * 8; t += NT) {
+    const int cc = t >> 3, part = t & 7;
+    c2 b0 = c2{0, 0}, b1 = c2{0, 0}, b2 = c2{0, 0};
+#pragma unroll
+    for (int w = 0; w < TPC / 8; ++w) {
+      const int src = (part + 8 * w) * C + cc;
+      b0 = cadd(b0, s_part[src][0]);
+      b1 = cadd(b1, s_part[src][1]);
+      b2 = cadd(b2, s_part[src][2]);
+    }
+    s_part2[cc][part][0] = b0;
+    s_part2[cc][part][1] = b1;
+    s_part2[cc][part][2] = b2;
+  }
+  __syncthreads();
   if (tid < cn) {
     c2 a0 = c2{0, 0}, a1 = c2{0, 0}, a2 = c2{0, 0};
-    for (int jj = 0; jj < TPC; ++jj) {
-      const int src = jj * C + tid;
-      a0 = cadd(a0, s_part[src][0]);
-      a1 = cadd(a1, s_part[src][1]);
-      a2 = cadd(a2, s_part[src][2]);
+#pragma unroll
+    for (int part = 0; part < 8; ++part) {
+      a0 = cadd(a0, s_part2[tid][part][0]);
+      a1 = cadd(a1, s_part2[tid][part][1]);
+      a2 = cadd(a2, s_part2[tid][part][2]);
     }
     c2 m1, m2, m3;
     if (STAT == 0) {  // a = (m1 - m3, (m2 + m3)/2, (m3 - m2)/2)
