@@ -79,13 +79,32 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
   const int N1 = L / P;
   const double dtau = beta / (double)L;
 
-  // products at tau_0..tau_8 and tau_{L-8}..tau_L: sE[c][e], e < 9 -> j = e, e >= 9 -> j = L - 17 + e
-  for (int t = tid; t < cn * 18; t += NT) {
-    const int c = t / 18, e = t - c * 18;
-    const int j = e < 9 ? e : L - 17 + e;
-    const int q = q0 + c;
-    const int dd = q % N, cc = (q / N) % N, bb = (q / n2) % N, aa = q / (n2 * N);
-    sE[c][e] = cmul(gtr[((long)j * nr + r) * n2 + dd * N + aa], gtr[((long)(L - j) * nr + mr) * n2 + bb * N + cc]);
+  // products at tau_0..tau_8 and tau_{L-8}..tau_L: sE[c][e], e < 9 -> j = e, e >= 9 -> j = L - 17 + e,
+  // and the inner twiddles sTwp; every load is issued before the first LDS store
+  // (clamped indices: entries past cn * 18 load a valid element and are dropped)
+  {
+    constexpr int NEB = (CG * 18 + NT - 1) / NT, NTP = (P + NT - 1) / NT;
+    v2d ga[NEB], gb[NEB], tp[NTP];
+#pragma unroll
+    for (int w = 0; w < NEB; ++w) {
+      const int t = min(tid + w * NT, cn * 18 - 1);
+      const int c = t / 18, e = t - c * 18;
+      const int j = e < 9 ? e : L - 17 + e;
+      const int q = q0 + c;
+      const int dd = q % N, cc = (q / N) % N, bb = (q / n2) % N, aa = q / (n2 * N);
+      ga[w] = ld_v2d(gtr + ((long)j * nr + r) * n2 + dd * N + aa);
+      gb[w] = ld_v2d(gtr + ((long)(L - j) * nr + mr) * n2 + bb * N + cc);
+    }
+#pragma unroll
+    for (int w = 0; w < NTP; ++w) tp[w] = ld_v2d(bd.twp + min(tid + w * NT, P - 1));
+#pragma unroll
+    for (int w = 0; w < NEB; ++w) {
+      const int t = tid + w * NT;
+      if (t < cn * 18) sE[t / 18][t % 18] = cmul(c2{ga[w].x, ga[w].y}, c2{gb[w].x, gb[w].y});
+    }
+#pragma unroll
+    for (int w = 0; w < NTP; ++w)
+      if (tid + w * NT < P) st_v2d(sTwp + tid + w * NT, tp[w]);
   }
   __syncthreads();
   if (tid < cn) {
@@ -96,7 +115,6 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
                       (c2 *)nullptr);
   }
 
-  for (int t = tid; t < P; t += NT) sTwp[t] = bd.twp[t];
   // Element offset of staged entry t of side A (G_da(tau, r)) or side B
   // (G_bc(L - tau, -r)) at tau = j2 N1 (+ j1 for step j1); AB: e = d with
   // a = a0, and e = c with b = b0.
