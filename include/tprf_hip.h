@@ -89,6 +89,16 @@ int tprf_ctx_tail_status_reset(tprf_ctx *ctx);
 int tprf_lattice_dyson_g_wk(tprf_ctx *ctx, double beta, int n_iw, double mu, long nk, int norb,
                             const tprf_c128 *e_k, int sigma_kind, const tprf_c128 *sigma, tprf_c128 *g_wk);
 
+/* a1 followed by a2 in one call, Sigma = 0: lattice_dyson_g0_wk (gf.cpp:41-58)
+ * then fourier_wk_to_wr (gf.cpp:209, fourier_lattice.cpp:39,49), as the
+ * reference's driver runs them one after the other; no single counterpart.
+ * e_k[nk][n][n] -> g_wr[2 n_iw][nk][n][n] (nk = dims[0] dims[1] dims[2]).
+ * For a 16 x 16 x 16 lattice and n <= 3 the inversion is done inside the
+ * transform's load phase and G(iw, k) is never stored; otherwise g_wr first
+ * holds G(iw, k) and is transformed in place.  n <= 6.                      */
+int tprf_lattice_dyson_g_wr(tprf_ctx *ctx, double beta, int n_iw, double mu, const int dims[3], int norb,
+                            const tprf_c128 *e_k, tprf_c128 *g_wr);
+
 /* ---- a2 / a6: lattice FFT ---------------------------------------------- */
 /* fourier_wk_to_wr, gf.hpp:276 / gf.cpp:209 -> fourier.hpp:165-197:
  *   out[w][r][t] = (1/N_k) sum_k exp(-2 pi i k.r / dims) in[w][k][t].

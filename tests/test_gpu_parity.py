@@ -68,6 +68,40 @@ def test_dyson(L, oracle, n, sigma):
     assert relerr(host(got.data), ref) <= 1e-13
 
 
+# ---------------------------------------------------------------- a1 + a2 in one call
+@pytest.mark.parametrize("dims,n,beta,n_iw", [((16, 16, 16), 1, 10.0, 16), ((16, 16, 16), 2, 20.0, 32),
+                                              ((16, 16, 16), 3, 40.0, 64), ((16, 16, 16), 3, 40.0, 256),
+                                              ((4, 3, 2), 2, 7.5, 16), ((16, 16, 16), 4, 5.0, 8),
+                                              ((8, 8, 1), 3, 10.0, 16)])
+def test_dyson_g_wr(L, oracle, dims, n, beta, n_iw):
+    """tprf_lattice_dyson_g_wr (16^3, n <= 3: inversion by cofactors inside the
+    transform; other shapes: Dyson then the transform in place) against the
+    oracle's lattice_dyson_g_wk followed by fourier_wk_to_wr."""
+    import torch
+    from tprf_amd import _lib
+    rng = np.random.default_rng(n * 100 + n_iw)
+    nk = dims[0] * dims[1] * dims[2]
+    e = hermitian_ek(rng, nk, n, scale=0.8)
+    mu = 0.23
+    e_d = gpu(e)
+    ctx = _lib.ctx(torch.cuda.current_device())
+    ref = oracle.fourier_lattice(oracle.lattice_dyson_g_wk(beta, n_iw, mu, e), dims, -1)
+    # the fused 16^3 kernel has two workgroup shapes (option 4 selects one, -1
+    # restores tuning)
+    variants = (0, 1, -1) if tuple(dims) == (16, 16, 16) and n <= 3 else (-1,)
+    for variant in variants:
+        _lib.call("tprf_ctx_set_option", ctx, 4, variant)
+        try:
+            out = torch.empty((2 * n_iw, nk, n, n), dtype=torch.complex128, device="cuda")
+            _lib.tail_status_reset(ctx)
+            _lib.call("tprf_lattice_dyson_g_wr", ctx, beta, n_iw, mu, _lib.dims3(dims), n, _lib.ptr(e_d),
+                      _lib.ptr(out))
+            _lib.tail_status(ctx)  # raises on a singular inversion
+            assert relerr(host(out), ref) <= 1e-12, variant
+        finally:
+            _lib.call("tprf_ctx_set_option", ctx, 4, -1)
+
+
 # ---------------------------------------------------------------- a2/a6 lattice FFT
 @pytest.mark.parametrize("dims,ntarget", [((4, 3, 2), 4), ((16, 16, 16), 9), ((32, 32, 1), 1), ((5, 7, 1), 3),
                                           ((12, 12, 12), 1), ((48, 48, 1), 4), ((64, 64, 1), 1), ((11, 13, 1), 2),
@@ -340,10 +374,10 @@ def oracle_chain(O, wl):
     return dict(g_wk=g_wk, g_wr=g_wr, g_tr=g_tr, chi0_wr=chi0_wr, chi0_wk=chi0_wk, chi_wk=O.solve_rpa_PH(chi0_wk, wl.U))
 
 
-def run_chain(wl):
+def run_chain(wl, fused=False):
     import torch
     from tprf_amd import chain
-    ch = chain.BubbleRPA(wl)
+    ch = chain.BubbleRPA(wl, fused=fused)
     ch.run()
     err, m0 = ch.check_status()
     torch.cuda.synchronize()
@@ -395,6 +429,23 @@ def test_chain_cubic_small(L, oracle):
     n4 = wl.norb ** 4
     got = host(ch.chi_wk).reshape(-1, n4)[good]
     assert relerr(got, ref["chi_wk"].reshape(-1, n4)[good]) <= TOL
+
+
+def test_chain_fused_equals_per_function(L):
+    """The chain with Dyson fused into the k -> r transform (G(iw, k) never
+    stored) against the chain of one call per reference function, full 3-orbital
+    16^3 k, 256 freqs: every later stage sees the same G(iw, r) to rounding."""
+    from tprf_amd import workloads
+    wl = workloads.cubic_3orb(seed=0)
+    ref, _ = run_chain(wl, fused=False)
+    got, err = run_chain(wl, fused=True)
+    assert err < 1e-6
+    assert got.g_wk is None
+    assert relerr(host(got.g_wr), host(ref.g_wr)) <= 1e-12
+    # the tail fit of omega -> tau weighs the high frequencies by up to
+    # w_max^3, so rounding-level differences of G(iw, r) grow by ~10 there
+    for name in ("g_tr", "chi0_wr", "chi0_wk"):
+        assert relerr(host(getattr(got, name)), host(getattr(ref, name))) <= TOL, name
 
 
 def test_chain_cubic_full_size_stagewise(L, oracle):

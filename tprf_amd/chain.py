@@ -15,18 +15,24 @@ from . import _lib
 
 
 class BubbleRPA:
-    STAGES = ("dyson", "wk_to_wr", "wr_to_tr", "chi0_wr", "wr_to_wk", "rpa")
+    """fused=True (default): the first two stages run as one call,
+    tprf_lattice_dyson_g_wr (G(iw, k) is then not stored for a 16^3 lattice);
+    fused=False: one call per reference function."""
+    API_STAGES = ("dyson", "wk_to_wr", "wr_to_tr", "chi0_wr", "wr_to_wk", "rpa")
+    FUSED_STAGES = ("dyson_wr", "wr_to_tr", "chi0_wr", "wr_to_wk", "rpa")
 
-    def __init__(self, wl, device="cuda", save_intermediates=True):
+    def __init__(self, wl, device="cuda", save_intermediates=True, fused=True):
         self.wl = wl
+        self.STAGES = self.FUSED_STAGES if fused else self.API_STAGES
         dev = torch.device(device)
         self.dev = dev
         n, nk = wl.norb, wl.nk
         c128 = torch.complex128
         self.e_k = torch.from_numpy(wl.e_k).to(dev).contiguous()
         self.U = torch.from_numpy(wl.U).to(dev).contiguous()
-        self.g_wk = torch.empty((2 * wl.n_iw, nk, n, n), dtype=c128, device=dev)
-        self.g_wr = torch.empty_like(self.g_wk)
+        # G(iw, k) is stored only by the one-call-per-function chain
+        self.g_wk = None if fused else torch.empty((2 * wl.n_iw, nk, n, n), dtype=c128, device=dev)
+        self.g_wr = torch.empty((2 * wl.n_iw, nk, n, n), dtype=c128, device=dev)
         self.g_tr = torch.empty((wl.ntau, nk, n, n), dtype=c128, device=dev)
         self.chi0_wr = torch.empty((wl.n_nu, nk, n, n, n, n), dtype=c128, device=dev)
         self.chi0_wk = torch.empty_like(self.chi0_wr)
@@ -40,6 +46,8 @@ class BubbleRPA:
         if name == "dyson":
             _lib.call("tprf_lattice_dyson_g_wk", c, wl.beta, wl.n_iw, wl.mu, nk, n, p(self.e_k), _lib.SIGMA_NONE, None,
                       p(self.g_wk))
+        elif name == "dyson_wr":
+            _lib.call("tprf_lattice_dyson_g_wr", c, wl.beta, wl.n_iw, wl.mu, self.dims, n, p(self.e_k), p(self.g_wr))
         elif name == "wk_to_wr":
             _lib.call("tprf_fourier_wk_to_wr", c, 2 * wl.n_iw, self.dims, n * n, p(self.g_wk), p(self.g_wr))
         elif name == "wr_to_tr":

@@ -69,6 +69,7 @@ struct tprf_ctx {
   std::map<std::tuple<int, int, int, int, long>, int> w2t_tuned;
   int lattice_variant = -1;
   std::map<std::tuple<long, long, int, int, int, int>, int> lattice_tuned;
+  std::map<std::tuple<int, int>, int> dyson_lattice_tuned;
 };
 
 namespace {
@@ -725,6 +726,32 @@ int tprf_lattice_dyson_g_wk(tprf_ctx *ctx, double beta, int n_iw, double mu, lon
                       tprf::launch_dyson(ctx->stream, beta, n_iw, mu, nk, norb, (const c2 *)e_k, sigma_kind,
                                          (const c2 *)sigma, (c2 *)g_wk, tprf::Status{ctx->d_status}),
                       "dyson kernel");
+}
+
+int tprf_lattice_dyson_g_wr(tprf_ctx *ctx, double beta, int n_iw, double mu, const int dims[3], int norb,
+                            const tprf_c128 *e_k, tprf_c128 *g_wr) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (!(beta > 0) || n_iw < 1 || !dims_ok(dims) || !e_k || !g_wr)
+    return fail(ctx, TPRF_ERR_ARG, "lattice_dyson_g_wr: bad argument");
+  if (norb < 1 || norb > 6) return fail(ctx, TPRF_ERR_UNSUPPORTED, "lattice_dyson_g_wr: norb must be 1..6");
+  const long nk = (long)dims[0] * dims[1] * dims[2];
+  if (!ctx->generic_kernels && dims[0] == 16 && dims[1] == 16 && dims[2] == 16 && norb <= 3) {
+    auto launch = [&](int v) {
+      return tprf::launch_dyson_lattice16r(ctx->stream, beta, n_iw, mu, norb, (const c2 *)e_k, (c2 *)g_wr,
+                                           tprf::Status{ctx->d_status}, v);
+    };
+    const int nvar = tprf::dyson_lattice16r_variants(norb);
+    const int forced = ctx->lattice_variant >= 0 ? ctx->lattice_variant % nvar : -1;
+    return tuned_launch(ctx, ctx->dyson_lattice_tuned, std::make_tuple(n_iw, norb), nvar, forced, launch,
+                        "fused dyson lattice kernel");
+  }
+  // other shapes: Dyson into g_wr, then the lattice transform in place
+  int st = check_launch(ctx,
+                        tprf::launch_dyson(ctx->stream, beta, n_iw, mu, nk, norb, (const c2 *)e_k, 0, nullptr,
+                                           (c2 *)g_wr, tprf::Status{ctx->d_status}),
+                        "dyson kernel");
+  if (st) return st;
+  return lattice_fft(ctx, 2L * n_iw, dims, (long)norb * norb, (const c2 *)g_wr, (c2 *)g_wr, -1, true);
 }
 
 int tprf_fourier_wk_to_wr(tprf_ctx *ctx, long nw, const int dims[3], long ntarget, const tprf_c128 *in,

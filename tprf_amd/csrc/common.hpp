@@ -113,6 +113,12 @@ struct Status {
 // k-points each (k = inst * nk_inst + k_local); mu is then ignored
 hipError_t launch_dyson(hipStream_t s, double beta, int n_iw, double mu, long nk, int n, const c2 *ek, int sigma_kind,
                         const c2 *sigma, c2 *g, Status st, const double *mu_arr = nullptr, long nk_inst = 1);
+// Dyson (Sigma = 0) fused into the 16^3 k -> r transform, n <= 3:
+// e_k[4096][n][n] -> g_wr[2 n_iw][4096][n][n]
+// variant: 0 = one orbital row per workgroup, 1 = one column (same results)
+hipError_t launch_dyson_lattice16r(hipStream_t s, double beta, int n_iw, double mu, int n, const c2 *ek, c2 *out,
+                                   Status st, int variant);
+int dyson_lattice16r_variants(int n);
 hipError_t launch_lattice_axis(hipStream_t s, int sign, const c2 *in, c2 *out, long outer, int len, long inner,
                                const FftDev &p, double scale);
 hipError_t launch_w2t(hipStream_t s, int statistic, double beta, int n_max, long ncol, const c2 *in, c2 *out,

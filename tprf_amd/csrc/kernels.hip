@@ -772,22 +772,12 @@ __global__ void __launch_bounds__(256) k_lattice3d(const c2 *__restrict__ in, c2
 //   axis 0: line p = (hi, lo) = (i1, i2) holds i0 = 0..15 (coalesced loads)
 //   axis 1: line p = (k0, i2) holds i1; rounds split i2 = lo in halves
 //   axis 2: line p = (k0, k1) holds i2; rounds split k0 = hi in halves
+// The three axis transforms and two transposes of the register-resident 16^3
+// FFT: thread t = C p + cc (p = 16 hi + lo) enters with the 16-point line
+// i0 = 0..15 at (i1, i2) = (hi, lo) of column cc and leaves with the line
+// k2 = 0..15 at (k0, k1) = (hi, lo).
 template <int C, int SIGN>
-__global__ void __launch_bounds__(256 * C) k_lattice16r(const c2 *__restrict__ in, c2 *__restrict__ out,
-                                                        long ntarget, double scale) {
-  constexpr int NSM = 2048 + 2048 / 16;  // half grid, one pad slot per 16 entries (the larger of the two paddings)
-  __shared__ c2 sm[NSM * C];
-  const long ntc = (ntarget + C - 1) / C;
-  const long tile = xcd_remap(blockIdx.x, gridDim.x);
-  const long w = tile / ntc;
-  const long c0 = (tile - w * ntc) * C;
-  const int t = threadIdx.x, cc = t % C, p = t / C, hi = p >> 4, lo = p & 15;
-  const bool live = c0 + cc < ntarget;  // columns past the end are neither read nor written
-  const c2 *src = in + w * 4096 * ntarget + c0 + cc;
-  c2 *dst = out + w * 4096 * ntarget + c0 + cc;
-  c2 v[16];
-#pragma unroll
-  for (int i = 0; i < 16; ++i) v[i] = live ? src[(long)(i * 256 + p) * ntarget] : c2{0.0, 0.0};
+__device__ __forceinline__ void l16r_transform(c2 *v, c2 *sm, int cc, int hi, int lo) {
   SmallDft<16, SIGN>::run(v, nullptr, 0);
   // transpose 1: element (k0, i1, i2) of half h (i2 in [8h, 8h + 8)) at
   // e = (k0 * 16 + i1) * 8 + (i2 - 8h), one pad slot per 128 entries
@@ -837,9 +827,161 @@ __global__ void __launch_bounds__(256 * C) k_lattice16r(const c2 *__restrict__ i
     __syncthreads();
   }
   SmallDft<16, SIGN>::run(v, nullptr, 0);
+}
+
+// Register-resident 16^3 lattice FFT of C adjacent target columns of one
+// outer index.  Thread t = C p + cc (p = 16 hi + lo) holds a 16-point line of
+// column cc in registers and transforms it with one radix-16 register DFT per
+// axis; lanes of adjacent columns are adjacent, so each row of the grid is
+// read and written as C * 16 contiguous bytes.  LDS (columns interleaved)
+// only carries the two transposes between axes, each in two rounds over half
+// of the grid; in each round the same threads write all of their 16 values
+// and then read their next line, so no second register array is live.
+//   axis 0: line p = (hi, lo) = (i1, i2) holds i0 = 0..15 (coalesced loads)
+//   axis 1: line p = (k0, i2) holds i1; rounds split i2 = lo in halves
+//   axis 2: line p = (k0, k1) holds i2; rounds split k0 = hi in halves
+template <int C, int SIGN>
+__global__ void __launch_bounds__(256 * C) k_lattice16r(const c2 *__restrict__ in, c2 *__restrict__ out,
+                                                        long ntarget, double scale) {
+  constexpr int NSM = 2048 + 2048 / 16;  // half grid, one pad slot per 16 entries (the larger of the two paddings)
+  __shared__ c2 sm[NSM * C];
+  const long ntc = (ntarget + C - 1) / C;
+  const long tile = xcd_remap(blockIdx.x, gridDim.x);
+  const long w = tile / ntc;
+  const long c0 = (tile - w * ntc) * C;
+  const int t = threadIdx.x, cc = t % C, p = t / C, hi = p >> 4, lo = p & 15;
+  const bool live = c0 + cc < ntarget;  // columns past the end are neither read nor written
+  const c2 *src = in + w * 4096 * ntarget + c0 + cc;
+  c2 *dst = out + w * 4096 * ntarget + c0 + cc;
+  c2 v[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) v[i] = live ? src[(long)(i * 256 + p) * ntarget] : c2{0.0, 0.0};
+  l16r_transform<C, SIGN>(v, sm, cc, hi, lo);
   if (live) {
 #pragma unroll
     for (int k2 = 0; k2 < 16; ++k2) dst[(long)(p * 16 + k2) * ntarget] = cscale(v[k2], scale);
+  }
+}
+
+// Entry [0][0] of Y^-1 for an n x n complex matrix Y, n <= 3, by cofactors:
+// (Y^-1)[0][0] = C[0][0] / det Y, det Y = sum_i Y[i][0] C[i][0], with the cyclic
+// cofactor formula C[i][j] = Y[i+1][j+1] Y[i+2][j+2] - Y[i+1][j+2] Y[i+2][j+1]
+// (indices mod 3).  A thread that needs G[a][b] = (M^-1)[a][b] passes the
+// cyclically relabelled Y[i][j] = M[(b + i) mod n][(a + j) mod n]: then
+// Y^-1[0][0] = G[a][b], and every register index stays a compile-time constant.
+template <int N>
+__device__ __forceinline__ c2 inv00(const v2d (&y)[N][N], bool &singular) {
+  auto Y = [&](int i, int j) { return c2{y[i % N][j % N].x, y[i % N][j % N].y}; };
+  c2 num, det;
+  if constexpr (N == 1) {
+    num = c2{1.0, 0.0};
+    det = Y(0, 0);
+  } else if constexpr (N == 2) {
+    num = Y(1, 1);
+    det = cfms(cmul(Y(0, 0), Y(1, 1)), Y(0, 1), Y(1, 0));
+  } else {
+    c2 cof[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) cof[i] = cfms(cmul(Y(i + 1, 1), Y(i + 2, 2)), Y(i + 1, 2), Y(i + 2, 1));
+    num = cof[0];
+    det = c2{0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) det = cfma(det, Y(i, 0), cof[i]);
+  }
+  if (det.x == 0.0 && det.y == 0.0) singular = true;
+  return cdiv(num, det);
+}
+
+// Dyson fused into the 16^3 k -> r transform (lattice_dyson_g0_wk, gf.cpp:41-58,
+// then fourier_wk_to_wr, fourier_lattice.cpp:39,49), Sigma = 0, n <= 3: the
+// workgroup of frequency w and orbital row a forms G[a][b](w, k) =
+// inverse((i w_n + mu) 1 - e_k)[a][b] of its k-points from e_k (held in L2)
+// instead of reading a stored G(w, k), then transforms the columns (a, b)
+// exactly as k_lattice16r<C, -1>.  G(w, k) is never written.  C = n: one
+// workgroup per (w, a), its n columns (a, b) adjacent (n * 16-byte row
+// pieces); C = 1: one workgroup per (w, a, b) (smaller workgroups, more of them
+// resident per CU).
+template <int N, int C>
+__global__ void __launch_bounds__(256 * C) k_dyson_lattice16r(double beta, int n_iw, double mu,
+                                                              const c2 *__restrict__ ek, c2 *__restrict__ out,
+                                                              double scale, double *status) {
+  static_assert(C == 1 || C == N, "one column or one orbital row per workgroup");
+  constexpr int NT = N * N, NG = NT / C;  // workgroups per frequency
+  constexpr int NSM = 2048 + 2048 / 16;
+  __shared__ c2 sm[NSM * C];
+  const long tile = xcd_remap(blockIdx.x, gridDim.x);
+  const long w = tile / NG;
+  const int g = (int)(tile - w * NG);
+  const int a = (C == N) ? g : g / N;
+  const int t = threadIdx.x, cc = t % C, p = t / C, hi = p >> 4, lo = p & 15;
+  const int b = (C == N) ? cc : g % N;  // this thread's column (a, b)
+  const double wn = kPi * (double)(2 * (w - n_iw) + 1) / beta;
+  // entry (i, j) of the relabelled matrix: M[(b + i) mod n][(a + j) mod n]
+  int off[N][N];
+  bool diag[N][N];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      const int r = (b + i) % N, c = (a + j) % N;
+      off[i][j] = r * N + c;
+      diag[i][j] = r == c;
+    }
+  bool singular = false;
+  c2 v[16];
+  // KB k-points at a time: their KB n^2 entries of e_k are loaded before any is used
+  constexpr int KB = N == 3 ? 2 : 4;
+#pragma unroll
+  for (int i4 = 0; i4 < 16; i4 += KB) {
+    v2d x[KB][N][N];
+#pragma unroll
+    for (int u = 0; u < KB; ++u) {
+      const c2 *e = ek + (long)((i4 + u) * 256 + p) * NT;
+#pragma unroll
+      for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) x[u][i][j] = ld_v2d(e + off[i][j]);
+    }
+#pragma unroll
+    for (int u = 0; u < KB; ++u) {
+#pragma unroll
+      for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          v2d m = -x[u][i][j];
+          if (diag[i][j]) { m.x += mu; m.y += wn; }
+          x[u][i][j] = m;
+        }
+      v[i4 + u] = inv00<N>(x[u], singular);
+    }
+  }
+  if (singular) status[2] = 1.0;
+  l16r_transform<C, -1>(v, sm, cc, hi, lo);
+  c2 *dst = out + w * 4096 * NT + a * N + b;
+#pragma unroll
+  for (int k2 = 0; k2 < 16; ++k2) dst[(long)(p * 16 + k2) * NT] = cscale(v[k2], scale);
+}
+
+template <int N, int C>
+static hipError_t launch_dl16(hipStream_t s, double beta, int n_iw, double mu, const c2 *ek, c2 *out, Status st) {
+  const long nb = 2L * n_iw * (N * N / C);
+  hipLaunchKernelGGL((k_dyson_lattice16r<N, C>), dim3(nb), dim3(256 * C), 0, s, beta, n_iw, mu, ek, out, 1.0 / 4096.0,
+                     st.d);
+  return hipGetLastError();
+}
+
+// variants (same results): 0 = one orbital row per workgroup, 1 = one column
+int dyson_lattice16r_variants(int n) { return n == 1 ? 1 : 2; }
+
+hipError_t launch_dyson_lattice16r(hipStream_t s, double beta, int n_iw, double mu, int n, const c2 *ek, c2 *out,
+                                   Status st, int variant) {
+  switch (n) {
+    case 1: return launch_dl16<1, 1>(s, beta, n_iw, mu, ek, out, st);
+    case 2: return variant == 1 ? launch_dl16<2, 1>(s, beta, n_iw, mu, ek, out, st)
+                                : launch_dl16<2, 2>(s, beta, n_iw, mu, ek, out, st);
+    case 3: return variant == 1 ? launch_dl16<3, 1>(s, beta, n_iw, mu, ek, out, st)
+                                : launch_dl16<3, 3>(s, beta, n_iw, mu, ek, out, st);
+    default: return hipErrorInvalidValue;
   }
 }
 
