@@ -68,12 +68,11 @@ int tprf_ctx_set_stream(tprf_ctx *ctx, void *hip_stream); /* hipStream_t; NULL =
 /* TPRF_OPT_W2T_VARIANT: the same for the workgroup width of the pruned
  * omega -> tau kernel.                                                      */
 #define TPRF_OPT_W2T_VARIANT 3
-/* TPRF_OPT_LATTICE_VARIANT: the same for the single-pass 16x16x16 lattice
- * FFT: 0, 1, 2 = 1, 2, 4 target columns per workgroup; 3, 4 = the persistent
- * pipelined kernel with 1 or 2 columns (the fused Dyson + k -> r kernel of
- * tprf_lattice_dyson_g_wr takes the value mod 2: one orbital row or one
- * column per workgroup).  In-place calls (in == out) are never timed; they
- * use the cached or default choice.                                        */
+/* TPRF_OPT_LATTICE_VARIANT: the same for the number of target columns per
+ * workgroup of the single-pass 16x16x16 lattice FFT (0, 1, 2 = 1, 2, 4
+ * columns; the fused Dyson + k -> r kernel of tprf_lattice_dyson_g_wr takes
+ * the value mod 2: one orbital row or one column per workgroup).  In-place
+ * calls (in == out) are never timed; they use the cached or default choice. */
 #define TPRF_OPT_LATTICE_VARIANT 4
 int tprf_ctx_set_option(tprf_ctx *ctx, int option, int value);
 const char *tprf_ctx_last_error(const tprf_ctx *ctx);

@@ -673,7 +673,7 @@ int tprf_ctx_set_option(tprf_ctx *ctx, int option, int value) {
     return TPRF_OK;
   }
   if (option == TPRF_OPT_LATTICE_VARIANT) {
-    if (value < -1 || value > 4) return fail(ctx, TPRF_ERR_ARG, "bad lattice variant");
+    if (value < -1 || value > 2) return fail(ctx, TPRF_ERR_ARG, "bad lattice variant");
     ctx->lattice_variant = value;
     return TPRF_OK;
   }

@@ -3,12 +3,9 @@
 // Every kernel computes in complex FP64.  Data are in the reference's
 // row-major gf layout [mesh0][mesh1][target], so each kernel's global
 // accesses are contiguous along the innermost (orbital / r) index.
-#include <algorithm>
-
 #include "common.hpp"
 #include "fft_device.hpp"
 #include "mats_device.hpp"
-#include "lds_dma.hpp"
 
 namespace tprf {
 
@@ -779,30 +776,7 @@ __global__ void __launch_bounds__(256) k_lattice3d(const c2 *__restrict__ in, c2
 // FFT: thread t = C p + cc (p = 16 hi + lo) enters with the 16-point line
 // i0 = 0..15 at (i1, i2) = (hi, lo) of column cc and leaves with the line
 // k2 = 0..15 at (k0, k1) = (hi, lo).
-// Register-resident 16^3 lattice FFT of C adjacent target columns of one
-// outer index.  Thread t = C p + cc (p = 16 hi + lo) holds a 16-point line of
-// column cc in registers and transforms it with one radix-16 register DFT per
-// axis; lanes of adjacent columns are adjacent, so each row of the grid is
-// read and written as C * 16 contiguous bytes.  LDS (columns interleaved)
-// only carries the two transposes between axes, each in two rounds over half
-// of the grid; in each round the same threads write all of their 16 values
-// and then read their next line, so no second register array is live.
-//   axis 0: line p = (hi, lo) = (i1, i2) holds i0 = 0..15 (coalesced loads)
-//   axis 1: line p = (k0, i2) holds i1; rounds split i2 = lo in halves
-//   axis 2: line p = (k0, k1) holds i2; rounds split k0 = hi in halves
-// The three axis transforms and two transposes of the register-resident 16^3
-// FFT: thread t = C p + cc (p = 16 hi + lo) enters with the 16-point line
-// i0 = 0..15 at (i1, i2) = (hi, lo) of column cc and leaves with the line
-// k2 = 0..15 at (k0, k1) = (hi, lo).
-template <bool KEEPVM>
-__device__ __forceinline__ void lds_sync() {
-  if constexpr (KEEPVM) barrier_keep_vm();
-  else __syncthreads();
-}
-
-// KEEPVM: the barriers wait for LDS only, leaving global loads and stores of
-// other tiles in flight.
-template <int C, int SIGN, bool KEEPVM = false>
+template <int C, int SIGN>
 __device__ __forceinline__ void l16r_transform(c2 *v, c2 *sm, int cc, int hi, int lo) {
   SmallDft<16, SIGN>::run(v, nullptr, 0);
   // transpose 1: element (k0, i1, i2) of half h (i2 in [8h, 8h + 8)) at
@@ -886,64 +860,6 @@ __global__ void __launch_bounds__(256 * C) k_lattice16r(const c2 *__restrict__ i
   if (live) {
 #pragma unroll
     for (int k2 = 0; k2 < 16; ++k2) dst[(long)(p * 16 + k2) * ntarget] = cscale(v[k2], scale);
-  }
-}
-
-// Persistent, software-pipelined form of k_lattice16r: a workgroup walks its
-// tiles (outer index w, C adjacent columns) and issues the loads of its next
-// tile (16 values per thread, held in registers) before transforming the
-// current one, so one tile's loads overlap the previous tile's transposes and
-// stores; the barriers wait for LDS only.  The tiles are split into eight
-// contiguous ranges, one per XCD (workgroup b runs on XCD b mod 8), so the
-// column groups of one row share an L2.  gridDim.x is a multiple of 8.
-template <int C, int SIGN>
-__global__ void __launch_bounds__(256 * C) __attribute__((amdgpu_waves_per_eu(2)))
-    k_lattice16r_pipe(const c2 *__restrict__ in, c2 *__restrict__ out, long ntarget, long ntiles, double scale) {
-  constexpr int NSM = 2048 + 2048 / 16;
-  __shared__ c2 sm[NSM * C];
-  const long ntc = (ntarget + C - 1) / C;
-  const long x = blockIdx.x % 8, nslot = gridDim.x / 8, slot = blockIdx.x / 8;
-  const long q = ntiles / 8, rem = ntiles % 8;
-  const long first = x * q + (x < rem ? x : rem), last = first + q + (x < rem ? 1 : 0);
-  const int t = threadIdx.x, cc = t % C, p = t / C, hi = p >> 4, lo = p & 15;
-  // the source column of a tile, clamped into range (columns past the end
-  // load a valid element that is then dropped), so every load is unconditional
-  auto src_of = [&](long tile) {
-    const long w = tile / ntc, c = (tile - w * ntc) * C + cc;
-    return in + w * 4096 * ntarget + (c < ntarget ? c : ntarget - 1);
-  };
-  auto live_of = [&](long tile) { return (tile - (tile / ntc) * ntc) * C + cc < ntarget; };
-  long tile = first + slot;
-  if (tile >= last) return;
-  v2d nx[16];
-  {
-    const c2 *src = src_of(tile);
-#pragma unroll
-    for (int i = 0; i < 16; ++i) nx[i] = ld_v2d(src + (long)(i * 256 + p) * ntarget);
-  }
-  int hh = hi, ll = lo, pp = p;
-  while (true) {
-    // opaque copies of the lane indices: the LDS and global addresses derived
-    // from them are then recomputed in each iteration instead of being hoisted
-    // out of the loop (where they would not fit in registers)
-    asm volatile("" : "+v"(hh), "+v"(ll), "+v"(pp));
-    c2 v[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) v[i] = c2{nx[i].x, nx[i].y};
-    const long next = tile + nslot;
-    // past the last tile the current one is loaded again (and not used)
-    const c2 *src = src_of(next < last ? next : tile);
-#pragma unroll
-    for (int i = 0; i < 16; ++i) nx[i] = ld_v2d(src + (long)(i * 256 + pp) * ntarget);
-    l16r_transform<C, SIGN, true>(v, sm, cc, hh, ll);
-    if (live_of(tile)) {
-      const long w = tile / ntc, c0 = (tile - w * ntc) * C;
-      c2 *dst = out + w * 4096 * ntarget + c0 + cc;
-#pragma unroll
-      for (int k2 = 0; k2 < 16; ++k2) dst[(long)(pp * 16 + k2) * ntarget] = cscale(v[k2], scale);
-    }
-    if (next >= last) break;
-    tile = next;
   }
 }
 
@@ -1079,38 +995,13 @@ static hipError_t launch_l16r(hipStream_t s, const c2 *in, c2 *out, long nw, lon
   return hipGetLastError();
 }
 
-// compute units of the current device (the persistent kernels' grid)
-static int device_cus() {
-  int dev = 0, n = 0;
-  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      n < 8)
-    n = 256;
-  return n;
-}
-
-template <int C>
-static hipError_t launch_l16r_pipe(hipStream_t s, const c2 *in, c2 *out, long nw, long ntarget, int sign,
-                                   double scale) {
-  const long ntiles = nw * ((ntarget + C - 1) / C);
-  const long per_cu = C == 1 ? 2 : 1;  // workgroups resident per CU (two waves per SIMD)
-  long g = std::min<long>(ntiles, device_cus() * per_cu);
-  g = (g + 7) / 8 * 8;
-  if (sign < 0)
-    hipLaunchKernelGGL((k_lattice16r_pipe<C, -1>), dim3(g), dim3(256 * C), 0, s, in, out, ntarget, ntiles, scale);
-  else
-    hipLaunchKernelGGL((k_lattice16r_pipe<C, 1>), dim3(g), dim3(256 * C), 0, s, in, out, ntarget, ntiles, scale);
-  return hipGetLastError();
-}
-
 template <int D0, int D1, int D2>
 static hipError_t launch_l3(hipStream_t s, const c2 *in, c2 *out, long nw, long ntarget, int sign, double scale,
                             int variant) {
   if constexpr (D0 == 16 && D1 == 16 && D2 == 16) {
-    // columns per workgroup 1, 2 or 4, or persistent pipelined with 1 or 2 (same results)
+    // columns per workgroup 1, 2 or 4 (same results)
     if (variant == 1) return launch_l16r<2>(s, in, out, nw, ntarget, sign, scale);
     if (variant == 2) return launch_l16r<4>(s, in, out, nw, ntarget, sign, scale);
-    if (variant == 3) return launch_l16r_pipe<1>(s, in, out, nw, ntarget, sign, scale);
-    if (variant == 4) return launch_l16r_pipe<2>(s, in, out, nw, ntarget, sign, scale);
     return launch_l16r<1>(s, in, out, nw, ntarget, sign, scale);
   } else {
     (void)variant;
@@ -1134,7 +1025,7 @@ static const int kLattice3dShapes[][3] = {{16, 16, 16}, {12, 12, 12}, {8, 8, 8},
 
 // number of single-pass kernel variants (same results) for a lattice shape, 0 without one
 int lattice3d_variants(const int dims[3]) {
-  if (dims[0] == 16 && dims[1] == 16 && dims[2] == 16) return 5;
+  if (dims[0] == 16 && dims[1] == 16 && dims[2] == 16) return 3;
   for (const auto &d : kLattice3dShapes)
     if (d[0] == dims[0] && d[1] == dims[1] && d[2] == dims[2]) return 1;
   return 0;
