@@ -155,6 +155,20 @@ int tprf_chi_w0r_from_chi_tr(tprf_ctx *ctx, double beta, int ntau, long nr, int 
 int tprf_solve_rpa_PH(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *chi0, const tprf_c128 *U,
                       tprf_c128 *chi);
 
+/* ---- analytic bubble ----------------------------------------------------- */
+/* lindhard_chi00(e_k, mesh, mu) and lindhard_chi00(e_k, mesh, beta, mu, delta),
+ * lattice_desc.py:411,495 / lindhard_chi00.cpp:34-108:
+ *   chi[m][q][a][b][c][d] = 1/N_k sum_{k,i,j} U_k(a,i) U_k^*(d,i) U_{k+q}(c,j) U_{k+q}^*(b,j)
+ *                           [f(beta e_ki) - f(beta e_{k+q,j})] / (z_m + e_{k+q,j} - e_ki)
+ * with (e_ki, U_k) the eigenpairs of e_k - mu; z[nz] (device) holds the
+ * complex frequencies: i nu_m for a bosonic Matsubara mesh, omega + i delta
+ * on a real-frequency mesh (the reference's w + idelta); where |z_m| and
+ * |e_{k+q,j} - e_ki| are both below 1e-10 the factor is the limit
+ * beta / (4 cosh^2(beta e_ki / 2)).  e_k[nk][n][n] Hermitian, chi[nz][nk][n^4].
+ * O(N_k^2) work.  n <= 4.                                                    */
+int tprf_lindhard_chi00(tprf_ctx *ctx, double beta, double mu, long nz, const tprf_c128 *z, const int dims[3],
+                        int norb, const tprf_c128 *e_k, tprf_c128 *chi);
+
 /* ---- differentiation: static chi_RPA(Q, i nu = 0) and its adjoints ------
  * No counterpart in the reference (TPRF has no derivatives).  Each adjoint is
  * the conjugate transpose of the complex-linear map named, or the reverse-mode

@@ -501,3 +501,86 @@ def test_chain_cubic_full_size_stagewise(L, oracle):
     ref = oracle.solve_rpa_PH(c0, wl.U)
     assert relerr(host(ch.chi_wk).reshape(-1, n, n, n, n)[pts], ref) <= 1e-12
     _ = Lt
+
+
+# ---------------------------------------------------------------- f3 analytic bubble
+@pytest.mark.parametrize("dims,n,nw", [((4, 4, 1), 1, 5), ((3, 4, 1), 2, 4), ((4, 3, 2), 3, 3), ((2, 2, 2), 4, 2),
+                                       ((8, 8, 8), 3, 2), ((17, 1, 1), 2, 70)])
+def test_lindhard_chi00(L, oracle, dims, n, nw):
+    """lindhard_chi00 on a bosonic Matsubara mesh against the oracle's
+    restatement (itself pinned to a numpy einsum form and to the reference's
+    chi00_square_lattice.py comparison with the tau route)."""
+    from tprf_amd.gf import Gf, MeshBrZone, MeshImFreq
+    rng = np.random.default_rng(sum(dims) * n + nw)
+    nk = int(np.prod(dims))
+    e = hermitian_ek(rng, nk, n)
+    beta, mu = 6.0, 0.15
+    got = L.lindhard_chi00(Gf(MeshBrZone(dims), data=gpu(e)), MeshImFreq(beta, "Boson", nw), mu)
+    ref = oracle.lindhard_chi00(e, dims, beta, mu, nw)
+    assert relerr(host(got.data), ref) <= 1e-12
+
+
+def test_lindhard_chi00_degenerate_and_real_frequency(L, oracle):
+    """Degenerate bands (a k-independent 2-orbital e_k: every e_{k+q,j} - e_ki is
+    0 or the level splitting, so the nu = 0 limit beta / (4 cosh^2) is taken),
+    and the real-frequency overload lindhard_chi00(e_k, mesh, beta, mu, delta)
+    against a numpy evaluation of the same formula at omega + i delta."""
+    from tprf_amd.gf import Gf, MeshBrZone, MeshImFreq, MeshReFreq
+    dims, beta, mu = (3, 3, 1), 4.0, 0.1
+    h = np.array([[0.3, 0.2 - 0.1j], [0.2 + 0.1j, -0.4]])
+    e = np.repeat(h[None], 9, axis=0)
+    got = L.lindhard_chi00(Gf(MeshBrZone(dims), data=gpu(e)), MeshImFreq(beta, "Boson", 3), mu)
+    ref = oracle.lindhard_chi00(e, dims, beta, mu, 3)
+    assert relerr(host(got.data), ref) <= 1e-12
+    # real frequencies, delta = 0.05, on a dispersive lattice
+    rng = np.random.default_rng(3)
+    e = hermitian_ek(rng, 9, 2)
+    mesh = MeshReFreq(-2.0, 2.0, 7)
+    got = L.lindhard_chi00(Gf(MeshBrZone(dims), data=gpu(e)), mesh, beta, mu, 0.05)
+    ev, U = np.linalg.eigh(e - mu * np.eye(2))
+    k = np.array(np.unravel_index(np.arange(9), dims)).T
+    f = lambda x: np.where(x < 0, 1 / (np.exp(np.minimum(x, 0)) + 1), np.exp(-np.maximum(x, 0)) / (1 + np.exp(-np.maximum(x, 0))))
+    ref = np.zeros((7, 9, 2, 2, 2, 2), dtype=complex)
+    for q in range(9):
+        kq = np.ravel_multi_index(((k + k[q]) % dims).T, dims)
+        de = ev[kq][:, None, :] - ev[:, :, None]
+        dn = f(beta * ev)[:, :, None] - f(beta * ev[kq])[:, None, :]
+        for m, w in enumerate(mesh.values()):
+            F = dn / (w + 0.05j + de)
+            ref[m, q] = np.einsum("kai,kdi,kcj,kbj,kij->abcd", U, U.conj(), U[kq], U[kq].conj(), F) / 9
+    assert relerr(host(got.data), ref) <= 1e-12
+
+
+def test_lindhard_matches_tau_route_at_size(L):
+    """The reference's own check (chi00_square_lattice.py:122-127: the tau-route
+    chi0 equals the Lindhard bubble to decimal 8) on the GPU alone, at the
+    3-orbital 8^3 lattice, beta 20, 512 fermionic frequencies (ntau 3073), nu = 0
+    and the first bosonic frequencies: tprf_lattice_dyson_g_wr ->
+    fourier_wr_to_tr -> chi0_wr_from_grt_PH -> chi_wk_from_chi_wr against
+    lindhard_chi00 in the orientation of the two reference functions
+    (chi_imtime.cpp:127 vs lindhard_chi00.cpp:82-84):
+    chi_tau[nu, q, abcd] = chi_lindhard[-nu, q, badc]; the complex hoppings of
+    this lattice break time reversal, so the plain identity does not hold."""
+    import torch
+    from tprf_amd import _lib, workloads
+    from tprf_amd.gf import Gf, MeshBrZone, MeshImFreq
+    wl = workloads.cubic_3orb(seed=2, L=8)
+    beta, n_iw, nb = 20.0, 512, 3
+    n, nk, dims = wl.norb, wl.nk, wl.dims
+    c = _lib.ctx()
+    e = gpu(wl.e_k)
+    g_wr = torch.empty((2 * n_iw, nk, n, n), dtype=torch.complex128, device="cuda")
+    _lib.call("tprf_lattice_dyson_g_wr", c, beta, n_iw, wl.mu, _lib.dims3(dims), n, _lib.ptr(e), _lib.ptr(g_wr))
+    g_w = Gf(MeshProduct_(MeshImFreq(beta, "Fermion", n_iw), dims), data=g_wr)
+    g_t = L.fourier_wr_to_tr(g_w)
+    chi_wr = L.chi0_wr_from_grt_PH(g_t, nw=nb)
+    chi_wk = L.chi_wk_from_chi_wr(chi_wr)
+    lind = L.lindhard_chi00(Gf(MeshBrZone(dims), data=e), MeshImFreq(beta, "Boson", nb), wl.mu)
+    oriented = np.einsum("...abcd->...badc", host(lind.data)[::-1])
+    np.testing.assert_array_almost_equal(host(chi_wk.data), oriented, decimal=8)
+    assert np.abs(host(chi_wk.data) - host(lind.data)).max() > 1e-3
+
+
+def MeshProduct_(wmesh, dims):
+    from tprf_amd.gf import MeshCycLat, MeshProduct
+    return MeshProduct(wmesh, MeshCycLat(dims))

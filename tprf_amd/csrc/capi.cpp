@@ -914,6 +914,23 @@ int tprf_solve_rpa_PH(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *chi0,
                       "rpa kernel");
 }
 
+int tprf_lindhard_chi00(tprf_ctx *ctx, double beta, double mu, long nz, const tprf_c128 *z, const int dims[3],
+                        int norb, const tprf_c128 *e_k, tprf_c128 *chi) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (!(beta > 0) || nz < 1 || !z || !dims_ok(dims) || !e_k || !chi)
+    return fail(ctx, TPRF_ERR_ARG, "lindhard_chi00: bad argument");
+  if (norb < 1 || norb > 4) return fail(ctx, TPRF_ERR_UNSUPPORTED, "lindhard_chi00: norb must be 1..4");
+  const long nk = (long)dims[0] * dims[1] * dims[2];
+  const size_t ev_bytes = ((sizeof(double) * nk * norb + 255) / 256) * 256;
+  void *w;
+  int st = get_work(ctx, ev_bytes + sizeof(c2) * (size_t)nk * norb * norb * norb, &w);
+  if (st) return st;
+  return check_launch(ctx,
+                      tprf::launch_lindhard(ctx->stream, beta, mu, nz, (const c2 *)z, dims, norb, (const c2 *)e_k,
+                                            (double *)w, (c2 *)((char *)w + ev_bytes), (c2 *)chi),
+                      "lindhard kernels");
+}
+
 // ---- differentiation -----------------------------------------------------
 
 int tprf_lattice_dyson_g_wk_batched(tprf_ctx *ctx, double beta, int n_iw, const double *mu, long nbatch, long nk,

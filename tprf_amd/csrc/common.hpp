@@ -119,6 +119,10 @@ hipError_t launch_dyson(hipStream_t s, double beta, int n_iw, double mu, long nk
 hipError_t launch_dyson_lattice16r(hipStream_t s, double beta, int n_iw, double mu, int n, const c2 *ek, c2 *out,
                                    Status st, int variant);
 int dyson_lattice16r_variants(int n);
+// lindhard.hip: eigenpairs of e_k - mu into ev[nk][n] and proj[nk][n][n][n]
+// (workspace), then chi[nz][nk][n^4] at the nz complex frequencies z (device)
+hipError_t launch_lindhard(hipStream_t s, double beta, double mu, long nz, const c2 *z, const int dims[3], int n,
+                           const c2 *ek, double *ev, c2 *proj, c2 *chi);
 hipError_t launch_lattice_axis(hipStream_t s, int sign, const c2 *in, c2 *out, long outer, int len, long inner,
                                const FftDev &p, double scale);
 hipError_t launch_w2t(hipStream_t s, int statistic, double beta, int n_max, long ncol, const c2 *in, c2 *out,

@@ -58,6 +58,32 @@ class MeshImFreq:
         return f"MeshImFreq(beta={self.beta}, S={self.statistic!r}, n_max={self.n_max})"
 
 
+class MeshReFreq:
+    """Real-frequency mesh of n_w points from omega_min to omega_max, both included."""
+
+    def __init__(self, omega_min, omega_max, n_w):
+        self.omega_min, self.omega_max, self.n_w = float(omega_min), float(omega_max), int(n_w)
+        if self.n_w < 1:
+            raise ValueError("n_w must be >= 1")
+
+    @property
+    def size(self):
+        return self.n_w
+
+    def values(self):
+        return np.linspace(self.omega_min, self.omega_max, self.n_w)
+
+    def __len__(self):
+        return self.size
+
+    def __eq__(self, o):
+        return isinstance(o, MeshReFreq) and (o.omega_min, o.omega_max, o.n_w) == (self.omega_min, self.omega_max,
+                                                                                    self.n_w)
+
+    def __repr__(self):
+        return f"MeshReFreq(omega_min={self.omega_min}, omega_max={self.omega_max}, n_w={self.n_w})"
+
+
 class MeshImTime:
     """tau_j = j beta / (n_tau - 1), j = 0..n_tau-1 (both ends included)."""
 

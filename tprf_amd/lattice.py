@@ -14,13 +14,13 @@ import numpy as np
 import torch
 
 from . import _lib
-from .gf import BOSON, FERMION, Gf, MeshBrZone, MeshCycLat, MeshImFreq, MeshImTime, MeshProduct
+from .gf import BOSON, FERMION, Gf, MeshBrZone, MeshCycLat, MeshImFreq, MeshImTime, MeshProduct, MeshReFreq
 
 __all__ = [
     "lattice_dyson_g0_wk", "lattice_dyson_g_wk", "fourier_wk_to_wr", "fourier_wr_to_wk", "fourier_wr_to_tr",
     "fourier_tr_to_wr", "chi0_tr_from_grt_PH", "chi0_wr_from_grt_PH", "chi0_w0r_from_grt_PH",
     "chi_w0r_from_chi_tr", "chi_wr_from_chi_tr", "chi_tr_from_chi_wr", "chi_wk_from_chi_wr",
-    "chi_wr_from_chi_wk", "solve_rpa_PH",
+    "chi_wr_from_chi_wk", "solve_rpa_PH", "lindhard_chi00",
 ]
 
 
@@ -255,3 +255,39 @@ def solve_rpa_PH(chi0, U):
     _lib.call("tprf_solve_rpa_PH", c, npts, n, _lib.ptr(x), _lib.ptr(u), _lib.ptr(out))
     _lib.tail_status(c)
     return Gf(chi0.mesh, data=out)
+
+
+def lindhard_chi00(e_k, mesh, *args, mu=None, beta=None, delta=0.0):
+    """Analytic particle-hole bubble chi00_abcd(w, q) (lindhard_chi00.cpp:34-108).
+
+    ``lindhard_chi00(e_k, mesh, mu)`` on a bosonic Matsubara mesh
+    (lattice_desc.py:411; the frequencies are i nu_m), or
+    ``lindhard_chi00(e_k, mesh, beta, mu, delta)`` on a real-frequency mesh
+    (lattice_desc.py:495; omega + i delta).  Returns a Gf on (mesh, k mesh) with
+    an n^4 target.  O(N_k^2) work."""
+    if isinstance(mesh, MeshImFreq):
+        if args:
+            (mu,) = args
+        if mesh.statistic != BOSON:
+            raise _lib.TprfError(-1, "lindhard_chi00: statistic is incorrect.")
+        beta = mesh.beta
+        z = mesh.values()
+    elif isinstance(mesh, MeshReFreq):
+        if args:
+            beta, mu, *rest = args
+            if rest:
+                (delta,) = rest
+        z = mesh.values() + 1j * float(delta)
+    else:
+        raise TypeError("lindhard_chi00: mesh must be MeshImFreq or MeshReFreq")
+    if mu is None or beta is None:
+        raise TypeError("lindhard_chi00: mu (and beta for a real-frequency mesh) required")
+    kmesh = e_k.mesh
+    n = e_k.target_shape[0]
+    e = _dev(e_k.data)
+    zd = torch.from_numpy(np.ascontiguousarray(z, dtype=np.complex128)).to(e.device)
+    c = _lib.ctx(e.device.index)
+    out = torch.empty((len(z), kmesh.size, n, n, n, n), dtype=torch.complex128, device=e.device)
+    _lib.call("tprf_lindhard_chi00", c, float(beta), float(mu), len(z), _lib.ptr(zd), _lib.dims3(kmesh.dims), n,
+              _lib.ptr(e), _lib.ptr(out))
+    return Gf(MeshProduct(mesh, MeshBrZone(kmesh.dims)), data=out)
