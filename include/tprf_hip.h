@@ -141,6 +141,15 @@ int tprf_chi0_tr_from_grt_PH(tprf_ctx *ctx, int ntau, const int dims[3], int nor
  * stored).  out[2nw-1][nr][n^4].                                            */
 int tprf_chi0_wr_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb, int nw,
                              const tprf_c128 *g_tr, tprf_c128 *chi0_wr);
+/* The same for a G with G_ab(tau, r)^* = G_ba(tau, -r) at every mesh point (the
+ * caller's promise): G from a Hermitian e_k with no self-energy, or with one
+ * obeying Sigma(i w)^H = Sigma(-i w).  No counterpart in the reference: it
+ * transforms only the orbital pairs a <= b and forms the columns of b > a from
+ * the symmetry chi0_badc(nu, r) = conj of the tau-reversed product transform
+ * of abcd; for such a G the result equals tprf_chi0_wr_from_grt_PH's to
+ * rounding.  For a G without the symmetry the columns b > a are wrong.      */
+int tprf_chi0_wr_from_grt_PH_herm(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb, int nw,
+                                  const tprf_c128 *g_tr, tprf_c128 *chi0_wr);
 /* chi0_w0r_from_grt_PH(g_tr), chi_imtime.cpp:185-228 (trapezoid, nu = 0). out[1][nr][n^4] */
 int tprf_chi0_w0r_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb,
                               const tprf_c128 *g_tr, tprf_c128 *chi0_wr);

@@ -364,6 +364,36 @@ def test_rpa(L, oracle, n, sparse):
     assert relerr(host(got.data), ref) <= 1e-12
 
 
+@pytest.mark.parametrize("dims,n,nb", [((4, 3, 2), 2, 20), ((6, 4, 1), 3, 16), ((4, 4, 2), 3, 40), ((3, 3, 3), 2, 50)])
+def test_chi0_wr_herm(L, oracle, dims, n, nb):
+    """tprf_chi0_wr_from_grt_PH_herm on a G(tau, r) from a Hermitian e_k (the
+    oracle's Dyson -> k -> r -> w -> tau chain), every workgroup shape that has
+    the Hermitian form (option 2), against the oracle's chi0_wr_from_grt_PH;
+    lattices with odd and even sides (self-mirrored r) and inner DFT lengths
+    64 and 128."""
+    import torch
+    from tprf_amd import _lib
+    rng = np.random.default_rng(100 * n + nb)
+    nk = int(np.prod(dims))
+    e = hermitian_ek(rng, nk, n, scale=0.7)
+    beta, n_iw = 8.0, 64
+    g_tr = oracle.fourier_w_to_t(oracle.fourier_lattice(oracle.lattice_dyson_g_wk(beta, n_iw, 0.1, e), dims, -1),
+                                 beta)
+    ntau = g_tr.shape[0]
+    ref = oracle.chi0_wr_from_grt_PH(g_tr, dims, beta, nb)
+    c = _lib.ctx()
+    gt = gpu(g_tr)
+    for variant in (0, 1, 3, 4, -1):
+        _lib.call("tprf_ctx_set_option", c, 2, variant)
+        try:
+            out = torch.full((2 * nb - 1, nk, n, n, n, n), float("nan"), dtype=torch.complex128, device="cuda")
+            _lib.call("tprf_chi0_wr_from_grt_PH_herm", c, beta, ntau, _lib.dims3(dims), n, nb, _lib.ptr(gt),
+                      _lib.ptr(out))
+            assert relerr(host(out), ref) <= TOL, variant
+        finally:
+            _lib.call("tprf_ctx_set_option", c, 2, -1)
+
+
 # ---------------------------------------------------------------- chains
 def oracle_chain(O, wl):
     g_wk = O.lattice_dyson_g_wk(wl.beta, wl.n_iw, wl.mu, wl.e_k)

@@ -9,6 +9,7 @@ Stages and the reference functions they replace:
 Every stage is one asynchronous C-ABI call on torch's current stream; the
 tail-fit status is read once, after the chain.
 """
+import numpy as np
 import torch
 
 from . import _lib
@@ -16,14 +17,19 @@ from . import _lib
 
 class BubbleRPA:
     """fused=True (default): the first two stages run as one call,
-    tprf_lattice_dyson_g_wr (G(iw, k) is then not stored for a 16^3 lattice);
-    fused=False: one call per reference function."""
+    tprf_lattice_dyson_g_wr (G(iw, k) is then not stored for a 16^3 lattice),
+    and, when e_k is Hermitian (so G_ab(tau, r)^* = G_ba(tau, -r)), the bubble
+    runs as tprf_chi0_wr_from_grt_PH_herm (orbital pairs a <= b transformed,
+    the others formed from the symmetry); fused=False: one call per reference
+    function."""
     API_STAGES = ("dyson", "wk_to_wr", "wr_to_tr", "chi0_wr", "wr_to_wk", "rpa")
     FUSED_STAGES = ("dyson_wr", "wr_to_tr", "chi0_wr", "wr_to_wk", "rpa")
 
     def __init__(self, wl, device="cuda", save_intermediates=True, fused=True):
         self.wl = wl
         self.STAGES = self.FUSED_STAGES if fused else self.API_STAGES
+        e = wl.e_k
+        self.herm = bool(fused) and bool(np.abs(e - np.conj(np.swapaxes(e, 1, 2))).max() <= 1e-14 * max(1.0, np.abs(e).max()))
         dev = torch.device(device)
         self.dev = dev
         n, nk = wl.norb, wl.nk
@@ -54,8 +60,8 @@ class BubbleRPA:
             _lib.call("tprf_fourier_wr_to_tr", c, _lib.FERMION, wl.beta, wl.n_iw, wl.ntau, nk * n * n, p(self.g_wr),
                       p(self.g_tr), None)
         elif name == "chi0_wr":
-            _lib.call("tprf_chi0_wr_from_grt_PH", c, wl.beta, wl.ntau, self.dims, n, wl.nb, p(self.g_tr),
-                      p(self.chi0_wr))
+            fn = "tprf_chi0_wr_from_grt_PH_herm" if self.herm else "tprf_chi0_wr_from_grt_PH"
+            _lib.call(fn, c, wl.beta, wl.ntau, self.dims, n, wl.nb, p(self.g_tr), p(self.chi0_wr))
         elif name == "wr_to_wk":
             _lib.call("tprf_fourier_wr_to_wk", c, wl.n_nu, self.dims, n ** 4, p(self.chi0_wr), p(self.chi0_wk))
         elif name == "rpa":

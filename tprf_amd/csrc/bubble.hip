@@ -37,15 +37,23 @@ namespace tprf {
 // issues 64-entry chunks of the tile and waits for its own older chunks with
 // vmcnt before the barrier that publishes them.
 // WPE: waves per SIMD the register allocation must allow (1: no limit).
-template <int N, int R1, int R2, int CG, int NT, bool AB, bool DMA, int WPE>
+// HERM (with AB): G satisfies G_ab(tau, r)^* = G_ba(tau, -r) (a Hermitian e_k
+// and no self-energy, or one with Sigma(i w)^H = Sigma(-i w)).  Then the
+// products obey x^r_badc(tau_j) = x^r_abcd(tau_{L-j})^* and
+// x^{-r}_dcba(tau_j) = x^r_abcd(tau_j)^*, so only the orbital pairs a <= b are
+// transformed: a workgroup with a < b also writes the columns badc of r and
+// dcba of -r, with S_badc(m) = (S(m) - x_0 + x_L)^*, S^{-r}_dcba(m) = S(-m)^*
+// and the conjugated pole coefficients of the mirrored and the direct column.
+template <int N, int R1, int R2, int CG, int NT, bool AB, bool DMA, int WPE, bool HERM = false>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) k_bubble_pair(double beta, int L, int nb, int d0, int d1, int d2,
                                                     const c2 *__restrict__ gtr, c2 *__restrict__ out, BubbleDev bd,
                                                     Vinv8 vinv) {
   constexpr int P = R1 * R2;
   constexpr int n2 = N * N, n4 = n2 * n2;
   static_assert(!AB || CG == n2, "one orbital pair per workgroup");
+  static_assert(!HERM || AB, "the Hermitian form works on orbital-pair workgroups");
   static_assert(NT >= P, "the outer twiddles of a step are staged one per thread");
-  constexpr int NG = (n4 + CG - 1) / CG;
+  constexpr int NG = HERM ? N * (N + 1) / 2 : (n4 + CG - 1) / CG;
   constexpr int U2 = (CG * R1 + NT - 1) / NT;
   constexpr int TLD = R2 + 1;  // padded LDS row of the transpose
   constexpr int NE = AB ? N : n2;  // staged entries of G per tau point and side
@@ -73,9 +81,17 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
   const int grp = (int)(tile - pair * NG);
   const long r = bd.rcanon[pair];
   const long mr = minus_r(r, d0, d1, d2);
-  const int q0 = grp * CG;
+  // the orbital pair of an AB workgroup: (grp / n, grp % n), or with HERM the
+  // grp-th pair a <= b in row-major order
+  int a0 = grp / N, b0 = grp % N;
+  if constexpr (HERM) {
+    a0 = 0;
+    int rest = grp;
+    while (rest >= N - a0) { rest -= N - a0; ++a0; }
+    b0 = a0 + rest;
+  }
+  const int q0 = AB ? (a0 * N + b0) * n2 : grp * CG;
   const int cn = (n4 - q0) < CG ? (n4 - q0) : CG;
-  const int a0 = q0 / (n2 * N), b0 = (q0 / n2) % N;  // the orbital pair of an AB workgroup
   const int N1 = L / P;
   const double dtau = beta / (double)L;
 
@@ -320,7 +336,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
       const int q = q0 + c;
       const int dd = q % N, cc = (q / N) % N, bb = (q / n2) % N, aa = q / (n2 * N);
       const int qm = ((cc * N + dd) * N + aa) * N + bb;
-      const c2 *a0 = sPole[c][0], *a1 = sPole[c][1];
+      const c2 *pa0 = sPole[c][0], *pa1 = sPole[c][1];
       const c2 xl_minus_x0 = csub(sE[c][17], sE[c][0]);
 #pragma unroll
       for (int k2 = 0; k2 < R2; ++k2) {
@@ -330,37 +346,78 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
         const int im = m + nb - 1;
         // y(m) = fact S(m) + sum_q a_q H_q(m), H_q = 1 / (i nu_m - b_q) - fact Phi_q(m)
         const c2 h0 = hmod(im), h1 = hmod(nnu + im), h2 = hmod(2 * nnu + im);
-        c2 v = cadd(cscale(acc[i][k2], fact), cfma(cfma(cmul(a0[0], h0), a0[1], h1), a0[2], h2));
+        c2 v = cadd(cscale(acc[i][k2], fact), cfma(cfma(cmul(pa0[0], h0), pa0[1], h1), pa0[2], h2));
         out[((long)im * nr + r) * n4 + q] = v;
         if (mirror) {
           const c2 sm = cadd(sS[c * P + ((P - k) & (P - 1))], xl_minus_x0);
-          v = cadd(cscale(sm, fact), cfma(cfma(cmul(a1[0], h0), a1[1], h1), a1[2], h2));
+          v = cadd(cscale(sm, fact), cfma(cfma(cmul(pa1[0], h0), pa1[1], h1), pa1[2], h2));
           out[((long)im * nr + mr) * n4 + qm] = v;
+        }
+        if constexpr (HERM) {
+          if (a0 < b0) {
+            // column badc of r: S = (S(m) - x_0 + x_L)^*, pole coefficients conj(a1)
+            const int qb = ((bb * N + aa) * N + dd) * N + cc;
+            const c2 sb = cconj(cadd(acc[i][k2], xl_minus_x0));
+            v = cadd(cscale(sb, fact),
+                     cfma(cfma(cmul(cconj(pa1[0]), h0), cconj(pa1[1]), h1), cconj(pa1[2]), h2));
+            out[((long)im * nr + r) * n4 + qb] = v;
+            if (mirror) {
+              // column dcba of -r: S = S(-m)^*, pole coefficients conj(a0)
+              const int qd = ((dd * N + cc) * N + bb) * N + aa;
+              const c2 sd = cconj(sS[c * P + ((P - k) & (P - 1))]);
+              v = cadd(cscale(sd, fact),
+                       cfma(cfma(cmul(cconj(pa0[0]), h0), cconj(pa0[1]), h1), cconj(pa0[2]), h2));
+              out[((long)im * nr + mr) * n4 + qd] = v;
+            }
+          }
         }
       }
     }
   }
 }
 
-template <int N, int R1, int R2, int CG, int NT, bool AB, bool DMA, int WPE>
-static hipError_t launch_one(hipStream_t s, double beta, int L, int nb, const int dims[3], const c2 *gtr, c2 *out,
-                             const BubbleDev &bd, const Vinv8 &vinv) {
+template <int N, int R1, int R2, int CG, int NT, bool AB, bool DMA, int WPE, bool HERM>
+static hipError_t launch_one_h(hipStream_t s, double beta, int L, int nb, const int dims[3], const c2 *gtr, c2 *out,
+                               const BubbleDev &bd, const Vinv8 &vinv) {
   constexpr int n4 = N * N * N * N;
-  constexpr int NG = (n4 + CG - 1) / CG;
+  constexpr int NG = HERM ? N * (N + 1) / 2 : (n4 + CG - 1) / CG;
   const long nb_blocks = (long)bd.ncanon * NG;
   if (DMA && L / (R1 * R2) < 2) {  // the global -> LDS pipeline needs two tiles
-    hipLaunchKernelGGL((k_bubble_pair<N, R1, R2, CG, NT, AB, false, 1>), dim3(nb_blocks), dim3(NT), 0, s, beta, L, nb,
-                       dims[0], dims[1], dims[2], gtr, out, bd, vinv);
+    hipLaunchKernelGGL((k_bubble_pair<N, R1, R2, CG, NT, AB, false, 1, HERM>), dim3(nb_blocks), dim3(NT), 0, s, beta, L,
+                       nb, dims[0], dims[1], dims[2], gtr, out, bd, vinv);
     return hipGetLastError();
   }
-  hipLaunchKernelGGL((k_bubble_pair<N, R1, R2, CG, NT, AB, DMA, WPE>), dim3(nb_blocks), dim3(NT), 0, s, beta, L, nb, dims[0],
-                     dims[1], dims[2], gtr, out, bd, vinv);
+  hipLaunchKernelGGL((k_bubble_pair<N, R1, R2, CG, NT, AB, DMA, WPE, HERM>), dim3(nb_blocks), dim3(NT), 0, s, beta, L,
+                     nb, dims[0], dims[1], dims[2], gtr, out, bd, vinv);
   return hipGetLastError();
+}
+
+// launches the plain form, or with herm the Hermitian form where the shape
+// has orbital-pair workgroups (hipErrorNotSupported otherwise)
+template <int N, int R1, int R2, int CG, int NT, bool AB, bool DMA, int WPE>
+static hipError_t launch_one(hipStream_t s, double beta, int L, int nb, const int dims[3], const c2 *gtr, c2 *out,
+                             const BubbleDev &bd, const Vinv8 &vinv, bool herm) {
+  if (herm) {
+    if constexpr (AB)
+      return launch_one_h<N, R1, R2, CG, NT, AB, DMA, WPE, true>(s, beta, L, nb, dims, gtr, out, bd, vinv);
+    else
+      return hipErrorNotSupported;
+  }
+  return launch_one_h<N, R1, R2, CG, NT, AB, DMA, WPE, false>(s, beta, L, nb, dims, gtr, out, bd, vinv);
 }
 
 bool bubble_pair_supported(int n, int P) { return (n == 2 || n == 3) && (P == 64 || P == 128); }
 
 int bubble_pair_variants() { return 5; }
+
+// the variants with orbital-pair workgroups (those the Hermitian form has)
+int bubble_pair_herm_variants(int n, int P, int *list) {
+  static const int v3[] = {0, 1, 3, 4}, v2a[] = {0, 3}, v2b[] = {0, 1, 3, 4};
+  const int *v = (n == 3) ? v3 : (P == 128 ? v2a : v2b);
+  const int cnt = (n == 3) ? 4 : (P == 128 ? 2 : 4);
+  for (int i = 0; i < cnt; ++i) list[i] = v[i];
+  return cnt;
+}
 
 // Workgroup shapes (same results, picked by the autotuner): variant 0 owns one
 // orbital pair; 1 is a narrower or differently grouped shape; 2 stages every G
@@ -368,9 +425,9 @@ int bubble_pair_variants() { return 5; }
 // straight into LDS (for n = 3, P = 128 the two differ in the occupancy the
 // register allocation is held to).
 hipError_t launch_bubble_pair(hipStream_t s, double beta, int L, int nb, const int dims[3], int n, const c2 *gtr,
-                              c2 *out, const BubbleDev &bd, const Vinv8 &vinv, int variant) {
+                              c2 *out, const BubbleDev &bd, const Vinv8 &vinv, int variant, bool herm) {
 #define TPRF_BP(N_, R1_, R2_, CG_, NT_, AB_, DMA_, WPE_) \
-  return launch_one<N_, R1_, R2_, CG_, NT_, AB_, DMA_, WPE_>(s, beta, L, nb, dims, gtr, out, bd, vinv)
+  return launch_one<N_, R1_, R2_, CG_, NT_, AB_, DMA_, WPE_>(s, beta, L, nb, dims, gtr, out, bd, vinv, herm)
   if (n == 3 && bd.P == 128) {
     if (variant == 0) TPRF_BP(3, 16, 8, 9, 256, true, false, 1);
     if (variant == 1) TPRF_BP(3, 16, 8, 9, 128, true, false, 1);

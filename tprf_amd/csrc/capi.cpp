@@ -65,6 +65,7 @@ struct tprf_ctx {
   size_t work_bytes = 0;
   int bubble_variant = -1;                                // -1: autotuned per shape
   std::map<std::tuple<int, int, int, int, long>, int> bubble_tuned;
+  std::map<std::tuple<int, int, int, int, long>, int> bubble_herm_tuned;
   int w2t_variant = -1;
   std::map<std::tuple<int, int, int, int, long>, int> w2t_tuned;
   int lattice_variant = -1;
@@ -848,8 +849,10 @@ int tprf_chi0_tr_from_grt_PH(tprf_ctx *ctx, int ntau, const int dims[3], int nor
                       "chi0_tr kernel");
 }
 
-int tprf_chi0_wr_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb, int nw,
-                             const tprf_c128 *g_tr, tprf_c128 *chi0_wr) {
+namespace {
+
+int chi0_wr_from_grt(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb, int nw, const tprf_c128 *g_tr,
+                     tprf_c128 *chi0_wr, bool herm) {
   if (!ctx) return TPRF_ERR_ARG;
   if (!(beta > 0) || ntau < 10 || !dims_ok(dims) || norb < 1 || norb > 4 || nw < 1 || !g_tr || !chi0_wr)
     return fail(ctx, TPRF_ERR_ARG, "chi0_wr_from_grt_PH: bad argument");
@@ -870,8 +873,22 @@ int tprf_chi0_wr_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dim
                                       ctx->vinv, v);
     };
     const auto key = std::make_tuple(norb, P, L, nw, (long)dims[0] * dims[1] * dims[2]);
-    return tuned_launch(ctx, ctx->bubble_tuned, key, tprf::bubble_pair_variants(), ctx->bubble_variant, launch,
-                        "bubble pair kernel");
+    if (!herm)
+      return tuned_launch(ctx, ctx->bubble_tuned, key, tprf::bubble_pair_variants(), ctx->bubble_variant, launch,
+                          "bubble pair kernel");
+    int list[5];
+    const int cnt = tprf::bubble_pair_herm_variants(norb, P, list);
+    auto launch_h = [&](int i) {
+      return tprf::launch_bubble_pair(ctx->stream, beta, L, nw, dims, norb, (const c2 *)g_tr, (c2 *)chi0_wr, *bd,
+                                      ctx->vinv, list[i], true);
+    };
+    int forced = -1;  // a forced plain variant without a Hermitian form uses the first one
+    if (ctx->bubble_variant >= 0) {
+      forced = 0;
+      for (int i = 0; i < cnt; ++i)
+        if (list[i] == ctx->bubble_variant) forced = i;
+    }
+    return tuned_launch(ctx, ctx->bubble_herm_tuned, key, cnt, forced, launch_h, "bubble pair kernel (Hermitian)");
   }
   const int ld = (L + 1) | 1;
   if (2L * ld * (long)sizeof(c2) > tprf::kMatsLdsBytes)
@@ -884,6 +901,18 @@ int tprf_chi0_wr_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dim
                       tprf::launch_chi0_wr(ctx->stream, beta, nw, dims, norb, (const c2 *)g_tr, (c2 *)chi0_wr, *mt,
                                            *p, ctx->vinv),
                       "chi0_wr kernel");
+}
+
+}  // namespace
+
+int tprf_chi0_wr_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb, int nw,
+                             const tprf_c128 *g_tr, tprf_c128 *chi0_wr) {
+  return chi0_wr_from_grt(ctx, beta, ntau, dims, norb, nw, g_tr, chi0_wr, false);
+}
+
+int tprf_chi0_wr_from_grt_PH_herm(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb, int nw,
+                                  const tprf_c128 *g_tr, tprf_c128 *chi0_wr) {
+  return chi0_wr_from_grt(ctx, beta, ntau, dims, norb, nw, g_tr, chi0_wr, true);
 }
 
 int tprf_chi0_w0r_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb,

@@ -136,11 +136,16 @@ hipError_t launch_chi0_w0r(hipStream_t s, double beta, int ntau, const int dims[
 hipError_t launch_chi_w0r(hipStream_t s, double beta, int ntau, long nr, int n, const c2 *chi_tr, c2 *out);
 hipError_t launch_rpa(hipStream_t s, long npts, int n, const c2 *chi0, const c2 *U, c2 *chi, Status st);
 // bubble.hip: returns hipErrorNotSupported when (n, P) has no specialisation
-// variant < bubble_pair_variants() selects the workgroup shape (same results)
+// variant < bubble_pair_variants() selects the workgroup shape (same results);
+// herm: the form for G_ab(tau, r)^* = G_ba(tau, -r), which transforms only the
+// orbital pairs a <= b (hipErrorNotSupported for shapes without orbital-pair
+// workgroups)
 hipError_t launch_bubble_pair(hipStream_t s, double beta, int L, int nb, const int dims[3], int n, const c2 *gtr,
-                              c2 *out, const BubbleDev &bd, const Vinv8 &vinv, int variant);
+                              c2 *out, const BubbleDev &bd, const Vinv8 &vinv, int variant, bool herm = false);
 bool bubble_pair_supported(int n, int P);
 int bubble_pair_variants();
+// variants usable with herm, written to list[0..count), count returned (<= 5)
+int bubble_pair_herm_variants(int n, int P, int *list);
 // w2t.hip: returns hipErrorNotSupported when M has no specialisation
 // variant < w2t_pruned_variants() selects the workgroup width (same results)
 hipError_t launch_w2t_pruned(hipStream_t s, int statistic, double beta, int n_max, long ncol, const c2 *in, c2 *out,
