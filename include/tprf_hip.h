@@ -163,6 +163,15 @@ int tprf_chi_w0r_from_chi_tr(tprf_ctx *ctx, double beta, int ntau, long nr, int 
  * chi0[npts][n^4], U[n^4] (device), out chi[npts][n^4].  n <= 4.           */
 int tprf_solve_rpa_PH(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *chi0, const tprf_c128 *U,
                       tprf_c128 *chi);
+/* solve_rpa_PH on a bosonic Matsubara mesh of 2 nb - 1 frequencies
+ * (chi0, chi [2 nb - 1][nk][n^4]) when M_chi0(-nu, q) = M_chi0(nu, q)^H and
+ * M_U = M_U^H (the caller's promise; it holds for chi0 from a Hermitian e_k
+ * and a real interaction with U_abcd = U_dcba, such as the Kanamori tensors).
+ * No counterpart in the reference: only the nb frequencies nu >= 0 of chi0
+ * are read and solved; chi(-nu, q) is written as M_chi(nu, q)^H, which is
+ * what the full solve gives for such inputs, to rounding.                   */
+int tprf_solve_rpa_PH_herm(tprf_ctx *ctx, int nb, long nk, int norb, const tprf_c128 *chi0, const tprf_c128 *U,
+                           tprf_c128 *chi);
 
 /* ---- analytic bubble ----------------------------------------------------- */
 /* lindhard_chi00(e_k, mesh, mu) and lindhard_chi00(e_k, mesh, beta, mu, delta),

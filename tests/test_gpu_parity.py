@@ -346,6 +346,35 @@ def test_chi0_w0r_trapz(L, oracle, gtr_case):
 
 
 # ---------------------------------------------------------------- a7 RPA
+@pytest.mark.parametrize("n,nb,nk", [(1, 5, 7), (2, 4, 13), (3, 6, 29), (3, 1, 40), (4, 2, 5)])
+def test_rpa_herm(L, oracle, n, nb, nk):
+    """tprf_solve_rpa_PH_herm: chi0 with M_chi0(-nu) = M_chi0(nu)^H and a
+    Hermitian M_U; the nu >= 0 half is solved and chi(-nu) written as the
+    conjugate transpose, against the oracle's full solve of every point."""
+    import torch
+    from tprf_amd import _lib
+    rng = np.random.default_rng(n * 31 + nb)
+    n2 = n * n
+
+    def from_M(M):  # M[(a,b),(d,c)] -> X[a,b,c,d]
+        return M.reshape(M.shape[:-2] + (n, n, n, n)).swapaxes(-1, -2)
+
+    Mp = rnd(rng, nb, nk, n2, n2) * 0.15
+    Mp[0] = 0.5 * (Mp[0] + np.conj(np.swapaxes(Mp[0], -1, -2)))   # nu = 0 is Hermitian
+    Mfull = np.concatenate([np.conj(np.swapaxes(Mp[:0:-1], -1, -2)), Mp], axis=0)
+    chi0 = from_M(Mfull)
+    MU = rnd(rng, n2, n2) * 0.4
+    U = from_M(0.5 * (MU + MU.conj().T))
+    ref = oracle.solve_rpa_PH(chi0, U)
+    x, u = gpu(chi0), gpu(U)
+    out = torch.full_like(x, float("nan"))
+    c = _lib.ctx()
+    _lib.tail_status_reset(c)
+    _lib.call("tprf_solve_rpa_PH_herm", c, nb, nk, n, _lib.ptr(x), _lib.ptr(u), _lib.ptr(out))
+    _lib.tail_status(c)
+    assert relerr(host(out), ref) <= 1e-12
+
+
 @pytest.mark.parametrize("sparse", [False, True])
 @pytest.mark.parametrize("n", [1, 2, 3, 4])
 def test_rpa(L, oracle, n, sparse):
@@ -471,11 +500,22 @@ def test_chain_fused_equals_per_function(L):
     got, err = run_chain(wl, fused=True)
     assert err < 1e-6
     assert got.g_wk is None
+    assert got.herm and got.herm_nu
     assert relerr(host(got.g_wr), host(ref.g_wr)) <= 1e-12
     # the tail fit of omega -> tau weighs the high frequencies by up to
     # w_max^3, so rounding-level differences of G(iw, r) grow by ~10 there
-    for name in ("g_tr", "chi0_wr", "chi0_wk"):
+    for name in ("g_tr", "chi0_wr"):
         assert relerr(host(getattr(got, name)), host(getattr(ref, name))) <= TOL, name
+    # with the symmetries the fused chain transforms nu >= 0 only
+    nb = wl.nb
+    assert relerr(host(got.chi0_wk)[nb - 1:], host(ref.chi0_wk)[nb - 1:]) <= TOL
+    # chi_RPA at every frequency, on the points whose RPA matrix has cond <= 100
+    # (cond reaches ~4e5 here, which amplifies rounding of chi0)
+    kappa = rpa_condition(host(ref.chi0_wk), wl.U)
+    good = kappa <= 100
+    n4 = wl.norb ** 4
+    assert good.mean() > 0.9
+    assert relerr(host(got.chi_wk).reshape(-1, n4)[good], host(ref.chi_wk).reshape(-1, n4)[good]) <= TOL
 
 
 def test_chain_cubic_full_size_stagewise(L, oracle):

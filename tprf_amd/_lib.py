@@ -52,6 +52,7 @@ _SIGNATURES = {
     "tprf_chi0_w0r_from_grt_PH": [_p, _d, _i, _p, _i, _p, _p],
     "tprf_chi_w0r_from_chi_tr": [_p, _d, _i, _l, _i, _p, _p],
     "tprf_solve_rpa_PH": [_p, _l, _i, _p, _p, _p],
+    "tprf_solve_rpa_PH_herm": [_p, _i, _l, _i, _p, _p, _p],
     "tprf_lindhard_chi00": [_p, _d, _d, _l, _p, _p, _i, _p, _p],
     "tprf_lattice_dyson_g_wk_batched": [_p, _d, _i, _p, _l, _l, _i, _p, _p],
     "tprf_lattice_dyson_g_wk_adjoint": [_p, _i, _l, _i, _l, _p, _p, _p, _p],

@@ -20,8 +20,12 @@ class BubbleRPA:
     tprf_lattice_dyson_g_wr (G(iw, k) is then not stored for a 16^3 lattice),
     and, when e_k is Hermitian (so G_ab(tau, r)^* = G_ba(tau, -r)), the bubble
     runs as tprf_chi0_wr_from_grt_PH_herm (orbital pairs a <= b transformed,
-    the others formed from the symmetry); fused=False: one call per reference
-    function."""
+    the others formed from the symmetry); if U_abcd = conj(U_dcba) as well,
+    M_chi0(-nu, q) = M_chi0(nu, q)^H and M_chi(-nu, q) = M_chi(nu, q)^H, so
+    r -> q transforms the frequencies nu >= 0 only and
+    tprf_solve_rpa_PH_herm solves those and writes chi(-nu) as the conjugate
+    transposes (chi0_wk then holds nu >= 0 only).  fused=False: one call per
+    reference function."""
     API_STAGES = ("dyson", "wk_to_wr", "wr_to_tr", "chi0_wr", "wr_to_wk", "rpa")
     FUSED_STAGES = ("dyson_wr", "wr_to_tr", "chi0_wr", "wr_to_wk", "rpa")
 
@@ -30,6 +34,9 @@ class BubbleRPA:
         self.STAGES = self.FUSED_STAGES if fused else self.API_STAGES
         e = wl.e_k
         self.herm = bool(fused) and bool(np.abs(e - np.conj(np.swapaxes(e, 1, 2))).max() <= 1e-14 * max(1.0, np.abs(e).max()))
+        u = wl.U
+        self.herm_nu = self.herm and bool(np.abs(u - np.conj(np.einsum("abcd->dcba", u))).max() <=
+                                          1e-14 * max(1.0, np.abs(u).max()))
         dev = torch.device(device)
         self.dev = dev
         n, nk = wl.norb, wl.nk
@@ -63,9 +70,16 @@ class BubbleRPA:
             fn = "tprf_chi0_wr_from_grt_PH_herm" if self.herm else "tprf_chi0_wr_from_grt_PH"
             _lib.call(fn, c, wl.beta, wl.ntau, self.dims, n, wl.nb, p(self.g_tr), p(self.chi0_wr))
         elif name == "wr_to_wk":
-            _lib.call("tprf_fourier_wr_to_wk", c, wl.n_nu, self.dims, n ** 4, p(self.chi0_wr), p(self.chi0_wk))
+            if self.herm_nu:  # nu >= 0 only: rows nb - 1 .. 2 nb - 2
+                _lib.call("tprf_fourier_wr_to_wk", c, wl.nb, self.dims, n ** 4, p(self.chi0_wr[wl.nb - 1:]),
+                          p(self.chi0_wk[wl.nb - 1:]))
+            else:
+                _lib.call("tprf_fourier_wr_to_wk", c, wl.n_nu, self.dims, n ** 4, p(self.chi0_wr), p(self.chi0_wk))
         elif name == "rpa":
-            _lib.call("tprf_solve_rpa_PH", c, wl.n_nu * nk, n, p(self.chi0_wk), p(self.U), p(self.chi_wk))
+            if self.herm_nu:
+                _lib.call("tprf_solve_rpa_PH_herm", c, wl.nb, nk, n, p(self.chi0_wk), p(self.U), p(self.chi_wk))
+            else:
+                _lib.call("tprf_solve_rpa_PH", c, wl.n_nu * nk, n, p(self.chi0_wk), p(self.U), p(self.chi_wk))
         else:
             raise KeyError(name)
 

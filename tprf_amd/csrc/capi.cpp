@@ -960,6 +960,17 @@ int tprf_lindhard_chi00(tprf_ctx *ctx, double beta, double mu, long nz, const tp
                       "lindhard kernels");
 }
 
+int tprf_solve_rpa_PH_herm(tprf_ctx *ctx, int nb, long nk, int norb, const tprf_c128 *chi0, const tprf_c128 *U,
+                           tprf_c128 *chi) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (nb < 1 || nk < 1 || !chi0 || !U || !chi) return fail(ctx, TPRF_ERR_ARG, "solve_rpa_PH_herm: bad argument");
+  if (norb < 1 || norb > 4) return fail(ctx, TPRF_ERR_UNSUPPORTED, "solve_rpa_PH_herm: norb must be 1..4");
+  return check_launch(ctx,
+                      tprf::launch_rpa_mirror(ctx->stream, nb, nk, norb, (const c2 *)chi0, (const c2 *)U, (c2 *)chi,
+                                              tprf::Status{ctx->d_status}),
+                      "rpa kernel (mirrored)");
+}
+
 // ---- differentiation -----------------------------------------------------
 
 int tprf_lattice_dyson_g_wk_batched(tprf_ctx *ctx, double beta, int n_iw, const double *mu, long nbatch, long nk,

@@ -135,6 +135,10 @@ hipError_t launch_chi0_tr(hipStream_t s, int ntau, const int dims[3], int n, con
 hipError_t launch_chi0_w0r(hipStream_t s, double beta, int ntau, const int dims[3], int n, const c2 *gtr, c2 *out);
 hipError_t launch_chi_w0r(hipStream_t s, double beta, int ntau, long nr, int n, const c2 *chi_tr, c2 *out);
 hipError_t launch_rpa(hipStream_t s, long npts, int n, const c2 *chi0, const c2 *U, c2 *chi, Status st);
+// RPA at the nonnegative bosonic frequencies of chi0[2 nb - 1][nk][n^4], each
+// result also written at -nu as its conjugate transpose in the PH grouping
+hipError_t launch_rpa_mirror(hipStream_t s, long nb, long nk, int n, const c2 *chi0, const c2 *U, c2 *chi,
+                             Status st);
 // bubble.hip: returns hipErrorNotSupported when (n, P) has no specialisation
 // variant < bubble_pair_variants() selects the workgroup shape (same results);
 // herm: the form for G_ab(tau, r)^* = G_ba(tau, -r), which transforms only the

@@ -486,9 +486,14 @@ __device__ __forceinline__ void wave_lds_sync() {
 // row is exchanged through LDS.
 // n = 3: registers held to four waves per SIMD (a few spills outside the
 // pivot loop cost less than the lost wave: 0.51 -> 0.47 ms at 127 x 4096 points)
-template <int N>
+// MIRROR: the points are the nb * nk points of the nonnegative bosonic
+// frequencies m = pt / nk >= 0 of arrays [2 nb - 1][nk][n^4] (read and written
+// at base + pt, base = (nb - 1) nk), and each result is also written at -m as
+// its conjugate transpose, M_chi(-nu, q) = M_chi(nu, q)^H, the identity that
+// holds when M_chi0(-nu, q) = M_chi0(nu, q)^H and M_U is Hermitian.
+template <int N, bool MIRROR = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N == 3 ? 4 : 1))) k_rpa(long npts, const c2 *__restrict__ chi0, const c2 *__restrict__ U,
-                                             c2 *__restrict__ chi, double *status) {
+                                             c2 *__restrict__ chi, double *status, long base = 0, long nk = 1) {
   constexpr int N2 = N * N;
   constexpr int P = 64 / N2;  // points per wave
   constexpr int NMW = (N2 * N2 + 63) / 64;  // words of the nonzero mask of M_U
@@ -526,7 +531,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N == 3
   const bool valid = (gidx < P) && (pt < npts);
   c2 X[N2], A[N2];
   if (valid) {
-    const c2 *row = chi0 + pt * N2 * N2 + i * N2;
+    const c2 *row = chi0 + (base + pt) * N2 * N2 + i * N2;
 #pragma unroll
     for (int c = 0; c < N; ++c)
 #pragma unroll
@@ -585,12 +590,46 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N == 3
   if (singular) status[2] = 1.0;
   if (valid) {
     const c2 inv = cdiv(c2{1.0, 0.0}, pv);
-    c2 *row = chi + pt * N2 * N2 + myk * N2;
+    c2 *row = chi + (base + pt) * N2 * N2 + myk * N2;
 #pragma unroll
     for (int c = 0; c < N; ++c)
 #pragma unroll
       for (int d = 0; d < N; ++d) row[c * N + d] = cmul(X[d * N + c], inv);
+    if constexpr (MIRROR) {
+      // this lane's row (a, b) of M_chi(nu) is column (a, b) of M_chi(-nu):
+      // chi_dcba(-nu) = conj(chi_abcd(nu)) for every (c, d)
+      const long m = pt / nk;
+      if (m > 0) {
+        const int a = myk / N, b = myk - (myk / N) * N;
+        c2 *mp = chi + (base + pt - 2 * m * nk) * N2 * N2;
+#pragma unroll
+        for (int c = 0; c < N; ++c)
+#pragma unroll
+          for (int d = 0; d < N; ++d) mp[((d * N + c) * N + b) * N + a] = cconj(cmul(X[d * N + c], inv));
+      }
+    }
   }
+}
+
+hipError_t launch_rpa_mirror(hipStream_t s, long nb, long nk, int n, const c2 *chi0, const c2 *U, c2 *chi,
+                             Status st) {
+  const long npts = nb * nk, base = (nb - 1) * nk;
+  switch (n) {
+#define TPRF_RPA_MCASE(N)                                                                            \
+  case N: {                                                                                         \
+    constexpr int P = 64 / (N * N);                                                                 \
+    const long nblk = (npts + 4 * P - 1) / (4 * P);                                                 \
+    hipLaunchKernelGGL((k_rpa<N, true>), dim3(nblk), dim3(256), 0, s, npts, chi0, U, chi, st.d, base, nk); \
+    break;                                                                                          \
+  }
+    TPRF_RPA_MCASE(1)
+    TPRF_RPA_MCASE(2)
+    TPRF_RPA_MCASE(3)
+    TPRF_RPA_MCASE(4)
+#undef TPRF_RPA_MCASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_rpa(hipStream_t s, long npts, int n, const c2 *chi0, const c2 *U, c2 *chi, Status st) {
