@@ -91,9 +91,10 @@ def cpu_baseline(wl, chain_result):
     kappa = s[:, 0] / s[:, -1]
     good = kappa <= 100
     n4 = n ** 4
-    got_chi0, got_chi = chain_result
+    got_chi0, got_chi, lo = chain_result
     parity = {
-        "chi0_wk_rel": rel(got_chi0, chi0_wk),
+        "chi0_wk_rel": rel(got_chi0[lo:], chi0_wk[lo:]),
+        "chi0_wk_frequencies": "nu >= 0 (the symmetric chain forms nu < 0 of chi_RPA only)" if lo else "all",
         "chi_rpa_rel_all": rel(got_chi, chi),
         "chi_rpa_rel_cond_le_100": rel(got_chi.reshape(-1, n4)[good], chi.reshape(-1, n4)[good]),
         "rpa_min_sigma": float(s[:, -1].min()),
@@ -245,6 +246,27 @@ def uneven_leg(reps=3):
     return out
 
 
+def per_function_leg(wl, steps, warmup):
+    """The same eval as one call per reference function (no fusion, no
+    symmetry shortcuts): the drop-in path a caller of the six functions gets."""
+    import torch
+    from tprf_amd import chain
+    ch = chain.BubbleRPA(wl, device="cuda:0", fused=False)
+    for _ in range(warmup):
+        ch.run()
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(steps):
+        ch.run()
+    t1.record()
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / steps
+    del ch
+    return {"stages": list(chain.BubbleRPA.API_STAGES), "ms_per_step": ms, "evals_per_s": 1e3 / ms}
+
+
 def calibration_run():
     """Two chain evals and one plain device copy of exactly 1 GiB (the reference
     figure for the FETCH_SIZE / WRITE_SIZE counters)."""
@@ -370,12 +392,16 @@ def main():
     }
     if not args.no_cpu:
         torch.cuda.synchronize()
-        got = (ch.chi0_wk.cpu().numpy(), ch.chi_wk.cpu().numpy())
+        # the symmetric chain stores chi0(nu, q) at nu >= 0 only (rows nb - 1 ..)
+        lo = wl.nb - 1 if ch.herm_nu else 0
+        got = (ch.chi0_wk.cpu().numpy(), ch.chi_wk.cpu().numpy(), lo)
         base, parity = cpu_baseline(wl, got)
         out["cpu_baseline"] = base
         out["parity_vs_oracle"] = parity
     if not args.no_extra:
         del ch
+        torch.cuda.empty_cache()
+        out["per_function_chain"] = per_function_leg(wl, args.steps, args.warmup)
         torch.cuda.empty_cache()
         out["grad_fwd"] = grad_leg()
         torch.cuda.empty_cache()

@@ -123,6 +123,17 @@ int tprf_fourier_wr_to_wk(tprf_ctx *ctx, long nw, const int dims[3], long ntarge
  * into primes <= 13 and be <= 6144.                                          */
 int tprf_fourier_wr_to_tr(tprf_ctx *ctx, int statistic, double beta, int n_max, int ntau, long ncol,
                           const tprf_c128 *in, tprf_c128 *out, const tprf_c128 *moments);
+/* fourier_wr_to_tr (fermions, fitted tail) of a lattice G(i w, r)[a][b] with
+ * G_ab(i w_n, r)^* = G_ba(i w_{-n-1}, -r) (the caller's promise; it holds for a
+ * Hermitian e_k with no self-energy): in[2 n_max][nk][n][n] ->
+ * out[ntau][nk][n][n].  No counterpart in the reference: only the columns
+ * (r, a, b) with r < -r, or r = -r and a <= b, are transformed, and
+ * G_ba(tau, -r) is written as the conjugate of G_ab(tau, r), which is what the
+ * full transform gives for such a G, to rounding.  TPRF_ERR_UNSUPPORTED where
+ * fourier_wr_to_tr would not use its input-pruned kernel (ntau - 1 a multiple
+ * of the smallest power of two M >= 2 n_max, 64 <= M <= 1024).            */
+int tprf_fourier_wr_to_tr_herm(tprf_ctx *ctx, double beta, int n_max, int ntau, const int dims[3], int norb,
+                               const tprf_c128 *in, tprf_c128 *out);
 
 /* ---- a5: imaginary time -> Matsubara ----------------------------------- */
 /* chi_wr_from_chi_tr(chi_tr, nw), chi_imtime.hpp:87 / chi_imtime.cpp:279-282 ->

@@ -423,6 +423,37 @@ def test_chi0_wr_herm(L, oracle, dims, n, nb):
             _lib.call("tprf_ctx_set_option", c, 2, -1)
 
 
+@pytest.mark.parametrize("dims,n", [((4, 3, 2), 2), ((5, 5, 1), 3), ((4, 4, 4), 1), ((3, 2, 2), 3)])
+def test_wr_to_tr_herm(L, oracle, dims, n):
+    """tprf_fourier_wr_to_tr_herm on G(i w, r) from a Hermitian e_k (odd and
+    even lattice sides, so self-mirrored r with a < b pairs), both workgroup
+    widths (option 3), against the oracle's full transform; the tail-fit
+    residual is reported as for the plain call."""
+    import torch
+    from tprf_amd import _lib
+    rng = np.random.default_rng(7 * n + sum(dims))
+    nk = int(np.prod(dims))
+    e = hermitian_ek(rng, nk, n, scale=0.8)
+    beta, n_iw = 12.0, 128
+    g_wr = oracle.fourier_lattice(oracle.lattice_dyson_g_wk(beta, n_iw, -0.2, e), dims, -1)
+    ref = oracle.fourier_w_to_t(g_wr, beta)
+    ntau = ref.shape[0]
+    c = _lib.ctx()
+    x = gpu(g_wr)
+    for variant in (0, 1, -1):
+        _lib.call("tprf_ctx_set_option", c, 3, variant)
+        try:
+            out = torch.full((ntau, nk, n, n), float("nan"), dtype=torch.complex128, device="cuda")
+            _lib.tail_status_reset(c)
+            _lib.call("tprf_fourier_wr_to_tr_herm", c, beta, n_iw, ntau, _lib.dims3(dims), n, _lib.ptr(x),
+                      _lib.ptr(out))
+            err, _ = _lib.tail_status(c)
+            assert err < 1e-6
+            assert relerr(host(out), ref) <= TOL, variant
+        finally:
+            _lib.call("tprf_ctx_set_option", c, 3, -1)
+
+
 # ---------------------------------------------------------------- chains
 def oracle_chain(O, wl):
     g_wk = O.lattice_dyson_g_wk(wl.beta, wl.n_iw, wl.mu, wl.e_k)

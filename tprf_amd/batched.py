@@ -67,7 +67,7 @@ class BatchedChain:
                 with torch.cuda.stream(st):
                     _lib.ctx(self.dev.index)  # library calls go to this lane's stream
                     for i in lane:
-                        for s in BubbleRPA.STAGES:
+                        for s in self.chains[i].STAGES:
                             self.chains[i].stage(s)
                 if j > 0:
                     ev = torch.cuda.Event()

@@ -18,9 +18,11 @@ from . import _lib
 class BubbleRPA:
     """fused=True (default): the first two stages run as one call,
     tprf_lattice_dyson_g_wr (G(iw, k) is then not stored for a 16^3 lattice),
-    and, when e_k is Hermitian (so G_ab(tau, r)^* = G_ba(tau, -r)), the bubble
-    runs as tprf_chi0_wr_from_grt_PH_herm (orbital pairs a <= b transformed,
-    the others formed from the symmetry); if U_abcd = conj(U_dcba) as well,
+    and, when e_k is Hermitian (so G_ab(tau, r)^* = G_ba(tau, -r)), omega -> tau
+    runs as tprf_fourier_wr_to_tr_herm (columns (r, a, b) with r < -r, or r = -r
+    and a <= b, transformed, their partners (-r, b, a) written as conjugates)
+    and the bubble as tprf_chi0_wr_from_grt_PH_herm (orbital pairs a <= b
+    transformed, the others formed from the symmetry); if U_abcd = conj(U_dcba) as well,
     M_chi0(-nu, q) = M_chi0(nu, q)^H and M_chi(-nu, q) = M_chi(nu, q)^H, so
     r -> q transforms the frequencies nu >= 0 only and
     tprf_solve_rpa_PH_herm solves those and writes chi(-nu) as the conjugate
@@ -64,8 +66,12 @@ class BubbleRPA:
         elif name == "wk_to_wr":
             _lib.call("tprf_fourier_wk_to_wr", c, 2 * wl.n_iw, self.dims, n * n, p(self.g_wk), p(self.g_wr))
         elif name == "wr_to_tr":
-            _lib.call("tprf_fourier_wr_to_tr", c, _lib.FERMION, wl.beta, wl.n_iw, wl.ntau, nk * n * n, p(self.g_wr),
-                      p(self.g_tr), None)
+            if self.herm:
+                _lib.call("tprf_fourier_wr_to_tr_herm", c, wl.beta, wl.n_iw, wl.ntau, self.dims, n, p(self.g_wr),
+                          p(self.g_tr))
+            else:
+                _lib.call("tprf_fourier_wr_to_tr", c, _lib.FERMION, wl.beta, wl.n_iw, wl.ntau, nk * n * n,
+                          p(self.g_wr), p(self.g_tr), None)
         elif name == "chi0_wr":
             fn = "tprf_chi0_wr_from_grt_PH_herm" if self.herm else "tprf_chi0_wr_from_grt_PH"
             _lib.call(fn, c, wl.beta, wl.ntau, self.dims, n, wl.nb, p(self.g_tr), p(self.chi0_wr))

@@ -47,6 +47,11 @@ struct BubbleEntry {
   int *rcanon = nullptr;
 };
 
+struct SymEntry {
+  int *col = nullptr, *mcol = nullptr;
+  long n = 0;
+};
+
 struct tprf_ctx {
   int device = 0;
   bool generic_kernels = false;
@@ -71,6 +76,7 @@ struct tprf_ctx {
   int lattice_variant = -1;
   std::map<std::tuple<long, long, int, int, int, int>, int> lattice_tuned;
   std::map<std::tuple<int, int>, int> dyson_lattice_tuned;
+  std::map<std::tuple<int, int, int, int>, SymEntry> symcols;  // canonical columns per (dims, n)
 };
 
 namespace {
@@ -657,6 +663,7 @@ int tprf_ctx_destroy(tprf_ctx *ctx) {
   for (auto &kv : ctx->static_w) (void)hipFree(kv.second);
   for (auto &kv : ctx->dtail) (void)hipFree(kv.second);
   for (auto &kv : ctx->otab) (void)hipFree(kv.second);
+  for (auto &kv : ctx->symcols) { (void)hipFree(kv.second.col); (void)hipFree(kv.second.mcol); }
   if (ctx->work) (void)hipFree(ctx->work);
   for (auto &kv : ctx->bubbles) {
     (void)hipFree(kv.second.tw1); (void)hipFree(kv.second.twp); (void)hipFree(kv.second.hmod);
@@ -765,8 +772,43 @@ int tprf_fourier_wr_to_wk(tprf_ctx *ctx, long nw, const int dims[3], long ntarge
   return lattice_fft(ctx, nw, dims, ntarget, (const c2 *)in, (c2 *)out, +1, false);
 }
 
-int tprf_fourier_wr_to_tr(tprf_ctx *ctx, int statistic, double beta, int n_max, int ntau, long ncol,
-                          const tprf_c128 *in, tprf_c128 *out, const tprf_c128 *moments) {
+namespace {
+
+// Canonical columns (r, a, b) of G(., r)[a][b] under (r, a, b) <-> (-r, b, a):
+// r < -r with every (a, b), and r = -r with a <= b; the partner of each
+// (-1 when it is the column itself).
+int get_symcols(tprf_ctx *ctx, const int dims[3], int n, tprf::SymCols *out) {
+  const auto key = std::make_tuple(dims[0], dims[1], dims[2], n);
+  auto it = ctx->symcols.find(key);
+  if (it == ctx->symcols.end()) {
+    const long nk = (long)dims[0] * dims[1] * dims[2], n2 = (long)n * n;
+    if (nk * n2 > 0x7fffffffL) return fail(ctx, TPRF_ERR_UNSUPPORTED, "symmetric transform: too many columns");
+    std::vector<int> col, mcol;
+    for (long r = 0; r < nk; ++r) {
+      const long i2 = r % dims[2], i1 = (r / dims[2]) % dims[1], i0 = r / ((long)dims[1] * dims[2]);
+      const long mr = (((dims[0] - i0) % dims[0]) * dims[1] + (dims[1] - i1) % dims[1]) * dims[2] +
+                      (dims[2] - i2) % dims[2];
+      if (mr < r) continue;
+      for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b) {
+          if (mr == r && b < a) continue;
+          col.push_back((int)(r * n2 + a * n + b));
+          mcol.push_back((mr == r && a == b) ? -1 : (int)(mr * n2 + b * n + a));
+        }
+    }
+    SymEntry e;
+    int st = upload(ctx, &e.col, col);
+    if (st) return st;
+    if ((st = upload(ctx, &e.mcol, mcol))) return st;
+    e.n = (long)col.size();
+    it = ctx->symcols.emplace(key, e).first;
+  }
+  *out = tprf::SymCols{it->second.col, it->second.mcol, it->second.n};
+  return TPRF_OK;
+}
+
+int wr_to_tr(tprf_ctx *ctx, int statistic, double beta, int n_max, int ntau, long ncol, const tprf_c128 *in,
+             tprf_c128 *out, const tprf_c128 *moments, const tprf::SymCols *sym) {
   if (!ctx) return TPRF_ERR_ARG;
   if ((statistic != TPRF_FERMION && statistic != TPRF_BOSON) || !(beta > 0) || n_max < 1 || ncol < 1 || !in || !out)
     return fail(ctx, TPRF_ERR_ARG, "fourier_wr_to_tr: bad argument");
@@ -797,6 +839,15 @@ int tprf_fourier_wr_to_tr(tprf_ctx *ctx, int statistic, double beta, int n_max, 
     const c2 *otab;
     if ((st = get_otab(ctx, statistic, beta, n_max, L, &otab))) return st;
     tprf::W2tDev wd{M, L / M, pL->tw, pM->tw, dtail, otab};
+    if (sym) {
+      auto launch = [&](int v) {
+        return tprf::launch_w2t_pruned_sym(ctx->stream, beta, n_max, ncol, (const c2 *)in, (c2 *)out, fit, *mt, wd,
+                                           tprf::Status{ctx->d_status}, v, *sym);
+      };
+      const auto key = std::make_tuple(statistic, M, L, 2, ncol);
+      return tuned_launch(ctx, ctx->w2t_tuned, key, tprf::w2t_pruned_variants(), ctx->w2t_variant, launch,
+                          "w2t pruned kernel (symmetric)");
+    }
     auto launch = [&](int v) {
       return tprf::launch_w2t_pruned(ctx->stream, statistic, beta, n_max, ncol, (const c2 *)in, (c2 *)out,
                                      (const c2 *)moments, fit, *mt, wd, tprf::Status{ctx->d_status}, v);
@@ -805,6 +856,7 @@ int tprf_fourier_wr_to_tr(tprf_ctx *ctx, int statistic, double beta, int n_max, 
     return tuned_launch(ctx, ctx->w2t_tuned, key, tprf::w2t_pruned_variants(), ctx->w2t_variant, launch,
                         "w2t pruned kernel");
   }
+  if (sym) return fail(ctx, TPRF_ERR_UNSUPPORTED, "fourier_wr_to_tr_herm: no pruned kernel for this mesh");
   const int ld = (L + 1) | 1;
   if (2L * ld * (long)sizeof(c2) > tprf::kMatsLdsBytes)
     return fail(ctx, TPRF_ERR_UNSUPPORTED, "fourier_wr_to_tr: ntau too large for one workgroup");
@@ -814,6 +866,24 @@ int tprf_fourier_wr_to_tr(tprf_ctx *ctx, int statistic, double beta, int n_max, 
                       tprf::launch_w2t(ctx->stream, statistic, beta, n_max, ncol, (const c2 *)in, (c2 *)out,
                                        (const c2 *)moments, fit, *mt, *p, tprf::Status{ctx->d_status}),
                       "w2t kernel");
+}
+
+}  // namespace
+
+int tprf_fourier_wr_to_tr(tprf_ctx *ctx, int statistic, double beta, int n_max, int ntau, long ncol,
+                          const tprf_c128 *in, tprf_c128 *out, const tprf_c128 *moments) {
+  return wr_to_tr(ctx, statistic, beta, n_max, ntau, ncol, in, out, moments, nullptr);
+}
+
+int tprf_fourier_wr_to_tr_herm(tprf_ctx *ctx, double beta, int n_max, int ntau, const int dims[3], int norb,
+                               const tprf_c128 *in, tprf_c128 *out) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (!dims_ok(dims) || norb < 1) return fail(ctx, TPRF_ERR_ARG, "fourier_wr_to_tr_herm: bad argument");
+  tprf::SymCols sym;
+  int st = get_symcols(ctx, dims, norb, &sym);
+  if (st) return st;
+  const long ncol = (long)dims[0] * dims[1] * dims[2] * norb * norb;
+  return wr_to_tr(ctx, TPRF_FERMION, beta, n_max, ntau, ncol, in, out, nullptr, &sym);
 }
 
 int tprf_fourier_tr_to_wr(tprf_ctx *ctx, int statistic, double beta, int ntau, int nw, long ncol,

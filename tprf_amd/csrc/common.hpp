@@ -102,6 +102,14 @@ struct W2tDev {
   const c2 *otab;        // [L][3] per-tau output records [phase, (D_0, D_1), (D_2, 0)] (forward kernel)
 };
 
+// Canonical columns of a symmetric transform: col[i] (increasing) are
+// transformed, and mcol[i] (-1: none) receives the conjugate of col[i]'s result.
+struct SymCols {
+  const int *col;
+  const int *mcol;
+  long n;
+};
+
 // Status words written by kernels (device memory, 4 doubles):
 //   [0] max RMS tail-fit residual, [1] max |m0|, [2] singular-matrix flag.
 struct Status {
@@ -157,6 +165,11 @@ hipError_t launch_w2t_pruned(hipStream_t s, int statistic, double beta, int n_ma
                              Status st, int variant);
 bool w2t_pruned_supported(int M);
 int w2t_pruned_variants();
+// the same on the canonical columns sym.col only (fermions, ncol the row
+// length), writing the conjugate of each result to sym.mcol as well
+hipError_t launch_w2t_pruned_sym(hipStream_t s, double beta, int n_max, long ncol, const c2 *in, c2 *out,
+                                 const TailFitDev &fit, const MatsDev &mt, const W2tDev &wd, Status st, int variant,
+                                 const SymCols &sym);
 // w2t.hip: transpose of the pruned w -> tau map (same M specialisations)
 hipError_t launch_w2t_adjoint(hipStream_t s, int statistic, double beta, int n_max, long ncol, const c2 *gbar_t,
                               c2 *gbar_w, int fitted, const TailFitDev &fit, const MatsDev &mt, const W2tDev &wd);

@@ -48,13 +48,18 @@ constexpr int kFitSplit = 4;  // partial sums per fitted coefficient
 // Fermionic M = 256, 512 with 256 threads: registers held to the three
 // workgroups per CU that the LDS allows (they fit without spilling); two
 // waves per SIMD otherwise.
-template <int STAT, int M, int NT>
+// SYM: only the columns sym.col[] are transformed (tiles over that list), and
+// each result is also written, conjugated, to column sym.mcol[] (when >= 0):
+// for G_ab(i w_n, r)^* = G_ba(i w_{-n-1}, -r) the transform of column
+// (-r, b, a) is the conjugate of that of (r, a, b).
+template <int STAT, int M, int NT, bool SYM = false>
 __global__ void __launch_bounds__(NT)
     __attribute__((amdgpu_waves_per_eu((STAT == 0 && (M == 256 || M == 512) && NT == 256) ? 3 : 2)))
     k_w2t_pruned(double beta, int n_max, long ncol,
                                                              const c2 *__restrict__ in, c2 *__restrict__ out,
                                                              const c2 *__restrict__ moments, TailFitDev fit,
-                                                             MatsDev mt, W2tDev wd, double *status) {
+                                                             MatsDev mt, W2tDev wd, double *status,
+                                                             SymCols sym = SymCols{nullptr, nullptr, 0}) {
   constexpr int TPC = M / 8;  // threads per column
   constexpr int C = NT / TPC;  // columns per workgroup
   // C >= 3: the output records of a residue (3 M entries) fit in buf
@@ -73,8 +78,11 @@ __global__ void __launch_bounds__(NT)
   const int c = tid % C, j = tid / C;
   const long tile = xcd_remap(blockIdx.x, gridDim.x);
   const long c0 = tile * C;
-  const int cn = (ncol - c0) < C ? (int)(ncol - c0) : C;
-  const long col = c0 + (c < cn ? c : 0);
+  const long ntot = SYM ? sym.n : ncol;
+  const int cn = (ntot - c0) < C ? (int)(ntot - c0) : C;
+  const long cidx = c0 + (c < cn ? c : 0);
+  const long col = SYM ? (long)sym.col[cidx] : cidx;
+  const long mcol = SYM ? (long)sym.mcol[cidx] : -1;
   const int L = mt.L, Q = wd.Q;
   const int nmesh = (STAT == 0) ? 2 * n_max : 2 * n_max - 1;
   const int first = (STAT == 0) ? -n_max : -(n_max - 1);
@@ -280,6 +288,10 @@ __global__ void __launch_bounds__(NT)
             y.y += a0.y * d01.x + a1.y * d01.y + a2.y * d2.x;
             out[(long)jt * ncol + col] = y;
             if (jt == 0) out[(long)L * ncol + col] = cscale(cadd(y, s_m1[c]), pm);
+            if (SYM && mcol >= 0) {
+              out[(long)jt * ncol + mcol] = cconj(y);
+              if (jt == 0) out[(long)L * ncol + mcol] = cconj(cscale(cadd(y, s_m1[c]), pm));
+            }
           }
         }
         __syncthreads();  // the next residue's first exchange overwrites buf
@@ -299,6 +311,10 @@ __global__ void __launch_bounds__(NT)
           y.y += a0.y * p0 + a1.y * p1 + a2.y * p2;
           out[(long)jt * ncol + col] = y;
           if (jt == 0) out[(long)L * ncol + col] = cscale(cadd(y, s_m1[c]), pm);
+          if (SYM && mcol >= 0) {
+            out[(long)jt * ncol + mcol] = cconj(y);
+            if (jt == 0) out[(long)L * ncol + mcol] = cconj(cscale(cadd(y, s_m1[c]), pm));
+          }
         }
       }
     }
@@ -332,6 +348,38 @@ static hipError_t launch_stat(hipStream_t s, double beta, int n_max, long ncol, 
 }
 
 int w2t_pruned_variants() { return 2; }
+
+template <int M, int NT>
+static hipError_t launch_sym_m(hipStream_t s, double beta, int n_max, long ncol, const c2 *in, c2 *out,
+                               const TailFitDev &fit, const MatsDev &mt, const W2tDev &wd, Status st,
+                               const SymCols &sym) {
+  constexpr int C = NT / (M / 8);
+  const long nb = (sym.n + C - 1) / C;
+  hipLaunchKernelGGL((k_w2t_pruned<0, M, NT, true>), dim3(nb), dim3(NT), 0, s, beta, n_max, ncol, in, out,
+                     (const c2 *)nullptr, fit, mt, wd, st.d, sym);
+  return hipGetLastError();
+}
+
+template <int NT>
+static hipError_t launch_sym_nt(hipStream_t s, double beta, int n_max, long ncol, const c2 *in, c2 *out,
+                                const TailFitDev &fit, const MatsDev &mt, const W2tDev &wd, Status st,
+                                const SymCols &sym) {
+  switch (wd.M) {
+    case 64: return launch_sym_m<64, NT>(s, beta, n_max, ncol, in, out, fit, mt, wd, st, sym);
+    case 128: return launch_sym_m<128, NT>(s, beta, n_max, ncol, in, out, fit, mt, wd, st, sym);
+    case 256: return launch_sym_m<256, NT>(s, beta, n_max, ncol, in, out, fit, mt, wd, st, sym);
+    case 512: return launch_sym_m<512, NT>(s, beta, n_max, ncol, in, out, fit, mt, wd, st, sym);
+    case 1024: return launch_sym_m<1024, NT>(s, beta, n_max, ncol, in, out, fit, mt, wd, st, sym);
+    default: return hipErrorNotSupported;
+  }
+}
+
+hipError_t launch_w2t_pruned_sym(hipStream_t s, double beta, int n_max, long ncol, const c2 *in, c2 *out,
+                                 const TailFitDev &fit, const MatsDev &mt, const W2tDev &wd, Status st, int variant,
+                                 const SymCols &sym) {
+  if (variant == 1) return launch_sym_nt<512>(s, beta, n_max, ncol, in, out, fit, mt, wd, st, sym);
+  return launch_sym_nt<256>(s, beta, n_max, ncol, in, out, fit, mt, wd, st, sym);
+}
 
 // variant 0: 256 threads per workgroup; 1: 512 threads (twice the columns per
 // workgroup, twice the contiguous bytes per written row)
