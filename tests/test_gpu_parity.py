@@ -89,17 +89,18 @@ def test_dyson_g_wr(L, oracle, dims, n, beta, n_iw):
     # the fused 16^3 kernel has two workgroup shapes (option 4 selects one, -1
     # restores tuning)
     variants = (0, 1, -1) if tuple(dims) == (16, 16, 16) and n <= 3 else (-1,)
-    for variant in variants:
-        _lib.call("tprf_ctx_set_option", ctx, 4, variant)
-        try:
-            out = torch.empty((2 * n_iw, nk, n, n), dtype=torch.complex128, device="cuda")
-            _lib.tail_status_reset(ctx)
-            _lib.call("tprf_lattice_dyson_g_wr", ctx, beta, n_iw, mu, _lib.dims3(dims), n, _lib.ptr(e_d),
-                      _lib.ptr(out))
-            _lib.tail_status(ctx)  # raises on a singular inversion
-            assert relerr(host(out), ref) <= 1e-12, variant
-        finally:
-            _lib.call("tprf_ctx_set_option", ctx, 4, -1)
+    # the Hermitian entry forms w_n >= 0 and writes the rest from the symmetry
+    for fn in ("tprf_lattice_dyson_g_wr", "tprf_lattice_dyson_g_wr_herm"):
+        for variant in variants:
+            _lib.call("tprf_ctx_set_option", ctx, 4, variant)
+            try:
+                out = torch.full((2 * n_iw, nk, n, n), float("nan"), dtype=torch.complex128, device="cuda")
+                _lib.tail_status_reset(ctx)
+                _lib.call(fn, ctx, beta, n_iw, mu, _lib.dims3(dims), n, _lib.ptr(e_d), _lib.ptr(out))
+                _lib.tail_status(ctx)  # raises on a singular inversion
+                assert relerr(host(out), ref) <= 1e-12, (fn, variant)
+            finally:
+                _lib.call("tprf_ctx_set_option", ctx, 4, -1)
 
 
 # ---------------------------------------------------------------- a2/a6 lattice FFT

@@ -736,8 +736,10 @@ int tprf_lattice_dyson_g_wk(tprf_ctx *ctx, double beta, int n_iw, double mu, lon
                       "dyson kernel");
 }
 
-int tprf_lattice_dyson_g_wr(tprf_ctx *ctx, double beta, int n_iw, double mu, const int dims[3], int norb,
-                            const tprf_c128 *e_k, tprf_c128 *g_wr) {
+namespace {
+
+int dyson_g_wr(tprf_ctx *ctx, double beta, int n_iw, double mu, const int dims[3], int norb, const tprf_c128 *e_k,
+               tprf_c128 *g_wr, bool sym) {
   if (!ctx) return TPRF_ERR_ARG;
   if (!(beta > 0) || n_iw < 1 || !dims_ok(dims) || !e_k || !g_wr)
     return fail(ctx, TPRF_ERR_ARG, "lattice_dyson_g_wr: bad argument");
@@ -746,12 +748,12 @@ int tprf_lattice_dyson_g_wr(tprf_ctx *ctx, double beta, int n_iw, double mu, con
   if (!ctx->generic_kernels && dims[0] == 16 && dims[1] == 16 && dims[2] == 16 && norb <= 3) {
     auto launch = [&](int v) {
       return tprf::launch_dyson_lattice16r(ctx->stream, beta, n_iw, mu, norb, (const c2 *)e_k, (c2 *)g_wr,
-                                           tprf::Status{ctx->d_status}, v);
+                                           tprf::Status{ctx->d_status}, v, sym);
     };
     const int nvar = tprf::dyson_lattice16r_variants(norb);
     const int forced = ctx->lattice_variant >= 0 ? ctx->lattice_variant % nvar : -1;
-    return tuned_launch(ctx, ctx->dyson_lattice_tuned, std::make_tuple(n_iw, norb), nvar, forced, launch,
-                        "fused dyson lattice kernel");
+    return tuned_launch(ctx, ctx->dyson_lattice_tuned, std::make_tuple(n_iw, norb * (sym ? -1 : 1)), nvar, forced,
+                        launch, "fused dyson lattice kernel");
   }
   // other shapes: Dyson into g_wr, then the lattice transform in place
   int st = check_launch(ctx,
@@ -760,6 +762,18 @@ int tprf_lattice_dyson_g_wr(tprf_ctx *ctx, double beta, int n_iw, double mu, con
                         "dyson kernel");
   if (st) return st;
   return lattice_fft(ctx, 2L * n_iw, dims, (long)norb * norb, (const c2 *)g_wr, (c2 *)g_wr, -1, true);
+}
+
+}  // namespace
+
+int tprf_lattice_dyson_g_wr(tprf_ctx *ctx, double beta, int n_iw, double mu, const int dims[3], int norb,
+                            const tprf_c128 *e_k, tprf_c128 *g_wr) {
+  return dyson_g_wr(ctx, beta, n_iw, mu, dims, norb, e_k, g_wr, false);
+}
+
+int tprf_lattice_dyson_g_wr_herm(tprf_ctx *ctx, double beta, int n_iw, double mu, const int dims[3], int norb,
+                                 const tprf_c128 *e_k, tprf_c128 *g_wr) {
+  return dyson_g_wr(ctx, beta, n_iw, mu, dims, norb, e_k, g_wr, true);
 }
 
 int tprf_fourier_wk_to_wr(tprf_ctx *ctx, long nw, const int dims[3], long ntarget, const tprf_c128 *in,

@@ -940,7 +940,10 @@ __device__ __forceinline__ c2 inv00(const v2d (&y)[N][N], bool &singular) {
 // workgroup per (w, a), its n columns (a, b) adjacent (n * 16-byte row
 // pieces); C = 1: one workgroup per (w, a, b) (smaller workgroups, more of them
 // resident per CU).
-template <int N, int C>
+// SYM (e_k Hermitian): only the frequencies w_n >= 0 (index w >= n_iw) are
+// formed and transformed; G(i w_{-n-1}, k) = G(i w_n, k)^H gives
+// G_ba(i w_{-n-1}, -r) = conj(G_ab(i w_n, r)), written alongside.
+template <int N, int C, bool SYM = false>
 __global__ void __launch_bounds__(256 * C) k_dyson_lattice16r(double beta, int n_iw, double mu,
                                                               const c2 *__restrict__ ek, c2 *__restrict__ out,
                                                               double scale, double *status) {
@@ -949,8 +952,8 @@ __global__ void __launch_bounds__(256 * C) k_dyson_lattice16r(double beta, int n
   constexpr int NSM = 2048 + 2048 / 16;
   __shared__ c2 sm[NSM * C];
   const long tile = xcd_remap(blockIdx.x, gridDim.x);
-  const long w = tile / NG;
-  const int g = (int)(tile - w * NG);
+  const long w = tile / NG + (SYM ? n_iw : 0);
+  const int g = (int)(tile - (tile / NG) * NG);
   const int a = (C == N) ? g : g / N;
   const int t = threadIdx.x, cc = t % C, p = t / C, hi = p >> 4, lo = p & 15;
   const int b = (C == N) ? cc : g % N;  // this thread's column (a, b)
@@ -999,13 +1002,26 @@ __global__ void __launch_bounds__(256 * C) k_dyson_lattice16r(double beta, int n
   c2 *dst = out + w * 4096 * NT + a * N + b;
 #pragma unroll
   for (int k2 = 0; k2 < 16; ++k2) dst[(long)(p * 16 + k2) * NT] = cscale(v[k2], scale);
+  if constexpr (SYM) {
+    // r = (hi, lo, k2) -> -r = (-hi, -lo, -k2) mod 16, frequency index 2 n_iw - 1 - w
+    c2 *mdst = out + (2L * n_iw - 1 - w) * 4096 * NT + b * N + a;
+    const int mh = (16 - hi) & 15, ml = (16 - lo) & 15;
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2)
+      mdst[(long)((mh * 16 + ml) * 16 + ((16 - k2) & 15)) * NT] = cconj(cscale(v[k2], scale));
+  }
 }
 
 template <int N, int C>
-static hipError_t launch_dl16(hipStream_t s, double beta, int n_iw, double mu, const c2 *ek, c2 *out, Status st) {
-  const long nb = 2L * n_iw * (N * N / C);
-  hipLaunchKernelGGL((k_dyson_lattice16r<N, C>), dim3(nb), dim3(256 * C), 0, s, beta, n_iw, mu, ek, out, 1.0 / 4096.0,
-                     st.d);
+static hipError_t launch_dl16(hipStream_t s, double beta, int n_iw, double mu, const c2 *ek, c2 *out, Status st,
+                              bool sym) {
+  const long nb = (sym ? 1L : 2L) * n_iw * (N * N / C);
+  if (sym)
+    hipLaunchKernelGGL((k_dyson_lattice16r<N, C, true>), dim3(nb), dim3(256 * C), 0, s, beta, n_iw, mu, ek, out,
+                       1.0 / 4096.0, st.d);
+  else
+    hipLaunchKernelGGL((k_dyson_lattice16r<N, C, false>), dim3(nb), dim3(256 * C), 0, s, beta, n_iw, mu, ek, out,
+                       1.0 / 4096.0, st.d);
   return hipGetLastError();
 }
 
@@ -1013,13 +1029,13 @@ static hipError_t launch_dl16(hipStream_t s, double beta, int n_iw, double mu, c
 int dyson_lattice16r_variants(int n) { return n == 1 ? 1 : 2; }
 
 hipError_t launch_dyson_lattice16r(hipStream_t s, double beta, int n_iw, double mu, int n, const c2 *ek, c2 *out,
-                                   Status st, int variant) {
+                                   Status st, int variant, bool sym) {
   switch (n) {
-    case 1: return launch_dl16<1, 1>(s, beta, n_iw, mu, ek, out, st);
-    case 2: return variant == 1 ? launch_dl16<2, 1>(s, beta, n_iw, mu, ek, out, st)
-                                : launch_dl16<2, 2>(s, beta, n_iw, mu, ek, out, st);
-    case 3: return variant == 1 ? launch_dl16<3, 1>(s, beta, n_iw, mu, ek, out, st)
-                                : launch_dl16<3, 3>(s, beta, n_iw, mu, ek, out, st);
+    case 1: return launch_dl16<1, 1>(s, beta, n_iw, mu, ek, out, st, sym);
+    case 2: return variant == 1 ? launch_dl16<2, 1>(s, beta, n_iw, mu, ek, out, st, sym)
+                                : launch_dl16<2, 2>(s, beta, n_iw, mu, ek, out, st, sym);
+    case 3: return variant == 1 ? launch_dl16<3, 1>(s, beta, n_iw, mu, ek, out, st, sym)
+                                : launch_dl16<3, 3>(s, beta, n_iw, mu, ek, out, st, sym);
     default: return hipErrorInvalidValue;
   }
 }
