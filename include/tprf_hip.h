@@ -133,13 +133,15 @@ int tprf_fourier_wr_to_tr(tprf_ctx *ctx, int statistic, double beta, int n_max, 
  * G_ab(i w_n, r)^* = G_ba(i w_{-n-1}, -r) (the caller's promise; it holds for a
  * Hermitian e_k with no self-energy): in[2 n_max][nk][n][n] ->
  * out[ntau][nk][n][n].  No counterpart in the reference: only the columns
- * (r, a, b) with r < -r, or r = -r and a <= b, are transformed, and
+ * (r, a, b) with r < -r, or r = -r and a <= b, are transformed.  full = 1:
  * G_ba(tau, -r) is written as the conjugate of G_ab(tau, r), which is what the
- * full transform gives for such a G, to rounding.  TPRF_ERR_UNSUPPORTED where
+ * full transform gives for such a G, to rounding; full = 0: the columns of
+ * r > -r are not written (only r <= -r hold data; the Hermitian bubble
+ * tprf_chi0_wr_from_grt_PH_herm reads no others).  TPRF_ERR_UNSUPPORTED where
  * fourier_wr_to_tr would not use its input-pruned kernel (ntau - 1 a multiple
  * of the smallest power of two M >= 2 n_max, 64 <= M <= 1024).            */
 int tprf_fourier_wr_to_tr_herm(tprf_ctx *ctx, double beta, int n_max, int ntau, const int dims[3], int norb,
-                               const tprf_c128 *in, tprf_c128 *out);
+                               const tprf_c128 *in, tprf_c128 *out, int full);
 
 /* ---- a5: imaginary time -> Matsubara ----------------------------------- */
 /* chi_wr_from_chi_tr(chi_tr, nw), chi_imtime.hpp:87 / chi_imtime.cpp:279-282 ->
@@ -161,10 +163,11 @@ int tprf_chi0_wr_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dim
 /* The same for a G with G_ab(tau, r)^* = G_ba(tau, -r) at every mesh point (the
  * caller's promise): G from a Hermitian e_k with no self-energy, or with one
  * obeying Sigma(i w)^H = Sigma(-i w).  No counterpart in the reference: it
- * transforms only the orbital pairs a <= b and forms the columns of b > a from
- * the symmetry chi0_badc(nu, r) = conj of the tau-reversed product transform
- * of abcd; for such a G the result equals tprf_chi0_wr_from_grt_PH's to
- * rounding.  For a G without the symmetry the columns b > a are wrong.      */
+ * reads only the columns of r <= -r (G(tau, -r) = G(tau, r)^H), transforms
+ * only the orbital pairs a <= b and forms the columns of b > a from the
+ * symmetry chi0_badc(nu, r) = conj of the tau-reversed product transform of
+ * abcd; for such a G the result equals tprf_chi0_wr_from_grt_PH's to
+ * rounding.  For a G without the symmetry the result is wrong.              */
 int tprf_chi0_wr_from_grt_PH_herm(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb, int nw,
                                   const tprf_c128 *g_tr, tprf_c128 *chi0_wr);
 /* chi0_w0r_from_grt_PH(g_tr), chi_imtime.cpp:185-228 (trapezoid, nu = 0). out[1][nr][n^4] */

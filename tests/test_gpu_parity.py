@@ -394,13 +394,14 @@ def test_rpa(L, oracle, n, sparse):
     assert relerr(host(got.data), ref) <= 1e-12
 
 
-@pytest.mark.parametrize("dims,n,nb", [((4, 3, 2), 2, 20), ((6, 4, 1), 3, 16), ((4, 4, 2), 3, 40), ((3, 3, 3), 2, 50)])
+@pytest.mark.parametrize("dims,n,nb", [((4, 3, 2), 2, 20), ((6, 4, 1), 3, 16), ((4, 4, 2), 3, 40), ((3, 3, 3), 2, 50),
+                                       ((5, 4, 1), 1, 12)])
 def test_chi0_wr_herm(L, oracle, dims, n, nb):
     """tprf_chi0_wr_from_grt_PH_herm on a G(tau, r) from a Hermitian e_k (the
     oracle's Dyson -> k -> r -> w -> tau chain), every workgroup shape that has
     the Hermitian form (option 2), against the oracle's chi0_wr_from_grt_PH;
     lattices with odd and even sides (self-mirrored r) and inner DFT lengths
-    64 and 128."""
+    64 and 128; n = 1 takes the generic (unpaired) kernel."""
     import torch
     from tprf_amd import _lib
     rng = np.random.default_rng(100 * n + nb)
@@ -412,7 +413,10 @@ def test_chi0_wr_herm(L, oracle, dims, n, nb):
     ntau = g_tr.shape[0]
     ref = oracle.chi0_wr_from_grt_PH(g_tr, dims, beta, nb)
     c = _lib.ctx()
-    gt = gpu(g_tr)
+    # only the columns of r <= -r are read: the others are NaN here
+    g_half = g_tr.copy()
+    g_half[:, ~canonical_r(dims)] = np.nan
+    gt = gpu(g_half)
     for variant in (0, 1, 3, 4, -1):
         _lib.call("tprf_ctx_set_option", c, 2, variant)
         try:
@@ -441,18 +445,33 @@ def test_wr_to_tr_herm(L, oracle, dims, n):
     ntau = ref.shape[0]
     c = _lib.ctx()
     x = gpu(g_wr)
-    for variant in (0, 1, -1):
-        _lib.call("tprf_ctx_set_option", c, 3, variant)
-        try:
-            out = torch.full((ntau, nk, n, n), float("nan"), dtype=torch.complex128, device="cuda")
-            _lib.tail_status_reset(c)
-            _lib.call("tprf_fourier_wr_to_tr_herm", c, beta, n_iw, ntau, _lib.dims3(dims), n, _lib.ptr(x),
-                      _lib.ptr(out))
-            err, _ = _lib.tail_status(c)
-            assert err < 1e-6
-            assert relerr(host(out), ref) <= TOL, variant
-        finally:
-            _lib.call("tprf_ctx_set_option", c, 3, -1)
+    canon = canonical_r(dims)
+    for full in (1, 0):
+        for variant in (0, 1, -1):
+            _lib.call("tprf_ctx_set_option", c, 3, variant)
+            try:
+                out = torch.full((ntau, nk, n, n), float("nan"), dtype=torch.complex128, device="cuda")
+                _lib.tail_status_reset(c)
+                _lib.call("tprf_fourier_wr_to_tr_herm", c, beta, n_iw, ntau, _lib.dims3(dims), n, _lib.ptr(x),
+                          _lib.ptr(out), full)
+                err, _ = _lib.tail_status(c)
+                assert err < 1e-6
+                got = host(out)
+                if full:
+                    assert relerr(got, ref) <= TOL, variant
+                else:  # only r <= -r written
+                    assert relerr(got[:, canon], ref[:, canon]) <= TOL, variant
+                    assert np.isnan(got[:, ~canon]).all()
+            finally:
+                _lib.call("tprf_ctx_set_option", c, 3, -1)
+
+
+def canonical_r(dims):
+    """Boolean mask of the lattice points r <= -r."""
+    from tprf_amd.gf import MeshCycLat
+    lat = MeshCycLat(dims)
+    nk = int(np.prod(dims))
+    return np.array([r <= lat.minus(r) for r in range(nk)])
 
 
 # ---------------------------------------------------------------- chains
@@ -536,8 +555,10 @@ def test_chain_fused_equals_per_function(L):
     assert relerr(host(got.g_wr), host(ref.g_wr)) <= 1e-12
     # the tail fit of omega -> tau weighs the high frequencies by up to
     # w_max^3, so rounding-level differences of G(iw, r) grow by ~10 there
-    for name in ("g_tr", "chi0_wr"):
-        assert relerr(host(getattr(got, name)), host(getattr(ref, name))) <= TOL, name
+    # the default chain stores G(tau, r) for r <= -r only
+    canon = canonical_r(wl.dims)
+    assert relerr(host(got.g_tr)[:, canon], host(ref.g_tr)[:, canon]) <= TOL
+    assert relerr(host(got.chi0_wr), host(ref.chi0_wr)) <= TOL
     # with the symmetries the fused chain transforms nu >= 0 only
     nb = wl.nb
     assert relerr(host(got.chi0_wk)[nb - 1:], host(ref.chi0_wk)[nb - 1:]) <= TOL

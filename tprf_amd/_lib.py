@@ -46,7 +46,7 @@ _SIGNATURES = {
     "tprf_fourier_wk_to_wr": [_p, _l, _p, _l, _p, _p],
     "tprf_fourier_wr_to_wk": [_p, _l, _p, _l, _p, _p],
     "tprf_fourier_wr_to_tr": [_p, _i, _d, _i, _i, _l, _p, _p, _p],
-    "tprf_fourier_wr_to_tr_herm": [_p, _d, _i, _i, _p, _i, _p, _p],
+    "tprf_fourier_wr_to_tr_herm": [_p, _d, _i, _i, _p, _i, _p, _p, _i],
     "tprf_fourier_tr_to_wr": [_p, _i, _d, _i, _i, _l, _p, _p],
     "tprf_chi0_tr_from_grt_PH": [_p, _i, _p, _i, _p, _p],
     "tprf_chi0_wr_from_grt_PH": [_p, _d, _i, _p, _i, _i, _p, _p],

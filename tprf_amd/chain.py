@@ -20,7 +20,7 @@ class BubbleRPA:
     tprf_lattice_dyson_g_wr (G(iw, k) is then not stored for a 16^3 lattice),
     and, when e_k is Hermitian (so G_ab(tau, r)^* = G_ba(tau, -r)), omega -> tau
     runs as tprf_fourier_wr_to_tr_herm (columns (r, a, b) with r < -r, or r = -r
-    and a <= b, transformed, their partners (-r, b, a) written as conjugates)
+    and a <= b, transformed; G(tau, r) is then stored for r <= -r only)
     and the bubble as tprf_chi0_wr_from_grt_PH_herm (orbital pairs a <= b
     transformed, the others formed from the symmetry); if U_abcd = conj(U_dcba) as well,
     M_chi0(-nu, q) = M_chi0(nu, q)^H and M_chi(-nu, q) = M_chi(nu, q)^H, so
@@ -67,9 +67,9 @@ class BubbleRPA:
         elif name == "wk_to_wr":
             _lib.call("tprf_fourier_wk_to_wr", c, 2 * wl.n_iw, self.dims, n * n, p(self.g_wk), p(self.g_wr))
         elif name == "wr_to_tr":
-            if self.herm:
+            if self.herm:  # only r <= -r is written: the Hermitian bubble reads no other column
                 _lib.call("tprf_fourier_wr_to_tr_herm", c, wl.beta, wl.n_iw, wl.ntau, self.dims, n, p(self.g_wr),
-                          p(self.g_tr))
+                          p(self.g_tr), 0)
             else:
                 _lib.call("tprf_fourier_wr_to_tr", c, _lib.FERMION, wl.beta, wl.n_iw, wl.ntau, nk * n * n,
                           p(self.g_wr), p(self.g_tr), None)

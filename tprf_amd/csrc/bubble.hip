@@ -38,7 +38,8 @@ namespace tprf {
 // vmcnt before the barrier that publishes them.
 // WPE: waves per SIMD the register allocation must allow (1: no limit).
 // HERM (with AB): G satisfies G_ab(tau, r)^* = G_ba(tau, -r) (a Hermitian e_k
-// and no self-energy, or one with Sigma(i w)^H = Sigma(-i w)).  Then the
+// and no self-energy, or one with Sigma(i w)^H = Sigma(-i w)).  Only the
+// columns of r <= -r are read: the -r side is conj(G_cb(tau, r)).  The
 // products obey x^r_badc(tau_j) = x^r_abcd(tau_{L-j})^* and
 // x^{-r}_dcba(tau_j) = x^r_abcd(tau_j)^*, so only the orbital pairs a <= b are
 // transformed: a workgroup with a < b also writes the columns badc of r and
@@ -109,14 +110,18 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
       const int q = q0 + c;
       const int dd = q % N, cc = (q / N) % N, bb = (q / n2) % N, aa = q / (n2 * N);
       ga[w] = ld_v2d(gtr + ((long)j * nr + r) * n2 + dd * N + aa);
-      gb[w] = ld_v2d(gtr + ((long)(L - j) * nr + mr) * n2 + bb * N + cc);
+      // HERM: G_bc(tau, -r) = conj(G_cb(tau, r)), read from r (conjugated below)
+      gb[w] = HERM ? ld_v2d(gtr + ((long)(L - j) * nr + r) * n2 + cc * N + bb)
+                   : ld_v2d(gtr + ((long)(L - j) * nr + mr) * n2 + bb * N + cc);
     }
 #pragma unroll
     for (int w = 0; w < NTP; ++w) tp[w] = ld_v2d(bd.twp + min(tid + w * NT, P - 1));
 #pragma unroll
     for (int w = 0; w < NEB; ++w) {
       const int t = tid + w * NT;
-      if (t < cn * 18) sE[t / 18][t % 18] = cmul(c2{ga[w].x, ga[w].y}, c2{gb[w].x, gb[w].y});
+      if (t < cn * 18)
+        sE[t / 18][t % 18] = HERM ? cmulc(c2{ga[w].x, ga[w].y}, c2{gb[w].x, gb[w].y})
+                                  : cmul(c2{ga[w].x, ga[w].y}, c2{gb[w].x, gb[w].y});
     }
 #pragma unroll
     for (int w = 0; w < NTP; ++w)
@@ -141,6 +146,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
   };
   auto off_b = [&](int t) {
     const int j2 = t / NE, e = t - j2 * NE;
+    if constexpr (HERM) return (((long)L - (long)N1 * j2) * nr + r) * n2 + e * N + b0;  // conj(G_eb(r))
     return (((long)L - (long)N1 * j2) * nr + mr) * n2 + (AB ? b0 * N + e : e);
   };
 
@@ -160,7 +166,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
 #pragma unroll
       for (int k = 0; k < R1; ++k) {
         const int j2 = R2 * k + s;
-        v[k] = AB ? cmul(tA[j2 * NE + dd], tB[j2 * NE + cc])
+        v[k] = HERM ? cmulc(tA[j2 * NE + dd], tB[j2 * NE + cc])
+             : AB ? cmul(tA[j2 * NE + dd], tB[j2 * NE + cc])
                   : cmul(tA[j2 * n2 + dd * N + aa], tB[j2 * n2 + bb * N + cc]);
       }
       SmallDft<R1, 1>::run(v, nullptr, 0);

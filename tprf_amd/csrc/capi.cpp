@@ -790,9 +790,10 @@ namespace {
 
 // Canonical columns (r, a, b) of G(., r)[a][b] under (r, a, b) <-> (-r, b, a):
 // r < -r with every (a, b), and r = -r with a <= b; the partner of each
-// (-1 when it is the column itself).
-int get_symcols(tprf_ctx *ctx, const int dims[3], int n, tprf::SymCols *out) {
-  const auto key = std::make_tuple(dims[0], dims[1], dims[2], n);
+// (-1 when it is the column itself, and with full = false also for r < -r,
+// whose partners are then not written).
+int get_symcols(tprf_ctx *ctx, const int dims[3], int n, bool full, tprf::SymCols *out) {
+  const auto key = std::make_tuple(dims[0], dims[1], dims[2], full ? n : -n);
   auto it = ctx->symcols.find(key);
   if (it == ctx->symcols.end()) {
     const long nk = (long)dims[0] * dims[1] * dims[2], n2 = (long)n * n;
@@ -807,7 +808,7 @@ int get_symcols(tprf_ctx *ctx, const int dims[3], int n, tprf::SymCols *out) {
         for (int b = 0; b < n; ++b) {
           if (mr == r && b < a) continue;
           col.push_back((int)(r * n2 + a * n + b));
-          mcol.push_back((mr == r && a == b) ? -1 : (int)(mr * n2 + b * n + a));
+          mcol.push_back(((mr == r && a == b) || (mr != r && !full)) ? -1 : (int)(mr * n2 + b * n + a));
         }
     }
     SymEntry e;
@@ -890,11 +891,11 @@ int tprf_fourier_wr_to_tr(tprf_ctx *ctx, int statistic, double beta, int n_max, 
 }
 
 int tprf_fourier_wr_to_tr_herm(tprf_ctx *ctx, double beta, int n_max, int ntau, const int dims[3], int norb,
-                               const tprf_c128 *in, tprf_c128 *out) {
+                               const tprf_c128 *in, tprf_c128 *out, int full) {
   if (!ctx) return TPRF_ERR_ARG;
   if (!dims_ok(dims) || norb < 1) return fail(ctx, TPRF_ERR_ARG, "fourier_wr_to_tr_herm: bad argument");
   tprf::SymCols sym;
-  int st = get_symcols(ctx, dims, norb, &sym);
+  int st = get_symcols(ctx, dims, norb, full != 0, &sym);
   if (st) return st;
   const long ncol = (long)dims[0] * dims[1] * dims[2] * norb * norb;
   return wr_to_tr(ctx, TPRF_FERMION, beta, n_max, ntau, ncol, in, out, nullptr, &sym);
@@ -983,7 +984,7 @@ int chi0_wr_from_grt(tprf_ctx *ctx, double beta, int ntau, const int dims[3], in
   if ((st = get_mats(ctx, TPRF_BOSON, beta, L, &mt))) return st;
   return check_launch(ctx,
                       tprf::launch_chi0_wr(ctx->stream, beta, nw, dims, norb, (const c2 *)g_tr, (c2 *)chi0_wr, *mt,
-                                           *p, ctx->vinv),
+                                           *p, ctx->vinv, herm),
                       "chi0_wr kernel");
 }
 

@@ -23,6 +23,8 @@ __host__ __device__ __forceinline__ c2 cdiv(c2 a, c2 b) {
   double d = 1.0 / (b.x * b.x + b.y * b.y);
   return c2{(a.x * b.x + a.y * b.y) * d, (a.y * b.x - a.x * b.y) * d};
 }
+// a * conj(b)
+__host__ __device__ __forceinline__ c2 cmulc(c2 a, c2 b) { return c2{a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y}; }
 // a * (i s): multiply by a purely imaginary number
 __host__ __device__ __forceinline__ c2 cmul_i(c2 a, double s) { return c2{-a.y * s, a.x * s}; }
 // acc + a b and acc - a b as two chained fused multiply-adds per component
@@ -139,8 +141,9 @@ hipError_t launch_w2t(hipStream_t s, int statistic, double beta, int n_max, long
                       const c2 *moments, const TailFitDev &fit, const MatsDev &mt, const FftDev &p, Status st);
 hipError_t launch_t2w(hipStream_t s, int statistic, double beta, int nw, long ncol, const c2 *in, c2 *out,
                       const MatsDev &mt, const FftDev &p, const Vinv8 &vinv);
+// herm: G_ab(tau, r)^* = G_ba(tau, -r), only the columns of r <= -r read
 hipError_t launch_chi0_wr(hipStream_t s, double beta, int nw, const int dims[3], int n, const c2 *gtr, c2 *out,
-                          const MatsDev &mt, const FftDev &p, const Vinv8 &vinv);
+                          const MatsDev &mt, const FftDev &p, const Vinv8 &vinv, bool herm = false);
 hipError_t launch_chi0_tr(hipStream_t s, int ntau, const int dims[3], int n, const c2 *gtr, c2 *out);
 hipError_t launch_chi0_w0r(hipStream_t s, double beta, int ntau, const int dims[3], int n, const c2 *gtr, c2 *out);
 hipError_t launch_chi_w0r(hipStream_t s, double beta, int ntau, long nr, int n, const c2 *chi_tr, c2 *out);

@@ -353,9 +353,11 @@ hipError_t launch_t2w(hipStream_t s, int statistic, double beta, int nw, long nc
 // (chi_imtime.cpp:127).
 
 // fused product + bosonic tau->nu: one workgroup per (r, C consecutive abcd)
+// herm: G_ab(tau, s)^* = G_ba(tau, -s) and only the columns of s <= -s are
+// read (the others as conj(G_ba(tau, -s))).
 __global__ void __launch_bounds__(256) k_chi0_wr(double beta, int nw, int d0, int d1, int d2, int n, int C,
                                                  const c2 *__restrict__ gtr, c2 *__restrict__ out, MatsDev mt,
-                                                 FftDev p, Vinv8 vinv) {
+                                                 FftDev p, Vinv8 vinv, int herm) {
   extern __shared__ c2 sm[];
   const int L = mt.L;
   const int ld = (L + 1) | 1;
@@ -373,8 +375,13 @@ __global__ void __launch_bounds__(256) k_chi0_wr(double beta, int nw, int d0, in
     const int j = t / cn, c = t - j * cn;
     const int q = q0 + c;
     const int dd = q % n, cc = (q / n) % n, bb = (q / n2) % n, aa = q / (n2 * n);
-    const c2 gp = gtr[((long)j * nr + r) * n2 + dd * n + aa];
-    const c2 gm = gtr[((long)(L - j) * nr + mr) * n2 + bb * n + cc];
+    // G_xy(tau_i, s), read from -s (transposed, conjugated) when herm and s > -s
+    auto g = [&](long i, long sv, long msv, int x, int y) {
+      if (herm && sv > msv) return cconj(gtr[(i * nr + msv) * n2 + y * n + x]);
+      return gtr[(i * nr + sv) * n2 + x * n + y];
+    };
+    const c2 gp = g(j, r, mr, dd, aa);
+    const c2 gm = g(L - j, mr, r, bb, cc);
     a[c * ld + j] = cmul(gp, gm);
   }
   __syncthreads();
@@ -382,7 +389,7 @@ __global__ void __launch_bounds__(256) k_chi0_wr(double beta, int nw, int d0, in
 }
 
 hipError_t launch_chi0_wr(hipStream_t s, double beta, int nw, const int dims[3], int n, const c2 *gtr, c2 *out,
-                          const MatsDev &mt, const FftDev &p, const Vinv8 &vinv) {
+                          const MatsDev &mt, const FftDev &p, const Vinv8 &vinv, bool herm) {
   const int n4 = n * n * n * n;
   int C = mats_cols_per_block(mt.L);
   if (C > n4) C = n4;
@@ -391,7 +398,7 @@ hipError_t launch_chi0_wr(hipStream_t s, double beta, int nw, const int dims[3],
   const long nb = nr * ((n4 + C - 1) / C);
   const size_t lds = 2 * (size_t)C * ld * sizeof(c2);
   hipLaunchKernelGGL(k_chi0_wr, dim3(nb), dim3(256), lds, s, beta, nw, dims[0], dims[1], dims[2], n, C, gtr, out, mt,
-                     p, vinv);
+                     p, vinv, herm ? 1 : 0);
   return hipGetLastError();
 }
 
