@@ -102,9 +102,11 @@ int tprf_lattice_dyson_g_wr(tprf_ctx *ctx, double beta, int n_iw, double mu, con
 /* The same for a Hermitian e_k (the caller's promise): on the fused 16^3 path
  * only the frequencies w_n >= 0 are inverted and transformed, and
  * G_ba(i w_{-n-1}, -r) is written as conj(G_ab(i w_n, r)) (from
- * G(i w_{-n-1}, k) = G(i w_n, k)^H); other shapes take the general path.    */
+ * G(i w_{-n-1}, k) = G(i w_n, k)^H); other shapes take the general path.
+ * full = 0: on the fused path only the columns of r <= -r are written (all
+ * that tprf_fourier_wr_to_tr_herm reads); full = 1: every column.           */
 int tprf_lattice_dyson_g_wr_herm(tprf_ctx *ctx, double beta, int n_iw, double mu, const int dims[3], int norb,
-                                 const tprf_c128 *e_k, tprf_c128 *g_wr);
+                                 const tprf_c128 *e_k, tprf_c128 *g_wr, int full);
 
 /* ---- a2 / a6: lattice FFT ---------------------------------------------- */
 /* fourier_wk_to_wr, gf.hpp:276 / gf.cpp:209 -> fourier.hpp:165-197:

@@ -89,16 +89,25 @@ def test_dyson_g_wr(L, oracle, dims, n, beta, n_iw):
     # the fused 16^3 kernel has two workgroup shapes (option 4 selects one, -1
     # restores tuning)
     variants = (0, 1, -1) if tuple(dims) == (16, 16, 16) and n <= 3 else (-1,)
-    # the Hermitian entry forms w_n >= 0 and writes the rest from the symmetry
-    for fn in ("tprf_lattice_dyson_g_wr", "tprf_lattice_dyson_g_wr_herm"):
+    # the Hermitian entry forms w_n >= 0 and writes the rest from the symmetry;
+    # with full = 0 the fused 16^3 path writes only the columns of r <= -r
+    fused = tuple(dims) == (16, 16, 16) and n <= 3
+    canon = canonical_r(dims)
+    for fn, extra in (("tprf_lattice_dyson_g_wr", ()), ("tprf_lattice_dyson_g_wr_herm", (1,)),
+                      ("tprf_lattice_dyson_g_wr_herm", (0,))):
         for variant in variants:
             _lib.call("tprf_ctx_set_option", ctx, 4, variant)
             try:
                 out = torch.full((2 * n_iw, nk, n, n), float("nan"), dtype=torch.complex128, device="cuda")
                 _lib.tail_status_reset(ctx)
-                _lib.call(fn, ctx, beta, n_iw, mu, _lib.dims3(dims), n, _lib.ptr(e_d), _lib.ptr(out))
+                _lib.call(fn, ctx, beta, n_iw, mu, _lib.dims3(dims), n, _lib.ptr(e_d), _lib.ptr(out), *extra)
                 _lib.tail_status(ctx)  # raises on a singular inversion
-                assert relerr(host(out), ref) <= 1e-12, (fn, variant)
+                got = host(out)
+                if extra == (0,) and fused:
+                    assert relerr(got[:, canon], ref[:, canon]) <= 1e-12, (fn, variant)
+                    assert np.isnan(got[:, ~canon]).all()
+                else:
+                    assert relerr(got, ref) <= 1e-12, (fn, extra, variant)
             finally:
                 _lib.call("tprf_ctx_set_option", ctx, 4, -1)
 
@@ -552,11 +561,11 @@ def test_chain_fused_equals_per_function(L):
     assert err < 1e-6
     assert got.g_wk is None
     assert got.herm and got.herm_nu
-    assert relerr(host(got.g_wr), host(ref.g_wr)) <= 1e-12
+    # the default chain stores G(i w, r) and G(tau, r) for r <= -r only
+    canon = canonical_r(wl.dims)
+    assert relerr(host(got.g_wr)[:, canon], host(ref.g_wr)[:, canon]) <= 1e-12
     # the tail fit of omega -> tau weighs the high frequencies by up to
     # w_max^3, so rounding-level differences of G(iw, r) grow by ~10 there
-    # the default chain stores G(tau, r) for r <= -r only
-    canon = canonical_r(wl.dims)
     assert relerr(host(got.g_tr)[:, canon], host(ref.g_tr)[:, canon]) <= TOL
     assert relerr(host(got.chi0_wr), host(ref.chi0_wr)) <= TOL
     # with the symmetries the fused chain transforms nu >= 0 only

@@ -42,7 +42,7 @@ _SIGNATURES = {
     "tprf_ctx_tail_status_reset": [_p],
     "tprf_lattice_dyson_g_wk": [_p, _d, _i, _d, _l, _i, _p, _i, _p, _p],
     "tprf_lattice_dyson_g_wr": [_p, _d, _i, _d, _p, _i, _p, _p],
-    "tprf_lattice_dyson_g_wr_herm": [_p, _d, _i, _d, _p, _i, _p, _p],
+    "tprf_lattice_dyson_g_wr_herm": [_p, _d, _i, _d, _p, _i, _p, _p, _i],
     "tprf_fourier_wk_to_wr": [_p, _l, _p, _l, _p, _p],
     "tprf_fourier_wr_to_wk": [_p, _l, _p, _l, _p, _p],
     "tprf_fourier_wr_to_tr": [_p, _i, _d, _i, _i, _l, _p, _p, _p],

@@ -62,8 +62,12 @@ class BubbleRPA:
             _lib.call("tprf_lattice_dyson_g_wk", c, wl.beta, wl.n_iw, wl.mu, nk, n, p(self.e_k), _lib.SIGMA_NONE, None,
                       p(self.g_wk))
         elif name == "dyson_wr":
-            fn = "tprf_lattice_dyson_g_wr_herm" if self.herm else "tprf_lattice_dyson_g_wr"
-            _lib.call(fn, c, wl.beta, wl.n_iw, wl.mu, self.dims, n, p(self.e_k), p(self.g_wr))
+            if self.herm:  # only r <= -r is written: the symmetric omega -> tau reads no other column
+                _lib.call("tprf_lattice_dyson_g_wr_herm", c, wl.beta, wl.n_iw, wl.mu, self.dims, n, p(self.e_k),
+                          p(self.g_wr), 0)
+            else:
+                _lib.call("tprf_lattice_dyson_g_wr", c, wl.beta, wl.n_iw, wl.mu, self.dims, n, p(self.e_k),
+                          p(self.g_wr))
         elif name == "wk_to_wr":
             _lib.call("tprf_fourier_wk_to_wr", c, 2 * wl.n_iw, self.dims, n * n, p(self.g_wk), p(self.g_wr))
         elif name == "wr_to_tr":

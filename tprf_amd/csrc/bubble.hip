@@ -45,6 +45,8 @@ namespace tprf {
 // transformed: a workgroup with a < b also writes the columns badc of r and
 // dcba of -r, with S_badc(m) = (S(m) - x_0 + x_L)^*, S^{-r}_dcba(m) = S(-m)^*
 // and the conjugated pole coefficients of the mirrored and the direct column.
+// For a == b the same identities pair (a, a, c, d) with (a, a, d, c), and only
+// the columns c <= d are transformed.
 template <int N, int R1, int R2, int CG, int NT, bool AB, bool DMA, int WPE, bool HERM = false>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) k_bubble_pair(double beta, int L, int nb, int d0, int d1, int d2,
                                                     const c2 *__restrict__ gtr, c2 *__restrict__ out, BubbleDev bd,
@@ -92,7 +94,17 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
     b0 = a0 + rest;
   }
   const int q0 = AB ? (a0 * N + b0) * n2 : grp * CG;
-  const int cn = (n4 - q0) < CG ? (n4 - q0) : CG;
+  // HERM, a == b: (a, a, c, d) and (a, a, d, c) are conjugate partners, so only
+  // the n (n + 1) / 2 columns c <= d are transformed; column slot c' maps to
+  // the c'-th pair c <= d in row-major order
+  const bool diag_pair = HERM && a0 == b0;
+  const int cn = diag_pair ? N * (N + 1) / 2 : ((n4 - q0) < CG ? (n4 - q0) : CG);
+  auto col_q = [&](int c) {
+    if (!diag_pair) return q0 + c;
+    int cc = 0, rest = c;
+    while (rest >= N - cc) { rest -= N - cc; ++cc; }
+    return q0 + cc * N + cc + rest;
+  };
   const int N1 = L / P;
   const double dtau = beta / (double)L;
 
@@ -107,7 +119,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
       const int t = min(tid + w * NT, cn * 18 - 1);
       const int c = t / 18, e = t - c * 18;
       const int j = e < 9 ? e : L - 17 + e;
-      const int q = q0 + c;
+      const int q = col_q(c);
       const int dd = q % N, cc = (q / N) % N, bb = (q / n2) % N, aa = q / (n2 * N);
       ga[w] = ld_v2d(gtr + ((long)j * nr + r) * n2 + dd * N + aa);
       // HERM: G_bc(tau, -r) = conj(G_cb(tau, r)), read from r (conjugated below)
@@ -160,7 +172,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
   auto stage1 = [&](const c2 *tA, const c2 *tB) {
     for (int u = tid; u < cn * R2; u += NT) {
       const int c = u / R2, s = u - c * R2;
-      const int q = q0 + c;
+      const int q = col_q(c);
       const int dd = q % N, cc = (q / N) % N, bb = (q / n2) % N, aa = q / (n2 * N);
       c2 v[R1];
 #pragma unroll
@@ -340,7 +352,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
     const int u = tid + i * NT;
     if (u < cn * R1) {
       const int c = u / R1, k1 = u - c * R1;
-      const int q = q0 + c;
+      const int q = col_q(c);
       const int dd = q % N, cc = (q / N) % N, bb = (q / n2) % N, aa = q / (n2 * N);
       const int qm = ((cc * N + dd) * N + aa) * N + bb;
       const c2 *pa0 = sPole[c][0], *pa1 = sPole[c][1];
@@ -361,7 +373,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
           out[((long)im * nr + mr) * n4 + qm] = v;
         }
         if constexpr (HERM) {
-          if (a0 < b0) {
+          if (a0 < b0 || cc != dd) {
             // column badc of r: S = (S(m) - x_0 + x_L)^*, pole coefficients conj(a1)
             const int qb = ((bb * N + aa) * N + dd) * N + cc;
             const c2 sb = cconj(cadd(acc[i][k2], xl_minus_x0));

@@ -738,8 +738,9 @@ int tprf_lattice_dyson_g_wk(tprf_ctx *ctx, double beta, int n_iw, double mu, lon
 
 namespace {
 
+// sym: 0 general, 1 Hermitian e_k (full output), 2 Hermitian e_k, columns of r <= -r only
 int dyson_g_wr(tprf_ctx *ctx, double beta, int n_iw, double mu, const int dims[3], int norb, const tprf_c128 *e_k,
-               tprf_c128 *g_wr, bool sym) {
+               tprf_c128 *g_wr, int sym) {
   if (!ctx) return TPRF_ERR_ARG;
   if (!(beta > 0) || n_iw < 1 || !dims_ok(dims) || !e_k || !g_wr)
     return fail(ctx, TPRF_ERR_ARG, "lattice_dyson_g_wr: bad argument");
@@ -752,8 +753,8 @@ int dyson_g_wr(tprf_ctx *ctx, double beta, int n_iw, double mu, const int dims[3
     };
     const int nvar = tprf::dyson_lattice16r_variants(norb);
     const int forced = ctx->lattice_variant >= 0 ? ctx->lattice_variant % nvar : -1;
-    return tuned_launch(ctx, ctx->dyson_lattice_tuned, std::make_tuple(n_iw, norb * (sym ? -1 : 1)), nvar, forced,
-                        launch, "fused dyson lattice kernel");
+    return tuned_launch(ctx, ctx->dyson_lattice_tuned, std::make_tuple(n_iw, norb + 8 * sym), nvar, forced, launch,
+                        "fused dyson lattice kernel");
   }
   // other shapes: Dyson into g_wr, then the lattice transform in place
   int st = check_launch(ctx,
@@ -768,12 +769,12 @@ int dyson_g_wr(tprf_ctx *ctx, double beta, int n_iw, double mu, const int dims[3
 
 int tprf_lattice_dyson_g_wr(tprf_ctx *ctx, double beta, int n_iw, double mu, const int dims[3], int norb,
                             const tprf_c128 *e_k, tprf_c128 *g_wr) {
-  return dyson_g_wr(ctx, beta, n_iw, mu, dims, norb, e_k, g_wr, false);
+  return dyson_g_wr(ctx, beta, n_iw, mu, dims, norb, e_k, g_wr, 0);
 }
 
 int tprf_lattice_dyson_g_wr_herm(tprf_ctx *ctx, double beta, int n_iw, double mu, const int dims[3], int norb,
-                                 const tprf_c128 *e_k, tprf_c128 *g_wr) {
-  return dyson_g_wr(ctx, beta, n_iw, mu, dims, norb, e_k, g_wr, true);
+                                 const tprf_c128 *e_k, tprf_c128 *g_wr, int full) {
+  return dyson_g_wr(ctx, beta, n_iw, mu, dims, norb, e_k, g_wr, full ? 1 : 2);
 }
 
 int tprf_fourier_wk_to_wr(tprf_ctx *ctx, long nw, const int dims[3], long ntarget, const tprf_c128 *in,

@@ -126,10 +126,11 @@ hipError_t launch_dyson(hipStream_t s, double beta, int n_iw, double mu, long nk
 // Dyson (Sigma = 0) fused into the 16^3 k -> r transform, n <= 3:
 // e_k[4096][n][n] -> g_wr[2 n_iw][4096][n][n]
 // variant: 0 = one orbital row per workgroup, 1 = one column (same results)
-// sym (e_k Hermitian): the frequencies w_n >= 0 only, G_ba(i w_{-n-1}, -r)
-// written as conj(G_ab(i w_n, r))
+// sym (e_k Hermitian): 1 = the frequencies w_n >= 0 only, G_ba(i w_{-n-1}, -r)
+// written as conj(G_ab(i w_n, r)); 2 = the same, writing only the columns of
+// r <= -r
 hipError_t launch_dyson_lattice16r(hipStream_t s, double beta, int n_iw, double mu, int n, const c2 *ek, c2 *out,
-                                   Status st, int variant, bool sym = false);
+                                   Status st, int variant, int sym = 0);
 int dyson_lattice16r_variants(int n);
 // lindhard.hip: eigenpairs of e_k - mu into ev[nk][n] and proj[nk][n][n][n]
 // (workspace), then chi[nz][nk][n^4] at the nz complex frequencies z (device)

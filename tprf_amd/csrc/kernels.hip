@@ -949,11 +949,12 @@ __device__ __forceinline__ c2 inv00(const v2d (&y)[N][N], bool &singular) {
 // resident per CU).
 // SYM (e_k Hermitian): only the frequencies w_n >= 0 (index w >= n_iw) are
 // formed and transformed; G(i w_{-n-1}, k) = G(i w_n, k)^H gives
-// G_ba(i w_{-n-1}, -r) = conj(G_ab(i w_n, r)), written alongside.
+// G_ba(i w_{-n-1}, -r) = conj(G_ab(i w_n, r)), written alongside.  half: only
+// the columns of r <= -r are written (at every frequency).
 template <int N, int C, bool SYM = false>
 __global__ void __launch_bounds__(256 * C) k_dyson_lattice16r(double beta, int n_iw, double mu,
                                                               const c2 *__restrict__ ek, c2 *__restrict__ out,
-                                                              double scale, double *status) {
+                                                              double scale, double *status, int half = 0) {
   static_assert(C == 1 || C == N, "one column or one orbital row per workgroup");
   constexpr int NT = N * N, NG = NT / C;  // workgroups per frequency
   constexpr int NSM = 2048 + 2048 / 16;
@@ -1007,25 +1008,29 @@ __global__ void __launch_bounds__(256 * C) k_dyson_lattice16r(double beta, int n
   if (singular) status[2] = 1.0;
   l16r_transform<C, -1>(v, sm, cc, hi, lo);
   c2 *dst = out + w * 4096 * NT + a * N + b;
-#pragma unroll
-  for (int k2 = 0; k2 < 16; ++k2) dst[(long)(p * 16 + k2) * NT] = cscale(v[k2], scale);
   if constexpr (SYM) {
     // r = (hi, lo, k2) -> -r = (-hi, -lo, -k2) mod 16, frequency index 2 n_iw - 1 - w
     c2 *mdst = out + (2L * n_iw - 1 - w) * 4096 * NT + b * N + a;
     const int mh = (16 - hi) & 15, ml = (16 - lo) & 15;
 #pragma unroll
-    for (int k2 = 0; k2 < 16; ++k2)
-      mdst[(long)((mh * 16 + ml) * 16 + ((16 - k2) & 15)) * NT] = cconj(cscale(v[k2], scale));
+    for (int k2 = 0; k2 < 16; ++k2) {
+      const int r = p * 16 + k2, mr = (mh * 16 + ml) * 16 + ((16 - k2) & 15);
+      if (!half || r <= mr) dst[(long)r * NT] = cscale(v[k2], scale);
+      if (!half || mr <= r) mdst[(long)mr * NT] = cconj(cscale(v[k2], scale));
+    }
+  } else {
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) dst[(long)(p * 16 + k2) * NT] = cscale(v[k2], scale);
   }
 }
 
 template <int N, int C>
 static hipError_t launch_dl16(hipStream_t s, double beta, int n_iw, double mu, const c2 *ek, c2 *out, Status st,
-                              bool sym) {
+                              int sym) {
   const long nb = (sym ? 1L : 2L) * n_iw * (N * N / C);
   if (sym)
     hipLaunchKernelGGL((k_dyson_lattice16r<N, C, true>), dim3(nb), dim3(256 * C), 0, s, beta, n_iw, mu, ek, out,
-                       1.0 / 4096.0, st.d);
+                       1.0 / 4096.0, st.d, sym == 2 ? 1 : 0);
   else
     hipLaunchKernelGGL((k_dyson_lattice16r<N, C, false>), dim3(nb), dim3(256 * C), 0, s, beta, n_iw, mu, ek, out,
                        1.0 / 4096.0, st.d);
@@ -1036,7 +1041,7 @@ static hipError_t launch_dl16(hipStream_t s, double beta, int n_iw, double mu, c
 int dyson_lattice16r_variants(int n) { return n == 1 ? 1 : 2; }
 
 hipError_t launch_dyson_lattice16r(hipStream_t s, double beta, int n_iw, double mu, int n, const c2 *ek, c2 *out,
-                                   Status st, int variant, bool sym) {
+                                   Status st, int variant, int sym) {
   switch (n) {
     case 1: return launch_dl16<1, 1>(s, beta, n_iw, mu, ek, out, st, sym);
     case 2: return variant == 1 ? launch_dl16<2, 1>(s, beta, n_iw, mu, ek, out, st, sym)
