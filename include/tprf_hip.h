@@ -169,9 +169,12 @@ int tprf_chi0_wr_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dim
  * only the orbital pairs a <= b and forms the columns of b > a from the
  * symmetry chi0_badc(nu, r) = conj of the tau-reversed product transform of
  * abcd; for such a G the result equals tprf_chi0_wr_from_grt_PH's to
- * rounding.  For a G without the symmetry the result is wrong.              */
+ * rounding.  For a G without the symmetry the result is wrong.  full = 1:
+ * every frequency is written; full = 0: the rows nu_m, m >= 0 (nw - 1 ..
+ * 2 nw - 2) are, the others may be left unwritten (for a caller that uses
+ * chi0(-nu) = chi0(nu)^H in the PH grouping, as tprf_solve_rpa_PH_herm does).  */
 int tprf_chi0_wr_from_grt_PH_herm(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb, int nw,
-                                  const tprf_c128 *g_tr, tprf_c128 *chi0_wr);
+                                  const tprf_c128 *g_tr, tprf_c128 *chi0_wr, int full);
 /* chi0_w0r_from_grt_PH(g_tr), chi_imtime.cpp:185-228 (trapezoid, nu = 0). out[1][nr][n^4] */
 int tprf_chi0_w0r_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb,
                               const tprf_c128 *g_tr, tprf_c128 *chi0_wr);

@@ -50,7 +50,7 @@ _SIGNATURES = {
     "tprf_fourier_tr_to_wr": [_p, _i, _d, _i, _i, _l, _p, _p],
     "tprf_chi0_tr_from_grt_PH": [_p, _i, _p, _i, _p, _p],
     "tprf_chi0_wr_from_grt_PH": [_p, _d, _i, _p, _i, _i, _p, _p],
-    "tprf_chi0_wr_from_grt_PH_herm": [_p, _d, _i, _p, _i, _i, _p, _p],
+    "tprf_chi0_wr_from_grt_PH_herm": [_p, _d, _i, _p, _i, _i, _p, _p, _i],
     "tprf_chi0_w0r_from_grt_PH": [_p, _d, _i, _p, _i, _p, _p],
     "tprf_chi_w0r_from_chi_tr": [_p, _d, _i, _l, _i, _p, _p],
     "tprf_solve_rpa_PH": [_p, _l, _i, _p, _p, _p],

@@ -24,10 +24,10 @@ class BubbleRPA:
     and the bubble as tprf_chi0_wr_from_grt_PH_herm (orbital pairs a <= b
     transformed, the others formed from the symmetry); if U_abcd = conj(U_dcba) as well,
     M_chi0(-nu, q) = M_chi0(nu, q)^H and M_chi(-nu, q) = M_chi(nu, q)^H, so
-    r -> q transforms the frequencies nu >= 0 only and
+    the bubble writes and r -> q transforms the frequencies nu >= 0 only and
     tprf_solve_rpa_PH_herm solves those and writes chi(-nu) as the conjugate
-    transposes (chi0_wk then holds nu >= 0 only).  fused=False: one call per
-    reference function."""
+    transposes (chi0_wr and chi0_wk then hold nu >= 0 only).  fused=False: one
+    call per reference function."""
     API_STAGES = ("dyson", "wk_to_wr", "wr_to_tr", "chi0_wr", "wr_to_wk", "rpa")
     FUSED_STAGES = ("dyson_wr", "wr_to_tr", "chi0_wr", "wr_to_wk", "rpa")
 
@@ -78,8 +78,12 @@ class BubbleRPA:
                 _lib.call("tprf_fourier_wr_to_tr", c, _lib.FERMION, wl.beta, wl.n_iw, wl.ntau, nk * n * n,
                           p(self.g_wr), p(self.g_tr), None)
         elif name == "chi0_wr":
-            fn = "tprf_chi0_wr_from_grt_PH_herm" if self.herm else "tprf_chi0_wr_from_grt_PH"
-            _lib.call(fn, c, wl.beta, wl.ntau, self.dims, n, wl.nb, p(self.g_tr), p(self.chi0_wr))
+            if self.herm:  # with herm_nu only nu >= 0 is written: r -> q and RPA read no other rows
+                _lib.call("tprf_chi0_wr_from_grt_PH_herm", c, wl.beta, wl.ntau, self.dims, n, wl.nb, p(self.g_tr),
+                          p(self.chi0_wr), 0 if self.herm_nu else 1)
+            else:
+                _lib.call("tprf_chi0_wr_from_grt_PH", c, wl.beta, wl.ntau, self.dims, n, wl.nb, p(self.g_tr),
+                          p(self.chi0_wr))
         elif name == "wr_to_wk":
             if self.herm_nu:  # nu >= 0 only: rows nb - 1 .. 2 nb - 2
                 _lib.call("tprf_fourier_wr_to_wk", c, wl.nb, self.dims, n ** 4, p(self.chi0_wr[wl.nb - 1:]),

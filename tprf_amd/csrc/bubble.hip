@@ -47,10 +47,12 @@ namespace tprf {
 // and the conjugated pole coefficients of the mirrored and the direct column.
 // For a == b the same identities pair (a, a, c, d) with (a, a, d, c), and only
 // the columns c <= d are transformed.
+// Outputs are written for the frequencies m_lo <= m < nb (m_lo = 1 - nb: all;
+// 0: nu >= 0 only).
 template <int N, int R1, int R2, int CG, int NT, bool AB, bool DMA, int WPE, bool HERM = false>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) k_bubble_pair(double beta, int L, int nb, int d0, int d1, int d2,
                                                     const c2 *__restrict__ gtr, c2 *__restrict__ out, BubbleDev bd,
-                                                    Vinv8 vinv) {
+                                                    Vinv8 vinv, int m_lo) {
   constexpr int P = R1 * R2;
   constexpr int n2 = N * N, n4 = n2 * n2;
   static_assert(!AB || CG == n2, "one orbital pair per workgroup");
@@ -361,7 +363,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
       for (int k2 = 0; k2 < R2; ++k2) {
         const int k = k1 + R1 * k2;
         const int m = k < P / 2 ? k : k - P;
-        if (m <= -nb || m >= nb) continue;
+        if (m < m_lo || m >= nb) continue;
         const int im = m + nb - 1;
         // y(m) = fact S(m) + sum_q a_q H_q(m), H_q = 1 / (i nu_m - b_q) - fact Phi_q(m)
         const c2 h0 = hmod(im), h1 = hmod(nnu + im), h2 = hmod(2 * nnu + im);
@@ -397,17 +399,17 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
 
 template <int N, int R1, int R2, int CG, int NT, bool AB, bool DMA, int WPE, bool HERM>
 static hipError_t launch_one_h(hipStream_t s, double beta, int L, int nb, const int dims[3], const c2 *gtr, c2 *out,
-                               const BubbleDev &bd, const Vinv8 &vinv) {
+                               const BubbleDev &bd, const Vinv8 &vinv, int m_lo) {
   constexpr int n4 = N * N * N * N;
   constexpr int NG = HERM ? N * (N + 1) / 2 : (n4 + CG - 1) / CG;
   const long nb_blocks = (long)bd.ncanon * NG;
   if (DMA && L / (R1 * R2) < 2) {  // the global -> LDS pipeline needs two tiles
     hipLaunchKernelGGL((k_bubble_pair<N, R1, R2, CG, NT, AB, false, 1, HERM>), dim3(nb_blocks), dim3(NT), 0, s, beta, L,
-                       nb, dims[0], dims[1], dims[2], gtr, out, bd, vinv);
+                       nb, dims[0], dims[1], dims[2], gtr, out, bd, vinv, m_lo);
     return hipGetLastError();
   }
   hipLaunchKernelGGL((k_bubble_pair<N, R1, R2, CG, NT, AB, DMA, WPE, HERM>), dim3(nb_blocks), dim3(NT), 0, s, beta, L,
-                     nb, dims[0], dims[1], dims[2], gtr, out, bd, vinv);
+                     nb, dims[0], dims[1], dims[2], gtr, out, bd, vinv, m_lo);
   return hipGetLastError();
 }
 
@@ -415,14 +417,14 @@ static hipError_t launch_one_h(hipStream_t s, double beta, int L, int nb, const 
 // has orbital-pair workgroups (hipErrorNotSupported otherwise)
 template <int N, int R1, int R2, int CG, int NT, bool AB, bool DMA, int WPE>
 static hipError_t launch_one(hipStream_t s, double beta, int L, int nb, const int dims[3], const c2 *gtr, c2 *out,
-                             const BubbleDev &bd, const Vinv8 &vinv, bool herm) {
+                             const BubbleDev &bd, const Vinv8 &vinv, bool herm, int m_lo) {
   if (herm) {
     if constexpr (AB)
-      return launch_one_h<N, R1, R2, CG, NT, AB, DMA, WPE, true>(s, beta, L, nb, dims, gtr, out, bd, vinv);
+      return launch_one_h<N, R1, R2, CG, NT, AB, DMA, WPE, true>(s, beta, L, nb, dims, gtr, out, bd, vinv, m_lo);
     else
       return hipErrorNotSupported;
   }
-  return launch_one_h<N, R1, R2, CG, NT, AB, DMA, WPE, false>(s, beta, L, nb, dims, gtr, out, bd, vinv);
+  return launch_one_h<N, R1, R2, CG, NT, AB, DMA, WPE, false>(s, beta, L, nb, dims, gtr, out, bd, vinv, m_lo);
 }
 
 bool bubble_pair_supported(int n, int P) { return (n == 2 || n == 3) && (P == 64 || P == 128); }
@@ -444,9 +446,10 @@ int bubble_pair_herm_variants(int n, int P, int *list) {
 // straight into LDS (for n = 3, P = 128 the two differ in the occupancy the
 // register allocation is held to).
 hipError_t launch_bubble_pair(hipStream_t s, double beta, int L, int nb, const int dims[3], int n, const c2 *gtr,
-                              c2 *out, const BubbleDev &bd, const Vinv8 &vinv, int variant, bool herm) {
+                              c2 *out, const BubbleDev &bd, const Vinv8 &vinv, int variant, bool herm, bool nonneg) {
+  const int m_lo = nonneg ? 0 : 1 - nb;
 #define TPRF_BP(N_, R1_, R2_, CG_, NT_, AB_, DMA_, WPE_) \
-  return launch_one<N_, R1_, R2_, CG_, NT_, AB_, DMA_, WPE_>(s, beta, L, nb, dims, gtr, out, bd, vinv, herm)
+  return launch_one<N_, R1_, R2_, CG_, NT_, AB_, DMA_, WPE_>(s, beta, L, nb, dims, gtr, out, bd, vinv, herm, m_lo)
   if (n == 3 && bd.P == 128) {
     if (variant == 0) TPRF_BP(3, 16, 8, 9, 256, true, false, 1);
     if (variant == 1) TPRF_BP(3, 16, 8, 9, 128, true, false, 1);

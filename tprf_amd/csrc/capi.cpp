@@ -938,7 +938,7 @@ int tprf_chi0_tr_from_grt_PH(tprf_ctx *ctx, int ntau, const int dims[3], int nor
 namespace {
 
 int chi0_wr_from_grt(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb, int nw, const tprf_c128 *g_tr,
-                     tprf_c128 *chi0_wr, bool herm) {
+                     tprf_c128 *chi0_wr, bool herm, bool nonneg = false) {
   if (!ctx) return TPRF_ERR_ARG;
   if (!(beta > 0) || ntau < 10 || !dims_ok(dims) || norb < 1 || norb > 4 || nw < 1 || !g_tr || !chi0_wr)
     return fail(ctx, TPRF_ERR_ARG, "chi0_wr_from_grt_PH: bad argument");
@@ -966,7 +966,7 @@ int chi0_wr_from_grt(tprf_ctx *ctx, double beta, int ntau, const int dims[3], in
     const int cnt = tprf::bubble_pair_herm_variants(norb, P, list);
     auto launch_h = [&](int i) {
       return tprf::launch_bubble_pair(ctx->stream, beta, L, nw, dims, norb, (const c2 *)g_tr, (c2 *)chi0_wr, *bd,
-                                      ctx->vinv, list[i], true);
+                                      ctx->vinv, list[i], true, nonneg);
     };
     int forced = -1;  // a forced plain variant without a Hermitian form uses the first one
     if (ctx->bubble_variant >= 0) {
@@ -997,8 +997,8 @@ int tprf_chi0_wr_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dim
 }
 
 int tprf_chi0_wr_from_grt_PH_herm(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb, int nw,
-                                  const tprf_c128 *g_tr, tprf_c128 *chi0_wr) {
-  return chi0_wr_from_grt(ctx, beta, ntau, dims, norb, nw, g_tr, chi0_wr, true);
+                                  const tprf_c128 *g_tr, tprf_c128 *chi0_wr, int full) {
+  return chi0_wr_from_grt(ctx, beta, ntau, dims, norb, nw, g_tr, chi0_wr, true, full == 0);
 }
 
 int tprf_chi0_w0r_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb,

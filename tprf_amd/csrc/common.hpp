@@ -157,9 +157,10 @@ hipError_t launch_rpa_mirror(hipStream_t s, long nb, long nk, int n, const c2 *c
 // variant < bubble_pair_variants() selects the workgroup shape (same results);
 // herm: the form for G_ab(tau, r)^* = G_ba(tau, -r), which transforms only the
 // orbital pairs a <= b (hipErrorNotSupported for shapes without orbital-pair
-// workgroups)
+// workgroups); nonneg: only the frequencies nu >= 0 are written
 hipError_t launch_bubble_pair(hipStream_t s, double beta, int L, int nb, const int dims[3], int n, const c2 *gtr,
-                              c2 *out, const BubbleDev &bd, const Vinv8 &vinv, int variant, bool herm = false);
+                              c2 *out, const BubbleDev &bd, const Vinv8 &vinv, int variant, bool herm = false,
+                              bool nonneg = false);
 bool bubble_pair_supported(int n, int P);
 int bubble_pair_variants();
 // variants usable with herm, written to list[0..count), count returned (<= 5)

@@ -217,19 +217,28 @@ __global__ void __launch_bounds__(NT)
   for (int r = 0; r < 8; ++r) raw[r] = cscale(raw[r], ib);
 
   for (int t = 0; t < Q; ++t) {
-    // pass 1 inputs: FFT index e = j + TPC r  <->  n = e or e - M, twiddled by exp(-2 pi i t n / L)
+    // pass 1 inputs: FFT index e = j + TPC r  <->  n = e or e - M, twiddled by
+    // exp(-2 pi i t n / L).  The eight table loads are unconditional (every
+    // index t n mod L is in the table) so they are all in flight together.
     c2 v[8];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const int e = j + TPC * r;
-      const int n = e < M / 2 ? e : e - M;
-      c2 x = raw[r];
-      if (t && n >= first && n < first + nmesh) {
+    for (int r = 0; r < 8; ++r) v[r] = raw[r];
+    if (t) {
+      v2d tw[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int e = j + TPC * r;
+        const int n = e < M / 2 ? e : e - M;
         int tn = t * n;  // |t n| < Q M / 2 <= L
         if (tn < 0) tn += L;
-        x = cmul(x, wd.twL[tn]);
+        tw[r] = ld_v2d(wd.twL + tn);
       }
-      v[r] = x;
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int e = j + TPC * r;
+        const int n = e < M / 2 ? e : e - M;
+        if (n >= first && n < first + nmesh) v[r] = cmul(raw[r], c2{tw[r].x, tw[r].y});
+      }
     }
     int Ns = 1;
 #pragma unroll
