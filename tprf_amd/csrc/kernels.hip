@@ -504,22 +504,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N == 3
   constexpr int N2 = N * N;
   constexpr int P = 64 / N2;  // points per wave
   constexpr int NMW = (N2 * N2 + 63) / 64;  // words of the nonzero mask of M_U
-  __shared__ c2 sU[N2 * N2];
   __shared__ c2 slot[4 * P][2 * N2];
   __shared__ unsigned long long sMask[NMW];
-  for (int t = threadIdx.x; t < N2 * N2; t += blockDim.x) {
-    // t = ((a*N + b)*N + c)*N + d  ->  M_U[(a,b),(d,c)]
-    const int d = t % N, c = (t / N) % N, ab = t / N2;
-    sU[ab * N2 + d * N + c] = U[t];
-  }
-  __syncthreads();
+  // M_U[l][j] with l = (a, b), j = (d, c) is U[((a N + b) N + c) N + d]
+  auto u_idx = [](int l, int j) { return l * N2 + (j % N) * N + j / N; };
   // Bit l N2 + j of the mask: M_U[l][j] != 0.  Interactions of physical models
   // are sparse in the particle-hole grouping (Kanamori: 21 of 81 entries for
   // three orbitals); zero entries add exact zeros to 1 - X U and are skipped
   // with uniform branches.
   if (threadIdx.x < 64 * NMW) {
     const int t = threadIdx.x;
-    const bool nz = t < N2 * N2 && (sU[t].x != 0.0 || sU[t].y != 0.0);
+    bool nz = false;
+    if (t < N2 * N2) {
+      const c2 u = U[u_idx(t / N2, t % N2)];
+      nz = u.x != 0.0 || u.y != 0.0;
+    }
     const unsigned long long m = __ballot(nz);
     if ((t & 63) == 0) sMask[t >> 6] = m;
   }
@@ -547,13 +546,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N == 3
 #pragma unroll
     for (int j = 0; j < N2; ++j) X[j] = c2{0, 0};
   }
+  // A = 1 - X M_U, column by column: the n^2 entries of column j of M_U sit
+  // at wave-uniform addresses (scalar loads, all issued before the products)
 #pragma unroll
   for (int j = 0; j < N2; ++j) {
+    c2 uj[N2];
+#pragma unroll
+    for (int l = 0; l < N2; ++l) uj[l] = U[u_idx(l, j)];
     c2 s = (i == j) ? c2{1.0, 0.0} : c2{0.0, 0.0};
 #pragma unroll
     for (int l = 0; l < N2; ++l) {
       const int bit = l * N2 + j;
-      if ((umask[bit >> 6] >> (bit & 63)) & 1ull) s = cfms(s, X[l], sU[l * N2 + j]);
+      if ((umask[bit >> 6] >> (bit & 63)) & 1ull) s = cfms(s, X[l], uj[l]);
     }
     A[j] = s;
   }
@@ -581,7 +585,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N == 3
       myk = k;
       pv = A[k];
 #pragma unroll
-      for (int j = 0; j < N2; ++j) { slot[slot_id][j] = A[j]; slot[slot_id][N2 + j] = X[j]; }
+      for (int j = k; j < N2; ++j) slot[slot_id][j] = A[j];  // columns < k are zero in every unused row
+#pragma unroll
+      for (int j = 0; j < N2; ++j) slot[slot_id][N2 + j] = X[j];
     }
     wave_lds_sync();  // slots are private to a wave
     if (!is_piv && gidx < P) {
