@@ -204,7 +204,8 @@ def test_chi_tr_from_chi_wr_boson(L, oracle):
 
 
 @pytest.mark.parametrize("stat,n_max,nt", [("Fermion", 64, -1), ("Fermion", 128, -1), ("Fermion", 32, 641),
-                                           ("Boson", 64, -1), ("Boson", 32, -1), ("Fermion", 512, -1)])
+                                           ("Boson", 64, -1), ("Boson", 32, -1), ("Fermion", 512, -1),
+                                           ("Fermion", 256, -1)])
 def test_wr_to_tr_pruned_and_generic(L, oracle, stat, n_max, nt):
     """The input-pruned kernel (L = Q M) and the generic full-length kernel both
     match the oracle, for both statistics."""
@@ -446,8 +447,9 @@ def test_chi0_wr_herm(L, oracle, dims, n, nb):
             _lib.call("tprf_ctx_set_option", c, 2, -1)
 
 
-@pytest.mark.parametrize("dims,n", [((4, 3, 2), 2), ((5, 5, 1), 3), ((4, 4, 4), 1), ((3, 2, 2), 3)])
-def test_wr_to_tr_herm(L, oracle, dims, n):
+@pytest.mark.parametrize("dims,n,n_iw", [((4, 3, 2), 2, 128), ((5, 5, 1), 3, 128), ((4, 4, 4), 1, 128),
+                                         ((3, 2, 2), 3, 128), ((3, 2, 2), 3, 256)])
+def test_wr_to_tr_herm(L, oracle, dims, n, n_iw):
     """tprf_fourier_wr_to_tr_herm on G(i w, r) from a Hermitian e_k (odd and
     even lattice sides, so self-mirrored r with a < b pairs), both workgroup
     widths (option 3), against the oracle's full transform; the tail-fit
@@ -457,7 +459,7 @@ def test_wr_to_tr_herm(L, oracle, dims, n):
     rng = np.random.default_rng(7 * n + sum(dims))
     nk = int(np.prod(dims))
     e = hermitian_ek(rng, nk, n, scale=0.8)
-    beta, n_iw = 12.0, 128
+    beta = 12.0
     g_wr = oracle.fourier_lattice(oracle.lattice_dyson_g_wk(beta, n_iw, -0.2, e), dims, -1)
     ref = oracle.fourier_w_to_t(g_wr, beta)
     ntau = ref.shape[0]
