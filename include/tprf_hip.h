@@ -184,19 +184,37 @@ int tprf_chi_w0r_from_chi_tr(tprf_ctx *ctx, double beta, int ntau, long nr, int 
 
 /* ---- a7: RPA ----------------------------------------------------------- */
 /* solve_rpa_PH(chi0, U), rpa.hpp:43 / rpa.cpp:29-68: for every mesh point,
- * M_chi = inverse(1 - M_chi0 M_U) M_chi0 with M_X[(a,b),(d,c)] = X_abcd.
- * chi0[npts][n^4], U[n^4] (device), out chi[npts][n^4].  n <= 4.           */
+ * M_chi = inverse(1 - M_chi0 M_U) M_chi0 with M_X[(a,b),(c,d)] = X_abcd, the
+ * matrix that group_indices_view(X, idx_group<0,1>, idx_group<3,2>) makes of
+ * a row-major tensor (rpa.cpp:40,54-55; for U_abcd = U_badc it equals the
+ * (a,b),(d,c) grouping of rpa.hpp's formula).  chi0[npts][n^4], U[n^4]
+ * (device), out chi[npts][n^4].  n <= 4.                                   */
 int tprf_solve_rpa_PH(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *chi0, const tprf_c128 *U,
                       tprf_c128 *chi);
 /* solve_rpa_PH on a bosonic Matsubara mesh of 2 nb - 1 frequencies
- * (chi0, chi [2 nb - 1][nk][n^4]) when M_chi0(-nu, q) = M_chi0(nu, q)^H and
- * M_U = M_U^H (the caller's promise; it holds for chi0 from a Hermitian e_k
- * and a real interaction with U_abcd = U_dcba, such as the Kanamori tensors).
- * No counterpart in the reference: only the nb frequencies nu >= 0 of chi0
- * are read and solved; chi(-nu, q) is written as M_chi(nu, q)^H, which is
- * what the full solve gives for such inputs, to rounding.                   */
+ * (chi0, chi [2 nb - 1][nk][n^4]) when chi0_abcd(-nu, q) = conj(chi0_dcba(nu, q)),
+ * U_abcd = conj(U_dcba) and U_abcd = U_badc (the caller's promise; it holds
+ * for chi0 from a Hermitian e_k and any real Hermitian interaction, such as
+ * the Kanamori tensors).  No counterpart in the reference: only the nb
+ * frequencies nu >= 0 of chi0 are read and solved; chi_dcba(-nu, q) is
+ * written as conj(chi_abcd(nu, q)), which is what the full solve gives for
+ * such inputs, to rounding.                                                 */
 int tprf_solve_rpa_PH_herm(tprf_ctx *ctx, int nb, long nk, int norb, const tprf_c128 *chi0, const tprf_c128 *U,
                            tprf_c128 *chi);
+
+/* ---- f4: screened interaction ------------------------------------------ */
+/* dynamical_screened_interaction_W(PI_wk, V_k) and (PI_wk, V_wk),
+ * dynamical_screened_interaction.hpp:44,82 / dynamical_screened_interaction.cpp
+ * :79-128 (static V: the _opt_bubble loops) and :33-74 (dynamic V): for every
+ * (frequency, k) point M_W = M_V inverse(1 - M_PI M_V), M_X[(a,b),(c,d)] =
+ * X_abcd (the grouping of group_indices_view on the row-major tensors; the
+ * two overloads give the same matrices).  PI[nw][nk][n^4]; V[nk][n^4]
+ * (dynamic = 0) or [nw][nk][n^4] (dynamic = 1); out W[nw][nk][n^4].  The
+ * frequency mesh may be Matsubara or real (the real-frequency overloads,
+ * dynamical_screened_interaction.hpp:63,100, do the same per point).  n <= 4;
+ * a singular denominator sets the status word as for the RPA.               */
+int tprf_dynamical_screened_interaction_W(tprf_ctx *ctx, long nw, long nk, int norb, const tprf_c128 *PI,
+                                          const tprf_c128 *V, int dynamic, tprf_c128 *W);
 
 /* ---- analytic bubble ----------------------------------------------------- */
 /* lindhard_chi00(e_k, mesh, mu) and lindhard_chi00(e_k, mesh, beta, mu, delta),

@@ -44,6 +44,7 @@ def lib():
             "or_chi0_w0r_from_grt_PH": [d, i, p, i, p, p],
             "or_chi_w0r_from_chi_tr": [d, i, l, i, p, p],
             "or_solve_rpa_PH": [l, i, p, p, p],
+            "or_screened_interaction_W": [l, l, i, i, p, p, p],
             "or_lindhard_chi00": [d, d, i, p, i, p, p],
             "or_eigh": [i, p, p, p],
             "or_fft1d": [i, i, p, p],
@@ -256,6 +257,21 @@ def solve_rpa_PH(chi0, U):
     npts = chi0.size // n ** 4
     out = np.empty_like(chi0)
     _check(lib().or_solve_rpa_PH(npts, n, _p(chi0), _p(U), _p(out)), "solve_rpa")
+    return out
+
+
+def screened_interaction_W(PI, V, dynamic=False):
+    """dynamical_screened_interaction_W (dynamical_screened_interaction.cpp):
+    PI[..., nk, n, n, n, n] with V[nk, n, n, n, n] (static, :79-116) or V of
+    PI's shape (dynamic, :33-74)."""
+    PI = _c(PI)
+    V = _c(V)
+    n = PI.shape[-1]
+    nk = V.shape[0] if not dynamic else 1
+    npts = PI.size // n ** 4
+    out = np.empty_like(PI)
+    _check(lib().or_screened_interaction_W(npts, nk, n, 1 if dynamic else 0, _p(PI), _p(V), _p(out)),
+           "screened_interaction_W")
     return out
 
 

@@ -117,7 +117,7 @@ class StaticChainRef:
         x = torch.einsum("jrda,jrbc->jrabcd", Gt, Gm)
         chi0_r = self.t_to_w0(x)
         chi0_q = torch.einsum("r,rabcd->abcd", self.phase, chi0_r)
-        X = chi0_q.permute(0, 1, 3, 2).reshape(n * n, n * n)
-        MU = U.permute(0, 1, 3, 2).reshape(n * n, n * n)
+        X = chi0_q.reshape(n * n, n * n)
+        MU = U.reshape(n * n, n * n)
         chim = torch.linalg.solve(torch.eye(n * n, dtype=torch.complex128) - X @ MU, X)
-        return chim.reshape(n, n, n, n).permute(0, 1, 3, 2)
+        return chim.reshape(n, n, n, n)

@@ -687,9 +687,14 @@ int or_chi_w0r_from_chi_tr(double beta, int ntau, long nr, int n, const cplx *ch
 }
 
 /* ------------------------------------------------------------------------ */
-/* a7: c++/triqs_tprf/lattice/rpa.cpp:29-64.  PH grouping (0,1),(3,2) (:40,
- * :54-55): M_X[(a,b),(d,c)] = X_abcd;  M_chi = inverse(1 - M_chi0 M_U) M_chi0
- * (:57).                                                                     */
+/* a7: c++/triqs_tprf/lattice/rpa.cpp:29-64.  M_chi = inverse(1 - M_chi0 M_U)
+ * M_chi0 (:57) on the matrices of group_indices_view(X, idx_group<0,1>,
+ * idx_group<3,2>) (:40, :54-55).  A view of a row-major tensor can only merge
+ * the pair (c, d) in memory order, so M_X[(a,b),(c,d)] = X_abcd; the
+ * reference's own dynamical_screened_interaction_w.py test, whose exchange
+ * entries (0,1,0,1) and (1,0,1,0) differ, holds only with this grouping (the
+ * same nda call there).  For U_abcd = U_badc (every real Hermitian
+ * interaction) it coincides with M[(a,b),(d,c)].                            */
 int or_solve_rpa_PH(long npts, int n, const cplx *chi0, const cplx *U, cplx *chi) {
   if (npts < 1 || n < 1 || n > 4) return OR_ERR_ARG;
   const int N = n * n;
@@ -698,7 +703,7 @@ int or_solve_rpa_PH(long npts, int n, const cplx *chi0, const cplx *U, cplx *chi
   for (int a = 0; a < n; ++a)
     for (int b = 0; b < n; ++b)
       for (int c = 0; c < n; ++c)
-        for (int d = 0; d < n; ++d) MU[(a * n + b) * N + (d * n + c)] = U[((a * n + b) * n + c) * n + d];
+        for (int d = 0; d < n; ++d) MU[(a * n + b) * N + (c * n + d)] = U[((a * n + b) * n + c) * n + d];
   int status = OR_OK;
 #pragma omp parallel for schedule(static)
   for (long p = 0; p < npts; ++p) {
@@ -707,7 +712,7 @@ int or_solve_rpa_PH(long npts, int n, const cplx *chi0, const cplx *U, cplx *chi
     for (int a = 0; a < n; ++a)
       for (int b = 0; b < n; ++b)
         for (int c = 0; c < n; ++c)
-          for (int d = 0; d < n; ++d) X[(a * n + b) * N + (d * n + c)] = x[((a * n + b) * n + c) * n + d];
+          for (int d = 0; d < n; ++d) X[(a * n + b) * N + (c * n + d)] = x[((a * n + b) * n + c) * n + d];
     for (int i = 0; i < N; ++i)
       for (int j = 0; j < N; ++j) {
         cplx s = (i == j) ? 1.0 : 0.0;
@@ -723,11 +728,55 @@ int or_solve_rpa_PH(long npts, int n, const cplx *chi0, const cplx *U, cplx *chi
       for (int b = 0; b < n; ++b)
         for (int c = 0; c < n; ++c)
           for (int d = 0; d < n; ++d) {
-            int i = a * n + b, j = d * n + c;
+            int i = a * n + b, j = c * n + d;
             cplx s = 0;
             for (int l = 0; l < N; ++l) s += Ainv[i * N + l] * X[l * N + j];
             o[((a * n + b) * n + c) * n + d] = s;
           }
+  }
+  return status;
+}
+
+/* ------------------------------------------------------------------------ */
+/* f4: dynamical_screened_interaction_W, c++/triqs_tprf/lattice/
+ * dynamical_screened_interaction.cpp.  Per (frequency, k) point:
+ *   dynamic = 0 (static V_k, dynamical_screened_interaction_W_opt_bubble,
+ *   :79-116):  denom = 1 - sum_ef PI_abef V_efcd (:100-106), inv_denom its
+ *   matrix inverse (:108), W_abcd = sum_ef V_abef inv_denom_efcd (:110-114);
+ *   dynamic = 1 (V per point, screened_interaction_from_generic_susceptibility
+ *   <bubble>, :33-74): M_W = M_V inverse(1 - M_PI M_V) (:57-62).
+ * Matrices M_X[(a,b),(c,d)] = X_abcd: the grouping of group_indices_view(X,
+ * idx_group<0,1>, idx_group<3,2>) on a row-major tensor (see a7 above).
+ * V is [nk][n^4] (dynamic = 0, point p uses V[p % nk]) or [npts][n^4].     */
+int or_screened_interaction_W(long npts, long nk, int n, int dynamic, const cplx *PI, const cplx *V, cplx *W) {
+  if (npts < 1 || nk < 1 || n < 1 || n > 4) return OR_ERR_ARG;
+  const int N = n * n;
+  const long n4 = (long)N * N;
+  int status = OR_OK;
+#pragma omp parallel for schedule(static)
+  for (long p = 0; p < npts; ++p) {
+    const cplx *pi = PI + p * n4;
+    const cplx *v = V + (dynamic ? p : p % nk) * n4;
+    cplx *w = W + p * n4;
+    cplx D[256], Dinv[256];
+    /* both paths: denominator 1 - M_PI M_V, then M_W = M_V inverse(denominator)
+     * (the static path writes the same sums as explicit index loops) */
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j < N; ++j) {
+        cplx s = (i == j) ? 1.0 : 0.0;
+        for (int l = 0; l < N; ++l) s -= pi[i * N + l] * v[l * N + j];
+        D[i * N + j] = s;
+      }
+    if (gj_inverse(N, D, Dinv) != OR_OK) {
+#pragma omp atomic write
+      status = OR_ERR_SINGULAR;
+    }
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j < N; ++j) {
+        cplx s = 0;
+        for (int l = 0; l < N; ++l) s += v[i * N + l] * Dinv[l * N + j];
+        w[i * N + j] = s;
+      }
   }
   return status;
 }

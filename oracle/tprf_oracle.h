@@ -61,6 +61,8 @@ int or_chi0_w0r_from_grt_PH(double beta, int ntau, const int dims[3], int n, con
 int or_chi_w0r_from_chi_tr(double beta, int ntau, long nr, int n, const or_c128 *chi_tr, or_c128 *chi_wr);
 /* a7 */
 int or_solve_rpa_PH(long npts, int n, const or_c128 *chi0, const or_c128 *U, or_c128 *chi);
+/* f4: dynamical_screened_interaction_W (static V: nk entries, dynamic: per point) */
+int or_screened_interaction_W(long npts, long nk, int n, int dynamic, const or_c128 *PI, const or_c128 *V, or_c128 *W);
 /* closed-form checker: Lindhard bubble (bosonic mesh n_max = nw) */
 int or_lindhard_chi00(double beta, double mu, int nw, const int dims[3], int n, const or_c128 *e_k,
                       or_c128 *chi_wk);

@@ -241,10 +241,11 @@ def test_rpa_point_adjoint_vs_torch(T, n):
     cbar = rnd(rng, npts, n, n, n, n)
     c0 = T.tensor(chi0, requires_grad=True)
     Ut = T.tensor(U, requires_grad=True)
-    X = c0.permute(0, 1, 2, 4, 3).reshape(npts, n * n, n * n)
-    MU = Ut.permute(0, 1, 2, 4, 3).reshape(npts, n * n, n * n)
+    # M_X[(a,b),(c,d)] = X_abcd (the grouping of rpa.cpp:40,54-55)
+    X = c0.reshape(npts, n * n, n * n)
+    MU = Ut.reshape(npts, n * n, n * n)
     chim = T.linalg.solve(T.eye(n * n, dtype=T.complex128) - X @ MU, X)
-    chi_ref = chim.reshape(npts, n, n, n, n).permute(0, 1, 2, 4, 3)
+    chi_ref = chim.reshape(npts, n, n, n, n)
     (T.conj(T.tensor(cbar)) * chi_ref).sum().real.backward()
     c = L.ctx()
     chi, c0bar, Ubar = (T.empty((npts, n, n, n, n), dtype=T.complex128, device="cuda") for _ in range(3))

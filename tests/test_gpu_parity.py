@@ -359,9 +359,10 @@ def test_chi0_w0r_trapz(L, oracle, gtr_case):
 # ---------------------------------------------------------------- a7 RPA
 @pytest.mark.parametrize("n,nb,nk", [(1, 5, 7), (2, 4, 13), (3, 6, 29), (3, 1, 40), (4, 2, 5)])
 def test_rpa_herm(L, oracle, n, nb, nk):
-    """tprf_solve_rpa_PH_herm: chi0 with M_chi0(-nu) = M_chi0(nu)^H and a
-    Hermitian M_U; the nu >= 0 half is solved and chi(-nu) written as the
-    conjugate transpose, against the oracle's full solve of every point."""
+    """tprf_solve_rpa_PH_herm: chi0 with chi0_abcd(-nu) = conj(chi0_dcba(nu))
+    (M(-nu) = M(nu)^H for M[(a,b),(d,c)] = X_abcd) and U Hermitian with
+    U_abcd = U_badc; the nu >= 0 half is solved and chi(-nu) written from the
+    same identity, against the oracle's full solve of every point."""
     import torch
     from tprf_amd import _lib
     rng = np.random.default_rng(n * 31 + nb)
@@ -376,6 +377,7 @@ def test_rpa_herm(L, oracle, n, nb, nk):
     chi0 = from_M(Mfull)
     MU = rnd(rng, n2, n2) * 0.4
     U = from_M(0.5 * (MU + MU.conj().T))
+    U = 0.5 * (U + U.transpose(1, 0, 3, 2))
     ref = oracle.solve_rpa_PH(chi0, U)
     x, u = gpu(chi0), gpu(U)
     out = torch.full_like(x, float("nan"))
@@ -530,8 +532,8 @@ def test_chain_plain_square(L, oracle):
 def rpa_condition(chi0_wk, U):
     """cond(1 - M_chi0 M_U) per (nu, q) point (2-norm)."""
     n = U.shape[0]
-    X = chi0_wk.reshape(-1, n, n, n, n).transpose(0, 1, 2, 4, 3).reshape(-1, n * n, n * n)
-    MU = U.transpose(0, 1, 3, 2).reshape(n * n, n * n)
+    X = chi0_wk.reshape(-1, n * n, n * n)
+    MU = U.reshape(n * n, n * n)
     s = np.linalg.svd(np.eye(n * n)[None] - X @ MU, compute_uv=False)
     return s[:, 0] / s[:, -1]
 
@@ -727,3 +729,57 @@ def test_lindhard_matches_tau_route_at_size(L):
 def MeshProduct_(wmesh, dims):
     from tprf_amd.gf import MeshCycLat, MeshProduct
     return MeshProduct(wmesh, MeshCycLat(dims))
+
+
+# ---------------------------------------------------------------- f4 screened interaction
+def test_screened_W_reference_kat(L, oracle):
+    """dynamical_screened_interaction_w.py:98-198 inputs (2 orbitals, exchange
+    entries (0,1,0,1) != (1,0,1,0)) through lattice.dynamical_screened_interaction_W:
+    static V_k and dynamic V_wk give W = V / (1 - PI V) entry by entry."""
+    from tprf_amd.gf import Gf, MeshBrZone, MeshImFreq, MeshProduct
+    n, nk, nw = 2, 8, 5
+    kmesh = MeshBrZone((nk, 1, 1))
+    kn = np.linalg.norm(kmesh.values(), axis=1)
+    V = np.zeros((nk, n, n, n, n), complex)
+    V[:, 0, 0, 0, 0] = 5.0 / (kn + 0.01)
+    V[:, 0, 1, 0, 1] = 2.0 / (kn + 0.05)
+    V[:, 1, 0, 1, 0] = 1.0 / (kn + 0.1)
+    V[:, 1, 1, 1, 1] = 7.0 / (kn + 0.06)
+    PI = np.zeros((2 * nw - 1, nk, n, n, n, n), complex)
+    PI[..., 0, 0, 0, 0] = -5.0
+    PI[..., 1, 0, 1, 0] = -2.0
+    PI[..., 0, 1, 0, 1] = -23.0
+    PI[..., 1, 1, 1, 1] = -12.0
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ref = np.where(V[None] == 0, 0, V[None] / (1.0 - PI * V[None]))
+    wk = MeshProduct(MeshImFreq(1000.0, "Boson", nw), kmesh)
+    PI_wk = Gf(wk, data=gpu(PI))
+    W1 = L.dynamical_screened_interaction_W(PI_wk, Gf(kmesh, data=gpu(V)))
+    assert relerr(host(W1.data), ref) <= 1e-13
+    W2 = L.dynamical_screened_interaction_W(PI_wk, Gf(wk, data=gpu(np.broadcast_to(V[None], PI.shape).copy())))
+    assert relerr(host(W2.data), ref) <= 1e-13
+    with pytest.raises(RuntimeError, match="k-space meshes are not the same"):
+        L.dynamical_screened_interaction_W(PI_wk, Gf(MeshBrZone((4, 1, 1)), data=gpu(V[:4])))
+
+
+@pytest.mark.parametrize("n,nw,nk", [(1, 7, 9), (2, 5, 11), (3, 4, 13), (3, 1, 64), (4, 3, 5)])
+def test_screened_W_vs_oracle(L, oracle, n, nw, nk):
+    """Random dense PI and V (static per k, and dynamic per point) against the
+    oracle's restatement of both overloads; odd point counts leave partly
+    filled waves."""
+    import torch
+    from tprf_amd import _lib
+    rng = np.random.default_rng(100 * n + nk)
+    PI = rnd(rng, nw, nk, n, n, n, n) * 0.25
+    V = rnd(rng, nk, n, n, n, n) * 0.25
+    Vw = rnd(rng, nw, nk, n, n, n, n) * 0.25
+    c = _lib.ctx()
+    for dynamic, v in ((0, V), (1, Vw)):
+        ref = oracle.screened_interaction_W(PI, v, dynamic=bool(dynamic))
+        out = torch.full((nw, nk, n, n, n, n), float("nan"), dtype=torch.complex128, device="cuda")
+        x, y = gpu(PI), gpu(v)
+        _lib.tail_status_reset(c)
+        _lib.call("tprf_dynamical_screened_interaction_W", c, nw, nk, n, _lib.ptr(x), _lib.ptr(y), dynamic,
+                  _lib.ptr(out))
+        _lib.tail_status(c)
+        assert relerr(host(out), ref) <= 1e-12, dynamic

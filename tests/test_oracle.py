@@ -274,16 +274,23 @@ def test_rpa_single_band(oracle):
 
 
 def test_rpa_vs_numpy_solve(oracle):
-    """PH grouping (a,b),(d,c) of rpa.cpp:40,54-55 written with numpy reshapes."""
+    """rpa.cpp:57 on the matrices M_X[(a,b),(c,d)] = X_abcd (the memory-order
+    grouping that group_indices_view gives, pinned by the screened-interaction
+    test below), with a random U that lacks the U_abcd = U_badc symmetry, so
+    the (a,b),(d,c) reading would fail."""
     rng = np.random.default_rng(2)
     n = 3
     chi0 = rnd(rng, 20, n, n, n, n) * 0.1
     U = rnd(rng, n, n, n, n) * 0.3
-    X = chi0.transpose(0, 1, 2, 4, 3).reshape(-1, n * n, n * n)
-    MU = U.transpose(0, 1, 3, 2).reshape(n * n, n * n)
+    X = chi0.reshape(-1, n * n, n * n)
+    MU = U.reshape(n * n, n * n)
     Y = np.linalg.solve(np.eye(n * n)[None] - X @ MU, X)
-    ref = Y.reshape(-1, n, n, n, n).transpose(0, 1, 2, 4, 3)
+    ref = Y.reshape(-1, n, n, n, n)
     assert relerr(oracle.solve_rpa_PH(chi0, U), ref) <= 1e-13
+    Xd = chi0.transpose(0, 1, 2, 4, 3).reshape(-1, n * n, n * n)
+    MUd = U.transpose(0, 1, 3, 2).reshape(n * n, n * n)
+    other = np.linalg.solve(np.eye(n * n)[None] - Xd @ MUd, Xd).reshape(-1, n, n, n, n).transpose(0, 1, 2, 4, 3)
+    assert relerr(other, ref) > 1e-3
 
 
 def test_rpa_fazekas_spinful_hubbard(oracle):
@@ -321,3 +328,54 @@ def test_workload_filling_and_hermiticity():
     assert abs(workloads.fermi(wl.beta * (ev - wl.mu)).mean() - 1 / 3) < 1e-10
     sq = workloads.plain_square()
     assert sq.e_k.shape == (1024, 1, 1) and abs(sq.e_k[0, 0, 0] + 4) < 1e-15
+
+
+def _screened_kat_inputs(nk=8, nw=5):
+    """The 2-orbital inputs of the reference's dynamical_screened_interaction_w.py
+    test (:98-170): V_k and PI with exchange entries (0,1,0,1) != (1,0,1,0)."""
+    n = 2
+    kv = np.linspace(0.0, 2 * np.pi, nk, endpoint=False)
+    V = np.zeros((nk, n, n, n, n), complex)
+    V[:, 0, 0, 0, 0] = 5.0 / (kv + 0.01)
+    V[:, 0, 1, 0, 1] = 2.0 / (kv + 0.05)
+    V[:, 1, 0, 1, 0] = 1.0 / (kv + 0.1)
+    V[:, 1, 1, 1, 1] = 7.0 / (kv + 0.06)
+    PI = np.zeros((nw, nk, n, n, n, n), complex)
+    PI[..., 0, 0, 0, 0] = -5.0
+    PI[..., 1, 0, 1, 0] = -2.0
+    PI[..., 0, 1, 0, 1] = -23.0
+    PI[..., 1, 1, 1, 1] = -12.0
+    return V, PI
+
+
+def test_screened_W_reference_kat(oracle):
+    """dynamical_screened_interaction_w.py:172-198: W = V / (1 - PI V) entry by
+    entry for these inputs, for the static-V_k and the dynamic-V_wk overloads."""
+    V, PI = _screened_kat_inputs()
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ref = np.where(V[None] == 0, 0, V[None] / (1.0 - PI * V[None]))
+    assert relerr(oracle.screened_interaction_W(PI, V), ref) <= 1e-14
+    Vw = np.ascontiguousarray(np.broadcast_to(V[None], PI.shape))
+    assert relerr(oracle.screened_interaction_W(PI, Vw, dynamic=True), ref) <= 1e-14
+
+
+def test_screened_W_dyson_and_single_band(oracle):
+    """n = 1: W = V / (1 - PI V).  n = 3, random dense V and PI: W solves the
+    Dyson equation W_abcd = V_abcd + sum V_abef PI_efgh W_ghcd (in the matrix
+    grouping of the reference's code); a transposed PI fails it."""
+    rng = np.random.default_rng(11)
+    V1 = rnd(rng, 6, 1, 1, 1, 1)
+    P1 = rnd(rng, 4, 6, 1, 1, 1, 1) * 0.2
+    assert relerr(oracle.screened_interaction_W(P1, V1), V1[None] / (1 - P1 * V1[None])) <= 1e-14
+    n = 3
+    V = rnd(rng, 5, n, n, n, n) * 0.3
+    PI = rnd(rng, 4, 5, n, n, n, n) * 0.3
+    W = oracle.screened_interaction_W(PI, V)
+    rhs = V[None] + np.einsum("wkabef,wkefgh,wkghcd->wkabcd", np.broadcast_to(V[None], W.shape), PI, W)
+    assert relerr(W, rhs) <= 1e-12
+    Wt = oracle.screened_interaction_W(np.ascontiguousarray(PI.transpose(0, 1, 2, 3, 5, 4)), V)
+    rhs_t = V[None] + np.einsum("wkabef,wkefgh,wkghcd->wkabcd", np.broadcast_to(V[None], W.shape), PI, Wt)
+    assert relerr(Wt, rhs_t) > 1e-3
+    # the dynamic overload with V repeated over frequencies equals the static one
+    Vw = np.ascontiguousarray(np.broadcast_to(V[None], PI.shape))
+    assert relerr(oracle.screened_interaction_W(PI, Vw, dynamic=True), W) <= 1e-14

@@ -22,8 +22,9 @@ class BubbleRPA:
     runs as tprf_fourier_wr_to_tr_herm (columns (r, a, b) with r < -r, or r = -r
     and a <= b, transformed; G(tau, r) is then stored for r <= -r only)
     and the bubble as tprf_chi0_wr_from_grt_PH_herm (orbital pairs a <= b
-    transformed, the others formed from the symmetry); if U_abcd = conj(U_dcba) as well,
-    M_chi0(-nu, q) = M_chi0(nu, q)^H and M_chi(-nu, q) = M_chi(nu, q)^H, so
+    transformed, the others formed from the symmetry); if also U_abcd =
+    conj(U_dcba) and U_abcd = U_badc, chi0_abcd(-nu, q) = conj(chi0_dcba(nu, q))
+    and the same holds for chi_RPA, so
     the bubble writes and r -> q transforms the frequencies nu >= 0 only and
     tprf_solve_rpa_PH_herm solves those and writes chi(-nu) as the conjugate
     transposes (chi0_wr and chi0_wk then hold nu >= 0 only).  fused=False: one
@@ -37,8 +38,9 @@ class BubbleRPA:
         e = wl.e_k
         self.herm = bool(fused) and bool(np.abs(e - np.conj(np.swapaxes(e, 1, 2))).max() <= 1e-14 * max(1.0, np.abs(e).max()))
         u = wl.U
-        self.herm_nu = self.herm and bool(np.abs(u - np.conj(np.einsum("abcd->dcba", u))).max() <=
-                                          1e-14 * max(1.0, np.abs(u).max()))
+        tol_u = 1e-14 * max(1.0, np.abs(u).max())
+        self.herm_nu = (self.herm and bool(np.abs(u - np.conj(np.einsum("abcd->dcba", u))).max() <= tol_u)
+                        and bool(np.abs(u - np.einsum("abcd->badc", u)).max() <= tol_u))
         dev = torch.device(device)
         self.dev = dev
         n, nk = wl.norb, wl.nk

@@ -240,10 +240,7 @@ __global__ void __launch_bounds__(64) k_rpa_point(long npts, const c2 *__restric
   constexpr int M = N * N, n4 = M * M;
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= npts) return;
-  auto tidx = [](int row, int col) {  // M[(a,b),(d,c)] <-> tensor abcd
-    const int a = row / N, b = row % N, d = col / N, c = col % N;
-    return ((a * N + b) * N + c) * N + d;
-  };
+  auto tidx = [](int row, int col) { return row * M + col; };  // M[(a,b),(c,d)] <-> tensor abcd
   c2 X[M][M], MU[M][M], A[M][M], Ai[M][M];
 #pragma unroll
   for (int i = 0; i < M; ++i)

@@ -1057,6 +1057,18 @@ int tprf_solve_rpa_PH_herm(tprf_ctx *ctx, int nb, long nk, int norb, const tprf_
                       "rpa kernel (mirrored)");
 }
 
+int tprf_dynamical_screened_interaction_W(tprf_ctx *ctx, long nw, long nk, int norb, const tprf_c128 *PI,
+                                          const tprf_c128 *V, int dynamic, tprf_c128 *W) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (nw < 1 || nk < 1 || !PI || !V || !W || (dynamic != 0 && dynamic != 1))
+    return fail(ctx, TPRF_ERR_ARG, "dynamical_screened_interaction_W: bad argument");
+  if (norb < 1 || norb > 4) return fail(ctx, TPRF_ERR_UNSUPPORTED, "dynamical_screened_interaction_W: norb must be 1..4");
+  return check_launch(ctx,
+                      tprf::launch_screened_W(ctx->stream, nw * nk, nk, norb, dynamic, (const c2 *)PI, (const c2 *)V,
+                                              (c2 *)W, tprf::Status{ctx->d_status}),
+                      "screened interaction kernel");
+}
+
 // ---- differentiation -----------------------------------------------------
 
 int tprf_lattice_dyson_g_wk_batched(tprf_ctx *ctx, double beta, int n_iw, const double *mu, long nbatch, long nk,

@@ -151,6 +151,10 @@ hipError_t launch_chi_w0r(hipStream_t s, double beta, int ntau, long nr, int n, 
 hipError_t launch_rpa(hipStream_t s, long npts, int n, const c2 *chi0, const c2 *U, c2 *chi, Status st);
 // RPA at the nonnegative bosonic frequencies of chi0[2 nb - 1][nk][n^4], each
 // result also written at -nu as its conjugate transpose in the PH grouping
+// screened interaction W = V inverse(1 - PI V) per point (V per k when
+// dynamic = 0, per point otherwise)
+hipError_t launch_screened_W(hipStream_t s, long npts, long nk, int n, int dynamic, const c2 *PI, const c2 *V, c2 *W,
+                             Status st);
 hipError_t launch_rpa_mirror(hipStream_t s, long nb, long nk, int n, const c2 *chi0, const c2 *U, c2 *chi,
                              Status st);
 // bubble.hip: returns hipErrorNotSupported when (n, P) has no specialisation

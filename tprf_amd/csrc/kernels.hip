@@ -506,8 +506,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N == 3
   constexpr int NMW = (N2 * N2 + 63) / 64;  // words of the nonzero mask of M_U
   __shared__ c2 slot[4 * P][2 * N2];
   __shared__ unsigned long long sMask[NMW];
-  // M_U[l][j] with l = (a, b), j = (d, c) is U[((a N + b) N + c) N + d]
-  auto u_idx = [](int l, int j) { return l * N2 + (j % N) * N + j / N; };
+  // M_U[l][j] with l = (a, b), j = (c, d) is U[((a N + b) N + c) N + d]
+  auto u_idx = [](int l, int j) { return l * N2 + j; };
   // Bit l N2 + j of the mask: M_U[l][j] != 0.  Interactions of physical models
   // are sparse in the particle-hole grouping (Kanamori: 21 of 81 entries for
   // three orbitals); zero entries add exact zeros to 1 - X U and are skipped
@@ -541,7 +541,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N == 3
 #pragma unroll
     for (int c = 0; c < N; ++c)
 #pragma unroll
-      for (int d = 0; d < N; ++d) X[d * N + c] = row[c * N + d];
+      for (int d = 0; d < N; ++d) X[c * N + d] = row[c * N + d];
   } else {
 #pragma unroll
     for (int j = 0; j < N2; ++j) X[j] = c2{0, 0};
@@ -607,7 +607,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N == 3
 #pragma unroll
     for (int c = 0; c < N; ++c)
 #pragma unroll
-      for (int d = 0; d < N; ++d) row[c * N + d] = cmul(X[d * N + c], inv);
+      for (int d = 0; d < N; ++d) row[c * N + d] = cmul(X[c * N + d], inv);
     if constexpr (MIRROR) {
       // this lane's row (a, b) of M_chi(nu) is column (a, b) of M_chi(-nu):
       // chi_dcba(-nu) = conj(chi_abcd(nu)) for every (c, d)
@@ -618,10 +618,121 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N == 3
 #pragma unroll
         for (int c = 0; c < N; ++c)
 #pragma unroll
-          for (int d = 0; d < N; ++d) mp[((d * N + c) * N + b) * N + a] = cconj(cmul(X[d * N + c], inv));
+          for (int d = 0; d < N; ++d) mp[((d * N + c) * N + b) * N + a] = cconj(cmul(X[c * N + d], inv));
       }
     }
   }
+}
+
+// ------------------------------------------------------------------------
+// f4 screened interaction (dynamical_screened_interaction_W,
+// dynamical_screened_interaction.cpp:33-128): per (frequency, k) point
+//   M_W = M_V inverse(1 - M_PI M_V) = inverse(1 - M_V M_PI) M_V
+// (push-through), M_X[(a,b),(c,d)] = X_abcd.  The second form is the RPA
+// solve above with V as the right-hand side and PI, which changes from point
+// to point, in place of U: N2 lanes own a point, lane i holding row i of
+// [A | V]; the point's PI is staged in a wave-private LDS slot to form
+// A = 1 - V PI, and the pivot row goes through LDS.  V is [nk][n^4] and read
+// at k = pt % nk (static), or [npts][n^4] (dynamic).
+template <int N>
+__global__ void __launch_bounds__(256) k_screened_W(long npts, long nk, int dynamic, const c2 *__restrict__ PI,
+                                                   const c2 *__restrict__ V, c2 *__restrict__ W, double *status) {
+  constexpr int N2 = N * N;
+  constexpr int P = 64 / N2;  // points per wave
+  __shared__ c2 sPi[4 * P][N2 * N2];
+  __shared__ c2 slot[4 * P][2 * N2];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int gidx = lane / N2, i = lane - gidx * N2;
+  const int slot_id = wave * P + (gidx < P ? gidx : 0);
+  const long pt = ((long)blockIdx.x * 4 + wave) * P + gidx;
+  const bool valid = (gidx < P) && (pt < npts);
+  c2 X[N2], A[N2];
+  if (valid) {
+    const c2 *vrow = V + (dynamic ? pt : pt % nk) * N2 * N2 + i * N2;
+    const c2 *prow = PI + pt * N2 * N2 + i * N2;
+#pragma unroll
+    for (int j = 0; j < N2; ++j) {
+      X[j] = vrow[j];
+      sPi[slot_id][i * N2 + j] = prow[j];
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < N2; ++j) X[j] = c2{0.0, 0.0};
+  }
+  wave_lds_sync();
+  // A = 1 - V PI, row i
+#pragma unroll
+  for (int j = 0; j < N2; ++j) {
+    c2 s = (i == j) ? c2{1.0, 0.0} : c2{0.0, 0.0};
+    if (gidx < P) {
+#pragma unroll
+      for (int l = 0; l < N2; ++l) s = cfms(s, X[l], sPi[slot_id][l * N2 + j]);
+    }
+    A[j] = s;
+  }
+  bool used = false;
+  int myk = 0;
+  c2 pv = c2{1.0, 0.0};
+  bool singular = false;
+#pragma unroll
+  for (int k = 0; k < N2; ++k) {
+    // pivot: the upper 32 bits of |A[k]|^2 as a key, compared by shuffles
+    const int key = (!used && valid) ? (int)(__double_as_longlong(cabs2(A[k])) >> 32) : -1;
+    int best_i = 0, best = -2;
+#pragma unroll
+    for (int s = 0; s < N2; ++s) {
+      const int m = __shfl(key, (lane - i) + s, 64);
+      if (m > best) { best = m; best_i = s; }
+    }
+    const bool is_piv = (i == best_i);
+    if (valid && best <= 0) singular = true;
+    if (is_piv && gidx < P) {
+      used = true;
+      myk = k;
+      pv = A[k];
+#pragma unroll
+      for (int j = k; j < N2; ++j) slot[slot_id][j] = A[j];
+#pragma unroll
+      for (int j = 0; j < N2; ++j) slot[slot_id][N2 + j] = X[j];
+    }
+    wave_lds_sync();
+    if (!is_piv && gidx < P) {
+      const c2 f = cdiv(A[k], slot[slot_id][k]);
+#pragma unroll
+      for (int j = k + 1; j < N2; ++j) A[j] = cfms(A[j], f, slot[slot_id][j]);
+      A[k] = c2{0.0, 0.0};
+#pragma unroll
+      for (int j = 0; j < N2; ++j) X[j] = cfms(X[j], f, slot[slot_id][N2 + j]);
+    }
+    wave_lds_sync();
+  }
+  if (singular) status[2] = 1.0;
+  if (valid) {
+    const c2 inv = cdiv(c2{1.0, 0.0}, pv);
+    c2 *row = W + pt * N2 * N2 + myk * N2;
+#pragma unroll
+    for (int j = 0; j < N2; ++j) row[j] = cmul(X[j], inv);
+  }
+}
+
+hipError_t launch_screened_W(hipStream_t s, long npts, long nk, int n, int dynamic, const c2 *PI, const c2 *V, c2 *W,
+                             Status st) {
+  switch (n) {
+#define TPRF_SW_CASE(N)                                                                                  \
+  case N: {                                                                                              \
+    constexpr int P = 64 / (N * N);                                                                      \
+    const long nblk = (npts + 4 * P - 1) / (4 * P);                                                      \
+    hipLaunchKernelGGL(k_screened_W<N>, dim3(nblk), dim3(256), 0, s, npts, nk, dynamic, PI, V, W, st.d); \
+    break;                                                                                               \
+  }
+    TPRF_SW_CASE(1)
+    TPRF_SW_CASE(2)
+    TPRF_SW_CASE(3)
+    TPRF_SW_CASE(4)
+#undef TPRF_SW_CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_rpa_mirror(hipStream_t s, long nb, long nk, int n, const c2 *chi0, const c2 *U, c2 *chi,

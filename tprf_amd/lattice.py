@@ -242,7 +242,8 @@ def chi_w0r_from_chi_tr(chi_tr):
 # a7 RPA -- lattice_desc.py:547 -> rpa.cpp:29-68
 
 def solve_rpa_PH(chi0, U):
-    """chi = (1 - chi0 U)^-1 chi0 in the PH grouping (ab),(dc) per mesh point."""
+    """chi = (1 - chi0 U)^-1 chi0 per mesh point, on the matrices M[(a,b),(c,d)]
+    of rpa.cpp:40,54-55 (memory-order grouping of the row-major tensors)."""
     x = _dev(chi0.data)
     n = chi0.target_shape[0]
     u = _dev(U if not isinstance(U, Gf) else U.data)
@@ -255,6 +256,38 @@ def solve_rpa_PH(chi0, U):
     _lib.call("tprf_solve_rpa_PH", c, npts, n, _lib.ptr(x), _lib.ptr(u), _lib.ptr(out))
     _lib.tail_status(c)
     return Gf(chi0.mesh, data=out)
+
+
+def dynamical_screened_interaction_W(PI_wk, V_k):
+    """Screened interaction W_abcd(w, k) from the bubble PI_wk and the bare
+    interaction V (dynamical_screened_interaction.hpp:44,63,81,100; binding
+    lattice_desc.py:892-970): M_W = M_V (1 - M_PI M_V)^-1 per (w, k) point, with
+    M_X[(a,b),(c,d)] = X_abcd.  V_k is static (a Gf on the k mesh, or an
+    [nk, n, n, n, n] array) or dynamic (a Gf on PI_wk's (w, k) mesh).  The
+    frequency mesh may be Matsubara or real."""
+    pi = _dev(PI_wk.data)
+    if len(PI_wk.target_shape) != 4 or not isinstance(PI_wk.mesh, MeshProduct):
+        raise ValueError("PI_wk must be a rank-4 Gf on a (frequency, k) mesh")
+    n = PI_wk.target_shape[0]
+    wmesh, kmesh = PI_wk.mesh[0], PI_wk.mesh[1]
+    vdata = V_k.data if isinstance(V_k, Gf) else V_k
+    dynamic = isinstance(V_k, Gf) and isinstance(V_k.mesh, MeshProduct)
+    if isinstance(V_k, Gf):
+        vk = V_k.mesh[1] if dynamic else V_k.mesh
+        if getattr(vk, "dims", None) != getattr(kmesh, "dims", None):
+            raise RuntimeError("dynamical_screened_interaction_W: k-space meshes are not the same")
+    v = _dev(vdata)
+    nw, nk = pi.shape[0], pi.shape[1]
+    want = (nw, nk, n, n, n, n) if dynamic else (nk, n, n, n, n)
+    if tuple(v.shape) != want:
+        raise RuntimeError("dynamical_screened_interaction_W: k-space meshes are not the same")
+    c = _lib.ctx(pi.device.index)
+    out = torch.empty_like(pi)
+    _lib.tail_status_reset(c)
+    _lib.call("tprf_dynamical_screened_interaction_W", c, nw, nk, n, _lib.ptr(pi), _lib.ptr(v), 1 if dynamic else 0,
+              _lib.ptr(out))
+    _lib.tail_status(c)
+    return Gf(PI_wk.mesh, data=out)
 
 
 def lindhard_chi00(e_k, mesh, *args, mu=None, beta=None, delta=0.0):
