@@ -487,17 +487,18 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // ------------------------------------------------------------------------
-// a7 RPA: for each point, M_chi = inverse(1 - M_chi0 M_U) M_chi0 (rpa.cpp:57),
+// a7 RPA: for each point, M_chi = inverse(1 - M_chi0 M_U) M_chi0 (rpa.cpp:57)
+// with M_X[(a,b),(c,d)] = X_abcd (the memory-order grouping that
+// group_indices_view makes of the row-major tensors, rpa.cpp:40,54-55),
 // solved as (1 - X U) Y = X by Gauss-Jordan with partial pivoting.  A group
 // of N2 = n^2 lanes owns one point, lane i holding row i of [A | X]; the pivot
 // row is exchanged through LDS.
-// n = 3: registers held to four waves per SIMD (a few spills outside the
-// pivot loop cost less than the lost wave: 0.51 -> 0.47 ms at 127 x 4096 points)
+// n = 3: registers held to four waves per SIMD.
 // MIRROR: the points are the nb * nk points of the nonnegative bosonic
 // frequencies m = pt / nk >= 0 of arrays [2 nb - 1][nk][n^4] (read and written
-// at base + pt, base = (nb - 1) nk), and each result is also written at -m as
-// its conjugate transpose, M_chi(-nu, q) = M_chi(nu, q)^H, the identity that
-// holds when M_chi0(-nu, q) = M_chi0(nu, q)^H and M_U is Hermitian.
+// at base + pt, base = (nb - 1) nk), and each result is also written at -m
+// by chi_dcba(-nu, q) = conj(chi_abcd(nu, q)), the identity that holds when
+// chi0 obeys it and U_abcd = conj(U_dcba) = U_badc.
 template <int N, bool MIRROR = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N == 3 ? 4 : 1))) k_rpa(long npts, const c2 *__restrict__ chi0, const c2 *__restrict__ U,
                                              c2 *__restrict__ chi, double *status, long base = 0, long nk = 1) {
