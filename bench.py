@@ -33,8 +33,8 @@ FP64_PEAK_TFLOPS = 78.6
 # The kernel is the fused bubble (Hermitian form, the default chain's) in the
 # workgroup shape the autotuner kept there.
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
-MEASURED_TRAFFIC = {"chi0_wr": {"kernel": "k_bubble_pair<3,16,8,9,128,true,false,1,true>", "fetch_kib": 445642,
-                                "write_kib": 566242}}
+MEASURED_TRAFFIC = {"chi0_wr": {"kernel": "k_bubble_pair<3,16,8,15,256,true,false,1,true,true>", "fetch_kib": 342492,
+                                "write_kib": 391825}}
 
 
 def stage_model(wl):

@@ -49,19 +49,25 @@ namespace tprf {
 // the columns c <= d are transformed.
 // Outputs are written for the frequencies m_lo <= m < nb (m_lo = 1 - nb: all;
 // 0: nu >= 0 only).
-template <int N, int R1, int R2, int CG, int NT, bool AB, bool DMA, int WPE, bool HERM = false>
+// MG (HERM, n = 3): a workgroup owns the off-diagonal pair (a1, b1) = (0,1),
+// (0,2) or (1,2) and the diagonal pair (a2, a2) = (0,0), (1,1) or (2,2): 9 + 6
+// columns, three workgroups per r pair, so that the DFT units fill the lanes
+// of 256 threads; the tile holds G_da for a in {a1, a2} and G_eb for b in
+// {b1, a2}.
+template <int N, int R1, int R2, int CG, int NT, bool AB, bool DMA, int WPE, bool HERM = false, bool MG = false>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) k_bubble_pair(double beta, int L, int nb, int d0, int d1, int d2,
                                                     const c2 *__restrict__ gtr, c2 *__restrict__ out, BubbleDev bd,
                                                     Vinv8 vinv, int m_lo) {
   constexpr int P = R1 * R2;
   constexpr int n2 = N * N, n4 = n2 * n2;
-  static_assert(!AB || CG == n2, "one orbital pair per workgroup");
+  static_assert(!AB || CG == n2 || MG, "one orbital pair per workgroup");
+  static_assert(!MG || (HERM && N == 3 && CG == 15), "merged pairs: Hermitian form, three orbitals");
   static_assert(!HERM || AB, "the Hermitian form works on orbital-pair workgroups");
   static_assert(NT >= P, "the outer twiddles of a step are staged one per thread");
-  constexpr int NG = HERM ? N * (N + 1) / 2 : (n4 + CG - 1) / CG;
+  constexpr int NG = MG ? N : (HERM ? N * (N + 1) / 2 : (n4 + CG - 1) / CG);
   constexpr int U2 = (CG * R1 + NT - 1) / NT;
   constexpr int TLD = R2 + 1;  // padded LDS row of the transpose
-  constexpr int NE = AB ? N : n2;  // staged entries of G per tau point and side
+  constexpr int NE = MG ? 2 * N : (AB ? N : n2);  // staged entries of G per tau point and side
   constexpr int NSTAGE = P * NE;
   constexpr int PF = (NSTAGE + NT - 1) / NT;
   static_assert(CG * R1 * TLD >= CG * P, "transpose buffer reused for the final exchange");
@@ -95,18 +101,31 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
     while (rest >= N - a0) { rest -= N - a0; ++a0; }
     b0 = a0 + rest;
   }
+  int a2 = 0;  // MG: the diagonal pair's orbital
+  if constexpr (MG) {
+    a0 = grp < 2 ? 0 : 1;
+    b0 = grp == 0 ? 1 : 2;
+    a2 = grp;
+  }
   const int q0 = AB ? (a0 * N + b0) * n2 : grp * CG;
   // HERM, a == b: (a, a, c, d) and (a, a, d, c) are conjugate partners, so only
   // the n (n + 1) / 2 columns c <= d are transformed; column slot c' maps to
   // the c'-th pair c <= d in row-major order
-  const bool diag_pair = HERM && a0 == b0;
-  const int cn = diag_pair ? N * (N + 1) / 2 : ((n4 - q0) < CG ? (n4 - q0) : CG);
+  const bool diag_pair = HERM && !MG && a0 == b0;
+  const int cn = MG ? CG : (diag_pair ? N * (N + 1) / 2 : ((n4 - q0) < CG ? (n4 - q0) : CG));
   auto col_q = [&](int c) {
+    if (MG && c >= n2) {  // the diagonal pair's columns c <= d
+      int cc = 0, rest = c - n2;
+      while (rest >= N - cc) { rest -= N - cc; ++cc; }
+      return (a2 * N + a2) * n2 + cc * N + cc + rest;
+    }
     if (!diag_pair) return q0 + c;
     int cc = 0, rest = c;
     while (rest >= N - cc) { rest -= N - cc; ++cc; }
     return q0 + cc * N + cc + rest;
   };
+  // MG: the tile slot (0: a1 / b1, 1: a2) of column slot c
+  auto tslot = [&](int c) { return (MG && c >= n2) ? N : 0; };
   const int N1 = L / P;
   const double dtau = beta / (double)L;
 
@@ -156,10 +175,18 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
   const long stepa = nr * n2;  // one tau point
   auto off_a = [&](int t) {
     const int j2 = t / NE, e = t - j2 * NE;
+    if constexpr (MG) {
+      const int x = e % N, a = e < N ? a0 : a2;
+      return ((long)N1 * j2 * nr + r) * n2 + x * N + a;
+    }
     return ((long)N1 * j2 * nr + r) * n2 + (AB ? e * N + a0 : e);
   };
   auto off_b = [&](int t) {
     const int j2 = t / NE, e = t - j2 * NE;
+    if constexpr (MG) {
+      const int x = e % N, b = e < N ? b0 : a2;
+      return (((long)L - (long)N1 * j2) * nr + r) * n2 + x * N + b;  // conj(G_xb(r))
+    }
     if constexpr (HERM) return (((long)L - (long)N1 * j2) * nr + r) * n2 + e * N + b0;  // conj(G_eb(r))
     return (((long)L - (long)N1 * j2) * nr + mr) * n2 + (AB ? b0 * N + e : e);
   };
@@ -180,7 +207,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
 #pragma unroll
       for (int k = 0; k < R1; ++k) {
         const int j2 = R2 * k + s;
-        v[k] = HERM ? cmulc(tA[j2 * NE + dd], tB[j2 * NE + cc])
+        v[k] = HERM ? cmulc(tA[j2 * NE + tslot(c) + dd], tB[j2 * NE + tslot(c) + cc])
              : AB ? cmul(tA[j2 * NE + dd], tB[j2 * NE + cc])
                   : cmul(tA[j2 * n2 + dd * N + aa], tB[j2 * n2 + bb * N + cc]);
       }
@@ -375,7 +402,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
           out[((long)im * nr + mr) * n4 + qm] = v;
         }
         if constexpr (HERM) {
-          if (a0 < b0 || cc != dd) {
+          if (aa < bb || cc != dd) {
             // column badc of r: S = (S(m) - x_0 + x_L)^*, pole coefficients conj(a1)
             const int qb = ((bb * N + aa) * N + dd) * N + cc;
             const c2 sb = cconj(cadd(acc[i][k2], xl_minus_x0));
@@ -397,19 +424,19 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
   }
 }
 
-template <int N, int R1, int R2, int CG, int NT, bool AB, bool DMA, int WPE, bool HERM>
+template <int N, int R1, int R2, int CG, int NT, bool AB, bool DMA, int WPE, bool HERM, bool MG = false>
 static hipError_t launch_one_h(hipStream_t s, double beta, int L, int nb, const int dims[3], const c2 *gtr, c2 *out,
                                const BubbleDev &bd, const Vinv8 &vinv, int m_lo) {
   constexpr int n4 = N * N * N * N;
-  constexpr int NG = HERM ? N * (N + 1) / 2 : (n4 + CG - 1) / CG;
+  constexpr int NG = MG ? N : (HERM ? N * (N + 1) / 2 : (n4 + CG - 1) / CG);
   const long nb_blocks = (long)bd.ncanon * NG;
   if (DMA && L / (R1 * R2) < 2) {  // the global -> LDS pipeline needs two tiles
-    hipLaunchKernelGGL((k_bubble_pair<N, R1, R2, CG, NT, AB, false, 1, HERM>), dim3(nb_blocks), dim3(NT), 0, s, beta, L,
-                       nb, dims[0], dims[1], dims[2], gtr, out, bd, vinv, m_lo);
+    hipLaunchKernelGGL((k_bubble_pair<N, R1, R2, CG, NT, AB, false, 1, HERM, MG>), dim3(nb_blocks), dim3(NT), 0, s, beta,
+                       L, nb, dims[0], dims[1], dims[2], gtr, out, bd, vinv, m_lo);
     return hipGetLastError();
   }
-  hipLaunchKernelGGL((k_bubble_pair<N, R1, R2, CG, NT, AB, DMA, WPE, HERM>), dim3(nb_blocks), dim3(NT), 0, s, beta, L,
-                     nb, dims[0], dims[1], dims[2], gtr, out, bd, vinv, m_lo);
+  hipLaunchKernelGGL((k_bubble_pair<N, R1, R2, CG, NT, AB, DMA, WPE, HERM, MG>), dim3(nb_blocks), dim3(NT), 0, s, beta,
+                     L, nb, dims[0], dims[1], dims[2], gtr, out, bd, vinv, m_lo);
   return hipGetLastError();
 }
 
@@ -429,18 +456,20 @@ static hipError_t launch_one(hipStream_t s, double beta, int L, int nb, const in
 
 bool bubble_pair_supported(int n, int P) { return (n == 2 || n == 3) && (P == 64 || P == 128); }
 
-int bubble_pair_variants() { return 5; }
+int bubble_pair_variants() { return 6; }
 
 // the variants with orbital-pair workgroups (those the Hermitian form has)
 int bubble_pair_herm_variants(int n, int P, int *list) {
-  static const int v3[] = {0, 1, 3, 4}, v2a[] = {0, 3}, v2b[] = {0, 1, 3, 4};
+  static const int v3[] = {0, 1, 3, 4, 5}, v2a[] = {0, 3}, v2b[] = {0, 1, 3, 4};
   const int *v = (n == 3) ? v3 : (P == 128 ? v2a : v2b);
-  const int cnt = (n == 3) ? 4 : (P == 128 ? 2 : 4);
+  const int cnt = (n == 3) ? 5 : (P == 128 ? 2 : 4);
   for (int i = 0; i < cnt; ++i) list[i] = v[i];
   return cnt;
 }
 
-// Workgroup shapes (same results, picked by the autotuner): variant 0 owns one
+// Workgroup shapes (same results, picked by the autotuner; variant 5, the
+// merged orbital pairs of the Hermitian form at n = 3, runs as 0 elsewhere):
+// variant 0 owns one
 // orbital pair; 1 is a narrower or differently grouped shape; 2 stages every G
 // entry for CG = n^4 / 3 (n = 3) or n^4 (n = 2) columns; 3 and 4 load tiles
 // straight into LDS (for n = 3, P = 128 the two differ in the occupancy the
@@ -448,6 +477,13 @@ int bubble_pair_herm_variants(int n, int P, int *list) {
 hipError_t launch_bubble_pair(hipStream_t s, double beta, int L, int nb, const int dims[3], int n, const c2 *gtr,
                               c2 *out, const BubbleDev &bd, const Vinv8 &vinv, int variant, bool herm, bool nonneg) {
   const int m_lo = nonneg ? 0 : 1 - nb;
+  if (variant == 5) {  // merged orbital pairs: the Hermitian form at n = 3 only
+    if (herm && n == 3 && bd.P == 128)
+      return launch_one_h<3, 16, 8, 15, 256, true, false, 1, true, true>(s, beta, L, nb, dims, gtr, out, bd, vinv, m_lo);
+    if (herm && n == 3 && bd.P == 64)
+      return launch_one_h<3, 8, 8, 15, 256, true, false, 1, true, true>(s, beta, L, nb, dims, gtr, out, bd, vinv, m_lo);
+    variant = 0;
+  }
 #define TPRF_BP(N_, R1_, R2_, CG_, NT_, AB_, DMA_, WPE_) \
   return launch_one<N_, R1_, R2_, CG_, NT_, AB_, DMA_, WPE_>(s, beta, L, nb, dims, gtr, out, bd, vinv, herm, m_lo)
   if (n == 3 && bd.P == 128) {
