@@ -336,7 +336,7 @@ def test_chi0_wr_pruned_pair_kernel(L, oracle, n, n_iw, nb, dims, ntau):
             _lib.call("tprf_ctx_set_option", c, 1, 0)
         assert relerr(host(got.data), ref) <= TOL, f"generic={generic}"
     # every workgroup shape the autotuner may pick (option 2 fixes it, -1 restores tuning)
-    for variant in range(6):
+    for variant in range(7):
         _lib.call("tprf_ctx_set_option", c, 2, variant)
         try:
             got = L.chi0_wr_from_grt_PH(gtr_gf(dims, beta, gt), nw=nb)
@@ -431,7 +431,7 @@ def test_chi0_wr_herm(L, oracle, dims, n, nb):
     g_half = g_tr.copy()
     g_half[:, ~canonical_r(dims)] = np.nan
     gt = gpu(g_half)
-    for variant in (0, 1, 3, 4, 5, -1):
+    for variant in (0, 1, 3, 4, 5, 6, -1):
         _lib.call("tprf_ctx_set_option", c, 2, variant)
         try:
             for full in (1, 0):

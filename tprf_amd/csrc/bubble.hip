@@ -456,19 +456,20 @@ static hipError_t launch_one(hipStream_t s, double beta, int L, int nb, const in
 
 bool bubble_pair_supported(int n, int P) { return (n == 2 || n == 3) && (P == 64 || P == 128); }
 
-int bubble_pair_variants() { return 6; }
+int bubble_pair_variants() { return 7; }
 
 // the variants with orbital-pair workgroups (those the Hermitian form has)
 int bubble_pair_herm_variants(int n, int P, int *list) {
-  static const int v3[] = {0, 1, 3, 4, 5}, v2a[] = {0, 3}, v2b[] = {0, 1, 3, 4};
-  const int *v = (n == 3) ? v3 : (P == 128 ? v2a : v2b);
-  const int cnt = (n == 3) ? 5 : (P == 128 ? 2 : 4);
+  static const int v3a[] = {0, 1, 3, 4, 5, 6}, v3b[] = {0, 1, 3, 4, 5}, v2a[] = {0, 3}, v2b[] = {0, 1, 3, 4};
+  const int *v = (n == 3) ? (P == 128 ? v3a : v3b) : (P == 128 ? v2a : v2b);
+  const int cnt = (n == 3) ? (P == 128 ? 6 : 5) : (P == 128 ? 2 : 4);
   for (int i = 0; i < cnt; ++i) list[i] = v[i];
   return cnt;
 }
 
-// Workgroup shapes (same results, picked by the autotuner; variant 5, the
-// merged orbital pairs of the Hermitian form at n = 3, runs as 0 elsewhere):
+// Workgroup shapes (same results, picked by the autotuner; variants 5 and 6,
+// the merged orbital pairs of the Hermitian form at n = 3 with the inner DFT
+// split 16 x 8 or 8 x 16, run as 0 elsewhere):
 // variant 0 owns one
 // orbital pair; 1 is a narrower or differently grouped shape; 2 stages every G
 // entry for CG = n^4 / 3 (n = 3) or n^4 (n = 2) columns; 3 and 4 load tiles
@@ -477,9 +478,13 @@ int bubble_pair_herm_variants(int n, int P, int *list) {
 hipError_t launch_bubble_pair(hipStream_t s, double beta, int L, int nb, const int dims[3], int n, const c2 *gtr,
                               c2 *out, const BubbleDev &bd, const Vinv8 &vinv, int variant, bool herm, bool nonneg) {
   const int m_lo = nonneg ? 0 : 1 - nb;
-  if (variant == 5) {  // merged orbital pairs: the Hermitian form at n = 3 only
-    if (herm && n == 3 && bd.P == 128)
+  if (variant == 5 || variant == 6) {  // merged orbital pairs: the Hermitian form at n = 3 only
+    if (herm && n == 3 && bd.P == 128) {
+      if (variant == 6)  // inner DFT split 8 x 16 instead of 16 x 8
+        return launch_one_h<3, 8, 16, 15, 256, true, false, 1, true, true>(s, beta, L, nb, dims, gtr, out, bd, vinv,
+                                                                            m_lo);
       return launch_one_h<3, 16, 8, 15, 256, true, false, 1, true, true>(s, beta, L, nb, dims, gtr, out, bd, vinv, m_lo);
+    }
     if (herm && n == 3 && bd.P == 64)
       return launch_one_h<3, 8, 8, 15, 256, true, false, 1, true, true>(s, beta, L, nb, dims, gtr, out, bd, vinv, m_lo);
     variant = 0;

@@ -962,7 +962,7 @@ int chi0_wr_from_grt(tprf_ctx *ctx, double beta, int ntau, const int dims[3], in
     if (!herm)
       return tuned_launch(ctx, ctx->bubble_tuned, key, tprf::bubble_pair_variants(), ctx->bubble_variant, launch,
                           "bubble pair kernel");
-    int list[5];
+    int list[8];
     const int cnt = tprf::bubble_pair_herm_variants(norb, P, list);
     auto launch_h = [&](int i) {
       return tprf::launch_bubble_pair(ctx->stream, beta, L, nw, dims, norb, (const c2 *)g_tr, (c2 *)chi0_wr, *bd,

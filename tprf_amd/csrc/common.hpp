@@ -167,7 +167,7 @@ hipError_t launch_bubble_pair(hipStream_t s, double beta, int L, int nb, const i
                               bool nonneg = false);
 bool bubble_pair_supported(int n, int P);
 int bubble_pair_variants();
-// variants usable with herm, written to list[0..count), count returned (<= 5)
+// variants usable with herm, written to list[0..count), count returned (<= 8)
 int bubble_pair_herm_variants(int n, int P, int *list);
 // w2t.hip: returns hipErrorNotSupported when M has no specialisation
 // variant < w2t_pruned_variants() selects the workgroup width (same results)
