@@ -202,6 +202,40 @@ def batched_leg(reps=3):
             "max_rel_diff_batched_vs_plain": diff}
 
 
+def screened_w_leg(reps=20):
+    """dynamical_screened_interaction_W at the chain's size (3 orbitals, 16^3 k,
+    127 bosonic frequencies, static V_k): mean kernel time over `reps` calls by
+    HIP events, and its HBM rate on the bytes it must move (PI and W once,
+    V_k once)."""
+    import numpy as np
+    import torch
+    from tprf_amd import _lib
+    n, nk, nw = 3, 16 ** 3, 127
+    rng = np.random.default_rng(3)
+    shape = (nw, nk, n, n, n, n)
+    pi = torch.from_numpy((rng.normal(size=shape) + 1j * rng.normal(size=shape)) * 0.05).cuda()
+    v = torch.from_numpy((rng.normal(size=shape[1:]) + 1j * rng.normal(size=shape[1:])) * 0.3).cuda()
+    w = torch.empty_like(pi)
+    c = _lib.ctx()
+    call = lambda: _lib.call("tprf_dynamical_screened_interaction_W", c, nw, nk, n, _lib.ptr(pi), _lib.ptr(v), 0,
+                             _lib.ptr(w))
+    for _ in range(3):
+        call()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        call()
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    nbytes = 2 * pi.numel() * 16 + v.numel() * 16
+    _lib.tail_status(c)
+    return {"workload": "3 orbitals, 16^3 k, 127 bosonic freqs, static V_k", "ms": ms,
+            "points_per_s": nw * nk / (ms * 1e-3), "hbm_gbs": nbytes / (ms * 1e-3) / 1e9,
+            "hbm_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+
 def uneven_leg(reps=3):
     """batched/plain at the uneven batch: 48 lattices drawn from the seed with
     L^3 k (L in {8, 12, 16}), norb in {1, 2, 3}, n_iw in {128, 256, 512}, beta in
@@ -292,7 +326,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-extra", action="store_true", help="skip the grad/fwd and batched/plain legs")
-    ap.add_argument("--only", choices=["grad", "batched", "uneven", "calibration"],
+    ap.add_argument("--only", choices=["grad", "batched", "uneven", "screened_w", "calibration"],
                     help="run one leg alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -309,7 +343,8 @@ def main():
             calibration_run()
             print(json.dumps({"calibration": "done"}))
         else:
-            print(json.dumps({"grad": grad_leg, "batched": batched_leg, "uneven": uneven_leg}[args.only]()))
+            print(json.dumps({"grad": grad_leg, "batched": batched_leg, "uneven": uneven_leg,
+                              "screened_w": screened_w_leg}[args.only]()))
         return
     wl = workloads.cubic_3orb(seed=args.seed)
     ch = chain.BubbleRPA(wl, device="cuda:0")
