@@ -229,6 +229,17 @@ int tprf_dynamical_screened_interaction_W(tprf_ctx *ctx, long nw, long nk, int n
  * O(N_k^2) work.  n <= 4.                                                    */
 int tprf_lindhard_chi00(tprf_ctx *ctx, double beta, double mu, long nz, const tprf_c128 *z, const int dims[3],
                         int norb, const tprf_c128 *e_k, tprf_c128 *chi);
+/* No counterpart in the reference.  tprf_lindhard_chi00 for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc): the same kernels and the same result, with
+ * the workspace (eigenvalues and band projectors, nk (8 n + 16 n^3) bytes) in
+ * a fixed 32 MiB file-scope device array of the library instead of the
+ * context's allocated scratch.  The call allocates nothing and frees nothing;
+ * larger sizes return TPRF_ERR_UNSUPPORTED.  Each GPU holds one such array
+ * until the process ends, shared by every call on that GPU: calls that
+ * overlap on one GPU (other streams, other contexts or threads) race.        */
+int tprf_lindhard_chi00_noalloc(tprf_ctx *ctx, double beta, double mu, long nz, const tprf_c128 *z,
+                                const int dims[3], int norb, const tprf_c128 *e_k, tprf_c128 *chi);
 
 /* ---- differentiation: static chi_RPA(Q, i nu = 0) and its adjoints ------
  * No counterpart in the reference (TPRF has no derivatives).  Each adjoint is

@@ -57,6 +57,7 @@ _SIGNATURES = {
     "tprf_solve_rpa_PH_herm": [_p, _i, _l, _i, _p, _p, _p],
     "tprf_dynamical_screened_interaction_W": [_p, _l, _l, _i, _p, _p, _i, _p],
     "tprf_lindhard_chi00": [_p, _d, _d, _l, _p, _p, _i, _p, _p],
+    "tprf_lindhard_chi00_noalloc": [_p, _d, _d, _l, _p, _p, _i, _p, _p],
     "tprf_lattice_dyson_g_wk_batched": [_p, _d, _i, _p, _l, _l, _i, _p, _p],
     "tprf_lattice_dyson_g_wk_adjoint": [_p, _i, _l, _i, _l, _p, _p, _p, _p],
     "tprf_fourier_wk_to_wr_adjoint": [_p, _l, _p, _l, _p, _p],

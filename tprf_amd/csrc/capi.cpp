@@ -66,6 +66,7 @@ struct tprf_ctx {
   std::map<std::tuple<double, int>, double *> static_w;  // nu = 0 weights c_j
   std::map<std::tuple<int, double, int, int>, double *> dtail;  // out-of-mesh pole model in tau (real)
   std::map<std::tuple<int, double, int, int>, c2 *> otab;       // per-tau output records of omega -> tau
+  void *scratch = nullptr;  // this device's copy of the no-allocation entry point's file-scope array
   void *work = nullptr;                                   // grow-only scratch
   size_t work_bytes = 0;
   int bubble_variant = -1;                                // -1: autotuned per shape
@@ -643,6 +644,7 @@ int tprf_ctx_create(tprf_ctx **out, int device) {
   if (e == hipSuccess) e = hipMalloc((void **)&ctx->d_status, 4 * sizeof(double));
   if (e == hipSuccess) e = hipMemset(ctx->d_status, 0, 4 * sizeof(double));
   if (e == hipSuccess) e = tprf::set_kernel_attributes();
+  if (e == hipSuccess) e = tprf::noalloc_scratch_address(device, &ctx->scratch);
   if (e != hipSuccess) {
     delete ctx;
     return TPRF_ERR_HIP;
@@ -1043,6 +1045,28 @@ int tprf_lindhard_chi00(tprf_ctx *ctx, double beta, double mu, long nz, const tp
   return check_launch(ctx,
                       tprf::launch_lindhard(ctx->stream, beta, mu, nz, (const c2 *)z, dims, norb, (const c2 *)e_k,
                                             (double *)w, (c2 *)((char *)w + ev_bytes), (c2 *)chi),
+                      "lindhard kernels");
+}
+
+int tprf_lindhard_chi00_noalloc(tprf_ctx *ctx, double beta, double mu, long nz, const tprf_c128 *z, const int dims[3],
+                                int norb, const tprf_c128 *e_k, tprf_c128 *chi) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (!(beta > 0) || nz < 1 || !z || !dims_ok(dims) || !e_k || !chi)
+    return fail(ctx, TPRF_ERR_ARG, "lindhard_chi00_noalloc: bad argument");
+  if (norb < 1 || norb > 4) return fail(ctx, TPRF_ERR_UNSUPPORTED, "lindhard_chi00_noalloc: norb must be 1..4");
+  const long nk = (long)dims[0] * dims[1] * dims[2];
+  // the same workspace layout as tprf_lindhard_chi00, in the file-scope array
+  const size_t ev_bytes = ((sizeof(double) * nk * norb + 255) / 256) * 256;
+  const size_t need = ev_bytes + sizeof(c2) * (size_t)nk * norb * norb * norb;
+  if (need > tprf::kNoallocScratchBytes)
+    return fail(ctx, TPRF_ERR_UNSUPPORTED,
+                "lindhard_chi00_noalloc: " + std::to_string(need) + " bytes of workspace needed, the fixed maximum is " +
+                    std::to_string(tprf::kNoallocScratchBytes) + " (nk * (8 n + 16 n^3) bytes, rounded)");
+  if (!ctx->scratch) return fail(ctx, TPRF_ERR_HIP, "lindhard_chi00_noalloc: no device workspace address");
+  char *w = (char *)ctx->scratch;
+  return check_launch(ctx,
+                      tprf::launch_lindhard(ctx->stream, beta, mu, nz, (const c2 *)z, dims, norb, (const c2 *)e_k,
+                                            (double *)w, (c2 *)(w + ev_bytes), (c2 *)chi),
                       "lindhard kernels");
 }
 

@@ -136,6 +136,11 @@ int dyson_lattice16r_variants(int n);
 // (workspace), then chi[nz][nk][n^4] at the nz complex frequencies z (device)
 hipError_t launch_lindhard(hipStream_t s, double beta, double mu, long nz, const c2 *z, const int dims[3], int n,
                            const c2 *ek, double *ev, c2 *proj, c2 *chi);
+// Size of the file-scope device array behind tprf_lindhard_chi00_noalloc, and
+// its device address on `device` (which must be the current device; the
+// address is looked up once per device and cached).
+constexpr size_t kNoallocScratchBytes = size_t(32) << 20;
+hipError_t noalloc_scratch_address(int device, void **ptr);
 hipError_t launch_lattice_axis(hipStream_t s, int sign, const c2 *in, c2 *out, long outer, int len, long inner,
                                const FftDev &p, double scale);
 hipError_t launch_w2t(hipStream_t s, int statistic, double beta, int n_max, long ncol, const c2 *in, c2 *out,

@@ -15,6 +15,8 @@
 // one workgroup per (q, block of frequencies) sums over k in LDS tiles.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "common.hpp"
 #include "fft_device.hpp"
 
@@ -208,6 +210,32 @@ hipError_t launch_lindhard(hipStream_t s, double beta, double mu, long nz, const
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// Device workspace of the no-allocation entry point tprf_lindhard_chi00_noalloc:
+// one file-scope array per GPU, held until the process ends.  Host code never
+// passes the array's name to a kernel or a copy (on the host it is only a
+// stand-in for the symbol); it uses the device address that
+// hipGetSymbolAddress returns for the device that is current, cached here per
+// device and kept by each context.  Every call on one GPU shares the array,
+// so calls that overlap on one GPU race.
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+hipError_t noalloc_scratch_address(int device, void **ptr) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static void *addr[kMaxDevices] = {};
+  if (device < 0 || device >= kMaxDevices || !ptr) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));  // the current device's copy
+    if (e != hipSuccess) return e;
+    addr[device] = p;
+  }
+  *ptr = addr[device];
+  return hipSuccess;
 }
 
 }  // namespace tprf
