@@ -94,7 +94,7 @@ def cpu_baseline(wl, chain_result):
     n4 = n ** 4
     got_chi0, got_chi, lo = chain_result
     parity = {
-        "chi0_wk_rel": rel(got_chi0[lo:], chi0_wk[lo:]),
+        "chi0_wk_rel": rel(got_chi0, chi0_wk[lo:]),
         "chi0_wk_frequencies": "nu >= 0 (the symmetric chain forms nu < 0 of chi_RPA only)" if lo else "all",
         "chi_rpa_rel_all": rel(got_chi, chi),
         "chi_rpa_rel_cond_le_100": rel(got_chi.reshape(-1, n4)[good], chi.reshape(-1, n4)[good]),
@@ -429,8 +429,7 @@ def main():
     if not args.no_cpu:
         torch.cuda.synchronize()
         # the symmetric chain stores chi0(nu, q) at nu >= 0 only (rows nb - 1 ..)
-        lo = wl.nb - 1 if ch.herm_nu else 0
-        got = (ch.chi0_wk.cpu().numpy(), ch.chi_wk.cpu().numpy(), lo)
+        got = (ch.chi0_wk.cpu().numpy(), ch.chi_wk.cpu().numpy(), ch.nu_lo)
         base, parity = cpu_baseline(wl, got)
         out["cpu_baseline"] = base
         out["parity_vs_oracle"] = parity

@@ -25,6 +25,8 @@
 #ifndef TPRF_HIP_H
 #define TPRF_HIP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -75,6 +77,20 @@ int tprf_ctx_set_stream(tprf_ctx *ctx, void *hip_stream); /* hipStream_t; NULL =
  * calls (in == out) are never timed; they use the cached or default choice. */
 #define TPRF_OPT_LATTICE_VARIANT 4
 int tprf_ctx_set_option(tprf_ctx *ctx, int option, int value);
+/* Device scratch (no counterpart in the reference).  The static bubble, the
+ * Dyson adjoint and lindhard_chi00 need scratch beyond their outputs.  By
+ * default the context allocates it; that buffer only grows, and an outgrown
+ * one is kept, not freed, until tprf_ctx_destroy (a graph captured earlier may
+ * still use it).  A caller that captures such calls into a HIP graph binds a
+ * workspace of its own for the capture, so that each graph owns its buffer and
+ * two graphs never share one: tprf_ctx_set_workspace(ctx, ptr, bytes) (device
+ * memory; NULL unbinds); while bound, an entry that needs more scratch than
+ * `bytes` returns TPRF_ERR_NOMEM.  tprf_ctx_workspace_size reports the largest
+ * scratch any call asked for since the context was made or the last reset
+ * (reset != 0 clears it after reading), e.g. after an eager warm-up of the
+ * calls to be captured.                                                       */
+int tprf_ctx_set_workspace(tprf_ctx *ctx, void *work, size_t bytes);
+int tprf_ctx_workspace_size(tprf_ctx *ctx, size_t *bytes, int reset);
 const char *tprf_ctx_last_error(const tprf_ctx *ctx);
 /* Largest RMS tail-fit residual of the fitted omega->tau calls since the last
  * reset (blocking: synchronises the stream).  The reference asserts < 1e-3

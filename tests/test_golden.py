@@ -64,7 +64,7 @@ def test_gpu_chain_matches_golden(name, fused):
     lat = MeshCycLat(ch.wl.dims)
     cols = [r for r in range(ch.wl.nk) if r <= lat.minus(r)] if fused else list(range(ch.wl.nk))
     assert relerr(ch.g_tr.cpu().numpy()[::8][:, cols], f["g_tr8"][:, cols]) <= 1e-10
-    lo = int(f["nb"]) - 1 if fused else 0
     assert fused == ch.herm_nu
-    assert relerr(ch.chi0_wk.cpu().numpy()[lo:], f["chi0_wk"][lo:]) <= 1e-10
+    assert ch.nu_lo == (int(f["nb"]) - 1 if fused else 0)
+    assert relerr(ch.chi0_wk.cpu().numpy(), f["chi0_wk"][ch.nu_lo:]) <= 1e-10
     assert relerr(ch.chi_wk.cpu().numpy(), f["chi_wk"]) <= 1e-10

@@ -582,8 +582,9 @@ def test_chain_fused_equals_per_function(L):
     assert relerr(host(got.g_tr)[:, canon], host(ref.g_tr)[:, canon]) <= TOL
     # with the symmetries the fused chain forms and transforms nu >= 0 only
     nb = wl.nb
-    assert relerr(host(got.chi0_wr)[nb - 1:], host(ref.chi0_wr)[nb - 1:]) <= TOL
-    assert relerr(host(got.chi0_wk)[nb - 1:], host(ref.chi0_wk)[nb - 1:]) <= TOL
+    assert got.nu_lo == nb - 1 and got.chi0_wr.shape[0] == nb
+    assert relerr(host(got.chi0_wr), host(ref.chi0_wr)[nb - 1:]) <= TOL
+    assert relerr(host(got.chi0_wk), host(ref.chi0_wk)[nb - 1:]) <= TOL
     # chi_RPA at every frequency, on the points whose RPA matrix has cond <= 100
     # (cond reaches ~4e5 here, which amplifies rounding of chi0)
     kappa = rpa_condition(host(ref.chi0_wk), wl.U)

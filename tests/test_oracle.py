@@ -152,6 +152,53 @@ def test_t_to_w_boson_two_poles(oracle):
     assert np.abs(chi_w - exact).max() < 1e-6
 
 
+def _one_fermion_tau(eps, tau, beta):
+    """G(tau) of the pole 1/(i w - eps), in the overflow-free form."""
+    if eps >= 0:
+        return -np.exp(-eps * tau) / (1 + np.exp(-beta * eps))
+    return -np.exp(eps * (beta - tau)) / (1 + np.exp(beta * eps))
+
+
+@pytest.mark.parametrize("eps", [-2.0, 0.3, 1.7])
+def test_t_to_w_fermion_single_pole(oracle, eps):
+    """Fermionic tau -> omega (fourier_matsubara.cpp:123-184 with the phase
+    e^{i pi tau / beta} and m1 = -(g_0 + g_L)) of the exact G(tau) of one pole
+    returns 1/(i w_n - eps): the residual after the three-pole model is smooth,
+    so the rectangle rule leaves ~1e-11 at 256 frequencies.  A wrong phase sign,
+    endpoint rule or moment sign gives O(1) errors."""
+    beta, n_iw = 10.0, 256
+    L = 6 * n_iw
+    tau = np.arange(L + 1) * beta / L
+    gw = oracle.fourier_t_to_w(_one_fermion_tau(eps, tau, beta).reshape(-1, 1), beta, n_iw,
+                               statistic=oracle.FERMION)[:, 0]
+    iw = 1j * np.pi * (2 * np.arange(-n_iw, n_iw) + 1) / beta
+    exact = 1 / (iw - eps)
+    assert np.abs(gw - exact).max() / np.abs(exact).max() < 1e-10
+
+
+def test_t_to_w_fermion_two_poles(oracle):
+    beta, n_iw = 10.0, 128
+    L = 6 * n_iw
+    tau = np.arange(L + 1) * beta / L
+    gt = 0.3 * _one_fermion_tau(0.7, tau, beta) + 0.7 * _one_fermion_tau(-1.1, tau, beta)
+    gw = oracle.fourier_t_to_w(gt.reshape(-1, 1), beta, n_iw, statistic=oracle.FERMION)[:, 0]
+    iw = 1j * np.pi * (2 * np.arange(-n_iw, n_iw) + 1) / beta
+    exact = 0.3 / (iw - 0.7) + 0.7 / (iw + 1.1)
+    assert np.abs(gw - exact).max() / np.abs(exact).max() < 1e-10
+
+
+@pytest.mark.parametrize("eps", [-2.0, 0.3, 1.7])
+def test_w_to_t_to_w_fermion_roundtrip(oracle, eps):
+    """omega -> tau (fitted tail) then tau -> omega returns G(i w_n) to ~1e-12 at
+    256 frequencies: the two transforms are each other's inverse on the mesh
+    up to the tail models' discretisation."""
+    beta, n_iw = 10.0, 256
+    iw = 1j * np.pi * (2 * np.arange(-n_iw, n_iw) + 1) / beta
+    g = (1 / (iw - eps)).reshape(-1, 1)
+    back = oracle.fourier_t_to_w(oracle.fourier_w_to_t(g, beta), beta, n_iw, statistic=oracle.FERMION)
+    assert np.abs(back - g).max() / np.abs(g).max() < 1e-11
+
+
 def test_vandermonde_inverse_exact(oracle):
     V = np.array([[m ** j for j in range(8)] for m in range(1, 9)], dtype=object)
     F = [[fractions.Fraction(int(V[i, j])) for j in range(8)] for i in range(8)]
@@ -318,6 +365,31 @@ def test_kanamori_tensor_rules():
     assert Us[0, 0, 1, 1] == J and Uc[0, 0, 1, 1] == 2 * (U - 2 * J) - J
     assert Us[0, 1, 0, 1] == J and Uc[0, 1, 0, 1] == J
     assert np.count_nonzero(Us) == 3 + 3 * 6
+
+
+def test_tb_dispersion_nearest_neighbour_and_cubic_workload():
+    """tb_dispersion (the TBLattice hopping dictionary, e(k) = sum_R T_R e^{i k.R}):
+    a hand-computed single-band square lattice, and the 3-orbital cubic workload's
+    e_k from its hopping matrices (+d -> T_d, -d -> T_d^H = T_d)."""
+    from tprf_amd import workloads
+    from tprf_amd.lattice_utils import tb_dispersion
+    t, tp = 1.0, -0.3
+    hop = {(1, 0): [[-t]], (-1, 0): [[-t]], (0, 1): [[-t]], (0, -1): [[-t]],
+           (1, 1): [[-tp]], (-1, -1): [[-tp]], (1, -1): [[-tp]], (-1, 1): [[-tp]]}
+    e = tb_dispersion(hop, (8, 6, 1), 1).numpy()[:, 0, 0]
+    kx, ky = np.meshgrid(2 * np.pi * np.arange(8) / 8, 2 * np.pi * np.arange(6) / 6, indexing="ij")
+    ref = -2 * t * (np.cos(kx) + np.cos(ky)) - 4 * tp * np.cos(kx) * np.cos(ky)
+    assert np.abs(e - ref.ravel()).max() < 1e-14
+    wl = workloads.cubic_3orb(seed=0, L=8)
+    Td = workloads.cubic_hoppings(0, 3)
+    hop = {}
+    for d in range(3):
+        R = [0, 0, 0]
+        R[d] = 1
+        hop[tuple(R)] = Td[d]
+        R[d] = -1
+        hop[tuple(R)] = Td[d].conj().T
+    assert np.abs(tb_dispersion(hop, (8, 8, 8), 3).numpy() - wl.e_k).max() < 1e-14
 
 
 def test_workload_filling_and_hermiticity():

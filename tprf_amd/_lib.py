@@ -40,6 +40,8 @@ _SIGNATURES = {
     "tprf_ctx_last_error": [_p],
     "tprf_ctx_tail_status": [_p, ctypes.POINTER(_d), ctypes.POINTER(_d)],
     "tprf_ctx_tail_status_reset": [_p],
+    "tprf_ctx_set_workspace": [_p, _p, ctypes.c_size_t],
+    "tprf_ctx_workspace_size": [_p, ctypes.POINTER(ctypes.c_size_t), _i],
     "tprf_lattice_dyson_g_wk": [_p, _d, _i, _d, _l, _i, _p, _i, _p, _p],
     "tprf_lattice_dyson_g_wr": [_p, _d, _i, _d, _p, _i, _p, _p],
     "tprf_lattice_dyson_g_wr_herm": [_p, _d, _i, _d, _p, _i, _p, _p, _i],
@@ -131,6 +133,22 @@ def tail_status(c):
 
 def tail_status_reset(c):
     check(lib().tprf_ctx_tail_status_reset(c), c, "tail status reset")
+
+
+def workspace_size(c, reset=False):
+    """Largest scratch (bytes) any call on context c asked for since the last reset."""
+    b = ctypes.c_size_t(0)
+    check(lib().tprf_ctx_workspace_size(c, ctypes.byref(b), 1 if reset else 0), c, "workspace size")
+    return b.value
+
+
+def set_workspace(c, t):
+    """Bind the device memory of tensor t (None unbinds) as context c's scratch."""
+    if t is None:
+        check(lib().tprf_ctx_set_workspace(c, None, 0), c, "set workspace")
+    else:
+        check(lib().tprf_ctx_set_workspace(c, ctypes.c_void_p(t.data_ptr()), t.numel() * t.element_size()), c,
+              "set workspace")
 
 
 def dims3(dims):

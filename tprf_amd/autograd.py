@@ -164,15 +164,28 @@ class StaticRPA:
                 self._backward_into(c, st["g_wk"], st["chi0_q"], st["U"], st["chibar"], st["chi2"],
                                     st["chi0bar"], st["Ubar"], st["ebar"], st["mubar"], recompute=False)
 
-            # warm-up outside the graph: plans, tables and scratch are allocated here
+            # warm-up outside the graph: plans and tables are made here, and the
+            # scratch the calls need is measured
+            c = self._ctx()
+            _lib.workspace_size(c, reset=True)
             side = torch.cuda.Stream(device=self.dev)
             side.wait_stream(torch.cuda.current_stream(self.dev))
             with torch.cuda.stream(side):
                 run()
             torch.cuda.current_stream(self.dev).wait_stream(side)
+            torch.cuda.synchronize(self.dev)
+            # the graph gets a workspace of its own (kept with its static buffers),
+            # so no later call that grows the context's scratch, and no other
+            # graph, touches the memory this graph's kernels use
+            st["work"] = torch.empty(max(_lib.workspace_size(c, reset=True), 256), dtype=torch.uint8,
+                                     device=self.dev)
             g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g):
-                run()
+            _lib.set_workspace(c, st["work"])
+            try:
+                with torch.cuda.graph(g):
+                    run()
+            finally:
+                _lib.set_workspace(c, None)
             self._graph, self._static = g, st
         st = self._static
         st["e_k"].copy_(e_k)

@@ -36,7 +36,6 @@ class BatchedChain:
 
     def run_eager(self):
         """Every lattice one after the other on the current stream (also the warm-up)."""
-        _lib.ctx(self.dev.index)  # bind the library context to the current stream
         for ch in self.chains:
             ch.run()
 
@@ -64,8 +63,7 @@ class BatchedChain:
                 st = main if j == 0 else side[j - 1]
                 if j > 0:
                     st.wait_event(fork)
-                with torch.cuda.stream(st):
-                    _lib.ctx(self.dev.index)  # library calls go to this lane's stream
+                with torch.cuda.stream(st):  # every stage binds the library context to this lane's stream
                     for i in lane:
                         for s in self.chains[i].STAGES:
                             self.chains[i].stage(s)
@@ -75,7 +73,7 @@ class BatchedChain:
                     done.append(ev)
             for ev in done:
                 main.wait_event(ev)
-            _lib.ctx(self.dev.index)
+        _lib.ctx(self.dev.index)  # back to torch's current stream
         self.graph, self.nstreams, self._streams = g, nstreams, [main] + side
         return g
 

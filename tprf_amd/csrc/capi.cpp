@@ -69,6 +69,10 @@ struct tprf_ctx {
   void *scratch = nullptr;  // this device's copy of the no-allocation entry point's file-scope array
   void *work = nullptr;                                   // grow-only scratch
   size_t work_bytes = 0;
+  std::vector<void *> work_retired;  // outgrown scratch, kept until destroy: a captured graph may still use it
+  void *ext_work = nullptr;          // caller-owned workspace (tprf_ctx_set_workspace)
+  size_t ext_work_bytes = 0;
+  size_t work_peak = 0;              // largest scratch asked for (tprf_ctx_workspace_size)
   int bubble_variant = -1;                                // -1: autotuned per shape
   std::map<std::tuple<int, int, int, int, long>, int> bubble_tuned;
   std::map<std::tuple<int, int, int, int, long>, int> bubble_herm_tuned;
@@ -458,18 +462,27 @@ int get_bubble(tprf_ctx *ctx, double beta, int L, int nb, const int dims[3], int
   return TPRF_OK;
 }
 
-// Context scratch of at least `bytes` (grow-only; the old buffer is freed
-// after the stream drains).
+// Scratch of at least `bytes`: the caller's bound workspace if there is one,
+// else the context's own buffer.  That one only grows; an outgrown buffer is
+// kept (not freed) until tprf_ctx_destroy, because kernels queued on it, or a
+// graph captured with its address, may still use it.
 int get_work(tprf_ctx *ctx, size_t bytes, void **out) {
+  if (bytes > ctx->work_peak) ctx->work_peak = bytes;
+  if (ctx->ext_work) {
+    if (ctx->ext_work_bytes < bytes)
+      return fail(ctx, TPRF_ERR_NOMEM,
+                  "bound workspace too small: " + std::to_string(bytes) + " bytes needed, " +
+                      std::to_string(ctx->ext_work_bytes) + " bound");
+    *out = ctx->ext_work;
+    return TPRF_OK;
+  }
   if (ctx->work_bytes < bytes) {
-    if (ctx->work) {
-      TPRF_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-      (void)hipFree(ctx->work);
-      ctx->work = nullptr;
-      ctx->work_bytes = 0;
-    }
-    if (hipMalloc(&ctx->work, bytes) != hipSuccess) return fail(ctx, TPRF_ERR_NOMEM, "scratch allocation failed");
-    ctx->work_bytes = bytes;
+    const size_t grow = bytes > 2 * ctx->work_bytes ? bytes : 2 * ctx->work_bytes;
+    void *p = nullptr;
+    if (hipMalloc(&p, grow) != hipSuccess) return fail(ctx, TPRF_ERR_NOMEM, "scratch allocation failed");
+    if (ctx->work) ctx->work_retired.push_back(ctx->work);
+    ctx->work = p;
+    ctx->work_bytes = grow;
   }
   *out = ctx->work;
   return TPRF_OK;
@@ -667,6 +680,7 @@ int tprf_ctx_destroy(tprf_ctx *ctx) {
   for (auto &kv : ctx->otab) (void)hipFree(kv.second);
   for (auto &kv : ctx->symcols) { (void)hipFree(kv.second.col); (void)hipFree(kv.second.mcol); }
   if (ctx->work) (void)hipFree(ctx->work);
+  for (void *p : ctx->work_retired) (void)hipFree(p);
   for (auto &kv : ctx->bubbles) {
     (void)hipFree(kv.second.tw1); (void)hipFree(kv.second.twp); (void)hipFree(kv.second.hmod);
     (void)hipFree(kv.second.rcanon);
@@ -703,6 +717,21 @@ int tprf_ctx_set_option(tprf_ctx *ctx, int option, int value) {
 int tprf_ctx_set_stream(tprf_ctx *ctx, void *stream) {
   if (!ctx) return TPRF_ERR_ARG;
   ctx->stream = (hipStream_t)stream;
+  return TPRF_OK;
+}
+
+int tprf_ctx_set_workspace(tprf_ctx *ctx, void *work, size_t bytes) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (work && bytes == 0) return fail(ctx, TPRF_ERR_ARG, "set_workspace: zero bytes");
+  ctx->ext_work = work;
+  ctx->ext_work_bytes = work ? bytes : 0;
+  return TPRF_OK;
+}
+
+int tprf_ctx_workspace_size(tprf_ctx *ctx, size_t *bytes, int reset) {
+  if (!ctx || !bytes) return TPRF_ERR_ARG;
+  *bytes = ctx->work_peak;
+  if (reset) ctx->work_peak = 0;
   return TPRF_OK;
 }
 
