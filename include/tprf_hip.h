@@ -208,10 +208,12 @@ int tprf_chi_w0r_from_chi_tr(tprf_ctx *ctx, double beta, int ntau, long nr, int 
 int tprf_solve_rpa_PH(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *chi0, const tprf_c128 *U,
                       tprf_c128 *chi);
 /* solve_rpa_PH on a bosonic Matsubara mesh of 2 nb - 1 frequencies
- * (chi0, chi [2 nb - 1][nk][n^4]) when chi0_abcd(-nu, q) = conj(chi0_dcba(nu, q)),
- * U_abcd = conj(U_dcba) and U_abcd = U_badc (the caller's promise; it holds
- * for chi0 from a Hermitian e_k and any real Hermitian interaction, such as
- * the Kanamori tensors).  No counterpart in the reference: only the nb
+ * (chi0, chi [2 nb - 1][nk][n^4]) when chi0_abcd(-nu, q) = conj(chi0_dcba(nu, q))
+ * and U_abcd = conj(U_dcba) (the caller's promise; it holds for chi0 from a
+ * Hermitian e_k and any Hermitian interaction, such as the Kanamori tensors:
+ * in the grouping M[(a,b),(c,d)] = X_abcd both read M(-nu) = S M(nu)^H S with
+ * S the swap (a,b) -> (b,a), and the push-through identity carries it to
+ * chi = (1 - chi0 U)^-1 chi0).  No counterpart in the reference: only the nb
  * frequencies nu >= 0 of chi0 are read and solved; chi_dcba(-nu, q) is
  * written as conj(chi_abcd(nu, q)), which is what the full solve gives for
  * such inputs, to rounding.                                                 */

@@ -360,9 +360,9 @@ def test_chi0_w0r_trapz(L, oracle, gtr_case):
 @pytest.mark.parametrize("n,nb,nk", [(1, 5, 7), (2, 4, 13), (3, 6, 29), (3, 1, 40), (4, 2, 5)])
 def test_rpa_herm(L, oracle, n, nb, nk):
     """tprf_solve_rpa_PH_herm: chi0 with chi0_abcd(-nu) = conj(chi0_dcba(nu))
-    (M(-nu) = M(nu)^H for M[(a,b),(d,c)] = X_abcd) and U Hermitian with
-    U_abcd = U_badc; the nu >= 0 half is solved and chi(-nu) written from the
-    same identity, against the oracle's full solve of every point."""
+    (M(-nu) = S M(nu)^H S for M[(a,b),(c,d)] = X_abcd, S the swap of a and b)
+    and U_abcd = conj(U_dcba); the nu >= 0 half is solved and chi(-nu) written
+    from the same identity, against the oracle's full solve of every point."""
     import torch
     from tprf_amd import _lib
     rng = np.random.default_rng(n * 31 + nb)

@@ -498,7 +498,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 // frequencies m = pt / nk >= 0 of arrays [2 nb - 1][nk][n^4] (read and written
 // at base + pt, base = (nb - 1) nk), and each result is also written at -m
 // by chi_dcba(-nu, q) = conj(chi_abcd(nu, q)), the identity that holds when
-// chi0 obeys it and U_abcd = conj(U_dcba) = U_badc.
+// chi0 obeys it and U_abcd = conj(U_dcba).
 template <int N, bool MIRROR = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N == 3 ? 4 : 1))) k_rpa(long npts, const c2 *__restrict__ chi0, const c2 *__restrict__ U,
                                              c2 *__restrict__ chi, double *status, long base = 0, long nk = 1) {
