@@ -199,6 +199,40 @@ def test_w_to_t_to_w_fermion_roundtrip(oracle, eps):
     assert np.abs(back - g).max() / np.abs(g).max() < 1e-11
 
 
+@pytest.mark.parametrize("n_iw,tol_m,tol_g,tol_chi0", [(256, 1e-7, 1e-9, 1e-12), (128, 1e-5, 1e-7, 1e-9)])
+def test_tail_fit_policy_vs_exact_moments(oracle, n_iw, tol_m, tol_g, tol_chi0):
+    """The fitted tail (TRIQS fit_tail, restated; fourier_matsubara.cpp:226) against
+    the exact high-frequency moments of G = [(i w + mu) - e_k]^-1 (m1 = delta_r0,
+    m2 = (e - mu)(r), m3 = ((e - mu)^2)(r)) on a 3-orbital 6^3 cubic lattice at
+    beta 40.  With 256 freqs (w_max ~ 40) the moments agree to ~1e-8, G(tau) to
+    ~5e-11 and chi0 to ~1e-13 (the 16^3 workload: 1.5e-8, 5.5e-11, 1.6e-13), so
+    there the fit policy moves chi0 far below the 1e-10 bar.  With 128 freqs
+    (w_max ~ 20) they agree only to ~2e-6, ~3e-8 and ~1.4e-10: on such meshes
+    parity with TPRF itself depends on TRIQS's exact fit window and order.  A
+    fit that took the wrong sign of m2 would move G(tau) at O(1)."""
+    from tprf_amd import workloads
+    wl = workloads.cubic_3orb(seed=0, L=6, n_iw=n_iw, nb=16)
+    n = wl.norb
+    g_wr = oracle.fourier_lattice(oracle.lattice_dyson_g_wk(wl.beta, wl.n_iw, wl.mu, wl.e_k), wl.dims, -1)
+    e = wl.e_k - wl.mu * np.eye(n)[None]
+    m = np.zeros((3, wl.nk, n, n), complex)
+    m[0, 0] = np.eye(n)
+    m[1] = oracle.fourier_lattice(e[None], wl.dims, -1)[0]
+    m[2] = oracle.fourier_lattice((e @ e)[None], wl.dims, -1)[0]
+    g_fit, mf, err = oracle.fourier_w_to_t(g_wr, wl.beta, return_moments=True)
+    g_ex = oracle.fourier_w_to_t(g_wr, wl.beta, moments=m)
+    assert err < 1e-12
+    for i in range(3):
+        assert np.abs(mf[i] - m[i]).max() <= tol_m * np.abs(m[i]).max()
+    assert relerr(g_fit, g_ex) <= tol_g
+    c_fit = oracle.chi0_wr_from_grt_PH(g_fit, wl.dims, wl.beta, wl.nb)
+    c_ex = oracle.chi0_wr_from_grt_PH(g_ex, wl.dims, wl.beta, wl.nb)
+    assert relerr(c_fit, c_ex) <= tol_chi0
+    bad = m.copy()
+    bad[1] *= -1
+    assert relerr(oracle.fourier_w_to_t(g_wr, wl.beta, moments=bad), g_ex) > 1e-3
+
+
 def test_vandermonde_inverse_exact(oracle):
     V = np.array([[m ** j for j in range(8)] for m in range(1, 9)], dtype=object)
     F = [[fractions.Fraction(int(V[i, j])) for j in range(8)] for i in range(8)]
