@@ -259,6 +259,52 @@ int tprf_lindhard_chi00(tprf_ctx *ctx, double beta, double mu, long nz, const tp
 int tprf_lindhard_chi00_noalloc(tprf_ctx *ctx, double beta, double mu, long nz, const tprf_c128 *z,
                                 const int dims[3], int norb, const tprf_c128 *e_k, tprf_c128 *chi);
 
+/* ---- batched forward chain ---------------------------------------------- */
+/* Many lattices of uneven (dims, norb, n_iw, beta, nb) through the whole
+ * forward chain in one call (SURVEY's tprf_forward_batched; no counterpart in
+ * the reference, whose driver handles one lattice per call,
+ * lattice_utils.py:191-269, followed by solve_rpa_PH, rpa.cpp:29-68).  Per
+ * instance: Dyson + k -> r (tprf_lattice_dyson_g_wr), omega -> tau
+ * (tprf_fourier_wr_to_tr), the fused bubble (tprf_chi0_wr_from_grt_PH), r -> q
+ * and the RPA, each in the symmetric form its flags allow:
+ *   TPRF_CHAIN_HERMITIAN_EK  e_k Hermitian (G(tau, r) is then stored for
+ *                            r <= -r only where the pruned omega -> tau fits);
+ *   TPRF_CHAIN_HERMITIAN_U   with the above, U_abcd = conj(U_dcba) (chi0_wr and
+ *                            chi0_wk then hold the rows nu >= 0 only; chi_wk
+ *                            holds every frequency).
+ * Every pointer is device memory the caller owns, in the reference layout;
+ * tprf_chain_buffer_sizes gives the element counts of g_wr, g_tr, chi0_wr,
+ * chi0_wk and chi_wk.  With nstreams > 1 the instances (largest first, each on
+ * the least-loaded lane) run on nstreams - 1 library streams next to the
+ * context's, forked from and joined to it by events, so the call as a whole
+ * is ordered on the context's stream and can be captured into one HIP graph
+ * from it.  nstreams <= 16.  The kernel autotuner settles its choices on the
+ * first call that is not being captured.                                    */
+#define TPRF_CHAIN_HERMITIAN_EK 1
+#define TPRF_CHAIN_HERMITIAN_U 2
+typedef struct {
+  double beta;
+  int n_iw;    /* fermionic n_max */
+  int ntau;    /* <= 0: 6 n_iw + 1 */
+  int nb;      /* bosonic n_max: 2 nb - 1 frequencies */
+  int dims[3];
+  int norb;
+  double mu;
+  int flags;
+} tprf_chain_desc;
+typedef struct {
+  const tprf_c128 *e_k; /* [nk][n][n] */
+  const tprf_c128 *U;   /* [n][n][n][n] */
+  tprf_c128 *g_wr;      /* [2 n_iw][nk][n][n]      workspace */
+  tprf_c128 *g_tr;      /* [ntau][nk][n][n]        workspace */
+  tprf_c128 *chi0_wr;   /* [2 nb - 1][nk][n^4]     workspace */
+  tprf_c128 *chi0_wk;   /* [2 nb - 1][nk][n^4]     output    */
+  tprf_c128 *chi_wk;    /* [2 nb - 1][nk][n^4]     output    */
+} tprf_chain_ptrs;
+int tprf_chain_buffer_sizes(const tprf_chain_desc *desc, long counts[5]);
+int tprf_forward_batched(tprf_ctx *ctx, int n_inst, const tprf_chain_desc *desc, const tprf_chain_ptrs *ptrs,
+                         int nstreams);
+
 /* ---- differentiation: static chi_RPA(Q, i nu = 0) and its adjoints ------
  * No counterpart in the reference (TPRF has no derivatives).  Each adjoint is
  * the conjugate transpose of the complex-linear map named, or the reverse-mode

@@ -27,6 +27,20 @@ ERRORS = {
     -9: "unsupported size",
 }
 
+TPRF_CHAIN_HERMITIAN_EK, TPRF_CHAIN_HERMITIAN_U = 1, 2
+
+
+class ChainDesc(ctypes.Structure):
+    """tprf_chain_desc: one instance of tprf_forward_batched."""
+    _fields_ = [("beta", ctypes.c_double), ("n_iw", ctypes.c_int), ("ntau", ctypes.c_int), ("nb", ctypes.c_int),
+                ("dims", ctypes.c_int * 3), ("norb", ctypes.c_int), ("mu", ctypes.c_double), ("flags", ctypes.c_int)]
+
+
+class ChainPtrs(ctypes.Structure):
+    """tprf_chain_ptrs: device pointers of one instance (inputs, workspace, outputs)."""
+    _fields_ = [(f, ctypes.c_void_p) for f in ("e_k", "U", "g_wr", "g_tr", "chi0_wr", "chi0_wk", "chi_wk")]
+
+
 _lib = None
 _lock = threading.Lock()
 _ctx = {}
@@ -69,6 +83,8 @@ _SIGNATURES = {
     "tprf_fourier_r_to_q": [_p, _p, _p, _l, _l, _p, _p],
     "tprf_fourier_r_to_q_adjoint": [_p, _p, _p, _l, _l, _p, _p],
     "tprf_solve_rpa_PH_adjoint": [_p, _l, _i, _p, _p, _p, _p, _p, _p],
+    "tprf_chain_buffer_sizes": [_p, _p],
+    "tprf_forward_batched": [_p, _i, _p, _p, _i],
 }
 
 

@@ -1,80 +1,73 @@
 """Forward chi0 + RPA of many lattices with uneven (nk, norb, n_iw, beta) in one
-HIP graph.
+call of the C entry tprf_forward_batched, captured into one HIP graph.
 
-Every lattice keeps its own resident buffers (tprf_amd.chain.BubbleRPA).  One
-eager pass fills the context's plans and tables and lets the library's kernel
-autotuner pick a workgroup shape per shape class; the chains are then captured
-into one graph, spread over K streams forked from and joined to the capture
-stream, largest lattices first on the least-loaded stream.  ``autotune`` picks
-K by timing each candidate graph.  A replay is one hipGraphLaunch for the whole
-batch.
+Every lattice keeps its own resident buffers (tprf_amd.chain.BubbleRPA); a
+table of per-instance descriptors and device pointers drives the library,
+which runs the instances on several streams forked from and joined to the
+caller's stream, largest first on the least-loaded stream.  One eager call
+fills the context's plans and tables and lets the kernel autotuner settle a
+workgroup shape per shape class; the call is then captured, and a replay is
+one hipGraphLaunch for the whole batch.  ``autotune`` picks the stream count by
+timing each candidate graph.
 """
+import ctypes
+
 import torch
 
 from . import _lib
 from .chain import BubbleRPA
 
 
-def _cost(wl):
-    """Algorithmic bytes of one forward eval (materialised tensors written once and
-    read once), the weight used to balance the streams."""
-    n, nk = wl.norb, wl.nk
-    g = 2 * wl.n_iw * nk * n * n
-    gt = wl.ntau * nk * n * n
-    chi = wl.n_nu * nk * n ** 4
-    return 16 * (5 * g + gt + 5 * chi)
-
-
 class BatchedChain:
     def __init__(self, workloads, device="cuda"):
         self.dev = torch.device(device)
-        self.chains = [BubbleRPA(wl, device=device) for wl in workloads]
-        self.order = sorted(range(len(self.chains)), key=lambda i: -_cost(self.chains[i].wl))
+        if self.dev.index is None:
+            self.dev = torch.device(self.dev.type, torch.cuda.current_device())
+        self.chains = [BubbleRPA(wl, device=self.dev) for wl in workloads]
+        n = len(self.chains)
+        self._desc = (_lib.ChainDesc * n)()
+        self._ptrs = (_lib.ChainPtrs * n)()
+        for i, ch in enumerate(self.chains):
+            wl, d, p = ch.wl, self._desc[i], self._ptrs[i]
+            d.beta, d.n_iw, d.ntau, d.nb, d.norb, d.mu = wl.beta, wl.n_iw, wl.ntau, wl.nb, wl.norb, wl.mu
+            for k in range(3):
+                d.dims[k] = ch.dims[k]
+            d.flags = (_lib.TPRF_CHAIN_HERMITIAN_EK if ch.herm else 0) | (_lib.TPRF_CHAIN_HERMITIAN_U if ch.herm_nu else 0)
+            for f, t in (("e_k", ch.e_k), ("U", ch.U), ("g_wr", ch.g_wr), ("g_tr", ch.g_tr),
+                         ("chi0_wr", ch._chi0_wr), ("chi0_wk", ch._chi0_wk), ("chi_wk", ch.chi_wk)):
+                setattr(p, f, t.data_ptr())
+            # the library's size rule must agree with the buffers the chain holds
+            counts = (ctypes.c_long * 5)()
+            _lib.check(_lib.lib().tprf_chain_buffer_sizes(ctypes.byref(d), counts), None, "chain buffer sizes")
+            got = [ch.g_wr.numel(), ch.g_tr.numel(), ch._chi0_wr.numel(), ch._chi0_wk.numel(), ch.chi_wk.numel()]
+            if list(counts) != got:
+                raise RuntimeError(f"instance {i}: buffer sizes {got} != library's {list(counts)}")
         self.graph = None
         self.nstreams = None
-        self._streams = []
+
+    def run_batched(self, nstreams=1):
+        """The whole batch as one library call on torch's current stream."""
+        c = _lib.ctx(self.dev.index)
+        _lib.call("tprf_forward_batched", c, len(self.chains), ctypes.byref(self._desc), ctypes.byref(self._ptrs),
+                  int(nstreams))
 
     def run_eager(self):
-        """Every lattice one after the other on the current stream (also the warm-up)."""
+        """Every lattice one after the other on the current stream, one library call
+        per stage: the per-instance loop of the same chain."""
         for ch in self.chains:
             ch.run()
 
-    def _assign(self, k):
-        load, lanes = [0] * k, [[] for _ in range(k)]
-        for i in self.order:
-            j = min(range(k), key=load.__getitem__)
-            lanes[j].append(i)
-            load[j] += _cost(self.chains[i].wl)
-        return lanes
-
     def capture(self, nstreams):
         """Capture the whole batch into one graph over ``nstreams`` streams."""
-        self.run_eager()  # tables, plans and kernel variants are settled before capture
+        self.run_batched(1)  # tables, plans and kernel variants are settled on one stream
+        self.run_batched(nstreams)  # the library's lane streams and events exist before capture
         torch.cuda.synchronize(self.dev)
-        lanes = self._assign(nstreams)
         main = torch.cuda.Stream(device=self.dev)
-        side = [torch.cuda.Stream(device=self.dev) for _ in range(nstreams - 1)]
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g, stream=main):
-            fork = torch.cuda.Event()
-            fork.record(main)
-            done = []
-            for j, lane in enumerate(lanes):
-                st = main if j == 0 else side[j - 1]
-                if j > 0:
-                    st.wait_event(fork)
-                with torch.cuda.stream(st):  # every stage binds the library context to this lane's stream
-                    for i in lane:
-                        for s in self.chains[i].STAGES:
-                            self.chains[i].stage(s)
-                if j > 0:
-                    ev = torch.cuda.Event()
-                    ev.record(st)
-                    done.append(ev)
-            for ev in done:
-                main.wait_event(ev)
+            self.run_batched(nstreams)
         _lib.ctx(self.dev.index)  # back to torch's current stream
-        self.graph, self.nstreams, self._streams = g, nstreams, [main] + side
+        self.graph, self.nstreams = g, nstreams
         return g
 
     def replay(self):

@@ -10,6 +10,7 @@
 #include <complex>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <map>
 #include <string>
 #include <tuple>
@@ -82,6 +83,8 @@ struct tprf_ctx {
   std::map<std::tuple<long, long, int, int, int, int>, int> lattice_tuned;
   std::map<std::tuple<int, int>, int> dyson_lattice_tuned;
   std::map<std::tuple<int, int, int, int>, SymEntry> symcols;  // canonical columns per (dims, n)
+  std::vector<hipStream_t> lanes;      // streams of tprf_forward_batched (default priority)
+  std::vector<hipEvent_t> lane_events; // fork event + one join event per lane
 };
 
 namespace {
@@ -685,6 +688,8 @@ int tprf_ctx_destroy(tprf_ctx *ctx) {
     (void)hipFree(kv.second.tw1); (void)hipFree(kv.second.twp); (void)hipFree(kv.second.hmod);
     (void)hipFree(kv.second.rcanon);
   }
+  for (hipEvent_t e : ctx->lane_events) (void)hipEventDestroy(e);
+  for (hipStream_t l : ctx->lanes) (void)hipStreamDestroy(l);
   (void)hipFree(ctx->d_status);
   delete ctx;
   return TPRF_OK;
@@ -854,6 +859,18 @@ int get_symcols(tprf_ctx *ctx, const int dims[3], int n, bool full, tprf::SymCol
   return TPRF_OK;
 }
 
+// The input-pruned omega -> tau kernel takes L = Q M, M the smallest power of
+// two >= 2 n_max, 64 <= M <= 1024, Q <= 64.
+int w2t_pruned_M(int n_max) {
+  int M = 64;
+  while (M < 2 * n_max) M *= 2;
+  return M;
+}
+bool w2t_pruned_fits(const tprf_ctx *ctx, int n_max, int L) {
+  const int M = w2t_pruned_M(n_max);
+  return !ctx->generic_kernels && L % M == 0 && tprf::w2t_pruned_supported(M) && L / M <= 64;
+}
+
 int wr_to_tr(tprf_ctx *ctx, int statistic, double beta, int n_max, int ntau, long ncol, const tprf_c128 *in,
              tprf_c128 *out, const tprf_c128 *moments, const tprf::SymCols *sym) {
   if (!ctx) return TPRF_ERR_ARG;
@@ -874,9 +891,8 @@ int wr_to_tr(tprf_ctx *ctx, int statistic, double beta, int n_max, int ntau, lon
     fit = *f;
   }
   // input-pruned kernel: L = Q M with M a power of two holding the whole mesh
-  int M = 64;
-  while (M < 2 * n_max) M *= 2;
-  if (!ctx->generic_kernels && L % M == 0 && tprf::w2t_pruned_supported(M) && L / M <= 64) {
+  const int M = w2t_pruned_M(n_max);
+  if (w2t_pruned_fits(ctx, n_max, L)) {
     const tprf::FftDev *pL, *pM;
     if ((st = get_fft(ctx, L, &pL))) return st;
     if ((st = get_fft(ctx, M, &pM))) return st;
@@ -1282,6 +1298,128 @@ int tprf_solve_rpa_PH_adjoint(tprf_ctx *ctx, long npts, int norb, const tprf_c12
                                              (const c2 *)chibar, (c2 *)chi, (c2 *)chi0bar, (c2 *)Ubar,
                                              tprf::Status{ctx->d_status}),
                       "rpa point kernel");
+}
+
+
+// ---- batched forward chain ------------------------------------------------
+
+namespace {
+
+long chain_nk(const tprf_chain_desc &d) { return (long)d.dims[0] * d.dims[1] * d.dims[2]; }
+int chain_ntau(const tprf_chain_desc &d) { return d.ntau > 0 ? d.ntau : 6 * d.n_iw + 1; }
+
+// One instance of the forward chain on ctx->stream: Dyson + k -> r, omega -> tau,
+// fused bubble, r -> q, RPA (the stages of the reference driver,
+// lattice_utils.py:241-266, then solve_rpa_PH), with the symmetric forms the
+// descriptor's flags allow.
+int chain_forward(tprf_ctx *ctx, const tprf_chain_desc &d, const tprf_chain_ptrs &p) {
+  const int n = d.norb, ntau = chain_ntau(d), L = ntau - 1;
+  const long nk = chain_nk(d), n4 = (long)n * n * n * n;
+  const bool herm = (d.flags & TPRF_CHAIN_HERMITIAN_EK) != 0;
+  const bool herm_nu = herm && (d.flags & TPRF_CHAIN_HERMITIAN_U) != 0;
+  const bool herm_w2t = herm && w2t_pruned_fits(ctx, d.n_iw, L);
+  int st = dyson_g_wr(ctx, d.beta, d.n_iw, d.mu, d.dims, n, p.e_k, p.g_wr, herm ? (herm_w2t ? 2 : 1) : 0);
+  if (st) return st;
+  if (herm_w2t)
+    st = tprf_fourier_wr_to_tr_herm(ctx, d.beta, d.n_iw, ntau, d.dims, n, p.g_wr, p.g_tr, 0);
+  else
+    st = wr_to_tr(ctx, TPRF_FERMION, d.beta, d.n_iw, ntau, nk * n * n, p.g_wr, p.g_tr, nullptr, nullptr);
+  if (st) return st;
+  st = chi0_wr_from_grt(ctx, d.beta, ntau, d.dims, n, d.nb, p.g_tr, p.chi0_wr, herm, herm_nu);
+  if (st) return st;
+  const long off = herm_nu ? (long)(d.nb - 1) * nk * n4 : 0;
+  const long rows = herm_nu ? d.nb : 2L * d.nb - 1;
+  st = lattice_fft(ctx, rows, d.dims, n4, (const c2 *)(p.chi0_wr + off), (c2 *)(p.chi0_wk + off), +1, false);
+  if (st) return st;
+  if (herm_nu) return tprf_solve_rpa_PH_herm(ctx, d.nb, nk, n, p.chi0_wk, p.U, p.chi_wk);
+  return tprf_solve_rpa_PH(ctx, (2L * d.nb - 1) * nk, n, p.chi0_wk, p.U, p.chi_wk);
+}
+
+// Bytes the instance's chain moves (materialised tensors written and read
+// once): the weight that balances the lanes.
+double chain_cost(const tprf_chain_desc &d) {
+  const double n = d.norb, nk = (double)chain_nk(d);
+  return 16.0 * (5.0 * 2 * d.n_iw * nk * n * n + chain_ntau(d) * nk * n * n + 5.0 * (2 * d.nb - 1) * nk * n * n * n * n);
+}
+
+}  // namespace
+
+int tprf_chain_buffer_sizes(const tprf_chain_desc *d, long counts[5]) {
+  if (!d || !counts || d->norb < 1 || d->n_iw < 1 || d->nb < 1) return TPRF_ERR_ARG;
+  const long nk = chain_nk(*d), n2 = (long)d->norb * d->norb, n4 = n2 * n2;
+  counts[0] = 2L * d->n_iw * nk * n2;
+  counts[1] = (long)chain_ntau(*d) * nk * n2;
+  counts[2] = counts[3] = counts[4] = (2L * d->nb - 1) * nk * n4;
+  return TPRF_OK;
+}
+
+int tprf_forward_batched(tprf_ctx *ctx, int n_inst, const tprf_chain_desc *desc, const tprf_chain_ptrs *ptrs,
+                         int nstreams) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (n_inst < 1 || !desc || !ptrs || nstreams < 1 || nstreams > 16)
+    return fail(ctx, TPRF_ERR_ARG, "forward_batched: bad argument");
+  for (int i = 0; i < n_inst; ++i) {
+    const tprf_chain_desc &d = desc[i];
+    const tprf_chain_ptrs &p = ptrs[i];
+    if (!(d.beta > 0) || d.n_iw < 1 || d.nb < 1 || d.norb < 1 || !dims_ok(d.dims) || !p.e_k || !p.U || !p.g_wr ||
+        !p.g_tr || !p.chi0_wr || !p.chi0_wk || !p.chi_wk)
+      return fail(ctx, TPRF_ERR_ARG, "forward_batched: bad descriptor or pointer for instance " + std::to_string(i));
+  }
+  if (nstreams > n_inst) nstreams = n_inst;
+  if (nstreams == 1) {
+    for (int i = 0; i < n_inst; ++i) {
+      const int st = chain_forward(ctx, desc[i], ptrs[i]);
+      if (st) return st;
+    }
+    return TPRF_OK;
+  }
+  // lanes: nstreams - 1 library streams next to the caller's, forked from and
+  // joined to it by events (so that the whole batch can be captured into one
+  // graph from the caller's stream); largest instances first, each on the
+  // least-loaded lane
+  while ((int)ctx->lanes.size() < nstreams - 1) {
+    hipStream_t l;
+    TPRF_HIP_TRY(ctx, hipStreamCreateWithFlags(&l, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+    ctx->lanes.push_back(l);
+  }
+  while ((int)ctx->lane_events.size() < nstreams) {
+    hipEvent_t e;
+    TPRF_HIP_TRY(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreateWithFlags");
+    ctx->lane_events.push_back(e);
+  }
+  std::vector<int> order(n_inst);
+  for (int i = 0; i < n_inst; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return chain_cost(desc[a]) > chain_cost(desc[b]); });
+  std::vector<double> load(nstreams, 0.0);
+  std::vector<std::vector<int>> lane_of(nstreams);
+  for (int i : order) {
+    const int j = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    lane_of[j].push_back(i);
+    load[j] += chain_cost(desc[i]);
+  }
+  hipStream_t main = ctx->stream;
+  TPRF_HIP_TRY(ctx, hipEventRecord(ctx->lane_events[0], main), "hipEventRecord");
+  int st = TPRF_OK;
+  for (int j = 0; j < nstreams && st == TPRF_OK; ++j) {
+    hipStream_t s = j == 0 ? main : ctx->lanes[j - 1];
+    if (j > 0) {
+      hipError_t e = hipStreamWaitEvent(s, ctx->lane_events[0], 0);
+      if (e != hipSuccess) { st = hip_fail(ctx, e, "hipStreamWaitEvent"); break; }
+    }
+    ctx->stream = s;
+    for (int i : lane_of[j]) {
+      st = chain_forward(ctx, desc[i], ptrs[i]);
+      if (st) break;
+    }
+    ctx->stream = main;
+    if (st == TPRF_OK && j > 0) {
+      hipError_t e = hipEventRecord(ctx->lane_events[j], s);
+      if (e == hipSuccess) e = hipStreamWaitEvent(main, ctx->lane_events[j], 0);
+      if (e != hipSuccess) st = hip_fail(ctx, e, "lane join");
+    }
+  }
+  ctx->stream = main;
+  return st;
 }
 
 }  // extern "C"
