@@ -160,14 +160,14 @@ def _timed(f, reps, warm=2):
     return (time.perf_counter() - t) * 1e3 / reps
 
 
-def grad_leg(reps=20):
+def grad_leg(reps=20, route="fused"):
     """grad/fwd at the differentiation workload: single-band square lattice 64x64 k,
     beta 20, 512 fermionic frequencies, t = 1, t' = -0.3, mu = -0.5, U = 2.0,
     L = Re tr chi_RPA(Q = (pi, pi), i nu = 0); wall(forward + backward) / wall(forward)."""
     import torch
     from tprf_amd import autograd as A
     dims, q = (64, 64), (32, 32, 0)
-    spec = A.StaticRPA(20.0, 512, dims, q, norb=1)
+    spec = A.StaticRPA(20.0, 512, dims, q, norb=1, route=route)
     x0 = {"t": 1.0, "tp": -0.3, "mu": -0.5, "U": 2.0}
     leaves = {k: torch.tensor(v, dtype=torch.float64, device="cuda", requires_grad=True) for k, v in x0.items()}
 
@@ -187,13 +187,13 @@ def grad_leg(reps=20):
     tf = _timed(fwd, reps)
     tfb = _timed(fwd_bwd, reps)
     return {"workload": "square 64x64 k, beta 20, 512 fermionic freqs, t 1, t' -0.3, mu -0.5, U 2.0, "
-                        "L = Re tr chi_RPA((pi,pi), 0)",
+                        "L = Re tr chi_RPA((pi,pi), 0)", "route": route,
             "fwd_ms": tf, "fwd_bwd_ms": tfb, "grad_over_fwd": tfb / tf,
             "grad": {k: float(v.grad) for k, v in leaves.items()},
             "L": float(fwd())}
 
 
-def batched_leg(reps=3):
+def batched_leg(reps=3, route="fused"):
     """batched/plain at the batched forward+adjoint workload: 48 two-orbital square
     lattices, 48x48 k, 256 fermionic frequencies, beta 20, t' ~ U[-0.4, 0],
     mu ~ U[-1, 1], inter-orbital hopping ~ U[0, 0.3], U ~ U[1, 3], J = U/6,
@@ -210,7 +210,7 @@ def batched_leg(reps=3):
     U = torch.tensor(g.uniform(1.0, 3.0, B), dtype=torch.float64, device="cuda")
     e_k = A.two_orbital_square_e_k(tp, v, dims)
     Uk = A.kanamori_spin_U(U, U / 6)
-    one = A.StaticRPA(20.0, 256, dims, q, norb=2)
+    one = A.StaticRPA(20.0, 256, dims, q, norb=2, route=route)
     res = []
 
     def plain():
@@ -223,7 +223,7 @@ def batched_leg(reps=3):
             A.trace_loss(c).backward()
             res.append((c.detach(), e1.grad, m1.grad, U1.grad))
 
-    batch = A.StaticRPA(20.0, 256, dims, q, norb=2, nbatch=B)
+    batch = A.StaticRPA(20.0, 256, dims, q, norb=2, nbatch=B, route=route)
     tp_ms = _timed(plain, reps, warm=1)
     tb_ms = _timed(lambda: batch.graphed_grad(e_k, mu, Uk), reps + 2, warm=1)
     chi, ebar, mubar, Ubar = batch.graphed_grad(e_k, mu, Uk)
@@ -233,7 +233,7 @@ def batched_leg(reps=3):
                    abs(float(mubar[i] - res[i][2])) / abs(float(res[i][2]))) for i in range(B))
     del batch
     return {"workload": "48 two-orbital square lattices 48x48 k, 256 fermionic freqs, beta 20, forward + adjoint "
-                        "of chi_RPA((pi,pi), 0)",
+                        "of chi_RPA((pi,pi), 0)", "route": route,
             "plain_ms": tp_ms, "batched_ms": tb_ms, "batched_over_plain": tp_ms / tb_ms,
             "max_rel_diff_batched_vs_plain": diff}
 
@@ -388,7 +388,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-extra", action="store_true", help="skip the grad/fwd and batched/plain legs")
-    ap.add_argument("--only", choices=["grad", "batched", "uneven", "screened_w", "calibration"],
+    ap.add_argument("--only", choices=["grad", "batched", "uneven", "screened_w", "calibration", "grad_matsubara",
+                                       "batched_matsubara"],
                     help="run one leg alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -406,7 +407,9 @@ def main():
             print(json.dumps({"calibration": "done"}))
         else:
             print(json.dumps({"grad": grad_leg, "batched": batched_leg, "uneven": uneven_leg,
-                              "screened_w": screened_w_leg}[args.only]()))
+                              "screened_w": screened_w_leg,
+                              "grad_matsubara": lambda: grad_leg(route="matsubara"),
+                              "batched_matsubara": lambda: batched_leg(route="matsubara")}[args.only]()))
         return
     wl = workloads.cubic_3orb(seed=args.seed)
     ch = chain.BubbleRPA(wl, device="cuda:0")

@@ -353,6 +353,42 @@ int tprf_chi0_static_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int
 int tprf_chi0_static_from_grt_PH_adjoint(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb,
                                          long nbatch, const tprf_c128 *g_tr, const tprf_c128 *chibar_r,
                                          tprf_c128 *gbar_tr);
+/* The same nu = 0 bubble of the fitted omega -> tau transform (fourier_wr_to_tr,
+ * fourier_matsubara.cpp:218-308) followed by chi0_wr_from_grt_PH at nu = 0,
+ * from G(i w, r) directly: G(tau, r) is never formed.  The composite map is
+ * a bilinear form in Matsubara space, chi0_abcd(r) = -(1/beta) sum_n
+ * G_da(i w_n, r) G_bc(i w_n, -r) plus a correction of rank about 30 from the
+ * tail model and the boundary-derivative fit (exact algebra, equal to the
+ * tau route to rounding).  g_wr[2 n_iw][nbatch][nr][n][n] -> chi0_r as above;
+ * the tail fit's residual and zeroth moment go to the status words as for
+ * fourier_wr_to_tr.  ntau <= 0: 6 n_iw + 1; ntau >= 18; norb <= 3.  The
+ * adjoint takes chibar_r and writes gbar_wr [2 n_iw][nbatch][nr][n][n].      */
+int tprf_chi0_static_from_gwr_PH(tprf_ctx *ctx, double beta, int n_iw, int ntau, const int dims[3], int norb,
+                                 long nbatch, const tprf_c128 *g_wr, tprf_c128 *chi0_r);
+int tprf_chi0_static_from_gwr_PH_adjoint(tprf_ctx *ctx, double beta, int n_iw, int ntau, const int dims[3], int norb,
+                                         long nbatch, const tprf_c128 *g_wr, const tprf_c128 *chibar_r,
+                                         tprf_c128 *gbar_wr);
+/* The static chain at one momentum in one call, straight from e_k:
+ * lattice_dyson_g_wk (gf.cpp:91-132) -> fourier_wk_to_wr (fourier_lattice.cpp:49)
+ * -> fourier_wr_to_tr (fourier_matsubara.cpp:218-308) -> the nu = 0 row of
+ * chi0_wr_from_grt_PH (chi_imtime.cpp:139-182) -> one momentum of
+ * chi_wk_from_chi_wr (fourier_lattice.cpp:55-58), as
+ *   chi0_abcd(Q, 0) = (1/nk) sum_k B(G_da(iw, k), G_bc(iw, k - Q))
+ * with B the nu = 0 bilinear form of tprf_chi0_static_from_gwr_PH; G(iw, k)
+ * is evaluated in registers and never stored.  mu[nbatch] and e_k
+ * [nbatch][nk][n][n] are device arrays, Q = 2 pi q / dims (q integers, any
+ * sign), chi0_q [nbatch][n^4].  proj (NULL or [29][nbatch][nk][n][n] complex)
+ * receives the per-k tail-fit projections, which the adjoint then reuses
+ * instead of recomputing them.  The adjoint takes chibar_q [nbatch][n^4] and
+ * writes ebar_k [nbatch][nk][n][n] and mubar[nbatch] (dL = Re sum conj(xbar) dx).
+ * The tail-fit status words are those of fourier_wr_to_tr.  n = 1..3.        */
+int tprf_chi0_static_q_from_ek(tprf_ctx *ctx, double beta, int n_iw, int ntau, const int dims[3], const int q[3],
+                               int norb, long nbatch, const double *mu, const tprf_c128 *e_k, tprf_c128 *chi0_q,
+                               tprf_c128 *proj);
+int tprf_chi0_static_q_from_ek_adjoint(tprf_ctx *ctx, double beta, int n_iw, int ntau, const int dims[3],
+                                       const int q[3], int norb, long nbatch, const double *mu, const tprf_c128 *e_k,
+                                       const tprf_c128 *chibar_q, const tprf_c128 *proj, tprf_c128 *ebar_k,
+                                       double *mubar);
 /* One momentum of fourier_wr_to_wk (fourier_lattice.cpp:55-58):
  * out[inst][t] = sum_r exp(+2 pi i Q.r / dims) in[inst][r][t], Q = q[0..2]
  * (integers), and its adjoint gbar_r[inst][r][t] = exp(-2 pi i Q.r / dims)

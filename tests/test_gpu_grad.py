@@ -185,6 +185,219 @@ def test_static_bubble_adjoint_dot(T, oracle, n, dims):
     assert abs(lhs - rhs) <= 1e-11 * abs(lhs), (lhs, rhs)
 
 
+# ---------------------------------------------------------------- static bubble from G(i w, r)
+def _gwr_batch(oracle, rng, n, dims, beta, n_iw, nb, scale=0.7, mu=0.1):
+    """G(i w, r)[2 n_iw][nb][nk][n][n] of nb random Hermitian lattices (smooth tails)."""
+    nk = int(np.prod(dims))
+    out = np.empty((2 * n_iw, nb, nk, n, n), dtype=complex)
+    for i in range(nb):
+        a = rnd(rng, nk, n, n) * scale
+        e = 0.5 * (a + np.conj(np.swapaxes(a, 1, 2)))
+        out[:, i] = oracle.fourier_lattice(oracle.lattice_dyson_g_wk(beta, n_iw, mu, e), dims, -1)
+    return out
+
+
+def _static_tau_route(L, c, g_wr, beta, n_iw, dims, n, nb):
+    """GPU tau route: fitted omega -> tau of every column, then the tau-space bubble at nu = 0."""
+    ntau = 6 * n_iw + 1
+    nr = int(np.prod(dims))
+    gt = T_empty((ntau,) + g_wr.shape[1:])
+    L.call("tprf_fourier_wr_to_tr", c, L.FERMION, beta, n_iw, ntau, nb * nr * n * n, L.ptr(gpu(g_wr)), L.ptr(gt),
+           None)
+    o = T_empty((nb, nr, n, n, n, n))
+    L.call("tprf_chi0_static_from_grt_PH", c, beta, ntau, L.dims3(dims), n, nb, L.ptr(gt), L.ptr(o))
+    return gt, o
+
+
+def T_empty(shape):
+    import torch
+    t = torch.empty(shape, dtype=torch.complex128, device="cuda")
+    _HOLD.append(t)
+    return t
+
+
+@pytest.mark.parametrize("n,dims,nb,n_iw", [(1, (8, 8, 1), 1, 32), (2, (4, 6, 1), 3, 32), (3, (4, 4, 2), 2, 64),
+                                            (2, (6, 6, 1), 1, 128), (3, (3, 3, 3), 1, 256)])
+def test_static_gwr_forward_vs_tau_route_and_oracle(T, oracle, n, dims, nb, n_iw):
+    """tprf_chi0_static_from_gwr_PH (the nu = 0 bubble as a bilinear form in
+    G(i w, r)) against the same map evaluated the reference's way, through
+    G(tau, r): on the GPU (fitted omega -> tau, then the tau-space bubble) at
+    1e-12, and the oracle per lattice at 1e-10."""
+    L = lib()
+    rng = np.random.default_rng(11 * n + nb)
+    beta = 5.0
+    g_wr = _gwr_batch(oracle, rng, n, dims, beta, n_iw, nb)
+    c = L.ctx()
+    nr = int(np.prod(dims))
+    L.tail_status_reset(c)
+    got = T_empty((nb, nr, n, n, n, n))
+    L.call("tprf_chi0_static_from_gwr_PH", c, beta, n_iw, -1, L.dims3(dims), n, nb, L.ptr(gpu(g_wr)), L.ptr(got))
+    err, _ = L.tail_status(c)
+    assert err < 1e-6
+    _, ref = _static_tau_route(L, c, g_wr, beta, n_iw, dims, n, nb)
+    print(f"n {n} dims {dims} nb {nb} n_iw {n_iw}: vs tau route {relerr(host(got), host(ref)):.2e}", flush=True)
+    assert relerr(host(got), host(ref)) <= 1e-12
+    for i in range(nb):
+        gt = oracle.fourier_w_to_t(g_wr[:, i], beta)
+        assert relerr(host(got)[i], oracle.chi0_wr_from_grt_PH(gt, dims, beta, 1)[0]) <= 1e-10
+
+
+@pytest.mark.parametrize("n,dims,nb,n_iw", [(1, (8, 8, 1), 1, 32), (2, (4, 6, 1), 2, 32), (3, (4, 4, 2), 1, 64)])
+def test_static_gwr_adjoint(T, oracle, n, dims, nb, n_iw):
+    """The adjoint of tprf_chi0_static_from_gwr_PH: the dot-product test on its
+    exact linearisation (chi is quadratic in G; a smooth direction d, so that
+    the tail fit sees a physical tail), and against the tau route's adjoint
+    chain (tau-space bubble adjoint, then the transposed fitted omega -> tau)."""
+    L = lib()
+    rng = np.random.default_rng(17 * n + nb)
+    beta = 5.0
+    g = _gwr_batch(oracle, rng, n, dims, beta, n_iw, nb)
+    d = 0.1 * _gwr_batch(oracle, rng, n, dims, beta, n_iw, nb, mu=-0.3)
+    nr = int(np.prod(dims))
+    ybar = rnd(rng, nb, nr, n, n, n, n)
+    c = L.ctx()
+
+    def chi(x):
+        o = T_empty((nb, nr, n, n, n, n))
+        L.call("tprf_chi0_static_from_gwr_PH", c, beta, n_iw, -1, L.dims3(dims), n, nb, L.ptr(gpu(x)), L.ptr(o))
+        return host(o)
+
+    Jd = 0.5 * (chi(g + d) - chi(g - d))
+    gbar = T_empty(g.shape)
+    L.call("tprf_chi0_static_from_gwr_PH_adjoint", c, beta, n_iw, -1, L.dims3(dims), n, nb, L.ptr(gpu(g)),
+           L.ptr(gpu(ybar)), L.ptr(gbar))
+    lhs, rhs = vdot(Jd, ybar), vdot(d, host(gbar))
+    print(f"adjoint n {n}: dot {abs(lhs - rhs) / abs(lhs):.2e}", flush=True)
+    assert abs(lhs - rhs) <= 1e-11 * abs(lhs), (lhs, rhs)
+    gt, _ = _static_tau_route(L, c, g, beta, n_iw, dims, n, nb)
+    ntau = 6 * n_iw + 1
+    gbar_t = T_empty(gt.shape)
+    L.call("tprf_chi0_static_from_grt_PH_adjoint", c, beta, ntau, L.dims3(dims), n, nb, L.ptr(gt), L.ptr(gpu(ybar)),
+           L.ptr(gbar_t))
+    gbar_w = T_empty(g.shape)
+    L.call("tprf_fourier_wr_to_tr_adjoint", c, L.FERMION, beta, n_iw, ntau, nb * nr * n * n, L.ptr(gbar_t),
+           L.ptr(gbar_w), 1)
+    print(f"adjoint n {n}: vs tau route {relerr(host(gbar), host(gbar_w)):.2e}", flush=True)
+    assert relerr(host(gbar), host(gbar_w)) <= 1e-10
+
+
+def test_static_routes_give_the_same_gradient(T):
+    """StaticRPA on the fused k-space route, the Matsubara route and the tau
+    route: chi_RPA(Q, 0) and the gradient with respect to (e_k, mu, U) of a
+    two-orbital lattice agree."""
+    from tprf_amd import autograd as A
+    rng = np.random.default_rng(3)
+    beta, n_iw, dims, q, n = 8.0, 64, (8, 8, 1), (4, 4, 0), 2
+    kx, ky = np.meshgrid(2 * np.pi * np.arange(8) / 8, 2 * np.pi * np.arange(8) / 8, indexing="ij")
+    h = rnd(rng, 2, 2) * 0.3
+    h = (h + h.conj().T) / 2
+    e = (-2 * (np.cos(kx) + np.cos(ky))).ravel()[:, None, None] * np.eye(2) + np.cos(kx).ravel()[:, None, None] * h
+    U = rnd(rng, 2, 2, 2, 2) * 0.2
+    out = {}
+    for route in ("fused", "matsubara", "tau"):
+        spec = A.StaticRPA(beta, n_iw, dims, q, norb=n, route=route)
+        e_g = T.tensor(e, device="cuda", requires_grad=True)
+        U_g = T.tensor(U, device="cuda", requires_grad=True)
+        mu_g = T.tensor(0.2, dtype=T.float64, device="cuda", requires_grad=True)
+        chi = A.chi_rpa_static(e_g, mu_g, U_g, spec)
+        A.trace_loss(chi).backward()
+        out[route] = (host(chi), host(e_g.grad), host(U_g.grad), float(mu_g.grad))
+    b = out["tau"]
+    for r in ("fused", "matsubara"):
+        a = out[r]
+        for i in range(3):
+            assert relerr(a[i], b[i]) <= 1e-11, (r, i)
+        assert abs(a[3] - b[3]) <= 1e-11 * abs(b[3]), r
+
+
+@pytest.mark.parametrize("n,dims,q,nb", [(1, (6, 4, 2), (1, 3, 1), 3), (2, (6, 4, 2), (-1, 2, -3), 2),
+                                         (2, (16, 16, 1), (8, 3, 0), 1)])
+def test_fused_static_matches_matsubara_route(T, n, dims, q, nb):
+    """tprf_chi0_static_q_from_ek (G(iw, k) evaluated in registers from the
+    characteristic polynomial, k-space pair sum) against the Matsubara route
+    (stored G(iw, k) -> k -> r -> per-r bilinear form -> r -> Q) on
+    non-Hermitian e_k, per-lattice mu, a random cotangent chibar and momenta of
+    either sign: chi0(Q, 0), chi_RPA and the (e_k, mu, U) cotangents within
+    1e-12 / 1e-11 relative max-norm; the adjoint with the kept projections and
+    with their recomputation agree to 1e-14."""
+    from tprf_amd import autograd as A
+    rng = np.random.default_rng(100 + n)
+    beta, n_iw = 6.0, 48
+    nk = int(np.prod(dims))
+    e = rnd(rng, nb, nk, n, n) * 0.4 + np.linspace(-1.0, 1.0, n)[None, None, :, None] * np.eye(n)
+    mu = T.tensor(rng.uniform(-0.5, 0.5, nb), dtype=T.float64, device="cuda")
+    U = T.tensor(rnd(rng, nb, n, n, n, n) * 0.1, device="cuda")
+    e_t = T.tensor(e, device="cuda")
+    chibar = T.tensor(rnd(rng, nb, n, n, n, n), device="cuda")
+    res = {}
+    for route in ("fused", "matsubara"):
+        spec = A.StaticRPA(beta, n_iw, dims, q, norb=n, nbatch=nb, route=route)
+        chi, chi0_q, keep = spec.forward(e_t, mu, U)
+        ebar, mubar, Ubar = spec.backward(keep, chi0_q, U, chibar)
+        res[route] = [host(x) for x in (chi0_q, chi, ebar, mubar, Ubar)]
+        if route == "fused":
+            L = lib()
+            eb2 = T.empty_like(ebar)
+            mb2 = T.empty_like(mubar)
+            # chi0bar from the RPA adjoint, then the fused adjoint without the kept projections
+            c = L.ctx()
+            chi2, x0b, ub2 = (T.empty_like(chi0_q) for _ in range(3))
+            L.call("tprf_solve_rpa_PH_adjoint", c, nb, n, L.ptr(chi0_q), L.ptr(U.contiguous()), L.ptr(chibar),
+                   L.ptr(chi2), L.ptr(x0b), L.ptr(ub2))
+            L.call("tprf_chi0_static_q_from_ek_adjoint", c, beta, n_iw, spec.ntau, L.dims3(dims), L.dims3(q), n, nb,
+                   L.dptr(mu), L.ptr(e_t), L.ptr(x0b), None, L.ptr(eb2), L.dptr(mb2))
+            T.cuda.synchronize()
+            assert relerr(host(eb2), host(ebar)) <= 1e-14
+            assert relerr(host(mb2), host(mubar)) <= 1e-14
+    a, b = res["fused"], res["matsubara"]
+    for i, (name, tol) in enumerate([("chi0", 1e-12), ("chi", 1e-12), ("ebar", 1e-11), ("mubar", 1e-11),
+                                     ("Ubar", 1e-11)]):
+        err = relerr(a[i], b[i])
+        print(f"n {n} dims {dims} q {q} nb {nb} {name}: {err:.2e}", flush=True)
+        assert err <= tol, name
+
+
+def test_fused_static_three_orbital(T):
+    """n = 3 (the single-point RPA takes n <= 2, so chi0 level only): the fused
+    chi0(Q, 0) and its (e_k, mu) cotangents for a random chi0bar against the
+    Matsubara-route stages called one by one (Dyson, k -> r, per-r bilinear
+    form, r -> Q and their adjoints), 1e-12 / 1e-11 relative max-norm."""
+    L = lib()
+    rng = np.random.default_rng(7)
+    n, nb, beta, n_iw, dims, q = 3, 2, 6.0, 48, (4, 4, 2), (1, -1, 1)
+    nk, nw = int(np.prod(dims)), 2 * n_iw
+    ntau = 6 * n_iw + 1
+    e = rnd(rng, nb, nk, n, n) * 0.4 + np.linspace(-1.0, 1.0, n)[None, None, :, None] * np.eye(n)
+    e_t = gpu(e)
+    mu = T.tensor(rng.uniform(-0.5, 0.5, nb), dtype=T.float64, device="cuda")
+    xbar = gpu(rnd(rng, nb, n, n, n, n))
+    c, p, d3 = L.ctx(), L.ptr, L.dims3
+    chi0 = T.empty((nb, n ** 4), dtype=T.complex128, device="cuda")
+    eb, mb = T.empty((nb, nk, n, n), dtype=T.complex128, device="cuda"), T.empty(nb, dtype=T.float64, device="cuda")
+    L.call("tprf_chi0_static_q_from_ek", c, beta, n_iw, ntau, d3(dims), d3(q), n, nb, L.dptr(mu), p(e_t), p(chi0),
+           None)
+    L.call("tprf_chi0_static_q_from_ek_adjoint", c, beta, n_iw, ntau, d3(dims), d3(q), n, nb, L.dptr(mu), p(e_t),
+           p(xbar), None, p(eb), L.dptr(mb))
+    g_wk = T.empty((nw, nb, nk, n, n), dtype=T.complex128, device="cuda")
+    g_wr, gbar = T.empty_like(g_wk), T.empty_like(g_wk)
+    chi_r = T.empty((nb, nk, n ** 4), dtype=T.complex128, device="cuda")
+    ref = T.empty_like(chi0)
+    eb_r, mb_r = T.empty_like(eb), T.empty_like(mb)
+    L.call("tprf_lattice_dyson_g_wk_batched", c, beta, n_iw, L.dptr(mu), nb, nk, n, p(e_t), p(g_wk))
+    L.call("tprf_fourier_wk_to_wr", c, nw * nb, d3(dims), n * n, p(g_wk), p(g_wr))
+    L.call("tprf_chi0_static_from_gwr_PH", c, beta, n_iw, ntau, d3(dims), n, nb, p(g_wr), p(chi_r))
+    L.call("tprf_fourier_r_to_q", c, d3(dims), d3(q), n ** 4, nb, p(chi_r), p(ref))
+    L.call("tprf_fourier_r_to_q_adjoint", c, d3(dims), d3(q), n ** 4, nb, p(xbar), p(chi_r))
+    L.call("tprf_chi0_static_from_gwr_PH_adjoint", c, beta, n_iw, ntau, d3(dims), n, nb, p(g_wr), p(chi_r), p(gbar))
+    L.call("tprf_fourier_wk_to_wr_adjoint", c, nw * nb, d3(dims), n * n, p(gbar), p(gbar))
+    L.call("tprf_lattice_dyson_g_wk_adjoint", c, n_iw, nk, n, nb, p(g_wk), p(gbar), p(eb_r), L.dptr(mb_r))
+    T.cuda.synchronize()
+    for name, a, b, tol in (("chi0", chi0, ref, 1e-12), ("ebar", eb, eb_r, 1e-11), ("mubar", mb, mb_r, 1e-11)):
+        err = relerr(host(a), host(b))
+        print(f"n 3 {name}: {err:.2e}", flush=True)
+        assert err <= tol, name
+
+
 # ---------------------------------------------------------------- one momentum
 @pytest.mark.parametrize("dims,q", [((8, 8, 1), (4, 4, 0)), ((6, 4, 2), (1, 3, 1)), ((64, 64, 1), (32, 32, 0))])
 def test_r_to_q_and_adjoint(T, oracle, dims, q):

@@ -80,6 +80,10 @@ _SIGNATURES = {
     "tprf_fourier_wr_to_tr_adjoint": [_p, _i, _d, _i, _i, _l, _p, _p, _i],
     "tprf_chi0_static_from_grt_PH": [_p, _d, _i, _p, _i, _l, _p, _p],
     "tprf_chi0_static_from_grt_PH_adjoint": [_p, _d, _i, _p, _i, _l, _p, _p, _p],
+    "tprf_chi0_static_from_gwr_PH": [_p, _d, _i, _i, _p, _i, _l, _p, _p],
+    "tprf_chi0_static_from_gwr_PH_adjoint": [_p, _d, _i, _i, _p, _i, _l, _p, _p, _p],
+    "tprf_chi0_static_q_from_ek": [_p, _d, _i, _i, _p, _p, _i, _l, _p, _p, _p, _p],
+    "tprf_chi0_static_q_from_ek_adjoint": [_p, _d, _i, _i, _p, _p, _i, _l, _p, _p, _p, _p, _p, _p],
     "tprf_fourier_r_to_q": [_p, _p, _p, _l, _l, _p, _p],
     "tprf_fourier_r_to_q_adjoint": [_p, _p, _p, _l, _l, _p, _p],
     "tprf_solve_rpa_PH_adjoint": [_p, _l, _i, _p, _p, _p, _p, _p, _p],

@@ -7,9 +7,21 @@ Forward (each step one C-ABI call on torch's current stream):
 (gf.cpp:91-132, fourier_lattice.cpp:49, fourier_matsubara.cpp:218-308,
 chi_imtime.cpp:139-182, fourier_lattice.cpp:55-58, rpa.cpp:29-64).
 
-Backward runs the hand-written adjoint kernels in reverse order.  Only
-G(iw, k) and chi0(Q, 0) are kept from the forward pass: G(iw, r) and G(tau, r)
-are recomputed by the two transforms instead of being stored.
+Backward runs the hand-written adjoint kernels in reverse order.
+
+route="fused" (default) runs the first five stages as one call,
+tprf_chi0_static_q_from_ek: the nu = 0 bubble of the fitted omega -> tau
+transform is a bilinear form in G(iw) (the Matsubara sum plus a rank-~30
+correction from the tail model and the boundary fit), and at one momentum Q
+it is (1/nk) sum_k B(G(k), G(k - Q)), so G(iw, k) is evaluated in registers
+and neither G(iw, k), G(iw, r) nor G(tau, r) is stored.  Its adjoint
+recomputes G the same way; only e_k, mu, the per-k projections of the tail
+fit and chi0(Q, 0) are kept between the passes.
+route="matsubara" stores G(iw, k), transforms it to G(iw, r) and evaluates
+the same bilinear form per r (tprf_chi0_static_from_gwr_PH), then projects
+r -> Q; only G(iw, k) and chi0(Q, 0) are kept, G(iw, r) is recomputed.
+route="tau" runs fourier_wr_to_tr and the tau-space bubble, the same map
+evaluated the reference's way (and recomputes G(tau, r) in backward).
 
 Gradients follow torch's complex convention (dL/dRe z + i dL/dIm z), so
 e_k, mu and U may come out of any torch expression (hoppings, Kanamori
@@ -35,7 +47,7 @@ class StaticRPA:
     launch over the whole batch; ``graphed_grad`` captures forward + backward of
     L = sum_i Re tr chi_i in one HIP graph."""
 
-    def __init__(self, beta, n_iw, dims, q, norb, ntau=None, nbatch=1, device=None):
+    def __init__(self, beta, n_iw, dims, q, norb, ntau=None, nbatch=1, device=None, route="fused"):
         self.beta = float(beta)
         self.n_iw = int(n_iw)
         self.dims = _dims3(dims)
@@ -48,10 +60,17 @@ class StaticRPA:
         dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
         self.dev = dev
         n, nk, B, c128 = self.n, self.nk, self.nb, torch.complex128
-        self._g_wr = torch.empty((self.nw, B, nk, n, n), dtype=c128, device=dev)
-        self._g_tr = torch.empty((self.ntau, B, nk, n, n), dtype=c128, device=dev)
-        self._x_tr = torch.empty((self.ntau, B, nk, n, n), dtype=c128, device=dev)
-        self._chi_r = torch.empty((B, nk, n, n, n, n), dtype=c128, device=dev)
+        if route not in ("fused", "matsubara", "tau"):
+            raise ValueError("route must be 'fused', 'matsubara' or 'tau'")
+        self.route = route
+        if route != "fused":
+            self._g_wr = torch.empty((self.nw, B, nk, n, n), dtype=c128, device=dev)
+            self._chi_r = torch.empty((B, nk, n, n, n, n), dtype=c128, device=dev)
+        if route == "tau":
+            self._g_tr = torch.empty((self.ntau, B, nk, n, n), dtype=c128, device=dev)
+            self._x_tr = torch.empty((self.ntau, B, nk, n, n), dtype=c128, device=dev)
+        elif route == "matsubara":
+            self._gbar_wr = torch.empty((self.nw, B, nk, n, n), dtype=c128, device=dev)
         self._cdims = _lib.dims3(self.dims)
         self._cq = _lib.dims3(self.q)
         self._graph = None
@@ -74,23 +93,59 @@ class StaticRPA:
         _lib.call("tprf_fourier_wr_to_tr", c, _lib.FERMION, self.beta, self.n_iw, self.ntau, B * nk * n * n,
                   p(self._g_wr), p(self._g_tr), None)
 
-    def _forward_into(self, c, e_k, mu, U, g_wk, chi0_q, chi):
+    def _keep(self, e_k, mu):
+        """What backward needs from forward: (G(i w, k),) on the matsubara and tau
+        routes; (e_k, mu, per-k projections [29, nbatch, nk, n, n]) on the fused one."""
+        n, nk, B = self.n, self.nk, self.nb
+        if self.route == "fused":
+            return (e_k, mu, self._new(29, B, nk, n, n))
+        return (self._new(self.nw, B, nk, n, n),)
+
+    def _forward_into(self, c, e_k, mu, U, keep, chi0_q, chi):
         p = _lib.ptr
         n, nk, B = self.n, self.nk, self.nb
+        if self.route == "fused":
+            _lib.call("tprf_chi0_static_q_from_ek", c, self.beta, self.n_iw, self.ntau, self._cdims, self._cq, n, B,
+                      _lib.dptr(mu), p(e_k), p(chi0_q), p(keep[2]))
+            _lib.call("tprf_solve_rpa_PH_adjoint", c, B, n, p(chi0_q), p(U), None, p(chi), None, None)
+            return
+        g_wk = keep[0]
         _lib.call("tprf_lattice_dyson_g_wk_batched", c, self.beta, self.n_iw, _lib.dptr(mu), B, nk, n, p(e_k),
                   p(g_wk))
-        self._g_tr_from_g_wk(c, g_wk)
-        _lib.call("tprf_chi0_static_from_grt_PH", c, self.beta, self.ntau, self._cdims, n, B, p(self._g_tr),
-                  p(self._chi_r))
+        if self.route == "tau":
+            self._g_tr_from_g_wk(c, g_wk)
+            _lib.call("tprf_chi0_static_from_grt_PH", c, self.beta, self.ntau, self._cdims, n, B, p(self._g_tr),
+                      p(self._chi_r))
+        else:
+            _lib.call("tprf_fourier_wk_to_wr", c, self.nw * B, self._cdims, n * n, p(g_wk), p(self._g_wr))
+            _lib.call("tprf_chi0_static_from_gwr_PH", c, self.beta, self.n_iw, self.ntau, self._cdims, n, B,
+                      p(self._g_wr), p(self._chi_r))
         _lib.call("tprf_fourier_r_to_q", c, self._cdims, self._cq, n ** 4, B, p(self._chi_r), p(chi0_q))
         _lib.call("tprf_solve_rpa_PH_adjoint", c, B, n, p(chi0_q), p(U), None, p(chi), None, None)
 
-    def _backward_into(self, c, g_wk, chi0_q, U, chibar, chi, chi0bar, Ubar, ebar, mubar, recompute=True):
+    def _backward_into(self, c, keep, chi0_q, U, chibar, chi, chi0bar, Ubar, ebar, mubar, recompute=True):
         p = _lib.ptr
         n, nk, B = self.n, self.nk, self.nb
         _lib.call("tprf_solve_rpa_PH_adjoint", c, B, n, p(chi0_q), p(U), p(chibar), p(chi), p(chi0bar), p(Ubar))
+        if self.route == "fused":
+            e_k, mu, proj = keep
+            _lib.call("tprf_chi0_static_q_from_ek_adjoint", c, self.beta, self.n_iw, self.ntau, self._cdims,
+                      self._cq, n, B, _lib.dptr(mu), p(e_k), p(chi0bar), p(proj), p(ebar), _lib.dptr(mubar))
+            return
+        g_wk = keep[0]
         chibar_r = self._chi_r
         _lib.call("tprf_fourier_r_to_q_adjoint", c, self._cdims, self._cq, n ** 4, B, p(chi0bar), p(chibar_r))
+        if self.route == "matsubara":
+            if recompute:  # G(i w, r) from the kept G(i w, k)
+                _lib.call("tprf_fourier_wk_to_wr", c, self.nw * B, self._cdims, n * n, p(g_wk), p(self._g_wr))
+            gbar_wr = self._gbar_wr
+            _lib.call("tprf_chi0_static_from_gwr_PH_adjoint", c, self.beta, self.n_iw, self.ntau, self._cdims, n, B,
+                      p(self._g_wr), p(chibar_r), p(gbar_wr))
+            # k -> r adjoint in place: gbar(i w, k) overwrites gbar(i w, r)
+            _lib.call("tprf_fourier_wk_to_wr_adjoint", c, self.nw * B, self._cdims, n * n, p(gbar_wr), p(gbar_wr))
+            _lib.call("tprf_lattice_dyson_g_wk_adjoint", c, self.n_iw, nk, n, B, p(g_wk), p(gbar_wr), p(ebar),
+                      _lib.dptr(mubar))
+            return
         if recompute:
             # recompute G(tau, r) from the kept G(iw, k)
             self._g_tr_from_g_wk(c, g_wk)
@@ -110,22 +165,23 @@ class StaticRPA:
 
     # -- eager API --------------------------------------------------------------
     def forward(self, e_k, mu, U, check_tail=True):
-        """Returns (chi[nbatch, n,n,n,n], chi0_q, g_wk); g_wk is what backward needs."""
+        """Returns (chi[nbatch, n,n,n,n], chi0_q, keep); keep (a tuple of tensors) is
+        what backward needs."""
         n, nk, B = self.n, self.nk, self.nb
         e_k, mu, U = self._inputs(e_k, mu, U)
         c = self._ctx()
-        g_wk = self._new(self.nw, B, nk, n, n)
+        keep = self._keep(e_k, mu)
         chi0_q, chi = self._new(B, n, n, n, n), self._new(B, n, n, n, n)
         if check_tail:
             _lib.tail_status_reset(c)
-        self._forward_into(c, e_k, mu, U, g_wk, chi0_q, chi)
+        self._forward_into(c, e_k, mu, U, keep, chi0_q, chi)
         if check_tail:
             err, _ = _lib.tail_status(c)
             if err >= 1e-3:
                 raise _lib.TprfError(-4, f"tail fit error {err:.3g} >= 1e-3")
-        return chi, chi0_q, g_wk
+        return chi, chi0_q, keep
 
-    def backward(self, g_wk, chi0_q, U, chibar):
+    def backward(self, keep, chi0_q, U, chibar):
         """Returns (ebar_k[nbatch, nk, n, n], mubar[nbatch], Ubar[nbatch, n, n, n, n])."""
         n, nk, B = self.n, self.nk, self.nb
         U = U.detach().to(self.dev, torch.complex128).reshape(B, n, n, n, n).contiguous()
@@ -133,7 +189,7 @@ class StaticRPA:
         chi, chi0bar, Ubar = (self._new(B, n, n, n, n) for _ in range(3))
         ebar = self._new(B, nk, n, n)
         mubar = self._new(B, dtype=torch.float64)
-        self._backward_into(self._ctx(), g_wk, chi0_q, U, chibar, chi, chi0bar, Ubar, ebar, mubar)
+        self._backward_into(self._ctx(), keep, chi0_q, U, chibar, chi, chi0bar, Ubar, ebar, mubar)
         return ebar, mubar, Ubar
 
     # -- one HIP graph: forward + backward of L = sum_i Re tr chi_i ------------
@@ -143,11 +199,12 @@ class StaticRPA:
         as views of static buffers, valid until the next call.  Forward and backward
         run back to back in one launch, so the backward uses G(tau, r) from the
         forward's transient work buffer instead of recomputing it (the per-call
-        autograd path recomputes it, since it keeps only G(i w, k) between calls)."""
+        autograd path recomputes it, since it keeps only G(i w, k) between calls):
+        G(i w, r) on the Matsubara route, G(tau, r) on the tau route."""
         n, nk, B = self.n, self.nk, self.nb
         e_k, mu, U = self._inputs(e_k, mu, U)
         if self._graph is None:
-            st = {"e_k": e_k.clone(), "mu": mu.clone(), "U": U.clone(), "g_wk": self._new(self.nw, B, nk, n, n),
+            st = {"e_k": e_k.clone(), "mu": mu.clone(), "U": U.clone(),
                   "chi0_q": self._new(B, n, n, n, n), "chi": self._new(B, n, n, n, n),
                   "chi2": self._new(B, n, n, n, n), "chi0bar": self._new(B, n, n, n, n),
                   "Ubar": self._new(B, n, n, n, n), "ebar": self._new(B, nk, n, n),
@@ -157,11 +214,12 @@ class StaticRPA:
                 for b in range(n):
                     cb[:, a, a, b, b] = 1.0  # d(Re sum_ab chi_aabb) in the complex-gradient convention
             st["chibar"] = cb.to(self.dev)
+            st["keep"] = self._keep(st["e_k"], st["mu"])
 
             def run():
                 c = self._ctx()
-                self._forward_into(c, st["e_k"], st["mu"], st["U"], st["g_wk"], st["chi0_q"], st["chi"])
-                self._backward_into(c, st["g_wk"], st["chi0_q"], st["U"], st["chibar"], st["chi2"],
+                self._forward_into(c, st["e_k"], st["mu"], st["U"], st["keep"], st["chi0_q"], st["chi"])
+                self._backward_into(c, st["keep"], st["chi0_q"], st["U"], st["chibar"], st["chi2"],
                                     st["chi0bar"], st["Ubar"], st["ebar"], st["mubar"], recompute=False)
 
             # warm-up outside the graph: plans and tables are made here, and the
@@ -198,18 +256,20 @@ class StaticRPA:
 class _StaticRPAFunction(torch.autograd.Function):
     @staticmethod
     def forward(fctx, e_k, mu, U, spec):
-        chi, chi0_q, g_wk = spec.forward(e_k, mu, U)
+        chi, chi0_q, keep = spec.forward(e_k, mu, U)
         fctx.spec = spec
         fctx.mu_like = (mu.device, mu.dtype, mu.shape) if torch.is_tensor(mu) else None
         fctx.e_meta = (e_k.device, e_k.dtype, e_k.shape)
         fctx.u_meta = (U.device, U.dtype, U.shape)
-        fctx.save_for_backward(g_wk, chi0_q, U.detach())
+        fctx.nkeep = len(keep)
+        fctx.save_for_backward(*keep, chi0_q, U.detach())
         return chi.reshape(U.shape)
 
     @staticmethod
     def backward(fctx, chibar):
-        g_wk, chi0_q, U = fctx.saved_tensors
-        ebar, mubar, Ubar = fctx.spec.backward(g_wk, chi0_q, U, chibar)
+        saved = fctx.saved_tensors
+        keep, chi0_q, U = saved[: fctx.nkeep], saved[-2], saved[-1]
+        ebar, mubar, Ubar = fctx.spec.backward(keep, chi0_q, U, chibar)
         ge = ebar.to(*fctx.e_meta[:2]).reshape(fctx.e_meta[2]) if fctx.needs_input_grad[0] else None
         gm = None
         if fctx.needs_input_grad[1] and fctx.mu_like is not None:

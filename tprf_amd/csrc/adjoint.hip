@@ -419,6 +419,11 @@ hipError_t launch_dyson_adjoint(hipStream_t s, int nw, long nk, int n, long nbat
   return hipGetLastError();
 }
 
+hipError_t launch_sum_partials(hipStream_t s, const double *part, long nbatch, int n, double *out) {
+  hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)nbatch), dim3(256), 0, s, part, n, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_bubble_static(hipStream_t s, int ntau, const int dims[3], int n, long nbatch, const c2 *gtr,
                                 const double *cw, c2 *part, int nchunk, c2 *chi_r) {
   const long nr = (long)dims[0] * dims[1] * dims[2] * nbatch;

@@ -34,6 +34,7 @@ struct TailEntry {
   int *idx = nullptr;
   c2 *pinv = nullptr;
   c2 *vander = nullptr;
+  int *fitpos = nullptr;  // [mesh size]: position f of the mesh index in idx, or -1
 };
 
 struct MatsEntry {
@@ -65,6 +66,7 @@ struct tprf_ctx {
   tprf::Vinv8 vinv;
   std::map<std::tuple<double, int, int, int, int, int>, BubbleEntry> bubbles;
   std::map<std::tuple<double, int>, double *> static_w;  // nu = 0 weights c_j
+  std::map<std::tuple<double, int, int>, std::pair<double *, tprf::StaticMatsDev>> static_mats;
   std::map<std::tuple<int, double, int, int>, double *> dtail;  // out-of-mesh pole model in tau (real)
   std::map<std::tuple<int, double, int, int>, c2 *> otab;       // per-tau output records of omega -> tau
   void *scratch = nullptr;  // this device's copy of the no-allocation entry point's file-scope array
@@ -227,6 +229,9 @@ int get_tail(tprf_ctx *ctx, int statistic, double beta, int n_max, const tprf::T
   if ((st = upload(ctx, &e.idx, idx))) return st;
   if ((st = upload(ctx, &e.pinv, pinv))) return st;
   if ((st = upload(ctx, &e.vander, vd))) return st;
+  std::vector<int> pos(N, -1);
+  for (int f = 0; f < nfit; ++f) pos[idx[f]] = f;
+  if ((st = upload(ctx, &e.fitpos, pos))) return st;
   e.dev.n_fit = nfit;
   e.dev.ncoef = nc;
   e.dev.idx = e.idx;
@@ -301,14 +306,14 @@ void vandermonde_inverse8(double *vinv) {
 // a_q-bar = sum_j D_q(tau_j) gbar_j.  D is small (the pole tail beyond the mesh)
 // while both of its terms are O(1); it is formed here in long double so that
 // neither kernel takes the difference of two large sums.
-static std::vector<double> dtail_values(int statistic, double beta, int n_max, int L) {
+static std::vector<long double> dtail_values_ld(int statistic, double beta, int n_max, int L) {
   typedef std::complex<long double> lc;
   const long double bf[3] = {0.0L, 1.0L, -1.0L}, bb[3] = {-0.5L, -1.0L, 1.0L};
   const long double *b = statistic == TPRF_FERMION ? bf : bb;
   const int first = statistic == TPRF_FERMION ? -n_max : -(n_max - 1);
   const int nmesh = statistic == TPRF_FERMION ? 2 * n_max : 2 * n_max - 1;
   const long double lb = beta;
-  std::vector<double> d(3 * (size_t)L);
+  std::vector<long double> d(3 * (size_t)L);
   std::vector<lc> pole(3 * (size_t)nmesh);
   for (int q = 0; q < 3; ++q)
     for (int i = 0; i < nmesh; ++i) {
@@ -343,10 +348,15 @@ static std::vector<double> dtail_values(int statistic, double beta, int n_max, i
       // real: the mesh is symmetric under w -> -w and the terms of w and -w are
       // complex conjugates (b_q real), so only rounding is left in v.imag()
       const lc v = lc(psi, 0) - s[q] / lb;
-      d[(size_t)q * L + j] = (double)v.real();
+      d[(size_t)q * L + j] = v.real();
     }
   }
   return d;
+}
+
+static std::vector<double> dtail_values(int statistic, double beta, int n_max, int L) {
+  const std::vector<long double> d = dtail_values_ld(statistic, beta, n_max, L);
+  return std::vector<double>(d.begin(), d.end());
 }
 
 int get_dtail(tprf_ctx *ctx, int statistic, double beta, int n_max, int L, const double **out) {
@@ -498,10 +508,7 @@ int get_work(tprf_ctx *ctx, size_t bytes, void **out) {
 //   m1 = x_L - x_0,  m2 = gl0 - gr0,  m3 = -(2/dtau)(gl1 + gr1),
 //   gl_i = sum_m V_{i,m} (x_m - x_0)/(m dtau),  gr_i = sum_m V_{i,m} (x_L - x_{L-m})/(m dtau),
 // all linear in x with real coefficients, so chi(0) = sum_j c_j x_j.
-int get_static_weights(tprf_ctx *ctx, double beta, int L, const double **out) {
-  auto key = std::make_tuple(beta, L);
-  auto it = ctx->static_w.find(key);
-  if (it != ctx->static_w.end()) { *out = it->second; return TPRF_OK; }
+std::vector<long double> static_weights_ld(const tprf_ctx *ctx, double beta, int L) {
   const double bb[3] = {-0.5, -1.0, 1.0};
   // a_q = sum_p A[q][p] m_p (bosonic three-pole model)
   const long double A[3][3] = {{4.0L / 3, 0, -4.0L / 3}, {-0.5L, -0.5L, 1}, {1.0L / 6, 0.5L, 1.0L / 3}};
@@ -531,6 +538,14 @@ int get_static_weights(tprf_ctx *ctx, double beta, int L, const double **out) {
     c[L] += -k2 * v0 + k3 * v1;
     c[L - m] -= -k2 * v0 + k3 * v1;
   }
+  return c;
+}
+
+int get_static_weights(tprf_ctx *ctx, double beta, int L, const double **out) {
+  auto key = std::make_tuple(beta, L);
+  auto it = ctx->static_w.find(key);
+  if (it != ctx->static_w.end()) { *out = it->second; return TPRF_OK; }
+  const std::vector<long double> c = static_weights_ld(ctx, beta, L);
   std::vector<double> cd(L + 1);
   for (int j = 0; j <= L; ++j) cd[j] = (double)c[j];
   double *d;
@@ -538,6 +553,81 @@ int get_static_weights(tprf_ctx *ctx, double beta, int L, const double **out) {
   if (st) return st;
   ctx->static_w.emplace(key, d);
   *out = d;
+  return TPRF_OK;
+}
+
+// Tables of the nu = 0 bubble evaluated in Matsubara space (static_mats.hip)
+// for a fermionic mesh of n_max = n_iw and L = ntau - 1, in long double:
+// per n >= 0 the row [cos(w_n tau_j) / beta (j = 0..8), sin(w_n tau_j) / beta
+// (j = 1..8), Re Y_q, Im Y_q, Re Y'_q, Im Y'_q (q = 0..3)] with
+//   Y_qn  = -(1/beta) sum_{j=0..L} c_j Z_jq exp(+i w_n tau_j)
+//   Y'_qn =  (1/beta) sum_{j=0..L} c_j Z_{L-j,q} exp(-i w_n tau_j),
+// Z_jq = D_q(tau_j) (j < L, q < 3), Z_Lq = -D_q(0), Z_L3 = -1, Z_j3 = 0 (j < L);
+// and e_j = -(dc_j + dc_{L-j}), f_j = dc_j - dc_{L-j}, dc_j = c_j - beta/L [j < L],
+// Q_pq = sum_j c_j Z_jp Z_{L-j,q}.  Needs L >= 17 (disjoint boundary sets).
+int get_static_mats(tprf_ctx *ctx, double beta, int n_iw, int L, const double **tab, tprf::StaticMatsDev *sm) {
+  auto key = std::make_tuple(beta, n_iw, L);
+  auto it = ctx->static_mats.find(key);
+  if (it == ctx->static_mats.end()) {
+    typedef long double ld;
+    const std::vector<ld> c = static_weights_ld(ctx, beta, L);
+    const std::vector<ld> D = dtail_values_ld(TPRF_FERMION, beta, n_iw, L);
+    const ld lb = beta;
+    std::vector<ld> Z((size_t)(L + 1) * 4, 0.0L);
+    for (int j = 0; j < L; ++j)
+      for (int q = 0; q < 3; ++q) Z[(size_t)j * 4 + q] = D[(size_t)q * L + j];
+    for (int q = 0; q < 3; ++q) Z[(size_t)L * 4 + q] = -D[(size_t)q * L];
+    Z[(size_t)L * 4 + 3] = -1.0L;
+    std::vector<ld> cs(2 * (size_t)L), sn(2 * (size_t)L);  // angle pi k / L, k < 2L
+    for (long k = 0; k < 2L * L; ++k) {
+      cs[k] = cosl(kPiL * (ld)k / (ld)L);
+      sn[k] = sinl(kPiL * (ld)k / (ld)L);
+    }
+    std::vector<double> t((size_t)n_iw * tprf::kStaticTabWidth);
+    for (int n = 0; n < n_iw; ++n) {
+      double *row = &t[(size_t)n * tprf::kStaticTabWidth];
+      auto ang = [&](long j) { return ((2L * n + 1) * j) % (2L * L); };  // w_n tau_j = pi ang / L
+      for (int j = 0; j <= 8; ++j) row[j] = (double)(cs[ang(j)] / lb);
+      for (int j = 1; j <= 8; ++j) row[8 + j] = (double)(sn[ang(j)] / lb);
+      for (int q = 0; q < 4; ++q) {
+        ld yr = 0, yi = 0, pr = 0, pi = 0;
+        for (long j = 0; j <= L; ++j) {
+          const long a = ang(j);
+          const ld wz = c[j] * Z[(size_t)j * 4 + q], wp = c[j] * Z[(size_t)(L - j) * 4 + q];
+          yr += wz * cs[a];
+          yi += wz * sn[a];
+          pr += wp * cs[a];
+          pi += wp * sn[a];
+        }
+        row[17 + q] = (double)(-yr / lb);
+        row[21 + q] = (double)(-yi / lb);
+        row[25 + q] = (double)(pr / lb);
+        row[29 + q] = (double)(-pi / lb);
+      }
+    }
+    tprf::StaticMatsDev d{};
+    d.inv_beta = 1.0 / beta;
+    const ld fact = lb / (ld)L;
+    auto dc = [&](long j) { return c[j] - (j < L ? fact : 0.0L); };
+    d.e[0] = (double)(-(dc(0) + dc(L)));
+    d.f[0] = 0.0;
+    for (int j = 1; j <= 8; ++j) {
+      d.e[j] = (double)(-(dc(j) + dc(L - j)));
+      d.f[j] = (double)(dc(j) - dc(L - j));
+    }
+    for (int p = 0; p < 4; ++p)
+      for (int q = 0; q < 4; ++q) {
+        ld acc = 0;
+        for (long j = 0; j <= L; ++j) acc += c[j] * Z[(size_t)j * 4 + p] * Z[(size_t)(L - j) * 4 + q];
+        d.Q[p * 4 + q] = (double)acc;
+      }
+    double *dt;
+    int st = upload(ctx, &dt, t);
+    if (st) return st;
+    it = ctx->static_mats.emplace(key, std::make_pair(dt, d)).first;
+  }
+  *tab = it->second.first;
+  *sm = it->second.second;
   return TPRF_OK;
 }
 
@@ -676,9 +766,10 @@ int tprf_ctx_destroy(tprf_ctx *ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   else (void)hipDeviceSynchronize();
   for (auto &kv : ctx->fft) (void)hipFree(kv.second.tw);
-  for (auto &kv : ctx->fits) { (void)hipFree(kv.second.idx); (void)hipFree(kv.second.pinv); (void)hipFree(kv.second.vander); }
+  for (auto &kv : ctx->fits) { (void)hipFree(kv.second.idx); (void)hipFree(kv.second.pinv); (void)hipFree(kv.second.vander); (void)hipFree(kv.second.fitpos); }
   for (auto &kv : ctx->mats) { (void)hipFree(kv.second.psi); (void)hipFree(kv.second.phase); }
   for (auto &kv : ctx->static_w) (void)hipFree(kv.second);
+  for (auto &kv : ctx->static_mats) (void)hipFree(kv.second.first);
   for (auto &kv : ctx->dtail) (void)hipFree(kv.second);
   for (auto &kv : ctx->otab) (void)hipFree(kv.second);
   for (auto &kv : ctx->symcols) { (void)hipFree(kv.second.col); (void)hipFree(kv.second.mcol); }
@@ -1259,6 +1350,121 @@ int tprf_chi0_static_from_grt_PH_adjoint(tprf_ctx *ctx, double beta, int ntau, c
                       tprf::launch_bubble_static_adjoint(ctx->stream, ntau, dims, norb, nbatch, (const c2 *)g_tr,
                                                          (const c2 *)chibar_r, cw, (c2 *)gbar_tr),
                       "static bubble adjoint kernel");
+}
+
+namespace {
+int static_gwr_check(tprf_ctx *ctx, double beta, int n_iw, int &ntau, const int dims[3], int norb, long nbatch,
+                     const char *what) {
+  if (!(beta > 0) || n_iw < 1 || !dims_ok(dims) || nbatch < 1) return fail(ctx, TPRF_ERR_ARG, std::string(what) + ": bad argument");
+  if (norb < 1 || norb > 3) return fail(ctx, TPRF_ERR_UNSUPPORTED, std::string(what) + ": norb must be 1..3");
+  if (ntau <= 0) ntau = 6 * n_iw + 1;
+  const int L = ntau - 1;
+  if (L < 2 * n_iw)
+    return fail(ctx, TPRF_ERR_MESH_TOO_SHORT,
+                "Inverse Fourier: The time mesh mush be at least twice as long as the freq mesh");
+  if (L < 17) return fail(ctx, TPRF_ERR_UNSUPPORTED, std::string(what) + ": ntau must be at least 18");
+  return TPRF_OK;
+}
+}  // namespace
+
+int tprf_chi0_static_from_gwr_PH(tprf_ctx *ctx, double beta, int n_iw, int ntau, const int dims[3], int norb,
+                                 long nbatch, const tprf_c128 *g_wr, tprf_c128 *chi0_r) {
+  if (!ctx) return TPRF_ERR_ARG;
+  int st = static_gwr_check(ctx, beta, n_iw, ntau, dims, norb, nbatch, "chi0_static_from_gwr_PH");
+  if (st) return st;
+  if (!g_wr || !chi0_r) return fail(ctx, TPRF_ERR_ARG, "chi0_static_from_gwr_PH: null pointer");
+  const tprf::TailFitDev *fit;
+  if ((st = get_tail(ctx, TPRF_FERMION, beta, n_iw, &fit))) return st;
+  const double *tab;
+  tprf::StaticMatsDev sm;
+  if ((st = get_static_mats(ctx, beta, n_iw, ntau - 1, &tab, &sm))) return st;
+  void *w;
+  if ((st = get_work(ctx, tprf::static_mats_work_bytes(nbatch, dims, norb, n_iw, false), &w))) return st;
+  return check_launch(ctx,
+                      tprf::launch_static_mats(ctx->stream, nbatch, dims, norb, n_iw, (const c2 *)g_wr, tab, *fit, sm,
+                                               (c2 *)w, (c2 *)chi0_r, ctx->d_status),
+                      "static bubble (Matsubara) kernels");
+}
+
+int tprf_chi0_static_from_gwr_PH_adjoint(tprf_ctx *ctx, double beta, int n_iw, int ntau, const int dims[3], int norb,
+                                         long nbatch, const tprf_c128 *g_wr, const tprf_c128 *chibar_r,
+                                         tprf_c128 *gbar_wr) {
+  if (!ctx) return TPRF_ERR_ARG;
+  int st = static_gwr_check(ctx, beta, n_iw, ntau, dims, norb, nbatch, "chi0_static_from_gwr_PH_adjoint");
+  if (st) return st;
+  if (!g_wr || !chibar_r || !gbar_wr) return fail(ctx, TPRF_ERR_ARG, "chi0_static_from_gwr_PH_adjoint: null pointer");
+  const tprf::TailFitDev *fit;
+  if ((st = get_tail(ctx, TPRF_FERMION, beta, n_iw, &fit))) return st;
+  const double *tab;
+  tprf::StaticMatsDev sm;
+  if ((st = get_static_mats(ctx, beta, n_iw, ntau - 1, &tab, &sm))) return st;
+  void *w;
+  if ((st = get_work(ctx, tprf::static_mats_work_bytes(nbatch, dims, norb, n_iw, true), &w))) return st;
+  return check_launch(ctx,
+                      tprf::launch_static_mats_adjoint(ctx->stream, nbatch, dims, norb, n_iw, (const c2 *)g_wr, tab,
+                                                       *fit, sm, (c2 *)w, (const c2 *)chibar_r, (c2 *)gbar_wr,
+                                                       ctx->d_status),
+                      "static bubble (Matsubara) adjoint kernels");
+}
+
+namespace {
+// common checks and tables of the two fused static entry points; qr = q reduced into [0, dims)
+int static_q_setup(tprf_ctx *ctx, double beta, int n_iw, int &ntau, const int dims[3], const int q[3], int norb,
+                   long nbatch, const char *what, int qr[3], const tprf::TailFitDev **fit, const int **fitpos,
+                   const double **tab, tprf::StaticMatsDev *sm) {
+  int st = static_gwr_check(ctx, beta, n_iw, ntau, dims, norb, nbatch, what);
+  if (st) return st;
+  if (!q) return fail(ctx, TPRF_ERR_ARG, std::string(what) + ": null momentum");
+  for (int d = 0; d < 3; ++d) qr[d] = (int)(((long)q[d] % dims[d] + dims[d]) % dims[d]);
+  if ((st = get_tail(ctx, TPRF_FERMION, beta, n_iw, fit))) return st;
+  *fitpos = ctx->fits.find(std::make_tuple((int)TPRF_FERMION, beta, n_iw))->second.fitpos;
+  return get_static_mats(ctx, beta, n_iw, ntau - 1, tab, sm);
+}
+}  // namespace
+
+int tprf_chi0_static_q_from_ek(tprf_ctx *ctx, double beta, int n_iw, int ntau, const int dims[3], const int q[3],
+                               int norb, long nbatch, const double *mu, const tprf_c128 *e_k, tprf_c128 *chi0_q,
+                               tprf_c128 *proj) {
+  if (!ctx) return TPRF_ERR_ARG;
+  const char *what = "chi0_static_q_from_ek";
+  int qr[3];
+  const tprf::TailFitDev *fit;
+  const int *fitpos;
+  const double *tab;
+  tprf::StaticMatsDev sm;
+  int st = static_q_setup(ctx, beta, n_iw, ntau, dims, q, norb, nbatch, what, qr, &fit, &fitpos, &tab, &sm);
+  if (st) return st;
+  if (!mu || !e_k || !chi0_q) return fail(ctx, TPRF_ERR_ARG, std::string(what) + ": null pointer");
+  void *w;
+  if ((st = get_work(ctx, tprf::static_q_work_bytes(nbatch, dims, norb, n_iw, false), &w))) return st;
+  return check_launch(ctx,
+                      tprf::launch_static_q(ctx->stream, nbatch, dims, qr, norb, n_iw, beta, mu, (const c2 *)e_k, tab,
+                                            fitpos, *fit, sm, (c2 *)w, (c2 *)proj, false, nullptr, (c2 *)chi0_q,
+                                            nullptr, nullptr, ctx->d_status),
+                      "static bubble (k space) kernels");
+}
+
+int tprf_chi0_static_q_from_ek_adjoint(tprf_ctx *ctx, double beta, int n_iw, int ntau, const int dims[3],
+                                       const int q[3], int norb, long nbatch, const double *mu, const tprf_c128 *e_k,
+                                       const tprf_c128 *chibar_q, const tprf_c128 *proj, tprf_c128 *ebar_k,
+                                       double *mubar) {
+  if (!ctx) return TPRF_ERR_ARG;
+  const char *what = "chi0_static_q_from_ek_adjoint";
+  int qr[3];
+  const tprf::TailFitDev *fit;
+  const int *fitpos;
+  const double *tab;
+  tprf::StaticMatsDev sm;
+  int st = static_q_setup(ctx, beta, n_iw, ntau, dims, q, norb, nbatch, what, qr, &fit, &fitpos, &tab, &sm);
+  if (st) return st;
+  if (!mu || !e_k || !chibar_q || !ebar_k || !mubar) return fail(ctx, TPRF_ERR_ARG, std::string(what) + ": null pointer");
+  void *w;
+  if ((st = get_work(ctx, tprf::static_q_work_bytes(nbatch, dims, norb, n_iw, true), &w))) return st;
+  return check_launch(ctx,
+                      tprf::launch_static_q(ctx->stream, nbatch, dims, qr, norb, n_iw, beta, mu, (const c2 *)e_k, tab,
+                                            fitpos, *fit, sm, (c2 *)w, (c2 *)proj, proj != nullptr,
+                                            (const c2 *)chibar_q, nullptr, (c2 *)ebar_k, mubar, ctx->d_status),
+                      "static bubble (k space) adjoint kernels");
 }
 
 int tprf_fourier_r_to_q(tprf_ctx *ctx, const int dims[3], const int q[3], long ntarget, long nbatch,

@@ -77,6 +77,18 @@ struct MatsDev {
   const c2 *phase;       // [L] exp(-i pi j / L) (fermions), unused for bosons
 };
 
+// The nu = 0 bubble in Matsubara space (static_mats.hip): per n >= 0 a table
+// row of kStaticTabWidth reals (cos(w_n tau_j) / beta, j = 0..8;
+// sin(w_n tau_j) / beta, j = 1..8; Re and Im of Y_q and Y'_q, q = 0..3), and
+// the coefficients of the small bilinear part, passed by value.
+constexpr int kStaticTabWidth = 33;
+struct StaticMatsDev {
+  double inv_beta;
+  double e[9];   // -(dc_j + dc_{L-j})
+  double f[9];   // dc_j - dc_{L-j} (f[0] unused)
+  double Q[16];  // Q_pq = sum_j c_j Z_jp Z_{L-j,q}
+};
+
 // Inverse 8x8 Vandermonde of the boundary-derivative fit, passed by value.
 struct Vinv8 {
   double v[64];
@@ -207,6 +219,23 @@ hipError_t launch_bubble_static(hipStream_t s, int ntau, const int dims[3], int 
 hipError_t launch_rpa_point(hipStream_t s, long npts, int n, const c2 *chi0, const c2 *U, const c2 *chibar, c2 *chi,
                             c2 *xbar, c2 *ubar, Status st);
 // kernels.hip: single-pass 3-D lattice FFT; hipErrorNotSupported for other dims
+// device workspace (bytes) of the launchers below
+size_t static_mats_work_bytes(long nbatch, const int dims[3], int n, int n_iw, bool adjoint);
+hipError_t launch_static_mats(hipStream_t s, long nbatch, const int dims[3], int n, int n_iw, const c2 *g,
+                              const double *tab, const TailFitDev &fit, const StaticMatsDev &sm, c2 *work, c2 *chi,
+                              double *status);
+hipError_t launch_static_mats_adjoint(hipStream_t s, long nbatch, const int dims[3], int n, int n_iw, const c2 *g,
+                                      const double *tab, const TailFitDev &fit, const StaticMatsDev &sm, c2 *work,
+                                      const c2 *chibar, c2 *gbar, double *status);
+// static_q.hip: chi0(Q, 0) straight from e_k (G formed in registers), and its
+// adjoint to (e_k, mu); proj[29][nbatch nk n^2] optional (nullptr: in work)
+size_t static_q_work_bytes(long nbatch, const int dims[3], int n, int n_iw, bool adjoint);
+hipError_t launch_static_q(hipStream_t s, long nbatch, const int dims[3], const int q[3], int n, int n_iw, double beta,
+                           const double *mu, const c2 *ek, const double *tab, const int *fitpos, const TailFitDev &fit,
+                           const StaticMatsDev &sm, c2 *work, c2 *proj, bool reuse, const c2 *chibar, c2 *chi0_q,
+                           c2 *ebar, double *mubar, double *status);
+// adjoint.hip: out[b] = sum of part[b n .. b n + n), b < nbatch
+hipError_t launch_sum_partials(hipStream_t s, const double *part, long nbatch, int n, double *out);
 hipError_t launch_lattice3d(hipStream_t s, int sign, const c2 *in, c2 *out, long nw, const int dims[3], long ntarget,
                             double scale, int variant = 0);
 int lattice3d_variants(const int dims[3]);
