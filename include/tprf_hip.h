@@ -234,6 +234,65 @@ int tprf_solve_rpa_PH_herm(tprf_ctx *ctx, int nb, long nk, int norb, const tprf_
 int tprf_dynamical_screened_interaction_W(tprf_ctx *ctx, long nw, long nk, int norb, const tprf_c128 *PI,
                                           const tprf_c128 *V, int dynamic, tprf_c128 *W);
 
+/* ---- f4: GW self-energy pieces (c++/triqs_tprf/lattice/gw.cpp) ---------- */
+/* split_into_dynamic_wk_and_constant_k (lattice_utility.cpp:57-84): per column
+ * of chi_wk [mesh][ncol] the zeroth coefficient of the tail fit (the window
+ * and order of fourier_wr_to_tr's fit) is chi_const_k [ncol], and
+ * chi_dyn_wk = chi_wk - chi_const_k.  statistic: TPRF_FERMION / TPRF_BOSON.  */
+int tprf_split_into_dynamic_wk_and_constant_k(tprf_ctx *ctx, int statistic, double beta, int n_max, long ncol,
+                                              const tprf_c128 *chi_wk, tprf_c128 *chi_dyn_wk,
+                                              tprf_c128 *chi_const_k);
+/* The density of each column of a fermionic g_w [2 n_max][ncol] (the TRIQS
+ * density() that rho_k_from_g_wk, gw.cpp:36-52, applies per k): the fitted
+ * moments t1..t3 regularise the Matsubara sum,
+ *   rho = (1/beta) sum_n [g_n - t1/iw - t2/(iw)^2 - t3/(iw)^3] + t1/2 - t2 beta/4. */
+int tprf_density(tprf_ctx *ctx, double beta, int n_max, long ncol, const tprf_c128 *g_w, tprf_c128 *rho);
+/* gw_dynamic_sigma (gw.cpp:66-102): sigma_ab(tau, r) = -sum_cd W_acdb(tau, r)
+ * G_cd(tau, r) at npts = ntau nr points; W [npts][n^4], g/sigma [npts][n^2].  */
+int tprf_gw_dynamic_sigma(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *W_tr, const tprf_c128 *g_tr,
+                          tprf_c128 *sigma_tr);
+/* fock_sigma(v_r, rho_r) (gw.cpp:149-167): sigma_ab(r) = -sum_cd V_acdb(r) rho_dc(r). */
+int tprf_fock_sigma_r(tprf_ctx *ctx, long nr, int norb, const tprf_c128 *v_r, const tprf_c128 *rho_r,
+                      tprf_c128 *sigma_r);
+/* hartree_sigma(v_k, rho_r) (gw.cpp:113-122): sigma_ab(r = 0) = sum_cd
+ * V_abcd(k = 0) rho_cd(r = 0), zero elsewhere; v_k [nk][n^4] (row 0 read).     */
+int tprf_hartree_sigma_r(tprf_ctx *ctx, long nr, int norb, const tprf_c128 *v_k, const tprf_c128 *rho_r,
+                         tprf_c128 *sigma_r);
+/* hartree_sigma(v_k, g_wk) / fock_sigma(v_k, g_wk) on the k mesh by the direct
+ * sum over q (gw.cpp:124-147 / :169-195), with rho_k the density per k:
+ *   fock = 0: sigma_ab(k) =  (1/N) sum_q sum_cd V_abcd(q) rho_cd(k + q)
+ *   fock = 1: sigma_ab(k) = -(1/N) sum_q sum_cd V_acdb(q) rho_dc(k + q).  O(N_k^2). */
+int tprf_static_sigma_k(tprf_ctx *ctx, int fock, const int dims[3], int norb, const tprf_c128 *v_k,
+                        const tprf_c128 *rho_k, tprf_c128 *sigma_k);
+/* add_dynamic_and_static (lattice_utility.cpp:86-101), in place:
+ * x_w[w][col] += s[col].                                                       */
+int tprf_add_dynamic_and_static(tprf_ctx *ctx, long nw, long ncol, tprf_c128 *x_w, const tprf_c128 *s);
+
+/* ---- f4: linearised Eliashberg (c++/triqs_tprf/lattice/eliashberg.cpp) -- */
+/* eliashberg_g_delta_g_product (eliashberg.cpp:31-58): on Delta's fermionic
+ * mesh (n_max_d <= n_max_g; G read at the same frequency)
+ *   F_dc(w, k) = sum_ef G_cf(w, k) conj(G_ed(w, -k)) Delta_ef(w, k).
+ * g_wk [2 n_max_g][nk][n^2], delta_wk and F_wk [2 n_max_d][nk][n^2].          */
+int tprf_eliashberg_g_delta_g_product(tprf_ctx *ctx, int n_max_g, int n_max_d, const int dims[3], int norb,
+                                      const tprf_c128 *g_wk, const tprf_c128 *delta_wk, tprf_c128 *F_wk);
+/* eliashberg_product by summation (eliashberg.cpp:92-120):
+ *   Delta_ab(w, k) = -1/(2 N beta) sum_{n, q} sum_cd Gamma_cadb(w - n, k - q) F_dc(n, q)
+ * with F from tprf_eliashberg_g_delta_g_product; gamma_wk on a bosonic mesh
+ * of 2 n_max_gamma - 1 >= 2 (2 n_max_d) points.  O(N_w^2 N_k^2).              */
+int tprf_eliashberg_product(tprf_ctx *ctx, double beta, int n_max_gamma, int n_max_g, int n_max_d, const int dims[3],
+                            int norb, const tprf_c128 *gamma_wk, const tprf_c128 *g_wk, const tprf_c128 *delta_wk,
+                            tprf_c128 *delta_out);
+/* eliashberg_dynamic_gamma_f_product / eliashberg_constant_gamma_f_product
+ * (eliashberg.cpp:175-186, 214-240): delta_ab = -1/2 sum_cd Gamma_cadb F_dc
+ * per point; npts = ntau nr (dynamic: Gamma_dyn_tr and F_tr) or nr (constant:
+ * Gamma_const_r and the tau = 0 row of F_tr).                                  */
+int tprf_eliashberg_gamma_f_product(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *gamma, const tprf_c128 *F,
+                                    tprf_c128 *delta);
+/* construct_phi_wk (eliashberg.cpp:336-360): Phi = U chi U per point on the
+ * matrices M[(a,b),(d,c)] = X_abcd; chi/phi [npts][n^4], U [n^4].             */
+int tprf_construct_phi_wk(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *chi, const tprf_c128 *U,
+                          tprf_c128 *phi);
+
 /* ---- analytic bubble ----------------------------------------------------- */
 /* lindhard_chi00(e_k, mesh, mu) and lindhard_chi00(e_k, mesh, beta, mu, delta),
  * lattice_desc.py:411,495 / lindhard_chi00.cpp:34-108:

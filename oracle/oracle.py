@@ -38,6 +38,7 @@ def lib():
             "or_lattice_dyson_g_wk": [d, i, d, i, i, p, i, p, p],
             "or_fourier_lattice": [l, p, l, p, p, i],
             "or_fourier_w_to_t": [i, d, i, i, l, p, p, p, p, p],
+            "or_tail_moments": [i, d, i, l, p, p, p],
             "or_fourier_t_to_w": [i, d, i, i, l, p, p],
             "or_chi0_tr_from_grt_PH": [i, p, i, p, p],
             "or_chi0_wr_from_grt_PH": [d, i, p, i, i, p, p],
@@ -283,3 +284,157 @@ def lindhard_chi00(e_k, dims, beta, mu, nw):
     out = np.empty((2 * nw - 1, nk, n, n, n, n), dtype=np.complex128)
     _check(lib().or_lindhard_chi00(beta, mu, nw, cd, n, _p(e_k), _p(out)), "lindhard")
     return out
+
+
+# ---------------------------------------------------------------------------
+# f4 consumers of chi (GW: c++/triqs_tprf/lattice/gw.cpp; Eliashberg:
+# c++/triqs_tprf/lattice/eliashberg.cpp).  The pointwise contractions are numpy
+# einsums over the reference's index patterns; the transforms are the oracle's.
+
+def tail_moments(xw, beta, statistic):
+    """Fitted moments t_0..t_3 [4, *cols] of every column of xw [mesh, *cols]."""
+    xw = _c(xw)
+    nmesh = xw.shape[0]
+    n_max = nmesh // 2 if statistic == FERMION else (nmesh + 1) // 2
+    ncol = int(np.prod(xw.shape[1:])) if xw.ndim > 1 else 1
+    t = np.empty((4, ncol), dtype=np.complex128)
+    err = ctypes.c_double(0.0)
+    _check(lib().or_tail_moments(statistic, beta, n_max, ncol, _p(xw), _p(t),
+                                 ctypes.cast(ctypes.pointer(err), ctypes.c_void_p)), "tail_moments")
+    return t.reshape((4,) + xw.shape[1:])
+
+
+def split_into_dynamic_wk_and_constant_k(chi_wk, beta, statistic=BOSON):
+    """lattice_utility.cpp:57-84: constant = fitted t_0 per column."""
+    c = tail_moments(chi_wk, beta, statistic)[0]
+    return chi_wk - c[None], c
+
+
+def density(g_w, beta):
+    """TRIQS density() of a fermionic Matsubara function per column (used by
+    rho_k_from_g_wk, gw.cpp:36-52): the Matsubara sum regularised by the fitted
+    moments t1..t3, whose own sums are 1/2, -beta/4 and 0."""
+    g_w = _c(g_w)
+    n_max = g_w.shape[0] // 2
+    t = tail_moments(g_w, beta, FERMION)
+    iw = 1j * np.pi * (2 * np.arange(-n_max, n_max) + 1) / beta
+    shp = (-1,) + (1,) * (g_w.ndim - 1)
+    iw = iw.reshape(shp)
+    rem = g_w - t[1][None] / iw - t[2][None] / iw ** 2 - t[3][None] / iw ** 3
+    return rem.sum(0) / beta + t[1] / 2 - t[2] * beta / 4
+
+
+def gw_dynamic_sigma(W_tr, g_tr):
+    """gw.cpp:88-102: sigma_ab = -sum_cd W_acdb g_cd per (tau, r)."""
+    return -np.einsum("...acdb,...cd->...ab", W_tr, g_tr)
+
+
+def fock_sigma_r(v_r, rho_r):
+    """gw.cpp:149-167: sigma_ab(r) = -sum_cd V_acdb(r) rho_dc(r)."""
+    return -np.einsum("...acdb,...dc->...ab", v_r, rho_r)
+
+
+def hartree_sigma_r(v_k, rho_r):
+    """gw.cpp:113-122: only r = 0, V(k = 0) rho(r = 0)."""
+    out = np.zeros_like(rho_r)
+    out[0] = np.einsum("abcd,cd->ab", v_k[0], rho_r[0])
+    return out
+
+
+def _dims3(dims):
+    return tuple(int(x) for x in dims) + (1,) * (3 - len(dims))
+
+
+def _kq_index(dims, sign):
+    """idx[k, q] = k + sign q on the lattice."""
+    g = np.stack(np.meshgrid(*[np.arange(d) for d in dims], indexing="ij"), -1).reshape(-1, 3)
+    j = (g[:, None, :] + sign * g[None, :, :]) % np.array(dims)
+    return (j[..., 0] * dims[1] + j[..., 1]) * dims[2] + j[..., 2]
+
+
+def static_sigma_k(v_k, rho_k, dims, fock):
+    """hartree_sigma / fock_sigma (v_k, g_wk) by the direct q sum (gw.cpp:124-147,
+    169-195); rho_k from ``density``."""
+    dims = _dims3(dims)
+    nk = v_k.shape[0]
+    rho_kq = rho_k[_kq_index(dims, +1)]  # [k, q, n, n]
+    if fock:
+        return -np.einsum("qacdb,kqdc->kab", v_k, rho_kq) / nk
+    return np.einsum("qabcd,kqcd->kab", v_k, rho_kq) / nk
+
+
+def gw_sigma(W_wk, g_wk, beta, dims):
+    """gw.cpp:204-244 composed from the oracle's transforms: W_wk [2n-1, nk, n^4]
+    bosonic with n_max equal to g_wk's fermionic n_max."""
+    dims = _dims3(dims)
+    n_max = g_wk.shape[0] // 2
+    W_dyn, W_const = split_into_dynamic_wk_and_constant_k(W_wk, beta, BOSON)
+    g_tr = fourier_w_to_t(fourier_lattice(g_wk, dims, -1), beta, FERMION)
+    W_tr = fourier_w_to_t(fourier_lattice(W_dyn, dims, -1), beta, BOSON)
+    s_tr = gw_dynamic_sigma(W_tr, g_tr)
+    W_const_r = fourier_lattice(W_const[None], dims, -1)[0]
+    rho_r = fourier_lattice(density(g_wk, beta)[None], dims, -1)[0]
+    fock_k = fourier_lattice(fock_sigma_r(W_const_r, rho_r)[None], dims, +1)[0]
+    s_wk = fourier_lattice(fourier_t_to_w(s_tr, beta, n_max, FERMION), dims, +1)
+    return s_wk + fock_k[None]
+
+
+def eliashberg_g_delta_g_product(g_wk, delta_wk, dims):
+    """eliashberg.cpp:31-58: F_dc(w, k) = sum_ef G_cf(w, k) conj(G_ed(w, -k)) Delta_ef(w, k)."""
+    dims = _dims3(dims)
+    nwg, nwd = g_wk.shape[0], delta_wk.shape[0]
+    off = (nwg - nwd) // 2
+    g = g_wk[off:off + nwd]
+    mk = _kq_index(dims, -1)[0]  # 0 - k
+    return np.einsum("wkcf,wked,wkef->wkdc", g, np.conj(g[:, mk]), delta_wk)
+
+
+def eliashberg_product(gamma_wk, g_wk, delta_wk, beta, dims):
+    """eliashberg.cpp:92-120 by brute force: Delta_ab(w, k) = -1/(2 N beta)
+    sum_{n, q} Gamma_cadb(w - n, k - q) F_dc(n, q)."""
+    dims = _dims3(dims)
+    F = eliashberg_g_delta_g_product(g_wk, delta_wk, dims)
+    nw, nk = delta_wk.shape[0], delta_wk.shape[1]
+    ng = (gamma_wk.shape[0] + 1) // 2
+    kmq = _kq_index(dims, -1)  # [k, q] -> k - q
+    w = np.arange(nw)
+    gidx = (w[:, None] - w[None, :]) + (ng - 1)  # [w, n]
+    G = gamma_wk[gidx][:, :, kmq]  # [w, n, k, q, c, a, d, b]
+    out = np.einsum("wnkqcadb,nqdc->wkab", G, F, optimize=True)
+    return -out / (2 * nk * beta)
+
+
+def eliashberg_gamma_f(gamma, F):
+    """eliashberg.cpp:175-186, 214-240: Delta_ab = -1/2 sum_cd Gamma_cadb F_dc."""
+    return -0.5 * np.einsum("...cadb,...dc->...ab", gamma, F)
+
+
+def eliashberg_product_fft(gamma_dyn_tr, gamma_const_r, g_wk, delta_wk, beta, dims):
+    """eliashberg.cpp:242-268 composed from the oracle's transforms."""
+    dims = _dims3(dims)
+    n_max = delta_wk.shape[0] // 2
+    F = eliashberg_g_delta_g_product(g_wk, delta_wk, dims)
+    F_tr = fourier_w_to_t(fourier_lattice(F, dims, -1), beta, FERMION)
+    d_tr = eliashberg_gamma_f(gamma_dyn_tr, F_tr)
+    d_r = eliashberg_gamma_f(gamma_const_r, F_tr[0])
+    d_wr = fourier_t_to_w(d_tr, beta, n_max, FERMION) + d_r[None]
+    return fourier_lattice(d_wr, dims, +1)
+
+
+def preprocess_gamma_for_fft(gamma_wk, beta, dims):
+    """eliashberg.py:196-254 with a fitted constant: (Gamma_dyn(tau, r), Gamma_const(r))."""
+    dims = _dims3(dims)
+    dyn, cst = split_into_dynamic_wk_and_constant_k(gamma_wk, beta, BOSON)
+    dyn_tr = fourier_w_to_t(fourier_lattice(dyn, dims, -1), beta, BOSON)
+    return dyn_tr, fourier_lattice(cst[None], dims, -1)[0]
+
+
+def construct_phi_wk(chi, U):
+    """eliashberg.cpp:336-360: Phi = U chi U on M[(a,b),(d,c)] = X_abcd."""
+    n = U.shape[0]
+    def grp(x):
+        return np.swapaxes(x, -1, -2).reshape(x.shape[:-4] + (n * n, n * n))
+    def ungrp(m):
+        return np.swapaxes(m.reshape(m.shape[:-2] + (n, n, n, n)), -1, -2)
+    Um = grp(U)
+    return ungrp(Um @ grp(chi) @ Um)

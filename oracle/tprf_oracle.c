@@ -474,6 +474,37 @@ int or_fourier_w_to_t(int statistic, double beta, int n_max, int ntau, long ncol
   return OR_OK;
 }
 
+/* Tail-fit moments t_k = c_k omega_max^k (k = 0..3) of every column, by the
+ * fit of or_fourier_w_to_t (the TRIQS fit_tail window and order, third-party,
+ * called at c++/triqs_tprf/lattice/lattice_utility.cpp:71 and by density()
+ * for rho_k_from_g_wk, gw.cpp:46).  t: [4][ncol].                          */
+int or_tail_moments(int statistic, double beta, int n_max, long ncol, const cplx *gw, cplx *t, double *fit_error) {
+  if (beta <= 0 || n_max < 1 || ncol < 1) return OR_ERR_ARG;
+  const int first = (statistic == OR_FERMION) ? -n_max : -(n_max - 1);
+  or_tail_window tw;
+  int st = or_tail_window_make(statistic, beta, n_max, &tw);
+  if (st) return st;
+  ls_qr *qr = (ls_qr *)malloc(sizeof(ls_qr));
+  if (!qr) return OR_ERR_NOMEM;
+  ls_qr_make(&tw, statistic, first, beta, qr);
+  double max_err = 0;
+  for (long c = 0; c < ncol; ++c) {
+    cplx gfit[128];
+    lcplx coef[10];
+    for (int r = 0; r < tw.n_fit; ++r) gfit[r] = gw[(long)tw.idx[r] * ncol + c];
+    double e = ls_qr_solve(qr, gfit, coef);
+    if (e > max_err) max_err = e;
+    long double wp = 1;
+    for (int k = 0; k < 4; ++k) {
+      t[(long)k * ncol + c] = (cplx)(coef[k] * wp);
+      wp *= (long double)tw.omega_max;
+    }
+  }
+  free(qr);
+  if (fit_error) *fit_error = max_err;
+  return OR_OK;
+}
+
 /* ------------------------------------------------------------------------ */
 /* exact inverse of V_{m,j} = m^{j-1} (m, j = 1..8): row j-1 of V^{-1} holds the
  * coefficients of t^{j-1} in the Lagrange basis polynomials l_m(t) =

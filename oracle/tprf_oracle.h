@@ -48,6 +48,9 @@ int or_fourier_lattice(long nouter, const int dims[3], long ntarget, const or_c1
 int or_fourier_w_to_t(int statistic, double beta, int n_max, int ntau, long ncol, const or_c128 *gw,
                       or_c128 *gt, const or_c128 *moments, or_c128 *moments_out, double *fit_error);
 /* a5: tau -> omega (fit_derivatives boundary moments), nw = n_max of the output mesh */
+/* tail-fit moments t_0..t_3 ([4][ncol]) of every column (split, density) */
+int or_tail_moments(int statistic, double beta, int n_max, long ncol, const or_c128 *gw, or_c128 *t,
+                    double *fit_error);
 int or_fourier_t_to_w(int statistic, double beta, int ntau, int nw, long ncol, const or_c128 *gt,
                       or_c128 *gw);
 /* a4 */

@@ -72,6 +72,17 @@ _SIGNATURES = {
     "tprf_solve_rpa_PH": [_p, _l, _i, _p, _p, _p],
     "tprf_solve_rpa_PH_herm": [_p, _i, _l, _i, _p, _p, _p],
     "tprf_dynamical_screened_interaction_W": [_p, _l, _l, _i, _p, _p, _i, _p],
+    "tprf_split_into_dynamic_wk_and_constant_k": [_p, _i, _d, _i, _l, _p, _p, _p],
+    "tprf_density": [_p, _d, _i, _l, _p, _p],
+    "tprf_gw_dynamic_sigma": [_p, _l, _i, _p, _p, _p],
+    "tprf_fock_sigma_r": [_p, _l, _i, _p, _p, _p],
+    "tprf_hartree_sigma_r": [_p, _l, _i, _p, _p, _p],
+    "tprf_static_sigma_k": [_p, _i, _p, _i, _p, _p, _p],
+    "tprf_add_dynamic_and_static": [_p, _l, _l, _p, _p],
+    "tprf_eliashberg_g_delta_g_product": [_p, _i, _i, _p, _i, _p, _p, _p],
+    "tprf_eliashberg_product": [_p, _d, _i, _i, _i, _p, _i, _p, _p, _p, _p],
+    "tprf_eliashberg_gamma_f_product": [_p, _l, _i, _p, _p, _p],
+    "tprf_construct_phi_wk": [_p, _l, _i, _p, _p, _p],
     "tprf_lindhard_chi00": [_p, _d, _d, _l, _p, _p, _i, _p, _p],
     "tprf_lindhard_chi00_noalloc": [_p, _d, _d, _l, _p, _p, _i, _p, _p],
     "tprf_lattice_dyson_g_wk_batched": [_p, _d, _i, _p, _l, _l, _i, _p, _p],

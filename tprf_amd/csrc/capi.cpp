@@ -1229,6 +1229,171 @@ int tprf_dynamical_screened_interaction_W(tprf_ctx *ctx, long nw, long nk, int n
                       "screened interaction kernel");
 }
 
+// ---- consumers of chi: GW and Eliashberg ----------------------------------
+
+namespace {
+int consumer_n(tprf_ctx *ctx, int norb, const char *what) {
+  if (norb < 1 || norb > 4) return fail(ctx, TPRF_ERR_UNSUPPORTED, std::string(what) + ": norb must be 1..4");
+  return TPRF_OK;
+}
+}  // namespace
+
+int tprf_split_into_dynamic_wk_and_constant_k(tprf_ctx *ctx, int statistic, double beta, int n_max, long ncol,
+                                              const tprf_c128 *chi_wk, tprf_c128 *chi_dyn_wk,
+                                              tprf_c128 *chi_const_k) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if ((statistic != TPRF_FERMION && statistic != TPRF_BOSON) || !(beta > 0) || n_max < 1 || ncol < 1 || !chi_wk ||
+      !chi_dyn_wk || !chi_const_k)
+    return fail(ctx, TPRF_ERR_ARG, "split_into_dynamic_wk_and_constant_k: bad argument");
+  const tprf::TailFitDev *fit;
+  int st;
+  if ((st = get_tail(ctx, statistic, beta, n_max, &fit))) return st;
+  const int nmesh = statistic == TPRF_FERMION ? 2 * n_max : 2 * n_max - 1;
+  return check_launch(ctx,
+                      tprf::launch_tail_split(ctx->stream, ncol, nmesh, *fit, (const c2 *)chi_wk, (c2 *)chi_dyn_wk,
+                                              (c2 *)chi_const_k),
+                      "tail split kernel");
+}
+
+int tprf_density(tprf_ctx *ctx, double beta, int n_max, long ncol, const tprf_c128 *g_w, tprf_c128 *rho) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (!(beta > 0) || n_max < 1 || ncol < 1 || !g_w || !rho) return fail(ctx, TPRF_ERR_ARG, "density: bad argument");
+  const tprf::TailFitDev *fit;
+  int st;
+  if ((st = get_tail(ctx, TPRF_FERMION, beta, n_max, &fit))) return st;
+  return check_launch(ctx,
+                      tprf::launch_density(ctx->stream, ncol, n_max, beta, *fit, (const c2 *)g_w, (c2 *)rho,
+                                           ctx->d_status),
+                      "density kernel");
+}
+
+int tprf_gw_dynamic_sigma(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *W_tr, const tprf_c128 *g_tr,
+                          tprf_c128 *sigma_tr) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (npts < 1 || !W_tr || !g_tr || !sigma_tr) return fail(ctx, TPRF_ERR_ARG, "gw_dynamic_sigma: bad argument");
+  int st = consumer_n(ctx, norb, "gw_dynamic_sigma");
+  if (st) return st;
+  const long n4 = (long)norb * norb * norb * norb;
+  return check_launch(ctx,
+                      tprf::launch_vertex_contract(ctx->stream, 0, norb, npts, (const c2 *)W_tr, n4, (const c2 *)g_tr,
+                                                   (c2 *)sigma_tr),
+                      "gw dynamic sigma kernel");
+}
+
+int tprf_fock_sigma_r(tprf_ctx *ctx, long nr, int norb, const tprf_c128 *v_r, const tprf_c128 *rho_r,
+                      tprf_c128 *sigma_r) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (nr < 1 || !v_r || !rho_r || !sigma_r) return fail(ctx, TPRF_ERR_ARG, "fock_sigma: bad argument");
+  int st = consumer_n(ctx, norb, "fock_sigma");
+  if (st) return st;
+  const long n4 = (long)norb * norb * norb * norb;
+  return check_launch(ctx,
+                      tprf::launch_vertex_contract(ctx->stream, 1, norb, nr, (const c2 *)v_r, n4, (const c2 *)rho_r,
+                                                   (c2 *)sigma_r),
+                      "fock sigma kernel");
+}
+
+int tprf_hartree_sigma_r(tprf_ctx *ctx, long nr, int norb, const tprf_c128 *v_k, const tprf_c128 *rho_r,
+                         tprf_c128 *sigma_r) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (nr < 1 || !v_k || !rho_r || !sigma_r) return fail(ctx, TPRF_ERR_ARG, "hartree_sigma: bad argument");
+  int st = consumer_n(ctx, norb, "hartree_sigma");
+  if (st) return st;
+  if ((st = check_launch(ctx, tprf::launch_zero(ctx->stream, nr * norb * norb, (c2 *)sigma_r), "zero kernel")))
+    return st;
+  // only r = 0 carries the Hartree term: V(q = 0) rho(r = 0)
+  return check_launch(ctx,
+                      tprf::launch_vertex_contract(ctx->stream, 2, norb, 1, (const c2 *)v_k, 0, (const c2 *)rho_r,
+                                                   (c2 *)sigma_r),
+                      "hartree sigma kernel");
+}
+
+int tprf_static_sigma_k(tprf_ctx *ctx, int fock, const int dims[3], int norb, const tprf_c128 *v_k,
+                        const tprf_c128 *rho_k, tprf_c128 *sigma_k) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (!dims_ok(dims) || (fock != 0 && fock != 1) || !v_k || !rho_k || !sigma_k)
+    return fail(ctx, TPRF_ERR_ARG, "hartree/fock sigma (k): bad argument");
+  int st = consumer_n(ctx, norb, "hartree/fock sigma (k)");
+  if (st) return st;
+  return check_launch(ctx,
+                      tprf::launch_static_sigma_k(ctx->stream, fock != 0, norb, dims, (const c2 *)v_k,
+                                                  (const c2 *)rho_k, (c2 *)sigma_k),
+                      "static sigma kernel");
+}
+
+int tprf_eliashberg_gamma_f_product(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *gamma, const tprf_c128 *F,
+                                    tprf_c128 *delta) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (npts < 1 || !gamma || !F || !delta) return fail(ctx, TPRF_ERR_ARG, "eliashberg gamma-F product: bad argument");
+  int st = consumer_n(ctx, norb, "eliashberg gamma-F product");
+  if (st) return st;
+  const long n4 = (long)norb * norb * norb * norb;
+  return check_launch(ctx,
+                      tprf::launch_vertex_contract(ctx->stream, 3, norb, npts, (const c2 *)gamma, n4, (const c2 *)F,
+                                                   (c2 *)delta),
+                      "eliashberg gamma-F kernel");
+}
+
+int tprf_eliashberg_g_delta_g_product(tprf_ctx *ctx, int n_max_g, int n_max_d, const int dims[3], int norb,
+                                      const tprf_c128 *g_wk, const tprf_c128 *delta_wk, tprf_c128 *F_wk) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (n_max_g < 1 || n_max_d < 1 || !dims_ok(dims) || !g_wk || !delta_wk || !F_wk)
+    return fail(ctx, TPRF_ERR_ARG, "eliashberg_g_delta_g_product: bad argument");
+  if (n_max_d > n_max_g)
+    return fail(ctx, TPRF_ERR_ARG,
+                "The size of the Matsubara frequency mesh of the Green's function (" + std::to_string(2 * n_max_g) +
+                    ") must be atleast the size of the mesh of Delta (" + std::to_string(2 * n_max_d) + ").");
+  int st = consumer_n(ctx, norb, "eliashberg_g_delta_g_product");
+  if (st) return st;
+  return check_launch(ctx,
+                      tprf::launch_g_delta_g(ctx->stream, norb, 2L * n_max_d, n_max_g - n_max_d, dims,
+                                             (const c2 *)g_wk, (const c2 *)delta_wk, (c2 *)F_wk),
+                      "eliashberg G delta G kernel");
+}
+
+int tprf_eliashberg_product(tprf_ctx *ctx, double beta, int n_max_gamma, int n_max_g, int n_max_d, const int dims[3],
+                            int norb, const tprf_c128 *gamma_wk, const tprf_c128 *g_wk, const tprf_c128 *delta_wk,
+                            tprf_c128 *delta_out) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (!(beta > 0) || n_max_gamma < 1 || n_max_g < 1 || n_max_d < 1 || !dims_ok(dims) || !gamma_wk || !g_wk ||
+      !delta_wk || !delta_out)
+    return fail(ctx, TPRF_ERR_ARG, "eliashberg_product: bad argument");
+  if (2 * (2 * n_max_d) > 2 * n_max_gamma - 1)
+    return fail(ctx, TPRF_ERR_ARG,
+                "The size of the Matsubara frequency mesh of Gamma (" + std::to_string(2 * n_max_gamma - 1) +
+                    ") must be atleast TWICE the size of the mesh of Delta (" + std::to_string(2 * n_max_d) + ").");
+  int st = consumer_n(ctx, norb, "eliashberg_product");
+  if (st) return st;
+  const long nk = (long)dims[0] * dims[1] * dims[2];
+  void *w;
+  if ((st = get_work(ctx, sizeof(c2) * (size_t)(2L * n_max_d * nk * norb * norb), &w))) return st;
+  if ((st = tprf_eliashberg_g_delta_g_product(ctx, n_max_g, n_max_d, dims, norb, g_wk, delta_wk, (tprf_c128 *)w)))
+    return st;
+  return check_launch(ctx,
+                      tprf::launch_eliashberg_sum(ctx->stream, norb, n_max_d, n_max_gamma, dims,
+                                                  -1.0 / (2.0 * beta * (double)nk), (const c2 *)gamma_wk,
+                                                  (const c2 *)w, (c2 *)delta_out),
+                      "eliashberg summation kernel");
+}
+
+int tprf_construct_phi_wk(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *chi, const tprf_c128 *U,
+                          tprf_c128 *phi) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (npts < 1 || !chi || !U || !phi) return fail(ctx, TPRF_ERR_ARG, "construct_phi_wk: bad argument");
+  int st = consumer_n(ctx, norb, "construct_phi_wk");
+  if (st) return st;
+  return check_launch(ctx,
+                      tprf::launch_construct_phi(ctx->stream, norb, npts, (const c2 *)chi, (const c2 *)U, (c2 *)phi),
+                      "construct phi kernel");
+}
+
+int tprf_add_dynamic_and_static(tprf_ctx *ctx, long nw, long ncol, tprf_c128 *x_w, const tprf_c128 *s) {
+  if (!ctx) return TPRF_ERR_ARG;
+  if (nw < 1 || ncol < 1 || !x_w || !s) return fail(ctx, TPRF_ERR_ARG, "add_dynamic_and_static: bad argument");
+  return check_launch(ctx, tprf::launch_add_static(ctx->stream, nw, ncol, (c2 *)x_w, (const c2 *)s),
+                      "add static kernel");
+}
+
 // ---- differentiation -----------------------------------------------------
 
 int tprf_lattice_dyson_g_wk_batched(tprf_ctx *ctx, double beta, int n_iw, const double *mu, long nbatch, long nk,

@@ -234,6 +234,23 @@ hipError_t launch_static_q(hipStream_t s, long nbatch, const int dims[3], const 
                            const double *mu, const c2 *ek, const double *tab, const int *fitpos, const TailFitDev &fit,
                            const StaticMatsDev &sm, c2 *work, c2 *proj, bool reuse, const c2 *chibar, c2 *chi0_q,
                            c2 *ebar, double *mubar, double *status);
+// consumers.hip: GW and Eliashberg pieces (mode of launch_vertex_contract:
+// 0 gw_dynamic_sigma, 1 fock (r), 2 hartree (r), 3 eliashberg gamma-F)
+hipError_t launch_vertex_contract(hipStream_t s, int mode, int n, long npts, const c2 *V, long vstride, const c2 *y,
+                                  c2 *out);
+hipError_t launch_static_sigma_k(hipStream_t s, bool fock, int n, const int dims[3], const c2 *V, const c2 *rho,
+                                 c2 *out);
+hipError_t launch_tail_split(hipStream_t s, long ncol, int nmesh, const TailFitDev &fit, const c2 *x, c2 *dyn,
+                             c2 *cst);
+hipError_t launch_density(hipStream_t s, long ncol, int n_max, double beta, const TailFitDev &fit, const c2 *x,
+                          c2 *rho, double *status);
+hipError_t launch_g_delta_g(hipStream_t s, int n, long nwd, int off, const int dims[3], const c2 *g, const c2 *delta,
+                            c2 *F);
+hipError_t launch_eliashberg_sum(hipStream_t s, int n, int nd, int ng, const int dims[3], double scale,
+                                 const c2 *gamma, const c2 *F, c2 *out);
+hipError_t launch_construct_phi(hipStream_t s, int n, long npts, const c2 *chi, const c2 *U, c2 *phi);
+hipError_t launch_add_static(hipStream_t s, long nw, long ncol, c2 *x, const c2 *st);
+hipError_t launch_zero(hipStream_t s, long n, c2 *x);
 // adjoint.hip: out[b] = sum of part[b n .. b n + n), b < nbatch
 hipError_t launch_sum_partials(hipStream_t s, const double *part, long nbatch, int n, double *out);
 hipError_t launch_lattice3d(hipStream_t s, int sign, const c2 *in, c2 *out, long nw, const int dims[3], long ntarget,

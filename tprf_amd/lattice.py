@@ -20,7 +20,11 @@ __all__ = [
     "lattice_dyson_g0_wk", "lattice_dyson_g_wk", "fourier_wk_to_wr", "fourier_wr_to_wk", "fourier_wr_to_tr",
     "fourier_tr_to_wr", "chi0_tr_from_grt_PH", "chi0_wr_from_grt_PH", "chi0_w0r_from_grt_PH",
     "chi_w0r_from_chi_tr", "chi_wr_from_chi_tr", "chi_tr_from_chi_wr", "chi_wk_from_chi_wr",
-    "chi_wr_from_chi_wk", "solve_rpa_PH", "lindhard_chi00",
+    "chi_wr_from_chi_wk", "solve_rpa_PH", "lindhard_chi00", "dynamical_screened_interaction_W",
+    "split_into_dynamic_wk_and_constant_k", "add_dynamic_and_static", "rho_k_from_g_wk", "gw_dynamic_sigma",
+    "hartree_sigma", "fock_sigma", "gw_sigma", "eliashberg_g_delta_g_product", "eliashberg_product",
+    "eliashberg_product_fft", "eliashberg_product_fft_constant", "dynamic_and_constant_to_tr",
+    "eliashberg_dynamic_gamma_f_product", "eliashberg_constant_gamma_f_product", "construct_phi_wk",
 ]
 
 
@@ -324,3 +328,252 @@ def lindhard_chi00(e_k, mesh, *args, mu=None, beta=None, delta=0.0):
     _lib.call("tprf_lindhard_chi00", c, float(beta), float(mu), len(z), _lib.ptr(zd), _lib.dims3(kmesh.dims), n,
               _lib.ptr(e), _lib.ptr(out))
     return Gf(MeshProduct(mesh, MeshBrZone(kmesh.dims)), data=out)
+
+
+# ---------------------------------------------------------------------------
+# f4 consumers of chi: GW -- gw.hpp:36-281 / gw.cpp, lattice_utility.cpp:57-101
+
+def _kmesh_of(g):
+    return g.mesh[1] if isinstance(g.mesh, MeshProduct) else g.mesh
+
+
+def split_into_dynamic_wk_and_constant_k(chi_wk):
+    """(chi_dyn_wk, chi_const_k): the zeroth tail-fit coefficient of every
+    (k, target) column is the constant part (lattice_utility.cpp:57-84)."""
+    wmesh, kmesh = chi_wk.mesh[0], chi_wk.mesh[1]
+    x = _dev(chi_wk.data)
+    c = _lib.ctx(x.device.index)
+    ncol = x[0].numel()
+    dyn = torch.empty_like(x)
+    cst = torch.empty(tuple(x.shape[1:]), dtype=torch.complex128, device=x.device)
+    _lib.call("tprf_split_into_dynamic_wk_and_constant_k", c, _stat_code(wmesh), wmesh.beta, wmesh.n_max, ncol,
+              _lib.ptr(x), _lib.ptr(dyn), _lib.ptr(cst))
+    return Gf(chi_wk.mesh, data=dyn), Gf(kmesh, data=cst)
+
+
+def add_dynamic_and_static(g_dyn_wk, g_stat_k):
+    """g_dyn_wk + g_stat_k at every frequency (lattice_utility.cpp:86-101)."""
+    x = _dev(g_dyn_wk.data).clone()
+    st = _dev(g_stat_k.data)
+    c = _lib.ctx(x.device.index)
+    _lib.call("tprf_add_dynamic_and_static", c, x.shape[0], x[0].numel(), _lib.ptr(x), _lib.ptr(st))
+    return Gf(g_dyn_wk.mesh, data=x)
+
+
+def rho_k_from_g_wk(g_wk):
+    """Density matrix rho_ab(k) of G(i w, k) per k (gw.cpp:36-52): the tail-fit
+    regularised Matsubara sum of every (k, a, b) column."""
+    wmesh, kmesh = g_wk.mesh[0], g_wk.mesh[1]
+    x = _dev(g_wk.data)
+    c = _lib.ctx(x.device.index)
+    out = torch.empty(tuple(x.shape[1:]), dtype=torch.complex128, device=x.device)
+    _lib.tail_status_reset(c)
+    _lib.call("tprf_density", c, wmesh.beta, wmesh.n_max, x[0].numel(), _lib.ptr(x), _lib.ptr(out))
+    _lib.tail_status(c)
+    return Gf(kmesh, data=out)
+
+
+def gw_dynamic_sigma(W_tr, g_tr):
+    """sigma_ab(tau, r) = -sum_cd W_acdb(tau, r) G_cd(tau, r) (gw.cpp:66-102)."""
+    Wtm, gtm = W_tr.mesh[0], g_tr.mesh[0]
+    if Wtm.size != gtm.size or Wtm.beta != gtm.beta:
+        raise _lib.TprfError(-1, "gw_sigma_tr: tau meshes are not the same.")
+    if Wtm.statistic != BOSON or gtm.statistic != FERMION:
+        raise _lib.TprfError(-1, "gw_sigma_tr: statistics are incorrect.")
+    if W_tr.mesh[1] != g_tr.mesh[1]:
+        raise _lib.TprfError(-1, "gw_sigma_tr: real-space meshes are not the same.")
+    w = _dev(W_tr.data)
+    g = _dev(g_tr.data)
+    n = g_tr.target_shape[0]
+    c = _lib.ctx(g.device.index)
+    out = torch.empty_like(g)
+    _lib.call("tprf_gw_dynamic_sigma", c, g.numel() // (n * n), n, _lib.ptr(w), _lib.ptr(g), _lib.ptr(out))
+    return Gf(g_tr.mesh, data=out)
+
+
+def _fourier_k_r(x, mesh, to_r):
+    """One-frequency lattice transform of a k (or r) function (make_gf_from_fourier)."""
+    d = _dev(x)
+    c = _lib.ctx(d.device.index)
+    out = torch.empty_like(d)
+    name = "tprf_fourier_wk_to_wr" if to_r else "tprf_fourier_wr_to_wk"
+    _lib.call(name, c, 1, _lib.dims3(mesh.dims), d.numel() // mesh.size, _lib.ptr(d), _lib.ptr(out))
+    return out
+
+
+def _static_sigma(v_k, g_wk, fock):
+    kmesh = g_wk.mesh[1]
+    if v_k.mesh != kmesh:
+        what = "fock_sigma" if fock else "hartree_sigma"
+        raise _lib.TprfError(-1, f"{what}: k-space meshes are not the same.")
+    rho = rho_k_from_g_wk(g_wk).data
+    v = _dev(v_k.data)
+    n = g_wk.target_shape[0]
+    c = _lib.ctx(v.device.index)
+    out = torch.empty_like(rho)
+    _lib.call("tprf_static_sigma_k", c, 1 if fock else 0, _lib.dims3(kmesh.dims), n, _lib.ptr(v), _lib.ptr(rho),
+              _lib.ptr(out))
+    return Gf(kmesh, data=out)
+
+
+def hartree_sigma(v_k, g):
+    """hartree_sigma(v_k, g_wk): sigma_ab(k) = (1/N) sum_q V_abcd(q) rho_cd(k + q)
+    (gw.cpp:124-147); hartree_sigma(v_k, rho_r): sigma_ab(r = 0) = V_abcd(k = 0)
+    rho_cd(r = 0) (gw.cpp:113-122)."""
+    if isinstance(g.mesh, MeshProduct):
+        return _static_sigma(v_k, g, fock=False)
+    v = _dev(v_k.data)
+    rho = _dev(g.data)
+    n = g.target_shape[0]
+    c = _lib.ctx(v.device.index)
+    out = torch.empty_like(rho)
+    _lib.call("tprf_hartree_sigma_r", c, g.mesh.size, n, _lib.ptr(v), _lib.ptr(rho), _lib.ptr(out))
+    return Gf(g.mesh, data=out)
+
+
+def fock_sigma(v, g):
+    """fock_sigma(v_k, g_wk): sigma_ab(k) = -(1/N) sum_q V_acdb(q) rho_dc(k + q)
+    (gw.cpp:169-195); fock_sigma(v_r, rho_r): sigma_ab(r) = -V_acdb(r) rho_dc(r)
+    (gw.cpp:149-167)."""
+    if isinstance(g.mesh, MeshProduct):
+        return _static_sigma(v, g, fock=True)
+    if v.mesh != g.mesh:
+        raise _lib.TprfError(-1, "fock_sigma: r-space meshes are not the same.")
+    vd = _dev(v.data)
+    rho = _dev(g.data)
+    n = g.target_shape[0]
+    c = _lib.ctx(vd.device.index)
+    out = torch.empty_like(rho)
+    _lib.call("tprf_fock_sigma_r", c, g.mesh.size, n, _lib.ptr(vd), _lib.ptr(rho), _lib.ptr(out))
+    return Gf(g.mesh, data=out)
+
+
+def gw_sigma(W, g_wk):
+    """GW self-energy (gw.cpp:204-244).  gw_sigma(W_wk, g_wk): W is split into a
+    dynamic part and its constant tail; the dynamic part gives
+    sigma(tau, r) = -W_dyn(tau, r) G(tau, r), transformed back to (i w, k); the
+    constant part gives the Fock term -V(r) rho(r) of the density rho(k) of
+    g_wk, transformed to k and added at every frequency.  gw_sigma(v_k, g_wk)
+    with a static v_k is the Fock self-energy (gw.cpp:198-201)."""
+    if not isinstance(W.mesh, MeshProduct):
+        return fock_sigma(W, g_wk)
+    Wwm, gwm = W.mesh[0], g_wk.mesh[0]
+    if Wwm.beta != gwm.beta:
+        raise _lib.TprfError(-1, "gw_sigma: inverse temperatures are not the same.")
+    if Wwm.statistic != BOSON or gwm.statistic != FERMION:
+        raise _lib.TprfError(-1, "gw_sigma: statistics are incorrect.")
+    if W.mesh[1] != g_wk.mesh[1]:
+        raise _lib.TprfError(-1, "gw_sigma: k-space meshes are not the same.")
+    kmesh = g_wk.mesh[1]
+    W_dyn_wk, W_const_k = split_into_dynamic_wk_and_constant_k(W)
+    g_tr = fourier_wr_to_tr(fourier_wk_to_wr(g_wk))
+    W_dyn_tr = chi_tr_from_chi_wr(chi_wr_from_chi_wk(W_dyn_wk))
+    sigma_dyn_tr = gw_dynamic_sigma(W_dyn_tr, g_tr)
+    rmesh = MeshCycLat(kmesh.dims)
+    W_const_r = Gf(rmesh, data=_fourier_k_r(W_const_k.data, kmesh, True))
+    rho_r = Gf(rmesh, data=_fourier_k_r(rho_k_from_g_wk(g_wk).data, kmesh, True))
+    sigma_fock_r = fock_sigma(W_const_r, rho_r)
+    sigma_fock_k = Gf(kmesh, data=_fourier_k_r(sigma_fock_r.data, kmesh, False))
+    sigma_wk = fourier_wr_to_wk(fourier_tr_to_wr(sigma_dyn_tr, nw=gwm.n_max))
+    return add_dynamic_and_static(sigma_wk, sigma_fock_k)
+
+
+# ---------------------------------------------------------------------------
+# f4 consumers of chi: linearised Eliashberg -- eliashberg.hpp:52-199 / eliashberg.cpp
+
+def eliashberg_g_delta_g_product(g_wk, delta_wk):
+    """F_dc(w, k) = sum_ef G_cf(w, k) conj(G_ed(w, -k)) Delta_ef(w, k) on Delta's
+    mesh (eliashberg.cpp:31-58)."""
+    g = _dev(g_wk.data)
+    dl = _dev(delta_wk.data)
+    n = g_wk.target_shape[0]
+    kmesh = delta_wk.mesh[1]
+    c = _lib.ctx(g.device.index)
+    out = torch.empty_like(dl)
+    _lib.call("tprf_eliashberg_g_delta_g_product", c, g_wk.mesh[0].n_max, delta_wk.mesh[0].n_max,
+              _lib.dims3(kmesh.dims), n, _lib.ptr(g), _lib.ptr(dl), _lib.ptr(out))
+    return Gf(delta_wk.mesh, data=out)
+
+
+def eliashberg_product(Gamma_pp, g_wk, delta_wk):
+    """Linearised Eliashberg product by summation (eliashberg.cpp:92-120):
+    Delta_ab(w, k) = -1/(2 N beta) sum_{n,q} Gamma_cadb(w - n, k - q) F_dc(n, q)."""
+    gm = _dev(Gamma_pp.data)
+    g = _dev(g_wk.data)
+    dl = _dev(delta_wk.data)
+    n = g_wk.target_shape[0]
+    wmesh, kmesh = delta_wk.mesh[0], delta_wk.mesh[1]
+    c = _lib.ctx(g.device.index)
+    out = torch.empty_like(dl)
+    _lib.call("tprf_eliashberg_product", c, wmesh.beta, Gamma_pp.mesh[0].n_max, g_wk.mesh[0].n_max, wmesh.n_max,
+              _lib.dims3(kmesh.dims), n, _lib.ptr(gm), _lib.ptr(g), _lib.ptr(dl), _lib.ptr(out))
+    return Gf(delta_wk.mesh, data=out)
+
+
+def dynamic_and_constant_to_tr(Gamma_pp_dyn_wk, Gamma_pp_const_k):
+    """(Gamma_dyn(tau, r), Gamma_const(r)) from (Gamma_dyn(i nu, k), Gamma_const(k))
+    (eliashberg.cpp:122-134)."""
+    kmesh = Gamma_pp_const_k.mesh
+    dyn_tr = chi_tr_from_chi_wr(chi_wr_from_chi_wk(Gamma_pp_dyn_wk))
+    const_r = Gf(MeshCycLat(kmesh.dims), data=_fourier_k_r(Gamma_pp_const_k.data, kmesh, True))
+    return dyn_tr, const_r
+
+
+def _gamma_f(gamma, F, mesh):
+    gd = _dev(gamma)
+    f = _dev(F)
+    n = f.shape[-1]
+    npts = f.numel() // (n * n)
+    c = _lib.ctx(f.device.index)
+    out = torch.empty_like(f)
+    _lib.call("tprf_eliashberg_gamma_f_product", c, npts, n, _lib.ptr(gd), _lib.ptr(f), _lib.ptr(out))
+    return Gf(mesh, data=out)
+
+
+def eliashberg_dynamic_gamma_f_product(Gamma_pp_dyn_tr, F_tr):
+    """Delta_ab(tau, r) = -1/2 sum_cd Gamma_cadb(tau, r) F_dc(tau, r) (eliashberg.cpp:214-240)."""
+    if Gamma_pp_dyn_tr.mesh[0].size != F_tr.mesh[0].size:
+        raise _lib.TprfError(-1, f"The size of the imaginary time mesh of Gamma ({Gamma_pp_dyn_tr.mesh[0].size}) "
+                                 f"must be the size of the mesh of Delta ({F_tr.mesh[0].size}).")
+    return _gamma_f(Gamma_pp_dyn_tr.data, F_tr.data, F_tr.mesh)
+
+
+def eliashberg_constant_gamma_f_product(Gamma_pp_const_r, F_tr):
+    """Delta_ab(r) = -1/2 sum_cd Gamma_cadb(r) F_dc(tau = 0, r) (eliashberg.cpp:175-186)."""
+    return _gamma_f(Gamma_pp_const_r.data, _dev(F_tr.data)[0], F_tr.mesh[1])
+
+
+def eliashberg_product_fft(Gamma_pp_dyn_tr, Gamma_pp_const_r, g_wk, delta_wk):
+    """Linearised Eliashberg product through the convolution theorem
+    (eliashberg.cpp:242-268): F = G G Delta to (tau, r); dynamic and constant
+    Gamma products; back to (i w, k)."""
+    F_wk = eliashberg_g_delta_g_product(g_wk, delta_wk)
+    F_tr = fourier_wr_to_tr(fourier_wk_to_wr(F_wk))
+    delta_tr = eliashberg_dynamic_gamma_f_product(Gamma_pp_dyn_tr, F_tr)
+    delta_r = eliashberg_constant_gamma_f_product(Gamma_pp_const_r, F_tr)
+    delta_wr = fourier_tr_to_wr(delta_tr)
+    delta_wr = add_dynamic_and_static(delta_wr, delta_r)
+    return fourier_wr_to_wk(delta_wr)
+
+
+def eliashberg_product_fft_constant(Gamma_pp_const_r, g_wk, delta_wk):
+    """The product for a Gamma constant in frequency (eliashberg.cpp:279-302)."""
+    F_wk = eliashberg_g_delta_g_product(g_wk, delta_wk)
+    F_tr = fourier_wr_to_tr(fourier_wk_to_wr(F_wk))
+    delta_r = eliashberg_constant_gamma_f_product(Gamma_pp_const_r, F_tr)
+    kmesh = delta_wk.mesh[1]
+    delta_k = Gf(kmesh, data=_fourier_k_r(delta_r.data, kmesh, False))
+    zero = Gf(F_wk.mesh, data=torch.zeros_like(_dev(F_wk.data)))
+    return add_dynamic_and_static(zero, delta_k)
+
+
+def construct_phi_wk(chi, U):
+    """Phi = U chi U per (w, k) on the matrices M[(a,b),(d,c)] = X_abcd
+    (eliashberg.cpp:336-360)."""
+    x = _dev(chi.data)
+    n = chi.target_shape[0]
+    u = _dev(U.data if isinstance(U, Gf) else U).reshape(n, n, n, n).contiguous()
+    c = _lib.ctx(x.device.index)
+    out = torch.empty_like(x)
+    _lib.call("tprf_construct_phi_wk", c, x.numel() // n ** 4, n, _lib.ptr(x), _lib.ptr(u), _lib.ptr(out))
+    return Gf(chi.mesh, data=out)
