@@ -33,6 +33,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "tprf_hip.h"
@@ -422,6 +423,225 @@ inline chi_wk_t imtime_bubble_chi0_wk(g_wk_cvt g_wk, int nw = 1, bool save_memor
   if (nw == 1) return chi_wk_from_chi_wr(chi0_w0r_from_grt_PH(g_tr));
   if (!save_memory) return chi_wk_from_chi_wr(chi_wr_from_chi_tr(chi0_tr_from_grt_PH(g_tr), nw));
   return chi_wk_from_chi_wr(chi0_wr_from_grt_PH(g_tr, nw));
+}
+
+// ---- f4: consumers of chi -- GW (gw.hpp) and Eliashberg (eliashberg.hpp) ----
+using e_r_t = gf<mesh::cyclat, mesh::none, 2>;
+using chi_k_t = gf<mesh::brzone, mesh::none, 4>;
+using chi_r_t = gf<mesh::cyclat, mesh::none, 4>;
+using e_r_cvt = const e_r_t &;
+using chi_k_cvt = const chi_k_t &;
+using chi_r_cvt = const chi_r_t &;
+using chi_k_vt = const chi_k_t &;
+using chi_r_vt = const chi_r_t &;
+using chi_tr_vt = const chi_tr_t &;
+using g_wk_vt = const g_wk_t &;
+using g_tr_vt = const g_tr_t &;
+
+namespace detail {
+// one-frequency lattice transform of a k (to_r) or r function (make_gf_from_fourier)
+template <class MI, class MO, int R>
+gf<MO, mesh::none, R> lattice1(const gf<MI, mesh::none, R> &x, MO mo, bool to_r) {
+  tprf_ctx *cx = context();
+  gf<MO, mesh::none, R> out(mo, mesh::none{}, x.norb());
+  check(to_r ? tprf_fourier_wk_to_wr(cx, 1, x.mesh0().dims.data(), x.target_size(), c(x.data()), c(out.data()))
+             : tprf_fourier_wr_to_wk(cx, 1, x.mesh0().dims.data(), x.target_size(), c(x.data()), c(out.data())),
+        cx, "lattice fourier");
+  return out;
+}
+template <class M0, class M1, int R>
+gf<M0, M1, R> plus_static(const gf<M0, M1, R> &dyn, const gf<M1, mesh::none, R> &st) {
+  tprf_ctx *cx = context();
+  gf<M0, M1, R> out(dyn.mesh0(), dyn.mesh1(), dyn.norb());
+  hip_check(hipMemcpyAsync(out.data(), dyn.data(), sizeof(dcomplex) * (size_t)dyn.size(), hipMemcpyDeviceToDevice,
+                           state().stream),
+            "hipMemcpyAsync");
+  check(tprf_add_dynamic_and_static(cx, dyn.mesh0().size(), dyn.mesh1().size() * dyn.target_size(), c(out.data()),
+                                    c(st.data())),
+        cx, "add_dynamic_and_static");
+  return out;
+}
+}  // namespace detail
+
+/* split_into_dynamic_wk_and_constant_k (lattice_utility.hpp; lattice_utility.cpp:57-84) */
+inline std::tuple<chi_wk_t, chi_k_t> split_into_dynamic_wk_and_constant_k(chi_wk_cvt chi_wk) {
+  tprf_ctx *cx = detail::context();
+  const mesh::imfreq &wm = chi_wk.mesh0();
+  chi_wk_t dyn(wm, chi_wk.mesh1(), chi_wk.norb());
+  chi_k_t cst(chi_wk.mesh1(), mesh::none{}, chi_wk.norb());
+  detail::check(tprf_split_into_dynamic_wk_and_constant_k(cx, wm.statistic, wm.beta, wm.n_max,
+                                                          chi_wk.mesh1().size() * chi_wk.target_size(),
+                                                          detail::c(chi_wk.data()), detail::c(dyn.data()),
+                                                          detail::c(cst.data())),
+                cx, "split_into_dynamic_wk_and_constant_k");
+  return {dyn, cst};
+}
+
+/* rho_k_from_g_wk (gw.hpp:36; gw.cpp:36-52) */
+inline e_k_t rho_k_from_g_wk(g_wk_cvt g_wk) {
+  tprf_ctx *cx = detail::context();
+  const mesh::imfreq &wm = g_wk.mesh0();
+  e_k_t rho(g_wk.mesh1(), mesh::none{}, g_wk.norb());
+  detail::check(tprf_ctx_tail_status_reset(cx), cx, "status reset");
+  detail::check(tprf_density(cx, wm.beta, wm.n_max, g_wk.mesh1().size() * g_wk.target_size(), detail::c(g_wk.data()),
+                             detail::c(rho.data())),
+                cx, "rho_k_from_g_wk");
+  detail::check_tail(cx, false);
+  return rho;
+}
+
+/* gw_dynamic_sigma (gw.hpp:158; gw.cpp:66-102) */
+inline g_tr_t gw_dynamic_sigma(chi_tr_cvt W_tr, g_tr_cvt g_tr) {
+  if (W_tr.mesh0().n_tau != g_tr.mesh0().n_tau || W_tr.mesh0().beta != g_tr.mesh0().beta)
+    throw error(TPRF_ERR_ARG, "gw_sigma_tr: tau meshes are not the same.");
+  if (W_tr.mesh0().statistic != mesh::Boson || g_tr.mesh0().statistic != mesh::Fermion)
+    throw error(TPRF_ERR_ARG, "gw_sigma_tr: statistics are incorrect.");
+  if (W_tr.mesh1().dims != g_tr.mesh1().dims) throw error(TPRF_ERR_ARG, "gw_sigma_tr: real-space meshes are not the same.");
+  tprf_ctx *cx = detail::context();
+  g_tr_t out(g_tr.mesh0(), g_tr.mesh1(), g_tr.norb());
+  detail::check(tprf_gw_dynamic_sigma(cx, g_tr.mesh0().size() * g_tr.mesh1().size(), g_tr.norb(),
+                                      detail::c(W_tr.data()), detail::c(g_tr.data()), detail::c(out.data())),
+                cx, "gw_dynamic_sigma");
+  return out;
+}
+
+/* hartree_sigma / fock_sigma on the k mesh (gw.hpp:121,148; gw.cpp:124-147, 169-195) */
+inline e_k_t hartree_sigma(chi_k_cvt v_k, g_wk_cvt g_wk) {
+  tprf_ctx *cx = detail::context();
+  e_k_t rho = rho_k_from_g_wk(g_wk), out(g_wk.mesh1(), mesh::none{}, g_wk.norb());
+  detail::check(tprf_static_sigma_k(cx, 0, g_wk.mesh1().dims.data(), g_wk.norb(), detail::c(v_k.data()),
+                                    detail::c(rho.data()), detail::c(out.data())),
+                cx, "hartree_sigma");
+  return out;
+}
+inline e_k_t fock_sigma(chi_k_cvt v_k, g_wk_cvt g_wk) {
+  tprf_ctx *cx = detail::context();
+  e_k_t rho = rho_k_from_g_wk(g_wk), out(g_wk.mesh1(), mesh::none{}, g_wk.norb());
+  detail::check(tprf_static_sigma_k(cx, 1, g_wk.mesh1().dims.data(), g_wk.norb(), detail::c(v_k.data()),
+                                    detail::c(rho.data()), detail::c(out.data())),
+                cx, "fock_sigma");
+  return out;
+}
+/* hartree_sigma / fock_sigma in r space (gw.hpp:122,149; gw.cpp:113-122, 149-167) */
+inline e_r_t hartree_sigma(chi_k_cvt v_k, e_r_cvt rho_r) {
+  tprf_ctx *cx = detail::context();
+  e_r_t out(rho_r.mesh0(), mesh::none{}, rho_r.norb());
+  detail::check(tprf_hartree_sigma_r(cx, rho_r.mesh0().size(), rho_r.norb(), detail::c(v_k.data()),
+                                     detail::c(rho_r.data()), detail::c(out.data())),
+                cx, "hartree_sigma");
+  return out;
+}
+inline e_r_t fock_sigma(chi_r_cvt v_r, e_r_cvt rho_r) {
+  tprf_ctx *cx = detail::context();
+  e_r_t out(rho_r.mesh0(), mesh::none{}, rho_r.norb());
+  detail::check(tprf_fock_sigma_r(cx, rho_r.mesh0().size(), rho_r.norb(), detail::c(v_r.data()),
+                                  detail::c(rho_r.data()), detail::c(out.data())),
+                cx, "fock_sigma");
+  return out;
+}
+
+/* gw_sigma (gw.hpp:92,160; gw.cpp:198-244): static (the Fock term) or dynamic */
+inline e_k_t gw_sigma(chi_k_cvt v_k, g_wk_cvt g_wk) { return fock_sigma(v_k, g_wk); }
+inline g_wk_t gw_sigma(chi_wk_cvt W_wk, g_wk_cvt g_wk) {
+  if (W_wk.mesh0().beta != g_wk.mesh0().beta) throw error(TPRF_ERR_ARG, "gw_sigma: inverse temperatures are not the same.");
+  if (W_wk.mesh0().statistic != mesh::Boson || g_wk.mesh0().statistic != mesh::Fermion)
+    throw error(TPRF_ERR_ARG, "gw_sigma: statistics are incorrect.");
+  if (W_wk.mesh1().dims != g_wk.mesh1().dims) throw error(TPRF_ERR_ARG, "gw_sigma: k-space meshes are not the same.");
+  auto [W_dyn_wk, W_const_k] = split_into_dynamic_wk_and_constant_k(W_wk);
+  g_tr_t g_tr = fourier_wr_to_tr(fourier_wk_to_wr(g_wk));
+  chi_tr_t W_dyn_tr = chi_tr_from_chi_wr(chi_wr_from_chi_wk(W_dyn_wk));
+  g_tr_t sigma_dyn_tr = gw_dynamic_sigma(W_dyn_tr, g_tr);
+  const mesh::cyclat rm = detail::to_r(g_wk.mesh1());
+  chi_r_t W_const_r = detail::lattice1(W_const_k, rm, true);
+  e_r_t rho_r = detail::lattice1(rho_k_from_g_wk(g_wk), rm, true);
+  e_k_t sigma_fock_k = detail::lattice1(fock_sigma(W_const_r, rho_r), g_wk.mesh1(), false);
+  g_wk_t sigma_wk = fourier_wr_to_wk(fourier_tr_to_wr(sigma_dyn_tr, g_wk.mesh0().n_max));
+  return detail::plus_static(sigma_wk, sigma_fock_k);
+}
+
+/* eliashberg_g_delta_g_product (eliashberg.hpp:150; eliashberg.cpp:31-58) */
+inline g_wk_t eliashberg_g_delta_g_product(g_wk_vt g_wk, g_wk_vt delta_wk) {
+  tprf_ctx *cx = detail::context();
+  g_wk_t F(delta_wk.mesh0(), delta_wk.mesh1(), delta_wk.norb());
+  detail::check(tprf_eliashberg_g_delta_g_product(cx, g_wk.mesh0().n_max, delta_wk.mesh0().n_max,
+                                                  delta_wk.mesh1().dims.data(), delta_wk.norb(), detail::c(g_wk.data()),
+                                                  detail::c(delta_wk.data()), detail::c(F.data())),
+                cx, "eliashberg_g_delta_g_product");
+  return F;
+}
+/* eliashberg_product by summation (eliashberg.hpp:52; eliashberg.cpp:92-120) */
+inline g_wk_t eliashberg_product(chi_wk_vt Gamma_pp, g_wk_vt g_wk, g_wk_vt delta_wk) {
+  tprf_ctx *cx = detail::context();
+  g_wk_t out(delta_wk.mesh0(), delta_wk.mesh1(), delta_wk.norb());
+  detail::check(tprf_eliashberg_product(cx, delta_wk.mesh0().beta, Gamma_pp.mesh0().n_max, g_wk.mesh0().n_max,
+                                        delta_wk.mesh0().n_max, delta_wk.mesh1().dims.data(), delta_wk.norb(),
+                                        detail::c(Gamma_pp.data()), detail::c(g_wk.data()),
+                                        detail::c(delta_wk.data()), detail::c(out.data())),
+                cx, "eliashberg_product");
+  return out;
+}
+/* dynamic_and_constant_to_tr (eliashberg.hpp:159; eliashberg.cpp:122-134) */
+inline std::tuple<chi_tr_t, chi_r_t> dynamic_and_constant_to_tr(chi_wk_vt Gamma_pp_dyn_wk, chi_k_vt Gamma_pp_const_k) {
+  chi_tr_t dyn_tr = chi_tr_from_chi_wr(chi_wr_from_chi_wk(Gamma_pp_dyn_wk));
+  chi_r_t const_r = detail::lattice1(Gamma_pp_const_k, detail::to_r(Gamma_pp_const_k.mesh0()), true);
+  return {dyn_tr, const_r};
+}
+/* eliashberg_dynamic_gamma_f_product / eliashberg_constant_gamma_f_product
+ * (eliashberg.hpp:169,177; eliashberg.cpp:175-186, 214-240) */
+inline g_tr_t eliashberg_dynamic_gamma_f_product(chi_tr_vt Gamma_pp_dyn_tr, g_tr_vt F_tr) {
+  if (Gamma_pp_dyn_tr.mesh0().n_tau != F_tr.mesh0().n_tau)
+    throw error(TPRF_ERR_ARG, "The size of the imaginary time mesh of Gamma must be the size of the mesh of Delta.");
+  tprf_ctx *cx = detail::context();
+  g_tr_t out(F_tr.mesh0(), F_tr.mesh1(), F_tr.norb());
+  detail::check(tprf_eliashberg_gamma_f_product(cx, F_tr.mesh0().size() * F_tr.mesh1().size(), F_tr.norb(),
+                                                detail::c(Gamma_pp_dyn_tr.data()), detail::c(F_tr.data()),
+                                                detail::c(out.data())),
+                cx, "eliashberg_dynamic_gamma_f_product");
+  return out;
+}
+inline e_r_t eliashberg_constant_gamma_f_product(chi_r_vt Gamma_pp_const_r, g_tr_vt F_tr) {
+  tprf_ctx *cx = detail::context();
+  e_r_t out(F_tr.mesh1(), mesh::none{}, F_tr.norb());
+  detail::check(tprf_eliashberg_gamma_f_product(cx, F_tr.mesh1().size(), F_tr.norb(), detail::c(Gamma_pp_const_r.data()),
+                                                detail::c(F_tr.data()), detail::c(out.data())),
+                cx, "eliashberg_constant_gamma_f_product");
+  return out;
+}
+/* eliashberg_product_fft (eliashberg.hpp:146; eliashberg.cpp:242-268) */
+inline g_wk_t eliashberg_product_fft(chi_tr_vt Gamma_pp_dyn_tr, chi_r_vt Gamma_pp_const_r, g_wk_vt g_wk,
+                                     g_wk_vt delta_wk) {
+  g_tr_t F_tr = fourier_wr_to_tr(fourier_wk_to_wr(eliashberg_g_delta_g_product(g_wk, delta_wk)));
+  g_tr_t delta_tr = eliashberg_dynamic_gamma_f_product(Gamma_pp_dyn_tr, F_tr);
+  e_r_t delta_r = eliashberg_constant_gamma_f_product(Gamma_pp_const_r, F_tr);
+  return fourier_wr_to_wk(detail::plus_static(fourier_tr_to_wr(delta_tr), delta_r));
+}
+/* eliashberg_product_fft_constant (eliashberg.hpp:148; eliashberg.cpp:279-302) */
+inline g_wk_t eliashberg_product_fft_constant(chi_r_vt Gamma_pp_const_r, g_wk_vt g_wk, g_wk_vt delta_wk) {
+  g_wk_t F_wk = eliashberg_g_delta_g_product(g_wk, delta_wk);
+  g_tr_t F_tr = fourier_wr_to_tr(fourier_wk_to_wr(F_wk));
+  e_k_t delta_k = detail::lattice1(eliashberg_constant_gamma_f_product(Gamma_pp_const_r, F_tr), delta_wk.mesh1(), false);
+  g_wk_t zero(F_wk.mesh0(), F_wk.mesh1(), F_wk.norb());
+  detail::hip_check(hipMemsetAsync(zero.data(), 0, sizeof(dcomplex) * (size_t)zero.size(), detail::state().stream),
+                    "hipMemsetAsync");
+  return detail::plus_static(zero, delta_k);
+}
+/* construct_phi_wk (eliashberg.hpp:199; eliashberg.cpp:336-360); U in host memory */
+inline chi_wk_t construct_phi_wk(chi_wk_vt chi, array4_view U) {
+  tprf_ctx *cx = detail::context();
+  const int n = chi.norb();
+  if (U.n != n) throw error(TPRF_ERR_ARG, "construct_phi_wk: U does not match chi's target");
+  const size_t un = (size_t)n * n * n * n;
+  dcomplex *ud = nullptr;
+  detail::hip_check(hipMalloc((void **)&ud, sizeof(dcomplex) * un), "hipMalloc");
+  std::shared_ptr<dcomplex> hold(ud, [](dcomplex *q) { (void)hipFree(q); });
+  detail::hip_check(hipMemcpyAsync(ud, U.data, sizeof(dcomplex) * un, hipMemcpyHostToDevice, detail::state().stream),
+                    "hipMemcpyAsync");
+  chi_wk_t out(chi.mesh0(), chi.mesh1(), n);
+  detail::check(tprf_construct_phi_wk(cx, chi.mesh0().size() * chi.mesh1().size(), n, detail::c(chi.data()),
+                                      detail::c(ud), detail::c(out.data())),
+                cx, "construct_phi_wk");
+  detail::hip_check(hipStreamSynchronize(detail::state().stream), "hipStreamSynchronize");  // U's copy in use
+  return out;
 }
 
 // ---- batched forward chain (tprf_forward_batched) --------------------------

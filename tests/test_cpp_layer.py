@@ -26,7 +26,11 @@ REFERENCE_FUNCTIONS = ["lattice_dyson_g0_wk", "lattice_dyson_g_wk", "fourier_wk_
                        "fourier_wr_to_tr", "fourier_tr_to_wr", "chi0_tr_from_grt_PH", "chi0_wr_from_grt_PH",
                        "chi0_w0r_from_grt_PH", "chi_w0r_from_chi_tr", "chi_wr_from_chi_tr", "chi_tr_from_chi_wr",
                        "chi_wk_from_chi_wr", "chi_wr_from_chi_wk", "solve_rpa_PH", "lindhard_chi00",
-                       "imtime_bubble_chi0_wk"]
+                       "imtime_bubble_chi0_wk", "split_into_dynamic_wk_and_constant_k", "rho_k_from_g_wk",
+                       "gw_dynamic_sigma", "hartree_sigma", "fock_sigma", "gw_sigma", "eliashberg_g_delta_g_product",
+                       "eliashberg_product", "dynamic_and_constant_to_tr", "eliashberg_dynamic_gamma_f_product",
+                       "eliashberg_constant_gamma_f_product", "eliashberg_product_fft",
+                       "eliashberg_product_fft_constant", "construct_phi_wk"]
 
 
 def test_header_declares_reference_signatures():
@@ -40,15 +44,20 @@ def test_header_declares_reference_signatures():
     assert "fourier_wr_to_tr(g_wr_cvt g_wr, int nt = -1)" in text
     assert "chi0_wr_from_grt_PH(g_tr_cvt g_tr, int nw)" in text
     assert "solve_rpa_PH(chi_wk_vt chi0, array4_view U)" in text
+    assert "gw_sigma(chi_wk_cvt W_wk, g_wk_cvt g_wk)" in text
+    assert "gw_dynamic_sigma(chi_tr_cvt W_tr, g_tr_cvt g_tr)" in text
+    assert "eliashberg_product_fft(chi_tr_vt Gamma_pp_dyn_tr, chi_r_vt Gamma_pp_const_r, g_wk_vt g_wk," in text
+    assert "construct_phi_wk(chi_wk_vt chi, array4_view U)" in text
 
 
-def test_header_compiles_and_links_with_host_compiler():
+@pytest.mark.parametrize("prog", ["chain_cpp", "consumers_cpp"])
+def test_header_compiles_and_links_with_host_compiler(prog):
     lib = os.path.join(ROOT, "tprf_amd", "libtprf_hip.so")
     assert os.path.exists(lib), "build first"
     with tempfile.TemporaryDirectory() as d:
-        out = os.path.join(d, "chain_cpp")
+        out = os.path.join(d, prog)
         cmd = ["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
-               "-I" + os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "cpp", "chain_cpp.cpp"), "-o", out,
+               "-I" + os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "cpp", prog + ".cpp"), "-o", out,
                "-L" + os.path.join(ROOT, "tprf_amd"), "-ltprf_hip", "-L/opt/rocm/lib", "-lamdhip64"]
         r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
         assert r.returncode == 0, r.stderr
@@ -96,3 +105,54 @@ def test_cpp_caller_chain_matches_oracle(oracle):
           f"{relerr(res[2], chi):.2e}, batch {relerr(res[3], chi):.2e} / {relerr(res[4], chi):.2e}", flush=True)
     assert kappa_ok
     _ = size
+
+
+@pytest.mark.gpu
+def test_cpp_caller_gw_and_eliashberg_match_oracle(oracle):
+    """tests/cpp/consumers_cpp (gw_sigma of W and of a static v, construct_phi_wk,
+    the summation and FFT Eliashberg products through the C++ signatures)
+    against the oracle: 1e-11 (gw_sigma, FFT product), 1e-13 (the pointwise
+    ones); the summation and FFT products agree to 1e-9."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("no GPU visible")
+    binp = os.path.join(ROOT, "tests", "cpp", "consumers_cpp")
+    assert os.path.exists(binp), "build first: tests/cpp/consumers_cpp"
+    from test_oracle_consumers import _eliashberg_inputs
+    beta, nw, dims, g, gamma, delta = _eliashberg_inputs(3)
+    n, nk = 2, int(np.prod(dims))
+    rng = np.random.default_rng(8)
+    inu = 1j * 2 * np.pi * np.arange(-(nw - 1), nw) / beta
+    V = (rng.normal(size=(nk, n, n, n, n)) + 1j * rng.normal(size=(nk, n, n, n, n))) * 0.2
+    W = V[None] + 0.3 * (2.0 / (inu ** 2 - 1.0))[:, None, None, None, None, None] * np.ones_like(V)[None]
+    U = rng.normal(size=(n, n, n, n)) + 0j
+    nbig = 2 * nw + 1
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as f:
+            np.array(list(dims) + [n, nw, nbig, 0, 0], dtype=np.int32).tofile(f)
+            np.array([beta], dtype=np.float64).tofile(f)
+            for a in (g, W, V, delta, gamma(nw), gamma(nbig), U):
+                np.ascontiguousarray(a, dtype=np.complex128).tofile(f)
+        print("running tests/cpp/consumers_cpp (2 orbitals, 3 k, n_max 30)", flush=True)
+        r = subprocess.run([binp, fin, fout], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        out = np.fromfile(fout, dtype=np.complex128)
+    sizes = [g.size, nk * n * n, gamma(nw).size, delta.size, delta.size]
+    parts, o = [], 0
+    for s_ in sizes:
+        parts.append(out[o:o + s_])
+        o += s_
+    assert o == out.size
+    sig = parts[0].reshape(g.shape)
+    assert relerr(sig, oracle.gw_sigma(W, g, beta, dims)) <= 1e-11
+    sk = parts[1].reshape(nk, n, n)
+    assert relerr(sk, oracle.static_sigma_k(V, oracle.density(g, beta), dims, fock=True)) <= 1e-12
+    phi = parts[2].reshape(gamma(nw).shape)
+    assert relerr(phi, oracle.construct_phi_wk(gamma(nw), U)) <= 1e-13
+    ds = parts[3].reshape(delta.shape)
+    assert relerr(ds, oracle.eliashberg_product(gamma(nbig), g, delta, beta, dims)) <= 1e-13
+    df = parts[4].reshape(delta.shape)
+    odyn, oconst = oracle.preprocess_gamma_for_fft(gamma(nw), beta, dims)
+    assert relerr(df, oracle.eliashberg_product_fft(odyn, oconst, g, delta, beta, dims)) <= 1e-11
+    assert np.abs(ds - df).max() <= 1e-9
