@@ -17,6 +17,8 @@ namespace tprf {
 constexpr int kQC = 0, kQS = 9, kQY = 17, kQYp = 21, kQM = 25, kQSlots = 29;  // S_j at kQS + j - 1
 
 __device__ __forceinline__ c2 cmul_i(c2 a) { return c2{-a.y, a.x}; }  // i a
+// acc + t a for real t (two FMAs)
+__device__ __forceinline__ c2 cfma_r(c2 acc, double t, c2 a) { return c2{fma(t, a.x, acc.x), fma(t, a.y, acc.y)}; }
 
 __device__ __forceinline__ c2 corr_value(const c2 *q1, const c2 *q2, long ncol, const StaticMatsDev &sm) {
   auto a = [&](int k) { return q1[(long)k * ncol]; };
@@ -87,15 +89,16 @@ __device__ __forceinline__ void qbar_spread(const c2 *qb, const double *t, c2 &s
   sb = c2{0, 0};
   db = c2{0, 0};
 #pragma unroll
-  for (int j = 0; j < 9; ++j) sb = cfma(sb, c2{t[j], 0.0}, qb[kQC + j]);
+  for (int j = 0; j < 9; ++j) sb = cfma_r(sb, t[j], qb[kQC + j]);
 #pragma unroll
-  for (int j = 0; j < 8; ++j) db = cfma(db, c2{t[9 + j], 0.0}, qb[kQS + j]);
+  for (int j = 0; j < 8; ++j) db = cfma_r(db, t[9 + j], qb[kQS + j]);
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    sb = cfma(sb, c2{t[17 + i], 0.0}, qb[kQY + i]);
-    sb = cfma(sb, c2{t[25 + i], 0.0}, qb[kQYp + i]);
-    db = csub(db, cmul_i(cscale(qb[kQY + i], t[21 + i])));
-    db = csub(db, cmul_i(cscale(qb[kQYp + i], t[29 + i])));
+    sb = cfma_r(sb, t[17 + i], qb[kQY + i]);
+    sb = cfma_r(sb, t[25 + i], qb[kQYp + i]);
+    // db -= i (Im Y) qbar: (x, y) -> (x + t y, y - t x)
+    db = c2{fma(t[21 + i], qb[kQY + i].y, db.x), fma(-t[21 + i], qb[kQY + i].x, db.y)};
+    db = c2{fma(t[29 + i], qb[kQYp + i].y, db.x), fma(-t[29 + i], qb[kQYp + i].x, db.y)};
   }
 }
 

@@ -125,13 +125,16 @@ __device__ __forceinline__ c2 charpoly(const c2 (&c)[N], c2 z) {
   return v;
 }
 
+// 1 / b: the reciprocal of |b|^2 by v_rcp_f64 refined with two Newton steps
+// (each squares the relative error, leaving rounding), instead of an IEEE
+// division; |b|^2 = |p(i w + mu - s)|^2 stays in the normal range here
 __device__ __forceinline__ c2 crecip(c2 b) {
-  const double d = 1.0 / (b.x * b.x + b.y * b.y);
-  return c2{b.x * d, -b.y * d};
+  const double d = fma(b.x, b.x, b.y * b.y);
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(r, fma(-d, r, 1.0), r);
+  r = fma(r, fma(-d, r, 1.0), r);
+  return c2{b.x * r, -b.y * r};
 }
-
-// acc + t a for real t
-__device__ __forceinline__ c2 cfma_r(c2 acc, double t, c2 a) { return c2{fma(t, a.x, acc.x), fma(t, a.y, acc.y)}; }
 
 // k - Q (sgn = -1) and k + Q (sgn = +1) on the lattice (i0 d1 + i1) d2 + i2
 __device__ __forceinline__ long shift_k(long k, int d0, int d1, int d2, int q0, int q1, int q2, int sgn) {
@@ -165,12 +168,14 @@ struct Elem {
 }  // namespace
 
 // F1. Partial projections of every k column (inst, k, x, y) of G over the
-//     frequency chunk blockIdx.y (n >= 0); part[sk][33][ncol] as in
-//     static_mats.hip.  mu[nbatch], e_k[nbatch][nk][N][N].
+//     frequency chunk blockIdx.y (n >= 0): part[sk][33][ncol] as in
+//     static_mats.hip, or with one chunk the slots 0..24 of q themselves.
+//     mu[nbatch], e_k[nbatch][nk][N][N].
 template <int N>
 __global__ void __launch_bounds__(256) k_sq_proj_part(long nbatch, long nk, int n_iw, int chunk, double beta,
                                                       const double *__restrict__ mu, const c2 *__restrict__ ek,
-                                                      const double *__restrict__ tab, c2 *__restrict__ part) {
+                                                      const double *__restrict__ tab, c2 *__restrict__ part,
+                                                      c2 *__restrict__ q) {
   constexpr int n2 = N * N;
   const long ncol = nbatch * nk * n2;
   const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -204,78 +209,130 @@ __global__ void __launch_bounds__(256) k_sq_proj_part(long nbatch, long nk, int 
       PB[i] = cfma_r(PB[i], t[29 + i], dl);
     }
   }
-  c2 *o = part + (long)blockIdx.y * 33 * ncol + col;
+  if (gridDim.y == 1) {  // one chunk: the projections themselves
+    c2 *o = q + col;
 #pragma unroll
-  for (int j = 0; j < 9; ++j) o[(long)j * ncol] = C[j];
+    for (int j = 0; j < 9; ++j) o[(long)j * ncol] = C[j];
 #pragma unroll
-  for (int j = 0; j < 8; ++j) o[(long)(9 + j) * ncol] = S[j];
+    for (int j = 0; j < 8; ++j) o[(long)(9 + j) * ncol] = S[j];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    o[(long)(17 + i) * ncol] = YA[i];
-    o[(long)(21 + i) * ncol] = YB[i];
-    o[(long)(25 + i) * ncol] = PA[i];
-    o[(long)(29 + i) * ncol] = PB[i];
+    for (int i = 0; i < 4; ++i) {
+      o[(long)(kQY + i) * ncol] = cadd(YA[i], cmul_i(YB[i]));
+      o[(long)(kQYp + i) * ncol] = cadd(PA[i], cmul_i(PB[i]));
+    }
+  } else {
+    c2 *o = part + (long)blockIdx.y * 33 * ncol + col;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) o[(long)j * ncol] = C[j];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) o[(long)(9 + j) * ncol] = S[j];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      o[(long)(17 + i) * ncol] = YA[i];
+      o[(long)(21 + i) * ncol] = YB[i];
+      o[(long)(25 + i) * ncol] = PA[i];
+      o[(long)(29 + i) * ncol] = PB[i];
+    }
   }
 }
 
-// F2. Projections q[29][ncol]: chunk sums added in order, and the tail fit of
-//     each k column (G evaluated at the fitted frequencies).
+// F1b. The tail fit of every k column.  A block holds 64 columns (lanes) and
+//      kFitG waves; wave g evaluates G at the fitted frequencies f = g, g +
+//      kFitG, ... (the same f across the wave, so the fit operator is read by
+//      scalar loads) and applies its share of the least-squares operator.  The
+//      partial coefficients meet in LDS (added in wave order), and each wave
+//      takes the residual on the values it kept.  Writes the moment slots
+//      q[kQM..kQM+3] and the status words.
+constexpr int kFitG = 4;
 template <int N>
-__global__ void __launch_bounds__(256) k_sq_proj_final(long nbatch, long nk, int n_iw, int sk, double beta,
+__global__ void __launch_bounds__(64 * kFitG) k_sq_fit(long nbatch, long nk, int n_iw, double beta,
                                                        const double *__restrict__ mu, const c2 *__restrict__ ek,
-                                                       const c2 *__restrict__ part, TailFitDev fit,
-                                                       c2 *__restrict__ q, double *status) {
-  constexpr int n2 = N * N;
+                                                       TailFitDev fit, c2 *__restrict__ q, double *status) {
+  constexpr int n2 = N * N, FP = (kMaxFit + kFitG - 1) / kFitG;
+  __shared__ c2 sc[kFitG][kMaxOrder][64];
+  __shared__ double sr[kFitG][64];
+  // g is the wave's index: uniform across it (readfirstlane makes that known)
+  const int lane = threadIdx.x & 63, g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long ncol = nbatch * nk * n2;
-  const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= ncol) return;
-  const int xy = (int)(col % n2), x = xy / N, y = xy - x * N;
-  const long ik = col / n2, inst = ik / nk;
-  c2 *o = q + col;
-  for (int j = 0; j < 17; ++j) {
-    c2 v = c2{0, 0};
-    for (int k = 0; k < sk; ++k) v = cadd(v, part[((long)k * 33 + j) * ncol + col]);
-    o[(long)j * ncol] = v;
-  }
-  for (int i = 0; i < 4; ++i) {
-    c2 ya = c2{0, 0}, yb = c2{0, 0}, pa = c2{0, 0}, pb = c2{0, 0};
-    for (int k = 0; k < sk; ++k) {
-      const c2 *pk = part + (long)k * 33 * ncol + col;
-      ya = cadd(ya, pk[(long)(17 + i) * ncol]);
-      yb = cadd(yb, pk[(long)(21 + i) * ncol]);
-      pa = cadd(pa, pk[(long)(25 + i) * ncol]);
-      pb = cadd(pb, pk[(long)(29 + i) * ncol]);
-    }
-    o[(long)(kQY + i) * ncol] = cadd(ya, cmul_i(yb));
-    o[(long)(kQYp + i) * ncol] = cadd(pa, cmul_i(pb));
-  }
+  const long col = (long)blockIdx.x * 64 + lane;
+  const bool live = col < ncol;
+  c2 v[FP], pc[kMaxOrder];
+#pragma unroll
+  for (int k = 0; k < kMaxOrder; ++k) pc[k] = c2{0, 0};
+#pragma unroll
+  for (int i = 0; i < FP; ++i) v[i] = c2{0, 0};
   Elem<N> E;
-  E.make(ek + ik * n2, mu[inst], x, y);
+  if (live) {
+    const int xy = (int)(col % n2), x = xy / N, y = xy - x * N;
+    const long ik = col / n2;
+    E.make(ek + ik * n2, mu[ik / nk], x, y);
+  }
+#pragma unroll
+  for (int i = 0; i < FP; ++i) {
+    const int f = g + i * kFitG;
+    if (f < fit.n_fit) {
+      if (live) v[i] = E.at(wn(fit.idx[f] - n_iw, beta));
+#pragma unroll
+      for (int k = 0; k < kMaxOrder; ++k)
+        if (k < fit.ncoef) pc[k] = cfma(pc[k], fit.pinv[(long)k * fit.n_fit + f], v[i]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kMaxOrder; ++k) sc[g][k][lane] = pc[k];
+  __syncthreads();
   c2 coef[kMaxOrder];
 #pragma unroll
-  for (int k = 0; k < kMaxOrder; ++k) coef[k] = c2{0, 0};
-  for (int f = 0; f < fit.n_fit; ++f) {
-    const c2 v = E.at(wn(fit.idx[f] - n_iw, beta));
+  for (int k = 0; k < kMaxOrder; ++k) {
+    c2 t = c2{0, 0};
 #pragma unroll
-    for (int k = 0; k < kMaxOrder; ++k)
-      if (k < fit.ncoef) coef[k] = cfma(coef[k], fit.pinv[(long)k * fit.n_fit + f], v);
+    for (int h = 0; h < kFitG; ++h) t = cadd(t, sc[h][k][lane]);
+    coef[k] = t;
   }
   double r2 = 0.0;
-  for (int f = 0; f < fit.n_fit; ++f) {
-    c2 v = E.at(wn(fit.idx[f] - n_iw, beta));
 #pragma unroll
-    for (int k = 0; k < kMaxOrder; ++k)
-      if (k < fit.ncoef) v = cfms(v, fit.vander[(long)f * fit.ncoef + k], coef[k]);
-    r2 += cabs2(v);
+  for (int i = 0; i < FP; ++i) {
+    const int f = g + i * kFitG;
+    if (f < fit.n_fit) {
+      c2 d = v[i];
+#pragma unroll
+      for (int k = 0; k < kMaxOrder; ++k)
+        if (k < fit.ncoef) d = cfms(d, fit.vander[(long)f * fit.ncoef + k], coef[k]);
+      r2 += cabs2(d);
+    }
   }
-  atomic_max_nonneg(&status[0], sqrt(r2 / fit.n_fit));
+  sr[g][lane] = r2;
+  __syncthreads();
+  if (!live || g != 0) return;
+  double rt = 0.0;
+#pragma unroll
+  for (int h = 0; h < kFitG; ++h) rt += sr[h][lane];
+  atomic_max_nonneg(&status[0], sqrt(rt / fit.n_fit));
   atomic_max_nonneg(&status[1], sqrt(cabs2(coef[0])));
   const c2 m1 = cscale(coef[1], fit.wpow[1]);
   c2 a[3];
   Poles<0>::coef(m1, cscale(coef[2], fit.wpow[2]), cscale(coef[3], fit.wpow[3]), a);
+  c2 *o = q + col;
 #pragma unroll
   for (int i = 0; i < 3; ++i) o[(long)(kQM + i) * ncol] = a[i];
   o[(long)(kQM + 3) * ncol] = m1;
+}
+
+// F2. With several chunks: q slot blockIdx.y (< 25) of every column from the
+//     chunk partials, added in chunk order (y and y' combine two partial
+//     slots: sum over cos-rows + i times sum over sin-rows).
+__global__ void __launch_bounds__(256) k_sq_proj_sum(long ncol, int sk, const c2 *__restrict__ part,
+                                                     c2 *__restrict__ q) {
+  const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ncol) return;
+  const int s = blockIdx.y;
+  const int s1 = s < 17 ? s : s < 21 ? s : 25 + (s - 21);
+  const int s2 = s < 17 ? -1 : s < 21 ? 21 + (s - 17) : 29 + (s - 21);
+  c2 v1 = c2{0, 0}, v2 = c2{0, 0};
+  for (int k = 0; k < sk; ++k) {
+    v1 = cadd(v1, part[((long)k * 33 + s1) * ncol + col]);
+    if (s2 >= 0) v2 = cadd(v2, part[((long)k * 33 + s2) * ncol + col]);
+  }
+  q[(long)s * ncol + col] = s2 < 0 ? v1 : cadd(v1, cmul_i(v2));
 }
 
 // F3. Thread (k, d) of one instance, mesh chunk blockIdx.y:
@@ -371,16 +428,22 @@ __global__ void __launch_bounds__(BT) k_sq_pair(long nbatch, long nk, int d0, in
   }
 }
 
-// F4. chi0_q[inst][abcd] = (1/N_k) sum over chunks and blocks.
-__global__ void __launch_bounds__(256) k_sq_pair_final(long nbatch, int n4, int nparts, double inv_nk,
-                                                       const c2 *__restrict__ part, c2 *__restrict__ chi0_q) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nbatch * n4) return;
-  const long inst = t / n4;
-  const int o = (int)(t - inst * n4);
+// F4. chi0_q[inst][abcd] = (1/N_k) sum over chunks and blocks: one wave per
+//     output, a fixed-order tree over the wave's partial sums.
+__global__ void __launch_bounds__(64) k_sq_pair_final(int n4, int nparts, double inv_nk, const c2 *__restrict__ part,
+                                                      c2 *__restrict__ chi0_q) {
+  __shared__ c2 red[64];
+  const long inst = blockIdx.x / n4;
+  const int o = (int)(blockIdx.x - inst * n4), tid = threadIdx.x;
   c2 v = c2{0, 0};
-  for (int p = 0; p < nparts; ++p) v = cadd(v, part[(inst * nparts + p) * n4 + o]);
-  chi0_q[t] = cscale(v, inv_nk);
+  for (int p = tid; p < nparts; p += 64) v = cadd(v, part[(inst * nparts + p) * n4 + o]);
+  red[tid] = v;
+  __syncthreads();
+  for (int w = 32; w > 0; w >>= 1) {
+    if (tid < w) red[tid] = cadd(red[tid], red[tid + w]);
+    __syncthreads();
+  }
+  if (tid == 0) chi0_q[inst * n4 + o] = cscale(red[0], inv_nk);
 }
 
 // A1. qbar[29][ncol] of every k column (x, y): role 1 in the outputs
@@ -542,9 +605,11 @@ __global__ void __launch_bounds__(256) k_sq_ebar(long nbatch, long nk, int d0, i
   mpart[(inst * gridDim.y + blockIdx.y) * nk * n2 + (k * n2 + xy)] = mub;
 }
 
-// A3. ebar_k[inst][k][i][j] = sum over chunks and the n^2 columns of k.
-__global__ void __launch_bounds__(256) k_sq_ebar_final(long nk_total, int n2, int sk, long ncol,
-                                                       const c2 *__restrict__ epart, c2 *__restrict__ ebar) {
+// A3. ebar_k[inst][k][i][j] = sum over chunks and the n^2 columns of k; the
+//     thread of (k, i j = 0) also sums k's mubar partials into mk[inst][k].
+__global__ void __launch_bounds__(256) k_sq_ebar_final(long nk, long nk_total, int n2, int sk, long ncol,
+                                                       const c2 *__restrict__ epart, const double *__restrict__ mpart,
+                                                       c2 *__restrict__ ebar, double *__restrict__ mk) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nk_total * n2) return;
   const long ik = t / n2;
@@ -553,6 +618,13 @@ __global__ void __launch_bounds__(256) k_sq_ebar_final(long nk_total, int n2, in
   for (int yk = 0; yk < sk; ++yk)
     for (int xy = 0; xy < n2; ++xy) v = cadd(v, epart[((long)yk * ncol + ik * n2 + xy) * n2 + ij]);
   ebar[t] = v;
+  if (ij == 0) {
+    const long inst = ik / nk, k = ik - inst * nk;
+    double m = 0.0;
+    for (int yk = 0; yk < sk; ++yk)
+      for (int xy = 0; xy < n2; ++xy) m += mpart[((inst * sk + yk) * nk + k) * n2 + xy];
+    mk[ik] = m;
+  }
 }
 
 namespace {
@@ -594,7 +666,8 @@ size_t static_q_work_bytes(long nbatch, const int dims[3], int n, int n_iw, bool
   if (!adjoint)
     elems += (size_t)nbatch * P.sk2 * P.nblk * n * n * n * n;
   else
-    elems += (size_t)kQSlots * P.ncol + (size_t)P.sk3 * P.ncol * n * n + ((size_t)P.sk3 * P.ncol + 1) / 2;
+    elems += (size_t)kQSlots * P.ncol + (size_t)P.sk3 * P.ncol * n * n + ((size_t)P.sk3 * P.ncol + 1) / 2 +
+             ((size_t)nbatch * dims[0] * dims[1] * dims[2] + 1) / 2;
   return elems * sizeof(c2);
 }
 
@@ -612,17 +685,19 @@ static hipError_t static_q_run(hipStream_t s, long nbatch, const int dims[3], co
   const unsigned bc = (unsigned)((ncol + 255) / 256);
   if (!reuse) {
     hipLaunchKernelGGL(k_sq_proj_part<N>, dim3(bc, P.sk), dim3(256), 0, s, nbatch, nk, n_iw, P.chunk, beta, mu, ek,
-                       tab, part);
-    hipLaunchKernelGGL(k_sq_proj_final<N>, dim3(bc), dim3(256), 0, s, nbatch, nk, n_iw, P.sk, beta, mu, ek, part,
-                       fit, q, status);
+                       tab, part, q);
+    hipLaunchKernelGGL(k_sq_fit<N>, dim3((unsigned)((ncol + 63) / 64)), dim3(64 * kFitG), 0, s, nbatch, nk, n_iw,
+                       beta, mu, ek, fit, q, status);
+    if (P.sk > 1)
+      hipLaunchKernelGGL(k_sq_proj_sum, dim3(bc, kQM), dim3(256), 0, s, ncol, P.sk, part, q);
   }
   if (!chibar) {  // forward
     constexpr int BT = pair_bt(N);
     hipLaunchKernelGGL((k_sq_pair<N, BT>), dim3((unsigned)(nbatch * P.nblk), P.sk2), dim3(BT), 0, s, nbatch, nk,
                        dims[0], dims[1], dims[2], qv[0], qv[1], qv[2], n_iw, P.chunk2, (int)P.nblk, beta, mu, ek, q,
                        sm, rest);
-    hipLaunchKernelGGL(k_sq_pair_final, dim3((unsigned)((nbatch * n4 + 255) / 256)), dim3(256), 0, s, nbatch, n4,
-                       (int)(P.sk2 * P.nblk), 1.0 / (double)nk, rest, chi0_q);
+    hipLaunchKernelGGL(k_sq_pair_final, dim3((unsigned)(nbatch * n4)), dim3(64), 0, s, n4, (int)(P.sk2 * P.nblk),
+                       1.0 / (double)nk, rest, chi0_q);
     return hipGetLastError();
   }
   c2 *qbar = rest, *epart = rest + (long)kQSlots * ncol;
@@ -632,11 +707,12 @@ static hipError_t static_q_run(hipStream_t s, long nbatch, const int dims[3], co
   hipLaunchKernelGGL(k_sq_ebar<N>, dim3(bc, P.sk3), dim3(256), 0, s, nbatch, nk, dims[0], dims[1], dims[2], qv[0],
                      qv[1], qv[2], n_iw, P.chunk3, beta, 1.0 / (double)nk, mu, ek, chibar, tab, fitpos, fit, qbar,
                      epart, mpart);
-  hipLaunchKernelGGL(k_sq_ebar_final, dim3((unsigned)((nbatch * nk * n2 + 255) / 256)), dim3(256), 0, s, nbatch * nk,
-                     n2, P.sk3, ncol, epart, ebar);
+  double *mk = mpart + ((P.sk3 * ncol + 1) / 2) * 2;  // after mpart, c2-aligned
+  hipLaunchKernelGGL(k_sq_ebar_final, dim3((unsigned)((nbatch * nk * n2 + 255) / 256)), dim3(256), 0, s, nk,
+                     nbatch * nk, n2, P.sk3, ncol, epart, mpart, ebar, mk);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  return launch_sum_partials(s, mpart, nbatch, (int)(P.sk3 * nk * n2), mubar);
+  return launch_sum_partials(s, mk, nbatch, (int)nk, mubar);
 }
 
 hipError_t launch_static_q(hipStream_t s, long nbatch, const int dims[3], const int q[3], int n, int n_iw, double beta,
