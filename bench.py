@@ -393,8 +393,9 @@ def main():
                     help="run one leg alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this path shards by the "
-              "design in DESIGN.md but is built and measured on one GPU)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the chain would shard "
+              "over GPUs by all-to-all transposes between its stages, but is built and measured on one GPU)",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -320,7 +320,7 @@ int tprf_lindhard_chi00_noalloc(tprf_ctx *ctx, double beta, double mu, long nz, 
 
 /* ---- batched forward chain ---------------------------------------------- */
 /* Many lattices of uneven (dims, norb, n_iw, beta, nb) through the whole
- * forward chain in one call (SURVEY's tprf_forward_batched; no counterpart in
+ * forward chain in one call (no counterpart in
  * the reference, whose driver handles one lattice per call,
  * lattice_utils.py:191-269, followed by solve_rpa_PH, rpa.cpp:29-68).  Per
  * instance: Dyson + k -> r (tprf_lattice_dyson_g_wr), omega -> tau

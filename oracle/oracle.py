@@ -1,7 +1,7 @@
 """numpy/ctypes front end of the CPU oracle (test infrastructure only).
 
-Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
-bench.py use this module, always as the checker.  It loads
+Only tests/, the smoke check of the package's entry module and the
+``cpu_baseline`` leg of bench.py use this module, always as the oracle the product is compared with.  It loads
 oracle/libtprf_oracle.so (built by ``make -C oracle``) and exposes one
 function per restated reference function, taking and returning complex128
 numpy arrays in the reference's row-major layout.

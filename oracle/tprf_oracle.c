@@ -2,7 +2,7 @@
  * tprf_oracle.c -- CPU restatement of the TRIQS/tprf 3.2 lattice bubble + RPA
  * chain (C99 + OpenMP).
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker the HIP product path is
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle the HIP product path is
  * compared with, and the CPU baseline timed beside it.  The product library
  * never links, loads or calls it.
  *

@@ -1,6 +1,6 @@
 /*
  * tprf_oracle.h -- CPU restatement of the TRIQS/tprf 3.2 lattice bubble + RPA
- * chain.  TEST INFRASTRUCTURE ONLY: this library is the checker that the HIP
+ * chain.  TEST INFRASTRUCTURE ONLY: this library is the oracle that the HIP
  * product path is compared against (and the timed CPU baseline).  It is never
  * linked into, loaded by, or called from the product library.
  *

@@ -27,7 +27,7 @@ def test_header_declares_hot_path():
 def test_library_exports_every_declared_symbol():
     import torch  # noqa: F401  (load torch's HIP runtime first, as the package does)
     path = os.path.join(ROOT, "tprf_amd", "libtprf_hip.so")
-    assert os.path.exists(path), "build first: python -c 'import __graft_entry__ as g; g.build()'"
+    assert os.path.exists(path), "build first: make -C tprf_amd/csrc"
     lib = ctypes.CDLL(path)
     missing = [n for n in declared() if not hasattr(lib, n)]
     assert not missing, missing

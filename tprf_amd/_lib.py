@@ -116,7 +116,7 @@ def lib():
         with _lock:
             if _lib is None:
                 if not os.path.exists(LIB_PATH):
-                    raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+                    raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C tprf_amd/csrc`")
                 l = ctypes.CDLL(LIB_PATH)
                 for name, args in _SIGNATURES.items():
                     f = getattr(l, name)
