@@ -365,7 +365,7 @@ def test_singular_dyson_and_rpa(T):
 def test_unsupported_sizes_are_refused(T):
     """Every size outside the kernels' limits returns TPRF_ERR_UNSUPPORTED (-9)
     from the host checks, before any launch: orbitals above the limits of
-    Dyson (6), RPA / W / Lindhard / bubble (4), the adjoints (RPA 2, Dyson 3),
+    Dyson (6), RPA / W / Lindhard / bubble and the RPA and Dyson adjoints (4),
     an omega -> tau mesh with no pruned kernel and ntau beyond one workgroup's
     LDS, and lattice FFT lengths with a prime factor above 13."""
     import torch
@@ -381,8 +381,8 @@ def test_unsupported_sizes_are_refused(T):
         "rpa_herm n=5": lambda: lib.tprf_solve_rpa_PH_herm(c, 1, 1, 5, p, p, p),
         "W n=5": lambda: lib.tprf_dynamical_screened_interaction_W(c, 1, 1, 5, p, p, 0, p),
         "lindhard n=5": lambda: lib.tprf_lindhard_chi00(c, 1.0, 0.0, 1, p, d, 5, p, p),
-        "rpa adjoint n=3": lambda: lib.tprf_solve_rpa_PH_adjoint(c, 1, 3, p, p, None, p, None, None),
-        "dyson adjoint n=4": lambda: lib.tprf_lattice_dyson_g_wk_adjoint(c, 4, 1, 4, 1, p, p, p,
+        "rpa adjoint n=5": lambda: lib.tprf_solve_rpa_PH_adjoint(c, 1, 5, p, p, None, p, None, None),
+        "dyson adjoint n=5": lambda: lib.tprf_lattice_dyson_g_wk_adjoint(c, 4, 1, 5, 1, p, p, p,
                                                                           _lib.dptr(x.real.contiguous())),
         # n_max 1025: M = 4096 > 1024, and ntau = 6151 needs 2 x 6151 x 16 bytes of LDS
         "w2t n_max=1025": lambda: lib.tprf_fourier_wr_to_tr(c, 0, 100.0, 1025, -1, 1, p, p, p),

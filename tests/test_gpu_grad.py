@@ -418,7 +418,7 @@ def test_r_to_q_and_adjoint(T, oracle, dims, q):
 
 
 # ---------------------------------------------------------------- Dyson adjoint
-@pytest.mark.parametrize("n", [1, 2, 3])
+@pytest.mark.parametrize("n", [1, 2, 3, 4])
 def test_dyson_adjoint_vs_torch(T, n):
     L = lib()
     rng = np.random.default_rng(13 + n)
@@ -444,8 +444,10 @@ def test_dyson_adjoint_vs_torch(T, n):
 
 
 # ---------------------------------------------------------------- RPA adjoint
-@pytest.mark.parametrize("n", [1, 2])
+@pytest.mark.parametrize("n", [1, 2, 3, 4])
 def test_rpa_point_adjoint_vs_torch(T, n):
+    """One thread per point (n <= 2) or one workgroup per point (n = 3, 4):
+    chi and both cotangents against torch autograd of the solve, 1e-12."""
     L = lib()
     rng = np.random.default_rng(17 + n)
     npts = 5
@@ -467,6 +469,36 @@ def test_rpa_point_adjoint_vs_torch(T, n):
     assert relerr(host(chi), chi_ref.detach().numpy()) <= 1e-13
     assert relerr(host(c0bar), c0.grad.numpy()) <= 1e-12
     assert relerr(host(Ubar), Ut.grad.numpy()) <= 1e-12
+
+
+def test_three_orbital_static_gradient_vs_torch_reference(T):
+    """StaticRPA with three orbitals (fused chi0, the workgroup-per-point RPA
+    adjoint): chi_RPA(Q, 0) and the (e_k, mu, U) gradients of Re tr chi against
+    the FP64 torch reference on a 4x4x2 lattice, 1e-9."""
+    from oracle.torch_ref import StaticChainRef
+    from tprf_amd import autograd as A
+    rng = np.random.default_rng(29)
+    beta, n_iw, dims, q, n = 4.0, 32, (4, 4, 2), (2, 1, 1), 3
+    nk = int(np.prod(dims))
+    h = rnd(rng, nk, n, n) * 0.4
+    e = (h + np.conj(np.swapaxes(h, 1, 2))) / 2 + np.diag([-0.5, 0.0, 0.6])[None]
+    U = rnd(rng, n, n, n, n) * 0.1
+    spec = A.StaticRPA(beta, n_iw, dims, q, norb=n)
+    e_g = T.tensor(e, device="cuda", requires_grad=True)
+    U_g = T.tensor(U, device="cuda", requires_grad=True)
+    mu_g = T.tensor(0.1, dtype=T.float64, device="cuda", requires_grad=True)
+    chi = A.chi_rpa_static(e_g, mu_g, U_g, spec)
+    A.trace_loss(chi).backward()
+    ref = StaticChainRef(beta, n_iw, dims, q)
+    e_r = T.tensor(e, requires_grad=True)
+    U_r = T.tensor(U, requires_grad=True)
+    mu_r = T.tensor(0.1, dtype=T.float64, requires_grad=True)
+    chi_r = ref.chi(e_r, mu_r, U_r)
+    A.trace_loss(chi_r).backward()
+    assert relerr(host(chi), chi_r.detach().numpy()) <= 1e-10
+    assert relerr(host(e_g.grad), e_r.grad.numpy()) <= 1e-9
+    assert relerr(host(U_g.grad), U_r.grad.numpy()) <= 1e-9
+    assert abs(float(mu_g.grad) - float(mu_r.grad)) <= 1e-9 * abs(float(mu_r.grad))
 
 
 # ---------------------------------------------------------------- full gradient

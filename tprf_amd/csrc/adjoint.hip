@@ -358,6 +358,117 @@ __global__ void __launch_bounds__(64) k_rpa_point(long npts, const c2 *__restric
     }
 }
 
+// RPA at single points for n = 3, 4 (M = n^2 = 9, 16 rows): one workgroup per
+// point, the matrices in LDS, Gauss-Jordan with partial pivoting on
+// [A | 1] (A = 1 - X U), then chi = A^-1 X and, with chibar given,
+// Xbar = A^-H chibar (1 + U chi)^H, Ubar = X^H A^-H chibar chi^H, as k_rpa_point.
+template <int M>
+__global__ void __launch_bounds__(256) k_rpa_point_wg(const c2 *__restrict__ chi0, const c2 *__restrict__ U,
+                                                      const c2 *__restrict__ chibar_t, c2 *__restrict__ chi,
+                                                      c2 *__restrict__ xbar_t, c2 *__restrict__ ubar_t,
+                                                      double *status) {
+  constexpr int MM = M * M;
+  __shared__ c2 X[M][M], MU[M][M], A[M][M], Ai[M][M], Y[M][M], T1[M][M], T2[M][M];
+  __shared__ c2 fcol[M], pinv_s;
+  __shared__ int piv;
+  const int tid = threadIdx.x;
+  const long p = blockIdx.x;
+  const c2 *x = chi0 + p * MM, *u = U + p * MM;
+  for (int t = tid; t < MM; t += blockDim.x) {
+    X[t / M][t % M] = x[t];
+    MU[t / M][t % M] = u[t];
+  }
+  __syncthreads();
+  for (int t = tid; t < MM; t += blockDim.x) {
+    const int i = t / M, j = t % M;
+    c2 s = c2{i == j ? 1.0 : 0.0, 0.0};
+    for (int l = 0; l < M; ++l) s = cfms(s, X[i][l], MU[l][j]);
+    A[i][j] = s;
+    Ai[i][j] = c2{i == j ? 1.0 : 0.0, 0.0};
+  }
+  __syncthreads();
+  bool singular = false;
+  for (int k = 0; k < M; ++k) {
+    if (tid == 0) {
+      int pv = k;
+      double best = cabs2(A[k][k]);
+      for (int i = k + 1; i < M; ++i)
+        if (cabs2(A[i][k]) > best) { best = cabs2(A[i][k]); pv = i; }
+      piv = pv;
+      singular = singular || best == 0.0;
+    }
+    __syncthreads();
+    const int pv = piv;
+    if (pv != k)
+      for (int t = tid; t < 2 * M; t += blockDim.x) {
+        c2 *ra = t < M ? &A[k][t] : &Ai[k][t - M], *rb = t < M ? &A[pv][t] : &Ai[pv][t - M];
+        const c2 tmp = *ra;
+        *ra = *rb;
+        *rb = tmp;
+      }
+    __syncthreads();
+    if (tid == 0) pinv_s = cdiv(c2{1.0, 0.0}, A[k][k]);
+    __syncthreads();
+    for (int t = tid; t < 2 * M; t += blockDim.x) {
+      c2 *r = t < M ? &A[k][t] : &Ai[k][t - M];
+      *r = cmul(*r, pinv_s);
+    }
+    __syncthreads();
+    for (int i = tid; i < M; i += blockDim.x) fcol[i] = A[i][k];
+    __syncthreads();
+    for (int t = tid; t < M * 2 * M; t += blockDim.x) {
+      const int i = t / (2 * M), j = t % (2 * M);
+      if (i == k) continue;
+      if (j < M) A[i][j] = cfms(A[i][j], fcol[i], A[k][j]);
+      else Ai[i][j - M] = cfms(Ai[i][j - M], fcol[i], Ai[k][j - M]);
+    }
+    __syncthreads();
+  }
+  if (tid == 0 && singular && status) status[2] = 1.0;
+  // Y = chi = Ai X
+  for (int t = tid; t < MM; t += blockDim.x) {
+    const int i = t / M, j = t % M;
+    c2 s = c2{0, 0};
+    for (int l = 0; l < M; ++l) s = cfma(s, Ai[i][l], X[l][j]);
+    Y[i][j] = s;
+    chi[p * MM + t] = s;
+  }
+  if (!chibar_t) return;  // uniform across the workgroup
+  __syncthreads();
+  const c2 *cb = chibar_t + p * MM;
+  // T1 = Z = Ai^H chibar ; T2 = W = 1 + U chi
+  for (int t = tid; t < MM; t += blockDim.x) {
+    const int i = t / M, j = t % M;
+    c2 z = c2{0, 0}, w = c2{i == j ? 1.0 : 0.0, 0.0};
+    for (int l = 0; l < M; ++l) {
+      z = cfma(z, cconj(Ai[l][i]), cb[l * M + j]);
+      w = cfma(w, MU[i][l], Y[l][j]);
+    }
+    T1[i][j] = z;
+    T2[i][j] = w;
+  }
+  __syncthreads();
+  // Xbar = Z W^H ; A <- X^H Z (A is free now)
+  for (int t = tid; t < MM; t += blockDim.x) {
+    const int i = t / M, j = t % M;
+    c2 xb = c2{0, 0}, xz = c2{0, 0};
+    for (int l = 0; l < M; ++l) {
+      xb = cfma(xb, T1[i][l], cconj(T2[j][l]));
+      xz = cfma(xz, cconj(X[l][i]), T1[l][j]);
+    }
+    xbar_t[p * MM + t] = xb;
+    A[i][j] = xz;
+  }
+  __syncthreads();
+  // Ubar = (X^H Z) chi^H
+  for (int t = tid; t < MM; t += blockDim.x) {
+    const int i = t / M, j = t % M;
+    c2 s = c2{0, 0};
+    for (int l = 0; l < M; ++l) s = cfma(s, A[i][l], cconj(Y[j][l]));
+    ubar_t[p * MM + t] = s;
+  }
+}
+
 // ---- launchers ---------------------------------------------------------
 hipError_t launch_q_project(hipStream_t s, const c2 *chi_r, c2 *chi_q, const int dims[3], const int q[3], int ntarget,
                             long nbatch) {
@@ -408,6 +519,7 @@ hipError_t launch_dyson_adjoint(hipStream_t s, int nw, long nk, int n, long nbat
     case 1: TPRF_DA(1); break;
     case 2: TPRF_DA(2); break;
     case 3: TPRF_DA(3); break;
+    case 4: TPRF_DA(4); break;
     default: return hipErrorInvalidValue;
   }
 #undef TPRF_DA
@@ -452,6 +564,8 @@ hipError_t launch_rpa_point(hipStream_t s, long npts, int n, const c2 *chi0, con
   switch (n) {
     case 1: hipLaunchKernelGGL(k_rpa_point<1>, dim3(nb), dim3(64), 0, s, npts, chi0, U, chibar, chi, xbar, ubar, st.d); break;
     case 2: hipLaunchKernelGGL(k_rpa_point<2>, dim3(nb), dim3(64), 0, s, npts, chi0, U, chibar, chi, xbar, ubar, st.d); break;
+    case 3: hipLaunchKernelGGL(k_rpa_point_wg<9>, dim3(npts), dim3(128), 0, s, chi0, U, chibar, chi, xbar, ubar, st.d); break;
+    case 4: hipLaunchKernelGGL(k_rpa_point_wg<16>, dim3(npts), dim3(256), 0, s, chi0, U, chibar, chi, xbar, ubar, st.d); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

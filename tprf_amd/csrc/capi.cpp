@@ -1187,7 +1187,7 @@ int tprf_lattice_dyson_g_wk_adjoint(tprf_ctx *ctx, int n_iw, long nk, int norb, 
   if (!ctx) return TPRF_ERR_ARG;
   if (n_iw < 1 || nk < 1 || nbatch < 1 || !g_wk || !gbar_wk || !ebar_k || !mubar)
     return fail(ctx, TPRF_ERR_ARG, "lattice_dyson_g_wk_adjoint: bad argument");
-  if (norb < 1 || norb > 3) return fail(ctx, TPRF_ERR_UNSUPPORTED, "lattice_dyson_g_wk_adjoint: norb must be 1..3");
+  if (norb < 1 || norb > 4) return fail(ctx, TPRF_ERR_UNSUPPORTED, "lattice_dyson_g_wk_adjoint: norb must be 1..4");
   const int nw = 2 * n_iw;
   // frequency chunks: enough (k, chunk) threads to fill the chip
   const long ktot = nk * nbatch;
@@ -1437,7 +1437,8 @@ int tprf_solve_rpa_PH_adjoint(tprf_ctx *ctx, long npts, int norb, const tprf_c12
   if (!ctx) return TPRF_ERR_ARG;
   if (npts < 1 || !chi0 || !U || !chi || (chibar && (!chi0bar || !Ubar)))
     return fail(ctx, TPRF_ERR_ARG, "solve_rpa_PH_adjoint: bad argument");
-  if (norb < 1 || norb > 2) return fail(ctx, TPRF_ERR_UNSUPPORTED, "solve_rpa_PH_adjoint: norb must be 1..2");
+  if (norb < 1 || norb > 4) return fail(ctx, TPRF_ERR_UNSUPPORTED, "solve_rpa_PH_adjoint: norb must be 1..4");
+  if (npts > 2147483647L) return fail(ctx, TPRF_ERR_UNSUPPORTED, "solve_rpa_PH_adjoint: too many points");
   return check_launch(ctx,
                       tprf::launch_rpa_point(ctx->stream, npts, norb, (const c2 *)chi0, (const c2 *)U,
                                              (const c2 *)chibar, (c2 *)chi, (c2 *)chi0bar, (c2 *)Ubar,
