@@ -213,8 +213,11 @@ int or_lattice_dyson_g_wk(double beta, int n_iw, double mu, int nk, int n, const
  * ntarget, stride = ntarget, dist = 1).                                     */
 int or_fourier_lattice(long nouter, const int dims[3], long ntarget, const cplx *in, cplx *out, int direction) {
   if (nouter < 1 || ntarget < 1 || (direction != -1 && direction != 1)) return OR_ERR_ARG;
-  for (int d = 0; d < 3; ++d)
-    if (dims[d] < 1 || dims[d] > 128) return OR_ERR_ARG;
+  int maxlen = 1;
+  for (int d = 0; d < 3; ++d) {
+    if (dims[d] < 1 || dims[d] > 65536) return OR_ERR_ARG;
+    if (dims[d] > maxlen) maxlen = dims[d];
+  }
   const long nk = (long)dims[0] * dims[1] * dims[2];
   const long slab = nk * ntarget;
   fft_plan plans[3];
@@ -226,7 +229,7 @@ int or_fourier_lattice(long nouter, const int dims[3], long ntarget, const cplx 
   const long stride[3] = {(long)dims[1] * dims[2] * ntarget, (long)dims[2] * ntarget, ntarget};
 #pragma omp parallel
   {
-    cplx *bi = (cplx *)malloc(sizeof(cplx) * 128), *bo = (cplx *)malloc(sizeof(cplx) * 128);
+    cplx *bi = (cplx *)malloc(sizeof(cplx) * (size_t)maxlen), *bo = (cplx *)malloc(sizeof(cplx) * (size_t)maxlen);
 #pragma omp for schedule(static)
     for (long w = 0; w < nouter; ++w) {
       cplx *x = out + w * slab;

@@ -366,8 +366,8 @@ def test_unsupported_sizes_are_refused(T):
     """Every size outside the kernels' limits returns TPRF_ERR_UNSUPPORTED (-9)
     from the host checks, before any launch: orbitals above the limits of
     Dyson (6), RPA / W / Lindhard / bubble and the RPA and Dyson adjoints (4),
-    an omega -> tau mesh with no pruned kernel and ntau beyond one workgroup's
-    LDS, and lattice FFT lengths with a prime factor above 13."""
+    and Matsubara-axis meshes with no pruned kernel and ntau beyond one in-place
+    column in LDS."""
     import torch
     from tprf_amd import _lib
     c = _lib.ctx()
@@ -384,9 +384,11 @@ def test_unsupported_sizes_are_refused(T):
         "rpa adjoint n=5": lambda: lib.tprf_solve_rpa_PH_adjoint(c, 1, 5, p, p, None, p, None, None),
         "dyson adjoint n=5": lambda: lib.tprf_lattice_dyson_g_wk_adjoint(c, 4, 1, 5, 1, p, p, p,
                                                                           _lib.dptr(x.real.contiguous())),
-        # n_max 1025: M = 4096 > 1024, and ntau = 6151 needs 2 x 6151 x 16 bytes of LDS
-        "w2t n_max=1025": lambda: lib.tprf_fourier_wr_to_tr(c, 0, 100.0, 1025, -1, 1, p, p, p),
-        "lattice dim 17^2": lambda: lib.tprf_fourier_wk_to_wr(c, 1, _lib.dims3((289, 1, 1)), 1, p, p),
+        # n_max 1700: no pruned kernel (M = 4096 > 1024), and ntau = 10201 needs
+        # 10201 x 16 bytes > 152 KiB for even one in-place column
+        "w2t n_max=1700": lambda: lib.tprf_fourier_wr_to_tr(c, 0, 100.0, 1700, -1, 1, p, p, p),
+        "t2w ntau=10201": lambda: lib.tprf_fourier_tr_to_wr(c, 0, 100.0, 10201, 100, 1, p, p),
+        "chi0_wr ntau=10201": lambda: lib.tprf_chi0_wr_from_grt_PH(c, 100.0, 10201, d, 1, 16, p, p),
     }
     for name, f in cases.items():
         st = f()

@@ -117,8 +117,12 @@ def test_dyson_g_wr(L, oracle, dims, n, beta, n_iw):
                                           ((12, 12, 12), 1), ((48, 48, 1), 4), ((64, 64, 1), 1), ((11, 13, 1), 2),
                                           ((1, 1, 1), 5), ((8, 1, 8), 81), ((12, 12, 12), 3), ((6, 6, 6), 16),
                                           ((24, 24, 1), 5), ((64, 64, 1), 4), ((8, 8, 4), 2), ((6, 6, 1), 81),
-                                          ((16, 16, 16), 81), ((8, 8, 8), 7)])
+                                          ((16, 16, 16), 81), ((8, 8, 8), 7), ((17, 1, 1), 2), ((19, 34, 1), 1),
+                                          ((289, 1, 1), 1), ((3, 23, 5), 2), ((1021, 1, 1), 1)])
 def test_lattice_fft(L, oracle, dims, ntarget):
+    """Against the oracle for every plan: the 16^3 single-pass kernels, the
+    listed fixed shapes and the per-axis transform, whose lengths with a prime
+    factor above 13 (17, 19, 23, 289 = 17^2, 1021) run run-time radix passes."""
     from tprf_amd import _lib
     import torch
     rng = np.random.default_rng(sum(dims) + ntarget)
@@ -240,6 +244,77 @@ def test_wr_to_tr_pruned_and_generic(L, oracle, stat, n_max, nt):
         finally:
             _lib.call("tprf_ctx_set_option", c, 3, -1)
         assert relerr(host(got.data), ref) <= TOL, f"variant={variant}"
+
+
+def test_single_column_meshes_fermion_n_iw_1025(L, oracle):
+    """ntau = 6151 (n_iw = 1025, the mesh of the reference's k <-> r round trip,
+    g_wk_to_from_g_wr.cpp:36-85, with its 4 x 4 lattice and Sigma = 1/(i w)) is
+    beyond both the pruned kernel (M = 2048 > 1024) and two LDS buffers of
+    128 KiB: omega -> tau and tau -> omega run one column per workgroup with the
+    FFT in place.  Both match the oracle, and the round trip returns G(i w)."""
+    from tprf_amd.gf import Gf, MeshImFreq, MeshImTime, MeshProduct, MeshCycLat
+    beta, n_iw = 100.0, 1025
+    mesh = MeshImFreq(beta, "Fermion", n_iw)
+    w = mesh.values()
+    k = 2 * np.pi * np.arange(4) / 4
+    ek = (-2 * (np.cos(k)[:, None] + np.cos(k)[None, :])).reshape(16)
+    g_wk = 1 / (w[:, None] - ek[None] - 1 / w[:, None])
+    g_wr = (np.fft.fft2(g_wk.reshape(-1, 4, 4), axes=(1, 2)) / 16).reshape(-1, 16, 1, 1)
+    ref_t = oracle.fourier_w_to_t(g_wr, beta)
+    assert ref_t.shape[0] == 6151
+    g = Gf(MeshProduct(mesh, MeshCycLat((4, 4, 1))), data=gpu(g_wr))
+    got_t = L.fourier_wr_to_tr(g)
+    assert relerr(host(got_t.data), ref_t) <= TOL
+    gt = Gf(MeshProduct(MeshImTime(beta, "Fermion", 6151), MeshCycLat((4, 4, 1))), data=gpu(ref_t))
+    got_w = L.fourier_tr_to_wr(gt, nw=n_iw)
+    ref_w = oracle.fourier_t_to_w(ref_t, beta, n_iw, statistic=oracle.FERMION)
+    assert relerr(host(got_w.data), ref_w) <= TOL
+    assert np.abs(host(got_w.data) - g_wr).max() <= 1e-9
+
+
+@pytest.mark.parametrize("n_max,nt", [(700, -1), (1600, -1), (64, 9001)])
+def test_single_column_meshes_boson(L, oracle, n_max, nt):
+    """Bosonic omega -> tau on meshes that need the single-column kernel (ntau 4200,
+    9601 and 9001; the last with few frequencies, where the pruned kernel's L / M
+    bound of 64 is exceeded), against the oracle."""
+    from tprf_amd import _lib
+    from tprf_amd.gf import Gf, MeshImFreq, MeshProduct, MeshCycLat
+    rng = np.random.default_rng(n_max)
+    beta = 30.0
+    mesh = MeshImFreq(beta, "Boson", n_max)
+    w = mesh.values()
+    eps = rng.uniform(0.2, 2.0, size=(3, 2))
+    data = (1.0 / (w[:, None, None] - eps[None]) - 1.0 / (w[:, None, None] + eps[None])).reshape(mesh.size, 3, 1, 2)
+    ref = oracle.fourier_w_to_t(data, beta, statistic=oracle.BOSON, ntau=None if nt < 0 else nt)
+    g = Gf(MeshProduct(mesh, MeshCycLat((3, 1, 1))), data=gpu(data))
+    c = _lib.ctx()
+    _lib.call("tprf_ctx_set_option", c, 1, 1)  # generic kernels, as for every mesh without a pruned form
+    try:
+        got = L.fourier_wr_to_tr(g, nt=nt)
+    finally:
+        _lib.call("tprf_ctx_set_option", c, 1, 0)
+    assert relerr(host(got.data), ref) <= TOL
+
+
+def test_single_column_fused_bubble(L, oracle):
+    """The generic fused product + tau -> nu kernel at ntau = 4801 (one column per
+    workgroup, in-place FFT) against the oracle and the stored-product chain."""
+    from tprf_amd import _lib
+    rng = np.random.default_rng(31)
+    dims, n, beta, n_iw, nb = (2, 2, 1), 2, 40.0, 800, 24
+    gw = g0_wr(oracle, rng, dims, n, beta, n_iw)
+    gt = oracle.fourier_w_to_t(gw, beta)
+    assert gt.shape[0] == 4801
+    ref = oracle.chi0_wr_from_grt_PH(gt, dims, beta, nb)
+    c = _lib.ctx()
+    _lib.call("tprf_ctx_set_option", c, 1, 1)
+    try:
+        got = L.chi0_wr_from_grt_PH(gtr_gf(dims, beta, gt), nw=nb)
+    finally:
+        _lib.call("tprf_ctx_set_option", c, 1, 0)
+    assert relerr(host(got.data), ref) <= TOL
+    chain = oracle.fourier_t_to_w(oracle.chi0_tr_from_grt_PH(gt, dims), beta, nb)
+    assert relerr(host(got.data), chain) <= TOL
 
 
 def test_wr_to_tr_errors(L):

@@ -31,7 +31,8 @@ def test_fft1d_vs_numpy(oracle, n):
     assert relerr(oracle.fft1d(x, +1), np.fft.ifft(x) * n) <= 1e-14
 
 
-@pytest.mark.parametrize("dims", [(4, 3, 2), (16, 16, 1), (5, 1, 7), (1, 1, 1)])
+@pytest.mark.parametrize("dims", [(4, 3, 2), (16, 16, 1), (5, 1, 7), (1, 1, 1), (289, 1, 1), (3, 23, 5),
+                                  (1021, 1, 1)])
 def test_lattice_fft_sign_and_scale(oracle, dims):
     """k->r = FFTW_FORWARD / N_k, r->k = FFTW_BACKWARD (fourier_lattice.cpp:39-58)."""
     rng = np.random.default_rng(sum(dims))

@@ -116,7 +116,11 @@ bool factorize(int n, std::vector<int> &f) {
   while (n % 2 == 0) { f.push_back(2); n /= 2; }
   for (int p : {3, 5, 7, 11, 13})
     while (n % p == 0) { f.push_back(p); n /= p; }
-  return n == 1 && (int)f.size() <= tprf::kMaxPass;
+  // larger primes: run-time radix passes
+  for (int p = 17; (long)p * p <= n; p += 2)
+    while (n % p == 0) { f.push_back(p); n /= p; }
+  if (n > 1) f.push_back(n);
+  return (int)f.size() <= tprf::kMaxPass;
 }
 
 template <typename T>
@@ -134,7 +138,7 @@ int get_fft(tprf_ctx *ctx, int n, const tprf::FftDev **out) {
   if (it != ctx->fft.end()) { *out = &it->second.dev; return TPRF_OK; }
   std::vector<int> f;
   if (n < 1 || !factorize(n, f))
-    return fail(ctx, TPRF_ERR_UNSUPPORTED, "FFT length " + std::to_string(n) + " has a prime factor above 13");
+    return fail(ctx, TPRF_ERR_UNSUPPORTED, "FFT length " + std::to_string(n) + " has too many factors");
   std::vector<c2> tw(n);
   for (int k = 0; k < n; ++k) {
     long double a = 2.0L * kPiL * (long double)k / (long double)n;
@@ -785,8 +789,7 @@ int wr_to_tr(tprf_ctx *ctx, int statistic, double beta, int n_max, int ntau, lon
                         "w2t pruned kernel");
   }
   if (sym) return fail(ctx, TPRF_ERR_UNSUPPORTED, "fourier_wr_to_tr_herm: no pruned kernel for this mesh");
-  const int ld = (L + 1) | 1;
-  if (2L * ld * (long)sizeof(c2) > tprf::kMatsLdsBytes)
+  if (!tprf::mats_supported(L))
     return fail(ctx, TPRF_ERR_UNSUPPORTED, "fourier_wr_to_tr: ntau too large for one workgroup");
   if ((st = get_fft(ctx, L, &p))) return st;
   if ((st = get_mats(ctx, statistic, beta, L, &mt))) return st;
@@ -824,8 +827,7 @@ int tprf_fourier_tr_to_wr(tprf_ctx *ctx, int statistic, double beta, int ntau, i
   if (nw < 1 || L < 2 * nw)
     return fail(ctx, TPRF_ERR_MESH_TOO_SHORT,
                 "Fourier: The time mesh mush be at least twice as long as the number of positive frequencies");
-  const int ld = (L + 1) | 1;
-  if (2L * ld * (long)sizeof(c2) > tprf::kMatsLdsBytes)
+  if (!tprf::mats_supported(L))
     return fail(ctx, TPRF_ERR_UNSUPPORTED, "fourier_tr_to_wr: ntau too large for one workgroup");
   const tprf::FftDev *p;
   const tprf::MatsDev *mt;
@@ -888,8 +890,7 @@ int chi0_wr_from_grt(tprf_ctx *ctx, double beta, int ntau, const int dims[3], in
     }
     return tuned_launch(ctx, ctx->bubble_herm_tuned, key, cnt, forced, launch_h, "bubble pair kernel (Hermitian)");
   }
-  const int ld = (L + 1) | 1;
-  if (2L * ld * (long)sizeof(c2) > tprf::kMatsLdsBytes)
+  if (!tprf::mats_supported(L))
     return fail(ctx, TPRF_ERR_UNSUPPORTED, "chi0_wr_from_grt_PH: ntau too large for one workgroup");
   const tprf::FftDev *p;
   const tprf::MatsDev *mt;

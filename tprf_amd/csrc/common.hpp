@@ -261,5 +261,14 @@ int lattice3d_default_variant(const int dims[3]);
 // largest LDS footprint (bytes) a Matsubara-axis kernel may request
 constexpr int kMatsLdsBytes = 128 * 1024;
 int mats_cols_per_block(int L);
+// Lengths whose two LDS buffers exceed kMatsLdsBytes run one column per
+// workgroup of kBigThreads, transformed in place in one buffer of at most
+// kMatsBigLdsBytes (L + 1 <= kBigMaxN).
+constexpr int kMatsBigLdsBytes = 152 * 1024;
+constexpr int kBigMaxN = kMatsBigLdsBytes / 16;
+constexpr int kBigThreads = 512;
+bool mats_single_column(int L);
+// whether the Matsubara-axis kernels take a mesh of length L on this device
+bool mats_supported(int L);
 
 }  // namespace tprf

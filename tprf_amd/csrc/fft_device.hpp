@@ -1,9 +1,11 @@
 // fft_device.hpp -- batched mixed-radix Stockham FFT on LDS (FP64 complex).
 //
 // A workgroup transforms `ncol` columns of length n held in LDS at
-// a[c * ld + j].  Each pass reads one buffer and writes the other (autosort,
-// no bit reversal), so two buffers of ncol * ld elements are needed; the
-// function returns the buffer holding the result.  S = -1 computes
+// a[c * ld + j].  Radices 2, 3, 4, 5, 7, 8, 11 and 13 have unrolled
+// butterflies; any other prime factor runs as a run-time radix of O(R^2)
+// work.  Each pass reads one buffer and writes the other (autosort, no bit
+// reversal), so two buffers of ncol * ld elements are needed; the function
+// returns the buffer holding the result.  S = -1 computes
 // y_k = sum_j x_j exp(-2 pi i jk/n) (FFTW_FORWARD), S = +1 the conjugate
 // kernel (FFTW_BACKWARD); neither scales.
 #pragma once
@@ -181,6 +183,28 @@ __device__ __forceinline__ void stockham_bfly(const c2 *__restrict__ src, c2 *__
   for (int r = 0; r < R; ++r) dst[o + r * Ns] = v[r];
 }
 
+// Output q of the radix-R butterfly j of the pass with stride Ns for a radix
+// known only at run time (a prime above 13): the twiddle w^(k twstep r) and the
+// DFT root w^((r q mod R) n / R) taken together from the length-n table, R
+// terms read from src.  O(R^2) per butterfly.
+template <int S>
+__device__ __forceinline__ c2 runtime_radix_out(const c2 *src, int j, int q, int R, int m, int Ns, int twstep,
+                                                const c2 *__restrict__ tw, int n) {
+  const int e = (j % Ns) * twstep;
+  c2 acc = src[j];
+  int rq = 0, er = 0;
+  for (int r = 1; r < R; ++r) {
+    rq += q;
+    if (rq >= R) rq -= R;
+    er += e;
+    if (er >= n) er -= n;
+    int idx = er + rq * m;
+    if (idx >= n) idx -= n;
+    acc = cadd(acc, cmul(src[j + r * m], twiddle<S>(tw, idx)));
+  }
+  return acc;
+}
+
 template <int S>
 __device__ c2 *fft_lds(c2 *a, c2 *b, int ld, int ncol, const FftDev &p, int tid, int nthr) {
   int Ns = 1;
@@ -203,13 +227,103 @@ __device__ c2 *fft_lds(c2 *a, c2 *b, int ld, int ncol, const FftDev &p, int tid,
         case 8: stockham_bfly<8, S>(src, dst, j, m, Ns, twstep, p.tw, p.n); break;
         case 11: stockham_bfly<11, S>(src, dst, j, m, Ns, twstep, p.tw, p.n); break;
         case 13: stockham_bfly<13, S>(src, dst, j, m, Ns, twstep, p.tw, p.n); break;
-        default: break;
+        default: {
+          const int o = (j - j % Ns) * R + j % Ns;
+          for (int q = 0; q < R; ++q) dst[o + q * Ns] = runtime_radix_out<S>(src, j, q, R, m, Ns, twstep, p.tw, p.n);
+        }
       }
     }
     __syncthreads();
     c2 *t = a;
     a = b;
     b = t;
+    Ns *= R;
+  }
+  return a;
+}
+
+// One column of length n <= NMAX transformed in place in a single LDS buffer
+// by NT threads: each Stockham pass first takes every butterfly of the thread
+// into registers (at most ceil(NMAX / (R NT)) of them, indices fixed at compile
+// time), then after a barrier writes them to their autosorted positions.  For
+// lengths whose two buffers do not fit in LDS.
+template <int R, int S, int NT, int NMAX>
+__device__ __forceinline__ void inplace_pass(c2 *a, int n, int Ns, const c2 *__restrict__ tw, int tid) {
+  constexpr int KB = (NMAX + R * NT - 1) / (R * NT);
+  const int m = n / R;
+  const int twstep = n / (Ns * R);
+  c2 v[KB][R];
+#pragma unroll
+  for (int s = 0; s < KB; ++s) {
+    const int j = tid + s * NT;
+    if (j < m) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[s][r] = a[j + r * m];
+      const int k = j % Ns;
+      if (k) {
+        const int e = k * twstep;
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[s][r] = cmul(v[s][r], twiddle<S>(tw, e * r));
+      }
+      SmallDft<R, S>::run(v[s], tw, n);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < KB; ++s) {
+    const int j = tid + s * NT;
+    if (j < m) {
+      const int k = j % Ns;
+      const int o = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) a[o + r * Ns] = v[s][r];
+    }
+  }
+  __syncthreads();
+}
+
+// in-place pass of a run-time radix: each thread keeps its outputs
+// t = q m + j (at most ceil(NMAX / NT)) in registers until every read is done
+template <int S, int NT, int NMAX>
+__device__ __forceinline__ void inplace_pass_runtime(c2 *a, int n, int R, int Ns, const c2 *__restrict__ tw,
+                                                     int tid) {
+  constexpr int KB = (NMAX + NT - 1) / NT;
+  const int m = n / R;
+  const int twstep = n / (Ns * R);
+  c2 v[KB];
+#pragma unroll
+  for (int s = 0; s < KB; ++s) {
+    const int t = tid + s * NT;
+    if (t < n) v[s] = runtime_radix_out<S>(a, t % m, t / m, R, m, Ns, twstep, tw, n);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < KB; ++s) {
+    const int t = tid + s * NT;
+    if (t < n) {
+      const int j = t % m, q = t / m;
+      a[(j - j % Ns) * R + j % Ns + q * Ns] = v[s];
+    }
+  }
+  __syncthreads();
+}
+
+template <int S, int NT, int NMAX>
+__device__ c2 *fft_lds_inplace(c2 *a, const FftDev &p, int tid) {
+  int Ns = 1;
+  for (int ps = 0; ps < p.npass; ++ps) {
+    const int R = p.radix[ps];
+    switch (R) {
+      case 2: inplace_pass<2, S, NT, NMAX>(a, p.n, Ns, p.tw, tid); break;
+      case 3: inplace_pass<3, S, NT, NMAX>(a, p.n, Ns, p.tw, tid); break;
+      case 4: inplace_pass<4, S, NT, NMAX>(a, p.n, Ns, p.tw, tid); break;
+      case 5: inplace_pass<5, S, NT, NMAX>(a, p.n, Ns, p.tw, tid); break;
+      case 7: inplace_pass<7, S, NT, NMAX>(a, p.n, Ns, p.tw, tid); break;
+      case 8: inplace_pass<8, S, NT, NMAX>(a, p.n, Ns, p.tw, tid); break;
+      case 11: inplace_pass<11, S, NT, NMAX>(a, p.n, Ns, p.tw, tid); break;
+      case 13: inplace_pass<13, S, NT, NMAX>(a, p.n, Ns, p.tw, tid); break;
+      default: inplace_pass_runtime<S, NT, NMAX>(a, p.n, R, Ns, p.tw, tid); break;
+    }
     Ns *= R;
   }
   return a;

@@ -171,15 +171,25 @@ int mats_cols_per_block(int L) {
   return C < 1 ? 1 : C;
 }
 
+// set once the device has granted kMatsBigLdsBytes to the single-column kernels
+static bool g_mats_big_ok = false;
+
+bool mats_single_column(int L) { return 2L * ((L + 1) | 1) * (long)sizeof(c2) > kMatsLdsBytes; }
+
+bool mats_supported(int L) {
+  return !mats_single_column(L) || (g_mats_big_ok && ((L + 1) | 1) * (long)sizeof(c2) <= kMatsBigLdsBytes);
+}
+
 // a3: Matsubara -> imaginary time for C columns (fourier_matsubara.cpp:218-308).
-template <int STAT>
-__global__ void __launch_bounds__(256) k_w2t(double beta, int n_max, long ncol, int C, const c2 *__restrict__ in,
-                                             c2 *__restrict__ out, const c2 *__restrict__ moments, TailFitDev fit,
-                                             MatsDev mt, FftDev p, double *status) {
+// BIG: one column per workgroup of kBigThreads, transformed in place.
+template <int STAT, bool BIG>
+__global__ void __launch_bounds__(BIG ? kBigThreads : 256)
+    k_w2t(double beta, int n_max, long ncol, int C, const c2 *__restrict__ in, c2 *__restrict__ out,
+          const c2 *__restrict__ moments, TailFitDev fit, MatsDev mt, FftDev p, double *status) {
   extern __shared__ c2 sm[];
   const int L = mt.L;
   const int ld = (L + 1) | 1;
-  c2 *a = sm, *b = sm + C * ld;
+  c2 *a = sm, *b = BIG ? nullptr : sm + C * ld;
   __shared__ c2 s_coef[16][kMaxOrder];
   __shared__ c2 s_a[16][3];
   __shared__ c2 s_m1[16];
@@ -237,7 +247,11 @@ __global__ void __launch_bounds__(256) k_w2t(double beta, int n_max, long ncol, 
   }
   __syncthreads();
   // 3. forward FFT of length L
-  c2 *y = fft_lds<-1>(a, b, ld, cn, p, threadIdx.x, blockDim.x);
+  c2 *y;
+  if constexpr (BIG)
+    y = fft_lds_inplace<-1, kBigThreads, kBigMaxN>(a, p, threadIdx.x);
+  else
+    y = fft_lds<-1>(a, b, ld, cn, p, threadIdx.x, blockDim.x);
   // 4. G(tau_j) = y_j [exp(-i pi j/L)] + sum a_i psi_i(tau_j); G(tau_L) = -/+ (G(tau_0) + m1)
   const double pm = (STAT == 0) ? -1.0 : 1.0;
   for (int t = threadIdx.x; t < (L + 1) * cn; t += blockDim.x) {
@@ -259,16 +273,26 @@ __global__ void __launch_bounds__(256) k_w2t(double beta, int n_max, long ncol, 
 
 hipError_t launch_w2t(hipStream_t s, int statistic, double beta, int n_max, long ncol, const c2 *in, c2 *out,
                       const c2 *moments, const TailFitDev &fit, const MatsDev &mt, const FftDev &p, Status st) {
-  const int C = mats_cols_per_block(mt.L);
   const int ld = (mt.L + 1) | 1;
+  if (mats_single_column(mt.L)) {
+    const size_t lds = (size_t)ld * sizeof(c2);
+    if (statistic == 0)
+      hipLaunchKernelGGL((k_w2t<0, true>), dim3(ncol), dim3(kBigThreads), lds, s, beta, n_max, ncol, 1, in, out,
+                         moments, fit, mt, p, st.d);
+    else
+      hipLaunchKernelGGL((k_w2t<1, true>), dim3(ncol), dim3(kBigThreads), lds, s, beta, n_max, ncol, 1, in, out,
+                         moments, fit, mt, p, st.d);
+    return hipGetLastError();
+  }
+  const int C = mats_cols_per_block(mt.L);
   const long nb = (ncol + C - 1) / C;
   const size_t lds = 2 * (size_t)C * ld * sizeof(c2);
   if (statistic == 0)
-    hipLaunchKernelGGL(k_w2t<0>, dim3(nb), dim3(256), lds, s, beta, n_max, ncol, C, in, out, moments, fit, mt, p,
-                       st.d);
+    hipLaunchKernelGGL((k_w2t<0, false>), dim3(nb), dim3(256), lds, s, beta, n_max, ncol, C, in, out, moments, fit,
+                       mt, p, st.d);
   else
-    hipLaunchKernelGGL(k_w2t<1>, dim3(nb), dim3(256), lds, s, beta, n_max, ncol, C, in, out, moments, fit, mt, p,
-                       st.d);
+    hipLaunchKernelGGL((k_w2t<1, false>), dim3(nb), dim3(256), lds, s, beta, n_max, ncol, C, in, out, moments, fit,
+                       mt, p, st.d);
   return hipGetLastError();
 }
 
@@ -276,7 +300,7 @@ hipError_t launch_w2t(hipStream_t s, int statistic, double beta, int n_max, long
 // samples of each column in a[c*ld + j], compute the boundary moments
 // (fit_derivatives, fourier_matsubara.cpp:42-97), subtract the pole model,
 // transform and write the 2nw-1 (bosons) or 2nw (fermions) frequencies.
-template <int STAT>
+template <int STAT, bool BIG>
 __device__ void t2w_core(c2 *a, c2 *b, int ld, int cn, double beta, int nw, const MatsDev &mt, const FftDev &p,
                          const Vinv8 &vinv, c2 (*s_a)[3], c2 *dst, long dst_row, long dst_col_stride) {
   const int L = mt.L;
@@ -304,7 +328,11 @@ __device__ void t2w_core(c2 *a, c2 *b, int ld, int cn, double beta, int nw, cons
     a[c * ld + j] = cscale(v, fact);
   }
   __syncthreads();
-  c2 *y = fft_lds<1>(a, b, ld, cn, p, threadIdx.x, blockDim.x);
+  c2 *y;
+  if constexpr (BIG)
+    y = fft_lds_inplace<1, kBigThreads, kBigMaxN>(a, p, threadIdx.x);
+  else
+    y = fft_lds<1>(a, b, ld, cn, p, threadIdx.x, blockDim.x);
   const int nout = (STAT == 0) ? 2 * nw : 2 * nw - 1;
   const int first = (STAT == 0) ? -nw : -(nw - 1);
   for (int t = threadIdx.x; t < nout * cn; t += blockDim.x) {
@@ -316,13 +344,14 @@ __device__ void t2w_core(c2 *a, c2 *b, int ld, int cn, double beta, int nw, cons
   }
 }
 
-template <int STAT>
-__global__ void __launch_bounds__(256) k_t2w(double beta, int nw, long ncol, int C, const c2 *__restrict__ in,
-                                             c2 *__restrict__ out, MatsDev mt, FftDev p, Vinv8 vinv) {
+template <int STAT, bool BIG>
+__global__ void __launch_bounds__(BIG ? kBigThreads : 256)
+    k_t2w(double beta, int nw, long ncol, int C, const c2 *__restrict__ in, c2 *__restrict__ out, MatsDev mt,
+          FftDev p, Vinv8 vinv) {
   extern __shared__ c2 sm[];
   const int L = mt.L;
   const int ld = (L + 1) | 1;
-  c2 *a = sm, *b = sm + C * ld;
+  c2 *a = sm, *b = BIG ? nullptr : sm + C * ld;
   __shared__ c2 s_a[16][3];
   const long tile = xcd_remap(blockIdx.x, gridDim.x);
   const long c0 = tile * C;
@@ -332,19 +361,29 @@ __global__ void __launch_bounds__(256) k_t2w(double beta, int nw, long ncol, int
     a[c * ld + j] = in[(long)j * ncol + c0 + c];
   }
   __syncthreads();
-  t2w_core<STAT>(a, b, ld, cn, beta, nw, mt, p, vinv, s_a, out + c0, ncol, 1);
+  t2w_core<STAT, BIG>(a, b, ld, cn, beta, nw, mt, p, vinv, s_a, out + c0, ncol, 1);
 }
 
 hipError_t launch_t2w(hipStream_t s, int statistic, double beta, int nw, long ncol, const c2 *in, c2 *out,
                       const MatsDev &mt, const FftDev &p, const Vinv8 &vinv) {
-  const int C = mats_cols_per_block(mt.L);
   const int ld = (mt.L + 1) | 1;
+  if (mats_single_column(mt.L)) {
+    const size_t lds = (size_t)ld * sizeof(c2);
+    if (statistic == 0)
+      hipLaunchKernelGGL((k_t2w<0, true>), dim3(ncol), dim3(kBigThreads), lds, s, beta, nw, ncol, 1, in, out, mt, p,
+                         vinv);
+    else
+      hipLaunchKernelGGL((k_t2w<1, true>), dim3(ncol), dim3(kBigThreads), lds, s, beta, nw, ncol, 1, in, out, mt, p,
+                         vinv);
+    return hipGetLastError();
+  }
+  const int C = mats_cols_per_block(mt.L);
   const long nb = (ncol + C - 1) / C;
   const size_t lds = 2 * (size_t)C * ld * sizeof(c2);
   if (statistic == 0)
-    hipLaunchKernelGGL(k_t2w<0>, dim3(nb), dim3(256), lds, s, beta, nw, ncol, C, in, out, mt, p, vinv);
+    hipLaunchKernelGGL((k_t2w<0, false>), dim3(nb), dim3(256), lds, s, beta, nw, ncol, C, in, out, mt, p, vinv);
   else
-    hipLaunchKernelGGL(k_t2w<1>, dim3(nb), dim3(256), lds, s, beta, nw, ncol, C, in, out, mt, p, vinv);
+    hipLaunchKernelGGL((k_t2w<1, false>), dim3(nb), dim3(256), lds, s, beta, nw, ncol, C, in, out, mt, p, vinv);
   return hipGetLastError();
 }
 
@@ -355,13 +394,14 @@ hipError_t launch_t2w(hipStream_t s, int statistic, double beta, int nw, long nc
 // fused product + bosonic tau->nu: one workgroup per (r, C consecutive abcd)
 // herm: G_ab(tau, s)^* = G_ba(tau, -s) and only the columns of s <= -s are
 // read (the others as conj(G_ba(tau, -s))).
-__global__ void __launch_bounds__(256) k_chi0_wr(double beta, int nw, int d0, int d1, int d2, int n, int C,
-                                                 const c2 *__restrict__ gtr, c2 *__restrict__ out, MatsDev mt,
-                                                 FftDev p, Vinv8 vinv, int herm) {
+template <bool BIG>
+__global__ void __launch_bounds__(BIG ? kBigThreads : 256)
+    k_chi0_wr(double beta, int nw, int d0, int d1, int d2, int n, int C, const c2 *__restrict__ gtr,
+              c2 *__restrict__ out, MatsDev mt, FftDev p, Vinv8 vinv, int herm) {
   extern __shared__ c2 sm[];
   const int L = mt.L;
   const int ld = (L + 1) | 1;
-  c2 *a = sm, *b = sm + C * ld;
+  c2 *a = sm, *b = BIG ? nullptr : sm + C * ld;
   __shared__ c2 s_a[16][3];
   const long nr = (long)d0 * d1 * d2;
   const int n2 = n * n, n4 = n2 * n2;
@@ -385,20 +425,25 @@ __global__ void __launch_bounds__(256) k_chi0_wr(double beta, int nw, int d0, in
     a[c * ld + j] = cmul(gp, gm);
   }
   __syncthreads();
-  t2w_core<1>(a, b, ld, cn, beta, nw, mt, p, vinv, s_a, out + r * n4 + q0, nr * n4, 1);
+  t2w_core<1, BIG>(a, b, ld, cn, beta, nw, mt, p, vinv, s_a, out + r * n4 + q0, nr * n4, 1);
 }
 
 hipError_t launch_chi0_wr(hipStream_t s, double beta, int nw, const int dims[3], int n, const c2 *gtr, c2 *out,
                           const MatsDev &mt, const FftDev &p, const Vinv8 &vinv, bool herm) {
   const int n4 = n * n * n * n;
-  int C = mats_cols_per_block(mt.L);
-  if (C > n4) C = n4;
   const int ld = (mt.L + 1) | 1;
   const long nr = (long)dims[0] * dims[1] * dims[2];
+  if (mats_single_column(mt.L)) {
+    hipLaunchKernelGGL(k_chi0_wr<true>, dim3(nr * n4), dim3(kBigThreads), (size_t)ld * sizeof(c2), s, beta, nw,
+                       dims[0], dims[1], dims[2], n, 1, gtr, out, mt, p, vinv, herm ? 1 : 0);
+    return hipGetLastError();
+  }
+  int C = mats_cols_per_block(mt.L);
+  if (C > n4) C = n4;
   const long nb = nr * ((n4 + C - 1) / C);
   const size_t lds = 2 * (size_t)C * ld * sizeof(c2);
-  hipLaunchKernelGGL(k_chi0_wr, dim3(nb), dim3(256), lds, s, beta, nw, dims[0], dims[1], dims[2], n, C, gtr, out, mt,
-                     p, vinv, herm ? 1 : 0);
+  hipLaunchKernelGGL(k_chi0_wr<false>, dim3(nb), dim3(256), lds, s, beta, nw, dims[0], dims[1], dims[2], n, C, gtr,
+                     out, mt, p, vinv, herm ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -781,14 +826,29 @@ hipError_t launch_rpa(hipStream_t s, long npts, int n, const c2 *chi0, const c2 
 namespace tprf {
 
 // Kernels that request more than 64 KiB of dynamic LDS opt in once per process.
+// The single-column kernels ask for up to kMatsBigLdsBytes; they are used only
+// where the device grants that much to one workgroup (with room for their
+// static arrays).
 hipError_t set_kernel_attributes() {
   const int bytes = kMatsLdsBytes;
-  const void *fns[] = {(const void *)k_w2t<0>, (const void *)k_w2t<1>, (const void *)k_t2w<0>,
-                       (const void *)k_t2w<1>, (const void *)k_chi0_wr};
+  const void *fns[] = {(const void *)k_w2t<0, false>, (const void *)k_w2t<1, false>, (const void *)k_t2w<0, false>,
+                       (const void *)k_t2w<1, false>, (const void *)k_chi0_wr<false>};
   for (const void *f : fns) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (e != hipSuccess) return e;
   }
+  int dev = 0, lds_max = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  if (e != hipSuccess) return e;
+  if (lds_max < kMatsBigLdsBytes + 8 * 1024) return hipSuccess;
+  const void *big[] = {(const void *)k_w2t<0, true>, (const void *)k_w2t<1, true>, (const void *)k_t2w<0, true>,
+                       (const void *)k_t2w<1, true>, (const void *)k_chi0_wr<true>};
+  for (const void *f : big) {
+    e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kMatsBigLdsBytes);
+    if (e != hipSuccess) return e;
+  }
+  g_mats_big_ok = true;
   return hipSuccess;
 }
 
