@@ -102,7 +102,7 @@ int tprf_ctx_tail_status_reset(tprf_ctx *ctx);
 /* ---- a1: Dyson --------------------------------------------------------- */
 /* lattice_dyson_g_wk(mu, e_k, sigma_w | sigma_wk), c++/triqs_tprf/lattice/gf.hpp:83,123,
  * gf.cpp:91-132; Sigma = 0 (sigma_kind NONE) is lattice_dyson_g0_wk, gf.cpp:41-64.
- * e_k[nk][n][n], sigma per sigma_kind, out g_wk[2 n_iw][nk][n][n].  n <= 6.  */
+ * e_k[nk][n][n], sigma per sigma_kind, out g_wk[2 n_iw][nk][n][n].  n <= 8.  */
 int tprf_lattice_dyson_g_wk(tprf_ctx *ctx, double beta, int n_iw, double mu, long nk, int norb,
                             const tprf_c128 *e_k, int sigma_kind, const tprf_c128 *sigma, tprf_c128 *g_wk);
 
@@ -112,7 +112,7 @@ int tprf_lattice_dyson_g_wk(tprf_ctx *ctx, double beta, int n_iw, double mu, lon
  * e_k[nk][n][n] -> g_wr[2 n_iw][nk][n][n] (nk = dims[0] dims[1] dims[2]).
  * For a 16 x 16 x 16 lattice and n <= 3 the inversion is done inside the
  * transform's load phase and G(iw, k) is never stored; otherwise g_wr first
- * holds G(iw, k) and is transformed in place.  n <= 6.                      */
+ * holds G(iw, k) and is transformed in place.  n <= 8.                      */
 int tprf_lattice_dyson_g_wr(tprf_ctx *ctx, double beta, int n_iw, double mu, const int dims[3], int norb,
                             const tprf_c128 *e_k, tprf_c128 *g_wr);
 /* The same for a Hermitian e_k (the caller's promise): on the fused 16^3 path
@@ -204,7 +204,7 @@ int tprf_chi_w0r_from_chi_tr(tprf_ctx *ctx, double beta, int ntau, long nr, int 
  * matrix that group_indices_view(X, idx_group<0,1>, idx_group<3,2>) makes of
  * a row-major tensor (rpa.cpp:40,54-55; for U_abcd = U_badc it equals the
  * (a,b),(d,c) grouping of rpa.hpp's formula).  chi0[npts][n^4], U[n^4]
- * (device), out chi[npts][n^4].  n <= 4.                                   */
+ * (device), out chi[npts][n^4].  n <= 8.                                   */
 int tprf_solve_rpa_PH(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *chi0, const tprf_c128 *U,
                       tprf_c128 *chi);
 /* solve_rpa_PH on a bosonic Matsubara mesh of 2 nb - 1 frequencies
@@ -229,7 +229,7 @@ int tprf_solve_rpa_PH_herm(tprf_ctx *ctx, int nb, long nk, int norb, const tprf_
  * two overloads give the same matrices).  PI[nw][nk][n^4]; V[nk][n^4]
  * (dynamic = 0) or [nw][nk][n^4] (dynamic = 1); out W[nw][nk][n^4].  The
  * frequency mesh may be Matsubara or real (the real-frequency overloads,
- * dynamical_screened_interaction.hpp:63,100, do the same per point).  n <= 4;
+ * dynamical_screened_interaction.hpp:63,100, do the same per point).  n <= 8;
  * a singular denominator sets the status word as for the RPA.               */
 int tprf_dynamical_screened_interaction_W(tprf_ctx *ctx, long nw, long nk, int norb, const tprf_c128 *PI,
                                           const tprf_c128 *V, int dynamic, tprf_c128 *W);
@@ -460,7 +460,7 @@ int tprf_fourier_r_to_q_adjoint(tprf_ctx *ctx, const int dims[3], const int q[3]
  * reverse-mode derivative: with chibar given (may be NULL for forward only),
  * chi0bar = A^-H chibar (1 + U chi)^H and Ubar = chi0^H A^-H chibar chi^H in
  * the PH matrix grouping, A = 1 - chi0 U.  chi0, U, chibar, chi, chi0bar,
- * Ubar [npts][n^4].  norb <= 2.                                              */
+ * Ubar [npts][n^4].  norb <= 4.                                              */
 int tprf_solve_rpa_PH_adjoint(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *chi0, const tprf_c128 *U,
                               const tprf_c128 *chibar, tprf_c128 *chi, tprf_c128 *chi0bar, tprf_c128 *Ubar);
 

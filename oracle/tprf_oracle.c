@@ -730,44 +730,56 @@ int or_chi_w0r_from_chi_tr(double beta, int ntau, long nr, int n, const cplx *ch
  * same nda call there).  For U_abcd = U_badc (every real Hermitian
  * interaction) it coincides with M[(a,b),(d,c)].                            */
 int or_solve_rpa_PH(long npts, int n, const cplx *chi0, const cplx *U, cplx *chi) {
-  if (npts < 1 || n < 1 || n > 4) return OR_ERR_ARG;
+  if (npts < 1 || n < 1 || n > 8) return OR_ERR_ARG;
   const int N = n * n;
   const long n4 = (long)N * N;
-  cplx MU[256];
+  cplx *MU = (cplx *)malloc(sizeof(cplx) * (size_t)n4);
+  if (!MU) return OR_ERR_NOMEM;
   for (int a = 0; a < n; ++a)
     for (int b = 0; b < n; ++b)
       for (int c = 0; c < n; ++c)
         for (int d = 0; d < n; ++d) MU[(a * n + b) * N + (c * n + d)] = U[((a * n + b) * n + c) * n + d];
   int status = OR_OK;
-#pragma omp parallel for schedule(static)
-  for (long p = 0; p < npts; ++p) {
-    cplx X[256], A[256], Ainv[256];
-    const cplx *x = chi0 + p * n4;
-    for (int a = 0; a < n; ++a)
-      for (int b = 0; b < n; ++b)
-        for (int c = 0; c < n; ++c)
-          for (int d = 0; d < n; ++d) X[(a * n + b) * N + (c * n + d)] = x[((a * n + b) * n + c) * n + d];
-    for (int i = 0; i < N; ++i)
-      for (int j = 0; j < N; ++j) {
-        cplx s = (i == j) ? 1.0 : 0.0;
-        for (int l = 0; l < N; ++l) s -= X[i * N + l] * MU[l * N + j];
-        A[i * N + j] = s;
-      }
-    if (gj_inverse(N, A, Ainv) != OR_OK) {
+#pragma omp parallel
+  {
+    cplx *X = (cplx *)malloc(sizeof(cplx) * 3 * (size_t)n4);
+    cplx *A = X + n4, *Ainv = A + n4;
+#pragma omp for schedule(static)
+    for (long p = 0; p < npts; ++p) {
+      if (!X) {
 #pragma omp atomic write
-      status = OR_ERR_SINGULAR;
+        status = OR_ERR_NOMEM;
+        continue;
+      }
+      const cplx *x = chi0 + p * n4;
+      for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b)
+          for (int c = 0; c < n; ++c)
+            for (int d = 0; d < n; ++d) X[(a * n + b) * N + (c * n + d)] = x[((a * n + b) * n + c) * n + d];
+      for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) {
+          cplx s = (i == j) ? 1.0 : 0.0;
+          for (int l = 0; l < N; ++l) s -= X[i * N + l] * MU[l * N + j];
+          A[i * N + j] = s;
+        }
+      if (gj_inverse(N, A, Ainv) != OR_OK) {
+#pragma omp atomic write
+        status = OR_ERR_SINGULAR;
+      }
+      cplx *o = chi + p * n4;
+      for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b)
+          for (int c = 0; c < n; ++c)
+            for (int d = 0; d < n; ++d) {
+              int i = a * n + b, j = c * n + d;
+              cplx s = 0;
+              for (int l = 0; l < N; ++l) s += Ainv[i * N + l] * X[l * N + j];
+              o[((a * n + b) * n + c) * n + d] = s;
+            }
     }
-    cplx *o = chi + p * n4;
-    for (int a = 0; a < n; ++a)
-      for (int b = 0; b < n; ++b)
-        for (int c = 0; c < n; ++c)
-          for (int d = 0; d < n; ++d) {
-            int i = a * n + b, j = c * n + d;
-            cplx s = 0;
-            for (int l = 0; l < N; ++l) s += Ainv[i * N + l] * X[l * N + j];
-            o[((a * n + b) * n + c) * n + d] = s;
-          }
+    free(X);
   }
+  free(MU);
   return status;
 }
 
@@ -783,34 +795,44 @@ int or_solve_rpa_PH(long npts, int n, const cplx *chi0, const cplx *U, cplx *chi
  * idx_group<0,1>, idx_group<3,2>) on a row-major tensor (see a7 above).
  * V is [nk][n^4] (dynamic = 0, point p uses V[p % nk]) or [npts][n^4].     */
 int or_screened_interaction_W(long npts, long nk, int n, int dynamic, const cplx *PI, const cplx *V, cplx *W) {
-  if (npts < 1 || nk < 1 || n < 1 || n > 4) return OR_ERR_ARG;
+  if (npts < 1 || nk < 1 || n < 1 || n > 8) return OR_ERR_ARG;
   const int N = n * n;
   const long n4 = (long)N * N;
   int status = OR_OK;
-#pragma omp parallel for schedule(static)
-  for (long p = 0; p < npts; ++p) {
-    const cplx *pi = PI + p * n4;
-    const cplx *v = V + (dynamic ? p : p % nk) * n4;
-    cplx *w = W + p * n4;
-    cplx D[256], Dinv[256];
-    /* both paths: denominator 1 - M_PI M_V, then M_W = M_V inverse(denominator)
-     * (the static path writes the same sums as explicit index loops) */
-    for (int i = 0; i < N; ++i)
-      for (int j = 0; j < N; ++j) {
-        cplx s = (i == j) ? 1.0 : 0.0;
-        for (int l = 0; l < N; ++l) s -= pi[i * N + l] * v[l * N + j];
-        D[i * N + j] = s;
-      }
-    if (gj_inverse(N, D, Dinv) != OR_OK) {
+#pragma omp parallel
+  {
+    cplx *D = (cplx *)malloc(sizeof(cplx) * 2 * (size_t)n4);
+    cplx *Dinv = D + n4;
+#pragma omp for schedule(static)
+    for (long p = 0; p < npts; ++p) {
+      if (!D) {
 #pragma omp atomic write
-      status = OR_ERR_SINGULAR;
-    }
-    for (int i = 0; i < N; ++i)
-      for (int j = 0; j < N; ++j) {
-        cplx s = 0;
-        for (int l = 0; l < N; ++l) s += v[i * N + l] * Dinv[l * N + j];
-        w[i * N + j] = s;
+        status = OR_ERR_NOMEM;
+        continue;
       }
+      const cplx *pi = PI + p * n4;
+      const cplx *v = V + (dynamic ? p : p % nk) * n4;
+      cplx *w = W + p * n4;
+      /* both paths: denominator 1 - M_PI M_V, then M_W = M_V inverse(denominator)
+       * (the static path writes the same sums as explicit index loops) */
+      for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) {
+          cplx s = (i == j) ? 1.0 : 0.0;
+          for (int l = 0; l < N; ++l) s -= pi[i * N + l] * v[l * N + j];
+          D[i * N + j] = s;
+        }
+      if (gj_inverse(N, D, Dinv) != OR_OK) {
+#pragma omp atomic write
+        status = OR_ERR_SINGULAR;
+      }
+      for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) {
+          cplx s = 0;
+          for (int l = 0; l < N; ++l) s += v[i * N + l] * Dinv[l * N + j];
+          w[i * N + j] = s;
+        }
+    }
+    free(D);
   }
   return status;
 }

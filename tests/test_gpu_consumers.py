@@ -63,7 +63,7 @@ def _meshes(beta, n_f, n_b, dims, ntau=None):
 
 
 # ---------------------------------------------------------------- pointwise pieces
-@pytest.mark.parametrize("n", [1, 2, 3, 4])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 6, 8])
 def test_vertex_contractions(T, oracle, n):
     """gw_dynamic_sigma, fock_sigma (r), hartree_sigma (r), the Eliashberg
     gamma-F products: the reference's index patterns against the oracle."""
@@ -92,7 +92,7 @@ def test_vertex_contractions(T, oracle, n):
     assert relerr(got, oracle.eliashberg_gamma_f(v, g[0])) <= 1e-14
 
 
-@pytest.mark.parametrize("n,dims", [(1, (8, 1, 1)), (2, (4, 3, 1)), (3, (3, 3, 2))])
+@pytest.mark.parametrize("n,dims", [(1, (8, 1, 1)), (2, (4, 3, 1)), (3, (3, 3, 2)), (5, (2, 2, 1))])
 def test_static_sigma_k_direct_sums(T, oracle, n, dims):
     """hartree_sigma(v_k, g_wk) and fock_sigma(v_k, g_wk): density per k, then the
     direct q sums, against the oracle."""
@@ -131,10 +131,11 @@ def test_split_into_dynamic_and_constant(T, oracle):
     assert relerr(host(cst.data), V) <= 1e-10
 
 
-@pytest.mark.parametrize("n", [1, 2, 3])
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 8])
 def test_g_delta_g_summation_product_and_phi(T, oracle, n):
     """eliashberg_g_delta_g_product (G on a larger mesh than Delta), the
-    summation product and construct_phi_wk against the oracle."""
+    summation product and construct_phi_wk (n >= 5: one workgroup per point)
+    against the oracle."""
     from tprf_amd import lattice as L
     from tprf_amd.gf import MeshBrZone, MeshImFreq, MeshProduct
     rng = np.random.default_rng(20 + n)

@@ -44,7 +44,7 @@ def kmesh_gf(e, dims):
 
 
 # ---------------------------------------------------------------- a1 Dyson
-@pytest.mark.parametrize("n", [1, 2, 3, 4])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 6, 7, 8])
 @pytest.mark.parametrize("sigma", ["none", "w", "wk"])
 def test_dyson(L, oracle, n, sigma):
     from tprf_amd.gf import Gf, MeshImFreq, MeshProduct, MeshBrZone
@@ -432,7 +432,7 @@ def test_chi0_w0r_trapz(L, oracle, gtr_case):
 
 
 # ---------------------------------------------------------------- a7 RPA
-@pytest.mark.parametrize("n,nb,nk", [(1, 5, 7), (2, 4, 13), (3, 6, 29), (3, 1, 40), (4, 2, 5)])
+@pytest.mark.parametrize("n,nb,nk", [(1, 5, 7), (2, 4, 13), (3, 6, 29), (3, 1, 40), (4, 2, 5), (5, 3, 7), (8, 2, 3)])
 def test_rpa_herm(L, oracle, n, nb, nk):
     """tprf_solve_rpa_PH_herm: chi0 with chi0_abcd(-nu) = conj(chi0_dcba(nu))
     (M(-nu) = S M(nu)^H S for M[(a,b),(c,d)] = X_abcd, S the swap of a and b)
@@ -446,7 +446,8 @@ def test_rpa_herm(L, oracle, n, nb, nk):
     def from_M(M):  # M[(a,b),(d,c)] -> X[a,b,c,d]
         return M.reshape(M.shape[:-2] + (n, n, n, n)).swapaxes(-1, -2)
 
-    Mp = rnd(rng, nb, nk, n2, n2) * 0.15
+    sc = 1.0 if n <= 4 else (4.0 / n) ** 2  # keeps 1 - M_chi0 M_U well conditioned
+    Mp = rnd(rng, nb, nk, n2, n2) * 0.15 * sc
     Mp[0] = 0.5 * (Mp[0] + np.conj(np.swapaxes(Mp[0], -1, -2)))   # nu = 0 is Hermitian
     Mfull = np.concatenate([np.conj(np.swapaxes(Mp[:0:-1], -1, -2)), Mp], axis=0)
     chi0 = from_M(Mfull)
@@ -464,14 +465,15 @@ def test_rpa_herm(L, oracle, n, nb, nk):
 
 
 @pytest.mark.parametrize("sparse", [False, True])
-@pytest.mark.parametrize("n", [1, 2, 3, 4])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 8])
 def test_rpa(L, oracle, n, sparse):
     """Dense U, and U with about two thirds of its entries exactly zero (the
-    kernel skips zero entries of the interaction matrix)."""
+    kernel skips zero entries of the interaction matrix).  n >= 5 runs the
+    workgroup-per-point solver with the matrices in LDS."""
     from tprf_amd.gf import Gf, MeshImFreq, MeshProduct, MeshBrZone
     rng = np.random.default_rng(40 + n)
     npts_w, nk = 5, 13
-    chi0 = rnd(rng, npts_w, nk, n, n, n, n) * 0.2
+    chi0 = rnd(rng, npts_w, nk, n, n, n, n) * 0.2 * (1.0 if n <= 4 else (4.0 / n) ** 2)
     U = rnd(rng, n, n, n, n) * 0.5
     if sparse:
         U = U * (rng.random(U.shape) < 0.35)
@@ -636,6 +638,24 @@ def test_chain_cubic_small(L, oracle):
     n4 = wl.norb ** 4
     got = host(ch.chi_wk).reshape(-1, n4)[good]
     assert relerr(got, ref["chi_wk"].reshape(-1, n4)[good]) <= TOL
+
+
+@pytest.mark.parametrize("norb,L_side", [(5, 4), (6, 3), (8, 2)])
+def test_chain_wide_orbitals(L, oracle, norb, L_side):
+    """The default chain for 5, 6 and 8 orbitals (a d shell, a spinful
+    three-orbital model, and the largest supported case): Dyson by rows per
+    lane for n = 7, 8, the generic fused bubble with n^4 columns per r, and RPA
+    by the workgroup-per-point solver; every stage against the oracle on
+    identical inputs, and chi_RPA end to end (U = 0.6, J = 0.1 keeps
+    1 - chi0 U well conditioned)."""
+    from tprf_amd import workloads
+    wl = workloads.cubic_3orb(seed=3, L=L_side, beta=5.0, n_iw=32, nb=8, U=0.6, J=0.1, norb=norb)
+    ch, err = run_chain(wl)
+    ref = oracle_chain(oracle, wl)
+    for name in ("g_wk", "g_wr", "g_tr", "chi0_wr", "chi0_wk"):
+        assert relerr(host(getattr(ch, name)), ref[name]) <= TOL, name
+    assert rpa_condition(ref["chi0_wk"], wl.U).max() < 100
+    assert relerr(host(ch.chi_wk), ref["chi_wk"]) <= TOL
 
 
 def test_chain_fused_equals_per_function(L):
@@ -838,7 +858,8 @@ def test_screened_W_reference_kat(L, oracle):
         L.dynamical_screened_interaction_W(PI_wk, Gf(MeshBrZone((4, 1, 1)), data=gpu(V[:4])))
 
 
-@pytest.mark.parametrize("n,nw,nk", [(1, 7, 9), (2, 5, 11), (3, 4, 13), (3, 1, 64), (4, 3, 5)])
+@pytest.mark.parametrize("n,nw,nk", [(1, 7, 9), (2, 5, 11), (3, 4, 13), (3, 1, 64), (4, 3, 5), (5, 3, 7), (6, 2, 5),
+                                     (8, 2, 3)])
 def test_screened_W_vs_oracle(L, oracle, n, nw, nk):
     """Random dense PI and V (static per k, and dynamic per point) against the
     oracle's restatement of both overloads; odd point counts leave partly
@@ -846,7 +867,8 @@ def test_screened_W_vs_oracle(L, oracle, n, nw, nk):
     import torch
     from tprf_amd import _lib
     rng = np.random.default_rng(100 * n + nk)
-    PI = rnd(rng, nw, nk, n, n, n, n) * 0.25
+    sc = 0.25 if n <= 4 else 1.0 / n ** 2
+    PI = rnd(rng, nw, nk, n, n, n, n) * sc
     V = rnd(rng, nk, n, n, n, n) * 0.25
     Vw = rnd(rng, nw, nk, n, n, n, n) * 0.25
     c = _lib.ctx()

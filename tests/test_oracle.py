@@ -355,15 +355,15 @@ def test_rpa_single_band(oracle):
     assert relerr(oracle.solve_rpa_PH(chi0, U), chi0 / (1 - 1.7 * chi0)) <= 1e-14
 
 
-def test_rpa_vs_numpy_solve(oracle):
+@pytest.mark.parametrize("n", [3, 5, 8])
+def test_rpa_vs_numpy_solve(oracle, n):
     """rpa.cpp:57 on the matrices M_X[(a,b),(c,d)] = X_abcd (the memory-order
     grouping that group_indices_view gives, pinned by the screened-interaction
     test below), with a random U that lacks the U_abcd = U_badc symmetry, so
-    the (a,b),(d,c) reading would fail."""
-    rng = np.random.default_rng(2)
-    n = 3
-    chi0 = rnd(rng, 20, n, n, n, n) * 0.1
-    U = rnd(rng, n, n, n, n) * 0.3
+    the (a,b),(d,c) reading would fail.  n up to 8 (64 x 64 matrices)."""
+    rng = np.random.default_rng(2 + n)
+    chi0 = rnd(rng, 20, n, n, n, n) * (0.3 / n ** 2)
+    U = rnd(rng, n, n, n, n) * 0.9
     X = chi0.reshape(-1, n * n, n * n)
     MU = U.reshape(n * n, n * n)
     Y = np.linalg.solve(np.eye(n * n)[None] - X @ MU, X)
@@ -466,17 +466,17 @@ def test_screened_W_reference_kat(oracle):
     assert relerr(oracle.screened_interaction_W(PI, Vw, dynamic=True), ref) <= 1e-14
 
 
-def test_screened_W_dyson_and_single_band(oracle):
-    """n = 1: W = V / (1 - PI V).  n = 3, random dense V and PI: W solves the
+@pytest.mark.parametrize("n", [3, 6])
+def test_screened_W_dyson_and_single_band(oracle, n):
+    """n = 1: W = V / (1 - PI V).  n = 3 and 6, random dense V and PI: W solves the
     Dyson equation W_abcd = V_abcd + sum V_abef PI_efgh W_ghcd (in the matrix
     grouping of the reference's code); a transposed PI fails it."""
-    rng = np.random.default_rng(11)
+    rng = np.random.default_rng(11 + n)
     V1 = rnd(rng, 6, 1, 1, 1, 1)
     P1 = rnd(rng, 4, 6, 1, 1, 1, 1) * 0.2
     assert relerr(oracle.screened_interaction_W(P1, V1), V1[None] / (1 - P1 * V1[None])) <= 1e-14
-    n = 3
-    V = rnd(rng, 5, n, n, n, n) * 0.3
-    PI = rnd(rng, 4, 5, n, n, n, n) * 0.3
+    V = rnd(rng, 5, n, n, n, n) * (0.9 / n)
+    PI = rnd(rng, 4, 5, n, n, n, n) * (0.9 / n)
     W = oracle.screened_interaction_W(PI, V)
     rhs = V[None] + np.einsum("wkabef,wkefgh,wkghcd->wkabcd", np.broadcast_to(V[None], W.shape), PI, W)
     assert relerr(W, rhs) <= 1e-12

@@ -23,6 +23,7 @@ using tprf::c2;
 
 namespace tprf {
 hipError_t set_kernel_attributes();
+hipError_t consumers_set_attributes();
 }
 
 struct FftEntry {
@@ -528,6 +529,7 @@ int tprf_ctx_create(tprf_ctx **out, int device) {
   if (e == hipSuccess) e = hipMalloc((void **)&ctx->d_status, 4 * sizeof(double));
   if (e == hipSuccess) e = hipMemset(ctx->d_status, 0, 4 * sizeof(double));
   if (e == hipSuccess) e = tprf::set_kernel_attributes();
+  if (e == hipSuccess) e = tprf::consumers_set_attributes();
   if (e == hipSuccess) e = tprf::noalloc_scratch_address(device, &ctx->scratch);
   if (e != hipSuccess) {
     delete ctx;
@@ -634,7 +636,7 @@ int tprf_lattice_dyson_g_wk(tprf_ctx *ctx, double beta, int n_iw, double mu, lon
   if (!(beta > 0) || n_iw < 1 || nk < 1 || !e_k || !g_wk) return fail(ctx, TPRF_ERR_ARG, "lattice_dyson_g_wk: bad argument");
   if (sigma_kind < 0 || sigma_kind > 2 || (sigma_kind && !sigma))
     return fail(ctx, TPRF_ERR_ARG, "lattice_dyson_g_wk: bad sigma");
-  if (norb < 1 || norb > 6) return fail(ctx, TPRF_ERR_UNSUPPORTED, "lattice_dyson_g_wk: norb must be 1..6");
+  if (norb < 1 || norb > 8) return fail(ctx, TPRF_ERR_UNSUPPORTED, "lattice_dyson_g_wk: norb must be 1..8");
   return check_launch(ctx,
                       tprf::launch_dyson(ctx->stream, beta, n_iw, mu, nk, norb, (const c2 *)e_k, sigma_kind,
                                          (const c2 *)sigma, (c2 *)g_wk, tprf::Status{ctx->d_status}),
@@ -649,7 +651,7 @@ int dyson_g_wr(tprf_ctx *ctx, double beta, int n_iw, double mu, const int dims[3
   if (!ctx) return TPRF_ERR_ARG;
   if (!(beta > 0) || n_iw < 1 || !dims_ok(dims) || !e_k || !g_wr)
     return fail(ctx, TPRF_ERR_ARG, "lattice_dyson_g_wr: bad argument");
-  if (norb < 1 || norb > 6) return fail(ctx, TPRF_ERR_UNSUPPORTED, "lattice_dyson_g_wr: norb must be 1..6");
+  if (norb < 1 || norb > 8) return fail(ctx, TPRF_ERR_UNSUPPORTED, "lattice_dyson_g_wr: norb must be 1..8");
   const long nk = (long)dims[0] * dims[1] * dims[2];
   if (!ctx->generic_kernels && dims[0] == 16 && dims[1] == 16 && dims[2] == 16 && norb <= 3) {
     auto launch = [&](int v) {
@@ -854,8 +856,9 @@ namespace {
 int chi0_wr_from_grt(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb, int nw, const tprf_c128 *g_tr,
                      tprf_c128 *chi0_wr, bool herm, bool nonneg = false) {
   if (!ctx) return TPRF_ERR_ARG;
-  if (!(beta > 0) || ntau < 10 || !dims_ok(dims) || norb < 1 || norb > 4 || nw < 1 || !g_tr || !chi0_wr)
+  if (!(beta > 0) || ntau < 10 || !dims_ok(dims) || norb < 1 || nw < 1 || !g_tr || !chi0_wr)
     return fail(ctx, TPRF_ERR_ARG, "chi0_wr_from_grt_PH: bad argument");
+  if (norb > 8) return fail(ctx, TPRF_ERR_UNSUPPORTED, "chi0_wr_from_grt_PH: norb must be 1..8");
   const int L = ntau - 1;
   if (L < 2 * nw)
     return fail(ctx, TPRF_ERR_MESH_TOO_SHORT,
@@ -935,7 +938,7 @@ int tprf_chi_w0r_from_chi_tr(tprf_ctx *ctx, double beta, int ntau, long nr, int 
 int tprf_solve_rpa_PH(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *chi0, const tprf_c128 *U, tprf_c128 *chi) {
   if (!ctx) return TPRF_ERR_ARG;
   if (npts < 1 || !chi0 || !U || !chi) return fail(ctx, TPRF_ERR_ARG, "solve_rpa_PH: bad argument");
-  if (norb < 1 || norb > 4) return fail(ctx, TPRF_ERR_UNSUPPORTED, "solve_rpa_PH: norb must be 1..4");
+  if (norb < 1 || norb > tprf::kWideMaxOrb) return fail(ctx, TPRF_ERR_UNSUPPORTED, "solve_rpa_PH: norb must be 1..8");
   return check_launch(ctx,
                       tprf::launch_rpa(ctx->stream, npts, norb, (const c2 *)chi0, (const c2 *)U, (c2 *)chi,
                                        tprf::Status{ctx->d_status}),
@@ -985,7 +988,7 @@ int tprf_solve_rpa_PH_herm(tprf_ctx *ctx, int nb, long nk, int norb, const tprf_
                            tprf_c128 *chi) {
   if (!ctx) return TPRF_ERR_ARG;
   if (nb < 1 || nk < 1 || !chi0 || !U || !chi) return fail(ctx, TPRF_ERR_ARG, "solve_rpa_PH_herm: bad argument");
-  if (norb < 1 || norb > 4) return fail(ctx, TPRF_ERR_UNSUPPORTED, "solve_rpa_PH_herm: norb must be 1..4");
+  if (norb < 1 || norb > tprf::kWideMaxOrb) return fail(ctx, TPRF_ERR_UNSUPPORTED, "solve_rpa_PH_herm: norb must be 1..8");
   return check_launch(ctx,
                       tprf::launch_rpa_mirror(ctx->stream, nb, nk, norb, (const c2 *)chi0, (const c2 *)U, (c2 *)chi,
                                               tprf::Status{ctx->d_status}),
@@ -997,7 +1000,7 @@ int tprf_dynamical_screened_interaction_W(tprf_ctx *ctx, long nw, long nk, int n
   if (!ctx) return TPRF_ERR_ARG;
   if (nw < 1 || nk < 1 || !PI || !V || !W || (dynamic != 0 && dynamic != 1))
     return fail(ctx, TPRF_ERR_ARG, "dynamical_screened_interaction_W: bad argument");
-  if (norb < 1 || norb > 4) return fail(ctx, TPRF_ERR_UNSUPPORTED, "dynamical_screened_interaction_W: norb must be 1..4");
+  if (norb < 1 || norb > tprf::kWideMaxOrb) return fail(ctx, TPRF_ERR_UNSUPPORTED, "dynamical_screened_interaction_W: norb must be 1..8");
   return check_launch(ctx,
                       tprf::launch_screened_W(ctx->stream, nw * nk, nk, norb, dynamic, (const c2 *)PI, (const c2 *)V,
                                               (c2 *)W, tprf::Status{ctx->d_status}),
@@ -1008,7 +1011,7 @@ int tprf_dynamical_screened_interaction_W(tprf_ctx *ctx, long nw, long nk, int n
 
 namespace {
 int consumer_n(tprf_ctx *ctx, int norb, const char *what) {
-  if (norb < 1 || norb > 4) return fail(ctx, TPRF_ERR_UNSUPPORTED, std::string(what) + ": norb must be 1..4");
+  if (norb < 1 || norb > 8) return fail(ctx, TPRF_ERR_UNSUPPORTED, std::string(what) + ": norb must be 1..8");
   return TPRF_OK;
 }
 }  // namespace
@@ -1176,7 +1179,7 @@ int tprf_lattice_dyson_g_wk_batched(tprf_ctx *ctx, double beta, int n_iw, const 
   if (!ctx) return TPRF_ERR_ARG;
   if (!(beta > 0) || n_iw < 1 || nk < 1 || nbatch < 1 || !mu || !e_k || !g_wk)
     return fail(ctx, TPRF_ERR_ARG, "lattice_dyson_g_wk_batched: bad argument");
-  if (norb < 1 || norb > 6) return fail(ctx, TPRF_ERR_UNSUPPORTED, "lattice_dyson_g_wk_batched: norb must be 1..6");
+  if (norb < 1 || norb > 8) return fail(ctx, TPRF_ERR_UNSUPPORTED, "lattice_dyson_g_wk_batched: norb must be 1..8");
   return check_launch(ctx,
                       tprf::launch_dyson(ctx->stream, beta, n_iw, 0.0, nk * nbatch, norb, (const c2 *)e_k, 0, nullptr,
                                          (c2 *)g_wk, tprf::Status{ctx->d_status}, mu, nk),

@@ -166,6 +166,11 @@ hipError_t launch_chi0_tr(hipStream_t s, int ntau, const int dims[3], int n, con
 hipError_t launch_chi0_w0r(hipStream_t s, double beta, int ntau, const int dims[3], int n, const c2 *gtr, c2 *out);
 hipError_t launch_chi_w0r(hipStream_t s, double beta, int ntau, long nr, int n, const c2 *chi_tr, c2 *out);
 hipError_t launch_rpa(hipStream_t s, long npts, int n, const c2 *chi0, const c2 *U, c2 *chi, Status st);
+// RPA, its mirrored form and W for 5 <= n <= kWideMaxOrb: one workgroup per
+// point, [1 - Lm Rm | Lm] in LDS (32 n^4 bytes)
+constexpr int kWideMaxOrb = 8;
+hipError_t launch_solve_wide(hipStream_t s, int n, long npts, const c2 *Lm, int lmode, const c2 *Rm, int rmode,
+                             c2 *out, Status st, long base, long nk, int mirror);
 // RPA at the nonnegative bosonic frequencies of chi0[2 nb - 1][nk][n^4], each
 // result also written at -nu as its conjugate transpose in the PH grouping
 // screened interaction W = V inverse(1 - PI V) per point (V per k when

@@ -211,6 +211,33 @@ __global__ void __launch_bounds__(256) k_construct_phi(long npts, const c2 *__re
   }
 }
 
+// The same for 5 <= n <= 8: one workgroup per point, T = M_U M_chi in LDS
+// (16 n^4 bytes), then Phi = T M_U.
+__global__ void __launch_bounds__(256) k_construct_phi_wide(int n, long npts, const c2 *__restrict__ chi,
+                                                            const c2 *__restrict__ U, c2 *__restrict__ phi) {
+  extern __shared__ c2 T[];
+  const int n2 = n * n;
+  const long n4 = (long)n2 * n2;
+  const long p = blockIdx.x;
+  if (p >= npts) return;
+  auto at = [n, n2](int r, int s) { return r * n2 + (s % n) * n + s / n; };
+  const c2 *x = chi + p * n4;
+  for (int t = threadIdx.x; t < n2 * n2; t += blockDim.x) {
+    const int i = t / n2, l = t - i * n2;
+    c2 v = c2{0, 0};
+    for (int k = 0; k < n2; ++k) v = cfma(v, U[at(i, k)], x[at(k, l)]);
+    T[t] = v;
+  }
+  __syncthreads();
+  c2 *o = phi + p * n4;
+  for (int t = threadIdx.x; t < n2 * n2; t += blockDim.x) {
+    const int i = t / n2, j = t - i * n2;
+    c2 v = c2{0, 0};
+    for (int l = 0; l < n2; ++l) v = cfma(v, T[i * n2 + l], U[at(l, j)]);
+    o[at(i, j)] = v;
+  }
+}
+
 // add_dynamic_and_static (lattice_utility.cpp:86-101): x[w][col] += s[col]
 __global__ void __launch_bounds__(256) k_add_static(long nw, long ncol, c2 *__restrict__ x, const c2 *__restrict__ s) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -247,6 +274,10 @@ hipError_t launch_vertex_contract(hipStream_t s, int mode, int n, long npts, con
     case 2: TPRF_VCN(2); break;
     case 3: TPRF_VCN(3); break;
     case 4: TPRF_VCN(4); break;
+    case 5: TPRF_VCN(5); break;
+    case 6: TPRF_VCN(6); break;
+    case 7: TPRF_VCN(7); break;
+    case 8: TPRF_VCN(8); break;
     default: return hipErrorInvalidValue;
   }
 #undef TPRF_VCN
@@ -267,6 +298,10 @@ hipError_t launch_static_sigma_k(hipStream_t s, bool fock, int n, const int dims
     case 2: TPRF_SK(2); break;
     case 3: TPRF_SK(3); break;
     case 4: TPRF_SK(4); break;
+    case 5: TPRF_SK(5); break;
+    case 6: TPRF_SK(6); break;
+    case 7: TPRF_SK(7); break;
+    case 8: TPRF_SK(8); break;
     default: return hipErrorInvalidValue;
   }
 #undef TPRF_SK
@@ -295,6 +330,10 @@ hipError_t launch_g_delta_g(hipStream_t s, int n, long nwd, int off, const int d
     case 2: TPRF_GDG(2); break;
     case 3: TPRF_GDG(3); break;
     case 4: TPRF_GDG(4); break;
+    case 5: TPRF_GDG(5); break;
+    case 6: TPRF_GDG(6); break;
+    case 7: TPRF_GDG(7); break;
+    case 8: TPRF_GDG(8); break;
     default: return hipErrorInvalidValue;
   }
 #undef TPRF_GDG
@@ -311,6 +350,10 @@ hipError_t launch_eliashberg_sum(hipStream_t s, int n, int nd, int ng, const int
     case 2: TPRF_ES(2); break;
     case 3: TPRF_ES(3); break;
     case 4: TPRF_ES(4); break;
+    case 5: TPRF_ES(5); break;
+    case 6: TPRF_ES(6); break;
+    case 7: TPRF_ES(7); break;
+    case 8: TPRF_ES(8); break;
     default: return hipErrorInvalidValue;
   }
 #undef TPRF_ES
@@ -325,10 +368,24 @@ hipError_t launch_construct_phi(hipStream_t s, int n, long npts, const c2 *chi, 
     case 2: TPRF_CP(2); break;
     case 3: TPRF_CP(3); break;
     case 4: TPRF_CP(4); break;
+    case 5:
+    case 6:
+    case 7:
+    case 8: {
+      hipLaunchKernelGGL(k_construct_phi_wide, dim3(npts), dim3(256), (size_t)16 * n * n * n * n, s, n, npts, chi, U,
+                         phi);
+      break;
+    }
     default: return hipErrorInvalidValue;
   }
 #undef TPRF_CP
   return hipGetLastError();
+}
+
+// 16 n^4 bytes of LDS for k_construct_phi_wide reach 64 KiB at n = 8
+hipError_t consumers_set_attributes() {
+  return hipFuncSetAttribute((const void *)k_construct_phi_wide, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             16 * 4096);
 }
 
 hipError_t launch_zero(hipStream_t s, long n, c2 *x) {

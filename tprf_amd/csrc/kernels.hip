@@ -89,6 +89,84 @@ __global__ void __launch_bounds__(256) k_dyson(double beta, int n_iw, double mu,
     for (int j = 0; j < N; ++j) o[i * N + j] = B[i][j];
 }
 
+// n = 7, 8: N lanes own a point (P = 64 / N points per wave), lane i holding
+// row i of [M | 1]; Gauss-Jordan with partial pivoting, the pivot row read from
+// its lane by shuffles, rows left in place (the row that pivoted column k
+// holds, after the sweep, pivot times row k of the inverse).
+template <int N>
+__global__ void __launch_bounds__(256) k_dyson_rows(double beta, int n_iw, double mu, const double *__restrict__ mu_arr,
+                                                    long nk_inst, long nk, const c2 *__restrict__ ek, int sigma_kind,
+                                                    const c2 *__restrict__ sigma, c2 *__restrict__ g, long npts,
+                                                    double *status) {
+  constexpr int P = 64 / N;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int gidx = lane / N, i = lane - gidx * N;
+  const long idx = ((long)blockIdx.x * 4 + wave) * P + gidx;
+  const bool valid = gidx < P && idx < npts;
+  c2 A[N], B[N];
+  if (valid) {
+    const long iw = idx / nk, k = idx - iw * nk;
+    const double m = mu_arr ? mu_arr[k / nk_inst] : mu;
+    const double wn = kPi * (double)(2 * (iw - n_iw) + 1) / beta;
+    const c2 *e = ek + k * N * N + i * N;
+    const c2 *sg = sigma_kind == 0 ? nullptr : sigma + (sigma_kind == 1 ? iw : idx) * N * N + i * N;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      c2 v = c2{-e[j].x, -e[j].y};
+      if (j == i) { v.x += m; v.y += wn; }
+      if (sg) v = csub(v, sg[j]);
+      A[j] = v;
+      B[j] = (j == i) ? c2{1.0, 0.0} : c2{0.0, 0.0};
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < N; ++j) { A[j] = c2{0.0, 0.0}; B[j] = c2{0.0, 0.0}; }
+  }
+  const int lane0 = lane - i;  // first lane of this point's group
+  bool used = false, singular = false;
+  int myk = 0;
+  c2 pv = c2{1.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const double key = (!used && valid) ? cabs2(A[k]) : -1.0;
+    double best = -2.0;
+    int best_i = 0;
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+      const double o = __shfl(key, lane0 + r, 64);
+      if (o > best) { best = o; best_i = r; }
+    }
+    if (valid && !(best > 0.0)) singular = true;
+    const int src = lane0 + best_i;
+    c2 pr[N], pb[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      pr[j] = c2{__shfl(A[j].x, src, 64), __shfl(A[j].y, src, 64)};
+      pb[j] = c2{__shfl(B[j].x, src, 64), __shfl(B[j].y, src, 64)};
+    }
+    if (i == best_i) {
+      used = true;
+      myk = k;
+      pv = A[k];
+    } else {
+      const c2 f = cdiv(A[k], pr[k]);
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        if (j > k) A[j] = cfms(A[j], f, pr[j]);
+        B[j] = cfms(B[j], f, pb[j]);
+      }
+      A[k] = c2{0.0, 0.0};
+    }
+  }
+  if (singular) status[2] = 1.0;
+  if (valid) {
+    const c2 inv = cdiv(c2{1.0, 0.0}, pv);
+    c2 *o = g + idx * N * N + myk * N;
+#pragma unroll
+    for (int j = 0; j < N; ++j) o[j] = cmul(B[j], inv);
+  }
+}
+
 hipError_t launch_dyson(hipStream_t s, double beta, int n_iw, double mu, long nk, int n, const c2 *ek, int sigma_kind,
                         const c2 *sigma, c2 *g, Status st, const double *mu_arr, long nk_inst) {
   const long npts = 2L * n_iw * nk;
@@ -108,6 +186,16 @@ hipError_t launch_dyson(hipStream_t s, double beta, int n_iw, double mu, long nk
     TPRF_DYSON_CASE(5)
     TPRF_DYSON_CASE(6)
 #undef TPRF_DYSON_CASE
+#define TPRF_DYSON_ROWS(N)                                                                                   \
+  case N: {                                                                                                  \
+    const long nbr = (npts + 4 * (64 / N) - 1) / (4 * (64 / N));                                             \
+    hipLaunchKernelGGL(k_dyson_rows<N>, dim3(nbr), dim3(256), 0, s, beta, n_iw, mu, mu_arr, nk_inst, nk, ek,  \
+                       sigma_kind, sigma, g, npts, st.d);                                                    \
+    break;                                                                                                   \
+  }
+    TPRF_DYSON_ROWS(7)
+    TPRF_DYSON_ROWS(8)
+#undef TPRF_DYSON_ROWS
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -761,6 +849,109 @@ __global__ void __launch_bounds__(256) k_screened_W(long npts, long nk, int dyna
   }
 }
 
+// ------------------------------------------------------------------------
+// RPA and W for 5 <= n <= 8 (M = n^2 = 25..64 rows): one workgroup per point,
+// the augmented matrix [A | B] (M x 2M, 32 M^2 bytes <= 128 KiB) in LDS,
+// Gauss-Jordan with partial pivoting and row swaps.  A = 1 - Lm Rm and B = Lm,
+// so that B ends as inverse(1 - Lm Rm) Lm: (Lm, Rm) = (chi0, U) gives M_chi
+// (rpa.cpp:57), (V, PI) gives M_W (the push-through form above).
+//   lmode 0: Lm per point at base + pt; 1: at pt % nk (static V)
+//   rmode 0: Rm per point at base + pt; 1: one matrix for every point (U)
+//   mirror: also write chi_dcba(-nu) = conj(chi_abcd(nu)) as k_rpa<N, true>
+__global__ void __launch_bounds__(256) k_solve_wide(int n, long npts, const c2 *__restrict__ Lm, int lmode,
+                                                    const c2 *__restrict__ Rm, int rmode, c2 *__restrict__ out,
+                                                    double *status, long base, long nk, int mirror) {
+  extern __shared__ c2 sm[];
+  __shared__ int s_p;
+  __shared__ c2 s_piv;
+  const int M = n * n, W2 = 2 * M;
+  c2 *T = sm;  // T[i W2 + j]: columns j < M of A, j >= M of B
+  const long pt = xcd_remap(blockIdx.x, gridDim.x);
+  if (pt >= npts) return;
+  const long MM = (long)M * M;
+  const c2 *lp = Lm + (lmode == 0 ? base + pt : pt % nk) * MM;
+  const c2 *rp = Rm + (rmode == 0 ? (base + pt) * MM : 0);
+  for (int t = threadIdx.x; t < M * M; t += blockDim.x) {
+    const int i = t / M, j = t - i * M;
+    T[i * W2 + M + j] = lp[t];
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < M * M; t += blockDim.x) {
+    const int i = t / M, j = t - i * M;
+    c2 acc = (i == j) ? c2{1.0, 0.0} : c2{0.0, 0.0};
+    const c2 *li = T + i * W2 + M;
+    for (int l = 0; l < M; ++l) acc = cfms(acc, li[l], rp[(long)l * M + j]);
+    T[i * W2 + j] = acc;
+  }
+  __syncthreads();
+  bool singular = false;
+  for (int k = 0; k < M; ++k) {
+    if (threadIdx.x < 64) {
+      // pivot: largest |A_ik|^2 over rows i >= k, lowest row on ties
+      const int i = k + threadIdx.x;
+      double v = (i < M) ? cabs2(T[i * W2 + k]) : -1.0;
+      int idx = (i < M) ? i : M;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(v, off, 64);
+        const int oi = __shfl_xor(idx, off, 64);
+        if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+      }
+      if (threadIdx.x == 0) {
+        s_p = idx;
+        s_piv = T[idx * W2 + k];
+        if (!(v > 0.0)) singular = true;
+      }
+    }
+    __syncthreads();
+    const int p = s_p;
+    const c2 inv = cdiv(c2{1.0, 0.0}, s_piv);
+    // swap rows k and p over the columns >= k, the new row k scaled by 1 / pivot
+    for (int j = k + threadIdx.x; j < W2; j += blockDim.x) {
+      const c2 a = T[p * W2 + j], b = T[k * W2 + j];
+      T[k * W2 + j] = cmul(a, inv);
+      if (p != k) T[p * W2 + j] = b;
+    }
+    __syncthreads();
+    // eliminate column k from every other row (columns > k; column k is not
+    // read again)
+    const int w = W2 - k - 1;
+    for (int t = threadIdx.x; t < M * w; t += blockDim.x) {
+      const int i = t / w;
+      if (i == k) continue;
+      const int j = k + 1 + (t - i * w);
+      T[i * W2 + j] = cfms(T[i * W2 + j], T[i * W2 + k], T[k * W2 + j]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && singular) status[2] = 1.0;
+  c2 *o = out + (base + pt) * MM;
+  for (int t = threadIdx.x; t < M * M; t += blockDim.x) {
+    const int i = t / M, j = t - i * M;
+    o[t] = T[i * W2 + M + j];
+  }
+  if (mirror) {
+    const long m = pt / nk;
+    if (m > 0) {
+      c2 *mp = out + (base + pt - 2 * m * nk) * MM;
+      for (int t = threadIdx.x; t < M * M; t += blockDim.x) {
+        const int i = t / M, j = t - i * M;
+        const int a = i / n, b = i - a * n, c = j / n, d = j - c * n;
+        mp[((d * n + c) * n + b) * n + a] = cconj(T[i * W2 + M + j]);
+      }
+    }
+  }
+}
+
+hipError_t launch_solve_wide(hipStream_t s, int n, long npts, const c2 *Lm, int lmode, const c2 *Rm, int rmode,
+                             c2 *out, Status st, long base, long nk, int mirror) {
+  if (n < 5 || n > kWideMaxOrb) return hipErrorInvalidValue;
+  const size_t lds = (size_t)32 * n * n * n * n;
+  hipLaunchKernelGGL(k_solve_wide, dim3(npts), dim3(256), lds, s, n, npts, Lm, lmode, Rm, rmode, out, st.d, base, nk,
+                     mirror);
+  return hipGetLastError();
+}
+
 hipError_t launch_screened_W(hipStream_t s, long npts, long nk, int n, int dynamic, const c2 *PI, const c2 *V, c2 *W,
                              Status st) {
   switch (n) {
@@ -776,7 +967,7 @@ hipError_t launch_screened_W(hipStream_t s, long npts, long nk, int n, int dynam
     TPRF_SW_CASE(3)
     TPRF_SW_CASE(4)
 #undef TPRF_SW_CASE
-    default: return hipErrorInvalidValue;
+    default: return launch_solve_wide(s, n, npts, V, dynamic ? 0 : 1, PI, 0, W, st, 0, nk, 0);
   }
   return hipGetLastError();
 }
@@ -797,7 +988,7 @@ hipError_t launch_rpa_mirror(hipStream_t s, long nb, long nk, int n, const c2 *c
     TPRF_RPA_MCASE(3)
     TPRF_RPA_MCASE(4)
 #undef TPRF_RPA_MCASE
-    default: return hipErrorInvalidValue;
+    default: return launch_solve_wide(s, n, npts, chi0, 0, U, 1, chi, st, base, nk, 1);
   }
   return hipGetLastError();
 }
@@ -816,7 +1007,7 @@ hipError_t launch_rpa(hipStream_t s, long npts, int n, const c2 *chi0, const c2 
     TPRF_RPA_CASE(3)
     TPRF_RPA_CASE(4)
 #undef TPRF_RPA_CASE
-    default: return hipErrorInvalidValue;
+    default: return launch_solve_wide(s, n, npts, chi0, 0, U, 1, chi, st, 0, 1, 0);
   }
   return hipGetLastError();
 }
@@ -835,6 +1026,11 @@ hipError_t set_kernel_attributes() {
                        (const void *)k_t2w<1, false>, (const void *)k_chi0_wr<false>};
   for (const void *f : fns) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e != hipSuccess) return e;
+  }
+  {
+    hipError_t e = hipFuncSetAttribute((const void *)k_solve_wide, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       32 * kWideMaxOrb * kWideMaxOrb * kWideMaxOrb * kWideMaxOrb);
     if (e != hipSuccess) return e;
   }
   int dev = 0, lds_max = 0;
