@@ -41,7 +41,6 @@ __device__ __forceinline__ void w2t_butterflies(c2 *v, int j, int Ns, const c2 *
 }
 
 constexpr int kW2tThreads = 256;
-constexpr int kFitSplit = 4;  // partial sums per fitted coefficient
 
 // NT threads per workgroup: C = NT / (M / 8) columns, i.e. C * 16 contiguous
 // bytes of every row written per workgroup.
@@ -55,21 +54,15 @@ constexpr int kFitSplit = 4;  // partial sums per fitted coefficient
 template <int STAT, int M, int NT, bool SYM = false>
 __global__ void __launch_bounds__(NT)
     __attribute__((amdgpu_waves_per_eu((STAT == 0 && (M == 256 || M == 512) && NT == 256) ? 3 : 2)))
-    k_w2t_pruned(double beta, int n_max, long ncol,
-                                                             const c2 *__restrict__ in, c2 *__restrict__ out,
-                                                             const c2 *__restrict__ moments, TailFitDev fit,
-                                                             MatsDev mt, W2tDev wd, double *status,
-                                                             SymCols sym = SymCols{nullptr, nullptr, 0}) {
+    k_w2t_pruned(double beta, int n_max, long ncol, const c2 *__restrict__ in, c2 *__restrict__ out,
+                 const c2 *__restrict__ moments, TailFitDev fit, MatsDev mt, W2tDev wd, double *status,
+                 SymCols sym = SymCols{nullptr, nullptr, 0}, int mom_in_out = 0) {
   constexpr int TPC = M / 8;  // threads per column
   constexpr int C = NT / TPC;  // columns per workgroup
   // C >= 3: the output records of a residue (3 M entries) fit in buf
   constexpr bool OTAB = C >= 3;
   using Plan = W2tPlan<M>;
   __shared__ c2 buf[M * C];
-  __shared__ c2 s_coef[C][kMaxOrder];
-  __shared__ c2 s_cpart[C][kMaxOrder][kFitSplit];
-  __shared__ double s_res[C][kMaxFit];
-  __shared__ double s_rpart[C][8];
   __shared__ c2 s_a[C][3];
   __shared__ c2 s_m1[C];
   __shared__ c2 s_tw[M];
@@ -87,10 +80,10 @@ __global__ void __launch_bounds__(NT)
   const int nmesh = (STAT == 0) ? 2 * n_max : 2 * n_max - 1;
   const int first = (STAT == 0) ? -n_max : -(n_max - 1);
 
-  // 1. this thread's eight inputs (the same FFT positions e = j + TPC r for every t),
-  //    staged once in registers and once in LDS for the tail fit.  All loads
-  //    (inputs and the twiddle table) are issued before the first LDS store
-  //    (outside the mesh a valid entry is loaded and discarded).
+  // 1. this thread's eight inputs (the same FFT positions e = j + TPC r for
+  //    every t), held in registers.  All loads (inputs and the twiddle table)
+  //    are issued before the first LDS store (outside the mesh a valid entry
+  //    is loaded and discarded).
   constexpr int NTW = (M + NT - 1) / NT;
   v2d twv[NTW], rv[8];
 #pragma unroll
@@ -111,95 +104,25 @@ __global__ void __launch_bounds__(NT)
     const int e = j + TPC * r;
     const int n = e < M / 2 ? e : e - M;
     raw[r] = (n >= first && n < first + nmesh) ? c2{rv[r].x, rv[r].y} : c2{0.0, 0.0};
-    buf[e * C + c] = raw[r];
   }
-  __syncthreads();
-  // data index i <-> FFT position ((first + i) mod M)
-  auto fitval = [&](int f, int cc) {
-    int e = first + fit.idx[f];  // in (-M/2, M/2)
-    if (e < 0) e += M;
-    return buf[e * C + cc];
-  };
-  // 2. tail moments (least-squares fit, or given).  The fit is spread over
-  //    (column, coefficient, quarter of the fitted points) threads and the
-  //    residual over (column, fitted point) threads, so no thread walks a long
-  //    chain of table loads.
-  if (moments == nullptr) {
-    const int nc = fit.ncoef, nf = fit.n_fit;
-    const int fq = (nf + kFitSplit - 1) / kFitSplit;
-    for (int t = tid; t < cn * nc * kFitSplit; t += NT) {
-      const int part = t % kFitSplit, cq = t / kFitSplit;
-      const int cc = cq / nc, q = cq - cc * nc;
-      const int f0 = part * fq, f1 = (f0 + fq) < nf ? (f0 + fq) : nf;
-      // batches of 8 terms with clamped (always valid) indices, so a batch's
-      // table loads are all issued before the first product waits on one;
-      // the same terms are summed in the same order
-      constexpr int kFq = (kMaxFit + kFitSplit - 1) / kFitSplit, kB = 4;
-      c2 s = c2{0.0, 0.0};
-#pragma unroll
-      for (int f8 = 0; f8 < kFq; f8 += kB) {
-        if (f0 + f8 >= f1) break;
-        c2 pv[kB], xv[kB];
-#pragma unroll
-        for (int ff = 0; ff < kB; ++ff) {
-          const int f = min(f0 + f8 + ff, nf - 1);
-          pv[ff] = fit.pinv[q * nf + f];
-          xv[ff] = fitval(f, cc);
-        }
-#pragma unroll
-        for (int ff = 0; ff < kB; ++ff)
-          if (f0 + f8 + ff < f1) s = cfma(s, pv[ff], xv[ff]);
-      }
-      s_cpart[cc][q][part] = s;
+  // 2. tail moments: given, or fitted by k_tail_moments and left in the
+  //    column's rows L - 2 .. L
+  if (tid < cn) {
+    c2 m1, m2, m3;
+    if (mom_in_out) {
+      // k_tail_moments left this column's moments in its rows L - 2 .. L,
+      // which only this workgroup writes, and only after this read
+      const long cc = SYM ? (long)sym.col[c0 + tid] : c0 + tid;
+      const c2 *mo = out + (long)(L - 2) * ncol + cc;
+      m1 = mo[0];
+      m2 = mo[ncol];
+      m3 = mo[2 * ncol];
+    } else {
+      m1 = moments[c0 + tid];
+      m2 = moments[ncol + c0 + tid];
+      m3 = moments[2 * ncol + c0 + tid];
     }
-    __syncthreads();
-    for (int t = tid; t < cn * nc; t += NT) {
-      const int cc = t / nc, q = t - cc * nc;
-      c2 s = s_cpart[cc][q][0];
-#pragma unroll
-      for (int part = 1; part < kFitSplit; ++part) s = cadd(s, s_cpart[cc][q][part]);
-      s_coef[cc][q] = s;
-    }
-    __syncthreads();
-    for (int t = tid; t < cn * nf; t += NT) {
-      const int cc = t / nf, f = t - cc * nf;
-      c2 v = fitval(f, cc);
-#pragma unroll
-      for (int q5 = 0; q5 < kMaxOrder; q5 += 5) {
-        c2 vd[5];
-#pragma unroll
-        for (int q = 0; q < 5; ++q) vd[q] = fit.vander[f * nc + min(q5 + q, nc - 1)];
-#pragma unroll
-        for (int q = 0; q < 5; ++q)
-          if (q5 + q < nc) v = cfms(v, vd[q], s_coef[cc][q5 + q]);
-      }
-      s_res[cc][f] = cabs2(v);
-    }
-    __syncthreads();
-    // residual sums in two levels (8 partial sums per column, then their
-    // total) rather than one chain of nf dependent LDS reads
-    for (int t = tid; t < cn * 8; t += NT) {
-      const int cc = t >> 3, part = t & 7;
-      double r = 0.0;
-#pragma unroll
-      for (int w = 0; w < kMaxFit / 8; ++w)
-        if (part + 8 * w < nf) r += s_res[cc][part + 8 * w];
-      s_rpart[cc][part] = r;
-    }
-    __syncthreads();
-    if (tid < cn) {
-      double r2 = 0.0;
-#pragma unroll
-      for (int part = 0; part < 8; ++part) r2 += s_rpart[tid][part];
-      const c2 m1 = cscale(s_coef[tid][1], fit.wpow[1]);
-      Poles<STAT>::coef(m1, cscale(s_coef[tid][2], fit.wpow[2]), cscale(s_coef[tid][3], fit.wpow[3]), s_a[tid]);
-      s_m1[tid] = m1;
-      atomic_max_nonneg(&status[0], sqrt(r2 / fit.n_fit));
-      atomic_max_nonneg(&status[1], sqrt(cabs2(s_coef[tid][0])));
-    }
-  } else if (tid < cn) {
-    const c2 m1 = moments[c0 + tid];
-    Poles<STAT>::coef(m1, moments[ncol + c0 + tid], moments[2 * ncol + c0 + tid], s_a[tid]);
+    Poles<STAT>::coef(m1, m2, m3, s_a[tid]);
     s_m1[tid] = m1;
   }
   __syncthreads();
@@ -330,6 +253,99 @@ __global__ void __launch_bounds__(NT)
   }
 }
 
+// The least-squares tail fit of every column as a kernel of its own: a
+// workgroup takes 64 columns (column colmap[i], or i), lane = column, and its
+// four waves a quarter of the fitted frequencies each (the table reads are
+// then wave-uniform); partial coefficients and residuals are summed through
+// LDS.  The moments m_q = w_max^q s_q, q = 1..3, go to mom[(q - 1) ncol + col],
+// the fit residual and |s_0| to the status words (one atomic per wave).  The
+// pruned transform then runs without its in-workgroup fit, whose chain of
+// dependent steps otherwise holds every wave of a workgroup.
+constexpr int kTmParts = 4;
+constexpr int kTmPer = kMaxFit / kTmParts;  // fitted frequencies per wave (at most)
+__global__ void __launch_bounds__(64 * kTmParts) k_tail_moments(long ntot, const int *__restrict__ colmap, long ncol,
+                                                                TailFitDev fit, const c2 *__restrict__ in,
+                                                                c2 *__restrict__ mom, double *status) {
+  __shared__ c2 s_part[kTmParts][kMaxOrder][64];
+  __shared__ double s_r2[kTmParts][64];
+  __shared__ c2 s_pinv[kMaxOrder * kMaxFit], s_vander[kMaxFit * kMaxOrder];
+  const int nct = fit.ncoef * fit.n_fit;
+  for (int t = threadIdx.x; t < nct; t += blockDim.x) {
+    s_pinv[t] = fit.pinv[t];
+    s_vander[t] = fit.vander[t];
+  }
+  const int lane = threadIdx.x & 63;
+  const int part = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long i = (long)blockIdx.x * 64 + lane;
+  const bool valid = i < ntot;
+  const long col = valid ? (colmap ? (long)colmap[i] : i) : (colmap ? (long)colmap[0] : 0);
+  const int nc = fit.ncoef, nf = fit.n_fit;
+  const int per = (nf + kTmParts - 1) / kTmParts;
+  const int f0 = part * per;
+  c2 x[kTmPer];
+#pragma unroll
+  for (int u = 0; u < kTmPer; ++u) x[u] = in[(long)fit.idx[min(f0 + u, nf - 1)] * ncol + col];
+  __syncthreads();  // the tables are in LDS (read as broadcasts below)
+  c2 s[kMaxOrder];
+#pragma unroll
+  for (int q = 0; q < kMaxOrder; ++q) s[q] = c2{0.0, 0.0};
+#pragma unroll
+  for (int u = 0; u < kTmPer; ++u)
+    if (u < per && f0 + u < nf) {
+#pragma unroll
+      for (int q = 0; q < kMaxOrder; ++q)
+        if (q < nc) s[q] = cfma(s[q], s_pinv[q * nf + f0 + u], x[u]);
+    }
+#pragma unroll
+  for (int q = 0; q < kMaxOrder; ++q) s_part[part][q][lane] = s[q];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kMaxOrder; ++q) {
+    c2 t = s_part[0][q][lane];
+#pragma unroll
+    for (int w = 1; w < kTmParts; ++w) t = cadd(t, s_part[w][q][lane]);
+    s[q] = t;
+  }
+  double r2 = 0.0;
+#pragma unroll
+  for (int u = 0; u < kTmPer; ++u)
+    if (u < per && f0 + u < nf) {
+      c2 v = x[u];
+#pragma unroll
+      for (int q = 0; q < kMaxOrder; ++q)
+        if (q < nc) v = cfms(v, s_vander[(f0 + u) * nc + q], s[q]);
+      r2 += cabs2(v);
+    }
+  s_r2[part][lane] = r2;
+  __syncthreads();
+  if (part != 0) return;
+  r2 = s_r2[0][lane];
+#pragma unroll
+  for (int w = 1; w < kTmParts; ++w) r2 += s_r2[w][lane];
+  if (valid) {
+    mom[col] = cscale(s[1], fit.wpow[1]);
+    mom[ncol + col] = cscale(s[2], fit.wpow[2]);
+    mom[2 * ncol + col] = cscale(s[3], fit.wpow[3]);
+  }
+  double res = valid ? sqrt(r2 / nf) : 0.0, m0 = valid ? sqrt(cabs2(s[0])) : 0.0;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    res = fmax(res, __shfl_xor(res, off, 64));
+    m0 = fmax(m0, __shfl_xor(m0, off, 64));
+  }
+  if (lane == 0) {
+    atomic_max_nonneg(&status[0], res);
+    atomic_max_nonneg(&status[1], m0);
+  }
+}
+
+hipError_t launch_tail_moments(hipStream_t s, long ntot, const int *colmap, long ncol, const TailFitDev &fit,
+                               const c2 *in, c2 *mom, Status st) {
+  hipLaunchKernelGGL(k_tail_moments, dim3((unsigned)((ntot + 63) / 64)), dim3(64 * kTmParts), 0, s, ntot, colmap,
+                     ncol, fit, in, mom, st.d);
+  return hipGetLastError();
+}
+
 bool w2t_pruned_supported(int M) { return M == 64 || M == 128 || M == 256 || M == 512 || M == 1024; }
 
 template <int STAT, int M, int NT>
@@ -337,8 +353,14 @@ static hipError_t launch_m(hipStream_t s, double beta, int n_max, long ncol, con
                            const TailFitDev &fit, const MatsDev &mt, const W2tDev &wd, Status st) {
   constexpr int C = NT / (M / 8);
   const long nb = (ncol + C - 1) / C;
+  // without given moments: the fit kernel first, its moments parked in out
+  const int in_out = moments == nullptr;
+  if (in_out) {
+    hipError_t e = launch_tail_moments(s, ncol, nullptr, ncol, fit, in, out + (long)(mt.L - 2) * ncol, st);
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL((k_w2t_pruned<STAT, M, NT>), dim3(nb), dim3(NT), 0, s, beta, n_max, ncol, in, out, moments, fit,
-                     mt, wd, st.d);
+                     mt, wd, st.d, SymCols{nullptr, nullptr, 0}, in_out);
   return hipGetLastError();
 }
 
@@ -364,8 +386,10 @@ static hipError_t launch_sym_m(hipStream_t s, double beta, int n_max, long ncol,
                                const SymCols &sym) {
   constexpr int C = NT / (M / 8);
   const long nb = (sym.n + C - 1) / C;
+  hipError_t e = launch_tail_moments(s, sym.n, sym.col, ncol, fit, in, out + (long)(mt.L - 2) * ncol, st);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL((k_w2t_pruned<0, M, NT, true>), dim3(nb), dim3(NT), 0, s, beta, n_max, ncol, in, out,
-                     (const c2 *)nullptr, fit, mt, wd, st.d, sym);
+                     (const c2 *)nullptr, fit, mt, wd, st.d, sym, 1);
   return hipGetLastError();
 }
 
