@@ -303,7 +303,7 @@ int tprf_construct_phi_wk(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *c
  * on a real-frequency mesh (the reference's w + idelta); where |z_m| and
  * |e_{k+q,j} - e_ki| are both below 1e-10 the factor is the limit
  * beta / (4 cosh^2(beta e_ki / 2)).  e_k[nk][n][n] Hermitian, chi[nz][nk][n^4].
- * O(N_k^2) work.  n <= 4.                                                    */
+ * O(N_k^2) work.  n <= 8.                                                    */
 int tprf_lindhard_chi00(tprf_ctx *ctx, double beta, double mu, long nz, const tprf_c128 *z, const int dims[3],
                         int norb, const tprf_c128 *e_k, tprf_c128 *chi);
 /* No counterpart in the reference.  tprf_lindhard_chi00 for callers that

@@ -365,7 +365,7 @@ def test_singular_dyson_and_rpa(T):
 def test_unsupported_sizes_are_refused(T):
     """Every size outside the kernels' limits returns TPRF_ERR_UNSUPPORTED (-9)
     from the host checks, before any launch: orbitals above the limits of
-    Dyson, RPA, W and the fused bubble (8), Lindhard and the RPA and Dyson
+    Dyson, RPA, W, Lindhard and the fused bubble (8), the RPA and Dyson
     adjoints (4), and Matsubara-axis meshes with no pruned kernel and ntau
     beyond one in-place column in LDS."""
     import torch
@@ -381,7 +381,7 @@ def test_unsupported_sizes_are_refused(T):
         "rpa_herm n=9": lambda: lib.tprf_solve_rpa_PH_herm(c, 1, 1, 9, p, p, p),
         "W n=9": lambda: lib.tprf_dynamical_screened_interaction_W(c, 1, 1, 9, p, p, 0, p),
         "chi0_wr n=9": lambda: lib.tprf_chi0_wr_from_grt_PH(c, 100.0, 61, d, 9, 4, p, p),
-        "lindhard n=5": lambda: lib.tprf_lindhard_chi00(c, 1.0, 0.0, 1, p, d, 5, p, p),
+        "lindhard n=9": lambda: lib.tprf_lindhard_chi00(c, 1.0, 0.0, 1, p, d, 9, p, p),
         "rpa adjoint n=5": lambda: lib.tprf_solve_rpa_PH_adjoint(c, 1, 5, p, p, None, p, None, None),
         "dyson adjoint n=5": lambda: lib.tprf_lattice_dyson_g_wk_adjoint(c, 4, 1, 5, 1, p, p, p,
                                                                           _lib.dptr(x.real.contiguous())),

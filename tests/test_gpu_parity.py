@@ -746,11 +746,12 @@ def test_chain_cubic_full_size_stagewise(L, oracle):
 
 # ---------------------------------------------------------------- f3 analytic bubble
 @pytest.mark.parametrize("dims,n,nw", [((4, 4, 1), 1, 5), ((3, 4, 1), 2, 4), ((4, 3, 2), 3, 3), ((2, 2, 2), 4, 2),
-                                       ((8, 8, 8), 3, 2), ((17, 1, 1), 2, 70)])
+                                       ((8, 8, 8), 3, 2), ((17, 1, 1), 2, 70), ((3, 2, 1), 5, 3), ((2, 2, 1), 8, 2)])
 def test_lindhard_chi00(L, oracle, dims, n, nw):
     """lindhard_chi00 on a bosonic Matsubara mesh against the oracle's
     restatement (itself pinned to a numpy einsum form and to the reference's
-    chi00_square_lattice.py comparison with the tau route)."""
+    chi00_square_lattice.py comparison with the tau route); n = 5, 8 run the
+    LDS eigen-solver and the workgroup-per-(q, frequency) sum."""
     from tprf_amd.gf import Gf, MeshBrZone, MeshImFreq
     rng = np.random.default_rng(sum(dims) * n + nw)
     nk = int(np.prod(dims))

@@ -286,10 +286,10 @@ def lindhard_numpy(e_k, dims, beta, mu, nw):
     return out
 
 
-def test_lindhard_vs_numpy(oracle):
-    rng = np.random.default_rng(8)
-    dims, n = (3, 2, 1), 2
-    a = rnd(rng, 6, n, n)
+@pytest.mark.parametrize("dims,n", [((3, 2, 1), 2), ((2, 2, 1), 6)])
+def test_lindhard_vs_numpy(oracle, dims, n):
+    rng = np.random.default_rng(8 + n)
+    a = rnd(rng, int(np.prod(dims)), n, n)
     e = 0.5 * (a + np.conj(np.swapaxes(a, 1, 2)))
     got = oracle.lindhard_chi00(e, dims, 5.0, 0.2, 3)
     assert relerr(got, lindhard_numpy(e, dims, 5.0, 0.2, 3)) <= 1e-12

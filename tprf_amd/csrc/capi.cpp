@@ -24,6 +24,7 @@ using tprf::c2;
 namespace tprf {
 hipError_t set_kernel_attributes();
 hipError_t consumers_set_attributes();
+hipError_t lindhard_set_attributes();
 }
 
 struct FftEntry {
@@ -530,6 +531,7 @@ int tprf_ctx_create(tprf_ctx **out, int device) {
   if (e == hipSuccess) e = hipMemset(ctx->d_status, 0, 4 * sizeof(double));
   if (e == hipSuccess) e = tprf::set_kernel_attributes();
   if (e == hipSuccess) e = tprf::consumers_set_attributes();
+  if (e == hipSuccess) e = tprf::lindhard_set_attributes();
   if (e == hipSuccess) e = tprf::noalloc_scratch_address(device, &ctx->scratch);
   if (e != hipSuccess) {
     delete ctx;
@@ -950,7 +952,7 @@ int tprf_lindhard_chi00(tprf_ctx *ctx, double beta, double mu, long nz, const tp
   if (!ctx) return TPRF_ERR_ARG;
   if (!(beta > 0) || nz < 1 || !z || !dims_ok(dims) || !e_k || !chi)
     return fail(ctx, TPRF_ERR_ARG, "lindhard_chi00: bad argument");
-  if (norb < 1 || norb > 4) return fail(ctx, TPRF_ERR_UNSUPPORTED, "lindhard_chi00: norb must be 1..4");
+  if (norb < 1 || norb > 8) return fail(ctx, TPRF_ERR_UNSUPPORTED, "lindhard_chi00: norb must be 1..8");
   const long nk = (long)dims[0] * dims[1] * dims[2];
   const size_t ev_bytes = ((sizeof(double) * nk * norb + 255) / 256) * 256;
   void *w;
@@ -967,7 +969,7 @@ int tprf_lindhard_chi00_noalloc(tprf_ctx *ctx, double beta, double mu, long nz, 
   if (!ctx) return TPRF_ERR_ARG;
   if (!(beta > 0) || nz < 1 || !z || !dims_ok(dims) || !e_k || !chi)
     return fail(ctx, TPRF_ERR_ARG, "lindhard_chi00_noalloc: bad argument");
-  if (norb < 1 || norb > 4) return fail(ctx, TPRF_ERR_UNSUPPORTED, "lindhard_chi00_noalloc: norb must be 1..4");
+  if (norb < 1 || norb > 8) return fail(ctx, TPRF_ERR_UNSUPPORTED, "lindhard_chi00_noalloc: norb must be 1..8");
   const long nk = (long)dims[0] * dims[1] * dims[2];
   // the same workspace layout as tprf_lindhard_chi00, in the file-scope array
   const size_t ev_bytes = ((sizeof(double) * nk * norb + 255) / 256) * 256;

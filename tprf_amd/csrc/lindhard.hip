@@ -187,6 +187,159 @@ __global__ void __launch_bounds__(256) k_lindhard(double beta, long nz, const c2
   if (live) chi[((v0 + v) * nk + q) * N4 + abcd] = cscale(acc, 1.0 / (double)nk);
 }
 
+// ------------------------------------------------------------------------
+// 5 <= n <= 8.  Eigenpairs: the same Jacobi sweeps as k_eigh_proj, thread per
+// k, with H and V in a private LDS region of 2 n^2 entries instead of
+// registers.
+constexpr int kEighWideThreads = 32;
+__global__ void __launch_bounds__(kEighWideThreads) k_eigh_proj_wide(int n, long nk, double mu,
+                                                                     const c2 *__restrict__ ek,
+                                                                     double *__restrict__ ev, c2 *__restrict__ proj) {
+  extern __shared__ c2 smw[];
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nk) return;
+  c2 *H = smw + threadIdx.x * 2 * n * n, *V = H + n * n;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      H[i * n + j] = ek[(k * n + i) * n + j];
+      V[i * n + j] = c2{i == j ? 1.0 : 0.0, 0.0};
+    }
+  for (int i = 0; i < n; ++i) {
+    H[i * n + i].x -= mu;
+    H[i * n + i].y = 0.0;
+  }
+  for (int sweep = 0; sweep < 16; ++sweep) {
+    double off = 0.0, diag = 0.0;
+    for (int p = 0; p < n; ++p) {
+      diag += H[p * n + p].x * H[p * n + p].x;
+      for (int q = p + 1; q < n; ++q) off += cabs2(H[p * n + q]);
+    }
+    if (off <= 1e-34 * diag || off == 0.0) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double bb = sqrt(cabs2(H[p * n + q]));
+        if (bb == 0.0) continue;
+        const c2 ph = c2{H[p * n + q].x / bb, -H[p * n + q].y / bb};  // e^{-i phi}
+        const double tau = (H[q * n + q].x - H[p * n + p].x) / (2.0 * bb);
+        const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+        const double c = 1.0 / sqrt(1.0 + t * t), sn = t * c;
+        const c2 g10 = cscale(ph, -sn), g11 = cscale(ph, c);
+        for (int r = 0; r < n; ++r) {
+          const c2 hp = H[r * n + p], hq = H[r * n + q];
+          H[r * n + p] = cadd(cscale(hp, c), cmul(hq, g10));
+          H[r * n + q] = cadd(cscale(hp, sn), cmul(hq, g11));
+          const c2 vp = V[r * n + p], vq = V[r * n + q];
+          V[r * n + p] = cadd(cscale(vp, c), cmul(vq, g10));
+          V[r * n + q] = cadd(cscale(vp, sn), cmul(vq, g11));
+        }
+        for (int r = 0; r < n; ++r) {
+          const c2 hp = H[p * n + r], hq = H[q * n + r];
+          H[p * n + r] = cadd(cscale(hp, c), cmul(hq, cconj(g10)));
+          H[q * n + r] = cadd(cscale(hp, sn), cmul(hq, cconj(g11)));
+        }
+        H[p * n + q] = c2{0.0, 0.0};
+        H[q * n + p] = c2{0.0, 0.0};
+        H[p * n + p].y = 0.0;
+        H[q * n + q].y = 0.0;
+      }
+  }
+  for (int i = 0; i < n; ++i) {
+    ev[k * n + i] = H[i * n + i].x;
+    for (int a = 0; a < n; ++a)
+      for (int d = 0; d < n; ++d) proj[((k * n + i) * n + a) * n + d] = cmul(V[a * n + i], cconj(V[d * n + i]));
+  }
+}
+
+// The k sum for 5 <= n <= 8: one workgroup per (q, frequency), thread t owning
+// the outputs abcd = t + 256 s (s < 16); tiles of KT = 2 k-points of
+// eigenvalues, projectors of k and k + q and the factors F_ij in LDS.
+constexpr int kLhWideKT = 2;
+__global__ void __launch_bounds__(256) k_lindhard_wide(int n, double beta, long nz, const c2 *__restrict__ z, int d0,
+                                                       int d1, int d2, const double *__restrict__ ev,
+                                                       const c2 *__restrict__ proj, c2 *__restrict__ chi) {
+  extern __shared__ c2 smw[];
+  constexpr int KT = kLhWideKT, NS = 16;
+  const int n2 = n * n, n3 = n2 * n, n4 = n2 * n2;
+  c2 *sP = smw;                        // [2][KT][n][n2]
+  c2 *sF = sP + 2 * KT * n3;           // [KT][n2]
+  double *sE = (double *)(sF + KT * n2);  // [2][KT][n]
+  const long nk = (long)d0 * d1 * d2;
+  const long tile = xcd_remap(blockIdx.x, gridDim.x);
+  const long q = tile / nz, v = tile - q * nz;
+  const c2 zz = z[v];
+  const long q0 = q / ((long)d1 * d2), q1 = (q / d2) % d1, q2 = q % d2;
+  const int t = threadIdx.x;
+  c2 acc[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) acc[s] = c2{0.0, 0.0};
+  for (long kt = 0; kt < nk; kt += KT) {
+    const int kn = (nk - kt) < KT ? (int)(nk - kt) : KT;
+    __syncthreads();
+    for (int e = t; e < 2 * KT * n3; e += blockDim.x) {
+      const int side = e / (KT * n3), r = e - side * (KT * n3);
+      const int kk = r / n3, w = r - kk * n3;
+      if (kk < kn) {
+        long k = kt + kk;
+        if (side) {
+          const long k0 = k / ((long)d1 * d2), k1 = (k / d2) % d1, k2 = k % d2;
+          k = (((k0 + q0) % d0) * d1 + (k1 + q1) % d1) * d2 + (k2 + q2) % d2;
+        }
+        sP[e] = proj[k * n3 + w];
+        if (w < n) sE[(side * KT + kk) * n + w] = ev[k * n + w];
+      }
+    }
+    __syncthreads();
+    for (int e = t; e < KT * n2; e += blockDim.x) {
+      const int kk = e / n2, ij = e - kk * n2, i = ij / n, j = ij - i * n;
+      if (kk < kn) {
+        const double eki = sE[kk * n + i], ekqj = sE[(KT + kk) * n + j];
+        const double de = ekqj - eki;
+        const double dn = fermi_fn(eki * beta) - fermi_fn(ekqj * beta);
+        c2 F;
+        if (sqrt(cabs2(zz)) < 1e-10 && fabs(de) < 1e-10) {
+          const double ch = cosh(0.5 * beta * eki);
+          F = c2{beta / (4.0 * ch * ch), 0.0};
+        } else {
+          F = cdiv(c2{dn, 0.0}, c2{zz.x + de, zz.y});
+        }
+        sF[e] = F;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int abcd = t + 256 * s;
+      if (abcd < n4) {
+        const int a = abcd / n3, b = (abcd / n2) % n, c = (abcd / n) % n, d = abcd % n;
+        for (int kk = 0; kk < kn; ++kk) {
+          const c2 *P0 = sP + kk * n3, *P1 = sP + (KT + kk) * n3, *F = sF + kk * n2;
+          for (int i = 0; i < n; ++i) {
+            c2 sum = c2{0.0, 0.0};
+            for (int j = 0; j < n; ++j) sum = cfma(sum, F[i * n + j], P1[j * n2 + c * n + b]);
+            acc[s] = cfma(acc[s], P0[i * n2 + a * n + d], sum);
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const int abcd = t + 256 * s;
+    if (abcd < n4) chi[(v * nk + q) * n4 + abcd] = cscale(acc[s], 1.0 / (double)nk);
+  }
+}
+
+size_t lindhard_wide_lds(int n) {
+  const int KT = kLhWideKT;
+  return sizeof(c2) * (size_t)(2 * KT * n * n * n + KT * n * n) + sizeof(double) * 2 * KT * n;
+}
+
+// dynamic LDS of the eigen kernel: 2 n^2 entries per thread, 64 KiB at n = 8
+hipError_t lindhard_set_attributes() {
+  return hipFuncSetAttribute((const void *)k_eigh_proj_wide, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)(sizeof(c2) * 2 * 64 * kEighWideThreads));
+}
+
 hipError_t launch_lindhard(hipStream_t s, double beta, double mu, long nz, const c2 *z, const int dims[3], int n,
                            const c2 *ek, double *ev, c2 *proj, c2 *chi) {
   const long nk = (long)dims[0] * dims[1] * dims[2];
@@ -207,6 +360,17 @@ hipError_t launch_lindhard(hipStream_t s, double beta, double mu, long nz, const
     TPRF_LH_CASE(3)
     TPRF_LH_CASE(4)
 #undef TPRF_LH_CASE
+    case 5:
+    case 6:
+    case 7:
+    case 8: {
+      const long nbe = (nk + kEighWideThreads - 1) / kEighWideThreads;
+      hipLaunchKernelGGL(k_eigh_proj_wide, dim3(nbe), dim3(kEighWideThreads),
+                         sizeof(c2) * 2 * n * n * kEighWideThreads, s, n, nk, mu, ek, ev, proj);
+      hipLaunchKernelGGL(k_lindhard_wide, dim3(nk * nz), dim3(256), lindhard_wide_lds(n), s, n, beta, nz, z, dims[0],
+                         dims[1], dims[2], ev, proj, chi);
+      break;
+    }
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
