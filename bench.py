@@ -388,6 +388,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-extra", action="store_true", help="skip the grad/fwd and batched/plain legs")
+    ap.add_argument("--roctx", action="store_true",
+                    help="open a roctx range per library entry point (for rocprofv3 --marker-trace)")
     ap.add_argument("--only", choices=["grad", "batched", "uneven", "screened_w", "calibration", "grad_matsubara",
                                        "batched_matsubara"],
                     help="run one leg alone and print its JSON (for profiling)")
@@ -402,6 +404,9 @@ def main():
     from tprf_amd import chain, workloads
 
     torch.cuda.set_device(0)
+    if args.roctx:
+        from tprf_amd import _lib
+        _lib.set_trace(_lib.ctx(0))
     if args.only:
         if args.only == "calibration":
             calibration_run()

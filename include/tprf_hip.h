@@ -76,6 +76,11 @@ int tprf_ctx_set_stream(tprf_ctx *ctx, void *hip_stream); /* hipStream_t; NULL =
  * the value mod 2: one orbital row or one column per workgroup).  In-place
  * calls (in == out) are never timed; they use the cached or default choice. */
 #define TPRF_OPT_LATTICE_VARIANT 4
+/* TPRF_OPT_TRACE = 1: every entry point opens a roctx range named after it
+ * (and tprf_forward_batched one per instance) over its host-side span, for
+ * rocprofv3 --marker-trace next to the kernel trace.  roctx is loaded from
+ * the rocprofiler SDK on first use; TPRF_ERR_UNSUPPORTED if it is absent.   */
+#define TPRF_OPT_TRACE 5
 int tprf_ctx_set_option(tprf_ctx *ctx, int option, int value);
 /* Device scratch (no counterpart in the reference).  The static bubble, the
  * Dyson adjoint and lindhard_chi00 need scratch beyond their outputs.  By

@@ -420,3 +420,24 @@ def test_rpa_herm_without_exchange_symmetry(T, oracle):
     _lib.call("tprf_solve_rpa_PH_herm", c, nb, nk, n, _lib.ptr(x), _lib.ptr(u), _lib.ptr(out))
     _lib.tail_status(c)
     assert relerr(host(out), ref) <= 1e-12
+
+
+def test_roctx_trace_option(T):
+    """TPRF_OPT_TRACE = 1 opens a roctx range per entry point (the rocprofiler
+    SDK's roctx library is part of the ROCm image); the traced chain gives
+    bit-identical results, and the option switches off again."""
+    import torch
+    from tprf_amd import _lib, chain, workloads
+    wl = workloads.plain_square(n=8, n_iw=16, nb=8)
+    ch = chain.BubbleRPA(wl)
+    ch.run()
+    torch.cuda.synchronize()
+    ref = ch.chi_wk.clone()
+    c = _lib.ctx()
+    _lib.set_trace(c, True)
+    try:
+        ch.run()
+        torch.cuda.synchronize()
+    finally:
+        _lib.set_trace(c, False)
+    assert torch.equal(ch.chi_wk, ref)

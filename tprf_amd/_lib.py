@@ -162,6 +162,14 @@ def tail_status(c):
     return err.value, m0.value
 
 
+OPT_TRACE = 5
+
+
+def set_trace(c, on=True):
+    """roctx ranges per entry point on context c (rocprofv3 --marker-trace)."""
+    check(lib().tprf_ctx_set_option(c, OPT_TRACE, 1 if on else 0), c, "trace option")
+
+
 def tail_status_reset(c):
     check(lib().tprf_ctx_tail_status_reset(c), c, "tail status reset")
 

@@ -6,12 +6,15 @@
 // and launch the kernels of kernels.hip on the context's stream.
 #include "../../include/tprf_hip.h"
 
+#include <dlfcn.h>
+
 #include <cmath>
 #include <complex>
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -90,7 +93,49 @@ struct tprf_ctx {
   std::map<std::tuple<int, int, int, int>, SymEntry> symcols;  // canonical columns per (dims, n)
   std::vector<hipStream_t> lanes;      // streams of tprf_forward_batched (default priority)
   std::vector<hipEvent_t> lane_events; // fork event + one join event per lane
+  bool trace = false;                  // roctx range per entry point (TPRF_OPT_TRACE)
 };
+
+namespace {
+
+// roctx from the rocprofiler SDK, loaded on first use (the library then has no
+// link-time dependency on the profiler).
+struct RoctxApi {
+  int (*push)(const char *) = nullptr;
+  int (*pop)() = nullptr;
+};
+
+const RoctxApi *roctx_api() {
+  static RoctxApi api;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    void *h = dlopen("librocprofiler-sdk-roctx.so.1", RTLD_NOW | RTLD_LOCAL);
+    if (!h) h = dlopen("librocprofiler-sdk-roctx.so", RTLD_NOW | RTLD_LOCAL);
+    if (!h) return;
+    api.push = (int (*)(const char *))dlsym(h, "roctxRangePushA");
+    api.pop = (int (*)())dlsym(h, "roctxRangePop");
+    if (!api.push || !api.pop) api.push = nullptr;
+  });
+  return api.push ? &api : nullptr;
+}
+
+// A roctx range over the host-side span of one entry point (argument checks,
+// table set-up and the enqueueing of its kernels) when the context traces.
+struct TraceScope {
+  const RoctxApi *api;
+  TraceScope(const tprf_ctx *ctx, const char *name) : api(ctx && ctx->trace ? roctx_api() : nullptr) {
+    if (api) api->push(name);
+  }
+  ~TraceScope() {
+    if (api) api->pop();
+  }
+  TraceScope(const TraceScope &) = delete;
+  TraceScope &operator=(const TraceScope &) = delete;
+};
+
+}  // namespace
+
+#define TPRF_TRACE(ctx, name) TraceScope tprf_trace_scope_((ctx), (name))
 
 namespace {
 
@@ -584,6 +629,11 @@ int tprf_ctx_set_option(tprf_ctx *ctx, int option, int value) {
     ctx->w2t_variant = value;
     return TPRF_OK;
   }
+  if (option == TPRF_OPT_TRACE) {
+    if (value && !roctx_api()) return fail(ctx, TPRF_ERR_UNSUPPORTED, "roctx library not found");
+    ctx->trace = value != 0;
+    return TPRF_OK;
+  }
   if (option == TPRF_OPT_BUBBLE_VARIANT) {
     if (value < -1 || value >= tprf::bubble_pair_variants()) return fail(ctx, TPRF_ERR_ARG, "bad bubble variant");
     ctx->bubble_variant = value;
@@ -634,6 +684,7 @@ int tprf_ctx_tail_status_reset(tprf_ctx *ctx) {
 
 int tprf_lattice_dyson_g_wk(tprf_ctx *ctx, double beta, int n_iw, double mu, long nk, int norb, const tprf_c128 *e_k,
                             int sigma_kind, const tprf_c128 *sigma, tprf_c128 *g_wk) {
+  TPRF_TRACE(ctx, "tprf_lattice_dyson_g_wk");
   if (!ctx) return TPRF_ERR_ARG;
   if (!(beta > 0) || n_iw < 1 || nk < 1 || !e_k || !g_wk) return fail(ctx, TPRF_ERR_ARG, "lattice_dyson_g_wk: bad argument");
   if (sigma_kind < 0 || sigma_kind > 2 || (sigma_kind && !sigma))
@@ -678,21 +729,25 @@ int dyson_g_wr(tprf_ctx *ctx, double beta, int n_iw, double mu, const int dims[3
 
 int tprf_lattice_dyson_g_wr(tprf_ctx *ctx, double beta, int n_iw, double mu, const int dims[3], int norb,
                             const tprf_c128 *e_k, tprf_c128 *g_wr) {
+  TPRF_TRACE(ctx, "tprf_lattice_dyson_g_wr");
   return dyson_g_wr(ctx, beta, n_iw, mu, dims, norb, e_k, g_wr, 0);
 }
 
 int tprf_lattice_dyson_g_wr_herm(tprf_ctx *ctx, double beta, int n_iw, double mu, const int dims[3], int norb,
                                  const tprf_c128 *e_k, tprf_c128 *g_wr, int full) {
+  TPRF_TRACE(ctx, "tprf_lattice_dyson_g_wr_herm");
   return dyson_g_wr(ctx, beta, n_iw, mu, dims, norb, e_k, g_wr, full ? 1 : 2);
 }
 
 int tprf_fourier_wk_to_wr(tprf_ctx *ctx, long nw, const int dims[3], long ntarget, const tprf_c128 *in,
                           tprf_c128 *out) {
+  TPRF_TRACE(ctx, "tprf_fourier_wk_to_wr");
   return lattice_fft(ctx, nw, dims, ntarget, (const c2 *)in, (c2 *)out, -1, true);
 }
 
 int tprf_fourier_wr_to_wk(tprf_ctx *ctx, long nw, const int dims[3], long ntarget, const tprf_c128 *in,
                           tprf_c128 *out) {
+  TPRF_TRACE(ctx, "tprf_fourier_wr_to_wk");
   return lattice_fft(ctx, nw, dims, ntarget, (const c2 *)in, (c2 *)out, +1, false);
 }
 
@@ -807,11 +862,13 @@ int wr_to_tr(tprf_ctx *ctx, int statistic, double beta, int n_max, int ntau, lon
 
 int tprf_fourier_wr_to_tr(tprf_ctx *ctx, int statistic, double beta, int n_max, int ntau, long ncol,
                           const tprf_c128 *in, tprf_c128 *out, const tprf_c128 *moments) {
+  TPRF_TRACE(ctx, "tprf_fourier_wr_to_tr");
   return wr_to_tr(ctx, statistic, beta, n_max, ntau, ncol, in, out, moments, nullptr);
 }
 
 int tprf_fourier_wr_to_tr_herm(tprf_ctx *ctx, double beta, int n_max, int ntau, const int dims[3], int norb,
                                const tprf_c128 *in, tprf_c128 *out, int full) {
+  TPRF_TRACE(ctx, "tprf_fourier_wr_to_tr_herm");
   if (!ctx) return TPRF_ERR_ARG;
   if (!dims_ok(dims) || norb < 1) return fail(ctx, TPRF_ERR_ARG, "fourier_wr_to_tr_herm: bad argument");
   tprf::SymCols sym;
@@ -823,6 +880,7 @@ int tprf_fourier_wr_to_tr_herm(tprf_ctx *ctx, double beta, int n_max, int ntau, 
 
 int tprf_fourier_tr_to_wr(tprf_ctx *ctx, int statistic, double beta, int ntau, int nw, long ncol,
                           const tprf_c128 *in, tprf_c128 *out) {
+  TPRF_TRACE(ctx, "tprf_fourier_tr_to_wr");
   if (!ctx) return TPRF_ERR_ARG;
   if ((statistic != TPRF_FERMION && statistic != TPRF_BOSON) || !(beta > 0) || ntau < 10 || ncol < 1 || !in || !out)
     return fail(ctx, TPRF_ERR_ARG, "fourier_tr_to_wr: bad argument");
@@ -846,6 +904,7 @@ int tprf_fourier_tr_to_wr(tprf_ctx *ctx, int statistic, double beta, int ntau, i
 
 int tprf_chi0_tr_from_grt_PH(tprf_ctx *ctx, int ntau, const int dims[3], int norb, const tprf_c128 *g_tr,
                              tprf_c128 *chi0_tr) {
+  TPRF_TRACE(ctx, "tprf_chi0_tr_from_grt_PH");
   if (!ctx) return TPRF_ERR_ARG;
   if (ntau < 2 || !dims_ok(dims) || norb < 1 || norb > 8 || !g_tr || !chi0_tr)
     return fail(ctx, TPRF_ERR_ARG, "chi0_tr_from_grt_PH: bad argument");
@@ -911,16 +970,19 @@ int chi0_wr_from_grt(tprf_ctx *ctx, double beta, int ntau, const int dims[3], in
 
 int tprf_chi0_wr_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb, int nw,
                              const tprf_c128 *g_tr, tprf_c128 *chi0_wr) {
+  TPRF_TRACE(ctx, "tprf_chi0_wr_from_grt_PH");
   return chi0_wr_from_grt(ctx, beta, ntau, dims, norb, nw, g_tr, chi0_wr, false);
 }
 
 int tprf_chi0_wr_from_grt_PH_herm(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb, int nw,
                                   const tprf_c128 *g_tr, tprf_c128 *chi0_wr, int full) {
+  TPRF_TRACE(ctx, "tprf_chi0_wr_from_grt_PH_herm");
   return chi0_wr_from_grt(ctx, beta, ntau, dims, norb, nw, g_tr, chi0_wr, true, full == 0);
 }
 
 int tprf_chi0_w0r_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb,
                               const tprf_c128 *g_tr, tprf_c128 *chi0_wr) {
+  TPRF_TRACE(ctx, "tprf_chi0_w0r_from_grt_PH");
   if (!ctx) return TPRF_ERR_ARG;
   if (!(beta > 0) || ntau < 2 || !dims_ok(dims) || norb < 1 || norb > 8 || !g_tr || !chi0_wr)
     return fail(ctx, TPRF_ERR_ARG, "chi0_w0r_from_grt_PH: bad argument");
@@ -930,6 +992,7 @@ int tprf_chi0_w0r_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int di
 
 int tprf_chi_w0r_from_chi_tr(tprf_ctx *ctx, double beta, int ntau, long nr, int norb, const tprf_c128 *chi_tr,
                              tprf_c128 *chi_wr) {
+  TPRF_TRACE(ctx, "tprf_chi_w0r_from_chi_tr");
   if (!ctx) return TPRF_ERR_ARG;
   if (!(beta > 0) || ntau < 2 || nr < 1 || norb < 1 || norb > 8 || !chi_tr || !chi_wr)
     return fail(ctx, TPRF_ERR_ARG, "chi_w0r_from_chi_tr: bad argument");
@@ -938,6 +1001,7 @@ int tprf_chi_w0r_from_chi_tr(tprf_ctx *ctx, double beta, int ntau, long nr, int 
 }
 
 int tprf_solve_rpa_PH(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *chi0, const tprf_c128 *U, tprf_c128 *chi) {
+  TPRF_TRACE(ctx, "tprf_solve_rpa_PH");
   if (!ctx) return TPRF_ERR_ARG;
   if (npts < 1 || !chi0 || !U || !chi) return fail(ctx, TPRF_ERR_ARG, "solve_rpa_PH: bad argument");
   if (norb < 1 || norb > tprf::kWideMaxOrb) return fail(ctx, TPRF_ERR_UNSUPPORTED, "solve_rpa_PH: norb must be 1..8");
@@ -949,6 +1013,7 @@ int tprf_solve_rpa_PH(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *chi0,
 
 int tprf_lindhard_chi00(tprf_ctx *ctx, double beta, double mu, long nz, const tprf_c128 *z, const int dims[3],
                         int norb, const tprf_c128 *e_k, tprf_c128 *chi) {
+  TPRF_TRACE(ctx, "tprf_lindhard_chi00");
   if (!ctx) return TPRF_ERR_ARG;
   if (!(beta > 0) || nz < 1 || !z || !dims_ok(dims) || !e_k || !chi)
     return fail(ctx, TPRF_ERR_ARG, "lindhard_chi00: bad argument");
@@ -966,6 +1031,7 @@ int tprf_lindhard_chi00(tprf_ctx *ctx, double beta, double mu, long nz, const tp
 
 int tprf_lindhard_chi00_noalloc(tprf_ctx *ctx, double beta, double mu, long nz, const tprf_c128 *z, const int dims[3],
                                 int norb, const tprf_c128 *e_k, tprf_c128 *chi) {
+  TPRF_TRACE(ctx, "tprf_lindhard_chi00_noalloc");
   if (!ctx) return TPRF_ERR_ARG;
   if (!(beta > 0) || nz < 1 || !z || !dims_ok(dims) || !e_k || !chi)
     return fail(ctx, TPRF_ERR_ARG, "lindhard_chi00_noalloc: bad argument");
@@ -988,6 +1054,7 @@ int tprf_lindhard_chi00_noalloc(tprf_ctx *ctx, double beta, double mu, long nz, 
 
 int tprf_solve_rpa_PH_herm(tprf_ctx *ctx, int nb, long nk, int norb, const tprf_c128 *chi0, const tprf_c128 *U,
                            tprf_c128 *chi) {
+  TPRF_TRACE(ctx, "tprf_solve_rpa_PH_herm");
   if (!ctx) return TPRF_ERR_ARG;
   if (nb < 1 || nk < 1 || !chi0 || !U || !chi) return fail(ctx, TPRF_ERR_ARG, "solve_rpa_PH_herm: bad argument");
   if (norb < 1 || norb > tprf::kWideMaxOrb) return fail(ctx, TPRF_ERR_UNSUPPORTED, "solve_rpa_PH_herm: norb must be 1..8");
@@ -999,6 +1066,7 @@ int tprf_solve_rpa_PH_herm(tprf_ctx *ctx, int nb, long nk, int norb, const tprf_
 
 int tprf_dynamical_screened_interaction_W(tprf_ctx *ctx, long nw, long nk, int norb, const tprf_c128 *PI,
                                           const tprf_c128 *V, int dynamic, tprf_c128 *W) {
+  TPRF_TRACE(ctx, "tprf_dynamical_screened_interaction_W");
   if (!ctx) return TPRF_ERR_ARG;
   if (nw < 1 || nk < 1 || !PI || !V || !W || (dynamic != 0 && dynamic != 1))
     return fail(ctx, TPRF_ERR_ARG, "dynamical_screened_interaction_W: bad argument");
@@ -1021,6 +1089,7 @@ int consumer_n(tprf_ctx *ctx, int norb, const char *what) {
 int tprf_split_into_dynamic_wk_and_constant_k(tprf_ctx *ctx, int statistic, double beta, int n_max, long ncol,
                                               const tprf_c128 *chi_wk, tprf_c128 *chi_dyn_wk,
                                               tprf_c128 *chi_const_k) {
+  TPRF_TRACE(ctx, "tprf_split_into_dynamic_wk_and_constant_k");
   if (!ctx) return TPRF_ERR_ARG;
   if ((statistic != TPRF_FERMION && statistic != TPRF_BOSON) || !(beta > 0) || n_max < 1 || ncol < 1 || !chi_wk ||
       !chi_dyn_wk || !chi_const_k)
@@ -1036,6 +1105,7 @@ int tprf_split_into_dynamic_wk_and_constant_k(tprf_ctx *ctx, int statistic, doub
 }
 
 int tprf_density(tprf_ctx *ctx, double beta, int n_max, long ncol, const tprf_c128 *g_w, tprf_c128 *rho) {
+  TPRF_TRACE(ctx, "tprf_density");
   if (!ctx) return TPRF_ERR_ARG;
   if (!(beta > 0) || n_max < 1 || ncol < 1 || !g_w || !rho) return fail(ctx, TPRF_ERR_ARG, "density: bad argument");
   const tprf::TailFitDev *fit;
@@ -1049,6 +1119,7 @@ int tprf_density(tprf_ctx *ctx, double beta, int n_max, long ncol, const tprf_c1
 
 int tprf_gw_dynamic_sigma(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *W_tr, const tprf_c128 *g_tr,
                           tprf_c128 *sigma_tr) {
+  TPRF_TRACE(ctx, "tprf_gw_dynamic_sigma");
   if (!ctx) return TPRF_ERR_ARG;
   if (npts < 1 || !W_tr || !g_tr || !sigma_tr) return fail(ctx, TPRF_ERR_ARG, "gw_dynamic_sigma: bad argument");
   int st = consumer_n(ctx, norb, "gw_dynamic_sigma");
@@ -1062,6 +1133,7 @@ int tprf_gw_dynamic_sigma(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *W
 
 int tprf_fock_sigma_r(tprf_ctx *ctx, long nr, int norb, const tprf_c128 *v_r, const tprf_c128 *rho_r,
                       tprf_c128 *sigma_r) {
+  TPRF_TRACE(ctx, "tprf_fock_sigma_r");
   if (!ctx) return TPRF_ERR_ARG;
   if (nr < 1 || !v_r || !rho_r || !sigma_r) return fail(ctx, TPRF_ERR_ARG, "fock_sigma: bad argument");
   int st = consumer_n(ctx, norb, "fock_sigma");
@@ -1075,6 +1147,7 @@ int tprf_fock_sigma_r(tprf_ctx *ctx, long nr, int norb, const tprf_c128 *v_r, co
 
 int tprf_hartree_sigma_r(tprf_ctx *ctx, long nr, int norb, const tprf_c128 *v_k, const tprf_c128 *rho_r,
                          tprf_c128 *sigma_r) {
+  TPRF_TRACE(ctx, "tprf_hartree_sigma_r");
   if (!ctx) return TPRF_ERR_ARG;
   if (nr < 1 || !v_k || !rho_r || !sigma_r) return fail(ctx, TPRF_ERR_ARG, "hartree_sigma: bad argument");
   int st = consumer_n(ctx, norb, "hartree_sigma");
@@ -1090,6 +1163,7 @@ int tprf_hartree_sigma_r(tprf_ctx *ctx, long nr, int norb, const tprf_c128 *v_k,
 
 int tprf_static_sigma_k(tprf_ctx *ctx, int fock, const int dims[3], int norb, const tprf_c128 *v_k,
                         const tprf_c128 *rho_k, tprf_c128 *sigma_k) {
+  TPRF_TRACE(ctx, "tprf_static_sigma_k");
   if (!ctx) return TPRF_ERR_ARG;
   if (!dims_ok(dims) || (fock != 0 && fock != 1) || !v_k || !rho_k || !sigma_k)
     return fail(ctx, TPRF_ERR_ARG, "hartree/fock sigma (k): bad argument");
@@ -1103,6 +1177,7 @@ int tprf_static_sigma_k(tprf_ctx *ctx, int fock, const int dims[3], int norb, co
 
 int tprf_eliashberg_gamma_f_product(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *gamma, const tprf_c128 *F,
                                     tprf_c128 *delta) {
+  TPRF_TRACE(ctx, "tprf_eliashberg_gamma_f_product");
   if (!ctx) return TPRF_ERR_ARG;
   if (npts < 1 || !gamma || !F || !delta) return fail(ctx, TPRF_ERR_ARG, "eliashberg gamma-F product: bad argument");
   int st = consumer_n(ctx, norb, "eliashberg gamma-F product");
@@ -1116,6 +1191,7 @@ int tprf_eliashberg_gamma_f_product(tprf_ctx *ctx, long npts, int norb, const tp
 
 int tprf_eliashberg_g_delta_g_product(tprf_ctx *ctx, int n_max_g, int n_max_d, const int dims[3], int norb,
                                       const tprf_c128 *g_wk, const tprf_c128 *delta_wk, tprf_c128 *F_wk) {
+  TPRF_TRACE(ctx, "tprf_eliashberg_g_delta_g_product");
   if (!ctx) return TPRF_ERR_ARG;
   if (n_max_g < 1 || n_max_d < 1 || !dims_ok(dims) || !g_wk || !delta_wk || !F_wk)
     return fail(ctx, TPRF_ERR_ARG, "eliashberg_g_delta_g_product: bad argument");
@@ -1134,6 +1210,7 @@ int tprf_eliashberg_g_delta_g_product(tprf_ctx *ctx, int n_max_g, int n_max_d, c
 int tprf_eliashberg_product(tprf_ctx *ctx, double beta, int n_max_gamma, int n_max_g, int n_max_d, const int dims[3],
                             int norb, const tprf_c128 *gamma_wk, const tprf_c128 *g_wk, const tprf_c128 *delta_wk,
                             tprf_c128 *delta_out) {
+  TPRF_TRACE(ctx, "tprf_eliashberg_product");
   if (!ctx) return TPRF_ERR_ARG;
   if (!(beta > 0) || n_max_gamma < 1 || n_max_g < 1 || n_max_d < 1 || !dims_ok(dims) || !gamma_wk || !g_wk ||
       !delta_wk || !delta_out)
@@ -1158,6 +1235,7 @@ int tprf_eliashberg_product(tprf_ctx *ctx, double beta, int n_max_gamma, int n_m
 
 int tprf_construct_phi_wk(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *chi, const tprf_c128 *U,
                           tprf_c128 *phi) {
+  TPRF_TRACE(ctx, "tprf_construct_phi_wk");
   if (!ctx) return TPRF_ERR_ARG;
   if (npts < 1 || !chi || !U || !phi) return fail(ctx, TPRF_ERR_ARG, "construct_phi_wk: bad argument");
   int st = consumer_n(ctx, norb, "construct_phi_wk");
@@ -1168,6 +1246,7 @@ int tprf_construct_phi_wk(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *c
 }
 
 int tprf_add_dynamic_and_static(tprf_ctx *ctx, long nw, long ncol, tprf_c128 *x_w, const tprf_c128 *s) {
+  TPRF_TRACE(ctx, "tprf_add_dynamic_and_static");
   if (!ctx) return TPRF_ERR_ARG;
   if (nw < 1 || ncol < 1 || !x_w || !s) return fail(ctx, TPRF_ERR_ARG, "add_dynamic_and_static: bad argument");
   return check_launch(ctx, tprf::launch_add_static(ctx->stream, nw, ncol, (c2 *)x_w, (const c2 *)s),
@@ -1178,6 +1257,7 @@ int tprf_add_dynamic_and_static(tprf_ctx *ctx, long nw, long ncol, tprf_c128 *x_
 
 int tprf_lattice_dyson_g_wk_batched(tprf_ctx *ctx, double beta, int n_iw, const double *mu, long nbatch, long nk,
                                     int norb, const tprf_c128 *e_k, tprf_c128 *g_wk) {
+  TPRF_TRACE(ctx, "tprf_lattice_dyson_g_wk_batched");
   if (!ctx) return TPRF_ERR_ARG;
   if (!(beta > 0) || n_iw < 1 || nk < 1 || nbatch < 1 || !mu || !e_k || !g_wk)
     return fail(ctx, TPRF_ERR_ARG, "lattice_dyson_g_wk_batched: bad argument");
@@ -1190,6 +1270,7 @@ int tprf_lattice_dyson_g_wk_batched(tprf_ctx *ctx, double beta, int n_iw, const 
 
 int tprf_lattice_dyson_g_wk_adjoint(tprf_ctx *ctx, int n_iw, long nk, int norb, long nbatch, const tprf_c128 *g_wk,
                                     const tprf_c128 *gbar_wk, tprf_c128 *ebar_k, double *mubar) {
+  TPRF_TRACE(ctx, "tprf_lattice_dyson_g_wk_adjoint");
   if (!ctx) return TPRF_ERR_ARG;
   if (n_iw < 1 || nk < 1 || nbatch < 1 || !g_wk || !gbar_wk || !ebar_k || !mubar)
     return fail(ctx, TPRF_ERR_ARG, "lattice_dyson_g_wk_adjoint: bad argument");
@@ -1217,11 +1298,13 @@ int tprf_lattice_dyson_g_wk_adjoint(tprf_ctx *ctx, int n_iw, long nk, int norb, 
 
 int tprf_fourier_wk_to_wr_adjoint(tprf_ctx *ctx, long nw, const int dims[3], long ntarget, const tprf_c128 *gbar_wr,
                                   tprf_c128 *gbar_wk) {
+  TPRF_TRACE(ctx, "tprf_fourier_wk_to_wr_adjoint");
   return lattice_fft(ctx, nw, dims, ntarget, (const c2 *)gbar_wr, (c2 *)gbar_wk, +1, true);
 }
 
 int tprf_fourier_wr_to_tr_adjoint(tprf_ctx *ctx, int statistic, double beta, int n_max, int ntau, long ncol,
                                   const tprf_c128 *gbar_tr, tprf_c128 *gbar_wr, int fitted) {
+  TPRF_TRACE(ctx, "tprf_fourier_wr_to_tr_adjoint");
   if (!ctx) return TPRF_ERR_ARG;
   if ((statistic != TPRF_FERMION && statistic != TPRF_BOSON) || !(beta > 0) || n_max < 1 || ncol < 1 || !gbar_tr ||
       !gbar_wr)
@@ -1258,6 +1341,7 @@ int tprf_fourier_wr_to_tr_adjoint(tprf_ctx *ctx, int statistic, double beta, int
 
 int tprf_chi0_static_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb, long nbatch,
                                  const tprf_c128 *g_tr, tprf_c128 *chi0_r) {
+  TPRF_TRACE(ctx, "tprf_chi0_static_from_grt_PH");
   if (!ctx) return TPRF_ERR_ARG;
   if (!(beta > 0) || ntau < 18 || !dims_ok(dims) || nbatch < 1 || !g_tr || !chi0_r)
     return fail(ctx, TPRF_ERR_ARG, "chi0_static_from_grt_PH: bad argument");
@@ -1283,6 +1367,7 @@ int tprf_chi0_static_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int
 int tprf_chi0_static_from_grt_PH_adjoint(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb,
                                          long nbatch, const tprf_c128 *g_tr, const tprf_c128 *chibar_r,
                                          tprf_c128 *gbar_tr) {
+  TPRF_TRACE(ctx, "tprf_chi0_static_from_grt_PH_adjoint");
   if (!ctx) return TPRF_ERR_ARG;
   if (!(beta > 0) || ntau < 18 || !dims_ok(dims) || nbatch < 1 || !g_tr || !chibar_r || !gbar_tr)
     return fail(ctx, TPRF_ERR_ARG, "chi0_static_from_grt_PH_adjoint: bad argument");
@@ -1314,6 +1399,7 @@ int static_gwr_check(tprf_ctx *ctx, double beta, int n_iw, int &ntau, const int 
 
 int tprf_chi0_static_from_gwr_PH(tprf_ctx *ctx, double beta, int n_iw, int ntau, const int dims[3], int norb,
                                  long nbatch, const tprf_c128 *g_wr, tprf_c128 *chi0_r) {
+  TPRF_TRACE(ctx, "tprf_chi0_static_from_gwr_PH");
   if (!ctx) return TPRF_ERR_ARG;
   int st = static_gwr_check(ctx, beta, n_iw, ntau, dims, norb, nbatch, "chi0_static_from_gwr_PH");
   if (st) return st;
@@ -1334,6 +1420,7 @@ int tprf_chi0_static_from_gwr_PH(tprf_ctx *ctx, double beta, int n_iw, int ntau,
 int tprf_chi0_static_from_gwr_PH_adjoint(tprf_ctx *ctx, double beta, int n_iw, int ntau, const int dims[3], int norb,
                                          long nbatch, const tprf_c128 *g_wr, const tprf_c128 *chibar_r,
                                          tprf_c128 *gbar_wr) {
+  TPRF_TRACE(ctx, "tprf_chi0_static_from_gwr_PH_adjoint");
   if (!ctx) return TPRF_ERR_ARG;
   int st = static_gwr_check(ctx, beta, n_iw, ntau, dims, norb, nbatch, "chi0_static_from_gwr_PH_adjoint");
   if (st) return st;
@@ -1370,6 +1457,7 @@ int static_q_setup(tprf_ctx *ctx, double beta, int n_iw, int &ntau, const int di
 int tprf_chi0_static_q_from_ek(tprf_ctx *ctx, double beta, int n_iw, int ntau, const int dims[3], const int q[3],
                                int norb, long nbatch, const double *mu, const tprf_c128 *e_k, tprf_c128 *chi0_q,
                                tprf_c128 *proj) {
+  TPRF_TRACE(ctx, "tprf_chi0_static_q_from_ek");
   if (!ctx) return TPRF_ERR_ARG;
   const char *what = "chi0_static_q_from_ek";
   int qr[3];
@@ -1393,6 +1481,7 @@ int tprf_chi0_static_q_from_ek_adjoint(tprf_ctx *ctx, double beta, int n_iw, int
                                        const int q[3], int norb, long nbatch, const double *mu, const tprf_c128 *e_k,
                                        const tprf_c128 *chibar_q, const tprf_c128 *proj, tprf_c128 *ebar_k,
                                        double *mubar) {
+  TPRF_TRACE(ctx, "tprf_chi0_static_q_from_ek_adjoint");
   if (!ctx) return TPRF_ERR_ARG;
   const char *what = "chi0_static_q_from_ek_adjoint";
   int qr[3];
@@ -1414,6 +1503,7 @@ int tprf_chi0_static_q_from_ek_adjoint(tprf_ctx *ctx, double beta, int n_iw, int
 
 int tprf_fourier_r_to_q(tprf_ctx *ctx, const int dims[3], const int q[3], long ntarget, long nbatch,
                         const tprf_c128 *in_r, tprf_c128 *out_q) {
+  TPRF_TRACE(ctx, "tprf_fourier_r_to_q");
   if (!ctx) return TPRF_ERR_ARG;
   if (!dims_ok(dims) || !q || ntarget < 1 || ntarget > 65535 || nbatch < 1 || nbatch > 65535 || !in_r || !out_q)
     return fail(ctx, TPRF_ERR_ARG, "fourier_r_to_q: bad argument");
@@ -1427,6 +1517,7 @@ int tprf_fourier_r_to_q(tprf_ctx *ctx, const int dims[3], const int q[3], long n
 
 int tprf_fourier_r_to_q_adjoint(tprf_ctx *ctx, const int dims[3], const int q[3], long ntarget, long nbatch,
                                 const tprf_c128 *gbar_q, tprf_c128 *gbar_r) {
+  TPRF_TRACE(ctx, "tprf_fourier_r_to_q_adjoint");
   if (!ctx) return TPRF_ERR_ARG;
   if (!dims_ok(dims) || !q || ntarget < 1 || ntarget > 65535 || nbatch < 1 || !gbar_q || !gbar_r)
     return fail(ctx, TPRF_ERR_ARG, "fourier_r_to_q_adjoint: bad argument");
@@ -1440,6 +1531,7 @@ int tprf_fourier_r_to_q_adjoint(tprf_ctx *ctx, const int dims[3], const int q[3]
 
 int tprf_solve_rpa_PH_adjoint(tprf_ctx *ctx, long npts, int norb, const tprf_c128 *chi0, const tprf_c128 *U,
                               const tprf_c128 *chibar, tprf_c128 *chi, tprf_c128 *chi0bar, tprf_c128 *Ubar) {
+  TPRF_TRACE(ctx, "tprf_solve_rpa_PH_adjoint");
   if (!ctx) return TPRF_ERR_ARG;
   if (npts < 1 || !chi0 || !U || !chi || (chibar && (!chi0bar || !Ubar)))
     return fail(ctx, TPRF_ERR_ARG, "solve_rpa_PH_adjoint: bad argument");
@@ -1465,6 +1557,7 @@ int chain_ntau(const tprf_chain_desc &d) { return d.ntau > 0 ? d.ntau : 6 * d.n_
 // lattice_utils.py:241-266, then solve_rpa_PH), with the symmetric forms the
 // descriptor's flags allow.
 int chain_forward(tprf_ctx *ctx, const tprf_chain_desc &d, const tprf_chain_ptrs &p) {
+  TPRF_TRACE(ctx, "tprf chain instance");
   const int n = d.norb, ntau = chain_ntau(d), L = ntau - 1;
   const long nk = chain_nk(d), n4 = (long)n * n * n * n;
   const bool herm = (d.flags & TPRF_CHAIN_HERMITIAN_EK) != 0;
@@ -1507,6 +1600,7 @@ int tprf_chain_buffer_sizes(const tprf_chain_desc *d, long counts[5]) {
 
 int tprf_forward_batched(tprf_ctx *ctx, int n_inst, const tprf_chain_desc *desc, const tprf_chain_ptrs *ptrs,
                          int nstreams) {
+  TPRF_TRACE(ctx, "tprf_forward_batched");
   if (!ctx) return TPRF_ERR_ARG;
   if (n_inst < 1 || !desc || !ptrs || nstreams < 1 || nstreams > 16)
     return fail(ctx, TPRF_ERR_ARG, "forward_batched: bad argument");
