@@ -408,7 +408,7 @@ int tprf_fourier_wr_to_tr_adjoint(tprf_ctx *ctx, int statistic, double beta, int
 /* nu = 0 row of chi0_wr_from_grt_PH (chi_imtime.cpp:139-182): the bosonic
  * tau -> nu transform at nu = 0 is a real linear functional
  * chi0_abcd(r) = sum_j c_j G_da(tau_j, r) G_bc(tau_{L-j}, -r);
- * g_tr[ntau][nbatch][nr][n][n] -> chi0_r[nbatch][nr][n^4].  norb <= 3.       */
+ * g_tr[ntau][nbatch][nr][n][n] -> chi0_r[nbatch][nr][n^4].  norb <= 4.       */
 int tprf_chi0_static_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int dims[3], int norb, long nbatch,
                                  const tprf_c128 *g_tr, tprf_c128 *chi0_r);
 /* Adjoint of tprf_chi0_static_from_grt_PH with respect to g_tr:

@@ -471,19 +471,24 @@ def test_rpa_point_adjoint_vs_torch(T, n):
     assert relerr(host(Ubar), Ut.grad.numpy()) <= 1e-12
 
 
-def test_three_orbital_static_gradient_vs_torch_reference(T):
+@pytest.mark.parametrize("n", [3, 4])
+def test_three_orbital_static_gradient_vs_torch_reference(T, n):
     """StaticRPA with three orbitals (fused chi0, the workgroup-per-point RPA
-    adjoint): chi_RPA(Q, 0) and the (e_k, mu, U) gradients of Re tr chi against
-    the FP64 torch reference on a 4x4x2 lattice, 1e-9."""
+    adjoint) and four (the default tau route for n = 4: the tau-space static
+    bubble and its adjoint, the Dyson adjoint at n = 4): chi_RPA(Q, 0) and the
+    (e_k, mu, U) gradients of Re tr chi against the FP64 torch reference on a
+    4x4x2 lattice, 1e-9."""
     from oracle.torch_ref import StaticChainRef
     from tprf_amd import autograd as A
-    rng = np.random.default_rng(29)
-    beta, n_iw, dims, q, n = 4.0, 32, (4, 4, 2), (2, 1, 1), 3
+    rng = np.random.default_rng(29 + (n - 3) * 7)
+    beta, n_iw, dims, q = 4.0, 32, (4, 4, 2), (2, 1, 1)
     nk = int(np.prod(dims))
     h = rnd(rng, nk, n, n) * 0.4
-    e = (h + np.conj(np.swapaxes(h, 1, 2))) / 2 + np.diag([-0.5, 0.0, 0.6])[None]
+    diag = {3: [-0.5, 0.0, 0.6], 4: [-0.5, -0.1, 0.3, 0.6]}[n]
+    e = (h + np.conj(np.swapaxes(h, 1, 2))) / 2 + np.diag(diag)[None]
     U = rnd(rng, n, n, n, n) * 0.1
     spec = A.StaticRPA(beta, n_iw, dims, q, norb=n)
+    assert spec.route == ("fused" if n == 3 else "tau")
     e_g = T.tensor(e, device="cuda", requires_grad=True)
     U_g = T.tensor(U, device="cuda", requires_grad=True)
     mu_g = T.tensor(0.1, dtype=T.float64, device="cuda", requires_grad=True)

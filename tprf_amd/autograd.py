@@ -22,6 +22,8 @@ the same bilinear form per r (tprf_chi0_static_from_gwr_PH), then projects
 r -> Q; only G(iw, k) and chi0(Q, 0) are kept, G(iw, r) is recomputed.
 route="tau" runs fourier_wr_to_tr and the tau-space bubble, the same map
 evaluated the reference's way (and recomputes G(tau, r) in backward).
+The fused and Matsubara routes take n <= 3 orbitals, the tau route n <= 4;
+route=None picks "fused" for n <= 3 and "tau" for n = 4.
 
 Gradients follow torch's complex convention (dL/dRe z + i dL/dIm z), so
 e_k, mu and U may come out of any torch expression (hoppings, Kanamori
@@ -47,7 +49,7 @@ class StaticRPA:
     launch over the whole batch; ``graphed_grad`` captures forward + backward of
     L = sum_i Re tr chi_i in one HIP graph."""
 
-    def __init__(self, beta, n_iw, dims, q, norb, ntau=None, nbatch=1, device=None, route="fused"):
+    def __init__(self, beta, n_iw, dims, q, norb, ntau=None, nbatch=1, device=None, route=None):
         self.beta = float(beta)
         self.n_iw = int(n_iw)
         self.dims = _dims3(dims)
@@ -60,6 +62,8 @@ class StaticRPA:
         dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
         self.dev = dev
         n, nk, B, c128 = self.n, self.nk, self.nb, torch.complex128
+        if route is None:
+            route = "fused" if n <= 3 else "tau"
         if route not in ("fused", "matsubara", "tau"):
             raise ValueError("route must be 'fused', 'matsubara' or 'tau'")
         self.route = route

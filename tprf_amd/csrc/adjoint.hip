@@ -502,6 +502,7 @@ hipError_t launch_bubble_static_adjoint(hipStream_t s, int ntau, const int dims[
     case 1: TPRF_SA(1); break;
     case 2: TPRF_SA(2); break;
     case 3: TPRF_SA(3); break;
+    case 4: TPRF_SA(4); break;
     default: return hipErrorInvalidValue;
   }
 #undef TPRF_SA
@@ -548,6 +549,7 @@ hipError_t launch_bubble_static(hipStream_t s, int ntau, const int dims[3], int 
     case 1: TPRF_SF(1); break;
     case 2: TPRF_SF(2); break;
     case 3: TPRF_SF(3); break;
+    case 4: TPRF_SF(4); break;
     default: return hipErrorInvalidValue;
   }
 #undef TPRF_SF

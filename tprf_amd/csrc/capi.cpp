@@ -1345,7 +1345,7 @@ int tprf_chi0_static_from_grt_PH(tprf_ctx *ctx, double beta, int ntau, const int
   if (!ctx) return TPRF_ERR_ARG;
   if (!(beta > 0) || ntau < 18 || !dims_ok(dims) || nbatch < 1 || !g_tr || !chi0_r)
     return fail(ctx, TPRF_ERR_ARG, "chi0_static_from_grt_PH: bad argument");
-  if (norb < 1 || norb > 3) return fail(ctx, TPRF_ERR_UNSUPPORTED, "chi0_static_from_grt_PH: norb must be 1..3");
+  if (norb < 1 || norb > 4) return fail(ctx, TPRF_ERR_UNSUPPORTED, "chi0_static_from_grt_PH: norb must be 1..4");
   const int L = ntau - 1;
   const double *cw;
   int st;
@@ -1371,8 +1371,8 @@ int tprf_chi0_static_from_grt_PH_adjoint(tprf_ctx *ctx, double beta, int ntau, c
   if (!ctx) return TPRF_ERR_ARG;
   if (!(beta > 0) || ntau < 18 || !dims_ok(dims) || nbatch < 1 || !g_tr || !chibar_r || !gbar_tr)
     return fail(ctx, TPRF_ERR_ARG, "chi0_static_from_grt_PH_adjoint: bad argument");
-  if (norb < 1 || norb > 3)
-    return fail(ctx, TPRF_ERR_UNSUPPORTED, "chi0_static_from_grt_PH_adjoint: norb must be 1..3");
+  if (norb < 1 || norb > 4)
+    return fail(ctx, TPRF_ERR_UNSUPPORTED, "chi0_static_from_grt_PH_adjoint: norb must be 1..4");
   const double *cw;
   int st;
   if ((st = get_static_weights(ctx, beta, ntau - 1, &cw))) return st;
