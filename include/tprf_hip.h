@@ -110,6 +110,18 @@ int tprf_ctx_tail_status_reset(tprf_ctx *ctx);
  * e_k[nk][n][n], sigma per sigma_kind, out g_wk[2 n_iw][nk][n][n].  n <= 8.  */
 int tprf_lattice_dyson_g_wk(tprf_ctx *ctx, double beta, int n_iw, double mu, long nk, int norb,
                             const tprf_c128 *e_k, int sigma_kind, const tprf_c128 *sigma, tprf_c128 *g_wk);
+/* The same on a mesh given by its frequencies z[nz] (device): G[z][k] =
+ * inverse((z + mu) 1 - e_k - Sigma).  With z = omega + i delta this is
+ * lattice_dyson_g0_fk / lattice_dyson_g_fk (gf.hpp:103,204,225; gf.cpp:71-86,
+ * 92-123, 142-148, the real-frequency mesh with its broadening); sigma per sigma_kind
+ * ([nz][n][n] or [nz][nk][n][n]).  out g_zk[nz][nk][n][n].  n <= 8.         */
+int tprf_lattice_dyson_g_zk(tprf_ctx *ctx, long nz, const tprf_c128 *z, double mu, long nk, int norb,
+                            const tprf_c128 *e_k, int sigma_kind, const tprf_c128 *sigma, tprf_c128 *g_zk);
+/* Its k average, the local Green's function g_z[nz][n][n] = (1/nk) sum_k G[z][k]
+ * (lattice_dyson_g_w, lattice_dyson_g_f: gf.hpp:245,267, gf.cpp:153-204; the
+ * k sum in index order).  Uses nz nk n^2 complex entries of scratch.        */
+int tprf_lattice_dyson_g_z(tprf_ctx *ctx, long nz, const tprf_c128 *z, double mu, long nk, int norb,
+                           const tprf_c128 *e_k, int sigma_kind, const tprf_c128 *sigma, tprf_c128 *g_z);
 
 /* a1 followed by a2 in one call, Sigma = 0: lattice_dyson_g0_wk (gf.cpp:41-58)
  * then fourier_wk_to_wr (gf.cpp:209, fourier_lattice.cpp:39,49), as the

@@ -133,6 +133,31 @@ def lattice_dyson_g_wk(beta, n_iw, mu, e_k, sigma=None):
     return out
 
 
+def lattice_dyson_g_zk(z, mu, e_k, sigma=None):
+    """G[z, k] = inverse((z + mu) 1 - e_k - Sigma) on given frequencies z (real
+    frequencies with their i delta: lattice_dyson_g0_fk / g_fk, gf.cpp:71-86,
+    92-123, 142-148); sigma None, [nz, n, n] or [nz, nk, n, n].  Restated with
+    numpy's inverse, point by point."""
+    z = np.asarray(z, dtype=np.complex128)
+    e_k = np.asarray(e_k, dtype=np.complex128)
+    nk, n = e_k.shape[0], e_k.shape[1]
+    M = (z[:, None, None, None] + mu) * np.eye(n)[None, None] - e_k[None]
+    if sigma is not None:
+        sigma = np.asarray(sigma, dtype=np.complex128)
+        M = M - (sigma[:, None] if sigma.ndim == 3 else sigma)
+    return np.linalg.inv(M)
+
+
+def lattice_dyson_g_z(z, mu, e_k, sigma=None):
+    """The k average (1/N_k) sum_k G[z, k], summed in k order as lattice_dyson_g_X
+    accumulates (gf.cpp:153-204: lattice_dyson_g_w, lattice_dyson_g_f)."""
+    g = lattice_dyson_g_zk(z, mu, e_k, sigma)
+    acc = np.zeros((g.shape[0],) + g.shape[2:], dtype=np.complex128)
+    for k in range(g.shape[1]):
+        acc += g[:, k]
+    return acc / g.shape[1]
+
+
 def fourier_lattice(x, dims, direction):
     """x[nouter, nk, *target]; direction -1 = k->r (scaled 1/N_k), +1 = r->k."""
     x = _c(x)

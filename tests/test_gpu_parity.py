@@ -68,6 +68,45 @@ def test_dyson(L, oracle, n, sigma):
     assert relerr(host(got.data), ref) <= 1e-13
 
 
+@pytest.mark.parametrize("n", [1, 2, 3, 7])
+def test_dyson_real_frequency_and_local(L, oracle, n):
+    """lattice_dyson_g0_fk, lattice_dyson_g_fk (Sigma(f, k) and Sigma(f)),
+    lattice_dyson_g_f and lattice_dyson_g_w (test/python/gf.py:150-232 and
+    :112-130 with random inputs) against the oracle, and fourier_fk_to_fr /
+    fourier_fr_to_fk as a round trip on the real-frequency mesh."""
+    from tprf_amd.gf import Gf, MeshBrZone, MeshImFreq, MeshProduct, MeshReFreq
+    rng = np.random.default_rng(70 + n)
+    dims, mu, delta = (4, 2, 1), 0.2, 0.01
+    nk = 8
+    e = hermitian_ek(rng, nk, n)
+    e_k = kmesh_gf(e, dims)
+    fmesh = MeshReFreq(-5.0, 5.0, 41)
+    z = fmesh.values() + 1j * delta
+    got = L.lattice_dyson_g0_fk(mu, e_k, fmesh, delta)
+    assert relerr(host(got.data), oracle.lattice_dyson_g_zk(z, mu, e)) <= 1e-13
+    s_fk = rnd(rng, fmesh.size, nk, n, n) * 0.3
+    s_f = rnd(rng, fmesh.size, n, n) * 0.3
+    got = L.lattice_dyson_g_fk(mu, e_k, sigma_fk=Gf(MeshProduct(fmesh, MeshBrZone(dims)), data=gpu(s_fk)),
+                               delta=delta)
+    assert relerr(host(got.data), oracle.lattice_dyson_g_zk(z, mu, e, s_fk)) <= 1e-13
+    sf = Gf(fmesh, data=gpu(s_f))
+    got = L.lattice_dyson_g_fk(mu, e_k, sigma_f=sf, delta=delta)
+    assert relerr(host(got.data), oracle.lattice_dyson_g_zk(z, mu, e, s_f)) <= 1e-13
+    got = L.lattice_dyson_g_f(mu, e_k, sf, delta)
+    assert relerr(host(got.data), oracle.lattice_dyson_g_z(z, mu, e, s_f)) <= 1e-13
+    wmesh = MeshImFreq(5.0, "Fermion", 16)
+    s_w = rnd(rng, wmesh.size, n, n) * 0.3
+    got = L.lattice_dyson_g_w(mu, e_k, Gf(wmesh, data=gpu(s_w)))
+    ref = oracle.lattice_dyson_g_z(wmesh.values(), mu, e, s_w)
+    assert relerr(host(got.data), ref) <= 1e-13
+    # the Matsubara local G equals the k average of lattice_dyson_g_wk
+    g_wk = oracle.lattice_dyson_g_wk(5.0, 16, mu, e, s_w)
+    assert relerr(host(got.data), g_wk.mean(axis=1)) <= 1e-13
+    g_fk = L.lattice_dyson_g0_fk(mu, e_k, fmesh, delta)
+    back = L.fourier_fr_to_fk(L.fourier_fk_to_fr(g_fk))
+    assert relerr(host(back.data), host(g_fk.data)) <= 1e-14
+
+
 # ---------------------------------------------------------------- a1 + a2 in one call
 @pytest.mark.parametrize("dims,n,beta,n_iw", [((16, 16, 16), 1, 10.0, 16), ((16, 16, 16), 2, 20.0, 32),
                                               ((16, 16, 16), 3, 40.0, 64), ((16, 16, 16), 3, 40.0, 256),

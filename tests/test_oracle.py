@@ -486,3 +486,24 @@ def test_screened_W_dyson_and_single_band(oracle, n):
     # the dynamic overload with V repeated over frequencies equals the static one
     Vw = np.ascontiguousarray(np.broadcast_to(V[None], PI.shape))
     assert relerr(oracle.screened_interaction_W(PI, Vw, dynamic=True), W) <= 1e-14
+
+
+def test_dyson_given_frequencies_pins(oracle):
+    """The numpy restatement on given frequencies: at z = i w_n it equals the C
+    oracle's lattice_dyson_g_wk (an independent restatement), for n = 1 it is
+    1 / (z + mu - e - Sigma), and the k average is the mean over k."""
+    rng = np.random.default_rng(5)
+    beta, n_iw, mu, nk, n = 7.0, 6, 0.3, 5, 3
+    a = rnd(rng, nk, n, n)
+    e = 0.5 * (a + np.conj(np.swapaxes(a, 1, 2)))
+    sw = rnd(rng, 2 * n_iw, n, n) * 0.2
+    iw = 1j * np.pi * (2 * np.arange(-n_iw, n_iw) + 1) / beta
+    ref = oracle.lattice_dyson_g_wk(beta, n_iw, mu, e, sw)
+    assert relerr(oracle.lattice_dyson_g_zk(iw, mu, e, sw), ref) <= 1e-13
+    z = np.linspace(-3, 3, 7) + 0.05j
+    e1 = rng.normal(size=(nk, 1, 1)) + 0j
+    s1 = rnd(rng, 7, nk, 1, 1) * 0.1
+    closed = 1.0 / (z[:, None] + mu - e1[None, :, 0, 0] - s1[..., 0, 0])
+    assert relerr(oracle.lattice_dyson_g_zk(z, mu, e1, s1)[..., 0, 0], closed) <= 1e-14
+    gz = oracle.lattice_dyson_g_zk(z, mu, e)
+    assert relerr(oracle.lattice_dyson_g_z(z, mu, e), gz.mean(axis=1)) <= 1e-14

@@ -57,6 +57,8 @@ _SIGNATURES = {
     "tprf_ctx_set_workspace": [_p, _p, ctypes.c_size_t],
     "tprf_ctx_workspace_size": [_p, ctypes.POINTER(ctypes.c_size_t), _i],
     "tprf_lattice_dyson_g_wk": [_p, _d, _i, _d, _l, _i, _p, _i, _p, _p],
+    "tprf_lattice_dyson_g_zk": [_p, _l, _p, _d, _l, _i, _p, _i, _p, _p],
+    "tprf_lattice_dyson_g_z": [_p, _l, _p, _d, _l, _i, _p, _i, _p, _p],
     "tprf_lattice_dyson_g_wr": [_p, _d, _i, _d, _p, _i, _p, _p],
     "tprf_lattice_dyson_g_wr_herm": [_p, _d, _i, _d, _p, _i, _p, _p, _i],
     "tprf_fourier_wk_to_wr": [_p, _l, _p, _l, _p, _p],

@@ -696,6 +696,42 @@ int tprf_lattice_dyson_g_wk(tprf_ctx *ctx, double beta, int n_iw, double mu, lon
                       "dyson kernel");
 }
 
+int tprf_lattice_dyson_g_zk(tprf_ctx *ctx, long nz, const tprf_c128 *z, double mu, long nk, int norb,
+                            const tprf_c128 *e_k, int sigma_kind, const tprf_c128 *sigma, tprf_c128 *g_zk) {
+  TPRF_TRACE(ctx, "tprf_lattice_dyson_g_zk");
+  if (!ctx) return TPRF_ERR_ARG;
+  if (nz < 1 || !z || nk < 1 || !e_k || !g_zk) return fail(ctx, TPRF_ERR_ARG, "lattice_dyson_g_zk: bad argument");
+  if (sigma_kind < 0 || sigma_kind > 2 || (sigma_kind && !sigma))
+    return fail(ctx, TPRF_ERR_ARG, "lattice_dyson_g_zk: bad sigma");
+  if (norb < 1 || norb > 8) return fail(ctx, TPRF_ERR_UNSUPPORTED, "lattice_dyson_g_zk: norb must be 1..8");
+  return check_launch(ctx,
+                      tprf::launch_dyson(ctx->stream, 1.0, 1, mu, nk, norb, (const c2 *)e_k, sigma_kind,
+                                         (const c2 *)sigma, (c2 *)g_zk, tprf::Status{ctx->d_status}, nullptr, 1,
+                                         (const c2 *)z, nz),
+                      "dyson kernel");
+}
+
+int tprf_lattice_dyson_g_z(tprf_ctx *ctx, long nz, const tprf_c128 *z, double mu, long nk, int norb,
+                           const tprf_c128 *e_k, int sigma_kind, const tprf_c128 *sigma, tprf_c128 *g_z) {
+  TPRF_TRACE(ctx, "tprf_lattice_dyson_g_z");
+  if (!ctx) return TPRF_ERR_ARG;
+  if (nz < 1 || !z || nk < 1 || !e_k || !g_z) return fail(ctx, TPRF_ERR_ARG, "lattice_dyson_g_z: bad argument");
+  if (sigma_kind < 0 || sigma_kind > 2 || (sigma_kind && !sigma))
+    return fail(ctx, TPRF_ERR_ARG, "lattice_dyson_g_z: bad sigma");
+  if (norb < 1 || norb > 8) return fail(ctx, TPRF_ERR_UNSUPPORTED, "lattice_dyson_g_z: norb must be 1..8");
+  void *w;
+  int st = get_work(ctx, sizeof(c2) * (size_t)nz * nk * norb * norb, &w);
+  if (st) return st;
+  st = check_launch(ctx,
+                    tprf::launch_dyson(ctx->stream, 1.0, 1, mu, nk, norb, (const c2 *)e_k, sigma_kind,
+                                       (const c2 *)sigma, (c2 *)w, tprf::Status{ctx->d_status}, nullptr, 1,
+                                       (const c2 *)z, nz),
+                    "dyson kernel");
+  if (st) return st;
+  return check_launch(ctx, tprf::launch_k_average(ctx->stream, nz, nk, norb, (const c2 *)w, (c2 *)g_z),
+                      "k average kernel");
+}
+
 namespace {
 
 // sym: 0 general, 1 Hermitian e_k (full output), 2 Hermitian e_k, columns of r <= -r only

@@ -133,8 +133,12 @@ struct Status {
 // ---- launchers (kernels.hip) -------------------------------------------
 // mu_arr (optional, device): one mu per lattice of a batch folded into k, nk_inst
 // k-points each (k = inst * nk_inst + k_local); mu is then ignored
+// zarr != nullptr: the nz frequencies z[i] in place of the Matsubara mesh
 hipError_t launch_dyson(hipStream_t s, double beta, int n_iw, double mu, long nk, int n, const c2 *ek, int sigma_kind,
-                        const c2 *sigma, c2 *g, Status st, const double *mu_arr = nullptr, long nk_inst = 1);
+                        const c2 *sigma, c2 *g, Status st, const double *mu_arr = nullptr, long nk_inst = 1,
+                        const c2 *zarr = nullptr, long nz = 0);
+// out[z][ab] = (1/nk) sum_k g[z][k][ab]
+hipError_t launch_k_average(hipStream_t s, long nz, long nk, int n, const c2 *g, c2 *out);
 // Dyson (Sigma = 0) fused into the 16^3 k -> r transform, n <= 3:
 // e_k[4096][n][n] -> g_wr[2 n_iw][4096][n][n]
 // variant: 0 = one orbital row per workgroup, 1 = one column (same results)

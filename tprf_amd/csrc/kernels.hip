@@ -12,16 +12,24 @@ namespace tprf {
 // ------------------------------------------------------------------------
 // a1 Dyson: G[w,k] = inverse((i w_n + mu) 1 - e_k - Sigma), one thread per
 // (w, k); Gauss-Jordan with partial pivoting on [M | 1] in registers.
+// zarr: the frequencies z[iw] of any mesh (real frequencies with their i delta,
+// or given Matsubara points) in place of i w_n: M = (z + mu) 1 - e_k - Sigma.
 template <int N>
 __global__ void __launch_bounds__(256) k_dyson(double beta, int n_iw, double mu, const double *__restrict__ mu_arr,
                                                long nk_inst, long nk, const c2 *__restrict__ ek, int sigma_kind,
                                                const c2 *__restrict__ sigma, c2 *__restrict__ g, long npts,
-                                               double *status) {
+                                               double *status, const c2 *__restrict__ zarr) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= npts) return;
   const long iw = idx / nk, k = idx - iw * nk;
   if (mu_arr) mu = mu_arr[k / nk_inst];  // a batch of lattices folded into k
-  const double wn = kPi * (double)(2 * (iw - n_iw) + 1) / beta;
+  double zr = 0.0, wn;
+  if (zarr) {
+    zr = zarr[iw].x;
+    wn = zarr[iw].y;
+  } else {
+    wn = kPi * (double)(2 * (iw - n_iw) + 1) / beta;
+  }
   c2 A[N][N], B[N][N];
   const c2 *e = ek + k * N * N;
   // all n^2 entries of e_k (and of Sigma) are loaded before the first is used
@@ -38,7 +46,7 @@ __global__ void __launch_bounds__(256) k_dyson(double beta, int n_iw, double mu,
 #pragma unroll
     for (int j = 0; j < N; ++j) {
       c2 v = c2{-ev[i * N + j].x, -ev[i * N + j].y};
-      if (i == j) { v.x += mu; v.y += wn; }
+      if (i == j) { v.x += mu + zr; v.y += wn; }
       if (sigma_kind != 0) v = csub(v, c2{sv[i * N + j].x, sv[i * N + j].y});
       A[i][j] = v;
       B[i][j] = (i == j) ? c2{1.0, 0.0} : c2{0.0, 0.0};
@@ -97,7 +105,7 @@ template <int N>
 __global__ void __launch_bounds__(256) k_dyson_rows(double beta, int n_iw, double mu, const double *__restrict__ mu_arr,
                                                     long nk_inst, long nk, const c2 *__restrict__ ek, int sigma_kind,
                                                     const c2 *__restrict__ sigma, c2 *__restrict__ g, long npts,
-                                                    double *status) {
+                                                    double *status, const c2 *__restrict__ zarr) {
   constexpr int P = 64 / N;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int gidx = lane / N, i = lane - gidx * N;
@@ -106,8 +114,8 @@ __global__ void __launch_bounds__(256) k_dyson_rows(double beta, int n_iw, doubl
   c2 A[N], B[N];
   if (valid) {
     const long iw = idx / nk, k = idx - iw * nk;
-    const double m = mu_arr ? mu_arr[k / nk_inst] : mu;
-    const double wn = kPi * (double)(2 * (iw - n_iw) + 1) / beta;
+    const double m = (mu_arr ? mu_arr[k / nk_inst] : mu) + (zarr ? zarr[iw].x : 0.0);
+    const double wn = zarr ? zarr[iw].y : kPi * (double)(2 * (iw - n_iw) + 1) / beta;
     const c2 *e = ek + k * N * N + i * N;
     const c2 *sg = sigma_kind == 0 ? nullptr : sigma + (sigma_kind == 1 ? iw : idx) * N * N + i * N;
 #pragma unroll
@@ -167,17 +175,37 @@ __global__ void __launch_bounds__(256) k_dyson_rows(double beta, int n_iw, doubl
   }
 }
 
+// out[z][ab] = (1/nk) sum_k g[z][k][ab], the k sum in index order (the order of
+// lattice_dyson_g_X's accumulation, gf.cpp:153-190), thread per (z, ab).
+__global__ void __launch_bounds__(256) k_k_average(long nz, long nk, int n2, const c2 *__restrict__ g,
+                                                   c2 *__restrict__ out) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nz * n2) return;
+  const long z = t / n2;
+  const int ab = (int)(t - z * n2);
+  const c2 *p = g + z * nk * n2 + ab;
+  c2 acc = c2{0.0, 0.0};
+  for (long k = 0; k < nk; ++k) acc = cadd(acc, p[k * n2]);
+  out[t] = cscale(acc, 1.0 / (double)nk);
+}
+
+hipError_t launch_k_average(hipStream_t s, long nz, long nk, int n, const c2 *g, c2 *out) {
+  const long nt = nz * n * n;
+  hipLaunchKernelGGL(k_k_average, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, nz, nk, n * n, g, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_dyson(hipStream_t s, double beta, int n_iw, double mu, long nk, int n, const c2 *ek, int sigma_kind,
-                        const c2 *sigma, c2 *g, Status st, const double *mu_arr, long nk_inst) {
-  const long npts = 2L * n_iw * nk;
+                        const c2 *sigma, c2 *g, Status st, const double *mu_arr, long nk_inst, const c2 *zarr,
+                        long nz) {
+  const long npts = (zarr ? nz : 2L * n_iw) * nk;
   const int bs = 256;
   const long nb = (npts + bs - 1) / bs;
   switch (n) {
 #define TPRF_DYSON_CASE(N)                                                                               \
   case N:                                                                                                \
     hipLaunchKernelGGL(k_dyson<N>, dim3(nb), dim3(bs), 0, s, beta, n_iw, mu, mu_arr, nk_inst, nk, ek, sigma_kind, \
-                       sigma, g,                                                                         \
-                       npts, st.d);                                                                      \
+                       sigma, g, npts, st.d, zarr);                                                      \
     break;
     TPRF_DYSON_CASE(1)
     TPRF_DYSON_CASE(2)
@@ -190,7 +218,7 @@ hipError_t launch_dyson(hipStream_t s, double beta, int n_iw, double mu, long nk
   case N: {                                                                                                  \
     const long nbr = (npts + 4 * (64 / N) - 1) / (4 * (64 / N));                                             \
     hipLaunchKernelGGL(k_dyson_rows<N>, dim3(nbr), dim3(256), 0, s, beta, n_iw, mu, mu_arr, nk_inst, nk, ek,  \
-                       sigma_kind, sigma, g, npts, st.d);                                                    \
+                       sigma_kind, sigma, g, npts, st.d, zarr);                                              \
     break;                                                                                                   \
   }
     TPRF_DYSON_ROWS(7)

@@ -81,6 +81,74 @@ def _dyson(mu, e_k, wmesh, sigma):
     return Gf(MeshProduct(wmesh, kmesh), data=out)
 
 
+def _dyson_z(mu, e_k, fmesh, z, sigma, local):
+    """G[z,k] = [(z + mu) 1 - e_k - Sigma]^-1 on the frequencies z of fmesh, or
+    its k average (local)."""
+    kmesh = e_k.mesh
+    n = e_k.target_shape[0]
+    ek = _dev(e_k.data)
+    c = _lib.ctx(ek.device.index)
+    zt = torch.from_numpy(np.ascontiguousarray(z, dtype=np.complex128)).to(ek.device)
+    kind, sp = _lib.SIGMA_NONE, None
+    if sigma is not None:
+        sd = _dev(sigma.data)
+        kind = _lib.SIGMA_WK if isinstance(sigma.mesh, MeshProduct) else _lib.SIGMA_W
+        smesh = sigma.mesh[0] if kind == _lib.SIGMA_WK else sigma.mesh
+        if smesh.size != fmesh.size or (kind == _lib.SIGMA_WK and sigma.mesh[1].size != kmesh.size):
+            raise ValueError("sigma's mesh does not match the frequency mesh and e_k")
+        sp = _lib.ptr(sd)
+    shape = (fmesh.size, n, n) if local else (fmesh.size, kmesh.size, n, n)
+    out = torch.empty(shape, dtype=torch.complex128, device=ek.device)
+    _lib.tail_status_reset(c)
+    _lib.call("tprf_lattice_dyson_g_z" if local else "tprf_lattice_dyson_g_zk", c, fmesh.size, _lib.ptr(zt),
+              float(mu), kmesh.size, n, _lib.ptr(ek), kind, sp, _lib.ptr(out))
+    _lib.tail_status(c)  # raises on a singular matrix
+    return Gf(fmesh if local else MeshProduct(fmesh, kmesh), data=out)
+
+
+def _real_freq_sigma(sigma, sigma_fk, sigma_f):
+    given = [x for x in (sigma, sigma_fk, sigma_f) if x is not None]
+    if len(given) != 1:
+        raise TypeError("give exactly one self-energy")
+    return given[0]
+
+
+def lattice_dyson_g0_fk(mu, e_k, mesh, delta):
+    """G0[f,k] = [(omega + i delta + mu) 1 - e_k]^-1 on a real-frequency mesh
+    (gf.cpp:71-86)."""
+    if not isinstance(mesh, MeshReFreq):
+        raise ValueError("lattice_dyson_g0_fk needs a real-frequency mesh")
+    return _dyson_z(mu, e_k, mesh, mesh.values() + 1j * float(delta), None, False)
+
+
+def lattice_dyson_g_fk(mu, e_k, sigma=None, delta=0.0, sigma_fk=None, sigma_f=None):
+    """G[f,k] = [(omega + i delta + mu) 1 - e_k - Sigma]^-1 with Sigma(f, k) or
+    Sigma(f) (gf.cpp:92-123, 142-148)."""
+    s = _real_freq_sigma(sigma, sigma_fk, sigma_f)
+    fmesh = s.mesh[0] if isinstance(s.mesh, MeshProduct) else s.mesh
+    if not isinstance(fmesh, MeshReFreq):
+        raise ValueError("lattice_dyson_g_fk needs a real-frequency self-energy")
+    return _dyson_z(mu, e_k, fmesh, fmesh.values() + 1j * float(delta), s, False)
+
+
+def lattice_dyson_g_w(mu, e_k, sigma_w):
+    """Local G(i w_n) = (1/N_k) sum_k [(i w_n + mu) 1 - e_k - Sigma(i w_n)]^-1
+    (gf.cpp:153-196)."""
+    wmesh = sigma_w.mesh
+    if not isinstance(wmesh, MeshImFreq) or wmesh.statistic != FERMION:
+        raise ValueError("lattice_dyson_g_w needs a fermionic Matsubara self-energy")
+    return _dyson_z(mu, e_k, wmesh, wmesh.values(), sigma_w, True)
+
+
+def lattice_dyson_g_f(mu, e_k, sigma_f, delta):
+    """Local G(omega) = (1/N_k) sum_k [(omega + i delta + mu) 1 - e_k - Sigma(omega)]^-1
+    (gf.cpp:153-204)."""
+    fmesh = sigma_f.mesh
+    if not isinstance(fmesh, MeshReFreq):
+        raise ValueError("lattice_dyson_g_f needs a real-frequency self-energy")
+    return _dyson_z(mu, e_k, fmesh, fmesh.values() + 1j * float(delta), sigma_f, True)
+
+
 # ---------------------------------------------------------------------------
 # a2 / a6 lattice FFTs -- gf.cpp:209-215, chi_imtime.cpp:289-297 -> fourier.hpp:165-231
 
@@ -108,6 +176,16 @@ def fourier_wk_to_wr(g_wk):
 def fourier_wr_to_wk(g_wr):
     """G(w, k) = sum_r e^{+i k.r} G(w, r) (FFTW_BACKWARD, no scale)."""
     return _lattice(g_wr, False)
+
+
+def fourier_fk_to_fr(g_fk):
+    """The k -> r transform on a real-frequency mesh (gf.cpp:229-232)."""
+    return _lattice(g_fk, True)
+
+
+def fourier_fr_to_fk(g_fr):
+    """The r -> k transform on a real-frequency mesh (gf.cpp:234-237)."""
+    return _lattice(g_fr, False)
 
 
 def chi_wk_from_chi_wr(chi_wr):
