@@ -28,6 +28,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <array>
+#include <cmath>
 #include <complex>
 #include <iostream>
 #include <memory>
@@ -136,6 +137,15 @@ struct lattice_dims {
  * (i0 d1 + i1) d2 + i2 (d2 = 1 in 2-D). */
 struct brzone : lattice_dims {};
 struct cyclat : lattice_dims {};
+/* Real-frequency mesh: n_w points omega_min .. omega_max, both included. */
+struct refreq {
+  double omega_min = -1, omega_max = 1;
+  int n_w = 2;
+  long size() const { return n_w; }
+  double value(long i) const {
+    return n_w == 1 ? omega_min : omega_min + (omega_max - omega_min) * (double)i / (double)(n_w - 1);
+  }
+};
 /* The single point of a mesh-less factor (e_k, Sigma(w) have one mesh). */
 struct none {
   long size() const { return 1; }
@@ -202,6 +212,12 @@ using g_tr_t = gf<mesh::imtime, mesh::cyclat, 2>;
 using chi_tr_t = gf<mesh::imtime, mesh::cyclat, 4>;
 using chi_wr_t = gf<mesh::imfreq, mesh::cyclat, 4>;
 using chi_wk_t = gf<mesh::imfreq, mesh::brzone, 4>;
+using g_f_t = gf<mesh::refreq, mesh::none, 2>;
+using g_fk_t = gf<mesh::refreq, mesh::brzone, 2>;
+using g_fr_t = gf<mesh::refreq, mesh::cyclat, 2>;
+using g_f_cvt = const g_f_t &;
+using g_fk_cvt = const g_fk_t &;
+using g_fr_cvt = const g_fr_t &;
 using e_k_cvt = const e_k_t &;
 using g_w_cvt = const g_w_t &;
 using g_wk_cvt = const g_wk_t &;
@@ -266,6 +282,78 @@ inline g_wk_t lattice_dyson_g_wk(double mu, e_k_cvt e_k, g_wk_cvt sigma_wk) {
   return g;
 }
 
+// ---- a1 on given frequencies: real frequencies and the local G (gf.hpp:103,204,225,245,267)
+namespace detail {
+/* The frequencies z_i (+ i delta) of a mesh as a device array. */
+inline std::shared_ptr<dcomplex> device_freqs(const std::vector<dcomplex> &z) {
+  dcomplex *p = nullptr;
+  hip_check(hipMalloc((void **)&p, sizeof(dcomplex) * z.size()), "hipMalloc");
+  std::shared_ptr<dcomplex> d(p, [](dcomplex *q) { (void)hipFree(q); });
+  hip_check(hipMemcpyAsync(p, z.data(), sizeof(dcomplex) * z.size(), hipMemcpyHostToDevice, state().stream),
+            "hipMemcpyAsync");
+  hip_check(hipStreamSynchronize(state().stream), "hipStreamSynchronize");
+  return d;
+}
+inline std::vector<dcomplex> freqs(const mesh::refreq &m, double delta) {
+  std::vector<dcomplex> z((size_t)m.size());
+  for (long i = 0; i < m.size(); ++i) z[(size_t)i] = dcomplex(m.value(i), delta);
+  return z;
+}
+inline std::vector<dcomplex> freqs(const mesh::imfreq &m) {
+  std::vector<dcomplex> z((size_t)m.size());
+  const long first = m.statistic == mesh::Fermion ? -m.n_max : -(m.n_max - 1);
+  const int odd = m.statistic == mesh::Fermion ? 1 : 0;
+  for (long i = 0; i < m.size(); ++i) z[(size_t)i] = dcomplex(0.0, M_PI * (double)(2 * (first + i) + odd) / m.beta);
+  return z;
+}
+template <class G>
+inline void dyson_z(const std::vector<dcomplex> &z, double mu, e_k_cvt e_k, int kind, const dcomplex *sigma, G &out,
+                    bool local, const char *what) {
+  tprf_ctx *cx = context();
+  auto zd = device_freqs(z);
+  check(tprf_ctx_tail_status_reset(cx), cx, "status reset");
+  auto f = local ? tprf_lattice_dyson_g_z : tprf_lattice_dyson_g_zk;
+  check(f(cx, (long)z.size(), c(zd.get()), mu, e_k.mesh0().size(), e_k.norb(), c(e_k.data()), kind, c(sigma),
+          c(out.data())),
+        cx, what);
+  check_tail(cx, false);
+  hip_check(hipStreamSynchronize(state().stream), "hipStreamSynchronize");  // z is freed on return
+}
+}  // namespace detail
+
+inline g_fk_t lattice_dyson_g0_fk(double mu, e_k_cvt e_k, mesh::refreq mesh, double delta) {
+  g_fk_t g(mesh, e_k.mesh0(), e_k.norb());
+  detail::dyson_z(detail::freqs(mesh, delta), mu, e_k, TPRF_SIGMA_NONE, nullptr, g, false, "lattice_dyson_g0_fk");
+  return g;
+}
+inline g_fk_t lattice_dyson_g_fk(double mu, e_k_cvt e_k, g_fk_cvt sigma_fk, double delta) {
+  if (sigma_fk.mesh1().size() != e_k.mesh0().size()) throw error(TPRF_ERR_ARG, "lattice_dyson_g_fk: k-mesh mismatch");
+  g_fk_t g(sigma_fk.mesh0(), e_k.mesh0(), e_k.norb());
+  detail::dyson_z(detail::freqs(sigma_fk.mesh0(), delta), mu, e_k, TPRF_SIGMA_WK, sigma_fk.data(), g, false,
+                  "lattice_dyson_g_fk");
+  return g;
+}
+inline g_fk_t lattice_dyson_g_fk(double mu, e_k_cvt e_k, g_f_cvt sigma_f, double delta) {
+  g_fk_t g(sigma_f.mesh0(), e_k.mesh0(), e_k.norb());
+  detail::dyson_z(detail::freqs(sigma_f.mesh0(), delta), mu, e_k, TPRF_SIGMA_W, sigma_f.data(), g, false,
+                  "lattice_dyson_g_fk");
+  return g;
+}
+inline g_w_t lattice_dyson_g_w(double mu, e_k_cvt e_k, g_w_cvt sigma_w) {
+  if (sigma_w.mesh0().statistic != mesh::Fermion)
+    throw error(TPRF_ERR_ARG, "lattice_dyson_g_w: fermionic mesh required");
+  g_w_t g(sigma_w.mesh0(), mesh::none{}, e_k.norb());
+  detail::dyson_z(detail::freqs(sigma_w.mesh0()), mu, e_k, TPRF_SIGMA_W, sigma_w.data(), g, true,
+                  "lattice_dyson_g_w");
+  return g;
+}
+inline g_f_t lattice_dyson_g_f(double mu, e_k_cvt e_k, g_f_cvt sigma_f, double delta) {
+  g_f_t g(sigma_f.mesh0(), mesh::none{}, e_k.norb());
+  detail::dyson_z(detail::freqs(sigma_f.mesh0(), delta), mu, e_k, TPRF_SIGMA_W, sigma_f.data(), g, true,
+                  "lattice_dyson_g_f");
+  return g;
+}
+
 // ---- a2 / a6 and inverses: lattice FFTs (gf.hpp:276,286; chi_imtime.hpp:123,141)
 inline g_wr_t fourier_wk_to_wr(g_wk_cvt g_wk) {
   tprf_ctx *cx = detail::context();
@@ -281,6 +369,24 @@ inline g_wk_t fourier_wr_to_wk(g_wr_cvt g_wr) {
   detail::check(tprf_fourier_wr_to_wk(cx, g_wr.mesh0().size(), g_wr.mesh1().dims.data(), g_wr.target_size(),
                                       detail::c(g_wr.data()), detail::c(out.data())),
                 cx, "fourier_wr_to_wk");
+  return out;
+}
+/* fourier_fk_to_fr / fourier_fr_to_fk (gf.hpp:316,325): the same transforms on a
+ * real-frequency mesh. */
+inline g_fr_t fourier_fk_to_fr(g_fk_cvt g_fk) {
+  tprf_ctx *cx = detail::context();
+  g_fr_t out(g_fk.mesh0(), detail::to_r(g_fk.mesh1()), g_fk.norb());
+  detail::check(tprf_fourier_wk_to_wr(cx, g_fk.mesh0().size(), g_fk.mesh1().dims.data(), g_fk.target_size(),
+                                      detail::c(g_fk.data()), detail::c(out.data())),
+                cx, "fourier_fk_to_fr");
+  return out;
+}
+inline g_fk_t fourier_fr_to_fk(g_fr_cvt g_fr) {
+  tprf_ctx *cx = detail::context();
+  g_fk_t out(g_fr.mesh0(), detail::to_k(g_fr.mesh1()), g_fr.norb());
+  detail::check(tprf_fourier_wr_to_wk(cx, g_fr.mesh0().size(), g_fr.mesh1().dims.data(), g_fr.target_size(),
+                                      detail::c(g_fr.data()), detail::c(out.data())),
+                cx, "fourier_fr_to_fk");
   return out;
 }
 inline chi_wk_t chi_wk_from_chi_wr(chi_wr_cvt chi_wr) {

@@ -50,7 +50,7 @@ def test_header_declares_reference_signatures():
     assert "construct_phi_wk(chi_wk_vt chi, array4_view U)" in text
 
 
-@pytest.mark.parametrize("prog", ["chain_cpp", "consumers_cpp"])
+@pytest.mark.parametrize("prog", ["chain_cpp", "consumers_cpp", "dyson_cpp"])
 def test_header_compiles_and_links_with_host_compiler(prog):
     lib = os.path.join(ROOT, "tprf_amd", "libtprf_hip.so")
     assert os.path.exists(lib), "build first"
@@ -156,3 +156,47 @@ def test_cpp_caller_gw_and_eliashberg_match_oracle(oracle):
     odyn, oconst = oracle.preprocess_gamma_for_fft(gamma(nw), beta, dims)
     assert relerr(df, oracle.eliashberg_product_fft(odyn, oconst, g, delta, beta, dims)) <= 1e-11
     assert np.abs(ds - df).max() <= 1e-9
+
+
+@pytest.mark.gpu
+def test_cpp_caller_real_frequency_dyson_matches_oracle(oracle):
+    """tests/cpp/dyson_cpp (lattice_dyson_g0_fk, both lattice_dyson_g_fk
+    overloads, lattice_dyson_g_f, lattice_dyson_g_w and the fk <-> fr round
+    trip through the C++ signatures) against the oracle, 1e-13."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("no GPU visible")
+    binp = os.path.join(ROOT, "tests", "cpp", "dyson_cpp")
+    assert os.path.exists(binp), "build first: tests/cpp/dyson_cpp"
+    rng = np.random.default_rng(12)
+    dims, n, nf, n_max = (3, 2, 1), 2, 21, 8
+    wmin, wmax, delta, beta, mu = -4.0, 4.0, 0.02, 6.0, 0.15
+    nk = int(np.prod(dims))
+    a = rng.normal(size=(nk, n, n)) + 1j * rng.normal(size=(nk, n, n))
+    e = 0.5 * (a + np.conj(np.swapaxes(a, 1, 2)))
+    s_fk = (rng.normal(size=(nf, nk, n, n)) + 1j * rng.normal(size=(nf, nk, n, n))) * 0.2
+    s_f = (rng.normal(size=(nf, n, n)) + 1j * rng.normal(size=(nf, n, n))) * 0.2
+    s_w = (rng.normal(size=(2 * n_max, n, n)) + 1j * rng.normal(size=(2 * n_max, n, n))) * 0.2
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as f:
+            np.array(list(dims) + [n, nf, n_max, 0, 0], dtype=np.int32).tofile(f)
+            np.array([wmin, wmax, delta, beta, mu], dtype=np.float64).tofile(f)
+            for x in (e, s_fk, s_f, s_w):
+                np.ascontiguousarray(x, dtype=np.complex128).tofile(f)
+        print(f"running tests/cpp/dyson_cpp ({n} orbitals, {nk} k, {nf} real frequencies)", flush=True)
+        r = subprocess.run([binp, fin, fout], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        out = np.fromfile(fout, dtype=np.complex128)
+    z = np.linspace(wmin, wmax, nf) + 1j * delta
+    iw = 1j * np.pi * (2 * np.arange(-n_max, n_max) + 1) / beta
+    refs = [oracle.lattice_dyson_g_zk(z, mu, e), oracle.lattice_dyson_g_zk(z, mu, e, s_fk),
+            oracle.lattice_dyson_g_zk(z, mu, e, s_f), oracle.lattice_dyson_g_z(z, mu, e, s_f),
+            oracle.lattice_dyson_g_z(iw, mu, e, s_w)]
+    refs.append(refs[0])
+    off = 0
+    for i, ref in enumerate(refs):
+        got = out[off:off + ref.size].reshape(ref.shape)
+        off += ref.size
+        assert relerr(got, ref) <= 1e-13, i
+    assert off == out.size
