@@ -450,7 +450,7 @@ def test_chi0_wr_pruned_pair_kernel(L, oracle, n, n_iw, nb, dims, ntau):
             _lib.call("tprf_ctx_set_option", c, 1, 0)
         assert relerr(host(got.data), ref) <= TOL, f"generic={generic}"
     # every workgroup shape the autotuner may pick (option 2 fixes it, -1 restores tuning)
-    for variant in range(7):
+    for variant in range(8):
         _lib.call("tprf_ctx_set_option", c, 2, variant)
         try:
             got = L.chi0_wr_from_grt_PH(gtr_gf(dims, beta, gt), nw=nb)
@@ -547,7 +547,7 @@ def test_chi0_wr_herm(L, oracle, dims, n, nb):
     g_half = g_tr.copy()
     g_half[:, ~canonical_r(dims)] = np.nan
     gt = gpu(g_half)
-    for variant in (0, 1, 3, 4, 5, 6, -1):
+    for variant in (0, 1, 3, 4, 5, 6, 7, -1):
         _lib.call("tprf_ctx_set_option", c, 2, variant)
         try:
             for full in (1, 0):
@@ -563,6 +563,46 @@ def test_chi0_wr_herm(L, oracle, dims, n, nb):
                         assert np.isnan(got[:nb - 1]).all(), variant
         finally:
             _lib.call("tprf_ctx_set_option", c, 2, -1)
+
+
+# (dims, n, n_iw, nb, ntau): L / 128 = 1, 2, 3 (odd), 6 and 12 tau tiles of the
+# workgroup-per-pair kernel; odd and even sides (self-mirrored r)
+@pytest.mark.parametrize("dims,n,n_iw,nb,ntau", [((2, 2, 1), 3, 32, 64, 129), ((3, 2, 2), 2, 64, 50, 257),
+                                                 ((2, 2, 1), 3, 64, 33, None), ((4, 4, 2), 3, 128, 64, None),
+                                                 ((3, 4, 2), 2, 128, 40, None), ((4, 3, 2), 3, 256, 64, None)])
+def test_chi0_wr_herm_workgroup_per_pair(L, oracle, dims, n, n_iw, nb, ntau):
+    """The Hermitian fused bubble with one workgroup per (r, -r) pair
+    (option 2 = 7) against the oracle's chi0_wr_from_grt_PH, all frequencies
+    and nu >= 0 only (the rows nu < 0 keep their NaN), on meshes of 1, 2, 3,
+    6 and 12 tau tiles."""
+    import torch
+    from tprf_amd import _lib
+    rng = np.random.default_rng(7 * n + nb + n_iw)
+    nk = int(np.prod(dims))
+    e = hermitian_ek(rng, nk, n, scale=0.7)
+    beta = 9.0
+    g_tr = oracle.fourier_w_to_t(oracle.fourier_lattice(oracle.lattice_dyson_g_wk(beta, n_iw, -0.2, e), dims, -1),
+                                 beta, ntau=ntau)
+    ntau = g_tr.shape[0]
+    ref = oracle.chi0_wr_from_grt_PH(g_tr, dims, beta, nb)
+    c = _lib.ctx()
+    g_half = g_tr.copy()
+    g_half[:, ~canonical_r(dims)] = np.nan
+    gt = gpu(g_half)
+    _lib.call("tprf_ctx_set_option", c, 2, 7)
+    try:
+        for full in (1, 0):
+            out = torch.full((2 * nb - 1, nk, n, n, n, n), float("nan"), dtype=torch.complex128, device="cuda")
+            _lib.call("tprf_chi0_wr_from_grt_PH_herm", c, beta, ntau, _lib.dims3(dims), n, nb, _lib.ptr(gt),
+                      _lib.ptr(out), full)
+            got = host(out)
+            if full:
+                assert relerr(got, ref) <= TOL
+            else:
+                assert relerr(got[nb - 1:], ref[nb - 1:]) <= TOL
+                assert np.isnan(got[:nb - 1]).all()
+    finally:
+        _lib.call("tprf_ctx_set_option", c, 2, -1)
 
 
 @pytest.mark.parametrize("dims,n,n_iw", [((4, 3, 2), 2, 128), ((5, 5, 1), 3, 128), ((4, 4, 4), 1, 128),

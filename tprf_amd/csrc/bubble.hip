@@ -456,20 +456,22 @@ static hipError_t launch_one(hipStream_t s, double beta, int L, int nb, const in
 
 bool bubble_pair_supported(int n, int P) { return (n == 2 || n == 3) && (P == 64 || P == 128); }
 
-int bubble_pair_variants() { return 7; }
+int bubble_pair_variants() { return 8; }
 
 // the variants with orbital-pair workgroups (those the Hermitian form has)
 int bubble_pair_herm_variants(int n, int P, int *list) {
-  static const int v3a[] = {0, 1, 3, 4, 5, 6}, v3b[] = {0, 1, 3, 4, 5}, v2a[] = {0, 3}, v2b[] = {0, 1, 3, 4};
+  static const int v3a[] = {0, 1, 3, 4, 5, 6, 7}, v3b[] = {0, 1, 3, 4, 5}, v2a[] = {0, 3, 7}, v2b[] = {0, 1, 3, 4};
   const int *v = (n == 3) ? (P == 128 ? v3a : v3b) : (P == 128 ? v2a : v2b);
-  const int cnt = (n == 3) ? (P == 128 ? 6 : 5) : (P == 128 ? 2 : 4);
+  const int cnt = (n == 3) ? (P == 128 ? 7 : 5) : (P == 128 ? 3 : 4);
   for (int i = 0; i < cnt; ++i) list[i] = v[i];
   return cnt;
 }
 
 // Workgroup shapes (same results, picked by the autotuner; variants 5 and 6,
 // the merged orbital pairs of the Hermitian form at n = 3 with the inner DFT
-// split 16 x 8 or 8 x 16, run as 0 elsewhere):
+// split 16 x 8 or 8 x 16, and variant 7, the Hermitian form with one
+// workgroup per (r, -r) pair (bubble_wg.hip, n = 2, 3, P = 128), run as 0
+// elsewhere):
 // variant 0 owns one
 // orbital pair; 1 is a narrower or differently grouped shape; 2 stages every G
 // entry for CG = n^4 / 3 (n = 3) or n^4 (n = 2) columns; 3 and 4 load tiles
@@ -478,6 +480,10 @@ int bubble_pair_herm_variants(int n, int P, int *list) {
 hipError_t launch_bubble_pair(hipStream_t s, double beta, int L, int nb, const int dims[3], int n, const c2 *gtr,
                               c2 *out, const BubbleDev &bd, const Vinv8 &vinv, int variant, bool herm, bool nonneg) {
   const int m_lo = nonneg ? 0 : 1 - nb;
+  if (variant == 7) {
+    if (herm && (n == 2 || n == 3) && bd.P == 128) return launch_bubble_wg(s, beta, L, nb, dims, n, gtr, out, bd, vinv, nonneg);
+    variant = 0;
+  }
   if (variant == 5 || variant == 6) {  // merged orbital pairs: the Hermitian form at n = 3 only
     if (herm && n == 3 && bd.P == 128) {
       if (variant == 6)  // inner DFT split 8 x 16 instead of 16 x 8

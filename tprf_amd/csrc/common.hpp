@@ -193,6 +193,11 @@ hipError_t launch_bubble_pair(hipStream_t s, double beta, int L, int nb, const i
                               bool nonneg = false);
 bool bubble_pair_supported(int n, int P);
 int bubble_pair_variants();
+// bubble_wg.hip: the Hermitian form with one workgroup per canonical (r, -r)
+// owning every canonical column (n = 2, 3; P = 128; hipErrorNotSupported
+// otherwise)
+hipError_t launch_bubble_wg(hipStream_t s, double beta, int L, int nb, const int dims[3], int n, const c2 *gtr,
+                            c2 *out, const BubbleDev &bd, const Vinv8 &vinv, bool nonneg);
 // variants usable with herm, written to list[0..count), count returned (<= 8)
 int bubble_pair_herm_variants(int n, int P, int *list);
 // w2t.hip: returns hipErrorNotSupported when M has no specialisation

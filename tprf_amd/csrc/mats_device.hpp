@@ -60,7 +60,9 @@ __device__ __forceinline__ long minus_r(long r, int d0, int d1, int d2) {
 // (fit_derivatives, fourier_matsubara.cpp:42-97, with m1 from :151,:163):
 // lo(m) = g(tau_m), hi(m) = g(tau_{L-m}), m = 0..8.  Returns the pole
 // coefficients a[3] of the statistic's model.
-template <int STAT, typename Lo, typename Hi>
+// RCP: the differences are multiplied by 1 / tau_m (one division per m
+// instead of four; results within an ulp of the quotients).
+template <int STAT, bool RCP = false, typename Lo, typename Hi>
 __device__ __forceinline__ void boundary_poles(Lo lo, Hi hi, double dtau, const Vinv8 &vinv, c2 *a, c2 *m1out) {
   c2 gl0 = c2{0, 0}, gl1 = c2{0, 0}, gr0 = c2{0, 0}, gr1 = c2{0, 0};
   const c2 g0 = lo(0), gL = hi(0);
@@ -68,8 +70,15 @@ __device__ __forceinline__ void boundary_poles(Lo lo, Hi hi, double dtau, const 
   for (int m = 1; m <= 8; ++m) {
     const double tm = (double)m * dtau;
     const c2 gm = lo(m), gLm = hi(m);
-    const c2 dl = c2{(gm.x - g0.x) / tm, (gm.y - g0.y) / tm};
-    const c2 dr = c2{(gL.x - gLm.x) / tm, (gL.y - gLm.y) / tm};
+    c2 dl, dr;
+    if constexpr (RCP) {
+      const double itm = 1.0 / tm;
+      dl = c2{(gm.x - g0.x) * itm, (gm.y - g0.y) * itm};
+      dr = c2{(gL.x - gLm.x) * itm, (gL.y - gLm.y) * itm};
+    } else {
+      dl = c2{(gm.x - g0.x) / tm, (gm.y - g0.y) / tm};
+      dr = c2{(gL.x - gLm.x) / tm, (gL.y - gLm.y) / tm};
+    }
     gl0 = cadd(gl0, cscale(dl, vinv.v[0 * 8 + m - 1]));
     gl1 = cadd(gl1, cscale(dl, vinv.v[1 * 8 + m - 1]));
     gr0 = cadd(gr0, cscale(dr, vinv.v[0 * 8 + m - 1]));
