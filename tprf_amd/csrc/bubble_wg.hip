@@ -30,7 +30,7 @@ namespace tprf {
 namespace {
 
 constexpr int kWgP = 128;   // inner DFT length
-constexpr int kTwRow = 136; // twiddle row: bins 0..63 at 0..63, 64..127 at 72..135
+constexpr int kTwRow = 128; // twiddle row: bin k at k
 constexpr int kSRow = 129;  // padded row of the per-column sums in the epilogue
 
 template <int N>
@@ -74,14 +74,6 @@ __device__ __forceinline__ int canon_index(int a, int b, int c, int d) {
   return noff + a * per + c * N - c * (c - 1) / 2 + (d - c);
 }
 
-// value of the adjacent lane (lane ^ 1), by two 32-bit DPP moves (quad_perm 1,0,3,2)
-__device__ __forceinline__ double swap_adjacent(double x) {
-  const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)(b & 0xffffffffLL), 0xB1, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0xB1, 0xF, 0xF, false);
-  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-
 // x, opaque to the optimiser: addresses computed from it are formed where they
 // are used instead of being hoisted out of the loops and held in registers
 __device__ __forceinline__ int opaque(int x) {
@@ -101,8 +93,8 @@ struct WgLds {
   static constexpr int twp = pole + NC * 7;              // [8][16] exp(2 pi i s k1 / P) at [k1][s]
   static constexpr int hmod = twp + kWgP;                // [3][nnu], nnu = 2 nb - 1 <= P - 1
   static constexpr int cmap = hmod + 3 * (kWgP - 1);     // [2][n^4] ints: 4 canonical column + variant
-  static constexpr int vinv = cmap + (2 * n4 + 3) / 4;  // the 8 x 8 inverse Vandermonde (32 c2)
-  static constexpr int braw = vinv + 32;                 // [18][n2]: G(tau_j, r), j = 0..8, L-8..L
+  static constexpr int cpole = cmap + (2 * n4 + 3) / 4;  // [3][18] doubles: pole coefficients from x_j
+  static constexpr int braw = cpole + 28;                // [18][n2]: G(tau_j, r), j = 0..8, L-8..L
   static constexpr int total = braw + 18 * n2;
   // between two pairs: the sums [NC][kSRow] in the transpose region
   static constexpr int sums = st;
@@ -157,7 +149,7 @@ __global__ void __launch_bounds__(64 * Canon<N>::NW)
   canon_col<N>(col, ca, cb, cc, cd);
   const int ea = cd * N + ca, eb = cc * N + cb;
   const int s = lane & 15;                       // stage 1: j2 = 16 k + s
-  const int k1 = (lane >> 1) & 7, h = lane & 1;  // stage 2: bins k1 + 8 (k2 + 8 h)
+  const int k1 = (lane >> 1) & 7, h = lane & 1;  // stage 2: bins k1 + 8 (2 k' + h)
 
   // ---- tables shared by every pair
   // stage-1 twiddles exp(2 pi i s k1 / P) at [k1][s]: the 16 lanes of one
@@ -165,10 +157,27 @@ __global__ void __launch_bounds__(64 * Canon<N>::NW)
   // conflict-free when their 16-byte pieces differ modulo 256 bytes)
   for (int p = tid; p < P; p += NT) st_v2d(lds + Lay::twp + p, ld_v2d(bd.twp + (((p >> 4) * (p & 15)) & (P - 1))));
   for (int f = tid; f < 3 * nnu; f += NT) st_v2d(lds + Lay::hmod + f, ld_v2d(bd.hmod + f));
-  // the boundary-fit operator, read from LDS only while the poles are formed
-  // (as a kernel argument it would be held in 128 scalar registers)
-  for (int f = tid; f < 64; f += NT) reinterpret_cast<double *>(lds + Lay::vinv)[f] = vinv.v[f];
-  const Vinv8 &lvinv = *reinterpret_cast<const Vinv8 *>(lds + Lay::vinv);
+  // The bosonic pole coefficients of a column (fourier_matsubara.cpp:42-97,
+  // :274-288) are linear in its boundary differences dl_m = x_m - x_0,
+  // dr_m = x_L - x_{L-m} (m = 1..8) and x_L - x_0:
+  //   a_i = w1_i (x_L - x_0) + sum_m (wl_im dl_m + wr_im dr_m),
+  // with m2 = sum_m V0_m (dl_m - dr_m) / tau_m, m3 = -(2 / dtau) sum_m V1_m
+  // (dl_m + dr_m) / tau_m and a = (4/3 (m1 - m3), m3 - (m1 + m2) / 2,
+  // m1 / 6 + m2 / 2 + m3 / 3).  Row i of cpl: w1_i, wl_i1..wl_i8, wr_i1..wr_i8.
+  double *cpl = reinterpret_cast<double *>(lds + Lay::cpole);
+  if (tid < 3) {
+    const int i = tid;
+    const double d1 = i == 0 ? 4.0 / 3.0 : (i == 1 ? -0.5 : 1.0 / 6.0);
+    const double d2 = i == 0 ? 0.0 : (i == 1 ? -0.5 : 0.5);
+    const double d3 = i == 0 ? -4.0 / 3.0 : (i == 1 ? 1.0 : 1.0 / 3.0);
+    cpl[i * 17] = d1;
+    for (int m = 1; m <= 8; ++m) {
+      const double itm = 1.0 / ((double)m * dtau);
+      const double g2 = d2 * vinv.v[m - 1] * itm, g3 = d3 * (-2.0 / dtau) * vinv.v[8 + m - 1] * itm;
+      cpl[i * 17 + m] = g2 + g3;
+      cpl[i * 17 + 8 + m] = g3 - g2;
+    }
+  }
   // column map: side 0 (r) and 1 (-r), q = wxyz -> 4 canonical index + variant
   // (0: abcd of r; 1: cdab of -r; 2: badc of r; 3: dcba of -r)
   {
@@ -185,18 +194,26 @@ __global__ void __launch_bounds__(64 * Canon<N>::NW)
     }
   }
 
-  // the boundary rows G(tau_j, r), j = 0..8 and L-8..L (row e: j = e or
-  // L - 17 + e), one entry per thread, held in a register through the pair's
-  // steps; every column's boundary products x_j are formed from them
-  v2d gbr = v2d{0.0, 0.0};
-  auto load_braw = [&](long rr) {
-    const int tt = opaque(tid);
-    if (tt < NBR) {
-      const int e = tt / n2, ent = tt - e * n2;
-      const long tau = e < 9 ? e : L - 17 + e;
-      gbr = ld_v2d(gtr + (tau * nr + rr) * n2 + ent);
+  // The boundary rows G(tau_j, r), j = 0..8 and L-8..L (row e: j = e or
+  // L - 17 + e), pass through the staged tiles: tau = N1 j2 + t is row j2 of
+  // slot 0 of tile t when t = tau mod N1 <= N1 / 2, else tau = N1 (j2 + 1) - t
+  // is row j2 of slot 1 of tile t = -tau mod N1.  Thread e n2 + ent copies its
+  // entry out of tile bsrc >> 16, LDS element bsrc & 0xffff.
+  int bsrc = -1;
+  if (tid < NBR) {
+    const int e = tid / n2, ent = tid - e * n2;
+    const int tau = e < 9 ? e : L - 17 + e;
+    const int r0 = tau % N1;
+    int tt, el;
+    if (r0 <= N1 / 2 && tau / N1 < P) {
+      tt = r0;
+      el = (tau / N1) * n2 + ent;
+    } else {
+      tt = (N1 - r0) % N1;
+      el = (P + (tau + tt) / N1 - 1) * n2 + ent;
     }
-  };
+    bsrc = (tt << 16) | el;
+  }
 
   // tile t: slot 0 holds G(N1 j2 + t), slot 1 G(N1 (j2 + 1) - t), j2 < P; the
   // twiddle rows of its steps t and N1 - t
@@ -231,7 +248,7 @@ __global__ void __launch_bounds__(64 * Canon<N>::NW)
       const int e = tt + i * NT;
       if (e < 2 * P) {
         const int which = e / P, k = e - which * P;
-        st_v2d(lds + Lay::tw + ((t & 1) * 2 + which) * kTwRow + (k < 64 ? k : k + 8), pt[i]);
+        st_v2d(lds + Lay::tw + ((t & 1) * 2 + which) * kTwRow + k, pt[i]);
       }
     }
   };
@@ -276,35 +293,41 @@ __global__ void __launch_bounds__(64 * Canon<N>::NW)
     for (int q = 0; q < 8; ++q) sT[(cw * 8 + q) * 16 + (s ^ (2 * q))] = v[q];
     __builtin_amdgcn_sched_barrier(0);
   };
-  // stage 2: lane pair (cw, k1, h = 0/1): DFT-8 over the even / odd s, the
-  // radix-2 step of the DFT-16 across the pair; lane h keeps the bins
-  // k1 + 8 (k2 + 8 h) and accumulates them with exp(2 pi i j1 m / L)
+  // stage 2: lane pair (cw, k1, h = 0/1): the DFT-16 over s of row (cw, k1)
+  // split by decimation in frequency: lane h forms u(s') = y(s') +- y(s' + 8),
+  // s' < 8, twiddled by exp(2 pi i h s' / 16), whose DFT-8 gives the bins
+  // kk = 2 k' + h; it keeps k1 + 8 kk and accumulates them with
+  // exp(2 pi i j1 m / L)
   auto stage2 = [&](const c2 *twrow) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     __builtin_amdgcn_sched_barrier(0);
+    const c2 *row = sT + (cw * 8 + k1) * 16;
+    v2d ry[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) ry[q] = ld_v2d(row + (q ^ (2 * k1)));
+    asm volatile("" : "+v"(ry[0]), "+v"(ry[1]), "+v"(ry[2]), "+v"(ry[3]), "+v"(ry[4]), "+v"(ry[5]), "+v"(ry[6]),
+                 "+v"(ry[7]), "+v"(ry[8]), "+v"(ry[9]), "+v"(ry[10]), "+v"(ry[11]), "+v"(ry[12]), "+v"(ry[13]),
+                 "+v"(ry[14]), "+v"(ry[15]));
+    const double sg = h ? -1.0 : 1.0;
     c2 v[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) v[q] = sT[(cw * 8 + k1) * 16 + ((2 * q + h) ^ (2 * k1))];
-    SmallDft<8, 1>::run(v, nullptr, 0);
+    for (int q = 0; q < 8; ++q) v[q] = c2{fma(sg, ry[q + 8].x, ry[q].x), fma(sg, ry[q + 8].y, ry[q].y)};
+    // the accumulation twiddles are read while the DFT-8 runs
+    v2d rw[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) rw[q] = ld_v2d(twrow + k1 + 8 * h + 16 * q);
     if (h) {
-      // exp(2 pi i k2 / 16), k2 = 1..7
+      // exp(2 pi i s' / 16), s' = 1..7
       const double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173, hh = 0.70710678118654752440;
       const c2 w16[7] = {c2{c1, s1}, c2{hh, hh}, c2{s1, c1}, c2{0.0, 1.0}, c2{-s1, c1}, c2{-hh, hh}, c2{-c1, s1}};
 #pragma unroll
       for (int q = 1; q < 8; ++q) v[q] = cmul(v[q], w16[q - 1]);
     }
-    const double sg = h ? -1.0 : 1.0;
+    SmallDft<8, 1>::run(v, nullptr, 0);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const double ox = swap_adjacent(v[q].x), oy = swap_adjacent(v[q].y);
-      const c2 x = c2{fma(sg, v[q].x, ox), fma(sg, v[q].y, oy)};
-      acc[q] = cfma(acc[q], twrow[k1 + 8 * q + 72 * h], x);
-    }
-    // the 16 LDS reads (transpose row, twiddles) together, then the arithmetic
-    __builtin_amdgcn_sched_group_barrier(0x100, 16, 0);
-    __builtin_amdgcn_sched_group_barrier(0x002, 400, 0);
+    for (int q = 0; q < 8; ++q) acc[q] = cfma(acc[q], c2{rw[q].x, rw[q].y}, v[q]);
     __builtin_amdgcn_sched_barrier(0);
     // the next stage 1 of this wave overwrites the transpose rows read here
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -314,7 +337,6 @@ __global__ void __launch_bounds__(64 * Canon<N>::NW)
 
   // first pair: boundary samples and tile 0 in flight together
   long r = bd.rcanon[pair];
-  load_braw(r);
   load_tile(r, 0);
   store_tile(0);
   if (ntile > 1) load_tile(r, 1);
@@ -330,7 +352,10 @@ __global__ void __launch_bounds__(64 * Canon<N>::NW)
     int t = 0, u = 0;
     for (int step = 0; step < N1; ++step) {
       const bool two = t > 0 && 2 * t != N1;
-      if (u == 0) __syncthreads();  // tile t staged
+      if (u == 0) {
+        __syncthreads();  // tile t staged
+        if ((bsrc >> 16) == t) lds[Lay::braw + tid] = lds[Lay::tile + (bsrc & 0xffff)];
+      }
       stage1(u);
       const bool last = !two || u == 1;
       if (last) {
@@ -353,35 +378,36 @@ __global__ void __launch_bounds__(64 * Canon<N>::NW)
     const int te = opaque(tid);  // epilogue indices are formed here, not held through the steps
     if (col_ok) {
 #pragma unroll
-      for (int q = 0; q < 8; ++q) lds[Lay::sums + col * kSRow + k1 + 8 * (q + 8 * h)] = acc[q];
+      for (int q = 0; q < 8; ++q) lds[Lay::sums + col * kSRow + k1 + 8 * h + 16 * q] = acc[q];
     }
-    if (te < NBR) st_v2d(lds + Lay::braw + te, gbr);
     const bool more = it + 1 < count_w;
     const long rn = more ? bd.rcanon[pair + W] : r;
-    if (more) {
-      load_braw(rn);
-      load_tile(rn, 0);
-    }
+    if (more) load_tile(rn, 0);
     __syncthreads();
-    // pole coefficients: thread 2 c + d, d = 0 from x, d = 1 from x reversed,
-    // with x_e = G_da(tau_e, r) conj(G_cb(tau_{L-e}, r)) from the boundary rows
-    if (te < 2 * NC) {
-      const int c = te >> 1, d = te & 1;
+    // pole coefficients: thread 6 c + 3 d + i forms a_i of column c from its
+    // boundary products x_e = G_da(tau_e, r) conj(G_cb(tau_{L-e}, r)) (e = j
+    // for j <= 8, e = 17 - (L - j) above), d = 0; for the reversed x (d = 1)
+    // dl and dr trade places and every difference changes sign
+    if (te < 6 * NC) {
+      const int c = te / 6, d = (te / 3) & 1, i = te % 3;
       int a, b, cc2, dd2;
       canon_col<N>(c, a, b, cc2, dd2);
       const c2 *br = lds + Lay::braw;
       c2 x[18];
 #pragma unroll
       for (int e = 0; e < 18; ++e) x[e] = cmulc(br[e * n2 + dd2 * N + a], br[(17 - e) * n2 + cc2 * N + b]);
-      c2 *pc = lds + Lay::pole + c * 7;
-      if (d == 0) {
-        boundary_poles<1, true>([&](int m) { return x[m]; }, [&](int m) { return x[17 - m]; }, dtau, lvinv, pc,
-                                (c2 *)nullptr);
-        pc[6] = csub(x[17], x[0]);
-      } else {
-        boundary_poles<1, true>([&](int m) { return x[17 - m]; }, [&](int m) { return x[m]; }, dtau, lvinv, pc + 3,
-                                (c2 *)nullptr);
+      const double *ci = cpl + i * 17;
+      const c2 m1 = csub(x[17], x[0]);
+      c2 sum = c2{ci[0] * m1.x, ci[0] * m1.y};
+#pragma unroll
+      for (int m = 1; m <= 8; ++m) {
+        const c2 dl = csub(x[m], x[0]), dr = csub(x[17], x[17 - m]);
+        const double wa = ci[d ? 8 + m : m], wb = ci[d ? m : 8 + m];
+        sum = c2{fma(wa, dl.x, fma(wb, dr.x, sum.x)), fma(wa, dl.y, fma(wb, dr.y, sum.y))};
       }
+      c2 *pc = lds + Lay::pole + c * 7;
+      pc[3 * d + i] = d ? c2{-sum.x, -sum.y} : sum;
+      if (d == 0 && i == 0) pc[6] = m1;
     }
     __syncthreads();
 
@@ -401,12 +427,15 @@ __global__ void __launch_bounds__(64 * Canon<N>::NW)
         const double sgn = cj ? -1.0 : 1.0;
         const c2 p0 = c2{pa[0].x, sgn * pa[0].y}, p1 = c2{pa[1].x, sgn * pa[1].y}, p2 = c2{pa[2].x, sgn * pa[2].y};
         const c2 *S = lds + Lay::sums + c * kSRow;
-        c2 *o = out + (side ? mr : r) * n4 + q;
-        for (int row = g; row < nim; row += GO) {
-          const int im = im_lo + row, m = im - (nb - 1);
-          const c2 sv = S[(side ? -m : m) & (P - 1)];
+        const long ostep = (long)GO * nr * n4;
+        c2 *o = out + ((long)(im_lo + g) * nr + (side ? mr : r)) * n4 + q;
+        const int msg = side ? -1 : 1;
+#pragma unroll 4
+        for (int row = g; row < nim; row += GO, o += ostep) {
+          const int im = im_lo + row;
+          const c2 sv = S[(msg * (im - (nb - 1))) & (P - 1)];
           const c2 sx = c2{(sv.x + xa.x) * fact, sgn * (sv.y + xa.y) * fact};
-          o[(long)im * nr * n4] = cfma(cfma(cfma(sx, p0, sH[im]), p1, sH[nnu + im]), p2, sH[2 * nnu + im]);
+          *o = cfma(cfma(cfma(sx, p0, sH[im]), p1, sH[nnu + im]), p2, sH[2 * nnu + im]);
         }
       }
     }
