@@ -102,6 +102,8 @@ const char *tprf_ctx_last_error(const tprf_ctx *ctx);
  * and warns above 1e-6 (fourier_matsubara.cpp:227-234); the fitted zeroth
  * moment is reported too (warning above 1e-6, :238-239).                   */
 int tprf_ctx_tail_status(tprf_ctx *ctx, double *max_fit_error, double *max_abs_m0);
+/* Clears the status words with a small kernel on the context's stream (no
+ * memset node, so the call can be captured into a HIP graph).              */
 int tprf_ctx_tail_status_reset(tprf_ctx *ctx);
 
 /* ---- a1: Dyson --------------------------------------------------------- */

@@ -105,6 +105,35 @@ def test_cubic_3orb_full_size_every_point_vs_oracle(T, oracle):
     assert ratio.max() <= 1.0
 
 
+def test_cubic_3orb_graph_replay_matches_eager(T):
+    """The default chain captured as one HIP graph (status reset + five stage
+    launches) and replayed four times, with the outputs overwritten by NaN and
+    unrelated eager work between the replays: every replay reproduces the
+    eager chain bit for bit, and the captured status reset clears the words."""
+    from tprf_amd import chain, workloads
+    wl = workloads.cubic_3orb(seed=0)
+    ch = chain.BubbleRPA(wl)
+    ch.run()
+    err, _ = ch.check_status()
+    assert err < 1e-6
+    ref0, ref = ch.chi0_wk.clone(), ch.chi_wk.clone()
+    ch.capture()
+    for i in range(4):
+        ch.chi0_wk.fill_(float("nan"))
+        ch.chi_wk.fill_(float("nan"))
+        x = T.randn(1 << 20, device="cuda:0", dtype=T.float64)
+        assert float((x * x).sum()) > 0  # eager kernels and a synchronisation between replays
+        ch.replay()
+        err_i, _ = ch.check_status()
+        T.cuda.synchronize()
+        print(f"replay {i}: tail residual {err_i:.2e}", flush=True)
+        assert err_i == err
+        assert T.equal(ch.chi0_wk, ref0) and T.equal(ch.chi_wk, ref), f"replay {i}"
+    ch.run()  # eager again after the replays
+    T.cuda.synchronize()
+    assert T.equal(ch.chi_wk, ref)
+
+
 # ---------------------------------------------------------------- cfg3 uneven batch
 def test_uneven_batch_48_every_lattice(T):
     """The 48 uneven lattices (8^3 / 12^3 / 16^3 k, 1-3 orbitals, 128/256/512

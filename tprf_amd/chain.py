@@ -140,13 +140,30 @@ class BubbleRPA:
             raise KeyError(name)
 
     def stage_reset(self):
-        """Clear the tail-fit / singularity status words (one small memset)."""
+        """Clear the tail-fit / singularity status words (one small fill kernel)."""
         _lib.tail_status_reset(self._ctx())
 
     def run(self):
         self.stage_reset()
         for s in self.STAGES:
             self.stage(s)
+
+    def capture(self):
+        """Capture run() (status reset and every stage) into one HIP graph on a
+        private stream; replay() then runs the whole eval as one graph launch.
+        The first run() settles the tables, plans and kernel variants, so the
+        capture allocates nothing."""
+        self.run()
+        torch.cuda.synchronize(self.dev)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=torch.cuda.Stream(device=self.dev)):
+            self.run()
+        self._ctx()  # the context back on torch's current stream
+        self.graph = g
+        return g
+
+    def replay(self):
+        self.graph.replay()
 
     def check_status(self):
         """Synchronise and return (max tail-fit residual, max |m0|); raises on singular inversions."""

@@ -678,8 +678,9 @@ int tprf_ctx_tail_status(tprf_ctx *ctx, double *max_fit_error, double *max_abs_m
 
 int tprf_ctx_tail_status_reset(tprf_ctx *ctx) {
   if (!ctx) return TPRF_ERR_ARG;
-  TPRF_HIP_TRY(ctx, hipMemsetAsync(ctx->d_status, 0, 4 * sizeof(double), ctx->stream), "hipMemsetAsync");
-  return TPRF_OK;
+  // a fill kernel, not hipMemsetAsync: callers capture this into HIP graphs
+  return check_launch(ctx, tprf::launch_zero(ctx->stream, 2, reinterpret_cast<c2 *>(ctx->d_status)),
+                      "status reset kernel");
 }
 
 int tprf_lattice_dyson_g_wk(tprf_ctx *ctx, double beta, int n_iw, double mu, long nk, int norb, const tprf_c128 *e_k,
