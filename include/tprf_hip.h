@@ -67,8 +67,11 @@ int tprf_ctx_set_stream(tprf_ctx *ctx, void *hip_stream); /* hipStream_t; NULL =
  * shape times every shape and keeps the fastest (not while the stream is being
  * captured into a graph).                                                   */
 #define TPRF_OPT_BUBBLE_VARIANT 2
-/* TPRF_OPT_W2T_VARIANT: the same for the workgroup width of the pruned
- * omega -> tau kernel.                                                      */
+/* TPRF_OPT_W2T_VARIANT: the same for the workgroup shape of the pruned
+ * omega -> tau kernel (0, 1 = 256, 512 threads per workgroup; 2 = for
+ * tprf_fourier_wr_to_tr_herm, one persistent workgroup per CU whose transform
+ * waves never load from global memory while four loader waves stage the next
+ * tile in LDS; elsewhere as 1).                                             */
 #define TPRF_OPT_W2T_VARIANT 3
 /* TPRF_OPT_LATTICE_VARIANT: the same for the number of target columns per
  * workgroup of the single-pass 16x16x16 lattice FFT (0, 1, 2 = 1, 2, 4

@@ -275,8 +275,8 @@ def test_wr_to_tr_pruned_and_generic(L, oracle, stat, n_max, nt):
         finally:
             _lib.call("tprf_ctx_set_option", c, 1, 0)
         assert relerr(host(got.data), ref) <= TOL, f"generic={generic}"
-    # both workgroup widths of the pruned kernel (option 3 fixes it, -1 restores tuning)
-    for variant in (0, 1):
+    # every workgroup shape of the pruned kernel (option 3 fixes it, -1 restores tuning)
+    for variant in (0, 1, 2):
         _lib.call("tprf_ctx_set_option", c, 3, variant)
         try:
             got = L.fourier_wr_to_tr(g, nt=nt)
@@ -605,13 +605,20 @@ def test_chi0_wr_herm_workgroup_per_pair(L, oracle, dims, n, n_iw, nb, ntau):
         _lib.call("tprf_ctx_set_option", c, 2, -1)
 
 
-@pytest.mark.parametrize("dims,n,n_iw", [((4, 3, 2), 2, 128), ((5, 5, 1), 3, 128), ((4, 4, 4), 1, 128),
-                                         ((3, 2, 2), 3, 128), ((3, 2, 2), 3, 256)])
-def test_wr_to_tr_herm(L, oracle, dims, n, n_iw):
+@pytest.mark.parametrize("dims,n,n_iw,nt", [((4, 3, 2), 2, 128, None), ((5, 5, 1), 3, 128, None),
+                                            ((4, 4, 4), 1, 128, None), ((3, 2, 2), 3, 128, None),
+                                            ((3, 2, 2), 3, 256, None), ((16, 16, 8), 3, 256, None),
+                                            ((15, 16, 14), 2, 128, None), ((16, 16, 14), 2, 128, 513),
+                                            ((16, 16, 8), 3, 256, 1025)])
+def test_wr_to_tr_herm(L, oracle, dims, n, n_iw, nt):
     """tprf_fourier_wr_to_tr_herm on G(i w, r) from a Hermitian e_k (odd and
-    even lattice sides, so self-mirrored r with a < b pairs), both workgroup
-    widths (option 3), against the oracle's full transform; the tail-fit
-    residual is reported as for the plain call."""
+    even lattice sides, so self-mirrored r with a < b pairs), every workgroup
+    shape (option 3), against the oracle's full transform; the tail-fit
+    residual is reported as for the plain call.  The large lattices give the
+    persistent shape (option 3 = 2) more tiles than a GPU has CUs, so its
+    workgroups take several tiles each, the last one ragged; ntau 513 and 1025
+    (L = 2 M) are its two-residue form, the default meshes (L = 3 M) the
+    three-residue one."""
     import torch
     from tprf_amd import _lib
     rng = np.random.default_rng(7 * n + sum(dims))
@@ -619,13 +626,13 @@ def test_wr_to_tr_herm(L, oracle, dims, n, n_iw):
     e = hermitian_ek(rng, nk, n, scale=0.8)
     beta = 12.0
     g_wr = oracle.fourier_lattice(oracle.lattice_dyson_g_wk(beta, n_iw, -0.2, e), dims, -1)
-    ref = oracle.fourier_w_to_t(g_wr, beta)
+    ref = oracle.fourier_w_to_t(g_wr, beta, ntau=nt)
     ntau = ref.shape[0]
     c = _lib.ctx()
     x = gpu(g_wr)
     canon = canonical_r(dims)
     for full in (1, 0):
-        for variant in (0, 1, -1):
+        for variant in (0, 1, 2, -1):
             _lib.call("tprf_ctx_set_option", c, 3, variant)
             try:
                 out = torch.full((ntau, nk, n, n), float("nan"), dtype=torch.complex128, device="cuda")

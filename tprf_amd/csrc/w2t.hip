@@ -42,6 +42,17 @@ __device__ __forceinline__ void w2t_butterflies(c2 *v, int j, int Ns, const c2 *
 
 constexpr int kW2tThreads = 256;
 
+// x, opaque to the optimiser: what is computed from it is computed where it is
+// used instead of being hoisted out of loops and held in registers
+__device__ __forceinline__ int opaque32(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+__device__ __forceinline__ long opaque64(long x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
 // NT threads per workgroup: C = NT / (M / 8) columns, i.e. C * 16 contiguous
 // bytes of every row written per workgroup.
 // Fermionic M = 256, 512 with 256 threads: registers held to the three
@@ -253,6 +264,258 @@ __global__ void __launch_bounds__(NT)
   }
 }
 
+// Wave-specialised persistent form of the symmetric transform above (fermions,
+// the moments left in rows L - 2 .. L of each column by k_tail_moments, Q >= 2,
+// M = 256 or 512).  gridDim.x workgroups (at most one per CU) take the tiles
+// s, s + gridDim.x, ... of their slot s (XCD-aware order).  Waves 0 .. 7 (the
+// transform) read only LDS inside the loop and write the outputs; waves 8 .. 11
+// (the loader) copy each residue's output records into buf and stage the next
+// tile's inputs (scaled by 1 / beta), columns and pole coefficients in LDS,
+// spread over the current tile's residues.  vmcnt counts a wave's loads and
+// stores together, in issue order: a load issued after stores can only be
+// waited for together with them, and an output store takes microseconds to
+// complete under this write load, so no wave here both stores and loads.
+// Barriers leave vector memory in flight (barrier_keep_vm); both roles pass
+// the same barriers: per residue two per exchange, one before the outputs
+// (records in buf) and one after them (buf free).
+constexpr int kW2tSpecLoad = 256;  // loader threads
+
+template <int M>
+__global__ void __launch_bounds__(512 + kW2tSpecLoad)
+    k_w2t_spec(double beta, int n_max, long ncol, const c2 *__restrict__ in, c2 *__restrict__ out, MatsDev mt,
+               W2tDev wd, SymCols sym) {
+  constexpr int NT = 512, NL = kW2tSpecLoad;
+  constexpr int TPC = M / 8;         // transform threads per column
+  constexpr int C = NT / TPC;        // columns per tile
+  constexpr int NE = M * C / NL;     // staged inputs per loader thread and tile
+  constexpr int NR = 3 * M / NL;     // records per loader thread and residue
+  static_assert(M == 256 || M == 512, "shapes of the wave-specialised form");
+  static_assert(NL % C == 0 && NE % 8 == 0, "loader layout");
+  using Plan = W2tPlan<M>;
+  __shared__ c2 buf[M * C];
+  __shared__ c2 stg[M * C];  // next tile's inputs / beta: [e][c], FFT position e
+  __shared__ c2 s_tw[M];
+  __shared__ c2 s_b[TPC];    // b_j = exp(-2 pi i j / L)
+  __shared__ c2 s_S[8];      // S_r = exp(-2 pi i (TPC r - [r >= 4] M) / L)
+  __shared__ c2 s_a[2][C][3];
+  __shared__ c2 s_m1[2][C];
+  __shared__ int s_col[2][C], s_mcol[2][C], s_cn[2];
+
+  const int tid = threadIdx.x;
+  const bool loader = tid >= NT;
+  const int L = mt.L, Q = wd.Q;
+  const long ntile = (sym.n + C - 1) / C;
+  const long G = gridDim.x;
+  const int first = -n_max, nmesh = 2 * n_max;
+  const double ib = 1.0 / beta;
+  auto live = [&](long tl) { return (sym.n - tl * C) < C ? (int)(sym.n - tl * C) : C; };
+  // FFT position e <-> mesh index n = e or e - M; rows outside the mesh are zero
+  auto in_row = [&](int e) {
+    const int n = e < M / 2 ? e : e - M;
+    return (n >= first && n < first + nmesh) ? n - first : -1;
+  };
+
+  // loader: thread lt = tid - NT owns column lc = lt % C of every tile and the
+  // FFT positions e = lt / C + (NL / C) k, k < NE
+  const int lt = tid - NT, lc = lt % C, le0 = lt / C;
+  v2d pin[NE / 2];  // a residue's share of the next tile's inputs (at most half)
+  auto stage_meta = [&](long tl, int pr) {
+    const int cn_ = live(tl);
+    const long c0 = tl * C;
+    if (lt < C) {
+      const int ci = lt < cn_ ? lt : 0;
+      s_col[pr][lt] = sym.col[c0 + ci];
+      s_mcol[pr][lt] = sym.mcol[c0 + ci];
+      if (lt < cn_) {
+        const c2 *mo = out + (long)(L - 2) * ncol + sym.col[c0 + lt];
+        const c2 m1 = mo[0];
+        Poles<0>::coef(m1, mo[ncol], mo[2 * ncol], s_a[pr][lt]);
+        s_m1[pr][lt] = m1;
+      }
+      if (lt == 0) s_cn[pr] = cn_;
+    }
+  };
+  // inputs k in [k0, k1) of tile tl: loads, then the LDS stores
+  // this loader thread's column of tile tl (entry 0 of the tile past its end)
+  auto lcol = [&](long tl, const int *list) {
+    const int cn_ = live(tl);
+    return (long)list[tl * C + (lc < cn_ ? lc : 0)];
+  };
+  auto load_in = [&](long col, int k0, int k1) {
+    const int ler = opaque32(le0);
+#pragma unroll
+    for (int k = 0; k < NE / 2; ++k)
+      if (k0 + k < k1) {
+        const int row = in_row(ler + (NL / C) * (k0 + k));
+        pin[k] = ld_v2d(in + (row >= 0 ? (long)row * ncol + col : col));
+      }
+  };
+  auto store_in = [&](int k0, int k1) {
+    const int lcr = opaque32(lc), ler = opaque32(le0);
+#pragma unroll
+    for (int k = 0; k < NE / 2; ++k)
+      if (k0 + k < k1) {
+        const int e = ler + (NL / C) * (k0 + k);
+        st_v2d(stg + e * C + lcr, in_row(e) >= 0 ? v2d{pin[k].x * ib, pin[k].y * ib} : v2d{0.0, 0.0});
+      }
+  };
+
+  long tile = xcd_remap(blockIdx.x, G);
+  const int chunk = (NE + Q - 1) / Q;  // staged inputs per residue (<= NE / 2 for Q >= 2)
+  v2d rec[NR];
+  // the records of residue t (the same for every tile), into rec
+  auto load_rec = [&](int t) {
+    const int ltr = opaque32(lt);
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      const int e = ltr + NL * i, jp = e / 3, part = e - 3 * jp;
+      rec[i] = ld_v2d(wd.otab + (long)(Q * jp + t) * 3 + part);
+    }
+  };
+  if (loader) {
+    stage_meta(tile, 0);
+    const long col0 = lcol(tile, sym.col);
+    for (int k0 = 0; k0 < NE; k0 += NE / 2) {
+      load_in(col0, k0, k0 + NE / 2);
+      store_in(k0, k0 + NE / 2);
+    }
+  } else {
+    for (int k = tid; k < M; k += NT) st_v2d(s_tw + k, ld_v2d(wd.twM + k));
+    if (tid < TPC) st_v2d(s_b + tid, ld_v2d(wd.twL + tid));
+    if (tid < 8) {
+      int k = TPC * tid - (tid >= 4 ? M : 0);
+      if (k < 0) k += L;
+      st_v2d(s_S + tid, ld_v2d(wd.twL + k));
+    }
+  }
+  barrier_keep_vm();
+
+  if (loader) {
+    // The same barriers as the transform threads below.  At the start of each
+    // residue the loader issues that residue's records and a chunk of the next
+    // tile's inputs (chunk t in residue t), and stores them to LDS just before
+    // the records barrier; in residue 0 it also stages the next tile's
+    // columns and pole coefficients.
+    for (int par = 0;; par ^= 1) {
+      const long next = tile + G;
+      const bool more = next < ntile;
+      const long ncol_l = more ? lcol(next, sym.col) : 0;
+      for (int t = 0; t < Q; ++t) {
+        load_rec(t);
+        if (more) load_in(ncol_l, t * chunk, min(NE, (t + 1) * chunk));
+#pragma unroll
+        for (int p = 0; p + 1 < Plan::npass; ++p) {
+          barrier_keep_vm();
+          barrier_keep_vm();
+        }
+        // buf is free: this residue's records; stg was read before this
+        // tile's first exchange: chunk t of the next tile's inputs
+        const int ltr = opaque32(lt);
+#pragma unroll
+        for (int i = 0; i < NR; ++i) st_v2d(buf + ltr + NL * i, rec[i]);
+        if (more) {
+          store_in(t * chunk, min(NE, (t + 1) * chunk));
+          if (t == 0) stage_meta(next, par ^ 1);
+        }
+        barrier_keep_vm();
+        barrier_keep_vm();
+      }
+      if (!more) break;
+      tile = next;
+    }
+    return;
+  }
+
+  const int c = tid % C, j = tid / C;
+  for (int par = 0;; par ^= 1) {
+    const long next = tile + G;
+    const bool more = next < ntile;
+    c2 raw[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const v2d x = ld_v2d(stg + (j + TPC * r) * C + c);
+      raw[r] = c2{x.x, x.y};
+    }
+    const int cn = s_cn[par];
+    const long col = s_col[par][c], mcol = s_mcol[par][c];
+    for (int t = 0; t < Q; ++t) {
+      // the thread's indices laundered per residue: the addresses formed from
+      // them are formed here instead of being held in registers across residues
+      const int jr = opaque32(j), cr = opaque32(c);
+      const long colr = opaque64(col), mcolr = opaque64(mcol);
+      c2 v[8];
+      if (t == 0) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) v[r] = raw[r];
+      } else {
+        const c2 bj = s_b[jr];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          const c2 w1 = cmul(bj, s_S[r]);
+          c2 w = w1;
+          for (int k = 1; k < t; ++k) w = cmul(w, w1);
+          v[r] = cmul(raw[r], w);  // raw is zero outside the mesh
+        }
+      }
+      int Ns = 1;
+#pragma unroll
+      for (int p = 0; p < Plan::npass; ++p) {
+        const int R = (p < Plan::n8) ? 8 : Plan::last;
+        const int B = 8 / R;
+        if (R == 8) w2t_butterflies<M, 8, -1>(v, jr, Ns, s_tw);
+        else if (R == 4) w2t_butterflies<M, 4, -1>(v, jr, Ns, s_tw);
+        else w2t_butterflies<M, 2, -1>(v, jr, Ns, s_tw);
+        if (p + 1 < Plan::npass) {
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int uu = u / R, r = u % R;
+            const int b = jr * B + uu;
+            const int k = b % Ns;
+            buf[((b - k) * R + k + r * Ns) * C + cr] = v[u];
+          }
+          barrier_keep_vm();
+          Ns *= R;
+          const int Rn = (p + 1 < Plan::n8) ? 8 : Plan::last;
+          const int Bn = 8 / Rn;
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int uu = u / Rn, r = u % Rn;
+            v[u] = buf[(jr * Bn + uu + r * (M / Rn)) * C + cr];
+          }
+          barrier_keep_vm();
+        } else {
+          barrier_keep_vm();  // the records are in buf
+          if (cr < cn) {
+            // outputs: bin jr' -> tau index Q jr' + t, records [phase, (D_0, D_1), (D_2, 0)]
+            const c2 a0 = s_a[par][cr][0], a1 = s_a[par][cr][1], a2 = s_a[par][cr][2];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              const int uu = u / R, r = u % R;
+              const int b = jr * B + uu;
+              const int k = b % Ns;
+              const int jp = (b - k) * R + k + r * Ns;
+              const int jt = Q * jp + t;
+              const c2 ph = buf[3 * jp], d01 = buf[3 * jp + 1], d2 = buf[3 * jp + 2];
+              c2 y = cmul(v[u], ph);
+              y.x += a0.x * d01.x + a1.x * d01.y + a2.x * d2.x;
+              y.y += a0.y * d01.x + a1.y * d01.y + a2.y * d2.x;
+              out[(long)jt * ncol + colr] = y;
+              if (jt == 0) out[(long)L * ncol + colr] = cscale(cadd(y, s_m1[par][cr]), -1.0);
+              if (mcolr >= 0) {
+                out[(long)jt * ncol + mcolr] = cconj(y);
+                if (jt == 0) out[(long)L * ncol + mcolr] = cconj(cscale(cadd(y, s_m1[par][cr]), -1.0));
+              }
+            }
+          }
+          barrier_keep_vm();  // buf is free for the next residue's exchanges
+        }
+      }
+    }
+    if (!more) break;
+    tile = next;
+  }
+}
+
 // The least-squares tail fit of every column as a kernel of its own: a
 // workgroup takes 64 columns (column colmap[i], or i), lane = column, and its
 // four waves a quarter of the fitted frequencies each (the table reads are
@@ -378,7 +641,7 @@ static hipError_t launch_stat(hipStream_t s, double beta, int n_max, long ncol, 
   }
 }
 
-int w2t_pruned_variants() { return 2; }
+int w2t_pruned_variants() { return 3; }
 
 template <int M, int NT>
 static hipError_t launch_sym_m(hipStream_t s, double beta, int n_max, long ncol, const c2 *in, c2 *out,
@@ -407,10 +670,51 @@ static hipError_t launch_sym_nt(hipStream_t s, double beta, int n_max, long ncol
   }
 }
 
+// The number of CUs of the current device (cached per device): the grid of
+// the persistent form.
+static hipError_t device_cus(int &cus) {
+  static int cache[64] = {0};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+  if (!cache[dev]) {
+    int cu = 0;
+    e = hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    cache[dev] = cu > 1 ? cu : 1;
+  }
+  cus = cache[dev];
+  return hipSuccess;
+}
+
+template <int M>
+static hipError_t launch_sym_spec(hipStream_t s, double beta, int n_max, long ncol, const c2 *in, c2 *out,
+                                  const TailFitDev &fit, const MatsDev &mt, const W2tDev &wd, Status st,
+                                  const SymCols &sym) {
+  constexpr int C = 512 / (M / 8);
+  int cus = 0;
+  hipError_t e = device_cus(cus);
+  if (e != hipSuccess) return e;
+  const long ntile = (sym.n + C - 1) / C;
+  const long nb = ntile < cus ? ntile : cus;
+  e = launch_tail_moments(s, sym.n, sym.col, ncol, fit, in, out + (long)(mt.L - 2) * ncol, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_w2t_spec<M>), dim3(nb), dim3(512 + kW2tSpecLoad), 0, s, beta, n_max, ncol, in, out, mt, wd,
+                     sym);
+  return hipGetLastError();
+}
+
+// variant 0: 256 threads per workgroup, 1: 512, 2: the wave-specialised
+// persistent form where it applies (M = 256 or 512, Q >= 2), else as 1
 hipError_t launch_w2t_pruned_sym(hipStream_t s, double beta, int n_max, long ncol, const c2 *in, c2 *out,
                                  const TailFitDev &fit, const MatsDev &mt, const W2tDev &wd, Status st, int variant,
                                  const SymCols &sym) {
-  if (variant == 1) return launch_sym_nt<512>(s, beta, n_max, ncol, in, out, fit, mt, wd, st, sym);
+  if (variant == 2 && wd.Q >= 2 && wd.M == 512)
+    return launch_sym_spec<512>(s, beta, n_max, ncol, in, out, fit, mt, wd, st, sym);
+  if (variant == 2 && wd.Q >= 2 && wd.M == 256)
+    return launch_sym_spec<256>(s, beta, n_max, ncol, in, out, fit, mt, wd, st, sym);
+  if (variant >= 1) return launch_sym_nt<512>(s, beta, n_max, ncol, in, out, fit, mt, wd, st, sym);
   return launch_sym_nt<256>(s, beta, n_max, ncol, in, out, fit, mt, wd, st, sym);
 }
 
@@ -419,7 +723,7 @@ hipError_t launch_w2t_pruned_sym(hipStream_t s, double beta, int n_max, long nco
 hipError_t launch_w2t_pruned(hipStream_t s, int statistic, double beta, int n_max, long ncol, const c2 *in, c2 *out,
                              const c2 *moments, const TailFitDev &fit, const MatsDev &mt, const W2tDev &wd,
                              Status st, int variant) {
-  if (variant == 1) {
+  if (variant >= 1) {  // 2: the wave-specialised form is for the symmetric call; here as 1
     if (statistic == 0) return launch_stat<0, 512>(s, beta, n_max, ncol, in, out, moments, fit, mt, wd, st);
     return launch_stat<1, 512>(s, beta, n_max, ncol, in, out, moments, fit, mt, wd, st);
   }
