@@ -127,13 +127,21 @@ __global__ void __launch_bounds__(64 * Canon<N>::NW)
 
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const long nr = (long)d0 * d1 * d2;
-  // this workgroup's pairs: first + i W for i < count_w
+  // this workgroup's units: the pairs xstart + wslot + i W, i < R, of its
+  // XCD's full rounds; the lo pairs left over are split by waves into KP
+  // parts each (part k: the columns of waves [k NW / KP, (k + 1) NW / KP)),
+  // taken by the first lo KP workgroups of the XCD, so that the last round
+  // costs a fraction of a pair instead of a whole one
   const long npair = bd.ncanon, q8 = npair >> 3, rem8 = npair & 7;
   const int xcd = blockIdx.x & 7, wslot = blockIdx.x >> 3, W = gridDim.x >> 3;
   const long xstart = xcd * q8 + (xcd < rem8 ? xcd : rem8), xcount = q8 + (xcd < rem8 ? 1 : 0);
-  if (wslot >= xcount) return;
-  const long count_w = (xcount - wslot + W - 1) / W;
-  long pair = xstart + wslot;
+  // (uniform: held in scalar registers)
+  const int R = __builtin_amdgcn_readfirstlane((int)(xcount / W));
+  const int lo = __builtin_amdgcn_readfirstlane((int)(xcount - (long)R * W));
+  const int KP = __builtin_amdgcn_readfirstlane(lo ? min(NW, W / lo) : 1);
+  const int nunit = __builtin_amdgcn_readfirstlane(R + ((lo && wslot < lo * KP) ? 1 : 0));
+  if (nunit == 0) return;
+  auto unit_pair = [&](int i) { return i < R ? xstart + wslot + (long)i * W : xstart + (long)R * W + wslot / KP; };
 
   const int N1 = L / P;
   const int ntile = N1 / 2 + 1;                  // tiles j1 = 0 .. N1 / 2
@@ -336,15 +344,23 @@ __global__ void __launch_bounds__(64 * Canon<N>::NW)
   };
 
   // first pair: boundary samples and tile 0 in flight together
-  long r = bd.rcanon[pair];
+  long r = bd.rcanon[unit_pair(0)];
   load_tile(r, 0);
   store_tile(0);
   if (ntile > 1) load_tile(r, 1);
 
   const double fact = beta / (double)L;
   const int im_lo = m_lo + nb - 1, nim = nnu - im_lo;
-  for (long it = 0;; ++it) {
+  for (int it = 0;; ++it) {
     const long mr = minus_r(r, d0, d1, d2);
+    // the waves whose columns this unit transforms (all but in a part)
+    const int part = wslot % KP;
+    const int wlo = it < R ? 0 : part * NW / KP, whi = it < R ? NW : (part + 1) * NW / KP;
+    const bool wact = wave >= wlo && wave < whi;
+    // the next unit's r, loaded now: its load_tile in the epilogue does not
+    // wait for a load just issued
+    const bool more = it + 1 < nunit;
+    const long rn = more ? bd.rcanon[unit_pair(it + 1)] : r;
 #pragma unroll
     for (int i = 0; i < 8; ++i) acc[i] = c2{0.0, 0.0};
     // steps in tile order: j1 = 0, then (1, N1 - 1), (2, N1 - 2), ...; u = 0 is
@@ -356,14 +372,14 @@ __global__ void __launch_bounds__(64 * Canon<N>::NW)
         __syncthreads();  // tile t staged
         if ((bsrc >> 16) == t) lds[Lay::braw + tid] = lds[Lay::tile + (bsrc & 0xffff)];
       }
-      stage1(u);
+      if (wact) stage1(u);
       const bool last = !two || u == 1;
       if (last) {
         __syncthreads();  // every wave is done with tile t
         if (t + 1 < ntile) store_tile(t + 1);
         if (t + 2 < ntile) load_tile(r, t + 2);
       }
-      stage2(lds + Lay::tw + ((t & 1) * 2 + u) * kTwRow);
+      if (wact) stage2(lds + Lay::tw + ((t & 1) * 2 + u) * kTwRow);
       if (last) {
         ++t;
         u = 0;
@@ -376,19 +392,17 @@ __global__ void __launch_bounds__(64 * Canon<N>::NW)
     // ---- this pair's sums and boundary products to LDS; the next pair's
     // boundary samples and tile 0 in flight during the outputs
     const int te = opaque(tid);  // epilogue indices are formed here, not held through the steps
-    if (col_ok) {
+    if (col_ok && wact) {
 #pragma unroll
       for (int q = 0; q < 8; ++q) lds[Lay::sums + col * kSRow + k1 + 8 * h + 16 * q] = acc[q];
     }
-    const bool more = it + 1 < count_w;
-    const long rn = more ? bd.rcanon[pair + W] : r;
     if (more) load_tile(rn, 0);
     __syncthreads();
     // pole coefficients: thread 6 c + 3 d + i forms a_i of column c from its
     // boundary products x_e = G_da(tau_e, r) conj(G_cb(tau_{L-e}, r)) (e = j
     // for j <= 8, e = 17 - (L - j) above), d = 0; for the reversed x (d = 1)
     // dl and dr trade places and every difference changes sign
-    if (te < 6 * NC) {
+    if (te < 6 * NC && te / 24 >= wlo && te / 24 < whi) {  // column te / 6 of an active wave
       const int c = te / 6, d = (te / 3) & 1, i = te % 3;
       int a, b, cc2, dd2;
       canon_col<N>(c, a, b, cc2, dd2);
@@ -409,6 +423,14 @@ __global__ void __launch_bounds__(64 * Canon<N>::NW)
       pc[3 * d + i] = d ? c2{-sum.x, -sum.y} : sum;
       if (d == 0 && i == 0) pc[6] = m1;
     }
+    // next pair: tile 0 staged (the tile region is unused since the steps
+    // ended) and tile 1 in flight before the output stores: vmcnt counts loads
+    // and stores in issue order, so a wait for a load issued after the output
+    // stores would also wait for them
+    if (more) {
+      store_tile(0);
+      if (ntile > 1) load_tile(rn, 1);
+    }
     __syncthreads();
 
     // outputs: thread (g, q) writes column q of the rows im_lo + g, + GO, ...
@@ -420,6 +442,7 @@ __global__ void __launch_bounds__(64 * Canon<N>::NW)
       for (int side = 0; side < (mr != r ? 2 : 1); ++side) {
         const int code = cmap[side * n4 + q];
         const int c = code >> 2, var = code & 3;
+        if (c / 4 < wlo || c / 4 >= whi) continue;  // a column of another part
         const bool addx = var == 1 || var == 2, cj = var >= 2;
         const c2 *pc = lds + Lay::pole + c * 7;
         const c2 xa = addx ? pc[6] : c2{0.0, 0.0};
@@ -440,13 +463,9 @@ __global__ void __launch_bounds__(64 * Canon<N>::NW)
       }
     }
     if (!more) break;
-    // next pair: tile 0 staged (the tile region is unused since the steps
-    // ended), tile 1 in flight; the first step's barrier orders the output
-    // reads of the sums before the transposes are overwritten
-    pair += W;
+    // the first step's barrier orders the output reads of the sums before the
+    // transposes are overwritten
     r = rn;
-    store_tile(0);
-    if (ntile > 1) load_tile(r, 1);
   }
 }
 
