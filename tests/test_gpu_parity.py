@@ -569,12 +569,15 @@ def test_chi0_wr_herm(L, oracle, dims, n, nb):
 # workgroup-per-pair kernel; odd and even sides (self-mirrored r)
 @pytest.mark.parametrize("dims,n,n_iw,nb,ntau", [((2, 2, 1), 3, 32, 64, 129), ((3, 2, 2), 2, 64, 50, 257),
                                                  ((2, 2, 1), 3, 64, 33, None), ((4, 4, 2), 3, 128, 64, None),
-                                                 ((3, 4, 2), 2, 128, 40, None), ((4, 3, 2), 3, 256, 64, None)])
+                                                 ((3, 4, 2), 2, 128, 40, None), ((4, 3, 2), 3, 256, 64, None),
+                                                 ((16, 16, 4), 3, 64, 16, None), ((12, 12, 8), 2, 64, 16, None)])
 def test_chi0_wr_herm_workgroup_per_pair(L, oracle, dims, n, n_iw, nb, ntau):
     """The Hermitian fused bubble with one workgroup per (r, -r) pair
     (option 2 = 7) against the oracle's chi0_wr_from_grt_PH, all frequencies
     and nu >= 0 only (the rows nu < 0 keep their NaN), on meshes of 1, 2, 3,
-    6 and 12 tau tiles."""
+    6 and 12 tau tiles.  The two large lattices (516 and 580 pairs) leave one
+    or eight or nine pairs per XCD after the full rounds of 32 workgroups,
+    which are split by waves into 12, 4 or 3 parts."""
     import torch
     from tprf_amd import _lib
     rng = np.random.default_rng(7 * n + nb + n_iw)
