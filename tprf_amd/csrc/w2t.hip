@@ -518,9 +518,9 @@ __global__ void __launch_bounds__(512 + kW2tSpecLoad)
 
 // The least-squares tail fit of every column as a kernel of its own: a
 // workgroup takes 64 columns (column colmap[i], or i), lane = column, and its
-// four waves a quarter of the fitted frequencies each (the table reads are
-// then wave-uniform); partial coefficients and residuals are summed through
-// LDS.  The moments m_q = w_max^q s_q, q = 1..3, go to mom[(q - 1) ncol + col],
+// four waves 16 of the (at most 64) fitted frequencies each (the table reads
+// are then wave-uniform); partial coefficients and residuals are summed
+// through LDS.  The moments m_q = w_max^q s_q, q = 1..3, go to mom[(q - 1) ncol + col],
 // the fit residual and |s_0| to the status words (one atomic per wave).  The
 // pruned transform then runs without its in-workgroup fit, whose chain of
 // dependent steps otherwise holds every wave of a workgroup.
@@ -531,20 +531,25 @@ __global__ void __launch_bounds__(64 * kTmParts) k_tail_moments(long ntot, const
                                                                 c2 *__restrict__ mom, double *status) {
   __shared__ c2 s_part[kTmParts][kMaxOrder][64];
   __shared__ double s_r2[kTmParts][64];
-  __shared__ c2 s_pinv[kMaxOrder * kMaxFit], s_vander[kMaxFit * kMaxOrder];
-  const int nct = fit.ncoef * fit.n_fit;
-  for (int t = threadIdx.x; t < nct; t += blockDim.x) {
-    s_pinv[t] = fit.pinv[t];
-    s_vander[t] = fit.vander[t];
+  // the operator and the Vandermonde rows, zero past n_fit rows and ncoef
+  // coefficients: every loop below has compile-time bounds and no branch, so
+  // its LDS reads are issued together
+  __shared__ c2 s_pinv[kMaxOrder][kMaxFit], s_vander[kMaxFit][kMaxOrder];
+  const int nc = fit.ncoef, nf = fit.n_fit;
+  for (int t = threadIdx.x; t < kMaxOrder * kMaxFit; t += blockDim.x) {
+    const int q = t / kMaxFit, f = t - q * kMaxFit;
+    s_pinv[q][f] = (q < nc && f < nf) ? fit.pinv[q * nf + f] : c2{0.0, 0.0};
+    const int fv = t / kMaxOrder, qv = t - fv * kMaxOrder;
+    s_vander[fv][qv] = (qv < nc && fv < nf) ? fit.vander[fv * nc + qv] : c2{0.0, 0.0};
   }
   const int lane = threadIdx.x & 63;
   const int part = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long i = (long)blockIdx.x * 64 + lane;
   const bool valid = i < ntot;
   const long col = valid ? (colmap ? (long)colmap[i] : i) : (colmap ? (long)colmap[0] : 0);
-  const int nc = fit.ncoef, nf = fit.n_fit;
-  const int per = (nf + kTmParts - 1) / kTmParts;
-  const int f0 = part * per;
+  // wave `part` takes the fitted rows f = kTmPer part + u (rows past n_fit
+  // load row n_fit - 1 and have zero weight)
+  const int f0 = part * kTmPer;
   c2 x[kTmPer];
 #pragma unroll
   for (int u = 0; u < kTmPer; ++u) x[u] = in[(long)fit.idx[min(f0 + u, nf - 1)] * ncol + col];
@@ -554,11 +559,8 @@ __global__ void __launch_bounds__(64 * kTmParts) k_tail_moments(long ntot, const
   for (int q = 0; q < kMaxOrder; ++q) s[q] = c2{0.0, 0.0};
 #pragma unroll
   for (int u = 0; u < kTmPer; ++u)
-    if (u < per && f0 + u < nf) {
 #pragma unroll
-      for (int q = 0; q < kMaxOrder; ++q)
-        if (q < nc) s[q] = cfma(s[q], s_pinv[q * nf + f0 + u], x[u]);
-    }
+    for (int q = 0; q < kMaxOrder; ++q) s[q] = cfma(s[q], s_pinv[q][f0 + u], x[u]);
 #pragma unroll
   for (int q = 0; q < kMaxOrder; ++q) s_part[part][q][lane] = s[q];
   __syncthreads();
@@ -571,14 +573,12 @@ __global__ void __launch_bounds__(64 * kTmParts) k_tail_moments(long ntot, const
   }
   double r2 = 0.0;
 #pragma unroll
-  for (int u = 0; u < kTmPer; ++u)
-    if (u < per && f0 + u < nf) {
-      c2 v = x[u];
+  for (int u = 0; u < kTmPer; ++u) {
+    c2 v = x[u];
 #pragma unroll
-      for (int q = 0; q < kMaxOrder; ++q)
-        if (q < nc) v = cfms(v, s_vander[(f0 + u) * nc + q], s[q]);
-      r2 += cabs2(v);
-    }
+    for (int q = 0; q < kMaxOrder; ++q) v = cfms(v, s_vander[f0 + u][q], s[q]);
+    r2 += f0 + u < nf ? cabs2(v) : 0.0;
+  }
   s_r2[part][lane] = r2;
   __syncthreads();
   if (part != 0) return;
