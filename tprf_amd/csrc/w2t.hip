@@ -372,12 +372,17 @@ __global__ void __launch_bounds__(512 + kW2tSpecLoad)
       rec[i] = ld_v2d(wd.otab + (long)(Q * jp + t) * 3 + part);
     }
   };
+  long colA = 0, mcolA = -1;  // loader: its column of the next tile and the mirror
   if (loader) {
     stage_meta(tile, 0);
     const long col0 = lcol(tile, sym.col);
     for (int k0 = 0; k0 < NE; k0 += NE / 2) {
       load_in(col0, k0, k0 + NE / 2);
       store_in(k0, k0 + NE / 2);
+    }
+    if (tile + G < ntile) {
+      colA = lcol(tile + G, sym.col);
+      mcolA = lcol(tile + G, sym.mcol);
     }
   } else {
     for (int k = tid; k < M; k += NT) st_v2d(s_tw + k, ld_v2d(wd.twM + k));
@@ -394,15 +399,30 @@ __global__ void __launch_bounds__(512 + kW2tSpecLoad)
     // The same barriers as the transform threads below.  At the start of each
     // residue the loader issues that residue's records and a chunk of the next
     // tile's inputs (chunk t in residue t), and stores them to LDS just before
-    // the records barrier; in residue 0 it also stages the next tile's
-    // columns and pole coefficients.
+    // the records barrier; in residue 0 it also loads the next tile's moments
+    // (stored as pole coefficients with its columns) and the columns of the
+    // tile after it.  The columns are loaded a tile ahead, so that no load
+    // at a residue's start waits for another.
+    long colB = 0, mcolB = -1;
+    v2d mom[3];
     for (int par = 0;; par ^= 1) {
-      const long next = tile + G;
+      const long next = tile + G, nn = next + G;
       const bool more = next < ntile;
-      const long ncol_l = more ? lcol(next, sym.col) : 0;
+      const int cnx = more ? live(next) : 0;
       for (int t = 0; t < Q; ++t) {
         load_rec(t);
-        if (more) load_in(ncol_l, t * chunk, min(NE, (t + 1) * chunk));
+        if (more) load_in(colA, t * chunk, min(NE, (t + 1) * chunk));
+        if (t == 0) {
+          if (more && lt < C) {
+            const c2 *mo = out + (long)(L - 2) * ncol + colA;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) mom[q] = ld_v2d(mo + q * ncol);
+          }
+          if (nn < ntile) {
+            colB = lcol(nn, sym.col);
+            mcolB = lcol(nn, sym.mcol);
+          }
+        }
 #pragma unroll
         for (int p = 0; p + 1 < Plan::npass; ++p) {
           barrier_keep_vm();
@@ -415,13 +435,24 @@ __global__ void __launch_bounds__(512 + kW2tSpecLoad)
         for (int i = 0; i < NR; ++i) st_v2d(buf + ltr + NL * i, rec[i]);
         if (more) {
           store_in(t * chunk, min(NE, (t + 1) * chunk));
-          if (t == 0) stage_meta(next, par ^ 1);
+          if (t == 0 && lt < C) {
+            s_col[par ^ 1][lt] = (int)colA;
+            s_mcol[par ^ 1][lt] = (int)mcolA;
+            if (lt < cnx) {
+              const c2 m1{mom[0].x, mom[0].y};
+              Poles<0>::coef(m1, c2{mom[1].x, mom[1].y}, c2{mom[2].x, mom[2].y}, s_a[par ^ 1][lt]);
+              s_m1[par ^ 1][lt] = m1;
+            }
+            if (lt == 0) s_cn[par ^ 1] = cnx;
+          }
         }
         barrier_keep_vm();
         barrier_keep_vm();
       }
       if (!more) break;
       tile = next;
+      colA = colB;
+      mcolA = mcolB;
     }
     return;
   }
